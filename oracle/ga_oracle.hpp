@@ -1,0 +1,3188 @@
+// GraphAligner seed-extension oracle: a literal CPU restatement of the reference
+// algorithm, used ONLY as test infrastructure (parity checker and CPU baseline).
+// Nothing in the product library links, loads or calls this code.
+//
+// Every function cites the reference file:line it restates. Reference semantics
+// that cannot be reproduced without the reference's absent third-party code are
+// replaced by a documented deterministic choice:
+//  * phmap::flat_hash_map iteration order (src/GraphAlignerBitvectorBanded.h:313,
+//    src/AlignmentGraph.cpp:596) -> band-addition order / ascending key order.
+//  * ComponentPriorityQueue ties on equal (component, score)
+//    (src/ComponentPriorityQueue.h:11-19) -> broken by insertion sequence
+//    number. Components are unique per node on acyclic graphs, so this only
+//    matters inside strongly connected components.
+//  * BBHash key order in MinimizerSeeder::initMaxCount (src/MinimizerSeeder.cpp:557-575)
+//    -> the key with the smallest count is the one left out.
+// Everything else (libstdc++ unordered_map / std::sort / std::priority_queue
+// behaviour) is reproduced by using the same standard containers and algorithms.
+#pragma once
+#include <algorithm>
+#include <array>
+#include <cassert>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <fstream>
+#include <functional>
+#include <limits>
+#include <map>
+#include <memory>
+#include <queue>
+#include <sstream>
+#include <string>
+#include <string_view>
+#include <tuple>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+namespace orc {
+
+// The reference redefines assert() to throw (src/ThreadReadAssertion.h:24-27); a
+// failed assertion drops the read (src/Aligner.cpp:493-500).
+struct AssertFail {
+	const char* what;
+};
+#define ORC_ASSERT(c) \
+	do { \
+		if (!(c)) throw ::orc::AssertFail{#c}; \
+	} while (0)
+
+using Word = uint64_t;
+using ScoreType = int32_t;
+static constexpr int WordSize = 64;
+static constexpr Word AllOnes = ~(Word)0;
+static constexpr Word AllZeros = 0;
+static constexpr Word LastBit = (Word)1 << 63;
+static constexpr ScoreType SMAX = std::numeric_limits<ScoreType>::max();
+static constexpr ScoreType SMIN = std::numeric_limits<ScoreType>::min();
+static constexpr size_t NMAX = std::numeric_limits<size_t>::max();
+
+static inline int popcount(Word x) { return __builtin_popcountll(x); }
+
+// ---------------------------------------------------------------------------
+// Character helpers (src/GraphAlignerCommon.h:176-279, src/CommonUtils.cpp:67-134)
+// ---------------------------------------------------------------------------
+inline bool ambiguousMatch(char ambiguousChar, char exactChar)
+{
+	switch (ambiguousChar)
+	{
+		case '-': return false;
+		case 'A': case 'a': return exactChar == 'A';
+		case 'u': case 'U': case 'T': case 't': return exactChar == 'T';
+		case 'C': case 'c': return exactChar == 'C';
+		case 'G': case 'g': return exactChar == 'G';
+		case 'N': case 'n': return true;
+		case 'R': case 'r': return exactChar == 'A' || exactChar == 'G';
+		case 'Y': case 'y': return exactChar == 'C' || exactChar == 'T';
+		case 'K': case 'k': return exactChar == 'G' || exactChar == 'T';
+		case 'M': case 'm': return exactChar == 'C' || exactChar == 'A';
+		case 'S': case 's': return exactChar == 'C' || exactChar == 'G';
+		case 'W': case 'w': return exactChar == 'A' || exactChar == 'T';
+		case 'B': case 'b': return exactChar == 'C' || exactChar == 'G' || exactChar == 'T';
+		case 'D': case 'd': return exactChar == 'A' || exactChar == 'G' || exactChar == 'T';
+		case 'H': case 'h': return exactChar == 'A' || exactChar == 'C' || exactChar == 'T';
+		case 'V': case 'v': return exactChar == 'A' || exactChar == 'C' || exactChar == 'G';
+		default: ORC_ASSERT(false); return false;
+	}
+}
+
+inline bool characterMatch(char sequenceCharacter, char graphCharacter)
+{
+	if (sequenceCharacter == graphCharacter) return true;
+	switch (sequenceCharacter)
+	{
+		case 'a': case 'A': return ambiguousMatch(graphCharacter, 'A');
+		case 'c': case 'C': return ambiguousMatch(graphCharacter, 'C');
+		case 'g': case 'G': return ambiguousMatch(graphCharacter, 'G');
+		case 't': case 'T': return ambiguousMatch(graphCharacter, 'T');
+		case '-': return false;
+	}
+	return (ambiguousMatch(sequenceCharacter, 'A') && ambiguousMatch(graphCharacter, 'A'))
+		|| (ambiguousMatch(sequenceCharacter, 'C') && ambiguousMatch(graphCharacter, 'C'))
+		|| (ambiguousMatch(sequenceCharacter, 'G') && ambiguousMatch(graphCharacter, 'G'))
+		|| (ambiguousMatch(sequenceCharacter, 'T') && ambiguousMatch(graphCharacter, 'T'));
+}
+
+inline char Complement(char c)
+{
+	switch (c)
+	{
+		case 'A': case 'a': return 'T';
+		case 'C': case 'c': return 'G';
+		case 'T': case 't': return 'A';
+		case 'G': case 'g': return 'C';
+		case 'N': case 'n': return 'N';
+		case 'U': case 'u': return 'A';
+		case 'R': case 'r': return 'Y';
+		case 'Y': case 'y': return 'R';
+		case 'K': case 'k': return 'M';
+		case 'M': case 'm': return 'K';
+		case 'S': case 's': return 'S';
+		case 'W': case 'w': return 'W';
+		case 'B': case 'b': return 'V';
+		case 'V': case 'v': return 'B';
+		case 'D': case 'd': return 'H';
+		case 'H': case 'h': return 'D';
+		default: ORC_ASSERT(false); return 'N';
+	}
+}
+
+inline std::string ReverseComplement(const std::string& str)
+{
+	std::string result;
+	result.reserve(str.size());
+	for (int i = (int)str.size() - 1; i >= 0; i--) result += Complement(str[i]);
+	return result;
+}
+
+// ---------------------------------------------------------------------------
+// GFA loading (src/GfaGraph.cpp:170-396)
+// ---------------------------------------------------------------------------
+struct NodePos
+{
+	int id = 0;
+	bool end = false;
+	NodePos() {}
+	NodePos(int id, bool end) : id(id), end(end) {}
+	NodePos Reverse() const { return NodePos{id, !end}; }
+	bool operator==(const NodePos& o) const { return id == o.id && end == o.end; }
+	bool operator!=(const NodePos& o) const { return !(*this == o); }
+};
+struct NodePosHash
+{
+	size_t operator()(const NodePos& x) const { return std::hash<int>()(x.id) ^ std::hash<bool>()(x.end); }
+};
+struct NodePosPairHash
+{
+	// src/GfaGraph.h:34-44 (Cantor pairing computed in double)
+	size_t operator()(const std::pair<NodePos, NodePos>& x) const
+	{
+		size_t pairing = .5 * (x.first.id + x.second.id) * (x.first.id + x.second.id + 1) + x.second.id;
+		return std::hash<size_t>()(pairing);
+	}
+};
+
+struct GfaGraph
+{
+	std::unordered_map<int, std::string> nodes;
+	std::unordered_map<NodePos, std::vector<NodePos>, NodePosHash> edges;
+	std::unordered_map<std::pair<NodePos, NodePos>, size_t, NodePosPairHash> varyingOverlaps;
+	size_t edgeOverlap = NMAX;
+	std::unordered_map<int, std::string> originalNodeName;
+
+	std::string OriginalNodeName(int nodeId) const
+	{
+		auto found = originalNodeName.find(nodeId);
+		if (found == originalNodeName.end()) return "";
+		return found->second;
+	}
+
+	static int getNameId(std::unordered_map<std::string, int>& assigned, const std::string& name)
+	{
+		auto found = assigned.find(name);
+		if (found == assigned.end())
+		{
+			int result = assigned.size();
+			assigned[name] = result;
+			return result;
+		}
+		return found->second;
+	}
+
+	// src/GfaGraph.cpp:182-216
+	void numberBackToIntegers()
+	{
+		std::unordered_map<int, std::string> newNodes;
+		std::unordered_map<NodePos, std::vector<NodePos>, NodePosHash> newEdges;
+		std::unordered_map<std::pair<NodePos, NodePos>, size_t, NodePosPairHash> newVaryingOverlaps;
+		for (auto pair : varyingOverlaps)
+		{
+			auto key = pair.first;
+			key.first.id = std::stoi(originalNodeName[key.first.id]);
+			key.second.id = std::stoi(originalNodeName[key.second.id]);
+			newVaryingOverlaps[key] = pair.second;
+		}
+		for (auto pair : nodes)
+		{
+			newNodes[std::stoi(originalNodeName[pair.first])] = pair.second;
+		}
+		for (auto edge : edges)
+		{
+			for (auto target : edge.second)
+			{
+				newEdges[NodePos{std::stoi(originalNodeName[edge.first.id]), edge.first.end}].push_back(NodePos{std::stoi(originalNodeName[target.id]), target.end});
+			}
+		}
+		varyingOverlaps = std::move(newVaryingOverlaps);
+		nodes = std::move(newNodes);
+		edges = std::move(newEdges);
+		originalNodeName.clear();
+	}
+
+	// src/GfaGraph.cpp:218-396 with allowVaryingOverlaps = true (src/Aligner.cpp:640)
+	static GfaGraph LoadFromStream(std::istream& file)
+	{
+		const bool allowVaryingOverlaps = true;
+		std::unordered_map<std::string, int> nameMapping;
+		GfaGraph result;
+		bool hasVaryingOverlaps = false;
+		while (file.good())
+		{
+			std::string line;
+			std::getline(file, line);
+			if (!file.good()) break;
+			if (line.size() == 0) continue;
+			if (line[0] != 'S' && line[0] != 'L') continue;
+			if (line[0] == 'S')
+			{
+				std::stringstream sstr{line};
+				std::string idstr, dummy, seq;
+				sstr >> dummy;
+				sstr >> idstr;
+				int id = getNameId(nameMapping, idstr);
+				sstr >> seq;
+				if (seq == "*") throw std::runtime_error("Nodes without sequence (*) are not currently supported");
+				result.nodes[id] = seq;
+			}
+			if (line[0] == 'L')
+			{
+				std::stringstream sstr{line};
+				std::string fromstr, tostr, fromstart, toend, dummy;
+				int overlap = 0;
+				sstr >> dummy;
+				sstr >> fromstr;
+				int from = getNameId(nameMapping, fromstr);
+				sstr >> fromstart;
+				sstr >> tostr;
+				int to = getNameId(nameMapping, tostr);
+				sstr >> toend;
+				sstr.get();
+				char test = sstr.peek();
+				if (test == '*')
+				{
+					overlap = 0;
+				}
+				else
+				{
+					sstr >> overlap;
+					char dummyc = 'z';
+					sstr >> dummyc;
+				}
+				if (overlap < 0) throw std::runtime_error("Edge overlap cannot be negative. Fix the graph");
+				if (result.edgeOverlap != NMAX && (size_t)overlap != result.edgeOverlap) hasVaryingOverlaps = true;
+				result.edgeOverlap = overlap;
+				NodePos frompos{from, fromstart == "+"};
+				NodePos topos{to, toend == "+"};
+				result.edges[frompos].push_back(topos);
+				if (allowVaryingOverlaps) result.varyingOverlaps[std::make_pair(frompos, topos)] = overlap;
+			}
+		}
+		if (hasVaryingOverlaps) result.edgeOverlap = 0;
+		if (result.edges.size() == 0) result.edgeOverlap = 0;
+		bool allIdsIntegers = true;
+		for (auto pair : nameMapping)
+		{
+			result.originalNodeName[pair.second] = pair.first;
+			if (allIdsIntegers)
+			{
+				char* p = (char*)pair.first.c_str();
+				long int test = strtol(pair.first.c_str(), &p, 10);
+				if (*p != '\0') allIdsIntegers = false;
+				else if (test <= std::numeric_limits<int>::min() || test >= std::numeric_limits<int>::max()) allIdsIntegers = false;
+			}
+		}
+		for (auto pair : result.varyingOverlaps)
+		{
+			auto from = pair.first.first;
+			auto to = pair.first.second;
+			if (result.nodes.count(from.id) == 0 || result.nodes.count(to.id) == 0) continue;
+			if (result.nodes.at(from.id).size() <= pair.second || result.nodes.at(to.id).size() <= pair.second)
+				throw std::runtime_error("Overlap between nodes is too big");
+		}
+		if (allIdsIntegers) result.numberBackToIntegers();
+		std::vector<NodePos> nonexistantEdges;
+		for (auto& edge : result.edges)
+		{
+			if (result.nodes.count(edge.first.id) == 0)
+			{
+				nonexistantEdges.push_back(edge.first);
+				continue;
+			}
+			for (size_t i = edge.second.size() - 1; i < edge.second.size() + 1; i--)
+			{
+				if (result.nodes.count(edge.second[i].id) == 0) edge.second.erase(edge.second.begin() + i);
+			}
+		}
+		for (auto nonexistant : nonexistantEdges) result.edges.erase(result.edges.find(nonexistant));
+		return result;
+	}
+};
+
+// ---------------------------------------------------------------------------
+// AlignmentGraph (src/AlignmentGraph.h, src/AlignmentGraph.cpp)
+// ---------------------------------------------------------------------------
+struct AmbiguousChunkSequence
+{
+	uint64_t A = 0, T = 0, C = 0, G = 0;
+};
+
+struct AlignmentGraph
+{
+	static constexpr int SPLIT_NODE_SIZE = 64;
+	static constexpr size_t BP_IN_CHUNK = 32;
+	static constexpr size_t CHUNKS_IN_NODE = 2;
+
+	std::vector<size_t> nodeLength;
+	std::unordered_map<int, std::vector<size_t>> nodeLookup;
+	std::unordered_map<int, size_t> originalNodeSize;
+	std::unordered_map<int, std::string> originalNodeName;
+	std::vector<size_t> nodeOffset;
+	std::vector<int> nodeIDs;
+	std::vector<std::vector<size_t>> inNeighbors;
+	std::vector<std::vector<size_t>> outNeighbors;
+	std::vector<bool> reverse;
+	std::vector<bool> linearizable;
+	std::vector<std::array<uint64_t, 2>> nodeSequences;
+	size_t bpSize = 0;
+	std::vector<AmbiguousChunkSequence> ambiguousNodeSequences;
+	std::vector<bool> ambiguousNodes;
+	std::vector<size_t> componentNumber;
+	std::vector<size_t> chainNumber;
+	std::vector<size_t> chainApproxPos;
+	size_t firstAmbiguous = NMAX;
+	size_t DBGoverlap = 0;
+
+	size_t NodeSize() const { return nodeLength.size(); }
+	size_t NodeLength(size_t i) const { return nodeLength[i]; }
+
+	// src/AlignmentGraph.cpp:56-97
+	void AddNode(int nodeId, const std::string& sequence, const std::string& name, bool reverseNode, const std::vector<size_t>& breakpoints)
+	{
+		if (nodeLookup.count(nodeId) != 0) return;
+		originalNodeSize[nodeId] = sequence.size();
+		originalNodeName[nodeId] = name;
+		for (size_t breakpoint = 1; breakpoint < breakpoints.size(); breakpoint++)
+		{
+			if (breakpoints[breakpoint] == breakpoints[breakpoint - 1]) continue;
+			for (size_t offset = breakpoints[breakpoint - 1]; offset < breakpoints[breakpoint]; offset += SPLIT_NODE_SIZE)
+			{
+				size_t size = SPLIT_NODE_SIZE;
+				if (breakpoints[breakpoint] - offset < size) size = breakpoints[breakpoint] - offset;
+				AddNodeInner(nodeId, offset, sequence.substr(offset, size), reverseNode);
+				if (offset > 0)
+				{
+					outNeighbors[outNeighbors.size() - 2].push_back(outNeighbors.size() - 1);
+					inNeighbors[inNeighbors.size() - 1].push_back(inNeighbors.size() - 2);
+				}
+			}
+		}
+	}
+
+	// src/AlignmentGraph.cpp:99-228
+	void AddNodeInner(int nodeId, int offset, const std::string& sequence, bool reverseNode)
+	{
+		bpSize += sequence.size();
+		nodeLookup[nodeId].push_back(nodeLength.size());
+		nodeLength.push_back(sequence.size());
+		nodeIDs.push_back(nodeId);
+		inNeighbors.emplace_back();
+		outNeighbors.emplace_back();
+		reverse.push_back(reverseNode);
+		nodeOffset.push_back(offset);
+		std::array<uint64_t, 2> normalSeq{0, 0};
+		AmbiguousChunkSequence amb;
+		bool ambiguous = false;
+		for (size_t i = 0; i < sequence.size(); i++)
+		{
+			size_t chunk = i / BP_IN_CHUNK;
+			size_t off = (i % BP_IN_CHUNK) * 2;
+			uint64_t bit = (uint64_t)1 << i;
+			switch (sequence[i])
+			{
+				case 'a': case 'A': amb.A |= bit; normalSeq[chunk] |= (uint64_t)0 << off; break;
+				case 'c': case 'C': amb.C |= bit; normalSeq[chunk] |= (uint64_t)1 << off; break;
+				case 'g': case 'G': amb.G |= bit; normalSeq[chunk] |= (uint64_t)2 << off; break;
+				case 't': case 'T': case 'u': case 'U': amb.T |= bit; normalSeq[chunk] |= (uint64_t)3 << off; break;
+				case 'r': case 'R': amb.A |= bit; amb.G |= bit; ambiguous = true; break;
+				case 'y': case 'Y': amb.C |= bit; amb.T |= bit; ambiguous = true; break;
+				case 's': case 'S': amb.G |= bit; amb.C |= bit; ambiguous = true; break;
+				case 'w': case 'W': amb.A |= bit; amb.T |= bit; ambiguous = true; break;
+				case 'k': case 'K': amb.G |= bit; amb.T |= bit; ambiguous = true; break;
+				case 'm': case 'M': amb.A |= bit; amb.C |= bit; ambiguous = true; break;
+				case 'b': case 'B': amb.C |= bit; amb.G |= bit; amb.T |= bit; ambiguous = true; break;
+				case 'd': case 'D': amb.A |= bit; amb.G |= bit; amb.T |= bit; ambiguous = true; break;
+				case 'h': case 'H': amb.A |= bit; amb.C |= bit; amb.T |= bit; ambiguous = true; break;
+				case 'v': case 'V': amb.A |= bit; amb.C |= bit; amb.G |= bit; ambiguous = true; break;
+				case 'n': case 'N': amb.A |= bit; amb.C |= bit; amb.G |= bit; amb.T |= bit; ambiguous = true; break;
+				default: throw std::runtime_error("invalid graph character");
+			}
+		}
+		ambiguousNodes.push_back(ambiguous);
+		if (ambiguous) ambiguousNodeSequences.push_back(amb);
+		else nodeSequences.push_back(normalSeq);
+	}
+
+	// src/AlignmentGraph.cpp:230-250
+	void AddEdgeNodeId(int node_id_from, int node_id_to, size_t startOffset)
+	{
+		size_t from = nodeLookup.at(node_id_from).back();
+		size_t to = NMAX;
+		for (auto node : nodeLookup[node_id_to])
+		{
+			if (nodeOffset[node] == startOffset) to = node;
+		}
+		if (to == NMAX) throw std::runtime_error("edge target offset not found");
+		if (std::find(inNeighbors[to].begin(), inNeighbors[to].end(), from) == inNeighbors[to].end()) inNeighbors[to].push_back(from);
+		if (std::find(outNeighbors[from].begin(), outNeighbors[from].end(), to) == outNeighbors[from].end()) outNeighbors[from].push_back(to);
+	}
+
+	// src/AlignmentGraph.cpp:252-296
+	void Finalize()
+	{
+		RenumberAmbiguousToEnd();
+		ambiguousNodes.clear();
+		findLinearizable();
+		doComponentOrder();
+		findChains();
+	}
+
+	template <typename T>
+	static std::vector<T> reorder(const std::vector<T>& vec, const std::vector<size_t>& renumbering)
+	{
+		std::vector<T> result;
+		result.resize(vec.size());
+		for (size_t i = 0; i < vec.size(); i++) result[renumbering[i]] = vec[i];
+		return result;
+	}
+	static std::vector<size_t> renumber(const std::vector<size_t>& vec, const std::vector<size_t>& renumbering)
+	{
+		std::vector<size_t> result;
+		for (size_t i = 0; i < vec.size(); i++) result.push_back(renumbering[vec[i]]);
+		return result;
+	}
+
+	// src/AlignmentGraph.cpp:915-985
+	void RenumberAmbiguousToEnd()
+	{
+		std::vector<size_t> renumbering;
+		size_t nonAmbiguousCount = 0;
+		size_t ambiguousCount = 0;
+		for (size_t i = 0; i < ambiguousNodes.size(); i++)
+		{
+			if (!ambiguousNodes[i])
+			{
+				renumbering.push_back(nonAmbiguousCount);
+				nonAmbiguousCount++;
+			}
+			else
+			{
+				renumbering.push_back(ambiguousNodes.size() - 1 - ambiguousCount);
+				ambiguousCount++;
+			}
+		}
+		firstAmbiguous = nonAmbiguousCount;
+		if (ambiguousCount == 0) return;
+		std::reverse(ambiguousNodeSequences.begin(), ambiguousNodeSequences.end());
+		nodeLength = reorder(nodeLength, renumbering);
+		nodeOffset = reorder(nodeOffset, renumbering);
+		nodeIDs = reorder(nodeIDs, renumbering);
+		inNeighbors = reorder(inNeighbors, renumbering);
+		outNeighbors = reorder(outNeighbors, renumbering);
+		std::vector<bool> newReverse(reverse.size());
+		for (size_t i = 0; i < reverse.size(); i++) newReverse[renumbering[i]] = reverse[i];
+		reverse = newReverse;
+		for (auto& pair : nodeLookup) pair.second = renumber(pair.second, renumbering);
+		for (size_t i = 0; i < inNeighbors.size(); i++)
+		{
+			inNeighbors[i] = renumber(inNeighbors[i], renumbering);
+			outNeighbors[i] = renumber(outNeighbors[i], renumbering);
+		}
+	}
+
+	// src/AlignmentGraph.cpp:632-724
+	void findLinearizable()
+	{
+		linearizable.resize(nodeLength.size(), false);
+		std::vector<bool> checked(nodeLength.size(), false);
+		std::vector<size_t> stack;
+		std::vector<bool> onStack(nodeLength.size(), false);
+		for (size_t node = 0; node < nodeLength.size(); node++)
+		{
+			if (checked[node]) continue;
+			if (inNeighbors[node].size() != 1)
+			{
+				checked[node] = true;
+				continue;
+			}
+			checked[node] = true;
+			stack.push_back(node);
+			onStack[node] = true;
+			while (true)
+			{
+				if (inNeighbors[stack.back()].size() != 1 || checked[stack.back()])
+				{
+					// both branches of src/AlignmentGraph.cpp:668-697 are identical
+					for (size_t i = 0; i < stack.size() - 1; i++)
+					{
+						checked[stack[i]] = true;
+						linearizable[stack[i]] = true;
+						onStack[stack[i]] = false;
+					}
+					linearizable[stack.back()] = false;
+					checked[stack.back()] = true;
+					onStack[stack.back()] = false;
+					stack.clear();
+					break;
+				}
+				auto neighbor = inNeighbors[stack.back()][0];
+				if (neighbor == node)
+				{
+					for (size_t i = 0; i < stack.size(); i++)
+					{
+						checked[stack[i]] = true;
+						linearizable[stack[i]] = false;
+						onStack[stack[i]] = false;
+					}
+					stack.clear();
+					break;
+				}
+				if (onStack[neighbor])
+				{
+					// src/AlignmentGraph.cpp:700-716, including its stack[stack.size()] read:
+					// the first iteration reads one past the end, which is never equal to
+					// a node on the stack (vector storage beyond size() is stale), so the
+					// loop effectively starts one below. Restated without the stale read.
+					size_t i = stack.size();
+					for (; i > 0; i--)
+					{
+						if (i < stack.size() && stack[i] == neighbor) break;
+						if (i < stack.size())
+						{
+							checked[stack[i]] = true;
+							linearizable[stack[i]] = false;
+							onStack[stack[i]] = false;
+						}
+					}
+					for (size_t j = 0; j < i; j++)
+					{
+						checked[stack[j]] = true;
+						linearizable[stack[j]] = true;
+						onStack[stack[j]] = false;
+					}
+					stack.clear();
+					break;
+				}
+				stack.push_back(inNeighbors[stack.back()][0]);
+				onStack[stack.back()] = true;
+			}
+		}
+	}
+
+	// src/AlignmentGraph.cpp:987-1094 (iterative Tarjan, reverse numbering)
+	void doComponentOrder()
+	{
+		std::vector<std::tuple<size_t, int, size_t>> callStack;
+		size_t i = 0;
+		std::vector<size_t> index(nodeLength.size(), NMAX);
+		std::vector<size_t> lowlink(nodeLength.size(), NMAX);
+		std::vector<bool> onStack(nodeLength.size(), false);
+		std::vector<size_t> stack;
+		size_t checknode = 0;
+		size_t nextComponent = 0;
+		componentNumber.assign(nodeLength.size(), NMAX);
+		while (true)
+		{
+			if (callStack.size() == 0)
+			{
+				while (checknode < nodeLength.size() && index[checknode] != NMAX) checknode++;
+				if (checknode == nodeLength.size()) break;
+				callStack.emplace_back(checknode, 0, 0);
+				checknode++;
+			}
+			auto top = callStack.back();
+			const size_t v = std::get<0>(top);
+			int state = std::get<1>(top);
+			size_t w;
+			size_t neighborI = std::get<2>(top);
+			callStack.pop_back();
+			if (state == 0)
+			{
+				index[v] = i;
+				lowlink[v] = i;
+				i += 1;
+				stack.push_back(v);
+				onStack[v] = true;
+				state = 1;
+			}
+			if (state == 2)
+			{
+				w = outNeighbors[v][neighborI];
+				lowlink[v] = std::min(lowlink[v], lowlink[w]);
+				neighborI++;
+				state = 1;
+			}
+			bool recursed = false;
+			if (state == 1)
+			{
+				while (neighborI < outNeighbors[v].size())
+				{
+					w = outNeighbors[v][neighborI];
+					if (index[w] == NMAX)
+					{
+						callStack.emplace_back(v, 2, neighborI);
+						callStack.emplace_back(w, 0, 0);
+						recursed = true;
+						break;
+					}
+					else if (onStack[w])
+					{
+						lowlink[v] = std::min(lowlink[v], index[w]);
+					}
+					neighborI += 1;
+				}
+			}
+			if (recursed) continue;
+			if (lowlink[v] == index[v])
+			{
+				do
+				{
+					w = stack.back();
+					stack.pop_back();
+					onStack[w] = false;
+					componentNumber[w] = nextComponent;
+				} while (w != v);
+				nextComponent++;
+			}
+		}
+		for (size_t k = 0; k < componentNumber.size(); k++) componentNumber[k] = nextComponent - 1 - componentNumber[k];
+	}
+
+	static size_t ufFind(std::vector<size_t>& parent, size_t item)
+	{
+		if (parent[item] == item) return item;
+		std::vector<size_t> stack;
+		stack.push_back(item);
+		while (parent[stack.back()] != stack.back()) stack.push_back(parent[stack.back()]);
+		for (size_t i : stack) parent[i] = stack.back();
+		return stack.back();
+	}
+	static void ufMerge(std::vector<size_t>& parent, std::vector<size_t>& rank, size_t left, size_t right)
+	{
+		left = ufFind(parent, left);
+		right = ufFind(parent, right);
+		if (rank[left] < rank[right]) std::swap(left, right);
+		parent[right] = left;
+		if (rank[left] == rank[right]) rank[left] += 1;
+	}
+
+	// src/AlignmentGraph.cpp:298-342
+	std::pair<bool, size_t> findBubble(const size_t start, const std::vector<bool>& ignorableTip)
+	{
+		std::vector<size_t> S;
+		S.push_back(start);
+		std::unordered_set<size_t> visited;
+		std::unordered_set<size_t> seen;
+		seen.insert(start);
+		while (S.size() > 0)
+		{
+			const size_t v = S.back();
+			S.pop_back();
+			seen.erase(v);
+			visited.insert(v);
+			if (outNeighbors[v].size() == 0) return std::make_pair(false, 0);
+			for (const size_t u : outNeighbors[v])
+			{
+				if (ignorableTip[u]) continue;
+				if (u == v) continue;
+				if (u == start) return std::make_pair(false, 0);
+				seen.insert(u);
+				bool hasNonvisitedParent = false;
+				for (const size_t w : inNeighbors[u])
+				{
+					if (w == u) continue;
+					if (!ignorableTip[w] && visited.count(w) == 0)
+					{
+						hasNonvisitedParent = true;
+						break;
+					}
+				}
+				if (!hasNonvisitedParent) S.push_back(u);
+			}
+			if (S.size() == 1 && seen.size() == 1 && seen.count(S[0]) == 1)
+			{
+				const size_t t = S.back();
+				for (const size_t u : outNeighbors[t])
+				{
+					if (u == start) return std::make_pair(false, 0);
+				}
+				return std::make_pair(true, t);
+			}
+		}
+		return std::make_pair(false, 0);
+	}
+
+	// src/AlignmentGraph.cpp:366-393
+	void chainBubble(const size_t start, const std::vector<bool>& ignorableTip, std::vector<size_t>& rank)
+	{
+		bool hasBubble;
+		size_t bubbleEnd;
+		std::tie(hasBubble, bubbleEnd) = findBubble(start, ignorableTip);
+		if (!hasBubble) return;
+		std::unordered_set<size_t> visited;
+		std::vector<size_t> stack;
+		stack.push_back(start);
+		ufMerge(chainNumber, rank, start, bubbleEnd);
+		while (stack.size() > 0)
+		{
+			const size_t top = stack.back();
+			stack.pop_back();
+			if (visited.count(top) == 1) continue;
+			if (ignorableTip[top]) continue;
+			visited.insert(top);
+			ufMerge(chainNumber, rank, start, top);
+			for (const auto neighbor : outNeighbors[top])
+			{
+				if (visited.count(neighbor) == 1) continue;
+				if (neighbor == bubbleEnd) continue;
+				stack.push_back(neighbor);
+			}
+		}
+	}
+
+	// src/AlignmentGraph.cpp:395-426
+	void fixChainApproxPos(const size_t start)
+	{
+		std::vector<std::pair<size_t, size_t>> stack;
+		size_t chain = chainNumber[start];
+		stack.emplace_back(start, (nodeLength.size() + 5) * SPLIT_NODE_SIZE);
+		while (stack.size() > 0)
+		{
+			size_t v, dist;
+			std::tie(v, dist) = stack.back();
+			stack.pop_back();
+			if (chainApproxPos[v] != NMAX) continue;
+			chainApproxPos[v] = dist;
+			for (const size_t u : outNeighbors[v])
+			{
+				if (chainNumber[u] != chain) continue;
+				if (chainApproxPos[u] != NMAX) continue;
+				stack.emplace_back(u, dist + nodeLength[u]);
+			}
+			for (const size_t u : inNeighbors[v])
+			{
+				if (chainNumber[u] != chain) continue;
+				if (chainApproxPos[u] != NMAX) continue;
+				stack.emplace_back(u, dist - nodeLength[v]);
+			}
+		}
+	}
+
+	// src/AlignmentGraph.cpp:428-527. The result map's iteration order
+	// (phmap in the reference) is replaced by ascending chain number.
+	std::map<size_t, std::unordered_set<size_t>> chainTips(std::vector<size_t>& rank, std::vector<bool>& ignorableTip)
+	{
+		std::vector<size_t> order;
+		for (size_t i = 0; i < NodeSize(); i++) order.push_back(i);
+		std::sort(order.begin(), order.end(), [this](size_t left, size_t right) { return componentNumber[left] < componentNumber[right]; });
+		std::vector<bool> fwTipComponent(componentNumber[order.back()] + 1, true);
+		for (size_t ind = order.size() - 1; ind < order.size(); ind--)
+		{
+			size_t i = order[ind];
+			if (!fwTipComponent[componentNumber[i]]) continue;
+			for (auto neighbor : outNeighbors[i])
+			{
+				if (componentNumber[neighbor] == componentNumber[i]) { fwTipComponent[componentNumber[i]] = false; break; }
+				if (!fwTipComponent[componentNumber[neighbor]]) { fwTipComponent[componentNumber[i]] = false; break; }
+			}
+		}
+		for (size_t ind = order.size() - 1; ind < order.size(); ind--)
+		{
+			size_t i = order[ind];
+			if (!fwTipComponent[componentNumber[i]]) continue;
+			for (auto neighbor : outNeighbors[i]) ufMerge(chainNumber, rank, i, neighbor);
+		}
+		std::vector<bool> bwTipComponent(componentNumber[order.back()] + 1, true);
+		for (size_t ind = 0; ind < order.size(); ind++)
+		{
+			size_t i = order[ind];
+			if (!bwTipComponent[componentNumber[i]]) continue;
+			for (auto neighbor : inNeighbors[i])
+			{
+				if (componentNumber[neighbor] == componentNumber[i]) { bwTipComponent[componentNumber[i]] = false; break; }
+				if (!bwTipComponent[componentNumber[neighbor]]) { bwTipComponent[componentNumber[i]] = false; break; }
+			}
+		}
+		for (size_t ind = 0; ind < order.size(); ind++)
+		{
+			size_t i = order[ind];
+			if (!bwTipComponent[componentNumber[i]]) continue;
+			for (auto neighbor : inNeighbors[i]) ufMerge(chainNumber, rank, i, neighbor);
+		}
+		std::map<size_t, std::unordered_set<size_t>> result;
+		for (size_t i = 0; i < NodeSize(); i++)
+		{
+			if (bwTipComponent[componentNumber[i]] || fwTipComponent[componentNumber[i]]) ignorableTip[i] = true;
+			if (bwTipComponent[componentNumber[i]])
+			{
+				for (auto neighbor : outNeighbors[i])
+				{
+					if (chainNumber[neighbor] == chainNumber[i]) continue;
+					result[chainNumber[i]].insert(neighbor);
+				}
+			}
+			if (fwTipComponent[componentNumber[i]])
+			{
+				for (auto neighbor : inNeighbors[i])
+				{
+					if (chainNumber[neighbor] == chainNumber[i]) continue;
+					result[chainNumber[i]].insert(neighbor);
+				}
+			}
+		}
+		return result;
+	}
+
+	// src/AlignmentGraph.cpp:529-569
+	void chainCycles(std::vector<size_t>& rank, std::vector<bool>& ignorableTip)
+	{
+		for (size_t i = 0; i < nodeLength.size(); i++)
+		{
+			size_t uniqueFwNeighbor = NMAX;
+			for (auto u : outNeighbors[i])
+			{
+				if (ignorableTip[u]) continue;
+				if (u == i) continue;
+				if (uniqueFwNeighbor == NMAX) uniqueFwNeighbor = u;
+				else uniqueFwNeighbor = NMAX - 1;
+			}
+			size_t uniqueBwNeighbor = NMAX;
+			for (auto u : inNeighbors[i])
+			{
+				if (ignorableTip[u]) continue;
+				if (u == i) continue;
+				if (uniqueBwNeighbor == NMAX) uniqueBwNeighbor = u;
+				else if (u != uniqueBwNeighbor) uniqueBwNeighbor = NMAX - 1;
+			}
+			if (uniqueFwNeighbor != uniqueBwNeighbor) continue;
+			if (uniqueFwNeighbor == NMAX || uniqueFwNeighbor == NMAX - 1) continue;
+			ignorableTip[i] = true;
+			ufMerge(chainNumber, rank, i, uniqueFwNeighbor);
+		}
+	}
+
+	// src/AlignmentGraph.cpp:571-630
+	void findChains()
+	{
+		chainNumber.resize(nodeLength.size(), NMAX);
+		for (size_t i = 0; i < chainNumber.size(); i++) chainNumber[i] = i;
+		std::vector<bool> ignorableTip(nodeLength.size(), false);
+		std::vector<size_t> rank(nodeLength.size(), 0);
+		for (const auto& pair : nodeLookup)
+		{
+			for (size_t i = 1; i < pair.second.size(); i++) ufMerge(chainNumber, rank, pair.second[0], pair.second[i]);
+		}
+		auto tipChainers = chainTips(rank, ignorableTip);
+		chainCycles(rank, ignorableTip);
+		for (const auto& pair : nodeLookup) chainBubble(pair.second.back(), ignorableTip, rank);
+		for (auto& pair : tipChainers)
+		{
+			size_t uniqueNeighbor = NMAX;
+			for (auto n : pair.second)
+			{
+				if (uniqueNeighbor == NMAX) uniqueNeighbor = chainNumber[n];
+				if (uniqueNeighbor != chainNumber[n])
+				{
+					uniqueNeighbor = NMAX - 1;
+					break;
+				}
+			}
+			if (uniqueNeighbor == NMAX - 1) continue;
+			ufMerge(chainNumber, rank, pair.first, *pair.second.begin());
+		}
+		for (size_t i = 0; i < chainNumber.size(); i++) ufFind(chainNumber, i);
+		chainApproxPos.assign(nodeLength.size(), NMAX);
+		for (size_t i = 0; i < chainNumber.size(); i++)
+		{
+			if (chainApproxPos[i] == NMAX) fixChainApproxPos(i);
+		}
+	}
+
+	// src/AlignmentGraph.cpp:730-774
+	char NodeSequences(size_t node, size_t pos) const
+	{
+		if (node < firstAmbiguous)
+		{
+			size_t chunk = pos / BP_IN_CHUNK;
+			size_t offset = (pos % BP_IN_CHUNK) * 2;
+			return "ACGT"[(nodeSequences[node][chunk] >> offset) & 3];
+		}
+		const auto& s = ambiguousNodeSequences[node - firstAmbiguous];
+		bool A = (s.A >> pos) & 1, C = (s.C >> pos) & 1, G = (s.G >> pos) & 1, T = (s.T >> pos) & 1;
+		if (A && !C && !G && !T) return 'A';
+		if (!A && C && !G && !T) return 'C';
+		if (!A && !C && G && !T) return 'G';
+		if (!A && !C && !G && T) return 'T';
+		if (A && !C && G && !T) return 'R';
+		if (!A && C && !G && T) return 'Y';
+		if (!A && C && G && !T) return 'S';
+		if (A && !C && !G && T) return 'W';
+		if (!A && !C && G && T) return 'K';
+		if (A && C && !G && !T) return 'M';
+		if (!A && C && G && T) return 'B';
+		if (A && !C && G && T) return 'D';
+		if (A && C && !G && T) return 'H';
+		if (A && C && G && !T) return 'V';
+		return 'N';
+	}
+
+	// src/AlignmentGraph.cpp:812-828
+	size_t GetUnitigNode(int nodeId, size_t offset) const
+	{
+		const auto& nodes = nodeLookup.at(nodeId);
+		size_t index = nodes.size() * ((double)offset / (double)originalNodeSize.at(nodeId));
+		if (index >= nodes.size()) index = nodes.size() - 1;
+		while (index < nodes.size() - 1 && (nodeOffset[nodes[index]] + NodeLength(nodes[index]) <= offset)) index++;
+		while (index > 0 && (nodeOffset[nodes[index]] > offset)) index--;
+		return nodes[index];
+	}
+
+	// src/AlignmentGraph.cpp:830-847
+	std::pair<int, size_t> GetReversePosition(int nodeId, size_t offset) const
+	{
+		size_t originalSize = originalNodeSize.at(nodeId);
+		size_t newOffset = originalSize - offset - 1;
+		int reverseNodeId = (nodeId % 2 == 0) ? (nodeId / 2) * 2 + 1 : (nodeId / 2) * 2;
+		return std::make_pair(reverseNodeId, newOffset);
+	}
+};
+
+// src/BigraphToDigraph.cpp:106-132
+inline std::pair<std::tuple<size_t, size_t, size_t>, std::tuple<size_t, size_t, size_t>> ConvertGFAEdgeToEdges(int from, bool fromPlus, int to, bool toPlus, size_t overlap)
+{
+	size_t fromLeft, fromRight, toLeft, toRight;
+	if (!fromPlus) { fromLeft = from * 2; fromRight = from * 2 + 1; }
+	else { fromLeft = from * 2 + 1; fromRight = from * 2; }
+	if (!toPlus) { toLeft = to * 2; toRight = to * 2 + 1; }
+	else { toLeft = to * 2 + 1; toRight = to * 2; }
+	return std::make_pair(std::make_tuple(fromRight, toRight, overlap), std::make_tuple(toLeft, fromLeft, overlap));
+}
+
+// src/BigraphToDigraph.cpp:215-267
+inline AlignmentGraph BuildFromGFA(const GfaGraph& graph)
+{
+	static const std::string allowedChars = "aAcCgGtTyYrRwWsSkKmMdDvVhHbBnN";
+	AlignmentGraph result;
+	result.DBGoverlap = graph.edgeOverlap;
+	std::unordered_map<int, std::vector<size_t>> breakpoints;
+	for (auto pair : graph.varyingOverlaps)
+	{
+		int to = pair.first.second.id * 2;
+		if (!pair.first.second.end) to += 1;
+		int from = pair.first.first.Reverse().id * 2;
+		if (!pair.first.first.Reverse().end) from += 1;
+		breakpoints[from].push_back(pair.second);
+		breakpoints[to].push_back(pair.second);
+	}
+	for (auto node : graph.nodes)
+	{
+		for (char c : node.second)
+			if (allowedChars.find(c) == std::string::npos) throw std::runtime_error("Invalid sequence character");
+		std::string name = graph.OriginalNodeName(node.first);
+		std::vector<size_t> breakpointsFw = breakpoints[node.first * 2];
+		std::vector<size_t> breakpointsBw = breakpoints[node.first * 2 + 1];
+		breakpointsFw.push_back(0);
+		breakpointsFw.push_back(node.second.size());
+		breakpointsBw.push_back(0);
+		breakpointsBw.push_back(node.second.size());
+		std::sort(breakpointsFw.begin(), breakpointsFw.end());
+		std::sort(breakpointsBw.begin(), breakpointsBw.end());
+		// ConvertGFANodeToNodes (src/BigraphToDigraph.cpp:101-104): +strand is rightEnd=true
+		result.AddNode(node.first * 2, node.second, name, false, breakpointsFw);
+		result.AddNode(node.first * 2 + 1, ReverseComplement(node.second), name, true, breakpointsBw);
+	}
+	for (auto edge : graph.edges)
+	{
+		for (auto target : edge.second)
+		{
+			auto overlap = graph.edgeOverlap;
+			if (graph.varyingOverlaps.count(std::make_pair(edge.first, target)) == 1)
+				overlap = graph.varyingOverlaps.at(std::make_pair(edge.first, target));
+			auto pair = ConvertGFAEdgeToEdges(edge.first.id, edge.first.end, target.id, target.end, overlap);
+			result.AddEdgeNodeId(std::get<0>(pair.first), std::get<1>(pair.first), std::get<2>(pair.first));
+			result.AddEdgeNodeId(std::get<0>(pair.second), std::get<1>(pair.second), std::get<2>(pair.second));
+		}
+	}
+	result.Finalize();
+	return result;
+}
+
+inline AlignmentGraph LoadGraphFromGFA(const std::string& path)
+{
+	std::ifstream file{path};
+	if (!file.good()) throw std::runtime_error("cannot open graph file");
+	auto gfa = GfaGraph::LoadFromStream(file);
+	return BuildFromGFA(gfa);
+}
+
+// ---------------------------------------------------------------------------
+// WordSlice (src/WordSlice.h:150-764)
+// ---------------------------------------------------------------------------
+struct WordSlice
+{
+	Word VP = 0;
+	Word VN = 0;
+	ScoreType scoreEnd = 0;
+	WordSlice() {}
+	WordSlice(Word VP, Word VN, ScoreType scoreEnd) : VP(VP), VN(VN), scoreEnd(scoreEnd) {}
+
+	// src/WordSlice.h:177-186
+	ScoreType getValue(int row) const
+	{
+		Word mask = AllZeros;
+		if (row < WordSize - 1) mask = (AllOnes << (row + 1));
+		return scoreEnd + popcount(VN & mask) - popcount(VP & mask);
+	}
+	// src/WordSlice.h:244-247
+	ScoreType getScoreBeforeStart() const { return scoreEnd - popcount(VP) + popcount(VN); }
+	// src/WordSlice.h:239-242
+	ScoreType getXScore(ScoreType seqOffset, int offset, ScoreType errorCost) const
+	{
+		return (seqOffset + offset + 1) * 100 - getValue(offset) * errorCost;
+	}
+
+	// src/WordSlice.h:313-347
+	ScoreType maxXScoreLocalMinima(ScoreType seqOffset, ScoreType errorCost, size_t cells, Word extraMask) const
+	{
+		ScoreType scoreBeforeStart = getScoreBeforeStart();
+		Word priorityCausedMinima = ~VP;
+		Word possibleLocalMinima = (VP & (priorityCausedMinima - VP));
+		possibleLocalMinima >>= 1;
+		possibleLocalMinima |= LastBit & (priorityCausedMinima | ~(priorityCausedMinima - VP)) & ~VP;
+		ScoreType result = SMIN;
+		possibleLocalMinima &= extraMask;
+		possibleLocalMinima |= 1;
+		if (cells < (size_t)WordSize)
+		{
+			possibleLocalMinima |= (Word)1 << (Word)(cells - 1);
+			possibleLocalMinima &= ~(AllOnes << cells);
+		}
+		else
+		{
+			possibleLocalMinima |= (Word)1 << (Word)(WordSize - 1);
+		}
+		ScoreType zeroScore = seqOffset * 100 - scoreBeforeStart * errorCost;
+		while (possibleLocalMinima != 0)
+		{
+			Word currentMinimumMask = possibleLocalMinima ^ (possibleLocalMinima - 1);
+			ScoreType cellsHere = popcount(currentMinimumMask);
+			ScoreType scoreHere = (ScoreType)popcount(VP & currentMinimumMask) - (ScoreType)popcount(VN & currentMinimumMask);
+			result = std::max(result, cellsHere * 100 - scoreHere * errorCost);
+			possibleLocalMinima &= ~currentMinimumMask;
+		}
+		result += zeroScore;
+		return result;
+	}
+	// src/WordSlice.h:223-237
+	ScoreType maxXScoreFirstSlices(ScoreType seqOffset, ScoreType errorCost, int cells, Word extraMask = AllOnes) const
+	{
+		ORC_ASSERT(cells > 0);
+		ORC_ASSERT(cells <= WordSize);
+		return maxXScoreLocalMinima(seqOffset, errorCost, cells, extraMask);
+	}
+	ScoreType maxXScore(ScoreType seqOffset, ScoreType errorCost, Word extraMask = AllOnes) const
+	{
+		return maxXScoreFirstSlices(seqOffset, errorCost, WordSize, extraMask);
+	}
+
+	// src/WordSlice.h:566-664
+	static std::pair<Word, Word> differenceMasksBitTwiddle(Word leftVP, Word leftVN, Word rightVP, Word rightVN, int scoreDifference)
+	{
+		Word leftSmaller = 0;
+		Word rightSmaller = 0;
+		Word VPcommon = ~(leftVP & rightVP);
+		Word VNcommon = ~(leftVN & rightVN);
+		leftVP &= VPcommon;
+		leftVN &= VNcommon;
+		rightVP &= VPcommon;
+		rightVN &= VNcommon;
+		Word twosmaller = leftVN & rightVP;
+		Word onesmaller = (rightVP & ~leftVN) | (leftVN & ~rightVP);
+		Word onebigger = (leftVP & ~rightVN) | (rightVN & ~leftVP);
+		Word twobigger = rightVN & leftVP;
+		onebigger |= twobigger;
+		onesmaller |= twosmaller;
+		if (scoreDifference > 0)
+		{
+			for (int i = 1; i < scoreDifference; i++)
+			{
+				Word leastSignificant = onebigger & ~(onebigger - 1);
+				onebigger ^= (~twobigger & leastSignificant);
+				twobigger &= ~leastSignificant;
+				if (onebigger == 0) return std::make_pair(AllOnes, AllZeros);
+			}
+			Word leastSignificant = onebigger & ~(onebigger - 1);
+			leftSmaller |= leastSignificant - 1;
+			onebigger ^= (~twobigger & leastSignificant);
+			twobigger &= ~leastSignificant;
+		}
+		else if (scoreDifference < 0)
+		{
+			for (int i = 1; i < -scoreDifference; i++)
+			{
+				Word leastSignificant = onesmaller & ~(onesmaller - 1);
+				onesmaller ^= (~twosmaller & leastSignificant);
+				twosmaller &= ~leastSignificant;
+				if (onesmaller == 0) return std::make_pair(AllZeros, AllOnes);
+			}
+			Word leastSignificant = onesmaller & ~(onesmaller - 1);
+			rightSmaller |= leastSignificant - 1;
+			onesmaller ^= (~twosmaller & leastSignificant);
+			twosmaller &= ~leastSignificant;
+		}
+		for (int i = 0; i < WordSize; i++)
+		{
+			if (onesmaller == 0)
+			{
+				if (onebigger == 0) break;
+				Word leastSignificant = onebigger & ~(onebigger - 1);
+				rightSmaller |= -leastSignificant;
+				break;
+			}
+			if (onebigger == 0)
+			{
+				Word leastSignificant = onesmaller & ~(onesmaller - 1);
+				leftSmaller |= -leastSignificant;
+				break;
+			}
+			Word leastSignificantBigger = onebigger & ~(onebigger - 1);
+			Word leastSignificantSmaller = onesmaller & ~(onesmaller - 1);
+			if (leastSignificantBigger > leastSignificantSmaller) leftSmaller |= leastSignificantBigger - leastSignificantSmaller;
+			else rightSmaller |= leastSignificantSmaller - leastSignificantBigger;
+			onebigger ^= (~twobigger & leastSignificantBigger);
+			twobigger &= ~leastSignificantBigger;
+			onesmaller ^= (~twosmaller & leastSignificantSmaller);
+			twosmaller &= ~leastSignificantSmaller;
+		}
+		return std::make_pair(leftSmaller, rightSmaller);
+	}
+
+	// src/WordSlice.h:516-541
+	static WordSlice mergeTwoSlices(WordSlice left, WordSlice right, Word leftSmaller, Word rightSmaller)
+	{
+		WordSlice result;
+		auto mask = (rightSmaller | ((leftSmaller | rightSmaller) - (rightSmaller << 1))) & ~leftSmaller;
+		uint64_t leftReduction = leftSmaller & (rightSmaller << 1);
+		uint64_t rightReduction = rightSmaller & (leftSmaller << 1);
+		if ((rightSmaller & 1) && left.getScoreBeforeStart() < right.getScoreBeforeStart()) rightReduction |= 1;
+		left.VN &= ~leftReduction;
+		right.VN &= ~rightReduction;
+		result.VN = (left.VN & ~mask) | (right.VN & mask);
+		result.VP = (left.VP & ~mask) | (right.VP & mask);
+		result.scoreEnd = std::min(left.scoreEnd, right.scoreEnd);
+		return result;
+	}
+	// src/WordSlice.h:502-511
+	static WordSlice mergeTwoSlices(WordSlice left, WordSlice right)
+	{
+		if (left.getScoreBeforeStart() > right.getScoreBeforeStart()) std::swap(left, right);
+		auto masks = differenceMasksBitTwiddle(left.VP, left.VN, right.VP, right.VN, right.getScoreBeforeStart() - left.getScoreBeforeStart());
+		return mergeTwoSlices(left, right, masks.first, masks.second);
+	}
+	WordSlice mergeWith(const WordSlice& other) const { return mergeTwoSlices(*this, other); }
+
+	// src/WordSlice.h:438-472
+	ScoreType changedMinScore(WordSlice oldSlice) const
+	{
+		ScoreType scoreBeforeStart = getScoreBeforeStart();
+		ScoreType otherScoreBeforeStart = oldSlice.getScoreBeforeStart();
+		Word possibleLocalMinima = (VP & (VN - VP));
+		possibleLocalMinima >>= 1;
+		possibleLocalMinima |= LastBit & (VN | ~(VN - VP)) & ~VP;
+		if (scoreEnd + popcount(VN) >= oldSlice.scoreEnd - popcount(oldSlice.VP))
+		{
+			auto masks = differenceMasksBitTwiddle(VP, VN, oldSlice.VP, oldSlice.VN, otherScoreBeforeStart - scoreBeforeStart);
+			auto smaller = masks.first;
+			if (smaller != AllOnes)
+			{
+				possibleLocalMinima |= (~smaller) >> 1;
+				possibleLocalMinima |= (~smaller) << 1;
+				possibleLocalMinima |= 1;
+				possibleLocalMinima |= LastBit;
+				possibleLocalMinima &= smaller;
+			}
+		}
+		ScoreType result = (scoreBeforeStart < otherScoreBeforeStart) ? scoreBeforeStart : SMAX;
+		while (possibleLocalMinima != 0)
+		{
+			Word currentMinimumMask = possibleLocalMinima ^ (possibleLocalMinima - 1);
+			ScoreType scoreHere = scoreBeforeStart + popcount(VP & currentMinimumMask) - popcount(VN & currentMinimumMask);
+			result = std::min(result, scoreHere);
+			possibleLocalMinima &= ~currentMinimumMask;
+		}
+		return result;
+	}
+	bool operator==(const WordSlice& o) const { return VP == o.VP && VN == o.VN && scoreEnd == o.scoreEnd; }
+};
+
+// src/GraphAlignerBitvectorCommon.h:1056-1060
+inline WordSlice getSourceSliceFromScore(ScoreType previousScore)
+{
+	return WordSlice{AllOnes, AllZeros, previousScore + WordSize};
+}
+
+// src/GraphAlignerBitvectorCommon.h:242-262
+inline WordSlice getNextSlice(Word Eq, WordSlice slice, Word& hinP, Word& hinN, Word& PhOut, Word& MhOut)
+{
+	Word Xv = Eq | slice.VN;
+	Eq |= hinN;
+	Word Xh = (((Eq & slice.VP) + slice.VP) ^ slice.VP) | Eq;
+	Word Ph = slice.VN | ~(Xh | slice.VP);
+	Word Mh = slice.VP & Xh;
+	Word tempMh = (Mh << 1) | hinN;
+	hinN = Mh >> (WordSize - 1);
+	Word tempPh = (Ph << 1) | hinP;
+	slice.VP = tempMh | ~(Xv | tempPh);
+	hinP = Ph >> (WordSize - 1);
+	slice.VN = tempPh & Xv;
+	slice.scoreEnd -= hinN;
+	slice.scoreEnd += hinP;
+	PhOut = Ph;
+	MhOut = Mh;
+	return slice;
+}
+
+// ---------------------------------------------------------------------------
+// Common types (src/GraphAlignerCommon.h, src/NodeSlice.h)
+// ---------------------------------------------------------------------------
+struct Params
+{
+	size_t alignmentBandwidth = 10;
+	size_t maxCellsPerSlice = NMAX;
+	bool sloppyOptimizations = true;
+	size_t minSeedClusterSize = 1;
+	double seedExtendDensity = -1;
+	ScoreType XscoreErrorCost = 294;
+	int Xdropcutoff = 14705;
+	double multimapScoreFraction = 0.9;
+	// src/GraphAlignerCommon.h:89-100
+	static ScoreType errorCostFromIdentity(double preciseClippingIdentityCutoff)
+	{
+		return 100 * (preciseClippingIdentityCutoff / (1.0 - preciseClippingIdentityCutoff) + 1.0);
+	}
+};
+
+struct MatrixPosition
+{
+	size_t node = 0;
+	size_t nodeOffset = 0;
+	size_t seqPos = 0;
+	MatrixPosition() {}
+	MatrixPosition(size_t node, size_t nodeOffset, size_t seqPos) : node(node), nodeOffset(nodeOffset), seqPos(seqPos) {}
+	bool operator==(const MatrixPosition& o) const { return node == o.node && nodeOffset == o.nodeOffset && seqPos == o.seqPos; }
+	bool operator!=(const MatrixPosition& o) const { return !(*this == o); }
+};
+
+struct TraceItem
+{
+	MatrixPosition DPposition;
+	bool nodeSwitch = false;
+	char sequenceCharacter = '-';
+	char graphCharacter = '-';
+	TraceItem() {}
+	TraceItem(MatrixPosition pos, bool nodeSwitch, const std::string_view& seq, const AlignmentGraph& graph)
+		: DPposition(pos), nodeSwitch(nodeSwitch),
+		  sequenceCharacter(pos.seqPos < seq.size() ? seq[pos.seqPos] : '-'),
+		  graphCharacter(graph.NodeSequences(pos.node, pos.nodeOffset)) {}
+};
+
+struct OnewayTrace
+{
+	std::vector<TraceItem> trace;
+	ScoreType score = 0;
+	static OnewayTrace TraceFailed()
+	{
+		OnewayTrace r;
+		r.score = SMAX;
+		return r;
+	}
+	bool failed() const { return score == SMAX; }
+};
+
+// NodeSliceMapItemStruct (src/NodeSlice.h:15-47) with NUM_CHUNKS == 1
+struct NodeItem
+{
+	WordSlice startSlice;
+	WordSlice endSlice;
+	bool exists = false;
+	Word HP = 0;
+	Word HN = 0;
+	ScoreType minScore = 0;
+};
+
+// NodeSlice<..., false> (src/NodeSlice.h:49-426): the reference keeps a phmap; here
+// the items are kept in the order nodes were added, with a hash index for lookup.
+struct NodeSlice
+{
+	std::vector<std::pair<size_t, NodeItem>> items;
+	std::unordered_map<size_t, size_t> index;
+	bool hasNode(size_t n) const { return index.count(n) == 1; }
+	NodeItem& node(size_t n) { return items[index.at(n)].second; }
+	const NodeItem& node(size_t n) const { return items[index.at(n)].second; }
+	// src/NodeSlice.h:261-270 (addNodeToMap)
+	void addNode(size_t n)
+	{
+		index[n] = items.size();
+		items.emplace_back(n, NodeItem{});
+		auto& item = items.back().second;
+		item.minScore = SMAX;
+		item.startSlice = WordSlice{0, 0, SMAX};
+		item.endSlice = WordSlice{0, 0, SMAX};
+	}
+	size_t size() const { return items.size(); }
+};
+
+struct DPSlice
+{
+	ScoreType minScore = SMAX;
+	size_t minScoreNode = NMAX;
+	size_t minScoreNodeOffset = NMAX;
+	ScoreType maxExactEndposScore = SMIN;
+	size_t maxExactEndposNode = NMAX;
+	NodeSlice scores;
+	size_t j = NMAX;
+	size_t cellsProcessed = 0;
+	size_t bandwidth = 0;
+	bool scoresNotValid = false;
+};
+
+struct NodeCalculationResult
+{
+	ScoreType minScore;
+	size_t minScoreNode;
+	size_t minScoreNodeOffset;
+	size_t maxExactEndposNode;
+	ScoreType maxExactEndposScore;
+	size_t cellsProcessed;
+};
+
+struct EdgeWithPriority
+{
+	size_t target;
+	WordSlice incoming;
+	bool skipFirst;
+	bool forceCalculation = false;
+};
+
+// ComponentPriorityQueue<EdgeWithPriority, true> (src/ComponentPriorityQueue.h:8-156),
+// keyed (component, score) with ties broken by insertion sequence.
+struct ComponentQueue
+{
+	struct Item
+	{
+		size_t component;
+		int score;
+		uint64_t seq;
+		size_t index;
+		bool operator>(const Item& o) const
+		{
+			if (component != o.component) return component > o.component;
+			if (score != o.score) return score > o.score;
+			return seq > o.seq;
+		}
+	};
+	std::priority_queue<Item, std::vector<Item>, std::greater<Item>> activeQueues;
+	std::unordered_set<size_t> active;
+	std::unordered_map<size_t, std::vector<EdgeWithPriority>> extras;
+	uint64_t seqCounter = 0;
+
+	size_t size() const { return activeQueues.size(); }
+	size_t topIndex() const { return activeQueues.top().index; }
+	// src/ComponentPriorityQueue.h:84-95
+	void insert(size_t component, int score, const EdgeWithPriority& item)
+	{
+		size_t index = item.target;
+		if (active.count(index) == 0)
+		{
+			activeQueues.push(Item{component, score, seqCounter++, index});
+			active.insert(index);
+		}
+		extras[index].push_back(item);
+	}
+	// src/ComponentPriorityQueue.h:60-69
+	void pop()
+	{
+		size_t index = activeQueues.top().index;
+		extras[index].clear();
+		active.erase(index);
+		activeQueues.pop();
+	}
+	const std::vector<EdgeWithPriority>& getExtras(size_t index) { return extras[index]; }
+	size_t extraSize(size_t index) const
+	{
+		auto found = extras.find(index);
+		if (found == extras.end()) return 0;
+		return found->second.size();
+	}
+	void clear()
+	{
+		while (activeQueues.size() > 0) activeQueues.pop();
+		active.clear();
+		extras.clear();
+		seqCounter = 0;
+	}
+};
+
+// EqVector (src/GraphAlignerBitvectorCommon.h:36-136); masks ordered A, C, G, T
+struct EqVector
+{
+	Word masks[4];
+};
+
+// Per-slice record kept for parity checks on the device path: the words the
+// reference stores for each slice (src/GraphAlignerBitvectorCommon.h:137-213).
+struct SliceSummary
+{
+	int64_t j;
+	ScoreType minScore;
+	int64_t minScoreNode;
+	int64_t minScoreNodeOffset;
+	ScoreType maxExactEndposScore;
+	int64_t maxExactEndposNode;
+	int64_t cellsProcessed;
+	int64_t numNodes;
+	uint64_t itemsHash;
+	int32_t scoresNotValid;
+};
+
+inline uint64_t mixHash(uint64_t h, uint64_t v)
+{
+	h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+	return h;
+}
+
+// Order-independent digest of a slice's band items (node id, start/end words, HP/HN, minScore).
+inline uint64_t itemsDigest(const NodeSlice& s)
+{
+	uint64_t total = 0;
+	for (const auto& p : s.items)
+	{
+		const auto& it = p.second;
+		uint64_t h = 0x12345;
+		h = mixHash(h, p.first);
+		h = mixHash(h, it.startSlice.VP);
+		h = mixHash(h, it.startSlice.VN);
+		h = mixHash(h, (uint32_t)it.startSlice.scoreEnd);
+		h = mixHash(h, it.endSlice.VP);
+		h = mixHash(h, it.endSlice.VN);
+		h = mixHash(h, (uint32_t)it.endSlice.scoreEnd);
+		h = mixHash(h, it.HP);
+		h = mixHash(h, it.HN);
+		h = mixHash(h, (uint32_t)it.minScore);
+		total += h;
+	}
+	return total;
+}
+
+// ---------------------------------------------------------------------------
+// Bitvector banded DP (src/GraphAlignerBitvectorCommon.h, src/GraphAlignerBitvectorBanded.h)
+// ---------------------------------------------------------------------------
+struct Extender
+{
+	const Params& params;
+	const AlignmentGraph& graph;
+	// per-extension band membership (AlignerGraphsizedState::currentBand/previousBand)
+	Extender(const Params& p, const AlignmentGraph& g) : params(p), graph(g) {}
+
+	// src/GraphAlignerBitvectorCommon.h:279-318
+	EqVector getEqVector(const std::string_view& sequence, size_t j) const
+	{
+		Word BA = 0, BT = 0, BC = 0, BG = 0;
+		for (int i = 0; i < WordSize && j + i < sequence.size(); i++)
+		{
+			Word mask = ((Word)1) << i;
+			switch (sequence[j + i])
+			{
+				case 'a': case 'A': BA |= mask; break;
+				case 'c': case 'C': BC |= mask; break;
+				case 'g': case 'G': BG |= mask; break;
+				case 't': case 'T': BT |= mask; break;
+				default:
+					if (characterMatch(sequence[j + i], 'A')) BA |= mask;
+					if (characterMatch(sequence[j + i], 'C')) BC |= mask;
+					if (characterMatch(sequence[j + i], 'T')) BT |= mask;
+					if (characterMatch(sequence[j + i], 'G')) BG |= mask;
+					break;
+			}
+		}
+		EqVector e;
+		e.masks[0] = BA;
+		e.masks[1] = BC;
+		e.masks[2] = BG;
+		e.masks[3] = BT;
+		return e;
+	}
+
+	// EqVector::getEqI for a node position (normal chunks or IUPAC masks)
+	Word eqAt(const EqVector& EqV, size_t node, size_t pos) const
+	{
+		if (node < graph.firstAmbiguous)
+		{
+			size_t code = (graph.nodeSequences[node][pos / 32] >> ((pos % 32) * 2)) & 3;
+			return EqV.masks[code];
+		}
+		const auto& s = graph.ambiguousNodeSequences[node - graph.firstAmbiguous];
+		Word result = 0;
+		if ((s.A >> pos) & 1) result |= EqV.masks[0];
+		if ((s.C >> pos) & 1) result |= EqV.masks[1];
+		if ((s.G >> pos) & 1) result |= EqV.masks[2];
+		if ((s.T >> pos) & 1) result |= EqV.masks[3];
+		return result;
+	}
+
+	// calculateNodeInner (src/GraphAlignerBitvectorCommon.h:1113-1478), non-multiseed
+	// path: the extra (seed-start) slice is always the fake slice, so the branches
+	// that test extraSlice.scoreEnd != INT_MAX are never taken.
+	template <bool AllowEarlyLeave, typename BandCheck, typename Callback>
+	NodeCalculationResult calculateNodeInner(size_t i, NodeItem& slice, const EqVector& EqV, NodeItem previousSlice, const std::vector<EdgeWithPriority>& incoming, BandCheck bandCheck, Callback callback, ScoreType seqOffset) const
+	{
+		ORC_ASSERT(incoming.size() > 0);
+		WordSlice newWs;
+		WordSlice ws;
+		bool hasWs = false;
+		NodeCalculationResult result;
+		result.minScore = SMAX;
+		result.minScoreNode = NMAX;
+		result.minScoreNodeOffset = NMAX;
+		result.maxExactEndposScore = SMIN;
+		result.maxExactEndposNode = NMAX;
+		result.cellsProcessed = 0;
+		auto nodeLength = graph.NodeLength(i);
+
+		Word Eq = eqAt(EqV, i, 0);
+		bool hasSkipless = false;
+		bool forceCalculation = false;
+		for (const auto& inc : incoming)
+		{
+			result.cellsProcessed++;
+			if (inc.forceCalculation) forceCalculation = true;
+			if (inc.skipFirst)
+			{
+				if (!hasWs)
+				{
+					ws = inc.incoming;
+					hasWs = true;
+				}
+				else
+				{
+					ws = ws.mergeWith(inc.incoming);
+				}
+				continue;
+			}
+			hasSkipless = true;
+			Word hinP, hinN;
+			ScoreType compareHin = SMAX;
+			if (previousSlice.exists) compareHin = previousSlice.startSlice.scoreEnd;
+			ScoreType incomingScoreBeforeStart = inc.incoming.getScoreBeforeStart();
+			if (compareHin < incomingScoreBeforeStart) { hinP = 0; hinN = 1; }
+			else if (compareHin > incomingScoreBeforeStart) { hinP = 1; hinN = 0; }
+			else { hinP = 0; hinN = 0; }
+			Word Ph, Mh;
+			WordSlice stepped = getNextSlice(Eq, inc.incoming, hinP, hinN, Ph, Mh);
+			if (!previousSlice.exists || stepped.getScoreBeforeStart() < previousSlice.startSlice.scoreEnd)
+			{
+				stepped.VP &= AllOnes ^ 1;
+				stepped.VN |= 1;
+			}
+			if (!hasWs)
+			{
+				ws = stepped;
+				hasWs = true;
+			}
+			else
+			{
+				ws = ws.mergeWith(stepped);
+			}
+		}
+		ORC_ASSERT(hasWs);
+		result.minScore = ws.scoreEnd;
+		result.minScoreNode = i;
+		result.minScoreNodeOffset = 0;
+		result.maxExactEndposScore = ws.maxXScore(seqOffset, params.XscoreErrorCost);
+		result.maxExactEndposNode = i;
+
+		if (!forceCalculation && slice.exists)
+		{
+			if (hasSkipless && graph.inNeighbors[i].size() == 1 && bandCheck(graph.inNeighbors[i][0]))
+			{
+				if (ws.scoreEnd > slice.startSlice.scoreEnd)
+				{
+					if (AllowEarlyLeave) return result;
+				}
+				else if (ws.scoreEnd < slice.startSlice.scoreEnd)
+				{
+				}
+				else
+				{
+					Word newBigger = (ws.VP & ~slice.startSlice.VP) | (slice.startSlice.VN & ~ws.VN);
+					Word oldBigger = (slice.startSlice.VP & ~ws.VP) | (ws.VN & ~slice.startSlice.VN);
+					if (newBigger > oldBigger)
+					{
+					}
+					else if (oldBigger > newBigger)
+					{
+						if (AllowEarlyLeave) return result;
+					}
+					else if (newBigger == 0 && oldBigger == 0)
+					{
+						if (AllowEarlyLeave) return result;
+					}
+					else
+					{
+						WordSlice test = ws.mergeWith(slice.startSlice);
+						if (test.scoreEnd == slice.startSlice.scoreEnd && test.VP == slice.startSlice.VP && test.VN == slice.startSlice.VN)
+						{
+							if (AllowEarlyLeave) return result;
+						}
+						ws = test;
+					}
+				}
+			}
+			else
+			{
+				WordSlice test = ws.mergeWith(slice.startSlice);
+				// src/GraphAlignerBitvectorCommon.h:1282 compares test.VP with startSlice.VN
+				if (test.scoreEnd == slice.startSlice.scoreEnd && test.VP == slice.startSlice.VP && test.VP == slice.startSlice.VN)
+				{
+					if (AllowEarlyLeave) return result;
+				}
+				ws = test;
+			}
+		}
+
+		if (previousSlice.exists)
+		{
+			if (ws.getScoreBeforeStart() > previousSlice.startSlice.scoreEnd)
+			{
+				ws = ws.mergeWith(getSourceSliceFromScore(previousSlice.startSlice.scoreEnd));
+			}
+		}
+
+		slice.HP = AllZeros;
+		slice.HN = AllZeros;
+
+		Word forceMask = AllZeros;
+		Word fixedHP = previousSlice.HP;
+		Word fixedHN = previousSlice.HN;
+		if (previousSlice.exists)
+		{
+			ScoreType scoreBefore = ws.getScoreBeforeStart();
+			ScoreType scoreComparison = previousSlice.startSlice.scoreEnd;
+			ORC_ASSERT(scoreBefore <= scoreComparison);
+			if (scoreBefore < scoreComparison)
+			{
+				// src/GraphAlignerBitvectorCommon.h:1364-1395 with NUM_CHUNKS == 1
+				forceMask |= (Word)1;
+				for (size_t fixoffset = 1; fixoffset < (size_t)WordSize; fixoffset++)
+				{
+					ScoreType newScoreComparison = scoreComparison;
+					newScoreComparison += (fixedHP >> fixoffset) & 1;
+					newScoreComparison -= (fixedHN >> fixoffset) & 1;
+					Word mask = ((Word)1) << fixoffset;
+					ORC_ASSERT(scoreBefore <= newScoreComparison);
+					if (scoreBefore < newScoreComparison)
+					{
+						fixedHP |= mask;
+						fixedHN &= ~mask;
+						forceMask |= mask;
+					}
+					if (scoreBefore == newScoreComparison)
+					{
+						fixedHP &= ~mask;
+						fixedHN &= ~mask;
+					}
+					scoreBefore++;
+					scoreComparison = newScoreComparison;
+					if (scoreBefore >= scoreComparison) break;
+				}
+			}
+		}
+		else
+		{
+			forceMask = AllOnes;
+		}
+		slice.startSlice = ws;
+		if (!AllowEarlyLeave) callback(ws);
+		slice.exists = true;
+		Word forceEq = AllOnes;
+		if (!previousSlice.exists) forceEq ^= 1;
+		size_t pos = 1;
+		for (; pos < nodeLength; pos++)
+		{
+			Eq = eqAt(EqV, i, pos);
+			Eq &= forceEq;
+			Word hinP = (fixedHP >> pos) & 1;
+			Word hinN = (fixedHN >> pos) & 1;
+			Word newHP, newHN;
+			newWs = getNextSlice(Eq, ws, hinP, hinN, newHP, newHN);
+			if ((forceMask >> pos) & 1)
+			{
+				newWs.VP &= AllOnes ^ 1;
+				newWs.VN |= 1;
+			}
+			hinP = newHP >> (WordSize - 1);
+			hinN = newHN >> (WordSize - 1);
+			ws = newWs;
+			if (ws.scoreEnd < result.minScore)
+			{
+				result.minScore = ws.scoreEnd;
+				result.minScoreNodeOffset = pos;
+			}
+			result.maxExactEndposScore = std::max(result.maxExactEndposScore, ws.maxXScore(seqOffset, params.XscoreErrorCost, newHN));
+			if (!AllowEarlyLeave) callback(ws);
+			slice.HP |= hinP << pos;
+			slice.HN |= hinN << pos;
+		}
+		result.cellsProcessed = pos;
+		slice.endSlice = ws;
+		if (previousSlice.exists) ORC_ASSERT(slice.endSlice.getScoreBeforeStart() <= previousSlice.endSlice.scoreEnd);
+		return result;
+	}
+
+	// src/GraphAlignerBitvectorCommon.h:1078-1102
+	std::vector<WordSlice> recalcNodeWordslice(size_t node, const NodeItem& slice, const EqVector& EqV, const NodeItem& previousSlice, ScoreType seqOffset) const
+	{
+		size_t nodeLength = graph.NodeLength(node);
+		std::vector<EdgeWithPriority> incoming;
+		incoming.push_back(EdgeWithPriority{node, slice.startSlice, true});
+		std::vector<WordSlice> result;
+		result.reserve(nodeLength);
+		auto sliceCopy = slice;
+		calculateNodeInner<false>(node, sliceCopy, EqV, previousSlice, incoming, [](size_t) { return false; }, [&result](const WordSlice& s) { result.push_back(s); }, seqOffset);
+		ORC_ASSERT(result.size() == nodeLength);
+		ORC_ASSERT(result[0] == slice.startSlice);
+		ORC_ASSERT(result.back() == slice.endSlice);
+		return result;
+	}
+
+	// calculateSlice (src/GraphAlignerBitvectorBanded.h:254-511), component-queue branch
+	NodeCalculationResult calculateSlice(const std::string_view& sequence, const size_t j, NodeSlice& currentSlice, const NodeSlice& previousSlice, ComponentQueue& calculableQueue, ScoreType previousQuitScore, int bandwidth, ScoreType previousMinScore) const
+	{
+		NodeCalculationResult result;
+		result.minScore = SMAX - bandwidth - 1;
+		result.minScoreNode = NMAX;
+		result.minScoreNodeOffset = NMAX;
+		result.maxExactEndposNode = 0;
+		result.maxExactEndposScore = SMIN;
+		result.cellsProcessed = 0;
+
+		EqVector EqV = getEqVector(sequence, j);
+		ORC_ASSERT(previousSlice.size() > 0);
+		for (const auto& p : previousSlice.items)
+		{
+			size_t node = p.first;
+			const NodeItem& item = p.second;
+			if (j != 0)
+			{
+				if (item.minScore > previousQuitScore) continue;
+				if (graph.linearizable[node])
+				{
+					auto neighbor = graph.inNeighbors[node][0];
+					if (previousSlice.hasNode(neighbor) && previousSlice.node(neighbor).endSlice.scoreEnd < previousQuitScore && previousSlice.node(neighbor).minScore < previousQuitScore) continue;
+				}
+			}
+			WordSlice startSlice = getSourceSliceFromScore(item.startSlice.scoreEnd);
+			calculableQueue.insert(graph.componentNumber[node], item.minScore, EdgeWithPriority{node, startSlice, true});
+		}
+		ORC_ASSERT(calculableQueue.size() > 0);
+
+		ScoreType currentMinScoreAtEndRow = result.minScore;
+		while (calculableQueue.size() > 0)
+		{
+			size_t i = calculableQueue.topIndex();
+			if (calculableQueue.extraSize(i) == 0)
+			{
+				calculableQueue.pop();
+				continue;
+			}
+			if (!currentSlice.hasNode(i)) currentSlice.addNode(i);
+			const std::vector<EdgeWithPriority> extras = calculableQueue.getExtras(i);
+			auto& thisNode = currentSlice.node(i);
+			auto oldEnd = thisNode.endSlice;
+			if (!thisNode.exists) oldEnd = WordSlice{0, 0, SMAX};
+			NodeItem previousThisNode;
+			if (previousSlice.hasNode(i))
+			{
+				previousThisNode = previousSlice.node(i);
+			}
+			else
+			{
+				previousThisNode.HP = AllOnes;
+				previousThisNode.HN = AllZeros;
+				previousThisNode.exists = false;
+			}
+			NodeCalculationResult nodeCalc = calculateNodeInner<true>(i, thisNode, EqV, previousThisNode, extras, [&previousSlice](size_t n) { return previousSlice.hasNode(n); }, [](const WordSlice&) {}, (ScoreType)j);
+			ORC_ASSERT(nodeCalc.maxExactEndposScore != SMIN);
+			calculableQueue.pop();
+			ORC_ASSERT(nodeCalc.minScore <= (ScoreType)j + bandwidth + (ScoreType)WordSize + (ScoreType)WordSize);
+			currentMinScoreAtEndRow = std::min(currentMinScoreAtEndRow, nodeCalc.minScore);
+			if (nodeCalc.minScore < thisNode.minScore) thisNode.minScore = nodeCalc.minScore;
+			auto newEnd = thisNode.endSlice;
+			if (newEnd.scoreEnd != oldEnd.scoreEnd || newEnd.VP != oldEnd.VP || newEnd.VN != oldEnd.VN)
+			{
+				ScoreType newEndMinScore = newEnd.changedMinScore(oldEnd);
+				ORC_ASSERT(newEndMinScore != SMAX);
+				if (newEndMinScore <= currentMinScoreAtEndRow + bandwidth)
+				{
+					for (auto neighbor : graph.outNeighbors[i])
+					{
+						calculableQueue.insert(graph.componentNumber[neighbor], newEndMinScore, EdgeWithPriority{neighbor, newEnd, false});
+					}
+				}
+			}
+			if (nodeCalc.minScore < result.minScore)
+			{
+				result.minScore = nodeCalc.minScore;
+				result.minScoreNode = nodeCalc.minScoreNode;
+				result.minScoreNodeOffset = nodeCalc.minScoreNodeOffset;
+			}
+			if (nodeCalc.maxExactEndposScore > result.maxExactEndposScore)
+			{
+				result.maxExactEndposScore = nodeCalc.maxExactEndposScore;
+				result.maxExactEndposNode = nodeCalc.maxExactEndposNode;
+			}
+			result.cellsProcessed += nodeCalc.cellsProcessed;
+			if (result.cellsProcessed > params.maxCellsPerSlice) break;
+		}
+		calculableQueue.clear();
+		return result;
+	}
+
+	// src/GraphAlignerBitvectorCommon.h:1551-1587
+	DPSlice getInitialSliceExactPosition(int bigraphNodeId, size_t offset) const
+	{
+		DPSlice result;
+		result.j = (size_t)-WordSize;
+		result.bandwidth = 1;
+		result.minScore = 0;
+		ORC_ASSERT(offset < graph.originalNodeSize.at(bigraphNodeId));
+		size_t nodeIndex = graph.GetUnitigNode(bigraphNodeId, offset);
+		size_t offsetInNode = offset - graph.nodeOffset[nodeIndex];
+		result.scores.addNode(nodeIndex);
+		result.minScoreNode = nodeIndex;
+		result.minScoreNodeOffset = offsetInNode;
+		result.maxExactEndposScore = 0;
+		result.maxExactEndposNode = nodeIndex;
+		auto& node = result.scores.node(nodeIndex);
+		node.startSlice = WordSlice{0, 0, (int)offsetInNode};
+		node.endSlice = WordSlice{0, 0, (int)graph.NodeLength(nodeIndex) - 1 - (int)offsetInNode};
+		node.minScore = 0;
+		node.exists = true;
+		node.HP = 0;
+		node.HN = 0;
+		for (size_t i = 1; i <= offsetInNode; i++) node.HN |= ((Word)1) << i;
+		for (size_t i = offsetInNode + 1; i < graph.NodeLength(nodeIndex); i++) node.HP |= ((Word)1) << i;
+		return result;
+	}
+
+	// getXdropSlices (src/GraphAlignerBitvectorBanded.h:580-725) + fillDPSlice (:513-563)
+	std::vector<DPSlice> getXdropSlices(const std::string_view& sequence, DPSlice initialSlice, size_t numSlices, double Xdropcutoff) const
+	{
+		std::vector<DPSlice> result;
+		result.reserve(numSlices + 1);
+		result.push_back(initialSlice);
+		ScoreType bestXScore = initialSlice.maxExactEndposScore;
+		ComponentQueue queue;
+		for (size_t slice = 0; slice < numSlices; slice++)
+		{
+			int bandwidth = params.alignmentBandwidth;
+			const DPSlice& lastSlice = result.back();
+			DPSlice newSlice;
+			newSlice.j = lastSlice.j + WordSize;
+			auto sliceResult = calculateSlice(sequence, newSlice.j, newSlice.scores, lastSlice.scores, queue, lastSlice.minScore + lastSlice.bandwidth, bandwidth, lastSlice.minScore);
+			newSlice.cellsProcessed = sliceResult.cellsProcessed;
+			newSlice.minScoreNode = sliceResult.minScoreNode;
+			newSlice.minScoreNodeOffset = sliceResult.minScoreNodeOffset;
+			newSlice.minScore = sliceResult.minScore;
+			newSlice.maxExactEndposScore = sliceResult.maxExactEndposScore;
+			newSlice.maxExactEndposNode = sliceResult.maxExactEndposNode;
+			double jd = (double)newSlice.j + WordSize;
+			ORC_ASSERT(sliceResult.minScore <= (ScoreType)newSlice.j + (ScoreType)WordSize);
+			ORC_ASSERT(sliceResult.maxExactEndposScore != SMIN);
+			ORC_ASSERT((double)sliceResult.maxExactEndposScore >= -jd * params.XscoreErrorCost);
+			ORC_ASSERT((double)sliceResult.maxExactEndposScore <= jd * params.XscoreErrorCost);
+			ORC_ASSERT(newSlice.minScore >= lastSlice.minScore);
+			newSlice.bandwidth = bandwidth;
+			ORC_ASSERT(newSlice.minScore != SMAX);
+			ORC_ASSERT(newSlice.minScoreNode != NMAX);
+			ORC_ASSERT(newSlice.minScoreNodeOffset != NMAX);
+			ORC_ASSERT(newSlice.scores.hasNode(newSlice.minScoreNode));
+			ORC_ASSERT(newSlice.minScoreNodeOffset < graph.NodeLength(newSlice.minScoreNode));
+			if (newSlice.maxExactEndposScore > bestXScore) bestXScore = newSlice.maxExactEndposScore;
+			if (newSlice.cellsProcessed >= params.maxCellsPerSlice) newSlice.scoresNotValid = true;
+			if (newSlice.maxExactEndposScore < bestXScore - Xdropcutoff) break;
+			result.push_back(std::move(newSlice));
+		}
+		return result;
+	}
+
+	// src/GraphAlignerBitvectorCommon.h:760-768
+	static void checkBacktraceCircularity(const OnewayTrace& result)
+	{
+		for (size_t i = result.trace.size() - 2; i < result.trace.size(); i--)
+		{
+			ORC_ASSERT(result.trace[i].DPposition != result.trace.back().DPposition);
+			if (result.trace[i].DPposition.seqPos != result.trace.back().DPposition.seqPos) return;
+		}
+	}
+
+	// src/GraphAlignerBitvectorCommon.h:770-815
+	std::vector<MatrixPosition> pickBacktraceInside(size_t verticalOffset, const std::vector<WordSlice>& nodeSlices, MatrixPosition pos, const std::string_view& sequence) const
+	{
+		size_t hori = pos.nodeOffset;
+		size_t vert = pos.seqPos - verticalOffset;
+		std::vector<MatrixPosition> result;
+		while (hori > 0 && vert > 0)
+		{
+			ScoreType scoreHere = nodeSlices[hori].getValue(vert);
+			ScoreType verticalScore = nodeSlices[hori].getValue(vert - 1);
+			ScoreType horizontalScore = nodeSlices[hori - 1].getValue(vert);
+			ScoreType diagonalScore = nodeSlices[hori - 1].getValue(vert - 1);
+			bool eq = characterMatch(sequence[vert + verticalOffset], graph.NodeSequences(pos.node, hori));
+			ORC_ASSERT(verticalScore >= scoreHere - 1);
+			ORC_ASSERT(horizontalScore >= scoreHere - 1);
+			ORC_ASSERT(diagonalScore >= scoreHere - (eq ? 0 : 1));
+			if (verticalScore == scoreHere - 1)
+			{
+				vert--;
+				result.emplace_back(pos.node, hori, vert + verticalOffset);
+				continue;
+			}
+			if (diagonalScore == scoreHere - (eq ? 0 : 1))
+			{
+				hori--;
+				vert--;
+				result.emplace_back(pos.node, hori, vert + verticalOffset);
+				continue;
+			}
+			if (horizontalScore == scoreHere - 1)
+			{
+				hori--;
+				result.emplace_back(pos.node, hori, vert + verticalOffset);
+				continue;
+			}
+			ORC_ASSERT(false);
+			break;
+		}
+		return result;
+	}
+
+	using PosPair = std::pair<MatrixPosition, bool>;
+
+	// src/GraphAlignerBitvectorCommon.h:936-1054 (fake extra slice: its values are INT_MAX)
+	PosPair pickBacktraceCorner(const NodeSlice& current, const NodeSlice& previous, size_t node, size_t j, const std::string_view& sequence, ScoreType quitScore, bool scoresNotValid, ScoreType previousQuitScore, bool previousScoresNotValid) const
+	{
+		ScoreType scoreHere = current.node(node).startSlice.getValue(0);
+		if (scoresNotValid || scoreHere > quitScore)
+		{
+			ScoreType smallestFound = scoreHere + 1;
+			MatrixPosition smallestPos{NMAX, NMAX, NMAX};
+			bool nodeChange = false;
+			if (previous.hasNode(node))
+			{
+				auto slice = previous.node(node).startSlice;
+				smallestFound = slice.scoreEnd;
+				smallestPos = MatrixPosition{node, 0, j - 1};
+				nodeChange = false;
+			}
+			for (auto neighbor : graph.inNeighbors[node])
+			{
+				if (previous.hasNode(neighbor))
+				{
+					auto neighborSlice = previous.node(neighbor).endSlice;
+					if (neighborSlice.scoreEnd <= smallestFound)
+					{
+						smallestFound = neighborSlice.scoreEnd;
+						smallestPos = MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, j - 1};
+						nodeChange = true;
+					}
+				}
+				if (current.hasNode(neighbor) && neighbor != node)
+				{
+					auto neighborSlice = current.node(neighbor).endSlice;
+					if (neighborSlice.getValue(0) < smallestFound)
+					{
+						smallestFound = neighborSlice.getValue(0);
+						smallestPos = MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, j};
+						nodeChange = true;
+					}
+				}
+			}
+			ORC_ASSERT(smallestPos.node != NMAX);
+			return std::make_pair(smallestPos, nodeChange);
+		}
+		bool eq = characterMatch(sequence[j], graph.NodeSequences(node, 0));
+		MatrixPosition bestInvalidBacktrace{NMAX, NMAX, NMAX};
+		ScoreType bestInvalidBacktraceScore = scoreHere + 1;
+		bool bestInvalidNodeSwitch = false;
+		if (previous.hasNode(node))
+		{
+			ORC_ASSERT(previous.node(node).startSlice.scoreEnd >= scoreHere - 1);
+			if (previous.node(node).startSlice.scoreEnd == scoreHere - 1) return std::make_pair(MatrixPosition{node, 0, j - 1}, false);
+			if (previousScoresNotValid || previous.node(node).startSlice.scoreEnd > previousQuitScore)
+			{
+				if (previous.node(node).startSlice.scoreEnd < bestInvalidBacktraceScore)
+				{
+					bestInvalidBacktraceScore = previous.node(node).startSlice.scoreEnd;
+					bestInvalidBacktrace = MatrixPosition{node, 0, j - 1};
+					bestInvalidNodeSwitch = false;
+				}
+			}
+		}
+		for (auto neighbor : graph.inNeighbors[node])
+		{
+			if (current.hasNode(neighbor))
+			{
+				ORC_ASSERT(current.node(neighbor).endSlice.getValue(0) >= scoreHere - 1);
+				if (current.node(neighbor).endSlice.getValue(0) == scoreHere - 1) return std::make_pair(MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, j}, true);
+			}
+			if (previous.hasNode(neighbor))
+			{
+				ScoreType cornerScore = previous.node(neighbor).endSlice.scoreEnd;
+				if (previousScoresNotValid || cornerScore > previousQuitScore)
+				{
+					if (cornerScore < bestInvalidBacktraceScore)
+					{
+						bestInvalidBacktraceScore = cornerScore;
+						bestInvalidBacktrace = MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, j - 1};
+						bestInvalidNodeSwitch = true;
+					}
+				}
+				else
+				{
+					ORC_ASSERT(previous.node(neighbor).endSlice.scoreEnd >= scoreHere - (eq ? 0 : 1));
+					if (cornerScore == scoreHere - (eq ? 0 : 1)) return std::make_pair(MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, j - 1}, true);
+				}
+			}
+		}
+		if (bestInvalidBacktraceScore < scoreHere + 1) return std::make_pair(bestInvalidBacktrace, bestInvalidNodeSwitch);
+		ORC_ASSERT(false);
+		return std::make_pair(MatrixPosition{0, 0, 0}, false);
+	}
+
+	// src/GraphAlignerBitvectorCommon.h:817-884
+	std::pair<PosPair, PosPair> pickBacktraceHorizontalCrossing(const NodeSlice& current, const NodeSlice& previous, size_t j, size_t node, MatrixPosition pos, const std::string_view& sequence, ScoreType quitScore, bool scoresNotValid, ScoreType previousQuitScore, bool previousScoresNotValid) const
+	{
+		auto startSlice = current.node(node).startSlice;
+		while (pos.seqPos % WordSize != 0 && (startSlice.VP & ((Word)1 << (pos.seqPos % WordSize)))) pos.seqPos--;
+		size_t offset = pos.seqPos % WordSize;
+		if (offset == 0)
+		{
+			return std::make_pair(std::make_pair(pos, false), pickBacktraceCorner(current, previous, node, j, sequence, quitScore, scoresNotValid, previousQuitScore, previousScoresNotValid));
+		}
+		bool eq = characterMatch(sequence[pos.seqPos], graph.NodeSequences(pos.node, pos.nodeOffset));
+		ScoreType scoreHere = startSlice.getValue(offset);
+		if (scoresNotValid || scoreHere > quitScore)
+		{
+			ScoreType smallestFound = startSlice.getValue(offset - 1);
+			MatrixPosition smallestPos{node, 0, pos.seqPos - 1};
+			bool nodeChange = false;
+			for (auto neighbor : graph.inNeighbors[node])
+			{
+				if (current.hasNode(neighbor))
+				{
+					auto neighborSlice = current.node(neighbor).endSlice;
+					if (neighborSlice.getValue(offset - 1) <= smallestFound)
+					{
+						smallestFound = neighborSlice.getValue(offset - 1);
+						smallestPos = MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, pos.seqPos - 1};
+						nodeChange = true;
+					}
+					if (neighborSlice.getValue(offset) < smallestFound && neighbor != node)
+					{
+						smallestFound = neighborSlice.getValue(offset);
+						smallestPos = MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, pos.seqPos};
+						nodeChange = true;
+					}
+				}
+			}
+			if (smallestPos == pos) ORC_ASSERT(false);
+			return std::make_pair(std::make_pair(pos, false), std::make_pair(smallestPos, nodeChange));
+		}
+		for (auto neighbor : graph.inNeighbors[node])
+		{
+			if (current.hasNode(neighbor))
+			{
+				auto neighborSlice = current.node(neighbor).endSlice;
+				ORC_ASSERT(neighborSlice.getValue(offset) >= scoreHere - 1);
+				ORC_ASSERT(neighborSlice.getValue(offset - 1) >= scoreHere - (eq ? 0 : 1));
+				if (neighborSlice.getValue(offset) == scoreHere - 1)
+					return std::make_pair(std::make_pair(pos, false), std::make_pair(MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, pos.seqPos}, true));
+				if (neighborSlice.getValue(offset - 1) == scoreHere - (eq ? 0 : 1))
+					return std::make_pair(std::make_pair(pos, false), std::make_pair(MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, pos.seqPos - 1}, true));
+			}
+		}
+		ORC_ASSERT(false);
+		return std::make_pair(std::make_pair(pos, false), std::make_pair(pos, false));
+	}
+
+	// src/GraphAlignerBitvectorCommon.h:886-934
+	std::pair<PosPair, PosPair> pickBacktraceVerticalCrossing(const NodeSlice& current, const NodeSlice& previous, const std::vector<WordSlice>& nodeScores, size_t j, size_t node, MatrixPosition pos, const std::string_view& sequence, ScoreType quitScore, bool scoresNotValid, ScoreType previousQuitScore, bool previousScoresNotValid) const
+	{
+		while (pos.nodeOffset > 0 && nodeScores[pos.nodeOffset - 1].getValue(0) == nodeScores[pos.nodeOffset].getValue(0) - 1) pos.nodeOffset--;
+		if (pos.nodeOffset == 0)
+		{
+			return std::make_pair(std::make_pair(pos, false), pickBacktraceCorner(current, previous, node, j, sequence, quitScore, scoresNotValid, previousQuitScore, previousScoresNotValid));
+		}
+		ORC_ASSERT(previous.hasNode(node));
+		bool eq = characterMatch(sequence[pos.seqPos], graph.NodeSequences(pos.node, pos.nodeOffset));
+		auto previousNode = previous.node(node);
+		ScoreType scoreHere = nodeScores[pos.nodeOffset].getValue(0);
+		ScoreType scoreDiagonal = previousNode.startSlice.scoreEnd;
+		for (size_t i = 1; i <= pos.nodeOffset - 1; i++)
+		{
+			scoreDiagonal += (previousNode.HP >> i) & 1;
+			scoreDiagonal -= (previousNode.HN >> i) & 1;
+		}
+		ScoreType scoreUp = scoreDiagonal;
+		scoreUp += (previousNode.HP >> pos.nodeOffset) & 1;
+		scoreUp -= (previousNode.HN >> pos.nodeOffset) & 1;
+		if (previousScoresNotValid || scoresNotValid || scoreHere > quitScore || scoreDiagonal > previousQuitScore || scoreUp > previousQuitScore)
+		{
+			if (scoreDiagonal < scoreUp) return std::make_pair(std::make_pair(pos, false), std::make_pair(MatrixPosition{pos.node, pos.nodeOffset - 1, pos.seqPos - 1}, false));
+			return std::make_pair(std::make_pair(pos, false), std::make_pair(MatrixPosition{pos.node, pos.nodeOffset, pos.seqPos - 1}, false));
+		}
+		ORC_ASSERT(scoreUp >= scoreHere - 1);
+		ORC_ASSERT(scoreDiagonal >= scoreHere - (eq ? 0 : 1));
+		if (scoreUp == scoreHere - 1) return std::make_pair(std::make_pair(pos, false), std::make_pair(MatrixPosition{pos.node, pos.nodeOffset, pos.seqPos - 1}, false));
+		if (scoreDiagonal == scoreHere - (eq ? 0 : 1)) return std::make_pair(std::make_pair(pos, false), std::make_pair(MatrixPosition{pos.node, pos.nodeOffset - 1, pos.seqPos - 1}, false));
+		ORC_ASSERT(false);
+		return std::make_pair(std::make_pair(pos, false), std::make_pair(pos, false));
+	}
+
+	NodeItem previousItemOrDefault(const DPSlice& prev, size_t node) const
+	{
+		if (prev.scores.hasNode(node)) return prev.scores.node(node);
+		NodeItem previous;
+		previous.HP = AllOnes;
+		previous.HN = AllZeros;
+		return previous;
+	}
+
+	// src/GraphAlignerBitvectorCommon.h:571-758 (multiseed == false)
+	OnewayTrace getReverseTraceFromTable(const std::string_view& sequence, const std::vector<DPSlice>& slice, MatrixPosition startPos, ScoreType startScore) const
+	{
+		size_t currentSlice = startPos.seqPos / WordSize + 1;
+		ORC_ASSERT(currentSlice < slice.size());
+		OnewayTrace result;
+		result.score = startScore;
+		result.trace.emplace_back(startPos, false, sequence, graph);
+		size_t currentNode = NMAX;
+		std::vector<WordSlice> nodeSlices;
+		EqVector EqV = getEqVector(sequence, slice[currentSlice].j);
+		ScoreType lastScore = SMAX - 1;
+		while (result.trace.back().DPposition.seqPos != (size_t)-1)
+		{
+			size_t newSlice = result.trace.back().DPposition.seqPos / WordSize + 1;
+			ORC_ASSERT(newSlice < slice.size());
+			size_t newNode = result.trace.back().DPposition.node;
+			if (newSlice != currentSlice || newNode != currentNode)
+			{
+				if (newSlice != currentSlice) EqV = getEqVector(sequence, slice[newSlice].j);
+				currentSlice = newSlice;
+				currentNode = newNode;
+				ORC_ASSERT(slice[currentSlice].scores.hasNode(currentNode));
+				ORC_ASSERT(currentSlice > 0);
+				NodeItem previous = previousItemOrDefault(slice[currentSlice - 1], currentNode);
+				nodeSlices = recalcNodeWordslice(currentNode, slice[currentSlice].scores.node(currentNode), EqV, previous, (ScoreType)slice[currentSlice].j);
+			}
+			const auto& cur = slice[currentSlice];
+			const auto& prv = slice[currentSlice - 1];
+			ScoreType quit = cur.minScore + (ScoreType)cur.bandwidth;
+			ScoreType prevQuit = prv.minScore + (ScoreType)prv.bandwidth;
+			auto back = result.trace.back().DPposition;
+			lastScore = nodeSlices[back.nodeOffset].getValue(back.seqPos % WordSize);
+			if (back.seqPos % WordSize == 0 && back.nodeOffset == 0)
+			{
+				auto bt = pickBacktraceCorner(cur.scores, prv.scores, currentNode, cur.j, sequence, quit, cur.scoresNotValid, prevQuit, prv.scoresNotValid);
+				result.trace.emplace_back(bt.first, bt.second, sequence, graph);
+				checkBacktraceCircularity(result);
+				continue;
+			}
+			if (back.seqPos % WordSize == 0)
+			{
+				if (!prv.scores.hasNode(currentNode))
+				{
+					for (size_t i = back.nodeOffset - 1; i > 0; i--)
+					{
+						result.trace.emplace_back(MatrixPosition{currentNode, i, result.trace.back().DPposition.seqPos}, false, sequence, graph);
+					}
+					result.trace.emplace_back(MatrixPosition{currentNode, 0, result.trace.back().DPposition.seqPos}, false, sequence, graph);
+					continue;
+				}
+				auto crossing = pickBacktraceVerticalCrossing(cur.scores, prv.scores, nodeSlices, cur.j, currentNode, back, sequence, quit, cur.scoresNotValid, prevQuit, prv.scoresNotValid);
+				ORC_ASSERT(crossing.first.first.node == back.node);
+				ORC_ASSERT(crossing.first.first.seqPos == back.seqPos);
+				ORC_ASSERT(crossing.first.first.nodeOffset <= back.nodeOffset);
+				if (crossing.first.first.nodeOffset != back.nodeOffset)
+				{
+					for (size_t nodeOffset = back.nodeOffset - 1; nodeOffset != crossing.first.first.nodeOffset; nodeOffset--)
+					{
+						result.trace.emplace_back(MatrixPosition{crossing.first.first.node, nodeOffset, crossing.first.first.seqPos}, false, sequence, graph);
+					}
+				}
+				if (crossing.first.first != result.trace.back().DPposition) result.trace.emplace_back(crossing.first.first, crossing.first.second, sequence, graph);
+				ORC_ASSERT(crossing.second.first != result.trace.back().DPposition);
+				result.trace.emplace_back(crossing.second.first, crossing.second.second, sequence, graph);
+				continue;
+			}
+			if (back.nodeOffset == 0)
+			{
+				auto crossing = pickBacktraceHorizontalCrossing(cur.scores, prv.scores, cur.j, currentNode, back, sequence, quit, cur.scoresNotValid, prevQuit, prv.scoresNotValid);
+				ORC_ASSERT(crossing.first.first.node == back.node);
+				ORC_ASSERT(crossing.first.first.nodeOffset == back.nodeOffset);
+				ORC_ASSERT(crossing.first.first.seqPos <= back.seqPos);
+				if (crossing.first.first.seqPos != back.seqPos)
+				{
+					for (size_t seqPos = back.seqPos - 1; seqPos != crossing.first.first.seqPos; seqPos--)
+					{
+						result.trace.emplace_back(MatrixPosition{crossing.first.first.node, crossing.first.first.nodeOffset, seqPos}, false, sequence, graph);
+					}
+				}
+				if (crossing.first.first != result.trace.back().DPposition) result.trace.emplace_back(crossing.first.first, crossing.first.second, sequence, graph);
+				ORC_ASSERT(crossing.second.first != result.trace.back().DPposition);
+				result.trace.emplace_back(crossing.second.first, crossing.second.second, sequence, graph);
+				checkBacktraceCircularity(result);
+				continue;
+			}
+			auto inner = pickBacktraceInside(cur.j, nodeSlices, back, sequence);
+			for (auto pos : inner) result.trace.emplace_back(pos, false, sequence, graph);
+		}
+		ORC_ASSERT(result.score >= lastScore);
+		result.score -= lastScore;
+		{
+			ORC_ASSERT(slice[0].scores.hasNode(result.trace.back().DPposition.node));
+			const auto& node = slice[0].scores.node(result.trace.back().DPposition.node);
+			size_t len = graph.NodeLength(result.trace.back().DPposition.node);
+			std::vector<ScoreType> beforeSliceScores(len);
+			beforeSliceScores[0] = node.startSlice.scoreEnd;
+			for (size_t i = 1; i < len; i++)
+			{
+				beforeSliceScores[i] = beforeSliceScores[i - 1] + ((node.HP >> i) & 1) - ((node.HN >> i) & 1);
+			}
+			ORC_ASSERT(beforeSliceScores.back() == node.endSlice.scoreEnd);
+			while (beforeSliceScores[result.trace.back().DPposition.nodeOffset] != 0 && result.trace.back().DPposition.nodeOffset > 0 && beforeSliceScores[result.trace.back().DPposition.nodeOffset - 1] == beforeSliceScores[result.trace.back().DPposition.nodeOffset] - 1)
+			{
+				result.trace.emplace_back(MatrixPosition{result.trace.back().DPposition.node, result.trace.back().DPposition.nodeOffset - 1, result.trace.back().DPposition.seqPos}, false, sequence, graph);
+			}
+			if (result.trace.back().DPposition.nodeOffset == 0 && beforeSliceScores[result.trace.back().DPposition.nodeOffset] != 0)
+			{
+				for (auto neighbor : graph.inNeighbors[result.trace.back().DPposition.node])
+				{
+					if (slice[0].scores.hasNode(neighbor) && slice[0].scores.node(neighbor).endSlice.getScoreBeforeStart() == beforeSliceScores[result.trace.back().DPposition.nodeOffset] - 1)
+					{
+						result.trace.emplace_back(MatrixPosition{neighbor, graph.NodeLength(neighbor) - 1, result.trace.back().DPposition.seqPos}, true, sequence, graph);
+						break;
+					}
+				}
+			}
+		}
+		return result;
+	}
+
+	// src/GraphAlignerBitvectorCommon.h:496-562
+	OnewayTrace getReverseTraceFromTableExactEndPos(const std::string_view& sequence, const std::vector<DPSlice>& slice) const
+	{
+		ORC_ASSERT(slice.size() > 1);
+		size_t bestIndex = 1;
+		for (size_t i = 1; i < slice.size(); i++)
+		{
+			if (slice[i].maxExactEndposScore > slice[bestIndex].maxExactEndposScore) bestIndex = i;
+		}
+		auto node = slice[bestIndex].maxExactEndposNode;
+		auto score = slice[bestIndex].maxExactEndposScore;
+		NodeItem previous = previousItemOrDefault(slice[bestIndex - 1], node);
+		EqVector EqV = getEqVector(sequence, slice[bestIndex].j);
+		ORC_ASSERT(slice[bestIndex].scores.hasNode(node));
+		std::vector<WordSlice> nodeSlices = recalcNodeWordslice(node, slice[bestIndex].scores.node(node), EqV, previous, (ScoreType)slice[bestIndex].j);
+		size_t nodeOffset = NMAX;
+		size_t bvOffset = NMAX;
+		const size_t j = slice[bestIndex].j;
+		for (size_t i = 0; i < nodeSlices.size(); i++)
+		{
+			auto maxScore = nodeSlices[i].maxXScoreFirstSlices((ScoreType)j, params.XscoreErrorCost, std::min((size_t)WordSize, (size_t)(sequence.size() - j)));
+			ORC_ASSERT(maxScore <= score);
+			if (maxScore == score)
+			{
+				for (size_t off = WordSize - 1; off < (size_t)WordSize; off--)
+				{
+					if (j + off >= sequence.size()) continue;
+					auto scoreHere = nodeSlices[i].getXScore((ScoreType)j, off, params.XscoreErrorCost);
+					ORC_ASSERT(scoreHere <= score);
+					if (scoreHere == score)
+					{
+						if (nodeOffset == NMAX || off > bvOffset)
+						{
+							nodeOffset = i;
+							bvOffset = off;
+						}
+					}
+				}
+			}
+		}
+		ORC_ASSERT(nodeOffset != NMAX);
+		ORC_ASSERT(bvOffset != NMAX);
+		ScoreType startScore = nodeSlices[nodeOffset].getValue(bvOffset);
+		MatrixPosition startPos{node, nodeOffset, j + bvOffset};
+		return getReverseTraceFromTable(sequence, slice, startPos, startScore);
+	}
+
+	// src/GraphAlignerBitvectorBanded.h:59-76
+	OnewayTrace getReverseTraceFromSeed(const std::string_view& sequence, int bigraphNodeId, size_t nodeOffset, std::vector<SliceSummary>* summaries = nullptr) const
+	{
+		size_t numSlices = (sequence.size() + WordSize - 1) / WordSize;
+		auto initial = getInitialSliceExactPosition(bigraphNodeId, nodeOffset);
+		auto slice = getXdropSlices(sequence, std::move(initial), numSlices, (double)params.Xdropcutoff);
+		if (summaries)
+		{
+			summaries->clear();
+			for (size_t i = 1; i < slice.size(); i++)
+			{
+				const auto& s = slice[i];
+				summaries->push_back(SliceSummary{(int64_t)s.j, s.minScore, (int64_t)s.minScoreNode, (int64_t)s.minScoreNodeOffset, s.maxExactEndposScore, (int64_t)s.maxExactEndposNode, (int64_t)s.cellsProcessed, (int64_t)s.scores.size(), itemsDigest(s.scores), s.scoresNotValid ? 1 : 0});
+			}
+		}
+		if (slice.size() <= 1) return OnewayTrace::TraceFailed();
+		ORC_ASSERT(slice.back().minScore >= 0);
+		ORC_ASSERT(slice.back().minScore <= (ScoreType)sequence.size() + (ScoreType)WordSize * 2);
+		return getReverseTraceFromTableExactEndPos(sequence, slice);
+	}
+};
+
+// ---------------------------------------------------------------------------
+// Seeds, alignments (src/GraphAlignerWrapper.h:11-50, src/GraphAlignerCommon.h:282-344)
+// ---------------------------------------------------------------------------
+struct SeedHit
+{
+	int nodeID = std::numeric_limits<int>::min();
+	size_t nodeOffset = NMAX;
+	size_t seqPos = NMAX;
+	size_t matchLen = NMAX;
+	bool reverse = false;
+	size_t alignmentGraphNodeId = NMAX;
+	size_t alignmentGraphNodeOffset = NMAX;
+	size_t rawSeedGoodness = 0;
+	size_t seedGoodness = 0;
+	size_t seedClusterSize = 0;
+};
+
+struct AlignmentItem
+{
+	std::shared_ptr<OnewayTrace> trace;
+	std::string GAFline;
+	size_t seedGoodness = 0;
+	size_t alignmentStart = 0;
+	size_t alignmentEnd = 0;
+	size_t alignmentScore = NMAX;
+	double alignmentXScore = -1;
+	int mappingQuality = 255;
+	bool alignmentFailed() const { return alignmentEnd == alignmentStart; }
+	size_t alignmentLength() const { return alignmentEnd - alignmentStart; }
+};
+
+struct AlignmentResult
+{
+	std::vector<AlignmentItem> alignments;
+	size_t seedsExtended = 0;
+};
+
+// ---------------------------------------------------------------------------
+// GraphAligner orchestration (src/GraphAligner.h)
+// ---------------------------------------------------------------------------
+struct Aligner
+{
+	const Params& params;
+	const AlignmentGraph& graph;
+	Extender bv;
+	Aligner(const Params& p, const AlignmentGraph& g) : params(p), graph(g), bv(p, g) {}
+
+	// src/GraphAligner.h:250-312
+	void orderSeedsByChaining(std::vector<SeedHit>& seedHits) const
+	{
+		std::map<size_t, std::vector<std::pair<size_t, size_t>>> seedPoses;
+		for (size_t i = 0; i < seedHits.size(); i++)
+		{
+			int forwardNodeId = seedHits[i].reverse ? seedHits[i].nodeID * 2 + 1 : seedHits[i].nodeID * 2;
+			size_t nodeIndex, realOffset;
+			if (seedHits[i].alignmentGraphNodeId == NMAX)
+			{
+				nodeIndex = graph.GetUnitigNode(forwardNodeId, seedHits[i].nodeOffset);
+				realOffset = seedHits[i].nodeOffset - graph.nodeOffset[nodeIndex];
+			}
+			else
+			{
+				nodeIndex = seedHits[i].alignmentGraphNodeId;
+				realOffset = seedHits[i].alignmentGraphNodeOffset;
+			}
+			seedPoses[graph.chainNumber[nodeIndex]].emplace_back(i, graph.chainApproxPos[nodeIndex] + realOffset - seedHits[i].seqPos);
+		}
+		for (auto& pair : seedPoses)
+		{
+			std::sort(pair.second.begin(), pair.second.end(), [](std::pair<size_t, size_t> l, std::pair<size_t, size_t> r) { return l.second < r.second; });
+			size_t clusterStart = 0;
+			for (size_t i = 1; i <= pair.second.size(); i++)
+			{
+				if (i < pair.second.size() && pair.second[i].second <= pair.second[i - 1].second + 100) continue;
+				std::sort(pair.second.begin() + clusterStart, pair.second.begin() + i, [&seedHits](std::pair<size_t, size_t> l, std::pair<size_t, size_t> r) { return seedHits[l.first].seqPos < seedHits[r.first].seqPos; });
+				size_t matchingBps = 0;
+				int lastEnd = std::numeric_limits<int>::min();
+				for (size_t k = clusterStart; k < i; k++)
+				{
+					int thisStart = (int)seedHits[pair.second[k].first].seqPos - (int)seedHits[pair.second[k].first].matchLen + 1;
+					int thisEnd = (int)seedHits[pair.second[k].first].seqPos;
+					matchingBps += (thisEnd - std::max(thisStart, lastEnd));
+					lastEnd = thisEnd;
+				}
+				for (size_t k = clusterStart; k < i; k++)
+				{
+					seedHits[pair.second[k].first].seedGoodness = matchingBps + seedHits[pair.second[k].first].rawSeedGoodness;
+					seedHits[pair.second[k].first].seedClusterSize = i - clusterStart;
+				}
+				clusterStart = i;
+			}
+		}
+		std::sort(seedHits.begin(), seedHits.end(), [](const SeedHit& l, const SeedHit& r) { return l.seedGoodness < r.seedGoodness; });
+		std::reverse(seedHits.begin(), seedHits.end());
+	}
+
+	// src/GraphAligner.h:746-766 (debug check, live in the reference build)
+	void verifyTrace(const std::vector<TraceItem>& trace) const
+	{
+		for (size_t i = 1; i < trace.size(); i++)
+		{
+			auto newpos = trace[i].DPposition;
+			auto oldpos = trace[i - 1].DPposition;
+			ORC_ASSERT(newpos.seqPos != oldpos.seqPos || newpos.node != oldpos.node || newpos.nodeOffset != oldpos.nodeOffset);
+			if (oldpos.node == newpos.node)
+			{
+				ORC_ASSERT(((newpos.nodeOffset == oldpos.nodeOffset || newpos.nodeOffset == oldpos.nodeOffset + 1) && (newpos.seqPos == oldpos.seqPos || newpos.seqPos == oldpos.seqPos + 1)) || (oldpos.nodeOffset == graph.NodeLength(newpos.node) - 1 && newpos.nodeOffset == 0 && (newpos.seqPos == oldpos.seqPos || newpos.seqPos == oldpos.seqPos + 1)));
+				continue;
+			}
+			ORC_ASSERT(std::find(graph.outNeighbors[oldpos.node].begin(), graph.outNeighbors[oldpos.node].end(), newpos.node) != graph.outNeighbors[oldpos.node].end());
+			ORC_ASSERT(newpos.seqPos == oldpos.seqPos || newpos.seqPos == oldpos.seqPos + 1);
+			ORC_ASSERT(oldpos.nodeOffset == graph.NodeLength(oldpos.node) - 1);
+			ORC_ASSERT(newpos.nodeOffset == 0);
+		}
+	}
+
+	struct Trace
+	{
+		OnewayTrace forward;
+		OnewayTrace backward;
+	};
+
+	// src/GraphAligner.h:475-520
+	Trace getTwoDirectionalTrace(const std::string& sequence, const std::string& revSequence, const SeedHit& seedHit) const
+	{
+		int forwardNodeId, backwardNodeId;
+		if (seedHit.reverse) { forwardNodeId = seedHit.nodeID * 2 + 1; backwardNodeId = seedHit.nodeID * 2; }
+		else { forwardNodeId = seedHit.nodeID * 2; backwardNodeId = seedHit.nodeID * 2 + 1; }
+		Trace result;
+		result.backward.score = SMAX;
+		result.forward.score = SMAX;
+		if (seedHit.seqPos > 0)
+		{
+			std::string_view backwardPart{revSequence.data() + revSequence.size() - seedHit.seqPos, seedHit.seqPos};
+			auto reversePos = graph.GetReversePosition(forwardNodeId, seedHit.nodeOffset);
+			result.backward = bv.getReverseTraceFromSeed(backwardPart, backwardNodeId, reversePos.second);
+		}
+		if (seedHit.seqPos < sequence.size() - 1)
+		{
+			std::string_view forwardPart{sequence.data() + seedHit.seqPos + 1, sequence.size() - seedHit.seqPos - 1};
+			result.forward = bv.getReverseTraceFromSeed(forwardPart, forwardNodeId, seedHit.nodeOffset);
+		}
+		if (!result.backward.failed())
+		{
+			auto reversePos = graph.GetReversePosition(forwardNodeId, seedHit.nodeOffset);
+			const auto& b = result.backward.trace.back().DPposition;
+			ORC_ASSERT(b.seqPos == (size_t)-1 && graph.nodeIDs[b.node] == backwardNodeId && graph.nodeOffset[b.node] + b.nodeOffset == reversePos.second);
+			std::reverse(result.backward.trace.begin(), result.backward.trace.end());
+		}
+		if (!result.forward.failed())
+		{
+			const auto& f = result.forward.trace.back().DPposition;
+			ORC_ASSERT(f.seqPos == (size_t)-1 && graph.nodeIDs[f.node] == forwardNodeId && graph.nodeOffset[f.node] + f.nodeOffset == seedHit.nodeOffset);
+			std::reverse(result.forward.trace.begin(), result.forward.trace.end());
+		}
+		return result;
+	}
+
+	// src/GraphAligner.h:522-559
+	void fixForwardTraceSeqPos(OnewayTrace& trace, size_t start, const std::string& sequence) const
+	{
+		if (trace.trace.size() == 0) return;
+		trace.score = 0;
+		for (size_t i = 0; i < trace.trace.size(); i++)
+		{
+			trace.trace[i].DPposition.seqPos += start;
+			auto nodeIndex = trace.trace[i].DPposition.node;
+			trace.trace[i].DPposition.node = graph.nodeIDs[nodeIndex];
+			trace.trace[i].DPposition.nodeOffset += graph.nodeOffset[nodeIndex];
+			ORC_ASSERT(trace.trace[i].DPposition.seqPos < sequence.size());
+			if (trace.trace[i].DPposition.seqPos == start - 1) trace.trace[i].sequenceCharacter = sequence[trace.trace[i].DPposition.seqPos];
+			ORC_ASSERT(trace.trace[i].sequenceCharacter == sequence[trace.trace[i].DPposition.seqPos]);
+			if (i == 0 && !characterMatch(trace.trace[0].sequenceCharacter, trace.trace[0].graphCharacter)) trace.score += 1;
+			if (i > 0)
+			{
+				if (trace.trace[i].DPposition.seqPos == trace.trace[i - 1].DPposition.seqPos) trace.score += 1;
+				else if (!trace.trace[i - 1].nodeSwitch && trace.trace[i].DPposition.node == trace.trace[i - 1].DPposition.node && trace.trace[i].DPposition.nodeOffset == trace.trace[i - 1].DPposition.nodeOffset) trace.score += 1;
+				else if (!characterMatch(trace.trace[i].sequenceCharacter, trace.trace[i].graphCharacter)) trace.score += 1;
+			}
+		}
+	}
+
+	// src/GraphAligner.h:562-601
+	void fixReverseTraceSeqPosAndOrder(OnewayTrace& trace, size_t end, const std::string& sequence) const
+	{
+		if (trace.trace.size() == 0) return;
+		std::reverse(trace.trace.begin(), trace.trace.end());
+		trace.score = 0;
+		for (size_t i = 0; i < trace.trace.size() - 1; i++) trace.trace[i].nodeSwitch = trace.trace[i + 1].nodeSwitch;
+		trace.trace.back().nodeSwitch = false;
+		for (size_t i = 0; i < trace.trace.size(); i++)
+		{
+			trace.trace[i].DPposition.seqPos = end - trace.trace[i].DPposition.seqPos;
+			size_t offset = graph.nodeOffset[trace.trace[i].DPposition.node] + trace.trace[i].DPposition.nodeOffset;
+			auto reversePos = graph.GetReversePosition(graph.nodeIDs[trace.trace[i].DPposition.node], offset);
+			trace.trace[i].DPposition.node = reversePos.first;
+			trace.trace[i].DPposition.nodeOffset = reversePos.second;
+			ORC_ASSERT(trace.trace[i].DPposition.seqPos < sequence.size());
+			trace.trace[i].sequenceCharacter = sequence[trace.trace[i].DPposition.seqPos];
+			trace.trace[i].graphCharacter = Complement(trace.trace[i].graphCharacter);
+			if (i == 0 && !characterMatch(trace.trace[0].sequenceCharacter, trace.trace[0].graphCharacter)) trace.score += 1;
+			if (i > 0)
+			{
+				if (trace.trace[i].DPposition.seqPos == trace.trace[i - 1].DPposition.seqPos) trace.score += 1;
+				else if (!trace.trace[i - 1].nodeSwitch && trace.trace[i].DPposition.node == trace.trace[i - 1].DPposition.node && trace.trace[i].DPposition.nodeOffset == trace.trace[i - 1].DPposition.nodeOffset) trace.score += 1;
+				else if (!characterMatch(trace.trace[i].sequenceCharacter, trace.trace[i].graphCharacter)) trace.score += 1;
+			}
+		}
+	}
+
+	// src/GraphAligner.h:631-692
+	AlignmentItem getAlignmentFromSeed(const std::string& sequence, const std::string& revSequence, const SeedHit& seedHit) const
+	{
+		auto trace = getTwoDirectionalTrace(sequence, revSequence, seedHit);
+		if (trace.forward.trace.size() > 0) verifyTrace(trace.forward.trace);
+		if (trace.backward.trace.size() > 0) verifyTrace(trace.backward.trace);
+		fixReverseTraceSeqPosAndOrder(trace.backward, seedHit.seqPos - 1, sequence);
+		fixForwardTraceSeqPos(trace.forward, seedHit.seqPos + 1, sequence);
+		if (trace.forward.failed() && trace.backward.failed()) return AlignmentItem{};
+		auto mergedTrace = std::move(trace.backward);
+		if (mergedTrace.failed())
+		{
+			mergedTrace = std::move(trace.forward);
+		}
+		else if (!trace.forward.failed())
+		{
+			ORC_ASSERT(mergedTrace.trace.back().DPposition == trace.forward.trace[0].DPposition);
+			mergedTrace.trace.pop_back();
+			mergedTrace.trace.insert(mergedTrace.trace.end(), trace.forward.trace.begin(), trace.forward.trace.end());
+			mergedTrace.score += trace.forward.score;
+		}
+		AlignmentItem result;
+		result.trace = std::make_shared<OnewayTrace>(std::move(mergedTrace));
+		ORC_ASSERT(result.trace->trace.size() > 0);
+		size_t seqstart = result.trace->trace[0].DPposition.seqPos;
+		size_t seqend = result.trace->trace.back().DPposition.seqPos;
+		ORC_ASSERT(seqend < sequence.size());
+		result.alignmentScore = result.trace->score;
+		result.alignmentStart = seqstart;
+		result.alignmentEnd = seqend + 1;
+		result.alignmentXScore = (ScoreType)result.alignmentLength() * 100 - params.XscoreErrorCost * (ScoreType)result.alignmentScore;
+		result.alignmentXScore /= 100.0;
+		return result;
+	}
+
+	// src/GraphAligner.h:403-457
+	bool exactAlignmentPart(const AlignmentItem& aln, const SeedHit& seedHit) const
+	{
+		const std::vector<TraceItem>& trace = aln.trace->trace;
+		ORC_ASSERT(trace.back().DPposition.seqPos > trace[0].DPposition.seqPos);
+		if (trace.back().DPposition.seqPos < seedHit.seqPos) return false;
+		if (trace[0].DPposition.seqPos > seedHit.seqPos) return false;
+		size_t high = trace.size();
+		size_t low = 0;
+		size_t mid = (seedHit.seqPos - trace[0].DPposition.seqPos) / (trace.back().DPposition.seqPos - trace[0].DPposition.seqPos);
+		while (trace[mid].DPposition.seqPos != seedHit.seqPos)
+		{
+			if (trace[mid].DPposition.seqPos < seedHit.seqPos)
+			{
+				low = mid;
+				mid = (high + low) / 2;
+				if (mid == low) mid += 1;
+			}
+			if (trace[mid].DPposition.seqPos > seedHit.seqPos)
+			{
+				high = mid;
+				mid = (high + low) / 2;
+			}
+			ORC_ASSERT(low < mid);
+			ORC_ASSERT(mid < high);
+		}
+		size_t down = mid;
+		size_t compareNode = seedHit.nodeID * 2;
+		if (seedHit.reverse) compareNode += 1;
+		while (trace[down].DPposition.seqPos == seedHit.seqPos)
+		{
+			if (compareNode == trace[down].DPposition.node && seedHit.nodeOffset == trace[down].DPposition.nodeOffset) return true;
+			if (down == 0) break;
+			down -= 1;
+		}
+		size_t up = mid;
+		while (trace[up].DPposition.seqPos == seedHit.seqPos)
+		{
+			if (compareNode == trace[up].DPposition.node && seedHit.nodeOffset == trace[up].DPposition.nodeOffset) return true;
+			up += 1;
+			if (up == trace.size()) break;
+		}
+		return false;
+	}
+
+	// src/GraphAligner.h:94-181
+	AlignmentResult AlignOneWay(const std::string& sequence, const std::vector<SeedHit>& seedHits) const
+	{
+		AlignmentResult result;
+		ORC_ASSERT(seedHits.size() > 0);
+		size_t seedScoreForEndToEndAln = 0;
+		size_t extendSeeds = params.seedExtendDensity * sequence.size() + 1;
+		if (params.seedExtendDensity == -1) extendSeeds = seedHits.size();
+		size_t worstExtendedSeedScore = 0;
+		std::string revSequence = ReverseComplement(sequence);
+		for (size_t i = 0; i < seedHits.size(); i++)
+		{
+			if (params.sloppyOptimizations && (seedHits[i].seedGoodness == seedScoreForEndToEndAln || seedHits[i].seedGoodness < seedScoreForEndToEndAln)) break;
+			if (result.seedsExtended >= extendSeeds && (seedHits[i].seedGoodness < worstExtendedSeedScore)) break;
+			if (seedHits[i].seedClusterSize < params.minSeedClusterSize) continue;
+			if (params.sloppyOptimizations)
+			{
+				bool found = false;
+				for (const auto& aln : result.alignments)
+				{
+					if (aln.alignmentStart <= seedHits[i].seqPos && aln.alignmentEnd >= seedHits[i].seqPos && aln.seedGoodness > seedHits[i].seedGoodness)
+					{
+						found = true;
+						break;
+					}
+				}
+				if (found) continue;
+			}
+			bool found = false;
+			for (const auto& aln : result.alignments)
+			{
+				if (exactAlignmentPart(aln, seedHits[i]))
+				{
+					found = true;
+					break;
+				}
+			}
+			if (found) continue;
+			worstExtendedSeedScore = seedHits[i].seedGoodness;
+			result.seedsExtended += 1;
+			auto item = getAlignmentFromSeed(sequence, revSequence, seedHits[i]);
+			if (item.alignmentFailed()) continue;
+			item.seedGoodness = seedHits[i].seedGoodness;
+			result.alignments.emplace_back(std::move(item));
+			if (params.sloppyOptimizations)
+			{
+				std::sort(result.alignments.begin(), result.alignments.end(), [](const AlignmentItem& l, const AlignmentItem& r) { return l.alignmentStart < r.alignmentStart; });
+				if (result.alignments[0].alignmentStart == 0)
+				{
+					size_t minSeedGoodness = result.alignments[0].seedGoodness;
+					size_t contiguousEnd = result.alignments[0].alignmentEnd;
+					for (size_t k = 1; k < result.alignments.size(); k++)
+					{
+						if (result.alignments[k].alignmentStart <= contiguousEnd)
+						{
+							minSeedGoodness = std::min(minSeedGoodness, result.alignments[k].seedGoodness);
+							contiguousEnd = std::max(contiguousEnd, result.alignments[k].alignmentEnd);
+						}
+					}
+					if (contiguousEnd == sequence.size()) seedScoreForEndToEndAln = minSeedGoodness;
+				}
+			}
+		}
+		return result;
+	}
+};
+
+// ---------------------------------------------------------------------------
+// GAF (src/GraphAlignerGAFAlignment.h:37-250)
+// ---------------------------------------------------------------------------
+inline void addCigarItem(std::stringstream& str, size_t editLength, int type)
+{
+	if (editLength == 0) return;
+	static const char ops[] = {'=', 'X', 'M', 'I', 'D'};
+	if (type < 0 || type > 4) return;
+	str << editLength << ops[type];
+}
+
+inline std::string traceToGAF(const std::string& readName, const std::string& sequence, const OnewayTrace& tracePair, double alignmentXScore, int mappingQuality, const AlignmentGraph& graph, bool cigarMatchMismatchMerge)
+{
+	enum { Match = 0, Mismatch = 1, MatchOrMismatch = 2, Insertion = 3, Deletion = 4, Empty = 5 };
+	const auto& trace = tracePair.trace;
+	std::stringstream cigar;
+	size_t readLen = sequence.size();
+	size_t readStart = trace[0].DPposition.seqPos;
+	size_t readEnd = trace.back().DPposition.seqPos + 1;
+	std::stringstream nodePath;
+	size_t nodePathLen = 0;
+	size_t nodePathStart = trace[0].DPposition.nodeOffset;
+	size_t matches = 0;
+	size_t blockLength = trace.size();
+	int curNodeId = trace[0].DPposition.node;
+	bool curReverse = (trace[0].DPposition.node % 2) == 1;
+	size_t curNodeOffset = trace[0].DPposition.nodeOffset;
+	int currentEdit = Empty;
+	size_t mismatches = 0, deletions = 0, insertions = 0, editLength = 0;
+	auto addPos = [&](int nodeId, bool rev) {
+		nodePath << (rev ? "<" : ">");
+		std::string nodeName = graph.originalNodeName.at(nodeId);
+		if (nodeName == "") nodePath << nodeId / 2;
+		else nodePath << nodeName;
+	};
+	if (cigarMatchMismatchMerge)
+	{
+		currentEdit = MatchOrMismatch;
+		editLength = 1;
+		if (characterMatch(trace[0].sequenceCharacter, trace[0].graphCharacter)) matches += 1;
+		else mismatches += 1;
+	}
+	else if (characterMatch(trace[0].sequenceCharacter, trace[0].graphCharacter))
+	{
+		currentEdit = Match;
+		editLength = 1;
+		matches += 1;
+	}
+	else
+	{
+		currentEdit = Mismatch;
+		editLength = 1;
+		mismatches += 1;
+	}
+	addPos(curNodeId, curReverse);
+	nodePathLen += graph.originalNodeSize.at(curNodeId);
+	for (size_t pos = 1; pos < trace.size(); pos++)
+	{
+		int newNodeId = trace[pos].DPposition.node;
+		bool newReverse = (trace[pos].DPposition.node % 2) == 1;
+		size_t newNodeOffset = trace[pos].DPposition.nodeOffset;
+		bool insideNode = !trace[pos - 1].nodeSwitch || (newNodeId == curNodeId && newReverse == curReverse && newNodeOffset > curNodeOffset);
+		if (!insideNode)
+		{
+			size_t skippedBefore = graph.originalNodeSize.at(curNodeId) - 1 - trace[pos - 1].DPposition.nodeOffset;
+			curNodeId = newNodeId;
+			curReverse = newReverse;
+			curNodeOffset = newNodeOffset;
+			addPos(curNodeId, curReverse);
+			size_t skippedAfter = trace[pos].DPposition.nodeOffset;
+			nodePathLen += graph.originalNodeSize.at(curNodeId) - (skippedBefore + skippedAfter);
+		}
+		int nextEdit;
+		if (trace[pos - 1].DPposition.seqPos == trace[pos].DPposition.seqPos) nextEdit = Deletion;
+		else if (insideNode && trace[pos - 1].DPposition.nodeOffset == trace[pos].DPposition.nodeOffset) nextEdit = Insertion;
+		else if (cigarMatchMismatchMerge) nextEdit = MatchOrMismatch;
+		else if (characterMatch(trace[pos].sequenceCharacter, trace[pos].graphCharacter)) nextEdit = Match;
+		else nextEdit = Mismatch;
+		if (currentEdit == Empty) currentEdit = nextEdit;
+		if (currentEdit != nextEdit)
+		{
+			addCigarItem(cigar, editLength, currentEdit);
+			currentEdit = nextEdit;
+			editLength = 0;
+		}
+		editLength += 1;
+		if (nextEdit == Deletion) deletions += 1;
+		else if (nextEdit == Insertion) insertions += 1;
+		else if (nextEdit == MatchOrMismatch)
+		{
+			if (characterMatch(trace[pos].sequenceCharacter, trace[pos].graphCharacter)) matches += 1;
+			else mismatches += 1;
+		}
+		else if (nextEdit == Match) matches += 1;
+		else mismatches += 1;
+	}
+	addCigarItem(cigar, editLength, currentEdit);
+	size_t nodePathEnd = nodePathLen - (graph.originalNodeSize.at(trace.back().DPposition.node) - 1 - trace.back().DPposition.nodeOffset);
+	std::stringstream sstr;
+	sstr << readName << "\t" << readLen << "\t" << readStart << "\t" << readEnd << "\t" << "+" << "\t" << nodePath.str() << "\t" << nodePathLen << "\t" << nodePathStart << "\t" << nodePathEnd << "\t" << matches << "\t" << blockLength << "\t" << mappingQuality;
+	sstr << "\t" << "NM:i:" << (mismatches + deletions + insertions);
+	if (alignmentXScore != -1) sstr << "\t" << "AS:f:" << alignmentXScore;
+	sstr << "\t" << "dv:f:" << 1.0 - ((double)matches / (double)(matches + mismatches + deletions + insertions));
+	sstr << "\t" << "id:f:" << ((double)matches / (double)(matches + mismatches + deletions + insertions));
+	sstr << "\t" << "cg:Z:" << cigar.str();
+	return sstr.str();
+}
+
+// ---------------------------------------------------------------------------
+// Selection (src/AlignmentSelection.cpp:13-95)
+// ---------------------------------------------------------------------------
+inline bool alignmentIncompatible(const AlignmentItem& left, const AlignmentItem& right)
+{
+	const float OverlapIncompatibleFractionCutoff = 0.05;
+	auto minOverlapLen = std::min((left.alignmentEnd - left.alignmentStart), (right.alignmentEnd - right.alignmentStart)) * OverlapIncompatibleFractionCutoff;
+	size_t leftStart = left.alignmentStart, leftEnd = left.alignmentEnd;
+	size_t rightStart = right.alignmentStart, rightEnd = right.alignmentEnd;
+	if (leftStart > rightStart)
+	{
+		std::swap(leftStart, rightStart);
+		std::swap(leftEnd, rightEnd);
+	}
+	int overlap = 0;
+	if (leftEnd > rightStart) overlap = leftEnd - rightStart;
+	return overlap > minOverlapLen;
+}
+
+inline std::vector<AlignmentItem> SelectAlignments(const std::vector<AlignmentItem>& all, double minAlignmentScore, double fraction)
+{
+	std::vector<AlignmentItem> filtered;
+	for (const auto& a : all)
+		if (a.alignmentXScore >= minAlignmentScore) filtered.push_back(a);
+	if (fraction != 0)
+	{
+		std::vector<AlignmentItem> result;
+		for (size_t i = 0; i < filtered.size(); i++)
+		{
+			bool skipped = false;
+			for (size_t j = 0; j < filtered.size(); j++)
+			{
+				if (i == j) continue;
+				if (!alignmentIncompatible(filtered[i], filtered[j])) continue;
+				if (filtered[i].alignmentXScore < filtered[j].alignmentXScore * fraction)
+				{
+					skipped = true;
+					break;
+				}
+			}
+			if (!skipped) result.push_back(filtered[i]);
+		}
+		filtered = std::move(result);
+	}
+	return filtered;
+}
+
+// ---------------------------------------------------------------------------
+// Minimizer seeder (src/MinimizerSeeder.cpp), single-bucket (-t 1) order
+// ---------------------------------------------------------------------------
+inline uint64_t kmerHash(uint64_t key)
+{
+	key = (~key) + (key << 21);
+	key = key ^ (key >> 24);
+	key = (key + (key << 3)) + (key << 8);
+	key = key ^ (key >> 14);
+	key = (key + (key << 2)) + (key << 4);
+	key = key ^ (key >> 28);
+	key = key + (key << 31);
+	return key;
+}
+inline bool validChar(char c)
+{
+	return c == 'a' || c == 'A' || c == 'c' || c == 'C' || c == 'g' || c == 'G' || c == 't' || c == 'T';
+}
+inline size_t charToInt(char c)
+{
+	switch (c)
+	{
+		case 'a': case 'A': return 0;
+		case 'c': case 'C': return 1;
+		case 'g': case 'G': return 2;
+		case 't': case 'T': return 3;
+	}
+	return 0;
+}
+
+// src/MinimizerSeeder.cpp:60-102
+template <typename F>
+void iterateKmers(const std::string& str, size_t kmerLength, size_t windowSize, F callback)
+{
+	const size_t realWindow = windowSize - kmerLength + 1;
+	if (str.size() < kmerLength) return;
+	const size_t mask = ~(0xFFFFFFFFFFFFFFFFull << (kmerLength * 2));
+	size_t offset = 0;
+start:
+	while (offset < str.size() && !validChar(str[offset])) offset++;
+	if (offset + kmerLength > str.size()) return;
+	size_t kmer = 0;
+	for (size_t i = 0; i < kmerLength; i++)
+	{
+		if (!validChar(str[offset + i]))
+		{
+			offset += i;
+			goto start;
+		}
+		kmer <<= 2;
+		kmer |= charToInt(str[offset + i]);
+	}
+	callback(offset + kmerLength - 1, kmer);
+	size_t lastKmer = kmer;
+	size_t lastPos = offset + kmerLength - 1;
+	for (size_t i = kmerLength; offset + i < str.size(); i++)
+	{
+		if (!validChar(str[offset + i]))
+		{
+			offset += i;
+			goto start;
+		}
+		kmer <<= 2;
+		kmer &= mask;
+		kmer |= charToInt(str[offset + i]);
+		if (lastKmer != kmer || lastPos <= offset + i - realWindow)
+		{
+			callback(offset + i, kmer);
+			lastKmer = kmer;
+			lastPos = offset + i;
+		}
+	}
+}
+
+// src/MinimizerSeeder.cpp:104-189
+template <typename F>
+void iterateMinimizersReal(const std::string& str, size_t minimizerLength, size_t windowSize, F callback)
+{
+	if (str.size() < minimizerLength) return;
+	const size_t realWindow = windowSize - minimizerLength + 1;
+	const size_t mask = ~(0xFFFFFFFFFFFFFFFFull << (minimizerLength * 2));
+	size_t offset = 0;
+	std::deque<std::tuple<size_t, size_t, size_t>> window;
+start:
+	while (offset < str.size() && !validChar(str[offset])) offset++;
+	if (offset + windowSize > str.size()) return;
+	size_t kmer = 0;
+	for (size_t i = 0; i < minimizerLength; i++)
+	{
+		if (!validChar(str[offset + i]))
+		{
+			offset += i;
+			goto start;
+		}
+		kmer <<= 2;
+		kmer |= charToInt(str[offset + i]);
+	}
+	window.clear();
+	window.emplace_back(offset + minimizerLength - 1, kmer, kmerHash(kmer));
+	for (size_t i = minimizerLength; i < minimizerLength + realWindow; i++)
+	{
+		if (!validChar(str[offset + i]))
+		{
+			offset += i;
+			goto start;
+		}
+		kmer <<= 2;
+		kmer &= mask;
+		kmer |= charToInt(str[offset + i]);
+		auto hashed = kmerHash(kmer);
+		while (!window.empty() && std::get<2>(window.back()) > hashed) window.pop_back();
+		window.emplace_back(offset + i, kmer, hashed);
+	}
+	{
+		auto iter = window.begin();
+		while (iter != window.end() && std::get<2>(*iter) == std::get<2>(window.front()))
+		{
+			callback(std::get<0>(*iter), std::get<1>(*iter));
+			++iter;
+		}
+	}
+	for (size_t i = minimizerLength + realWindow; offset + i < str.size(); i++)
+	{
+		if (!validChar(str[offset + i]))
+		{
+			offset += i;
+			goto start;
+		}
+		kmer <<= 2;
+		kmer &= mask;
+		kmer |= charToInt(str[offset + i]);
+		auto hashed = kmerHash(kmer);
+		size_t oldMinimum = std::get<2>(window.front());
+		bool frontPopped = false;
+		while (!window.empty() && std::get<0>(window.front()) <= offset + i - realWindow)
+		{
+			frontPopped = true;
+			window.pop_front();
+		}
+		if (frontPopped)
+		{
+			while (window.size() >= 2 && std::get<2>(window.front()) == std::get<2>(*(window.begin() + 1))) window.pop_front();
+		}
+		while (!window.empty() && std::get<2>(window.back()) > hashed) window.pop_back();
+		window.emplace_back(offset + i, kmer, hashed);
+		if (std::get<2>(window.front()) != oldMinimum)
+		{
+			auto iter = window.begin();
+			while (iter != window.end() && std::get<2>(*iter) == std::get<2>(window.front()))
+			{
+				callback(std::get<0>(*iter), std::get<1>(*iter));
+				++iter;
+			}
+		}
+		else if (std::get<2>(window.back()) == std::get<2>(window.front()))
+		{
+			callback(std::get<0>(window.back()), std::get<1>(window.back()));
+		}
+	}
+}
+
+struct MinimizerIndex
+{
+	const AlignmentGraph& graph;
+	size_t k, w;
+	size_t maxCount = 0;
+	// k-mer -> positions (splitNode << 6 | offset) in the reference's storage order
+	std::unordered_map<uint64_t, std::vector<uint64_t>> positions;
+
+	// src/MinimizerSeeder.cpp:299-491 + initMaxCount :557-575
+	MinimizerIndex(const AlignmentGraph& g, size_t k, size_t w, double keepLeastFrequentFraction) : graph(g), k(k), w(w)
+	{
+		std::unordered_map<size_t, size_t> nodeMinimizerStart;
+		for (size_t i = 0; i < graph.NodeSize(); i++)
+		{
+			nodeMinimizerStart[graph.nodeIDs[i]] = std::max(nodeMinimizerStart[graph.nodeIDs[i]], (size_t)0);
+			bool skipStart = false;
+			for (auto n : graph.inNeighbors[i])
+			{
+				if (graph.nodeIDs[n] != graph.nodeIDs[i])
+				{
+					skipStart = true;
+					break;
+				}
+			}
+			if (skipStart) nodeMinimizerStart[graph.nodeIDs[i]] = std::max(nodeMinimizerStart[graph.nodeIDs[i]], graph.nodeOffset[i]);
+		}
+		std::vector<std::pair<uint64_t, uint64_t>> discovered;
+		for (auto iter = graph.nodeLookup.begin(); iter != graph.nodeLookup.end(); ++iter)
+		{
+			int nodeId = iter->first;
+			std::string sequence;
+			sequence.resize(graph.originalNodeSize.at(nodeId));
+			for (size_t pos = 0; pos < sequence.size(); pos++)
+			{
+				size_t here = graph.GetUnitigNode(nodeId, pos);
+				sequence[pos] = graph.NodeSequences(here, pos - graph.nodeOffset[here]);
+			}
+			iterateMinimizersReal(sequence, k, w, [&](size_t pos, size_t kmer) {
+				if (pos < nodeMinimizerStart.at(nodeId)) return;
+				size_t splitNode = graph.GetUnitigNode(nodeId, pos);
+				size_t remainingOffset = pos - graph.nodeOffset[splitNode];
+				discovered.emplace_back(kmer, (splitNode << 6) + remainingOffset);
+			});
+		}
+		// positions of one k-mer are stored in reverse discovery order (:470-480)
+		for (size_t i = discovered.size(); i > 0; i--) positions[discovered[i - 1].first].push_back(discovered[i - 1].second);
+		std::vector<size_t> counts;
+		for (const auto& p : positions) counts.push_back(p.second.size());
+		std::sort(counts.begin(), counts.end());
+		if (counts.size() > 0) counts.erase(counts.begin());
+		maxCount = 0;
+		if (counts.size() == 0) return;
+		size_t index = counts.size() * keepLeastFrequentFraction;
+		if (index == counts.size()) index = counts.size() - 1;
+		maxCount = counts[index] + 1;
+	}
+
+	// src/MinimizerSeeder.cpp:521-555
+	std::vector<SeedHit> getSeeds(const std::string& sequence, double density) const
+	{
+		std::vector<std::tuple<size_t, const std::vector<uint64_t>*, size_t>> matchIndices;
+		iterateKmers(sequence, k, w, [&](size_t pos, size_t kmer) {
+			auto found = positions.find(kmer);
+			if (found == positions.end()) return;
+			size_t count = found->second.size();
+			if (count >= maxCount) return;
+			matchIndices.emplace_back(pos, &found->second, count);
+		});
+		std::vector<SeedHit> result;
+		size_t maxHits = sequence.size() * density;
+		if (density == -1) maxHits = NMAX;
+		std::sort(matchIndices.begin(), matchIndices.end(), [](const auto& l, const auto& r) { return std::get<2>(l) < std::get<2>(r); });
+		size_t seedsHere = 0;
+		size_t allowedCount = 0;
+		for (const auto& match : matchIndices)
+		{
+			size_t count = std::get<2>(match);
+			if (seedsHere >= maxHits && count > allowedCount) break;
+			allowedCount = count;
+			for (uint64_t mergepos : *std::get<1>(match))
+			{
+				size_t splitNode = mergepos >> 6;
+				size_t offset = mergepos & 63;
+				SeedHit hit;
+				hit.nodeID = graph.nodeIDs[splitNode] / 2;
+				hit.nodeOffset = offset + graph.nodeOffset[splitNode];
+				hit.seqPos = std::get<0>(match);
+				hit.matchLen = k;
+				hit.rawSeedGoodness = maxCount - count;
+				hit.reverse = graph.reverse[splitNode];
+				hit.alignmentGraphNodeId = splitNode;
+				hit.alignmentGraphNodeOffset = offset;
+				result.push_back(hit);
+			}
+			seedsHere += count;
+		}
+		return result;
+	}
+};
+
+// ---------------------------------------------------------------------------
+// Per-read pipeline (src/Aligner.cpp:390-606)
+// ---------------------------------------------------------------------------
+struct ReadOutcome
+{
+	int status = 0; // 0 aligned, 1 no seeds, 2 assertion (read dropped), 3 no alignment
+	std::string gaf;
+	size_t seedsExtended = 0;
+};
+
+inline ReadOutcome alignRead(const AlignmentGraph& graph, const MinimizerIndex& index, const Params& params, double minimizerSeedDensity, const std::string& name, const std::string& sequence)
+{
+	ReadOutcome out;
+	Aligner aligner(params, graph);
+	std::vector<AlignmentItem> alignments;
+	try
+	{
+		std::vector<SeedHit> seeds = index.getSeeds(sequence, minimizerSeedDensity);
+		if (seeds.size() == 0)
+		{
+			out.status = 1;
+			return out;
+		}
+		aligner.orderSeedsByChaining(seeds);
+		auto result = aligner.AlignOneWay(sequence, seeds);
+		out.seedsExtended = result.seedsExtended;
+		alignments = std::move(result.alignments);
+	}
+	catch (const AssertFail&)
+	{
+		out.status = 2;
+		return out;
+	}
+	if (alignments.size() > 0) alignments = SelectAlignments(alignments, 0, params.multimapScoreFraction);
+	if (alignments.size() == 0)
+	{
+		out.status = 3;
+		return out;
+	}
+	std::sort(alignments.begin(), alignments.end(), [](const AlignmentItem& l, const AlignmentItem& r) { return l.alignmentStart < r.alignmentStart; });
+	std::sort(alignments.begin(), alignments.end(), [](const AlignmentItem& l, const AlignmentItem& r) { return l.alignmentXScore > r.alignmentXScore; });
+	std::string text;
+	for (auto& a : alignments)
+	{
+		text += traceToGAF(name, sequence, *a.trace, a.alignmentXScore, a.mappingQuality, graph, false);
+		text += '\n';
+	}
+	out.gaf = std::move(text);
+	return out;
+}
+
+} // namespace orc

@@ -1,0 +1,206 @@
+"""ctypes bindings of the CPU oracle (oracle/_build/libga_oracle.so).
+
+Test infrastructure: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the parity checker, never by the product.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(os.path.dirname(_HERE), "oracle", "_build", "libga_oracle.so")
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("bandwidth", ctypes.c_int64),
+        ("maxCellsPerSlice", ctypes.c_int64),
+        ("errorCost", ctypes.c_int32),
+        ("xdrop", ctypes.c_int32),
+        ("seedExtendDensity", ctypes.c_double),
+        ("multimapScoreFraction", ctypes.c_double),
+        ("sloppy", ctypes.c_int32),
+        ("minClusterSize", ctypes.c_int32),
+    ]
+
+
+class SliceSummary(ctypes.Structure):
+    _fields_ = [
+        ("j", ctypes.c_int64),
+        ("minScore", ctypes.c_int32),
+        ("minScoreNode", ctypes.c_int64),
+        ("minScoreNodeOffset", ctypes.c_int64),
+        ("maxExactEndposScore", ctypes.c_int32),
+        ("maxExactEndposNode", ctypes.c_int64),
+        ("cellsProcessed", ctypes.c_int64),
+        ("numNodes", ctypes.c_int64),
+        ("itemsHash", ctypes.c_uint64),
+        ("scoresNotValid", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle library not built: run make -C oracle")
+        L = ctypes.CDLL(_LIB_PATH)
+        vp = ctypes.c_void_p
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.orc_graph_load.restype = vp
+        L.orc_graph_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.orc_graph_free.argtypes = [vp]
+        L.orc_graph_num_nodes.restype = ctypes.c_int64
+        L.orc_graph_num_nodes.argtypes = [vp]
+        L.orc_graph_num_edges.restype = ctypes.c_int64
+        L.orc_graph_num_edges.argtypes = [vp]
+        L.orc_graph_first_ambiguous.restype = ctypes.c_int64
+        L.orc_graph_first_ambiguous.argtypes = [vp]
+        L.orc_graph_export.argtypes = [vp] + [ctypes.c_void_p] * 13
+        L.orc_index_build.restype = vp
+        L.orc_index_build.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.orc_index_free.argtypes = [vp]
+        L.orc_index_maxcount.restype = ctypes.c_int64
+        L.orc_index_maxcount.argtypes = [vp]
+        L.orc_index_num_kmers.restype = ctypes.c_int64
+        L.orc_index_num_kmers.argtypes = [vp]
+        L.orc_get_seeds.argtypes = [vp, vp, ctypes.c_char_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int, i64p, ctypes.c_int64, i64p]
+        L.orc_extend.argtypes = [vp, ctypes.POINTER(OrcParams), ctypes.c_char_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int64,
+                                 i64p, ctypes.c_int64, i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(SliceSummary), ctypes.c_int64, i64p]
+        L.orc_align_reads.restype = ctypes.c_double
+        L.orc_align_reads.argtypes = [vp, vp, ctypes.POINTER(OrcParams), ctypes.c_double, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), i64p, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int32)]
+        L.orc_free.argtypes = [vp]
+        u64 = ctypes.c_uint64
+        L.orc_merge.argtypes = [u64, u64, ctypes.c_int32, u64, u64, ctypes.c_int32, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int32)]
+        L.orc_difference_masks.argtypes = [u64, u64, u64, u64, ctypes.c_int32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.orc_max_xscore.restype = ctypes.c_int32
+        L.orc_max_xscore.argtypes = [u64, u64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, u64]
+        L.orc_changed_min_score.restype = ctypes.c_int32
+        L.orc_changed_min_score.argtypes = [u64, u64, ctypes.c_int32, u64, u64, ctypes.c_int32]
+        L.orc_next_slice.argtypes = [u64, u64, u64, ctypes.c_int32, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.orc_slice_summary_size.restype = ctypes.c_int64
+        assert L.orc_slice_summary_size() == ctypes.sizeof(SliceSummary)
+        _lib = L
+    return _lib
+
+
+def preset_params(preset="vg"):
+    """Preset values of src/AlignerMain.cpp:157-183 and the X-drop default :220-231."""
+    p = OrcParams()
+    p.errorCost = 294
+    p.xdrop = 14705
+    p.multimapScoreFraction = 0.9
+    p.sloppy = 1
+    p.minClusterSize = 1
+    if preset == "vg":
+        p.bandwidth = 10
+        p.maxCellsPerSlice = 0
+        p.seedExtendDensity = -1.0
+    elif preset == "dbg":
+        p.bandwidth = 5
+        p.maxCellsPerSlice = 10000
+        p.seedExtendDensity = 0.002
+    else:
+        raise ValueError(preset)
+    return p
+
+
+PRESET_SEEDING = {"vg": dict(k=15, w=20, density=10.0, discard=0.001),
+                  "dbg": dict(k=19, w=30, density=5.0, discard=0.0002)}
+
+
+class Graph:
+    def __init__(self, path):
+        err = ctypes.create_string_buffer(512)
+        self.h = lib().orc_graph_load(path.encode(), err, 512)
+        if not self.h:
+            raise RuntimeError("oracle graph load failed: " + err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_graph_free(self.h)
+            self.h = None
+
+    def export(self):
+        L = lib()
+        n = L.orc_graph_num_nodes(self.h)
+        e = L.orc_graph_num_edges(self.h)
+        a = {k: np.zeros(n, dtype=np.int64) for k in ["nodeLength", "nodeIDs", "nodeOffset", "comp", "chain", "chainPos"]}
+        lin = np.zeros(n, dtype=np.uint8)
+        rev = np.zeros(n, dtype=np.uint8)
+        seq2 = np.zeros(2 * n, dtype=np.uint64)
+        inOff = np.zeros(n + 1, dtype=np.int64)
+        outOff = np.zeros(n + 1, dtype=np.int64)
+        inAdj = np.zeros(max(e, 1), dtype=np.int64)
+        outAdj = np.zeros(max(e, 1), dtype=np.int64)
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        L.orc_graph_export(self.h, P(a["nodeLength"]), P(a["nodeIDs"]), P(a["nodeOffset"]), P(a["comp"]), P(lin), P(rev),
+                           P(a["chain"]), P(a["chainPos"]), P(seq2), P(inOff), P(inAdj), P(outOff), P(outAdj))
+        a.update(linearizable=lin, reverse=rev, seq2=seq2, inOff=inOff, inAdj=inAdj[:e], outOff=outOff, outAdj=outAdj[:e],
+                 firstAmbiguous=L.orc_graph_first_ambiguous(self.h))
+        return a
+
+
+class Index:
+    def __init__(self, graph, k, w, discard):
+        self.graph = graph
+        self.h = lib().orc_index_build(graph.h, k, w, discard)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_index_free(self.h)
+            self.h = None
+
+    @property
+    def max_count(self):
+        return lib().orc_index_maxcount(self.h)
+
+    def seeds(self, seq, density, order=False):
+        cap = max(1024, 64 * len(seq))
+        out = np.zeros(10 * cap, dtype=np.int64)
+        n = ctypes.c_int64()
+        rc = lib().orc_get_seeds(self.graph.h, self.h, seq.encode(), len(seq), density, int(order),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(n))
+        assert rc == 0
+        return out[: 10 * n.value].reshape(-1, 10)
+
+
+def extend(graph, params, seq, bigraph_node, offset):
+    """getReverseTraceFromSeed: returns (rc, score, trace[n,4], summaries list)."""
+    cap = 4 * len(seq) + 1024
+    trace = np.zeros(4 * cap, dtype=np.int64)
+    nt = ctypes.c_int64()
+    score = ctypes.c_int32()
+    scap = len(seq) // 64 + 4
+    sums = (SliceSummary * scap)()
+    ns = ctypes.c_int64()
+    rc = lib().orc_extend(graph.h, ctypes.byref(params), seq.encode(), len(seq), bigraph_node, offset,
+                          trace.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(nt), ctypes.byref(score),
+                          sums, scap, ctypes.byref(ns))
+    rows = [(s.j, s.minScore, s.minScoreNode, s.minScoreNodeOffset, s.maxExactEndposScore, s.maxExactEndposNode,
+             s.cellsProcessed, s.numNodes, s.itemsHash, s.scoresNotValid) for s in sums[: ns.value]]
+    return rc, score.value, trace[: 4 * nt.value].reshape(-1, 4), rows
+
+
+def align_reads(graph, index, params, density, names, seqs, nthreads=1):
+    n = len(seqs)
+    names_b = [x.encode() for x in names]
+    seqs_b = [x.encode() for x in seqs]
+    na = (ctypes.c_char_p * n)(*names_b)
+    sa = (ctypes.c_char_p * n)(*seqs_b)
+    lens = np.array([len(s) for s in seqs], dtype=np.int64)
+    out = ctypes.c_void_p()
+    status = np.zeros(n, dtype=np.int32)
+    secs = lib().orc_align_reads(graph.h, index.h, ctypes.byref(params), density, n, na, sa,
+                                 lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nthreads, ctypes.byref(out),
+                                 status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    text = ctypes.string_at(out.value).decode()
+    lib().orc_free(out)
+    return text, status, secs
