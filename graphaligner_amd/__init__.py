@@ -1,0 +1,6 @@
+"""MI355X-native seed extension for GraphAligner-style long-read to graph alignment.
+
+Python mirror of the reference boundary (src/GraphAlignerWrapper.h:52-60) over the
+C ABI in include/graphaligner_amd.h; the work runs in the native library.
+"""
+from .api import Graph, Context, Index, params_preset, extend_batch, align_reads  # noqa: F401

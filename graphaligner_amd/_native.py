@@ -1,0 +1,64 @@
+"""ctypes bindings of the C ABI (include/graphaligner_amd.h).
+
+The product path: every call goes into _build/libgraphaligner_amd.so. If the
+library is missing the import fails loudly; GPU entry points fail with
+GA_ENODEVICE when no HIP device is present.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libgraphaligner_amd.so")
+
+GA_OK, GA_EXT_FAILED, GA_ASSERT, GA_EOVERFLOW, GA_EBADARG, GA_EIO, GA_ENODEVICE, GA_EHIP, GA_ENOMEM = range(9)
+STATUS_NAMES = {0: "GA_OK", 1: "GA_EXT_FAILED", 2: "GA_ASSERT", 3: "GA_EOVERFLOW", 4: "GA_EBADARG", 5: "GA_EIO",
+                6: "GA_ENODEVICE", 7: "GA_EHIP", 8: "GA_ENOMEM"}
+
+# every entry point the header declares
+EXPORTED = ["ga_params_preset", "ga_graph_load_gfa", "ga_graph_free", "ga_graph_num_nodes", "ga_graph_num_edges",
+            "ga_graph_first_ambiguous", "ga_graph_export", "ga_context_create", "ga_context_free", "ga_extend_batch",
+            "ga_index_build", "ga_index_free", "ga_index_max_count", "ga_index_num_kmers", "ga_seeds",
+            "ga_order_seeds", "ga_align_reads", "ga_free", "ga_gaf_lines"]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("bandwidth", ctypes.c_int64), ("maxCellsPerSlice", ctypes.c_int64),
+                ("preciseClippingIdentity", ctypes.c_double), ("xdrop", ctypes.c_int32), ("tryAllSeeds", ctypes.c_int32),
+                ("seedExtendDensity", ctypes.c_double), ("seedClusterMinSize", ctypes.c_int64),
+                ("multimapScoreFraction", ctypes.c_double), ("minAlignmentScore", ctypes.c_double),
+                ("minimizerLength", ctypes.c_int32), ("minimizerWindowSize", ctypes.c_int32),
+                ("minimizerSeedDensity", ctypes.c_double), ("minimizerDiscardMostNumerousFraction", ctypes.c_double),
+                ("cigarMatchMismatchMerge", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class SliceSummary(ctypes.Structure):
+    _fields_ = [("j", ctypes.c_int64), ("minScore", ctypes.c_int32), ("pad0", ctypes.c_int32),
+                ("minScoreNode", ctypes.c_int64), ("minScoreNodeOffset", ctypes.c_int64),
+                ("maxExactEndposScore", ctypes.c_int32), ("pad1", ctypes.c_int32), ("maxExactEndposNode", ctypes.c_int64),
+                ("cellsProcessed", ctypes.c_int64), ("numNodes", ctypes.c_int64), ("itemsDigest", ctypes.c_uint64),
+                ("scoresNotValid", ctypes.c_int32), ("pad2", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("graphaligner_amd native library missing (" + LIB_PATH + "): run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+        L.ga_params_preset.argtypes = [ctypes.c_char_p, ctypes.POINTER(Params)]
+        L.ga_graph_load_gfa.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+        L.ga_graph_free.argtypes = [vp]
+        for f in ("ga_graph_num_nodes", "ga_graph_num_edges", "ga_graph_first_ambiguous"):
+            getattr(L, f).restype = i64
+            getattr(L, f).argtypes = [vp]
+        L.ga_graph_export.argtypes = [vp] + [vp] * 13
+        L.ga_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+        L.ga_context_free.argtypes = [vp]
+        L.ga_extend_batch.argtypes = [vp, vp, ctypes.POINTER(Params), i64, ctypes.POINTER(ctypes.c_char_p), vp, vp, vp, i64, vp,
+                                      vp, vp, vp, ctypes.POINTER(SliceSummary), i64, vp, ctypes.POINTER(ctypes.c_double)]
+        _lib = L
+    return _lib

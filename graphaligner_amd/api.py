@@ -1,0 +1,118 @@
+"""Python interface over the C ABI, shaped like the reference's wrapper functions.
+
+* Graph          ~ AlignmentGraph from GfaGraph::LoadFromFile + DirectedGraph::BuildFromGFA
+* extend_batch   ~ GraphAlignerBitvectorBanded::getReverseTraceFromSeed, batched
+* Index          ~ MinimizerSeeder
+* align_reads    ~ the per-read pipeline of src/Aligner.cpp:390-606 with GAF output
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+
+def _check(rc, what, err=None):
+    if rc != N.GA_OK:
+        msg = N.STATUS_NAMES.get(rc, str(rc))
+        if err is not None and err.value:
+            msg += ": " + err.value.decode(errors="replace")
+        raise RuntimeError(f"{what} failed with {msg}")
+
+
+def params_preset(preset="vg"):
+    p = N.Params()
+    _check(N.lib().ga_params_preset(preset.encode() if preset else None, ctypes.byref(p)), "ga_params_preset")
+    return p
+
+
+class Graph:
+    def __init__(self, path):
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        _check(N.lib().ga_graph_load_gfa(path.encode(), ctypes.byref(h), err, 1024), "ga_graph_load_gfa(" + path + ")", err)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().ga_graph_free(self.h)
+            self.h = None
+
+    @property
+    def num_nodes(self):
+        return N.lib().ga_graph_num_nodes(self.h)
+
+    def export(self):
+        L = N.lib()
+        n = L.ga_graph_num_nodes(self.h)
+        e = L.ga_graph_num_edges(self.h)
+        a = {k: np.zeros(n, dtype=np.int64) for k in ["nodeLength", "nodeIDs", "nodeOffset", "comp", "chain", "chainPos"]}
+        lin = np.zeros(n, dtype=np.uint8)
+        rev = np.zeros(n, dtype=np.uint8)
+        seq2 = np.zeros(2 * n, dtype=np.uint64)
+        inOff = np.zeros(n + 1, dtype=np.int64)
+        outOff = np.zeros(n + 1, dtype=np.int64)
+        inAdj = np.zeros(max(e, 1), dtype=np.int64)
+        outAdj = np.zeros(max(e, 1), dtype=np.int64)
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        _check(L.ga_graph_export(self.h, P(a["nodeLength"]), P(a["nodeIDs"]), P(a["nodeOffset"]), P(a["comp"]), P(lin), P(rev),
+                                 P(a["chain"]), P(a["chainPos"]), P(seq2), P(inOff), P(inAdj), P(outOff), P(outAdj)), "ga_graph_export")
+        a.update(linearizable=lin, reverse=rev, seq2=seq2, inOff=inOff, inAdj=inAdj[:e], outOff=outOff, outAdj=outAdj[:e],
+                 firstAmbiguous=L.ga_graph_first_ambiguous(self.h))
+        return a
+
+
+class Context:
+    """One GPU. Raises if no HIP device is present: there is no CPU path."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        _check(N.lib().ga_context_create(device, ctypes.byref(h), err, 1024), "ga_context_create", err)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().ga_context_free(self.h)
+            self.h = None
+
+
+def extend_batch(ctx, graph, params, seqs, bigraph_nodes, offsets, summaries=False):
+    """Returns per extension (status, score, trace[n,4] end first, summaries)."""
+    n = len(seqs)
+    seqs_b = [s.encode() for s in seqs]
+    lens = np.array([len(s) for s in seqs], dtype=np.int64)
+    cap = int(2 * lens.max() + 256) if n else 256
+    traces = np.zeros(4 * cap * max(n, 1), dtype=np.int64)
+    ntrace = np.zeros(n, dtype=np.int64)
+    scores = np.zeros(n, dtype=np.int32)
+    status = np.zeros(n, dtype=np.int32)
+    nsl = np.zeros(n, dtype=np.int64)
+    scap = int(lens.max() // 64 + 4) if n else 1
+    sums = (N.SliceSummary * (scap * n))() if summaries else None
+    ms = ctypes.c_double()
+    arr = (ctypes.c_char_p * n)(*seqs_b)
+    nodes = np.asarray(bigraph_nodes, dtype=np.int32)
+    offs = np.asarray(offsets, dtype=np.int64)
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    rc = N.lib().ga_extend_batch(ctx.h, graph.h, ctypes.byref(params), n, arr, P(lens), P(nodes), P(offs), cap, P(traces),
+                                 P(ntrace), P(scores), P(status), sums, scap, P(nsl), ctypes.byref(ms))
+    _check(rc, "ga_extend_batch")
+    out = []
+    for i in range(n):
+        t = traces[4 * cap * i: 4 * cap * i + 4 * ntrace[i]].reshape(-1, 4)
+        rows = []
+        if summaries:
+            for s in sums[i * scap: i * scap + nsl[i]]:
+                rows.append((s.j, s.minScore, s.minScoreNode, s.minScoreNodeOffset, s.maxExactEndposScore, s.maxExactEndposNode,
+                             s.cellsProcessed, s.numNodes, s.itemsDigest, s.scoresNotValid))
+        out.append((int(status[i]), int(scores[i]), t, rows))
+    return out, ms.value
+
+
+class Index:
+    pass
+
+
+def align_reads(*args, **kwargs):
+    raise NotImplementedError

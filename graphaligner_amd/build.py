@@ -1,0 +1,57 @@
+"""In-tree build of the native libraries (no JIT cache, no install step).
+
+* graphaligner_amd/_build/libgraphaligner_amd.so: the product (host C++ + gfx950 HIP)
+* graphaligner_amd/_build/libga_synth.so: synthetic workload generator
+* oracle/_build/libga_oracle.so and tests/_build/libga_emu.so: test infrastructure
+"""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "graphaligner_amd")
+CSRC = os.path.join(PKG, "csrc")
+OUT = os.path.join(PKG, "_build")
+
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-variable", "-Wno-sign-compare", "-Wno-unused-function"]
+PRODUCT_SOURCES = ["ga_kernels.hip", "ga_runtime.cpp", "ga_graph.cpp", "ga_align.cpp", "ga_index.cpp", "ga_capi.cpp"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed: " + " ".join(cmd) + "\n" + r.stdout[-8000:])
+    return r.stdout
+
+
+def build_product():
+    os.makedirs(OUT, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in PRODUCT_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = os.path.join(OUT, "libgraphaligner_amd.so.tmp")
+    _run(["hipcc"] + HIP_FLAGS + ["-o", tmp] + srcs + ["-lpthread"])
+    os.replace(tmp, os.path.join(OUT, "libgraphaligner_amd.so"))
+
+
+def build_synth():
+    os.makedirs(OUT, exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", os.path.join(OUT, "libga_synth.so"),
+          os.path.join(CSRC, "ga_synth.cpp")])
+
+
+def build_test_infrastructure():
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    tb = os.path.join(ROOT, "tests", "_build")
+    os.makedirs(tb, exist_ok=True)
+    _run(["g++", "-O2", "-g", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-variable", "-o", os.path.join(tb, "libga_emu.so"),
+          os.path.join(ROOT, "tests", "emu", "ga_emu.cpp"), os.path.join(CSRC, "ga_graph.cpp")])
+
+
+def build_all():
+    build_synth()
+    build_test_infrastructure()
+    build_product()
+
+
+if __name__ == "__main__":
+    build_all()

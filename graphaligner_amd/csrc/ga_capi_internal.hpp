@@ -1,0 +1,23 @@
+// Definitions behind the opaque C handles.
+#pragma once
+#include <memory>
+#include "../../include/graphaligner_amd.h"
+#include "ga_graph.hpp"
+#include "ga_runtime.hpp"
+
+struct ga_graph
+{
+	ga::GraphHost host;
+};
+
+struct ga_context
+{
+	std::unique_ptr<ga::Context> ctx;
+};
+
+namespace ga {
+void copyErr(char* err, size_t cap, const char* msg);
+int32_t errorCostOf(const ga_params_t& p);
+int32_t xdropOf(const ga_params_t& p);
+gad::ExtParams extParamsOf(const ga_params_t& p);
+} // namespace ga

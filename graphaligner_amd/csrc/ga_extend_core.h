@@ -1,0 +1,1681 @@
+// Seed extension on one wavefront: the Myers bit-parallel banded DP over the
+// alignment graph (64-row slices, component-ordered band queue, X-drop) and the
+// traceback, for one extension request.
+//
+// All 64 lanes of the wave run the same control flow on the same (uniform)
+// values; the 64-bit VP/VN column words are uniform registers. Work over the 64
+// rows of a column (cellwise minimum of two columns, per-row X-scores, row
+// comparisons) and over band/queue entries (node search, queue minimum) is
+// spread over the lanes and finished with ballots and butterfly reductions.
+//
+// The logic restates src/GraphAlignerBitvectorCommon.h and
+// src/GraphAlignerBitvectorBanded.h (getReverseTraceFromSeed, single-seed path):
+//  * calculateNodeInner          -> calcNode
+//  * calculateSlice/getXdropSlices -> forward
+//  * getReverseTraceFromTable(ExactEndPos) and pickBacktrace* -> traceback
+// with the band queue's ties on (component, score) broken by insertion order.
+//
+// The same source builds for gfx950 and, with GA_WAVE_EMULATION defined, as a
+// host library whose lane helpers loop over the 64 lanes; that build exists only
+// for the CPU test suite.
+#pragma once
+#include <stdint.h>
+#include "ga_device_types.h"
+
+#ifdef GA_WAVE_EMULATION
+#include <algorithm>
+#define GA_HD inline
+#define GA_HHD inline
+#else
+#include <hip/hip_runtime.h>
+#define GA_HD __device__ __forceinline__
+#define GA_HHD __host__ __device__ inline
+#endif
+
+namespace gad {
+
+constexpr int32_t SMAX = 0x7fffffff;
+constexpr int32_t SMIN = (int32_t)0x80000000;
+constexpr uint64_t ONES = ~(uint64_t)0;
+constexpr uint64_t LASTBIT = (uint64_t)1 << 63;
+constexpr uint32_t NIL = 0xffffffffu;
+constexpr uint32_t SKIPFIRST = 0x80000000u;
+constexpr uint32_t LISTEND = 0x7fffffffu; // end of a queued-edge list (low 31 bits of exNext)
+
+struct Slice
+{
+	uint64_t VP;
+	uint64_t VN;
+	int32_t end;
+};
+
+// Match masks of rows j..j+63 for A, C, G, T
+struct Eq4
+{
+	uint64_t m[4];
+};
+GA_HD uint64_t eqSelect(const Eq4& e, uint32_t code)
+{
+	uint64_t lo = (code & 1) ? e.m[1] : e.m[0];
+	uint64_t hi = (code & 1) ? e.m[3] : e.m[2];
+	return (code & 2) ? hi : lo;
+}
+
+GA_HD int32_t popc(uint64_t x) { return __builtin_popcountll(x); }
+GA_HD int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+GA_HD int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+GA_HD Slice mkslice(uint64_t vp, uint64_t vn, int32_t end)
+{
+	Slice s;
+	s.VP = vp;
+	s.VN = vn;
+	s.end = end;
+	return s;
+}
+GA_HD bool same(const Slice& a, const Slice& b) { return a.VP == b.VP && a.VN == b.VN && a.end == b.end; }
+// WordSlice::getScoreBeforeStart (src/WordSlice.h:244-247)
+GA_HD int32_t before(const Slice& s) { return s.end - popc(s.VP) + popc(s.VN); }
+// WordSlice::getValue (src/WordSlice.h:177-186)
+GA_HD int32_t valueAt(const Slice& s, int row)
+{
+	uint64_t mask = row < 63 ? (ONES << (row + 1)) : 0;
+	return s.end + popc(s.VN & mask) - popc(s.VP & mask);
+}
+// getSourceSliceFromScore (src/GraphAlignerBitvectorCommon.h:1056-1060)
+GA_HD Slice sourceSlice(int32_t score) { return mkslice(ONES, 0, score + 64); }
+
+// getNextSlice (src/GraphAlignerBitvectorCommon.h:242-262)
+GA_HD Slice nextSlice(uint64_t Eq, Slice s, uint64_t hinP, uint64_t hinN, uint64_t& Ph, uint64_t& Mh)
+{
+	uint64_t Xv = Eq | s.VN;
+	Eq |= hinN;
+	uint64_t Xh = (((Eq & s.VP) + s.VP) ^ s.VP) | Eq;
+	Ph = s.VN | ~(Xh | s.VP);
+	Mh = s.VP & Xh;
+	uint64_t tempMh = (Mh << 1) | hinN;
+	uint64_t tempPh = (Ph << 1) | hinP;
+	Slice r;
+	r.VP = tempMh | ~(Xv | tempPh);
+	r.VN = tempPh & Xv;
+	r.end = s.end - (int32_t)(Mh >> 63) + (int32_t)(Ph >> 63);
+	return r;
+}
+
+// Candidate rows of WordSlice::maxXScoreLocalMinima (src/WordSlice.h:313-334)
+GA_HD uint64_t maxXCandidates(const Slice& s, int cells, uint64_t extraMask)
+{
+	uint64_t pc = ~s.VP;
+	uint64_t m = s.VP & (pc - s.VP);
+	m >>= 1;
+	m |= LASTBIT & (pc | ~(pc - s.VP)) & ~s.VP;
+	m &= extraMask;
+	m |= 1;
+	if (cells < 64)
+	{
+		m |= (uint64_t)1 << (cells - 1);
+		m &= ~(ONES << cells);
+	}
+	else
+	{
+		m |= LASTBIT;
+	}
+	return m;
+}
+
+// 4-bit base set of a read character (bit0 A, bit1 C, bit2 G, bit3 T), 16 for a
+// character that makes the reference's characterMatch assert (src/GraphAlignerCommon.h:273-276).
+GA_HD uint32_t readBaseSet(unsigned char c)
+{
+	switch (c)
+	{
+		case 'A': case 'a': return 1;
+		case 'C': case 'c': return 2;
+		case 'G': case 'g': return 4;
+		case 'T': case 't': case 'U': case 'u': return 8;
+		case 'R': case 'r': return 5;
+		case 'Y': case 'y': return 10;
+		case 'K': case 'k': return 12;
+		case 'M': case 'm': return 3;
+		case 'S': case 's': return 6;
+		case 'W': case 'w': return 9;
+		case 'B': case 'b': return 14;
+		case 'D': case 'd': return 13;
+		case 'H': case 'h': return 11;
+		case 'V': case 'v': return 7;
+		case 'N': case 'n': return 15;
+		case '-': return 0;
+	}
+	return 16;
+}
+
+// ---------------------------------------------------------------------------
+// Lane-parallel helpers. Each takes uniform inputs and returns a uniform result.
+// ---------------------------------------------------------------------------
+#ifndef GA_WAVE_EMULATION
+
+GA_HD int laneId() { return __lane_id(); }
+GA_HD uint64_t prefixMask(int lane) { return lane == 63 ? ONES : (((uint64_t)1 << (lane + 1)) - 1); }
+// value of row `lane` of slice s
+GA_HD int32_t laneValue(const Slice& s, int32_t bef, uint64_t pm) { return bef + popc(s.VP & pm) - popc(s.VN & pm); }
+GA_HD int32_t waveMaxI(int32_t v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = imax(v, __shfl_xor(v, o));
+	return v;
+}
+GA_HD int32_t waveMinI(int32_t v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = imin(v, __shfl_xor(v, o));
+	return v;
+}
+GA_HD uint64_t waveSumU64(uint64_t v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+	return v;
+}
+
+// Cellwise minimum of two columns over rows -1..63, in slice form. Equals
+// WordSlice::mergeTwoSlices (src/WordSlice.h:502-541) for valid slices.
+GA_HD Slice lpMerge(const Slice& a, const Slice& b)
+{
+	int l = laneId();
+	uint64_t pm = prefixMask(l);
+	int32_t ba = before(a), bb = before(b);
+	int32_t bm = imin(ba, bb);
+	int32_t v = imin(laneValue(a, ba, pm), laneValue(b, bb, pm));
+	int32_t up = __shfl_up(v, 1);
+	if (l == 0) up = bm;
+	int32_t d = v - up;
+	Slice r;
+	r.VP = __ballot(d == 1);
+	r.VN = __ballot(d == -1);
+	r.end = imin(a.end, b.end);
+	return r;
+}
+
+// rows where a < b
+GA_HD uint64_t lpLessMask(const Slice& a, const Slice& b)
+{
+	uint64_t pm = prefixMask(laneId());
+	return __ballot(laneValue(a, before(a), pm) < laneValue(b, before(b), pm));
+}
+
+// minimum of the row values selected by mask (SMAX if empty)
+GA_HD int32_t lpMinOver(const Slice& s, uint64_t mask)
+{
+	int l = laneId();
+	int32_t v = ((mask >> l) & 1) ? laneValue(s, before(s), prefixMask(l)) : SMAX;
+	return waveMinI(v);
+}
+
+// max over candidate rows r of (seqOffset + r + 1) * 100 - value(r) * cost
+GA_HD int32_t lpMaxX(const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand)
+{
+	int l = laneId();
+	int32_t v = SMIN;
+	if ((cand >> l) & 1) v = (seqOffset + l + 1) * 100 - laneValue(s, before(s), prefixMask(l)) * cost;
+	return waveMaxI(v);
+}
+
+// per-lane running maximum of X-scores, reduced once per node
+struct LaneAcc
+{
+	int32_t v;
+};
+GA_HD void accInit(LaneAcc& a) { a.v = SMIN; }
+GA_HD void accAdd(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand)
+{
+	int l = laneId();
+	if ((cand >> l) & 1) a.v = imax(a.v, (seqOffset + l + 1) * 100 - laneValue(s, before(s), prefixMask(l)) * cost);
+}
+GA_HD int32_t accReduce(const LaneAcc& a) { return waveMaxI(a.v); }
+
+// index of key in arr[0, n), or -1
+GA_HD int32_t lpFind(const uint32_t* arr, uint32_t n, uint32_t key)
+{
+	int l = laneId();
+	for (uint32_t base = 0; base < n; base += 64)
+	{
+		uint32_t i = base + l;
+		bool hit = i < n && arr[i] == key;
+		uint64_t m = __ballot(hit);
+		if (m) return (int32_t)(base + __builtin_ctzll(m));
+	}
+	return -1;
+}
+
+// Eq vector of rows j..j+63 (src/GraphAlignerBitvectorCommon.h:279-318)
+GA_HD bool lpEq(const char* seq, uint32_t len, uint32_t j, Eq4& eq)
+{
+	int l = laneId();
+	uint32_t set = 0;
+	if (j + l < len) set = readBaseSet((unsigned char)seq[j + l]);
+	eq.m[0] = __ballot(set & 1);
+	eq.m[1] = __ballot((set >> 1) & 1);
+	eq.m[2] = __ballot((set >> 2) & 1);
+	eq.m[3] = __ballot((set >> 3) & 1);
+	return __ballot(set == 16) != 0;
+}
+
+// index of the smallest (key, seq) among n queue entries
+GA_HD uint32_t lpArgMin(const uint64_t* key, const uint32_t* seq, uint32_t n)
+{
+	int l = laneId();
+	uint64_t bk = ONES;
+	uint32_t bs = NIL, bi = NIL;
+	for (uint32_t i = l; i < n; i += 64)
+	{
+		uint64_t k = key[i];
+		uint32_t s = seq[i];
+		if (k < bk || (k == bk && s < bs))
+		{
+			bk = k;
+			bs = s;
+			bi = i;
+		}
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+	{
+		uint64_t ok = __shfl_xor(bk, o);
+		uint32_t os = __shfl_xor(bs, o);
+		uint32_t oi = __shfl_xor(bi, o);
+		if (ok < bk || (ok == bk && os < bs))
+		{
+			bk = ok;
+			bs = os;
+			bi = oi;
+		}
+	}
+	return bi;
+}
+
+GA_HD uint64_t mixHash(uint64_t h, uint64_t v) { return h ^ (v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)); }
+GA_HD uint64_t itemHash(const Item& it)
+{
+	uint64_t h = 0x12345;
+	h = mixHash(h, it.node);
+	h = mixHash(h, it.sVP);
+	h = mixHash(h, it.sVN);
+	h = mixHash(h, (uint32_t)it.sEnd);
+	h = mixHash(h, it.eVP);
+	h = mixHash(h, it.eVN);
+	h = mixHash(h, (uint32_t)it.eEnd);
+	h = mixHash(h, it.HP);
+	h = mixHash(h, it.HN);
+	h = mixHash(h, (uint32_t)it.minScore);
+	return h;
+}
+// order-independent digest of a slice's band items (test output only)
+GA_HD uint64_t lpDigest(const Item* items, uint32_t start, uint32_t count)
+{
+	uint64_t h = 0;
+	for (uint32_t i = laneId(); i < count; i += 64) h += itemHash(items[start + i]);
+	return waveSumU64(h);
+}
+
+// the node's column slices, column c held by lane c
+struct ColumnStore
+{
+	uint64_t vp, vn;
+	int32_t end;
+};
+GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
+{
+	if (laneId() == col)
+	{
+		c.vp = s.VP;
+		c.vn = s.VN;
+		c.end = s.end;
+	}
+}
+GA_HD Slice colGet(const ColumnStore& c, int col)
+{
+	Slice s;
+	s.VP = __shfl(c.vp, col);
+	s.VN = __shfl(c.vn, col);
+	s.end = __shfl(c.end, col);
+	return s;
+}
+
+#else // GA_WAVE_EMULATION: the same helpers as loops over 64 lanes
+
+inline uint64_t prefixMask(int lane) { return lane == 63 ? ONES : (((uint64_t)1 << (lane + 1)) - 1); }
+inline int32_t laneValue(const Slice& s, int32_t bef, uint64_t pm) { return bef + popc(s.VP & pm) - popc(s.VN & pm); }
+inline Slice lpMerge(const Slice& a, const Slice& b)
+{
+	int32_t ba = before(a), bb = before(b);
+	int32_t up = imin(ba, bb);
+	Slice r;
+	r.VP = 0;
+	r.VN = 0;
+	for (int l = 0; l < 64; l++)
+	{
+		uint64_t pm = prefixMask(l);
+		int32_t v = imin(laneValue(a, ba, pm), laneValue(b, bb, pm));
+		if (v - up == 1) r.VP |= (uint64_t)1 << l;
+		if (v - up == -1) r.VN |= (uint64_t)1 << l;
+		up = v;
+	}
+	r.end = imin(a.end, b.end);
+	return r;
+}
+inline uint64_t lpLessMask(const Slice& a, const Slice& b)
+{
+	uint64_t m = 0;
+	int32_t ba = before(a), bb = before(b);
+	for (int l = 0; l < 64; l++)
+		if (laneValue(a, ba, prefixMask(l)) < laneValue(b, bb, prefixMask(l))) m |= (uint64_t)1 << l;
+	return m;
+}
+inline int32_t lpMinOver(const Slice& s, uint64_t mask)
+{
+	int32_t r = SMAX;
+	for (int l = 0; l < 64; l++)
+		if ((mask >> l) & 1) r = imin(r, laneValue(s, before(s), prefixMask(l)));
+	return r;
+}
+inline int32_t lpMaxX(const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand)
+{
+	int32_t r = SMIN;
+	for (int l = 0; l < 64; l++)
+		if ((cand >> l) & 1) r = imax(r, (seqOffset + l + 1) * 100 - laneValue(s, before(s), prefixMask(l)) * cost);
+	return r;
+}
+struct LaneAcc
+{
+	int32_t v[64];
+};
+inline void accInit(LaneAcc& a)
+{
+	for (int l = 0; l < 64; l++) a.v[l] = SMIN;
+}
+inline void accAdd(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand)
+{
+	for (int l = 0; l < 64; l++)
+		if ((cand >> l) & 1) a.v[l] = imax(a.v[l], (seqOffset + l + 1) * 100 - laneValue(s, before(s), prefixMask(l)) * cost);
+}
+inline int32_t accReduce(const LaneAcc& a)
+{
+	int32_t r = SMIN;
+	for (int l = 0; l < 64; l++) r = imax(r, a.v[l]);
+	return r;
+}
+inline int32_t lpFind(const uint32_t* arr, uint32_t n, uint32_t key)
+{
+	for (uint32_t i = 0; i < n; i++)
+		if (arr[i] == key) return (int32_t)i;
+	return -1;
+}
+inline bool lpEq(const char* seq, uint32_t len, uint32_t j, Eq4& eq)
+{
+	eq.m[0] = eq.m[1] = eq.m[2] = eq.m[3] = 0;
+	bool invalid = false;
+	for (int l = 0; l < 64; l++)
+	{
+		if (j + l >= len) continue;
+		uint32_t set = readBaseSet((unsigned char)seq[j + l]);
+		if (set == 16) invalid = true;
+		for (int b = 0; b < 4; b++)
+			if ((set >> b) & 1) eq.m[b] |= (uint64_t)1 << l;
+	}
+	return invalid;
+}
+inline uint32_t lpArgMin(const uint64_t* key, const uint32_t* seq, uint32_t n)
+{
+	uint32_t bi = 0;
+	for (uint32_t i = 1; i < n; i++)
+		if (key[i] < key[bi] || (key[i] == key[bi] && seq[i] < seq[bi])) bi = i;
+	return bi;
+}
+GA_HD uint64_t mixHash(uint64_t h, uint64_t v) { return h ^ (v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)); }
+GA_HD uint64_t itemHash(const Item& it)
+{
+	uint64_t h = 0x12345;
+	h = mixHash(h, it.node);
+	h = mixHash(h, it.sVP);
+	h = mixHash(h, it.sVN);
+	h = mixHash(h, (uint32_t)it.sEnd);
+	h = mixHash(h, it.eVP);
+	h = mixHash(h, it.eVN);
+	h = mixHash(h, (uint32_t)it.eEnd);
+	h = mixHash(h, it.HP);
+	h = mixHash(h, it.HN);
+	h = mixHash(h, (uint32_t)it.minScore);
+	return h;
+}
+inline uint64_t lpDigest(const Item* items, uint32_t start, uint32_t count)
+{
+	uint64_t h = 0;
+	for (uint32_t i = 0; i < count; i++) h += itemHash(items[start + i]);
+	return h;
+}
+struct ColumnStore
+{
+	Slice c[64];
+};
+inline void colSet(ColumnStore& c, int col, const Slice& s) { c.c[col] = s; }
+inline Slice colGet(const ColumnStore& c, int col) { return c.c[col]; }
+
+#endif
+
+// ---------------------------------------------------------------------------
+// One extension
+// ---------------------------------------------------------------------------
+struct NodeResult
+{
+	int32_t minScore;
+	int32_t minScoreNodeOffset;
+	int32_t maxX;
+	int32_t cells;
+};
+
+// Band queue entry key: component in the high word, score (order-preserving) low.
+GA_HD uint64_t queueKey(uint32_t comp, int32_t score) { return ((uint64_t)comp << 32) | (uint32_t)((uint32_t)score ^ 0x80000000u); }
+
+GA_HD uint64_t packTrace(uint32_t node, int32_t seqPos, uint32_t offset, uint32_t nodeSwitch)
+{
+	uint32_t hi = ((uint32_t)(seqPos + 1) << 7) | (offset << 1) | nodeSwitch;
+	return ((uint64_t)hi << 32) | node;
+}
+
+struct Pos
+{
+	uint32_t node;
+	uint32_t off;
+	int32_t seq;
+};
+
+GA_HD bool samePos(const Pos& a, const Pos& b) { return a.node == b.node && a.off == b.off && a.seq == b.seq; }
+GA_HD Pos mkpos(uint32_t node, uint32_t off, int32_t seq)
+{
+	Pos p;
+	p.node = node;
+	p.off = off;
+	p.seq = seq;
+	return p;
+}
+
+struct Extension
+{
+	// inputs
+	DevGraph g;
+	ExtParams p;
+	const char* seq;
+	uint32_t len;
+	uint32_t startNode;
+	uint32_t startOffset;
+	// workspace
+	ArenaCaps caps;
+	SliceHdr* hdr;
+	Item* items;
+	uint32_t* itemNode;
+	uint32_t* slotNode;
+	int32_t* slotItem;
+	uint32_t* slotHead;
+	uint32_t* slotTail;
+	uint32_t* slotActive;
+	uint64_t* exVP;
+	uint64_t* exVN;
+	int32_t* exEnd;
+	uint32_t* exNext;
+	uint64_t* pendKey;
+	uint32_t* pendSeq;
+	uint32_t* pendSlot;
+	// outputs
+	uint64_t* trace;
+	uint32_t traceCap;
+	SliceSummary* summaries; // optional
+	// state
+	uint32_t nItems, kept, nSlots, nEx, nPend, seqCounter;
+	uint32_t ntrace;
+	int32_t status;
+	uint32_t overflowWhat;
+	uint64_t wordSteps, nodeCalcs;
+	uint32_t maxBand;
+	Pos back;
+
+	GA_HD uint32_t nodeLen(uint32_t n) const { return g.nodeLen[n]; }
+	GA_HD uint32_t nodeBaseSet(uint32_t n, uint32_t pos) const
+	{
+		if (n < g.firstAmbiguous)
+		{
+			uint32_t code = (uint32_t)(g.seq2[2 * n + (pos >> 5)] >> ((pos & 31) * 2)) & 3;
+			return 1u << code;
+		}
+		const uint64_t* m = g.amb4 + 4 * (uint64_t)(n - g.firstAmbiguous);
+		return (uint32_t)((m[0] >> pos) & 1) | (uint32_t)(((m[1] >> pos) & 1) << 1) | (uint32_t)(((m[2] >> pos) & 1) << 2) | (uint32_t)(((m[3] >> pos) & 1) << 3);
+	}
+	GA_HD uint64_t eqAt(const Eq4& eq, uint32_t n, uint32_t pos) const
+	{
+		if (n < g.firstAmbiguous)
+		{
+			uint32_t code = (uint32_t)(g.seq2[2 * n + (pos >> 5)] >> ((pos & 31) * 2)) & 3;
+			return eqSelect(eq, code);
+		}
+		uint32_t set = nodeBaseSet(n, pos);
+		uint64_t r = 0;
+		if (set & 1) r |= eq.m[0];
+		if (set & 2) r |= eq.m[1];
+		if (set & 4) r |= eq.m[2];
+		if (set & 8) r |= eq.m[3];
+		return r;
+	}
+	// characterMatch(read char, graph char) (src/GraphAlignerCommon.h:176-198)
+	GA_HD bool charMatch(int32_t seqPos, uint32_t node, uint32_t off) const
+	{
+		uint32_t rs = readBaseSet((unsigned char)seq[seqPos]);
+		return (rs & nodeBaseSet(node, off) & 15) != 0;
+	}
+	GA_HD void fail(int32_t st, uint32_t what = 0)
+	{
+		if (status == EXT_OK)
+		{
+			status = st;
+			overflowWhat = what;
+		}
+	}
+
+	// --- band queue (ComponentPriorityQueue with insertion-order ties) ---
+	GA_HD int32_t findSlot(uint32_t node) const { return lpFind(slotNode, nSlots, node); }
+
+	GA_HD void queueInsert(uint32_t comp, int32_t score, uint32_t target, const Slice& s, bool skipFirst)
+	{
+		int32_t slot = findSlot(target);
+		if (slot < 0)
+		{
+			if (nSlots >= caps.maxSlots) { fail(EXT_OVERFLOW, 2); return; }
+			slot = nSlots++;
+			slotNode[slot] = target;
+			slotItem[slot] = -1;
+			slotHead[slot] = NIL;
+			slotTail[slot] = NIL;
+			slotActive[slot] = 0;
+		}
+		if (nEx >= caps.maxExtras) { fail(EXT_OVERFLOW, 3); return; }
+		if (!slotActive[slot])
+		{
+			if (nPend >= caps.maxPending) { fail(EXT_OVERFLOW, 4); return; }
+			pendKey[nPend] = queueKey(comp, score);
+			pendSeq[nPend] = seqCounter++;
+			pendSlot[nPend] = (uint32_t)slot;
+			nPend++;
+			slotActive[slot] = 1;
+		}
+		uint32_t e = nEx++;
+		exVP[e] = s.VP;
+		exVN[e] = s.VN;
+		exEnd[e] = s.end;
+		exNext[e] = (skipFirst ? SKIPFIRST : 0) | LISTEND;
+		uint32_t tail = slotTail[slot];
+		if (tail == NIL) slotHead[slot] = e;
+		else exNext[tail] = (exNext[tail] & SKIPFIRST) | e;
+		slotTail[slot] = e;
+	}
+
+	// --- one node's columns for one slice (calculateNodeInner) ---
+	// Forward mode (Recalc = false) with early leave; recalc mode stores every
+	// column in `cols` and leaves the X-scores out.
+	template <bool Recalc>
+	GA_HD NodeResult calcNode(uint32_t node, Item& it, bool exists, const Eq4& eq, const Item& prev, bool prevExists, uint32_t firstExtra, const Slice& recalcIncoming, int32_t seqOffset, bool prevHasInNeighbor, ColumnStore* cols)
+	{
+		NodeResult res;
+		res.minScore = SMAX;
+		res.minScoreNodeOffset = -1;
+		res.maxX = SMIN;
+		res.cells = 0;
+		uint32_t L = nodeLen(node);
+		uint64_t Eq0 = eqAt(eq, node, 0);
+		Slice ws = mkslice(0, 0, 0);
+		bool hasWs = false, hasSkipless = false;
+		if (Recalc)
+		{
+			res.cells = 1;
+			ws = recalcIncoming;
+			hasWs = true;
+		}
+		else
+		{
+			for (uint32_t e = firstExtra; e < caps.maxExtras; e = exNext[e] & ~SKIPFIRST)
+			{
+				res.cells++;
+				Slice inc = mkslice(exVP[e], exVN[e], exEnd[e]);
+				if (exNext[e] & SKIPFIRST)
+				{
+					ws = hasWs ? lpMerge(ws, inc) : inc;
+					hasWs = true;
+					continue;
+				}
+				hasSkipless = true;
+				int32_t compareHin = prevExists ? prev.sEnd : SMAX;
+				int32_t ib = before(inc);
+				uint64_t hinP = 0, hinN = 0;
+				if (compareHin < ib) hinN = 1;
+				else if (compareHin > ib) hinP = 1;
+				uint64_t Ph, Mh;
+				Slice stepped = nextSlice(Eq0, inc, hinP, hinN, Ph, Mh);
+				wordSteps++;
+				if (!prevExists || before(stepped) < prev.sEnd)
+				{
+					stepped.VP &= ONES ^ 1;
+					stepped.VN |= 1;
+				}
+				ws = hasWs ? lpMerge(ws, stepped) : stepped;
+				hasWs = true;
+			}
+		}
+		res.minScore = ws.end;
+		res.minScoreNodeOffset = 0;
+		if (!Recalc) res.maxX = lpMaxX(ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, ONES));
+
+		if (exists)
+		{
+			Slice start = mkslice(it.sVP, it.sVN, it.sEnd);
+			if (hasSkipless && (g.inOff[node + 1] - g.inOff[node]) == 1 && prevHasInNeighbor)
+			{
+				if (ws.end > start.end)
+				{
+					if (!Recalc) return res;
+				}
+				else if (ws.end < start.end)
+				{
+				}
+				else
+				{
+					uint64_t newBigger = (ws.VP & ~start.VP) | (start.VN & ~ws.VN);
+					uint64_t oldBigger = (start.VP & ~ws.VP) | (ws.VN & ~start.VN);
+					if (newBigger > oldBigger)
+					{
+					}
+					else if (oldBigger > newBigger)
+					{
+						if (!Recalc) return res;
+					}
+					else if (newBigger == 0 && oldBigger == 0)
+					{
+						if (!Recalc) return res;
+					}
+					else
+					{
+						Slice test = lpMerge(ws, start);
+						if (same(test, start))
+						{
+							if (!Recalc) return res;
+						}
+						ws = test;
+					}
+				}
+			}
+			else
+			{
+				Slice test = lpMerge(ws, start);
+				// src/GraphAlignerBitvectorCommon.h:1282 compares test.VP with startSlice.VN
+				if (test.end == start.end && test.VP == start.VP && test.VP == start.VN)
+				{
+					if (!Recalc) return res;
+				}
+				ws = test;
+			}
+		}
+
+		if (prevExists && before(ws) > prev.sEnd) ws = lpMerge(ws, sourceSlice(prev.sEnd));
+
+		uint64_t forceMask = 0;
+		uint64_t fixedHP = prev.HP, fixedHN = prev.HN;
+		if (prevExists)
+		{
+			int32_t scoreBefore = before(ws);
+			int32_t cmp = prev.sEnd;
+			if (scoreBefore > cmp)
+			{
+				fail(EXT_ASSERT);
+				return res;
+			}
+			if (scoreBefore < cmp)
+			{
+				forceMask |= 1;
+				for (uint32_t fo = 1; fo < 64; fo++)
+				{
+					int32_t ncmp = cmp + (int32_t)((fixedHP >> fo) & 1) - (int32_t)((fixedHN >> fo) & 1);
+					uint64_t mask = (uint64_t)1 << fo;
+					if (scoreBefore > ncmp)
+					{
+						fail(EXT_ASSERT);
+						return res;
+					}
+					if (scoreBefore < ncmp)
+					{
+						fixedHP |= mask;
+						fixedHN &= ~mask;
+						forceMask |= mask;
+					}
+					if (scoreBefore == ncmp)
+					{
+						fixedHP &= ~mask;
+						fixedHN &= ~mask;
+					}
+					scoreBefore++;
+					cmp = ncmp;
+					if (scoreBefore >= cmp) break;
+				}
+			}
+		}
+		else
+		{
+			forceMask = ONES;
+		}
+		it.sVP = ws.VP;
+		it.sVN = ws.VN;
+		it.sEnd = ws.end;
+		if (Recalc) colSet(*cols, 0, ws);
+		uint64_t forceEq = prevExists ? ONES : (ONES ^ 1);
+		uint64_t HP = 0, HN = 0;
+		LaneAcc acc;
+		if (!Recalc)
+		{
+			accInit(acc);
+		}
+		for (uint32_t pos = 1; pos < L; pos++)
+		{
+			uint64_t Eq = eqAt(eq, node, pos) & forceEq;
+			uint64_t Ph, Mh;
+			Slice nw = nextSlice(Eq, ws, (fixedHP >> pos) & 1, (fixedHN >> pos) & 1, Ph, Mh);
+			if ((forceMask >> pos) & 1)
+			{
+				nw.VP &= ONES ^ 1;
+				nw.VN |= 1;
+			}
+			ws = nw;
+			if (ws.end < res.minScore)
+			{
+				res.minScore = ws.end;
+				res.minScoreNodeOffset = pos;
+			}
+			if (Recalc) colSet(*cols, pos, ws);
+			else accAdd(acc, ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, Mh));
+			HP |= (Ph >> 63) << pos;
+			HN |= (Mh >> 63) << pos;
+		}
+		if (!Recalc)
+		{
+			wordSteps += L - 1;
+			if (L > 1) res.maxX = imax(res.maxX, accReduce(acc));
+		}
+		res.cells = L;
+		it.HP = HP;
+		it.HN = HN;
+		it.eVP = ws.VP;
+		it.eVN = ws.VN;
+		it.eEnd = ws.end;
+		if (prevExists && before(ws) > prev.eEnd) fail(EXT_ASSERT);
+		return res;
+	}
+
+	// WordSlice::changedMinScore (src/WordSlice.h:438-472)
+	GA_HD int32_t changedMinScore(const Slice& s, const Slice& old) const
+	{
+		int32_t b = before(s), ob = before(old);
+		uint64_t m = (s.VP & (s.VN - s.VP)) >> 1;
+		m |= LASTBIT & (s.VN | ~(s.VN - s.VP)) & ~s.VP;
+		if (s.end + popc(s.VN) >= old.end - popc(old.VP))
+		{
+			uint64_t smaller = lpLessMask(s, old);
+			if (smaller != ONES)
+			{
+				m |= (~smaller) >> 1;
+				m |= (~smaller) << 1;
+				m |= 1;
+				m |= LASTBIT;
+				m &= smaller;
+			}
+		}
+		int32_t r = b < ob ? b : SMAX;
+		return imin(r, lpMinOver(s, m));
+	}
+
+	GA_HD Item loadItem(uint32_t i) const { return items[i]; }
+	GA_HD void storeItem(uint32_t i, const Item& it) { items[i] = it; }
+	GA_HD int32_t findInSlice(uint32_t sliceIdx, uint32_t node) const
+	{
+		const SliceHdr& h = hdr[sliceIdx];
+		int32_t k = lpFind(itemNode + h.itemStart, h.itemCount, node);
+		return k < 0 ? -1 : (int32_t)(h.itemStart + k);
+	}
+	GA_HD static Item defaultPrev()
+	{
+		Item d;
+		d.sVP = d.sVN = d.eVP = d.eVN = 0;
+		d.HP = ONES;
+		d.HN = 0;
+		d.sEnd = d.eEnd = d.minScore = 0;
+		d.node = NIL;
+		return d;
+	}
+
+	// --- forward pass: getXdropSlices + calculateSlice ---
+	GA_HD void forward()
+	{
+		uint32_t L0 = nodeLen(startNode);
+		Item first;
+		first.node = startNode;
+		first.sVP = first.sVN = first.eVP = first.eVN = 0;
+		first.sEnd = (int32_t)startOffset;
+		first.eEnd = (int32_t)L0 - 1 - (int32_t)startOffset;
+		first.HP = 0;
+		first.HN = 0;
+		for (uint32_t i = 1; i <= startOffset; i++) first.HN |= (uint64_t)1 << i;
+		for (uint32_t i = startOffset + 1; i < L0; i++) first.HP |= (uint64_t)1 << i;
+		first.minScore = 0;
+		items[0] = first;
+		itemNode[0] = startNode;
+		SliceHdr h0;
+		h0.minScore = 0;
+		h0.minScoreNode = (int32_t)startNode;
+		h0.minScoreNodeOffset = (int32_t)startOffset;
+		h0.maxExactEndposScore = 0;
+		h0.maxExactEndposNode = (int32_t)startNode;
+		h0.bandwidth = 1;
+		h0.scoresNotValid = 0;
+		h0.itemStart = 0;
+		h0.itemCount = 1;
+		h0.pad = 0;
+		h0.cells = 0;
+		hdr[0] = h0;
+		nItems = 1;
+		kept = 1;
+		int32_t bestX = 0;
+		const int32_t b = p.bandwidth;
+		uint32_t numSlices = (len + 63) / 64;
+		if (numSlices + 1 > caps.maxSlices)
+		{
+			fail(EXT_OVERFLOW, 1);
+			return;
+		}
+		for (uint32_t s = 0; s < numSlices && status == EXT_OK; s++)
+		{
+			const int32_t j = (int32_t)(s * 64);
+			const SliceHdr P = hdr[kept - 1];
+			const int32_t prevQuit = P.minScore + P.bandwidth;
+			Eq4 eq;
+			if (lpEq(seq, len, (uint32_t)j, eq))
+			{
+				fail(EXT_ASSERT);
+				return;
+			}
+			const uint32_t sliceItemStart = nItems;
+			nSlots = 0;
+			nEx = 0;
+			nPend = 0;
+			seqCounter = 0;
+			for (uint32_t k = 0; k < P.itemCount; k++)
+			{
+				uint32_t it = P.itemStart + k;
+				Item pi = loadItem(it);
+				if (j != 0)
+				{
+					if (pi.minScore > prevQuit) continue;
+					if (g.linearizable[pi.node])
+					{
+						uint32_t nb = g.inAdj[g.inOff[pi.node]];
+						int32_t q = findInSlice(kept - 1, nb);
+						if (q >= 0)
+						{
+							Item qi = loadItem(q);
+							if (qi.eEnd < prevQuit && qi.minScore < prevQuit) continue;
+						}
+					}
+				}
+				queueInsert(g.comp[pi.node], pi.minScore, pi.node, sourceSlice(pi.sEnd), true);
+			}
+			if (status != EXT_OK) return;
+			if (nPend == 0)
+			{
+				fail(EXT_ASSERT);
+				return;
+			}
+			int32_t curMin = SMAX - b - 1;
+			int32_t rMin = SMAX - b - 1, rMinNode = -1, rMinOff = -1, rMaxX = SMIN, rMaxXNode = 0;
+			int64_t cells = 0;
+			while (nPend > 0)
+			{
+				uint32_t pi = lpArgMin(pendKey, pendSeq, nPend);
+				uint32_t slot = pendSlot[pi];
+				uint32_t node = slotNode[slot];
+				int32_t itIdx = slotItem[slot];
+				bool exists = itIdx >= 0;
+				Item it;
+				if (!exists)
+				{
+					if (nItems >= caps.maxItems)
+					{
+						fail(EXT_OVERFLOW, 5);
+						return;
+					}
+					itIdx = (int32_t)nItems++;
+					slotItem[slot] = itIdx;
+					itemNode[itIdx] = node;
+					it.node = node;
+					it.minScore = SMAX;
+					it.sVP = it.sVN = it.eVP = it.eVN = 0;
+					it.sEnd = it.eEnd = SMAX;
+					it.HP = it.HN = 0;
+				}
+				else
+				{
+					it = loadItem(itIdx);
+				}
+				Slice oldEnd = exists ? mkslice(it.eVP, it.eVN, it.eEnd) : mkslice(0, 0, SMAX);
+				int32_t pq = findInSlice(kept - 1, node);
+				Item prevIt = pq >= 0 ? loadItem(pq) : defaultPrev();
+				bool prevHasIn = false;
+				if (exists && (g.inOff[node + 1] - g.inOff[node]) == 1) prevHasIn = findInSlice(kept - 1, g.inAdj[g.inOff[node]]) >= 0;
+				NodeResult nc = calcNode<false>(node, it, exists, eq, prevIt, pq >= 0, slotHead[slot], mkslice(0, 0, 0), j, prevHasIn, nullptr);
+				nodeCalcs++;
+				if (status != EXT_OK) return;
+				// pop: the node leaves the queue and its queued edges are dropped
+				nPend--;
+				pendKey[pi] = pendKey[nPend];
+				pendSeq[pi] = pendSeq[nPend];
+				pendSlot[pi] = pendSlot[nPend];
+				slotActive[slot] = 0;
+				slotHead[slot] = NIL;
+				slotTail[slot] = NIL;
+				if (nc.maxX == SMIN || nc.minScore > j + b + 128)
+				{
+					fail(EXT_ASSERT);
+					return;
+				}
+				curMin = imin(curMin, nc.minScore);
+				if (nc.minScore < it.minScore) it.minScore = nc.minScore;
+				storeItem(itIdx, it);
+				Slice newEnd = mkslice(it.eVP, it.eVN, it.eEnd);
+				if (!same(newEnd, oldEnd))
+				{
+					int32_t m = changedMinScore(newEnd, oldEnd);
+					if (m == SMAX)
+					{
+						fail(EXT_ASSERT);
+						return;
+					}
+					if (m <= curMin + b)
+					{
+						for (uint32_t e = g.outOff[node]; e < g.outOff[node + 1]; e++)
+						{
+							uint32_t nb = g.outAdj[e];
+							queueInsert(g.comp[nb], m, nb, newEnd, false);
+						}
+						if (status != EXT_OK) return;
+					}
+				}
+				if (nc.minScore < rMin)
+				{
+					rMin = nc.minScore;
+					rMinNode = (int32_t)node;
+					rMinOff = nc.minScoreNodeOffset;
+				}
+				if (nc.maxX > rMaxX)
+				{
+					rMaxX = nc.maxX;
+					rMaxXNode = (int32_t)node;
+				}
+				cells += nc.cells;
+				if (cells > p.maxCellsPerSlice) break;
+			}
+			// fillDPSlice / getXdropSlices checks (src/GraphAlignerBitvectorBanded.h:543-549,633-638)
+			double jd = (double)j + 64.0;
+			if (rMin > j + 64 || rMaxX == SMIN || (double)rMaxX < -jd * p.errorCost || (double)rMaxX > jd * p.errorCost || rMin < P.minScore || rMinNode < 0)
+			{
+				fail(EXT_ASSERT);
+				return;
+			}
+			if (rMaxX > bestX) bestX = rMaxX;
+			SliceHdr hn;
+			hn.minScore = rMin;
+			hn.minScoreNode = rMinNode;
+			hn.minScoreNodeOffset = rMinOff;
+			hn.maxExactEndposScore = rMaxX;
+			hn.maxExactEndposNode = rMaxXNode;
+			hn.bandwidth = b;
+			hn.scoresNotValid = cells >= p.maxCellsPerSlice ? 1 : 0;
+			hn.itemStart = sliceItemStart;
+			hn.itemCount = nItems - sliceItemStart;
+			hn.pad = 0;
+			hn.cells = cells;
+			if ((double)rMaxX < (double)bestX - (double)p.xdrop)
+			{
+				nItems = sliceItemStart;
+				break;
+			}
+			hdr[kept] = hn;
+			kept++;
+			if (hn.itemCount > maxBand) maxBand = hn.itemCount;
+		}
+	}
+
+	// --- traceback ---
+	GA_HD void pushTrace(const Pos& pos, bool nodeSwitch)
+	{
+		if (ntrace >= traceCap)
+		{
+			fail(EXT_OVERFLOW, 6);
+			return;
+		}
+		trace[ntrace++] = packTrace(pos.node, pos.seq, pos.off, nodeSwitch ? 1 : 0);
+		back = pos;
+	}
+	GA_HD Pos tracePos(uint32_t i) const
+	{
+		uint64_t v = trace[i];
+		uint32_t hi = (uint32_t)(v >> 32);
+		return mkpos((uint32_t)v, (hi >> 1) & 63, (int32_t)(hi >> 7) - 1);
+	}
+	// src/GraphAlignerBitvectorCommon.h:760-768
+	GA_HD void checkCircularity()
+	{
+		for (uint32_t i = ntrace - 1; i-- > 0;)
+		{
+			Pos q = tracePos(i);
+			if (samePos(q, back))
+			{
+				fail(EXT_ASSERT);
+				return;
+			}
+			if (q.seq != back.seq) return;
+		}
+	}
+
+	struct Pick
+	{
+		Pos pos;
+		bool sw;
+	};
+	GA_HD static Pick mkpick(const Pos& p, bool sw)
+	{
+		Pick r;
+		r.pos = p;
+		r.sw = sw;
+		return r;
+	}
+	GA_HD int32_t nodeLast(uint32_t n) const { return (int32_t)nodeLen(n) - 1; }
+
+	// pickBacktraceCorner (src/GraphAlignerBitvectorCommon.h:936-1054)
+	GA_HD Pick corner(uint32_t cur, uint32_t node, int32_t j, int32_t quit, bool notValid, int32_t prevQuit, bool prevNotValid)
+	{
+		uint32_t prv = cur - 1;
+		int32_t ci = findInSlice(cur, node);
+		Item cn = loadItem(ci);
+		int32_t scoreHere = valueAt(mkslice(cn.sVP, cn.sVN, cn.sEnd), 0);
+		int32_t pn = findInSlice(prv, node);
+		uint32_t inB = g.inOff[node], inE = g.inOff[node + 1];
+		if (notValid || scoreHere > quit)
+		{
+			int32_t smallest = scoreHere + 1;
+			Pos sp = mkpos(NIL, NIL, SMIN);
+			bool sw = false;
+			if (pn >= 0)
+			{
+				smallest = loadItem(pn).sEnd;
+				sp = mkpos(node, 0, j - 1);
+				sw = false;
+			}
+			for (uint32_t e = inB; e < inE; e++)
+			{
+				uint32_t nb = g.inAdj[e];
+				int32_t q = findInSlice(prv, nb);
+				if (q >= 0)
+				{
+					int32_t v = loadItem(q).eEnd;
+					if (v <= smallest)
+					{
+						smallest = v;
+						sp = mkpos(nb, nodeLast(nb), j - 1);
+						sw = true;
+					}
+				}
+				int32_t c = findInSlice(cur, nb);
+				if (c >= 0 && nb != node)
+				{
+					Item ci2 = loadItem(c);
+					int32_t v = valueAt(mkslice(ci2.eVP, ci2.eVN, ci2.eEnd), 0);
+					if (v < smallest)
+					{
+						smallest = v;
+						sp = mkpos(nb, nodeLast(nb), j);
+						sw = true;
+					}
+				}
+			}
+			if (sp.node == NIL) fail(EXT_ASSERT);
+			return mkpick(sp, sw);
+		}
+		bool eq = charMatch(j, node, 0);
+		Pos bestInvalid = mkpos(NIL, NIL, SMIN);
+		int32_t bestInvalidScore = scoreHere + 1;
+		bool bestInvalidSw = false;
+		if (pn >= 0)
+		{
+			int32_t ps = loadItem(pn).sEnd;
+			if (ps < scoreHere - 1) fail(EXT_ASSERT);
+			if (ps == scoreHere - 1) return mkpick(mkpos(node, 0, j - 1), false);
+			if (prevNotValid || ps > prevQuit)
+			{
+				if (ps < bestInvalidScore)
+				{
+					bestInvalidScore = ps;
+					bestInvalid = mkpos(node, 0, j - 1);
+					bestInvalidSw = false;
+				}
+			}
+		}
+		for (uint32_t e = inB; e < inE; e++)
+		{
+			uint32_t nb = g.inAdj[e];
+			int32_t c = findInSlice(cur, nb);
+			if (c >= 0)
+			{
+				Item ci2 = loadItem(c);
+				int32_t v = valueAt(mkslice(ci2.eVP, ci2.eVN, ci2.eEnd), 0);
+				if (v < scoreHere - 1) fail(EXT_ASSERT);
+				if (v == scoreHere - 1) return mkpick(mkpos(nb, nodeLast(nb), j), true);
+			}
+			int32_t q = findInSlice(prv, nb);
+			if (q >= 0)
+			{
+				int32_t cornerScore = loadItem(q).eEnd;
+				if (prevNotValid || cornerScore > prevQuit)
+				{
+					if (cornerScore < bestInvalidScore)
+					{
+						bestInvalidScore = cornerScore;
+						bestInvalid = mkpos(nb, nodeLast(nb), j - 1);
+						bestInvalidSw = true;
+					}
+				}
+				else
+				{
+					if (cornerScore < scoreHere - (eq ? 0 : 1)) fail(EXT_ASSERT);
+					if (cornerScore == scoreHere - (eq ? 0 : 1)) return mkpick(mkpos(nb, nodeLast(nb), j - 1), true);
+				}
+			}
+		}
+		if (bestInvalidScore < scoreHere + 1) return mkpick(bestInvalid, bestInvalidSw);
+		fail(EXT_ASSERT);
+		return mkpick(mkpos(0, 0, 0), false);
+	}
+
+	// pickBacktraceHorizontalCrossing (src/GraphAlignerBitvectorCommon.h:817-884)
+	GA_HD void horizontal(uint32_t cur, uint32_t node, Pos pos, int32_t j, int32_t quit, bool notValid, int32_t prevQuit, bool prevNotValid, Pick& first, Pick& second)
+	{
+		Item cn = loadItem(findInSlice(cur, node));
+		Slice startSlice = mkslice(cn.sVP, cn.sVN, cn.sEnd);
+		while ((pos.seq & 63) != 0 && ((startSlice.VP >> (pos.seq & 63)) & 1)) pos.seq--;
+		int32_t offset = pos.seq & 63;
+		first = mkpick(pos, false);
+		if (offset == 0)
+		{
+			second = corner(cur, node, j, quit, notValid, prevQuit, prevNotValid);
+			return;
+		}
+		bool eq = charMatch(pos.seq, pos.node, pos.off);
+		int32_t scoreHere = valueAt(startSlice, offset);
+		uint32_t inB = g.inOff[node], inE = g.inOff[node + 1];
+		if (notValid || scoreHere > quit)
+		{
+			int32_t smallest = valueAt(startSlice, offset - 1);
+			Pos sp = mkpos(node, 0, pos.seq - 1);
+			bool sw = false;
+			for (uint32_t e = inB; e < inE; e++)
+			{
+				uint32_t nb = g.inAdj[e];
+				int32_t c = findInSlice(cur, nb);
+				if (c < 0) continue;
+				Item ni = loadItem(c);
+				Slice ns = mkslice(ni.eVP, ni.eVN, ni.eEnd);
+				if (valueAt(ns, offset - 1) <= smallest)
+				{
+					smallest = valueAt(ns, offset - 1);
+					sp = mkpos(nb, nodeLast(nb), pos.seq - 1);
+					sw = true;
+				}
+				if (valueAt(ns, offset) < smallest && nb != node)
+				{
+					smallest = valueAt(ns, offset);
+					sp = mkpos(nb, nodeLast(nb), pos.seq);
+					sw = true;
+				}
+			}
+			if (samePos(sp, pos)) fail(EXT_ASSERT);
+			second = mkpick(sp, sw);
+			return;
+		}
+		for (uint32_t e = inB; e < inE; e++)
+		{
+			uint32_t nb = g.inAdj[e];
+			int32_t c = findInSlice(cur, nb);
+			if (c < 0) continue;
+			Item ni = loadItem(c);
+			Slice ns = mkslice(ni.eVP, ni.eVN, ni.eEnd);
+			if (valueAt(ns, offset) < scoreHere - 1 || valueAt(ns, offset - 1) < scoreHere - (eq ? 0 : 1)) fail(EXT_ASSERT);
+			if (valueAt(ns, offset) == scoreHere - 1)
+			{
+				second = mkpick(mkpos(nb, nodeLast(nb), pos.seq), true);
+				return;
+			}
+			if (valueAt(ns, offset - 1) == scoreHere - (eq ? 0 : 1))
+			{
+				second = mkpick(mkpos(nb, nodeLast(nb), pos.seq - 1), true);
+				return;
+			}
+		}
+		fail(EXT_ASSERT);
+		second = mkpick(pos, false);
+	}
+
+	// pickBacktraceVerticalCrossing (src/GraphAlignerBitvectorCommon.h:886-934)
+	GA_HD void vertical(uint32_t cur, uint32_t node, Pos pos, int32_t j, int32_t quit, bool notValid, int32_t prevQuit, bool prevNotValid, const ColumnStore& cols, Pick& first, Pick& second)
+	{
+		while (pos.off > 0 && valueAt(colGet(cols, pos.off - 1), 0) == valueAt(colGet(cols, pos.off), 0) - 1) pos.off--;
+		first = mkpick(pos, false);
+		if (pos.off == 0)
+		{
+			second = corner(cur, node, j, quit, notValid, prevQuit, prevNotValid);
+			return;
+		}
+		int32_t pn = findInSlice(cur - 1, node);
+		if (pn < 0)
+		{
+			fail(EXT_ASSERT);
+			return;
+		}
+		bool eq = charMatch(pos.seq, pos.node, pos.off);
+		Item prevNode = loadItem(pn);
+		int32_t scoreHere = valueAt(colGet(cols, pos.off), 0);
+		int32_t scoreDiagonal = prevNode.sEnd;
+		for (uint32_t i = 1; i + 1 <= pos.off; i++) scoreDiagonal += (int32_t)((prevNode.HP >> i) & 1) - (int32_t)((prevNode.HN >> i) & 1);
+		int32_t scoreUp = scoreDiagonal + (int32_t)((prevNode.HP >> pos.off) & 1) - (int32_t)((prevNode.HN >> pos.off) & 1);
+		if (prevNotValid || notValid || scoreHere > quit || scoreDiagonal > prevQuit || scoreUp > prevQuit)
+		{
+			if (scoreDiagonal < scoreUp) second = mkpick(mkpos(pos.node, pos.off - 1, pos.seq - 1), false);
+			else second = mkpick(mkpos(pos.node, pos.off, pos.seq - 1), false);
+			return;
+		}
+		if (scoreUp < scoreHere - 1 || scoreDiagonal < scoreHere - (eq ? 0 : 1)) fail(EXT_ASSERT);
+		if (scoreUp == scoreHere - 1)
+		{
+			second = mkpick(mkpos(pos.node, pos.off, pos.seq - 1), false);
+			return;
+		}
+		if (scoreDiagonal == scoreHere - (eq ? 0 : 1))
+		{
+			second = mkpick(mkpos(pos.node, pos.off - 1, pos.seq - 1), false);
+			return;
+		}
+		fail(EXT_ASSERT);
+		second = mkpick(pos, false);
+	}
+
+	// recalcNodeWordslice (src/GraphAlignerBitvectorCommon.h:1078-1102)
+	GA_HD void recalc(uint32_t sliceIdx, uint32_t node, const Eq4& eq, ColumnStore& cols)
+	{
+		int32_t ci = findInSlice(sliceIdx, node);
+		if (ci < 0)
+		{
+			fail(EXT_ASSERT);
+			return;
+		}
+		Item stored = loadItem(ci);
+		int32_t pq = findInSlice(sliceIdx - 1, node);
+		Item prevIt = pq >= 0 ? loadItem(pq) : defaultPrev();
+		Item copy = stored;
+		calcNode<true>(node, copy, true, eq, prevIt, pq >= 0, NIL, mkslice(stored.sVP, stored.sVN, stored.sEnd), (int32_t)((sliceIdx - 1) * 64), false, &cols);
+		if (copy.sVP != stored.sVP || copy.sVN != stored.sVN || copy.sEnd != stored.sEnd || copy.eVP != stored.eVP || copy.eVN != stored.eVN || copy.eEnd != stored.eEnd) fail(EXT_ASSERT);
+	}
+
+	// getReverseTraceFromTableExactEndPos + getReverseTraceFromTable
+	// (src/GraphAlignerBitvectorCommon.h:496-758), writing trace items end first.
+	GA_HD int32_t traceback()
+	{
+		uint32_t bestIndex = 1;
+		for (uint32_t i = 1; i < kept; i++)
+			if (hdr[i].maxExactEndposScore > hdr[bestIndex].maxExactEndposScore) bestIndex = i;
+		uint32_t node = (uint32_t)hdr[bestIndex].maxExactEndposNode;
+		int32_t score = hdr[bestIndex].maxExactEndposScore;
+		int32_t j = (int32_t)((bestIndex - 1) * 64);
+		Eq4 eq;
+		lpEq(seq, len, (uint32_t)j, eq);
+		ColumnStore cols;
+		recalc(bestIndex, node, eq, cols);
+		if (status != EXT_OK) return 0;
+		int32_t cells = imin(64, (int32_t)len - j);
+		int32_t nodeOffset = -1, bvOffset = -1;
+		uint32_t L = nodeLen(node);
+		for (uint32_t i = 0; i < L; i++)
+		{
+			Slice c = colGet(cols, i);
+			int32_t maxScore = lpMaxX(c, j, p.errorCost, maxXCandidates(c, cells, ONES));
+			if (maxScore > score)
+			{
+				fail(EXT_ASSERT);
+				return 0;
+			}
+			if (maxScore == score)
+			{
+				for (int32_t off = 63; off >= 0; off--)
+				{
+					if (j + off >= (int32_t)len) continue;
+					int32_t xh = (j + off + 1) * 100 - valueAt(c, off) * p.errorCost;
+					if (xh > score)
+					{
+						fail(EXT_ASSERT);
+						return 0;
+					}
+					if (xh == score && (nodeOffset < 0 || off > bvOffset))
+					{
+						nodeOffset = (int32_t)i;
+						bvOffset = off;
+					}
+				}
+			}
+		}
+		if (nodeOffset < 0)
+		{
+			fail(EXT_ASSERT);
+			return 0;
+		}
+		int32_t startScore = valueAt(colGet(cols, nodeOffset), bvOffset);
+		Pos startPos = mkpos(node, (uint32_t)nodeOffset, j + bvOffset);
+
+		// getReverseTraceFromTable
+		int32_t resultScore = startScore;
+		ntrace = 0;
+		pushTrace(startPos, false);
+		uint32_t currentSlice = (uint32_t)(startPos.seq / 64 + 1);
+		uint32_t currentNode = NIL;
+		lpEq(seq, len, (currentSlice - 1) * 64, eq);
+		int32_t lastScore = SMAX - 1;
+		while (back.seq != -1 && status == EXT_OK)
+		{
+			uint32_t newSlice = (uint32_t)(back.seq / 64 + 1);
+			uint32_t newNode = back.node;
+			if (newSlice != currentSlice || newNode != currentNode)
+			{
+				if (newSlice != currentSlice) lpEq(seq, len, (newSlice - 1) * 64, eq);
+				currentSlice = newSlice;
+				currentNode = newNode;
+				if (currentSlice == 0)
+				{
+					fail(EXT_ASSERT);
+					break;
+				}
+				recalc(currentSlice, currentNode, eq, cols);
+				if (status != EXT_OK) break;
+			}
+			const SliceHdr& ch = hdr[currentSlice];
+			const SliceHdr& ph = hdr[currentSlice - 1];
+			int32_t cj = (int32_t)((currentSlice - 1) * 64);
+			int32_t quit = ch.minScore + ch.bandwidth;
+			int32_t prevQuit = ph.minScore + ph.bandwidth;
+			bool nv = ch.scoresNotValid != 0, pnv = ph.scoresNotValid != 0;
+			Pos bk = back;
+			lastScore = valueAt(colGet(cols, bk.off), bk.seq & 63);
+			if ((bk.seq & 63) == 0 && bk.off == 0)
+			{
+				Pick bt = corner(currentSlice, currentNode, cj, quit, nv, prevQuit, pnv);
+				if (status != EXT_OK) break;
+				pushTrace(bt.pos, bt.sw);
+				checkCircularity();
+				continue;
+			}
+			if ((bk.seq & 63) == 0)
+			{
+				if (findInSlice(currentSlice - 1, currentNode) < 0)
+				{
+					for (uint32_t i = bk.off - 1; i > 0; i--) pushTrace(mkpos(currentNode, i, back.seq), false);
+					pushTrace(mkpos(currentNode, 0, back.seq), false);
+					continue;
+				}
+				Pick first, second;
+				vertical(currentSlice, currentNode, bk, cj, quit, nv, prevQuit, pnv, cols, first, second);
+				if (status != EXT_OK) break;
+				if (first.pos.node != bk.node || first.pos.seq != bk.seq || first.pos.off > bk.off)
+				{
+					fail(EXT_ASSERT);
+					break;
+				}
+				if (first.pos.off != bk.off)
+					for (uint32_t o = bk.off - 1; o != first.pos.off; o--) pushTrace(mkpos(first.pos.node, o, first.pos.seq), false);
+				if (!samePos(first.pos, back)) pushTrace(first.pos, first.sw);
+				if (samePos(second.pos, back))
+				{
+					fail(EXT_ASSERT);
+					break;
+				}
+				pushTrace(second.pos, second.sw);
+				continue;
+			}
+			if (bk.off == 0)
+			{
+				Pick first, second;
+				horizontal(currentSlice, currentNode, bk, cj, quit, nv, prevQuit, pnv, first, second);
+				if (status != EXT_OK) break;
+				if (first.pos.node != bk.node || first.pos.off != bk.off || first.pos.seq > bk.seq)
+				{
+					fail(EXT_ASSERT);
+					break;
+				}
+				if (first.pos.seq != bk.seq)
+					for (int32_t sp = bk.seq - 1; sp != first.pos.seq; sp--) pushTrace(mkpos(first.pos.node, first.pos.off, sp), false);
+				if (!samePos(first.pos, back)) pushTrace(first.pos, first.sw);
+				if (samePos(second.pos, back))
+				{
+					fail(EXT_ASSERT);
+					break;
+				}
+				pushTrace(second.pos, second.sw);
+				checkCircularity();
+				continue;
+			}
+			// pickBacktraceInside (src/GraphAlignerBitvectorCommon.h:770-815)
+			uint32_t hori = bk.off;
+			int32_t vert = bk.seq - cj;
+			while (hori > 0 && vert > 0 && status == EXT_OK)
+			{
+				Slice ch0 = colGet(cols, hori);
+				Slice ch1 = colGet(cols, hori - 1);
+				int32_t here = valueAt(ch0, vert);
+				int32_t vertical = valueAt(ch0, vert - 1);
+				int32_t horizontal = valueAt(ch1, vert);
+				int32_t diagonal = valueAt(ch1, vert - 1);
+				bool eqc = charMatch(vert + cj, bk.node, hori);
+				if (vertical < here - 1 || horizontal < here - 1 || diagonal < here - (eqc ? 0 : 1))
+				{
+					fail(EXT_ASSERT);
+					break;
+				}
+				if (vertical == here - 1)
+				{
+					vert--;
+					pushTrace(mkpos(bk.node, hori, vert + cj), false);
+					continue;
+				}
+				if (diagonal == here - (eqc ? 0 : 1))
+				{
+					hori--;
+					vert--;
+					pushTrace(mkpos(bk.node, hori, vert + cj), false);
+					continue;
+				}
+				if (horizontal == here - 1)
+				{
+					hori--;
+					pushTrace(mkpos(bk.node, hori, vert + cj), false);
+					continue;
+				}
+				fail(EXT_ASSERT);
+				break;
+			}
+		}
+		if (status != EXT_OK) return 0;
+		if (resultScore < lastScore)
+		{
+			fail(EXT_ASSERT);
+			return 0;
+		}
+		resultScore -= lastScore;
+		// final walk inside the initial slice (src/GraphAlignerBitvectorCommon.h:721-756)
+		{
+			int32_t i0 = findInSlice(0, back.node);
+			if (i0 < 0)
+			{
+				fail(EXT_ASSERT);
+				return 0;
+			}
+			Item n0 = loadItem(i0);
+			uint32_t L0 = nodeLen(back.node);
+			// before-slice score at column c = sEnd + prefix(HP) - prefix(HN) over bits 1..c
+			auto beforeScore = [&](uint32_t c) -> int32_t {
+				uint64_t m = c >= 63 ? ONES : ((((uint64_t)1) << (c + 1)) - 1);
+				m &= ~(uint64_t)1;
+				return n0.sEnd + popc(n0.HP & m) - popc(n0.HN & m);
+			};
+			if (beforeScore(L0 - 1) != n0.eEnd)
+			{
+				fail(EXT_ASSERT);
+				return 0;
+			}
+			while (beforeScore(back.off) != 0 && back.off > 0 && beforeScore(back.off - 1) == beforeScore(back.off) - 1)
+			{
+				pushTrace(mkpos(back.node, back.off - 1, back.seq), false);
+				if (status != EXT_OK) return 0;
+			}
+			if (back.off == 0 && beforeScore(0) != 0)
+			{
+				for (uint32_t e = g.inOff[back.node]; e < g.inOff[back.node + 1]; e++)
+				{
+					uint32_t nb = g.inAdj[e];
+					int32_t q = findInSlice(0, nb);
+					if (q < 0) continue;
+					Item qi = loadItem(q);
+					if (before(mkslice(qi.eVP, qi.eVN, qi.eEnd)) == beforeScore(0) - 1)
+					{
+						pushTrace(mkpos(nb, nodeLast(nb), back.seq), true);
+						break;
+					}
+				}
+			}
+		}
+		return resultScore;
+	}
+
+	GA_HD void run(ExtResult& out)
+	{
+		status = EXT_OK;
+		overflowWhat = 0;
+		wordSteps = 0;
+		nodeCalcs = 0;
+		maxBand = 0;
+		ntrace = 0;
+		nItems = kept = nSlots = nEx = nPend = seqCounter = 0;
+		back = mkpos(NIL, 0, 0);
+		forward();
+		if (summaries)
+		{
+			for (uint32_t i = 1; i < kept; i++)
+			{
+				const SliceHdr& h = hdr[i];
+				SliceSummary ss;
+				ss.j = (int64_t)(i - 1) * 64;
+				ss.minScore = h.minScore;
+				ss.pad0 = 0;
+				ss.minScoreNode = h.minScoreNode;
+				ss.minScoreNodeOffset = h.minScoreNodeOffset;
+				ss.maxExactEndposScore = h.maxExactEndposScore;
+				ss.pad1 = 0;
+				ss.maxExactEndposNode = h.maxExactEndposNode;
+				ss.cellsProcessed = h.cells;
+				ss.numNodes = h.itemCount;
+				ss.itemsHash = lpDigest(items, h.itemStart, h.itemCount);
+				ss.scoresNotValid = h.scoresNotValid;
+				ss.pad2 = 0;
+				summaries[i - 1] = ss;
+			}
+		}
+		int32_t score = SMAX;
+		if (status == EXT_OK)
+		{
+			if (kept <= 1) status = EXT_FAILED;
+			else if (hdr[kept - 1].minScore < 0 || hdr[kept - 1].minScore > (int32_t)len + 128) fail(EXT_ASSERT);
+			else score = traceback();
+		}
+		out.status = status;
+		out.score = status == EXT_OK ? score : SMAX;
+		out.ntrace = status == EXT_OK ? ntrace : 0;
+		out.nslices = kept > 0 ? kept - 1 : 0;
+		out.wordSteps = wordSteps;
+		out.nodeCalcs = nodeCalcs;
+		out.maxBand = maxBand;
+		out.overflowWhat = overflowWhat;
+	}
+};
+
+GA_HHD uint64_t alignUp(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
+
+// Bytes of one wave's workspace for the given capacities.
+GA_HHD uint64_t arenaBytes(const ArenaCaps& c)
+{
+	uint64_t b = 0;
+	b += alignUp((uint64_t)c.maxSlices * sizeof(SliceHdr));
+	b += alignUp((uint64_t)c.maxItems * sizeof(Item));
+	b += alignUp((uint64_t)c.maxItems * 4);
+	b += 5 * alignUp((uint64_t)c.maxSlots * 4);
+	b += 2 * alignUp((uint64_t)c.maxExtras * 8) + 2 * alignUp((uint64_t)c.maxExtras * 4);
+	b += alignUp((uint64_t)c.maxPending * 8) + 2 * alignUp((uint64_t)c.maxPending * 4);
+	return b;
+}
+
+GA_HD void carveArena(Extension& e, unsigned char* base, const ArenaCaps& c)
+{
+	e.caps = c;
+	uint64_t o = 0;
+	auto take = [&](uint64_t bytes) {
+		unsigned char* p = base + o;
+		o += alignUp(bytes);
+		return p;
+	};
+	e.hdr = (SliceHdr*)take((uint64_t)c.maxSlices * sizeof(SliceHdr));
+	e.items = (Item*)take((uint64_t)c.maxItems * sizeof(Item));
+	e.itemNode = (uint32_t*)take((uint64_t)c.maxItems * 4);
+	e.slotNode = (uint32_t*)take((uint64_t)c.maxSlots * 4);
+	e.slotItem = (int32_t*)take((uint64_t)c.maxSlots * 4);
+	e.slotHead = (uint32_t*)take((uint64_t)c.maxSlots * 4);
+	e.slotTail = (uint32_t*)take((uint64_t)c.maxSlots * 4);
+	e.slotActive = (uint32_t*)take((uint64_t)c.maxSlots * 4);
+	e.exVP = (uint64_t*)take((uint64_t)c.maxExtras * 8);
+	e.exVN = (uint64_t*)take((uint64_t)c.maxExtras * 8);
+	e.exEnd = (int32_t*)take((uint64_t)c.maxExtras * 4);
+	e.exNext = (uint32_t*)take((uint64_t)c.maxExtras * 4);
+	e.pendKey = (uint64_t*)take((uint64_t)c.maxPending * 8);
+	e.pendSeq = (uint32_t*)take((uint64_t)c.maxPending * 4);
+	e.pendSlot = (uint32_t*)take((uint64_t)c.maxPending * 4);
+}
+
+// Runs one request on the calling wave.
+GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out)
+{
+	Extension e;
+	carveArena(e, arena, caps);
+	e.g = g;
+	e.p = p;
+	e.seq = seqbuf + r.seqOffset;
+	e.len = r.len;
+	e.startNode = r.startNode;
+	e.startOffset = r.startOffset;
+	e.trace = tracebuf + r.traceOffset;
+	e.traceCap = r.traceCap;
+	e.summaries = summaries;
+	e.run(out);
+}
+
+} // namespace gad

@@ -1,0 +1,49 @@
+// Host-side alignment graph: GFA -> bidirected-to-directed graph -> split nodes,
+// component order, linearizable flags and chains, laid out as flat arrays (CSR
+// adjacency, 2-bit packed node sequences) ready to be copied into HBM.
+//
+// Follows src/GfaGraph.cpp:170-396, src/BigraphToDigraph.cpp:101-132,215-267 and
+// src/AlignmentGraph.cpp (node numbering, IUPAC renumbering, Tarjan order,
+// linearizable detection, chains), keeping the same standard-container iteration
+// orders because node numbering depends on them.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace ga {
+
+struct GraphHost
+{
+	// split nodes (<= 64 bp), indexed 0..N-1; ambiguous (IUPAC) nodes are numbered last
+	std::vector<uint32_t> nodeLen;
+	std::vector<int32_t> nodeId;       // bigraph node id: 2*id (+) or 2*id+1 (-)
+	std::vector<uint32_t> nodeOff;     // offset of the split node inside its bigraph node
+	std::vector<uint8_t> reverse;
+	std::vector<uint8_t> linearizable;
+	std::vector<uint64_t> seq2;        // two 32-bp chunks per non-ambiguous node, A0 C1 G2 T3, LSB first
+	std::vector<uint64_t> amb4;        // A,C,G,T masks per ambiguous node (index node - firstAmbiguous)
+	std::vector<uint32_t> comp;        // strongly connected component number (topological)
+	std::vector<uint32_t> chain;
+	std::vector<uint64_t> chainPos;
+	std::vector<uint32_t> inOff, inAdj, outOff, outAdj;
+	uint32_t firstAmbiguous = 0;
+	uint64_t dbgOverlap = 0;
+	uint64_t bpSize = 0;
+	// bigraph nodes
+	std::unordered_map<int, std::vector<uint32_t>> lookup; // bigraph id -> split nodes in offset order
+	std::unordered_map<int, uint64_t> originalSize;
+	std::unordered_map<int, std::string> originalName;
+	std::vector<int> lookupOrder;                         // bigraph ids in the lookup map's iteration order
+
+	size_t numNodes() const { return nodeLen.size(); }
+	char nodeChar(uint32_t node, uint32_t pos) const;
+	uint32_t unitigNode(int id, uint64_t offset) const;
+	std::pair<int, uint64_t> reversePosition(int id, uint64_t offset) const;
+};
+
+// Throws std::runtime_error on malformed input.
+GraphHost loadGfaGraph(const std::string& path);
+
+} // namespace ga

@@ -1,0 +1,215 @@
+// Device runtime (see ga_runtime.hpp).
+#include "ga_runtime.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include "ga_kernels.hpp"
+
+namespace ga {
+
+DeviceBuffer::~DeviceBuffer()
+{
+	if (ptr) (void)hipFree(ptr);
+}
+
+void DeviceBuffer::reserve(size_t b)
+{
+	if (b <= bytes && ptr) return;
+	if (ptr) GA_HIP(hipFree(ptr));
+	ptr = nullptr;
+	bytes = 0;
+	size_t want = std::max<size_t>(b, 256);
+	GA_HIP(hipMalloc(&ptr, want));
+	bytes = want;
+}
+
+Context::Context(int device) : device_(device)
+{
+	int count = 0;
+	hipError_t e = hipGetDeviceCount(&count);
+	if (e != hipSuccess || count == 0) throw HipError("no HIP device available: the GPU path needs an MI355X (gfx950)");
+	if (device < 0 || device >= count) throw HipError("HIP device ordinal out of range");
+	GA_HIP(hipSetDevice(device));
+	hipDeviceProp_t prop;
+	GA_HIP(hipGetDeviceProperties(&prop, device));
+	cus_ = prop.multiProcessorCount;
+	GA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+	GA_HIP(hipEventCreate(&ev0_));
+	GA_HIP(hipEventCreate(&ev1_));
+}
+
+Context::~Context()
+{
+	(void)hipSetDevice(device_);
+	graphs_.clear();
+	if (ev0_) (void)hipEventDestroy(ev0_);
+	if (ev1_) (void)hipEventDestroy(ev1_);
+	if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <typename T>
+static void upload(DeviceBuffer& buf, const std::vector<T>& v, hipStream_t s)
+{
+	buf.reserve(std::max<size_t>(1, v.size()) * sizeof(T));
+	if (!v.empty()) GA_HIP(hipMemcpyAsync(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
+{
+	auto it = graphs_.find(&g);
+	if (it != graphs_.end()) return it->second->view;
+	GA_HIP(hipSetDevice(device_));
+	auto dg = std::make_unique<DeviceGraph>();
+	upload(dg->nodeLen, g.nodeLen, stream_);
+	upload(dg->seq2, g.seq2, stream_);
+	upload(dg->amb4, g.amb4, stream_);
+	upload(dg->comp, g.comp, stream_);
+	upload(dg->lin, g.linearizable, stream_);
+	upload(dg->inOff, g.inOff, stream_);
+	upload(dg->inAdj, g.inAdj, stream_);
+	upload(dg->outOff, g.outOff, stream_);
+	upload(dg->outAdj, g.outAdj, stream_);
+	GA_HIP(hipStreamSynchronize(stream_));
+	auto& v = dg->view;
+	v.nodeLen = dg->nodeLen.as<uint32_t>();
+	v.seq2 = dg->seq2.as<uint64_t>();
+	v.amb4 = dg->amb4.as<uint64_t>();
+	v.comp = dg->comp.as<uint32_t>();
+	v.linearizable = dg->lin.as<uint8_t>();
+	v.inOff = dg->inOff.as<uint32_t>();
+	v.inAdj = dg->inAdj.as<uint32_t>();
+	v.outOff = dg->outOff.as<uint32_t>();
+	v.outAdj = dg->outAdj.as<uint32_t>();
+	v.numNodes = g.numNodes();
+	v.firstAmbiguous = g.firstAmbiguous;
+	auto& ref = graphs_[&g];
+	ref = std::move(dg);
+	return ref->view;
+}
+
+static uint32_t traceCapFor(uint32_t len) { return 2 * len + 256; }
+
+void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out)
+{
+	const uint32_t n = which.size();
+	if (n == 0) return;
+	std::vector<gad::ExtRequest> reqs(n);
+	uint64_t seqBytes = 0, traceItems = 0, sumItems = 0;
+	for (uint32_t k = 0; k < n; k++)
+	{
+		const ExtJob& j = jobs[which[k]];
+		seqBytes += j.len;
+		traceItems += traceCapFor(j.len);
+		sumItems += j.len / 64 + 2;
+	}
+	std::vector<char> seqHost(seqBytes + 1);
+	std::vector<uint64_t> sumOff(n);
+	uint64_t so = 0, to = 0, sum = 0;
+	for (uint32_t k = 0; k < n; k++)
+	{
+		const ExtJob& j = jobs[which[k]];
+		std::memcpy(seqHost.data() + so, j.seq, j.len);
+		reqs[k].seqOffset = so;
+		reqs[k].traceOffset = to;
+		reqs[k].len = j.len;
+		reqs[k].startNode = j.startNode;
+		reqs[k].startOffset = j.startOffset;
+		reqs[k].traceCap = traceCapFor(j.len);
+		sumOff[k] = sum;
+		so += j.len;
+		to += traceCapFor(j.len);
+		sum += j.len / 64 + 2;
+	}
+	uint64_t stride = gad::waveArenaBytes(caps);
+	uint64_t waves = std::min<uint64_t>(n, (uint64_t)cus_ * wavesPerCU);
+	waves = std::max<uint64_t>(1, std::min<uint64_t>(waves, arenaBudgetBytes / stride));
+	arena_.reserve(waves * stride);
+	reqs_.reserve(n * sizeof(gad::ExtRequest));
+	seqs_.reserve(seqHost.size());
+	traces_.reserve(traceItems * sizeof(uint64_t));
+	results_.reserve(n * sizeof(gad::ExtResult));
+	counter_.reserve(sizeof(uint32_t));
+	if (wantSummaries)
+	{
+		sums_.reserve(sumItems * sizeof(gad::SliceSummary));
+		sumOffsets_.reserve(n * sizeof(uint64_t));
+		GA_HIP(hipMemcpyAsync(sumOffsets_.ptr, sumOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+	}
+	GA_HIP(hipMemcpyAsync(reqs_.ptr, reqs.data(), n * sizeof(gad::ExtRequest), hipMemcpyHostToDevice, stream_));
+	GA_HIP(hipMemcpyAsync(seqs_.ptr, seqHost.data(), seqHost.size(), hipMemcpyHostToDevice, stream_));
+	GA_HIP(hipMemsetAsync(counter_.ptr, 0, sizeof(uint32_t), stream_));
+	GA_HIP(hipEventRecord(ev0_, stream_));
+	GA_HIP(gad::launchExtend(stream_, (uint32_t)waves, dg, p, reqs_.as<gad::ExtRequest>(), n, seqs_.as<char>(), traces_.as<uint64_t>(), results_.as<gad::ExtResult>(), arena_.as<unsigned char>(), stride, caps, counter_.as<uint32_t>(), wantSummaries ? sums_.as<gad::SliceSummary>() : nullptr, wantSummaries ? sumOffsets_.as<uint64_t>() : nullptr));
+	GA_HIP(hipEventRecord(ev1_, stream_));
+	std::vector<gad::ExtResult> res(n);
+	GA_HIP(hipMemcpyAsync(res.data(), results_.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, stream_));
+	std::vector<uint64_t> tr(traceItems);
+	GA_HIP(hipMemcpyAsync(tr.data(), traces_.ptr, traceItems * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+	std::vector<gad::SliceSummary> sm;
+	if (wantSummaries)
+	{
+		sm.resize(sumItems);
+		GA_HIP(hipMemcpyAsync(sm.data(), sums_.ptr, sumItems * sizeof(gad::SliceSummary), hipMemcpyDeviceToHost, stream_));
+	}
+	GA_HIP(hipStreamSynchronize(stream_));
+	float ms = 0;
+	GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+	out.kernelMs += ms;
+	out.launches++;
+	for (uint32_t k = 0; k < n; k++)
+	{
+		uint32_t jid = which[k];
+		const gad::ExtResult& r = res[k];
+		out.results[jid] = r;
+		if (r.status == gad::EXT_OVERFLOW) continue;
+		out.wordSteps += r.wordSteps;
+		out.traceOffset[jid] = out.trace.size();
+		out.trace.insert(out.trace.end(), tr.begin() + reqs[k].traceOffset, tr.begin() + reqs[k].traceOffset + r.ntrace);
+		if (wantSummaries)
+		{
+			out.summaryOffset[jid] = out.summaries.size();
+			out.summaries.insert(out.summaries.end(), sm.begin() + sumOff[k], sm.begin() + sumOff[k] + r.nslices);
+		}
+	}
+}
+
+void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out)
+{
+	GA_HIP(hipSetDevice(device_));
+	const gad::DevGraph& dg = deviceGraph(g);
+	out.results.assign(jobs.size(), gad::ExtResult{});
+	out.traceOffset.assign(jobs.size(), 0);
+	out.summaryOffset.assign(jobs.size(), 0);
+	out.trace.clear();
+	out.summaries.clear();
+	std::vector<uint32_t> which(jobs.size());
+	uint32_t maxLen = 0;
+	for (uint32_t i = 0; i < jobs.size(); i++)
+	{
+		which[i] = i;
+		maxLen = std::max(maxLen, jobs[i].len);
+	}
+	// longest first, so long extensions do not form the tail of the launch
+	std::stable_sort(which.begin(), which.end(), [&](uint32_t a, uint32_t b) { return jobs[a].len > jobs[b].len; });
+	gad::ArenaCaps caps;
+	caps.maxSlices = maxLen / 64 + 3;
+	caps.maxItems = caps.maxSlices * 48 + 1024;
+	caps.maxSlots = 4096;
+	caps.maxExtras = 16384;
+	caps.maxPending = 4096;
+	caps.pad = 0;
+	for (int round = 0; round < 4 && !which.empty(); round++)
+	{
+		runOnce(dg, p, jobs, which, caps, wantSummaries, out);
+		std::vector<uint32_t> again;
+		for (uint32_t id : which)
+			if (out.results[id].status == gad::EXT_OVERFLOW) again.push_back(id);
+		which.swap(again);
+		caps.maxItems *= 8;
+		caps.maxSlots *= 8;
+		caps.maxExtras *= 8;
+		caps.maxPending *= 8;
+	}
+}
+
+} // namespace ga
