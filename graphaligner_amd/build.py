@@ -43,8 +43,12 @@ def build_test_infrastructure():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     tb = os.path.join(ROOT, "tests", "_build")
     os.makedirs(tb, exist_ok=True)
-    _run(["g++", "-O2", "-g", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-variable", "-o", os.path.join(tb, "libga_emu.so"),
-          os.path.join(ROOT, "tests", "emu", "ga_emu.cpp"), os.path.join(CSRC, "ga_graph.cpp")])
+    common = ["g++", "-O2", "-g", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-variable"]
+    srcs = [os.path.join(ROOT, "tests", "emu", "ga_emu.cpp"), os.path.join(CSRC, "ga_graph.cpp")]
+    # device extension code built for the host: lane helpers as 64-lane loops, and
+    # the one-extension-per-thread helpers the gfx950 kernel uses
+    _run(common + ["-o", os.path.join(tb, "libga_emu.so")] + srcs)
+    _run(common + ["-DGA_THREAD_PER_EXTENSION", "-DGA_THREAD_HOST_TEST", "-o", os.path.join(tb, "libga_thread.so")] + srcs)
 
 
 def build_all():

@@ -22,10 +22,25 @@
 #include <stdint.h>
 #include "ga_device_types.h"
 
-#ifdef GA_WAVE_EMULATION
+#if defined(GA_WAVE_EMULATION)
 #include <algorithm>
 #define GA_HD inline
 #define GA_HHD inline
+#elif defined(GA_SIMT_EMULATION)
+// host test build running the lane code on 64 threads; the harness defines the
+// wave intrinsics before including this header
+#define GA_HD inline
+#define GA_HHD inline
+#elif defined(GA_THREAD_PER_EXTENSION) && defined(GA_THREAD_HOST_TEST)
+// host test build of the one-extension-per-thread code
+#include <algorithm>
+#define GA_HD inline
+#define GA_HHD inline
+#elif defined(GA_THREAD_PER_EXTENSION)
+// gfx950 build with one extension per thread: no cross-lane operations
+#include <hip/hip_runtime.h>
+#define GA_HD __device__ __forceinline__
+#define GA_HHD __host__ __device__ inline
 #else
 #include <hip/hip_runtime.h>
 #define GA_HD __device__ __forceinline__
@@ -41,6 +56,7 @@ constexpr uint64_t LASTBIT = (uint64_t)1 << 63;
 constexpr uint32_t NIL = 0xffffffffu;
 constexpr uint32_t SKIPFIRST = 0x80000000u;
 constexpr uint32_t LISTEND = 0x7fffffffu; // end of a queued-edge list (low 31 bits of exNext)
+constexpr uint32_t kMaxPopsPerSlice = 1u << 20;
 
 struct Slice
 {
@@ -151,7 +167,236 @@ GA_HD uint32_t readBaseSet(unsigned char c)
 // ---------------------------------------------------------------------------
 // Lane-parallel helpers. Each takes uniform inputs and returns a uniform result.
 // ---------------------------------------------------------------------------
-#ifndef GA_WAVE_EMULATION
+#if defined(GA_THREAD_PER_EXTENSION)
+
+// differenceMasksBitTwiddle (src/WordSlice.h:566-664): rows where left < right and
+// rows where right < left, given scoreDifference = right.before - left.before
+GA_HD void diffMasks(uint64_t leftVP, uint64_t leftVN, uint64_t rightVP, uint64_t rightVN, int scoreDifference, uint64_t& leftSmaller, uint64_t& rightSmaller)
+{
+	leftSmaller = 0;
+	rightSmaller = 0;
+	uint64_t VPcommon = ~(leftVP & rightVP);
+	uint64_t VNcommon = ~(leftVN & rightVN);
+	leftVP &= VPcommon;
+	leftVN &= VNcommon;
+	rightVP &= VPcommon;
+	rightVN &= VNcommon;
+	uint64_t twosmaller = leftVN & rightVP;
+	uint64_t onesmaller = (rightVP & ~leftVN) | (leftVN & ~rightVP);
+	uint64_t onebigger = (leftVP & ~rightVN) | (rightVN & ~leftVP);
+	uint64_t twobigger = rightVN & leftVP;
+	onebigger |= twobigger;
+	onesmaller |= twosmaller;
+	if (scoreDifference > 0)
+	{
+		for (int i = 1; i < scoreDifference; i++)
+		{
+			uint64_t ls = onebigger & ~(onebigger - 1);
+			onebigger ^= (~twobigger & ls);
+			twobigger &= ~ls;
+			if (onebigger == 0)
+			{
+				leftSmaller = ONES;
+				rightSmaller = 0;
+				return;
+			}
+		}
+		uint64_t ls = onebigger & ~(onebigger - 1);
+		leftSmaller |= ls - 1;
+		onebigger ^= (~twobigger & ls);
+		twobigger &= ~ls;
+	}
+	else if (scoreDifference < 0)
+	{
+		for (int i = 1; i < -scoreDifference; i++)
+		{
+			uint64_t ls = onesmaller & ~(onesmaller - 1);
+			onesmaller ^= (~twosmaller & ls);
+			twosmaller &= ~ls;
+			if (onesmaller == 0)
+			{
+				leftSmaller = 0;
+				rightSmaller = ONES;
+				return;
+			}
+		}
+		uint64_t ls = onesmaller & ~(onesmaller - 1);
+		rightSmaller |= ls - 1;
+		onesmaller ^= (~twosmaller & ls);
+		twosmaller &= ~ls;
+	}
+	for (int i = 0; i < 64; i++)
+	{
+		if (onesmaller == 0)
+		{
+			if (onebigger == 0) break;
+			uint64_t ls = onebigger & ~(onebigger - 1);
+			rightSmaller |= -ls;
+			break;
+		}
+		if (onebigger == 0)
+		{
+			uint64_t ls = onesmaller & ~(onesmaller - 1);
+			leftSmaller |= -ls;
+			break;
+		}
+		uint64_t lb = onebigger & ~(onebigger - 1);
+		uint64_t lsm = onesmaller & ~(onesmaller - 1);
+		if (lb > lsm) leftSmaller |= lb - lsm;
+		else rightSmaller |= lsm - lb;
+		onebigger ^= (~twobigger & lb);
+		twobigger &= ~lb;
+		onesmaller ^= (~twosmaller & lsm);
+		twosmaller &= ~lsm;
+	}
+}
+
+// mergeTwoSlices (src/WordSlice.h:502-541)
+GA_HD Slice lpMerge(Slice left, Slice right)
+{
+	if (before(left) > before(right))
+	{
+		Slice t = left;
+		left = right;
+		right = t;
+	}
+	uint64_t leftSmaller, rightSmaller;
+	diffMasks(left.VP, left.VN, right.VP, right.VN, before(right) - before(left), leftSmaller, rightSmaller);
+	uint64_t mask = (rightSmaller | ((leftSmaller | rightSmaller) - (rightSmaller << 1))) & ~leftSmaller;
+	uint64_t leftReduction = leftSmaller & (rightSmaller << 1);
+	uint64_t rightReduction = rightSmaller & (leftSmaller << 1);
+	if ((rightSmaller & 1) && before(left) < before(right)) rightReduction |= 1;
+	left.VN &= ~leftReduction;
+	right.VN &= ~rightReduction;
+	Slice r;
+	r.VN = (left.VN & ~mask) | (right.VN & mask);
+	r.VP = (left.VP & ~mask) | (right.VP & mask);
+	r.end = imin(left.end, right.end);
+	return r;
+}
+GA_HD uint64_t lpLessMask(const Slice& a, const Slice& b)
+{
+	uint64_t ls, rs;
+	diffMasks(a.VP, a.VN, b.VP, b.VN, before(b) - before(a), ls, rs);
+	return ls;
+}
+// minimum over the rows of mask, walking from one selected row to the next
+GA_HD int32_t lpMinOver(const Slice& s, uint64_t mask)
+{
+	int32_t b = before(s);
+	int32_t r = SMAX;
+	while (mask != 0)
+	{
+		uint64_t cur = mask ^ (mask - 1);
+		r = imin(r, b + popc(s.VP & cur) - popc(s.VN & cur));
+		mask &= ~cur;
+	}
+	return r;
+}
+GA_HD int32_t lpMaxX(const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand)
+{
+	int32_t b = before(s);
+	int32_t r = SMIN;
+	while (cand != 0)
+	{
+		uint64_t cur = cand ^ (cand - 1);
+		int32_t cells = popc(cur);
+		r = imax(r, (seqOffset + cells) * 100 - (b + popc(s.VP & cur) - popc(s.VN & cur)) * cost);
+		cand &= ~cur;
+	}
+	return r;
+}
+struct LaneAcc
+{
+	int32_t v;
+};
+GA_HD void accInit(LaneAcc& a) { a.v = SMIN; }
+GA_HD void accAdd(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand) { a.v = imax(a.v, lpMaxX(s, seqOffset, cost, cand)); }
+GA_HD int32_t accReduce(const LaneAcc& a) { return a.v; }
+GA_HD int32_t lpFind(const uint32_t* arr, uint32_t n, uint32_t key)
+{
+	for (uint32_t i = 0; i < n; i++)
+		if (arr[i] == key) return (int32_t)i;
+	return -1;
+}
+GA_HD bool lpEq(const char* seq, uint32_t len, uint32_t j, Eq4& eq)
+{
+	eq.m[0] = eq.m[1] = eq.m[2] = eq.m[3] = 0;
+	bool invalid = false;
+	for (uint32_t l = 0; l < 64 && j + l < len; l++)
+	{
+		uint32_t set = readBaseSet((unsigned char)seq[j + l]);
+		if (set == 16) invalid = true;
+		uint64_t bit = (uint64_t)1 << l;
+		if (set & 1) eq.m[0] |= bit;
+		if (set & 2) eq.m[1] |= bit;
+		if (set & 4) eq.m[2] |= bit;
+		if (set & 8) eq.m[3] |= bit;
+	}
+	return invalid;
+}
+GA_HD uint32_t lpArgMin(const uint64_t* key, const uint32_t* seq, uint32_t n)
+{
+	uint32_t bi = 0;
+	uint64_t bk = key[0];
+	uint32_t bs = seq[0];
+	for (uint32_t i = 1; i < n; i++)
+	{
+		uint64_t k = key[i];
+		uint32_t s = seq[i];
+		if (k < bk || (k == bk && s < bs))
+		{
+			bk = k;
+			bs = s;
+			bi = i;
+		}
+	}
+	return bi;
+}
+GA_HD uint64_t mixHash(uint64_t h, uint64_t v) { return h ^ (v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)); }
+GA_HD uint64_t itemHash(const Item& it)
+{
+	uint64_t h = 0x12345;
+	h = mixHash(h, it.node);
+	h = mixHash(h, it.sVP);
+	h = mixHash(h, it.sVN);
+	h = mixHash(h, (uint32_t)it.sEnd);
+	h = mixHash(h, it.eVP);
+	h = mixHash(h, it.eVN);
+	h = mixHash(h, (uint32_t)it.eEnd);
+	h = mixHash(h, it.HP);
+	h = mixHash(h, it.HN);
+	h = mixHash(h, (uint32_t)it.minScore);
+	return h;
+}
+GA_HD uint64_t lpDigest(const Item* items, uint32_t start, uint32_t count)
+{
+	uint64_t h = 0;
+	for (uint32_t i = 0; i < count; i++) h += itemHash(items[start + i]);
+	return h;
+}
+// a node's column slices, in the thread's workspace
+struct ColumnStore
+{
+	uint64_t* vp;
+	uint64_t* vn;
+	int32_t* end;
+};
+GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
+{
+	c.vp[col] = s.VP;
+	c.vn[col] = s.VN;
+	c.end[col] = s.end;
+}
+GA_HD Slice colGet(const ColumnStore& c, int col) { return mkslice(c.vp[col], c.vn[col], c.end[col]); }
+GA_HD void bindColumns(ColumnStore& c, uint64_t* vp, uint64_t* vn, int32_t* end)
+{
+	c.vp = vp;
+	c.vn = vn;
+	c.end = end;
+}
+
+#elif !defined(GA_WAVE_EMULATION)
 
 GA_HD int laneId() { return __lane_id(); }
 GA_HD uint64_t prefixMask(int lane) { return lane == 63 ? ONES : (((uint64_t)1 << (lane + 1)) - 1); }
@@ -339,6 +584,7 @@ GA_HD Slice colGet(const ColumnStore& c, int col)
 	s.end = __shfl(c.end, col);
 	return s;
 }
+GA_HD void bindColumns(ColumnStore&, uint64_t*, uint64_t*, int32_t*) {}
 
 #else // GA_WAVE_EMULATION: the same helpers as loops over 64 lanes
 
@@ -458,6 +704,7 @@ struct ColumnStore
 };
 inline void colSet(ColumnStore& c, int col, const Slice& s) { c.c[col] = s; }
 inline Slice colGet(const ColumnStore& c, int col) { return c.c[col]; }
+inline void bindColumns(ColumnStore&, uint64_t*, uint64_t*, int32_t*) {}
 
 #endif
 
@@ -524,6 +771,9 @@ struct Extension
 	uint64_t* pendKey;
 	uint32_t* pendSeq;
 	uint32_t* pendSlot;
+	uint64_t* colVP;
+	uint64_t* colVN;
+	int32_t* colEnd;
 	// outputs
 	uint64_t* trace;
 	uint32_t traceCap;
@@ -938,8 +1188,15 @@ struct Extension
 			int32_t curMin = SMAX - b - 1;
 			int32_t rMin = SMAX - b - 1, rMinNode = -1, rMinOff = -1, rMaxX = SMIN, rMaxXNode = 0;
 			int64_t cells = 0;
+			uint32_t popsThisSlice = 0;
 			while (nPend > 0)
 			{
+				// hard bound on queue pops per slice: every loop of the kernel terminates
+				if (++popsThisSlice > kMaxPopsPerSlice)
+				{
+					fail(EXT_OVERFLOW, 7);
+					return;
+				}
 				uint32_t pi = lpArgMin(pendKey, pendSeq, nPend);
 				uint32_t slot = pendSlot[pi];
 				uint32_t node = slotNode[slot];
@@ -1346,6 +1603,7 @@ struct Extension
 		Eq4 eq;
 		lpEq(seq, len, (uint32_t)j, eq);
 		ColumnStore cols;
+		bindColumns(cols, colVP, colVN, colEnd);
 		recalc(bestIndex, node, eq, cols);
 		if (status != EXT_OK) return 0;
 		int32_t cells = imin(64, (int32_t)len - j);
@@ -1395,8 +1653,14 @@ struct Extension
 		uint32_t currentNode = NIL;
 		lpEq(seq, len, (currentSlice - 1) * 64, eq);
 		int32_t lastScore = SMAX - 1;
+		uint32_t steps = 0;
 		while (back.seq != -1 && status == EXT_OK)
 		{
+			if (++steps > traceCap)
+			{
+				fail(EXT_OVERFLOW, 8);
+				break;
+			}
 			uint32_t newSlice = (uint32_t)(back.seq / 64 + 1);
 			uint32_t newNode = back.node;
 			if (newSlice != currentSlice || newNode != currentNode)
@@ -1632,6 +1896,7 @@ GA_HHD uint64_t arenaBytes(const ArenaCaps& c)
 	b += 5 * alignUp((uint64_t)c.maxSlots * 4);
 	b += 2 * alignUp((uint64_t)c.maxExtras * 8) + 2 * alignUp((uint64_t)c.maxExtras * 4);
 	b += alignUp((uint64_t)c.maxPending * 8) + 2 * alignUp((uint64_t)c.maxPending * 4);
+	b += 2 * alignUp(64 * 8) + alignUp(64 * 4);
 	return b;
 }
 
@@ -1659,7 +1924,15 @@ GA_HD void carveArena(Extension& e, unsigned char* base, const ArenaCaps& c)
 	e.pendKey = (uint64_t*)take((uint64_t)c.maxPending * 8);
 	e.pendSeq = (uint32_t*)take((uint64_t)c.maxPending * 4);
 	e.pendSlot = (uint32_t*)take((uint64_t)c.maxPending * 4);
+	e.colVP = (uint64_t*)take(64 * 8);
+	e.colVN = (uint64_t*)take(64 * 8);
+	e.colEnd = (int32_t*)take(64 * 4);
 }
+
+// Per-request capacities that scale with the read length. caps.maxItems is the
+// band-item budget per slice.
+GA_HHD uint32_t requestSlices(uint32_t len) { return len / 64 + 3; }
+GA_HHD uint32_t requestItems(uint32_t len, const ArenaCaps& caps) { return requestSlices(len) * caps.maxItems + 1024; }
 
 // Runs one request on the calling wave.
 GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out)

@@ -91,55 +91,67 @@ static uint32_t traceCapFor(uint32_t len) { return 2 * len + 256; }
 
 void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out)
 {
+	// split into launches whose workspaces fit the memory budget
+	size_t begin = 0;
+	while (begin < which.size())
+	{
+		uint64_t bytes = 0;
+		size_t end = begin;
+		while (end < which.size())
+		{
+			uint64_t b = gad::requestArenaBytes(jobs[which[end]].len, caps);
+			if (end > begin && bytes + b > arenaBudgetBytes) break;
+			bytes += b;
+			end++;
+		}
+		std::vector<uint32_t> part(which.begin() + begin, which.begin() + end);
+		runLaunch(dg, p, jobs, part, caps, wantSummaries, out);
+		begin = end;
+	}
+}
+
+void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out)
+{
 	const uint32_t n = which.size();
 	if (n == 0) return;
 	std::vector<gad::ExtRequest> reqs(n);
-	uint64_t seqBytes = 0, traceItems = 0, sumItems = 0;
+	std::vector<uint64_t> arenaOff(n), sumOff(n);
+	uint64_t seqBytes = 0, traceItems = 0, sumItems = 0, arenaTotal = 0;
 	for (uint32_t k = 0; k < n; k++)
 	{
 		const ExtJob& j = jobs[which[k]];
-		seqBytes += j.len;
-		traceItems += traceCapFor(j.len);
-		sumItems += j.len / 64 + 2;
-	}
-	std::vector<char> seqHost(seqBytes + 1);
-	std::vector<uint64_t> sumOff(n);
-	uint64_t so = 0, to = 0, sum = 0;
-	for (uint32_t k = 0; k < n; k++)
-	{
-		const ExtJob& j = jobs[which[k]];
-		std::memcpy(seqHost.data() + so, j.seq, j.len);
-		reqs[k].seqOffset = so;
-		reqs[k].traceOffset = to;
+		reqs[k].seqOffset = seqBytes;
+		reqs[k].traceOffset = traceItems;
 		reqs[k].len = j.len;
 		reqs[k].startNode = j.startNode;
 		reqs[k].startOffset = j.startOffset;
 		reqs[k].traceCap = traceCapFor(j.len);
-		sumOff[k] = sum;
-		so += j.len;
-		to += traceCapFor(j.len);
-		sum += j.len / 64 + 2;
+		arenaOff[k] = arenaTotal;
+		sumOff[k] = sumItems;
+		seqBytes += j.len;
+		traceItems += traceCapFor(j.len);
+		sumItems += j.len / 64 + 2;
+		arenaTotal += gad::requestArenaBytes(j.len, caps);
 	}
-	uint64_t stride = gad::waveArenaBytes(caps);
-	uint64_t waves = std::min<uint64_t>(n, (uint64_t)cus_ * wavesPerCU);
-	waves = std::max<uint64_t>(1, std::min<uint64_t>(waves, arenaBudgetBytes / stride));
-	arena_.reserve(waves * stride);
+	std::vector<char> seqHost(seqBytes + 1);
+	for (uint32_t k = 0; k < n; k++) std::memcpy(seqHost.data() + reqs[k].seqOffset, jobs[which[k]].seq, reqs[k].len);
+	arena_.reserve(arenaTotal);
+	arenaOffsets_.reserve(n * sizeof(uint64_t));
 	reqs_.reserve(n * sizeof(gad::ExtRequest));
 	seqs_.reserve(seqHost.size());
 	traces_.reserve(traceItems * sizeof(uint64_t));
 	results_.reserve(n * sizeof(gad::ExtResult));
-	counter_.reserve(sizeof(uint32_t));
 	if (wantSummaries)
 	{
 		sums_.reserve(sumItems * sizeof(gad::SliceSummary));
 		sumOffsets_.reserve(n * sizeof(uint64_t));
 		GA_HIP(hipMemcpyAsync(sumOffsets_.ptr, sumOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
 	}
+	GA_HIP(hipMemcpyAsync(arenaOffsets_.ptr, arenaOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
 	GA_HIP(hipMemcpyAsync(reqs_.ptr, reqs.data(), n * sizeof(gad::ExtRequest), hipMemcpyHostToDevice, stream_));
 	GA_HIP(hipMemcpyAsync(seqs_.ptr, seqHost.data(), seqHost.size(), hipMemcpyHostToDevice, stream_));
-	GA_HIP(hipMemsetAsync(counter_.ptr, 0, sizeof(uint32_t), stream_));
 	GA_HIP(hipEventRecord(ev0_, stream_));
-	GA_HIP(gad::launchExtend(stream_, (uint32_t)waves, dg, p, reqs_.as<gad::ExtRequest>(), n, seqs_.as<char>(), traces_.as<uint64_t>(), results_.as<gad::ExtResult>(), arena_.as<unsigned char>(), stride, caps, counter_.as<uint32_t>(), wantSummaries ? sums_.as<gad::SliceSummary>() : nullptr, wantSummaries ? sumOffsets_.as<uint64_t>() : nullptr));
+	GA_HIP(gad::launchExtend(stream_, n, dg, p, reqs_.as<gad::ExtRequest>(), seqs_.as<char>(), traces_.as<uint64_t>(), results_.as<gad::ExtResult>(), arena_.as<unsigned char>(), arenaOffsets_.as<uint64_t>(), caps, wantSummaries ? sums_.as<gad::SliceSummary>() : nullptr, wantSummaries ? sumOffsets_.as<uint64_t>() : nullptr));
 	GA_HIP(hipEventRecord(ev1_, stream_));
 	std::vector<gad::ExtResult> res(n);
 	GA_HIP(hipMemcpyAsync(res.data(), results_.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, stream_));
@@ -160,6 +172,8 @@ void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const st
 	{
 		uint32_t jid = which[k];
 		const gad::ExtResult& r = res[k];
+		if (r.status < gad::EXT_OK || r.status > gad::EXT_OVERFLOW || r.ntrace > reqs[k].traceCap || r.nslices > reqs[k].len / 64 + 1)
+			throw HipError("extension kernel returned an inconsistent result record");
 		out.results[jid] = r;
 		if (r.status == gad::EXT_OVERFLOW) continue;
 		out.wordSteps += r.wordSteps;
@@ -183,20 +197,15 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 	out.trace.clear();
 	out.summaries.clear();
 	std::vector<uint32_t> which(jobs.size());
-	uint32_t maxLen = 0;
-	for (uint32_t i = 0; i < jobs.size(); i++)
-	{
-		which[i] = i;
-		maxLen = std::max(maxLen, jobs[i].len);
-	}
-	// longest first, so long extensions do not form the tail of the launch
+	for (uint32_t i = 0; i < jobs.size(); i++) which[i] = i;
+	// longest first: the 64 threads of a wave then carry similar lengths
 	std::stable_sort(which.begin(), which.end(), [&](uint32_t a, uint32_t b) { return jobs[a].len > jobs[b].len; });
 	gad::ArenaCaps caps;
-	caps.maxSlices = maxLen / 64 + 3;
-	caps.maxItems = caps.maxSlices * 48 + 1024;
-	caps.maxSlots = 4096;
-	caps.maxExtras = 16384;
-	caps.maxPending = 4096;
+	caps.maxSlices = 0;   // per request
+	caps.maxItems = 48;   // band items per slice
+	caps.maxSlots = 1024;
+	caps.maxExtras = 4096;
+	caps.maxPending = 1024;
 	caps.pad = 0;
 	for (int round = 0; round < 4 && !which.empty(); round++)
 	{

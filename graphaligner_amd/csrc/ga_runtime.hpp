@@ -75,17 +75,17 @@ public:
 	// Runs all jobs; statuses EXT_OVERFLOW are rerun with larger workspaces and
 	// only remain if the largest workspace still does not fit.
 	void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out);
-	uint32_t wavesPerCU = 16;
-	uint64_t arenaBudgetBytes = 24ull << 30;
+	uint64_t arenaBudgetBytes = 32ull << 30;
 
 private:
 	void runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out);
+	void runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out);
 	int device_;
 	int cus_ = 0;
 	hipStream_t stream_ = nullptr;
 	hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 	std::unordered_map<const GraphHost*, std::unique_ptr<DeviceGraph>> graphs_;
-	DeviceBuffer arena_, reqs_, seqs_, traces_, results_, counter_, sums_, sumOffsets_;
+	DeviceBuffer arena_, arenaOffsets_, reqs_, seqs_, traces_, results_, sums_, sumOffsets_;
 };
 
 } // namespace ga

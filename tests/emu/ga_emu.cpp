@@ -2,7 +2,9 @@
 // with GA_WAVE_EMULATION: lane helpers run as loops over 64 lanes). Test
 // infrastructure only: lets the CPU suite compare the device algorithm with the
 // oracle before it runs on a GPU. The product library never contains this build.
+#ifndef GA_THREAD_PER_EXTENSION
 #define GA_WAVE_EMULATION 1
+#endif
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>

@@ -8,14 +8,13 @@ import numpy as np
 from oracle_ffi import SliceSummary
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "_build", "libga_emu.so")
-_lib = None
+_LIBS = {"wave": os.path.join(_HERE, "_build", "libga_emu.so"), "thread": os.path.join(_HERE, "_build", "libga_thread.so")}
+_loaded = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        L = ctypes.CDLL(_LIB)
+def lib(mode="wave"):
+    if mode not in _loaded:
+        L = ctypes.CDLL(_LIBS[mode])
         L.emu_graph_load.restype = ctypes.c_void_p
         L.emu_graph_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.emu_graph_free.argtypes = [ctypes.c_void_p]
@@ -24,20 +23,21 @@ def lib():
                                  ctypes.c_int64, ctypes.c_int, ctypes.c_int64, i64p, ctypes.c_int64, i64p,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(SliceSummary), ctypes.c_int64, i64p,
                                  ctypes.POINTER(ctypes.c_uint64)]
-        _lib = L
-    return _lib
+        _loaded[mode] = L
+    return _loaded[mode]
 
 
 class Graph:
-    def __init__(self, path):
+    def __init__(self, path, mode="wave"):
+        self.mode = mode
         err = ctypes.create_string_buffer(512)
-        self.h = lib().emu_graph_load(path.encode(), err, 512)
+        self.h = lib(mode).emu_graph_load(path.encode(), err, 512)
         if not self.h:
             raise RuntimeError(err.value.decode())
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().emu_graph_free(self.h)
+            lib(self.mode).emu_graph_free(self.h)
             self.h = None
 
 
@@ -50,7 +50,7 @@ def extend(graph, params, seq, bigraph_node, offset):
     sums = (SliceSummary * scap)()
     ns = ctypes.c_int64()
     ws = ctypes.c_uint64()
-    rc = lib().emu_extend(graph.h, params.bandwidth, params.maxCellsPerSlice, params.errorCost, params.xdrop, seq.encode(), len(seq),
+    rc = lib(graph.mode).emu_extend(graph.h, params.bandwidth, params.maxCellsPerSlice, params.errorCost, params.xdrop, seq.encode(), len(seq),
                           bigraph_node, offset, trace.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(nt),
                           ctypes.byref(score), sums, scap, ctypes.byref(ns), ctypes.byref(ws))
     rows = [(s.j, s.minScore, s.minScoreNode, s.minScoreNodeOffset, s.maxExactEndposScore, s.maxExactEndposNode,

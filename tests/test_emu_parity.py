@@ -1,5 +1,6 @@
-"""The device extension algorithm, built for the host with its lane helpers as
-loops (tests/emu), against the oracle: traces, scores and per-slice DP words."""
+"""The device extension algorithm built for the host (tests/emu) against the oracle:
+traces, scores and per-slice DP words. "wave": lane helpers as loops over 64
+lanes; "thread": the one-extension-per-thread helpers the gfx950 kernel runs."""
 import numpy as np
 import pytest
 
@@ -8,11 +9,12 @@ import oracle_ffi as o
 from workloads import extension_cases, workload
 
 
+@pytest.mark.parametrize("mode", ["wave", "thread"])
 @pytest.mark.parametrize("preset,n", [("mini", 80), ("mini_sv", 80), ("vg100k", 24)])
-def test_emulated_device_logic_matches_oracle(preset, n, workdir):
+def test_emulated_device_logic_matches_oracle(preset, n, mode, workdir):
     gfa, names, seqs = workload(preset, workdir)
     og = o.Graph(gfa)
-    eg = e.Graph(gfa)
+    eg = e.Graph(gfa, mode)
     idx = o.Index(og, 15, 20, 0.001)
     params = o.preset_params("vg")
     for seq, node, off in extension_cases(og, idx, seqs, n):
@@ -23,10 +25,11 @@ def test_emulated_device_logic_matches_oracle(preset, n, workdir):
         assert a[3] == b[3]
 
 
-def test_emulated_tangle_effort_and_small_band(workdir):
+@pytest.mark.parametrize("mode", ["wave", "thread"])
+def test_emulated_tangle_effort_and_small_band(mode, workdir):
     gfa, names, seqs = workload("mini_sv", workdir)
     og = o.Graph(gfa)
-    eg = e.Graph(gfa)
+    eg = e.Graph(gfa, mode)
     idx = o.Index(og, 15, 20, 0.001)
     params = o.preset_params("vg")
     params.bandwidth = 3

@@ -24,7 +24,7 @@ def _run(ctx, preset, n, workdir, tweak=None, seed=7):
     if tweak:
         tweak(oparams, gparams)
     cases = extension_cases(og, idx, seqs, n, seed=seed)
-    print(f"[gpu-extend] preset={preset} cases={len(cases)}", flush=True)
+    print(f"[gpu-extend] preset={preset} cases={len(cases)} running", flush=True)
     g = ga.Graph(gfa)
     got, ms = ga.extend_batch(ctx, g, gparams, [c[0] for c in cases], [c[1] for c in cases], [c[2] for c in cases], summaries=True)
     bad = 0
