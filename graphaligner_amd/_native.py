@@ -18,7 +18,9 @@ STATUS_NAMES = {0: "GA_OK", 1: "GA_EXT_FAILED", 2: "GA_ASSERT", 3: "GA_EOVERFLOW
 EXPORTED = ["ga_params_preset", "ga_graph_load_gfa", "ga_graph_free", "ga_graph_num_nodes", "ga_graph_num_edges",
             "ga_graph_first_ambiguous", "ga_graph_export", "ga_context_create", "ga_context_free", "ga_extend_batch",
             "ga_index_build", "ga_index_free", "ga_index_max_count", "ga_index_num_kmers", "ga_seeds",
-            "ga_order_seeds", "ga_align_reads", "ga_free", "ga_gaf_lines"]
+            "ga_align_reads", "ga_free"]
+
+READ_STATUS_NAMES = {0: "aligned", 1: "no seeds", 2: "assertion (read dropped)", 3: "no alignment", 4: "workspace overflow"}
 
 
 class Params(ctypes.Structure):
@@ -37,6 +39,13 @@ class SliceSummary(ctypes.Structure):
                 ("maxExactEndposScore", ctypes.c_int32), ("pad1", ctypes.c_int32), ("maxExactEndposNode", ctypes.c_int64),
                 ("cellsProcessed", ctypes.c_int64), ("numNodes", ctypes.c_int64), ("itemsDigest", ctypes.c_uint64),
                 ("scoresNotValid", ctypes.c_int32), ("pad2", ctypes.c_int32)]
+
+
+class AlignStats(ctypes.Structure):
+    _fields_ = [("lookupMs", ctypes.c_double), ("extendKernelMs", ctypes.c_double), ("hostMs", ctypes.c_double),
+                ("totalMs", ctypes.c_double), ("rounds", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("extensions", ctypes.c_int64), ("wordSteps", ctypes.c_int64), ("seeds", ctypes.c_int64),
+                ("extendedBp", ctypes.c_int64)]
 
 
 _lib = None
@@ -60,5 +69,16 @@ def lib():
         L.ga_context_free.argtypes = [vp]
         L.ga_extend_batch.argtypes = [vp, vp, ctypes.POINTER(Params), i64, ctypes.POINTER(ctypes.c_char_p), vp, vp, vp, i64, vp,
                                       vp, vp, vp, ctypes.POINTER(SliceSummary), i64, vp, ctypes.POINTER(ctypes.c_double)]
+        L.ga_index_build.argtypes = [vp, ctypes.POINTER(Params), ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+        L.ga_index_free.argtypes = [vp]
+        L.ga_index_max_count.restype = i64
+        L.ga_index_max_count.argtypes = [vp]
+        L.ga_index_num_kmers.restype = i64
+        L.ga_index_num_kmers.argtypes = [vp]
+        L.ga_seeds.argtypes = [vp, vp, vp, ctypes.c_double, i32, i64, ctypes.POINTER(ctypes.c_char_p), vp,
+                               ctypes.POINTER(vp), vp]
+        L.ga_align_reads.argtypes = [vp, vp, vp, ctypes.POINTER(Params), i64, ctypes.POINTER(ctypes.c_char_p),
+                                     ctypes.POINTER(ctypes.c_char_p), vp, ctypes.POINTER(vp), vp, ctypes.POINTER(AlignStats)]
+        L.ga_free.argtypes = [vp]
         _lib = L
     return _lib

@@ -111,8 +111,66 @@ def extend_batch(ctx, graph, params, seqs, bigraph_nodes, offsets, summaries=Fal
 
 
 class Index:
-    pass
+    """Minimizer index (MinimizerSeeder) built from params' minimizer settings."""
+
+    def __init__(self, graph, params):
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        _check(N.lib().ga_index_build(graph.h, ctypes.byref(params), ctypes.byref(h), err, 1024), "ga_index_build", err)
+        self.h = h
+        self.graph = graph
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().ga_index_free(self.h)
+            self.h = None
+
+    @property
+    def max_count(self):
+        return N.lib().ga_index_max_count(self.h)
+
+    @property
+    def num_kmers(self):
+        return N.lib().ga_index_num_kmers(self.h)
 
 
-def align_reads(*args, **kwargs):
-    raise NotImplementedError
+def _reads(seqs):
+    n = len(seqs)
+    b = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+    arr = (ctypes.c_char_p * max(n, 1))(*b)
+    lens = np.array([len(x) for x in b], dtype=np.int64)
+    return n, arr, lens
+
+
+def seeds(ctx, graph, index, seqs, density, order=True):
+    """Per read an int64 array [nseeds, 10] in SeedHit field order."""
+    n, arr, lens = _reads(seqs)
+    counts = np.zeros(max(n, 1), dtype=np.int64)
+    out = ctypes.c_void_p()
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    _check(N.lib().ga_seeds(ctx.h, graph.h, index.h, density, int(order), n, arr, P(lens), ctypes.byref(out), P(counts)), "ga_seeds")
+    total = int(counts[:n].sum())
+    flat = np.ctypeslib.as_array(ctypes.cast(out, ctypes.POINTER(ctypes.c_int64)), shape=(max(10 * total, 1),)).copy()
+    N.lib().ga_free(out)
+    res, pos = [], 0
+    for i in range(n):
+        c = int(counts[i])
+        res.append(flat[10 * pos: 10 * (pos + c)].reshape(-1, 10))
+        pos += c
+    return res
+
+
+def align_reads(ctx, graph, index, params, names, seqs):
+    """GAF text of all reads (lines of read 0 first), per-read status, stats dict."""
+    n, arr, lens = _reads(seqs)
+    na = (ctypes.c_char_p * max(n, 1))(*[x.encode() for x in names])
+    out = ctypes.c_void_p()
+    status = np.zeros(max(n, 1), dtype=np.int32)
+    st = N.AlignStats()
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    _check(N.lib().ga_align_reads(ctx.h, graph.h, index.h, ctypes.byref(params), n, na, arr, P(lens), ctypes.byref(out),
+                                  P(status), ctypes.byref(st)), "ga_align_reads")
+    text = ctypes.string_at(out.value).decode()
+    N.lib().ga_free(out)
+    stats = {f: getattr(st, f) for f, _ in N.AlignStats._fields_}
+    return text, status[:n], stats

@@ -49,6 +49,9 @@ def build_test_infrastructure():
     # the one-extension-per-thread helpers the gfx950 kernel uses
     _run(common + ["-o", os.path.join(tb, "libga_emu.so")] + srcs)
     _run(common + ["-DGA_THREAD_PER_EXTENSION", "-DGA_THREAD_HOST_TEST", "-o", os.path.join(tb, "libga_thread.so")] + srcs)
+    # the product's host pipeline over a backend running the device code on host threads
+    _run(common + ["-pthread", "-o", os.path.join(tb, "libga_pipe.so"), os.path.join(ROOT, "tests", "emu", "ga_pipeline_emu.cpp")]
+         + [os.path.join(CSRC, s) for s in ("ga_align.cpp", "ga_index.cpp", "ga_graph.cpp")])
 
 
 def build_all():

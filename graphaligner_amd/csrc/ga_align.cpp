@@ -1,0 +1,892 @@
+// Whole-read alignment on one GPU (see ga_align.hpp).
+#include "ga_align.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <thread>
+
+namespace ga {
+
+namespace {
+
+// A reference assertion on the read's path; the read is dropped
+// (src/Aligner.cpp:493-500).
+struct ReadAssert
+{
+};
+
+inline void check(bool c)
+{
+	if (!c) throw ReadAssert{};
+}
+
+constexpr int32_t kScoreMax = INT32_MAX;
+constexpr uint64_t kNoPos = ~0ull;
+
+// Bases an IUPAC letter stands for (A 1, C 2, G 4, T/U 8); '-' stands for none,
+// letters outside the table for -1 (src/CommonUtils.cpp:67-134).
+int baseSet(char c)
+{
+	switch (c)
+	{
+		case '-': return 0;
+		case 'A': case 'a': return 1;
+		case 'C': case 'c': return 2;
+		case 'G': case 'g': return 4;
+		case 'T': case 't': case 'U': case 'u': return 8;
+		case 'N': case 'n': return 15;
+		case 'R': case 'r': return 1 | 4;
+		case 'Y': case 'y': return 2 | 8;
+		case 'K': case 'k': return 4 | 8;
+		case 'M': case 'm': return 2 | 1;
+		case 'S': case 's': return 2 | 4;
+		case 'W': case 'w': return 1 | 8;
+		case 'B': case 'b': return 2 | 4 | 8;
+		case 'D': case 'd': return 1 | 4 | 8;
+		case 'H': case 'h': return 1 | 2 | 8;
+		case 'V': case 'v': return 1 | 2 | 4;
+	}
+	return -1;
+}
+
+inline bool graphHasBase(char g, int base)
+{
+	int s = baseSet(g);
+	check(s >= 0);
+	return (s & base) != 0;
+}
+
+// src/GraphAlignerCommon.h:176-279 (characterMatch / ambiguousMatch): an exact
+// read base matches any graph letter standing for it; otherwise the first base
+// (in A, C, G, T order) both letters stand for decides.
+bool characterMatch(char s, char g)
+{
+	if (s == g) return true;
+	switch (s)
+	{
+		case 'a': case 'A': return graphHasBase(g, 1);
+		case 'c': case 'C': return graphHasBase(g, 2);
+		case 'g': case 'G': return graphHasBase(g, 4);
+		case 't': case 'T': return graphHasBase(g, 8);
+		case '-': return false;
+	}
+	int ss = baseSet(s);
+	check(ss >= 0);
+	for (int b = 1; b <= 8; b <<= 1)
+		if ((ss & b) && graphHasBase(g, b)) return true;
+	return false;
+}
+
+// CommonUtils::Complement (src/CommonUtils.cpp:67-97); 0 for letters outside the table.
+char complementOf(char c)
+{
+	switch (c)
+	{
+		case 'A': case 'a': return 'T';
+		case 'C': case 'c': return 'G';
+		case 'T': case 't': return 'A';
+		case 'G': case 'g': return 'C';
+		case 'N': case 'n': return 'N';
+		case 'U': case 'u': return 'A';
+		case 'R': case 'r': return 'Y';
+		case 'Y': case 'y': return 'R';
+		case 'K': case 'k': return 'M';
+		case 'M': case 'm': return 'K';
+		case 'S': case 's': return 'S';
+		case 'W': case 'w': return 'W';
+		case 'B': case 'b': return 'V';
+		case 'V': case 'v': return 'B';
+		case 'D': case 'd': return 'H';
+		case 'H': case 'h': return 'D';
+	}
+	return 0;
+}
+
+// One trace item (src/GraphAlignerCommon.h:110-151). node/nodeOffset are split
+// node and offset before the fix-ups, bigraph id and offset after them.
+struct TItem
+{
+	uint64_t node;
+	uint64_t nodeOffset;
+	uint64_t seqPos;
+	bool nodeSwitch;
+	char seqChar;
+	char graphChar;
+};
+
+struct OneWay
+{
+	std::vector<TItem> trace;
+	int32_t score = kScoreMax;
+	bool failed() const { return score == kScoreMax; }
+};
+
+struct Alignment
+{
+	std::vector<TItem> trace;
+	int32_t traceScore = 0;
+	uint64_t seedGoodness = 0;
+	uint64_t alignmentStart = 0;
+	uint64_t alignmentEnd = 0;
+	uint64_t alignmentScore = kNoPos;
+	double alignmentXScore = -1;
+	int mappingQuality = 255;
+	bool alignmentFailed() const { return alignmentEnd == alignmentStart; }
+	uint64_t alignmentLength() const { return alignmentEnd - alignmentStart; }
+};
+
+template <typename F>
+void parallelFor(size_t n, int threads, F fn)
+{
+	if (n == 0) return;
+	int t = std::max(1, std::min<int>(threads, (int)std::min<size_t>(n, 64)));
+	if (t == 1)
+	{
+		for (size_t i = 0; i < n; i++) fn(i);
+		return;
+	}
+	std::atomic<size_t> next{0};
+	auto work = [&]() {
+		while (true)
+		{
+			size_t i = next.fetch_add(1);
+			if (i >= n) break;
+			fn(i);
+		}
+	};
+	std::vector<std::thread> pool;
+	for (int k = 0; k < t; k++) pool.emplace_back(work);
+	for (auto& th : pool) th.join();
+}
+
+double msSince(std::chrono::steady_clock::time_point t0)
+{
+	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Decodes one device trace (packed items, end first) into items with the
+// characters OnewayTrace holds right after the traceback.
+OneWay decodeTrace(const GraphHost& g, const uint64_t* packed, uint32_t n, int32_t score, const char* part, uint64_t partLen)
+{
+	OneWay t;
+	t.score = score;
+	t.trace.resize(n);
+	for (uint32_t k = 0; k < n; k++)
+	{
+		uint64_t v = packed[k];
+		uint32_t hi = (uint32_t)(v >> 32);
+		TItem& it = t.trace[k];
+		it.node = (uint32_t)v;
+		it.nodeOffset = (hi >> 1) & 63;
+		it.seqPos = (uint64_t)((int64_t)(hi >> 7) - 1);
+		it.nodeSwitch = (hi & 1) != 0;
+		it.seqChar = it.seqPos < partLen ? part[it.seqPos] : '-';
+		it.graphChar = g.nodeChar((uint32_t)it.node, (uint32_t)it.nodeOffset);
+	}
+	return t;
+}
+
+// GraphAligner::verifyTrace (src/GraphAligner.h:746-766) on split-node positions.
+void verifyTrace(const GraphHost& g, const std::vector<TItem>& trace)
+{
+	for (size_t i = 1; i < trace.size(); i++)
+	{
+		const TItem& o = trace[i - 1];
+		const TItem& n = trace[i];
+		check(n.seqPos != o.seqPos || n.node != o.node || n.nodeOffset != o.nodeOffset);
+		bool seqStep = n.seqPos == o.seqPos || n.seqPos == o.seqPos + 1;
+		if (o.node == n.node)
+		{
+			bool inside = (n.nodeOffset == o.nodeOffset || n.nodeOffset == o.nodeOffset + 1) && seqStep;
+			bool wrap = o.nodeOffset == g.nodeLen[n.node] - 1 && n.nodeOffset == 0 && seqStep;
+			check(inside || wrap);
+			continue;
+		}
+		bool edge = false;
+		for (uint32_t e = g.outOff[o.node]; e < g.outOff[o.node + 1]; e++)
+			if (g.outAdj[e] == n.node) edge = true;
+		check(edge);
+		check(seqStep);
+		check(o.nodeOffset == g.nodeLen[o.node] - 1);
+		check(n.nodeOffset == 0);
+	}
+}
+
+// Edit score of a fixed-up trace (src/GraphAligner.h:540-557).
+int32_t traceEditScore(const std::vector<TItem>& t)
+{
+	int32_t score = 0;
+	for (size_t i = 0; i < t.size(); i++)
+	{
+		if (i == 0)
+		{
+			if (!characterMatch(t[0].seqChar, t[0].graphChar)) score += 1;
+			continue;
+		}
+		if (t[i].seqPos == t[i - 1].seqPos) score += 1;
+		else if (!t[i - 1].nodeSwitch && t[i].node == t[i - 1].node && t[i].nodeOffset == t[i - 1].nodeOffset) score += 1;
+		else if (!characterMatch(t[i].seqChar, t[i].graphChar)) score += 1;
+	}
+	return score;
+}
+
+// GraphAligner::fixForwardTraceSeqPos (src/GraphAligner.h:522-559)
+void fixForward(const GraphHost& g, OneWay& t, uint64_t start, const char* seq, uint64_t len)
+{
+	if (t.trace.empty()) return;
+	for (TItem& it : t.trace)
+	{
+		it.seqPos += start;
+		uint64_t split = it.node;
+		it.node = (uint64_t)(int64_t)g.nodeId[split];
+		it.nodeOffset += g.nodeOff[split];
+		check(it.seqPos < len);
+		if (it.seqPos == start - 1) it.seqChar = seq[it.seqPos];
+		check(it.seqChar == seq[it.seqPos]);
+	}
+	t.score = traceEditScore(t.trace);
+}
+
+// GraphAligner::fixReverseTraceSeqPosAndOrder (src/GraphAligner.h:562-601)
+void fixReverse(const GraphHost& g, OneWay& t, uint64_t end, const char* seq, uint64_t len)
+{
+	if (t.trace.empty()) return;
+	std::reverse(t.trace.begin(), t.trace.end());
+	for (size_t i = 0; i + 1 < t.trace.size(); i++) t.trace[i].nodeSwitch = t.trace[i + 1].nodeSwitch;
+	t.trace.back().nodeSwitch = false;
+	for (TItem& it : t.trace)
+	{
+		it.seqPos = end - it.seqPos;
+		uint64_t split = it.node;
+		int id = g.nodeId[split];
+		uint64_t offset = g.nodeOff[split] + it.nodeOffset;
+		auto rp = g.reversePosition(id, offset);
+		it.node = (uint64_t)(int64_t)rp.first;
+		it.nodeOffset = rp.second;
+		check(it.seqPos < len);
+		it.seqChar = seq[it.seqPos];
+		char c = complementOf(it.graphChar);
+		check(c != 0);
+		it.graphChar = c;
+	}
+	t.score = traceEditScore(t.trace);
+}
+
+// GraphAligner::exactAlignmentPart (src/GraphAligner.h:403-457)
+bool exactAlignmentPart(const Alignment& aln, const SeedHit& seed)
+{
+	const std::vector<TItem>& trace = aln.trace;
+	check(trace.back().seqPos > trace[0].seqPos);
+	if (trace.back().seqPos < seed.seqPos) return false;
+	if (trace[0].seqPos > seed.seqPos) return false;
+	size_t high = trace.size();
+	size_t low = 0;
+	size_t mid = (seed.seqPos - trace[0].seqPos) / (trace.back().seqPos - trace[0].seqPos);
+	auto at = [&](size_t i) -> const TItem& {
+		check(i < trace.size());
+		return trace[i];
+	};
+	while (at(mid).seqPos != seed.seqPos)
+	{
+		if (at(mid).seqPos < seed.seqPos)
+		{
+			low = mid;
+			mid = (high + low) / 2;
+			if (mid == low) mid += 1;
+		}
+		if (at(mid).seqPos > seed.seqPos)
+		{
+			high = mid;
+			mid = (high + low) / 2;
+		}
+		check(low < mid);
+		check(mid < high);
+	}
+	uint64_t compareNode = (uint64_t)(int64_t)(seed.nodeID * 2 + (seed.reverse ? 1 : 0));
+	size_t down = mid;
+	while (trace[down].seqPos == seed.seqPos)
+	{
+		if (compareNode == trace[down].node && seed.nodeOffset == trace[down].nodeOffset) return true;
+		if (down == 0) break;
+		down -= 1;
+	}
+	size_t up = mid;
+	while (trace[up].seqPos == seed.seqPos)
+	{
+		if (compareNode == trace[up].node && seed.nodeOffset == trace[up].nodeOffset) return true;
+		up += 1;
+		if (up == trace.size()) break;
+	}
+	return false;
+}
+
+// AlignmentSelection::alignmentIncompatible (src/AlignmentSelection.cpp:13-30)
+bool alignmentIncompatible(const Alignment& left, const Alignment& right)
+{
+	const float cutoff = 0.05;
+	auto minOverlapLen = std::min(left.alignmentEnd - left.alignmentStart, right.alignmentEnd - right.alignmentStart) * cutoff;
+	uint64_t ls = left.alignmentStart, le = left.alignmentEnd, rs = right.alignmentStart, re = right.alignmentEnd;
+	if (ls > rs)
+	{
+		std::swap(ls, rs);
+		std::swap(le, re);
+	}
+	int overlap = 0;
+	if (le > rs) overlap = (int)(le - rs);
+	return overlap > minOverlapLen;
+}
+
+// AlignmentSelection::SelectAlignments (src/AlignmentSelection.cpp:60-95)
+std::vector<Alignment> selectAlignments(std::vector<Alignment>& all, double minScore, double fraction)
+{
+	std::vector<Alignment> kept;
+	for (auto& a : all)
+		if (a.alignmentXScore >= minScore) kept.push_back(std::move(a));
+	if (fraction == 0) return kept;
+	std::vector<bool> skip(kept.size(), false);
+	for (size_t i = 0; i < kept.size(); i++)
+	{
+		for (size_t j = 0; j < kept.size(); j++)
+		{
+			if (i == j) continue;
+			if (!alignmentIncompatible(kept[i], kept[j])) continue;
+			if (kept[i].alignmentXScore < kept[j].alignmentXScore * fraction)
+			{
+				skip[i] = true;
+				break;
+			}
+		}
+	}
+	std::vector<Alignment> result;
+	for (size_t i = 0; i < kept.size(); i++)
+		if (!skip[i]) result.push_back(std::move(kept[i]));
+	return result;
+}
+
+void addCigar(std::ostringstream& s, uint64_t n, int type)
+{
+	static const char ops[] = {'=', 'X', 'M', 'I', 'D'};
+	if (n == 0 || type < 0 || type > 4) return;
+	s << n << ops[type];
+}
+
+// GAF line of one alignment (src/GraphAlignerGAFAlignment.h:37-250).
+std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, const Alignment& a, bool merge)
+{
+	enum { Match = 0, Mismatch = 1, MatchOrMismatch = 2, Insertion = 3, Deletion = 4, Empty = 5 };
+	const auto& t = a.trace;
+	std::ostringstream cigar, path;
+	uint64_t pathLen = 0, matches = 0, mismatches = 0, deletions = 0, insertions = 0, editLength = 0;
+	int curId = (int)(int64_t)t[0].node;
+	bool curRev = (t[0].node % 2) == 1;
+	uint64_t curOff = t[0].nodeOffset;
+	int currentEdit = Empty;
+	auto addNode = [&](int id, bool rev) {
+		path << (rev ? "<" : ">");
+		const std::string& nm = g.originalName.at(id);
+		if (nm.empty()) path << id / 2;
+		else path << nm;
+	};
+	bool firstMatch = characterMatch(t[0].seqChar, t[0].graphChar);
+	currentEdit = merge ? MatchOrMismatch : (firstMatch ? Match : Mismatch);
+	editLength = 1;
+	if (firstMatch) matches++;
+	else mismatches++;
+	addNode(curId, curRev);
+	pathLen += g.originalSize.at(curId);
+	for (size_t pos = 1; pos < t.size(); pos++)
+	{
+		int newId = (int)(int64_t)t[pos].node;
+		bool newRev = (t[pos].node % 2) == 1;
+		uint64_t newOff = t[pos].nodeOffset;
+		bool insideNode = !t[pos - 1].nodeSwitch || (newId == curId && newRev == curRev && newOff > curOff);
+		if (!insideNode)
+		{
+			uint64_t skippedBefore = g.originalSize.at(curId) - 1 - t[pos - 1].nodeOffset;
+			curId = newId;
+			curRev = newRev;
+			curOff = newOff;
+			addNode(curId, curRev);
+			uint64_t skippedAfter = t[pos].nodeOffset;
+			pathLen += g.originalSize.at(curId) - (skippedBefore + skippedAfter);
+		}
+		int nextEdit;
+		bool match = false;
+		if (t[pos - 1].seqPos == t[pos].seqPos) nextEdit = Deletion;
+		else if (insideNode && t[pos - 1].nodeOffset == t[pos].nodeOffset) nextEdit = Insertion;
+		else
+		{
+			match = characterMatch(t[pos].seqChar, t[pos].graphChar);
+			nextEdit = merge ? MatchOrMismatch : (match ? Match : Mismatch);
+		}
+		if (currentEdit != nextEdit)
+		{
+			addCigar(cigar, editLength, currentEdit);
+			currentEdit = nextEdit;
+			editLength = 0;
+		}
+		editLength++;
+		if (nextEdit == Deletion) deletions++;
+		else if (nextEdit == Insertion) insertions++;
+		else if (match) matches++;
+		else mismatches++;
+	}
+	addCigar(cigar, editLength, currentEdit);
+	uint64_t pathEnd = pathLen - (g.originalSize.at((int)(int64_t)t.back().node) - 1 - t.back().nodeOffset);
+	double total = (double)(matches + mismatches + deletions + insertions);
+	std::ostringstream s;
+	s << name << "\t" << readLen << "\t" << t[0].seqPos << "\t" << (t.back().seqPos + 1) << "\t+\t" << path.str() << "\t" << pathLen << "\t" << t[0].nodeOffset << "\t" << pathEnd << "\t" << matches << "\t" << t.size() << "\t" << a.mappingQuality;
+	s << "\tNM:i:" << (mismatches + deletions + insertions);
+	if (a.alignmentXScore != -1) s << "\tAS:f:" << a.alignmentXScore;
+	s << "\tdv:f:" << 1.0 - ((double)matches / total);
+	s << "\tid:f:" << ((double)matches / total);
+	s << "\tcg:Z:" << cigar.str();
+	return s.str();
+}
+
+// Seed loop state of one read (GraphAligner::AlignOneWay, src/GraphAligner.h:94-181).
+struct ReadState
+{
+	uint32_t read = 0;
+	std::vector<SeedHit> seeds;
+	size_t next = 0;
+	std::vector<Alignment> alignments;
+	uint64_t seedScoreForEndToEndAln = 0;
+	uint64_t worstExtendedSeedScore = 0;
+	uint64_t seedsExtended = 0;
+	uint64_t extendSeeds = 0;
+	bool done = false;
+	int status = READ_ALIGNED;
+	// seed being extended this round and its device jobs (-1: none)
+	size_t pending = 0;
+	int64_t bwdJob = -1, fwdJob = -1;
+};
+
+struct Job
+{
+	uint32_t state;
+	bool backward;
+};
+
+} // namespace
+
+std::vector<SeedHit> seedsFromMatches(const GraphHost& g, const MinimizerIndex& idx, const gad::SeedMatch* matches, uint32_t nmatches, uint64_t readLen, double density)
+{
+	std::vector<gad::SeedMatch> order(matches, matches + nmatches);
+	// same comparator and input order as the reference's sort of match indices
+	std::sort(order.begin(), order.end(), [](const gad::SeedMatch& l, const gad::SeedMatch& r) { return l.count < r.count; });
+	uint64_t maxHits = density == -1 ? kNoPos : (uint64_t)(readLen * density);
+	std::vector<SeedHit> result;
+	uint64_t seedsHere = 0, allowedCount = 0;
+	for (const auto& m : order)
+	{
+		uint64_t count = m.count;
+		if (seedsHere >= maxHits && count > allowedCount) break;
+		allowedCount = count;
+		for (uint32_t k = 0; k < m.count; k++)
+		{
+			uint64_t merged = idx.positions[m.start + k];
+			uint64_t split = merged >> 6;
+			uint64_t offset = merged & 63;
+			SeedHit h;
+			h.nodeID = g.nodeId[split] / 2;
+			h.nodeOffset = offset + g.nodeOff[split];
+			h.seqPos = m.pos;
+			h.matchLen = idx.k;
+			h.rawSeedGoodness = idx.maxCount - count;
+			h.reverse = g.reverse[split] != 0;
+			h.alignmentGraphNodeId = split;
+			h.alignmentGraphNodeOffset = offset;
+			result.push_back(h);
+		}
+		seedsHere += count;
+	}
+	return result;
+}
+
+void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds)
+{
+	// seeds grouped by chain, each with its approximate diagonal
+	std::map<uint64_t, std::vector<std::pair<uint64_t, uint64_t>>> byChain;
+	for (size_t i = 0; i < seeds.size(); i++)
+	{
+		uint64_t node = seeds[i].alignmentGraphNodeId;
+		uint64_t off = seeds[i].alignmentGraphNodeOffset;
+		byChain[g.chain[node]].emplace_back(i, g.chainPos[node] + off - seeds[i].seqPos);
+	}
+	for (auto& c : byChain)
+	{
+		auto& v = c.second;
+		std::sort(v.begin(), v.end(), [](const std::pair<uint64_t, uint64_t>& l, const std::pair<uint64_t, uint64_t>& r) { return l.second < r.second; });
+		size_t clusterStart = 0;
+		for (size_t i = 1; i <= v.size(); i++)
+		{
+			if (i < v.size() && v[i].second <= v[i - 1].second + 100) continue;
+			std::sort(v.begin() + clusterStart, v.begin() + i, [&seeds](const std::pair<uint64_t, uint64_t>& l, const std::pair<uint64_t, uint64_t>& r) { return seeds[l.first].seqPos < seeds[r.first].seqPos; });
+			uint64_t matchingBps = 0;
+			int lastEnd = INT_MIN;
+			for (size_t k = clusterStart; k < i; k++)
+			{
+				const SeedHit& s = seeds[v[k].first];
+				int thisStart = (int)s.seqPos - (int)s.matchLen + 1;
+				int thisEnd = (int)s.seqPos;
+				matchingBps += (thisEnd - std::max(thisStart, lastEnd));
+				lastEnd = thisEnd;
+			}
+			for (size_t k = clusterStart; k < i; k++)
+			{
+				seeds[v[k].first].seedGoodness = matchingBps + seeds[v[k].first].rawSeedGoodness;
+				seeds[v[k].first].seedClusterSize = i - clusterStart;
+			}
+			clusterStart = i;
+		}
+	}
+	std::sort(seeds.begin(), seeds.end(), [](const SeedHit& l, const SeedHit& r) { return l.seedGoodness < r.seedGoodness; });
+	std::reverse(seeds.begin(), seeds.end());
+}
+
+void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs)
+{
+	ReadBatch rb;
+	ctx.loadReads(idx, seqs, lens, rb);
+	if (lookupMs) *lookupMs = rb.lookupMs;
+	out.assign(seqs.size(), {});
+	parallelFor(seqs.size(), 16, [&](size_t i) {
+		out[i] = seedsFromMatches(g, idx, rb.matches.data() + rb.matchOffset[i], rb.matchCount[i], lens[i], density);
+		if (order) orderSeedsByChaining(g, out[i]);
+	});
+}
+
+namespace {
+
+// Picks the next seed of the read's loop to extend (src/GraphAligner.h:105-140);
+// returns false when the loop is over.
+bool advance(ReadState& st, const AlignOptions& opt)
+{
+	while (st.next < st.seeds.size())
+	{
+		const SeedHit& s = st.seeds[st.next];
+		if (opt.sloppyOptimizations && s.seedGoodness <= st.seedScoreForEndToEndAln) return false;
+		if (st.seedsExtended >= st.extendSeeds && s.seedGoodness < st.worstExtendedSeedScore) return false;
+		if (s.seedClusterSize < opt.minSeedClusterSize)
+		{
+			st.next++;
+			continue;
+		}
+		bool covered = false;
+		if (opt.sloppyOptimizations)
+		{
+			for (const auto& a : st.alignments)
+			{
+				if (a.alignmentStart <= s.seqPos && a.alignmentEnd >= s.seqPos && a.seedGoodness > s.seedGoodness)
+				{
+					covered = true;
+					break;
+				}
+			}
+		}
+		if (!covered)
+		{
+			for (const auto& a : st.alignments)
+			{
+				if (exactAlignmentPart(a, s))
+				{
+					covered = true;
+					break;
+				}
+			}
+		}
+		if (covered)
+		{
+			st.next++;
+			continue;
+		}
+		st.worstExtendedSeedScore = s.seedGoodness;
+		st.seedsExtended++;
+		st.pending = st.next;
+		return true;
+	}
+	return false;
+}
+
+} // namespace
+
+void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const std::vector<const char*>& names, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<ReadOutcome>& out, AlignStats& stats)
+{
+	auto t0 = std::chrono::steady_clock::now();
+	const size_t n = seqs.size();
+	out.assign(n, ReadOutcome{});
+	stats = AlignStats{};
+	for (size_t i = 0; i < n; i++)
+		if (lens[i] > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
+	for (size_t batchStart = 0; batchStart < n; batchStart += opt.readsPerBatch)
+	{
+		size_t batchEnd = std::min(n, batchStart + (size_t)opt.readsPerBatch);
+		std::vector<const char*> bseqs(seqs.begin() + batchStart, seqs.begin() + batchEnd);
+		std::vector<uint32_t> blens(lens.begin() + batchStart, lens.begin() + batchEnd);
+		ReadBatch rb;
+		ctx.loadReads(idx, bseqs, blens, rb);
+		stats.lookupMs += rb.lookupMs;
+		const size_t bn = bseqs.size();
+		std::vector<ReadState> states(bn);
+		auto th = std::chrono::steady_clock::now();
+		parallelFor(bn, opt.hostThreads, [&](size_t i) {
+			ReadState& st = states[i];
+			st.read = (uint32_t)i;
+			ReadOutcome& o = out[batchStart + i];
+			try
+			{
+				st.seeds = seedsFromMatches(g, idx, rb.matches.data() + rb.matchOffset[i], rb.matchCount[i], blens[i], opt.minimizerSeedDensity);
+				if (st.seeds.empty())
+				{
+					o.status = READ_NO_SEEDS;
+					st.done = true;
+					return;
+				}
+				orderSeedsByChaining(g, st.seeds);
+				// ReverseComplement of the read asserts on letters it does not know
+				for (uint32_t k = 0; k < blens[i]; k++) check(complementOf(bseqs[i][k]) != 0);
+				st.extendSeeds = opt.seedExtendDensity == -1 ? st.seeds.size() : (uint64_t)(opt.seedExtendDensity * blens[i] + 1);
+			}
+			catch (const ReadAssert&)
+			{
+				st.status = READ_ASSERT;
+				st.done = true;
+			}
+		});
+		stats.hostMs += msSince(th);
+		for (auto& st : states) stats.seeds += st.seeds.size();
+		std::vector<uint32_t> active;
+		for (uint32_t i = 0; i < bn; i++)
+			if (!states[i].done) active.push_back(i);
+		while (!active.empty())
+		{
+			th = std::chrono::steady_clock::now();
+			std::vector<uint8_t> extendNow(active.size(), 0);
+			parallelFor(active.size(), opt.hostThreads, [&](size_t a) {
+				ReadState& st = states[active[a]];
+				try
+				{
+					extendNow[a] = advance(st, opt) ? 1 : 0;
+				}
+				catch (const ReadAssert&)
+				{
+					st.status = READ_ASSERT;
+				}
+				if (!extendNow[a]) st.done = true;
+			});
+			std::vector<ExtJob> jobs;
+			std::vector<uint32_t> extending;
+			for (size_t a = 0; a < active.size(); a++)
+			{
+				if (!extendNow[a]) continue;
+				uint32_t r = active[a];
+				ReadState& st = states[r];
+				const SeedHit& s = st.seeds[st.pending];
+				const uint64_t L = blens[r];
+				int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
+				int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
+				st.bwdJob = st.fwdJob = -1;
+				if (s.seqPos > 0)
+				{
+					auto rp = g.reversePosition(forwardId, s.nodeOffset);
+					uint32_t node = g.unitigNode(backwardId, rp.second);
+					ExtJob j{nullptr, (uint32_t)s.seqPos, node, (uint32_t)(rp.second - g.nodeOff[node])};
+					j.devOffset = ctx.reverseBase() + rb.offset[r] + (L - s.seqPos);
+					st.bwdJob = jobs.size();
+					jobs.push_back(j);
+				}
+				if (s.seqPos < L - 1)
+				{
+					uint32_t node = g.unitigNode(forwardId, s.nodeOffset);
+					ExtJob j{nullptr, (uint32_t)(L - s.seqPos - 1), node, (uint32_t)(s.nodeOffset - g.nodeOff[node])};
+					j.devOffset = rb.offset[r] + s.seqPos + 1;
+					st.fwdJob = jobs.size();
+					jobs.push_back(j);
+				}
+				extending.push_back(r);
+			}
+			stats.hostMs += msSince(th);
+			if (extending.empty()) break;
+			ExtBatch eb;
+			ctx.extend(g, opt.ext, jobs, false, eb, ctx.readsBase());
+			stats.rounds++;
+			stats.launches += eb.launches;
+			stats.extensions += jobs.size();
+			stats.extendKernelMs += eb.kernelMs;
+			stats.wordSteps += eb.wordSteps;
+			for (const auto& j : jobs) stats.extendedBp += j.len;
+			th = std::chrono::steady_clock::now();
+			parallelFor(extending.size(), opt.hostThreads, [&](size_t a) {
+				ReadState& st = states[extending[a]];
+				const uint32_t r = extending[a];
+				const char* seq = bseqs[r];
+				const uint64_t L = blens[r];
+				const SeedHit s = st.seeds[st.pending];
+				st.next = st.pending + 1;
+				try
+				{
+					// getTwoDirectionalTrace (src/GraphAligner.h:475-520)
+					OneWay bwd, fwd;
+					std::string revPart;
+					int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
+					int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
+					auto rp = g.reversePosition(forwardId, s.nodeOffset);
+					for (int dir = 0; dir < 2; dir++)
+					{
+						int64_t j = dir == 0 ? st.bwdJob : st.fwdJob;
+						if (j < 0) continue;
+						const gad::ExtResult& res = eb.results[j];
+						if (res.status == gad::EXT_OVERFLOW)
+						{
+							st.status = READ_OVERFLOW;
+							st.done = true;
+							return;
+						}
+						check(res.status != gad::EXT_ASSERT);
+						OneWay t;
+						if (res.status == gad::EXT_OK)
+						{
+							if (dir == 0)
+							{
+								revPart.resize(s.seqPos);
+								for (uint64_t k = 0; k < s.seqPos; k++) revPart[k] = complementOf(seq[s.seqPos - 1 - k]);
+								t = decodeTrace(g, eb.trace.data() + eb.traceOffset[j], res.ntrace, res.score, revPart.data(), revPart.size());
+							}
+							else
+							{
+								t = decodeTrace(g, eb.trace.data() + eb.traceOffset[j], res.ntrace, res.score, seq + s.seqPos + 1, L - s.seqPos - 1);
+							}
+							check(!t.trace.empty());
+						}
+						(dir == 0 ? bwd : fwd) = std::move(t);
+					}
+					if (!bwd.failed())
+					{
+						const TItem& b = bwd.trace.back();
+						check(b.seqPos == kNoPos && g.nodeId[b.node] == backwardId && g.nodeOff[b.node] + b.nodeOffset == rp.second);
+						std::reverse(bwd.trace.begin(), bwd.trace.end());
+					}
+					if (!fwd.failed())
+					{
+						const TItem& f = fwd.trace.back();
+						check(f.seqPos == kNoPos && g.nodeId[f.node] == forwardId && g.nodeOff[f.node] + f.nodeOffset == s.nodeOffset);
+						std::reverse(fwd.trace.begin(), fwd.trace.end());
+					}
+					// getAlignmentFromSeed (src/GraphAligner.h:631-692)
+					if (!fwd.trace.empty()) verifyTrace(g, fwd.trace);
+					if (!bwd.trace.empty()) verifyTrace(g, bwd.trace);
+					fixReverse(g, bwd, s.seqPos - 1, seq, L);
+					fixForward(g, fwd, s.seqPos + 1, seq, L);
+					if (fwd.failed() && bwd.failed()) return;
+					OneWay merged = std::move(bwd);
+					if (merged.failed())
+					{
+						merged = std::move(fwd);
+					}
+					else if (!fwd.failed())
+					{
+						const TItem& x = merged.trace.back();
+						const TItem& y = fwd.trace[0];
+						check(x.node == y.node && x.nodeOffset == y.nodeOffset && x.seqPos == y.seqPos);
+						merged.trace.pop_back();
+						merged.trace.insert(merged.trace.end(), fwd.trace.begin(), fwd.trace.end());
+						merged.score += fwd.score;
+					}
+					check(!merged.trace.empty());
+					Alignment item;
+					uint64_t seqStart = merged.trace[0].seqPos;
+					uint64_t seqEnd = merged.trace.back().seqPos;
+					check(seqEnd < L);
+					item.traceScore = merged.score;
+					item.trace = std::move(merged.trace);
+					item.alignmentScore = (uint64_t)(int64_t)item.traceScore;
+					item.alignmentStart = seqStart;
+					item.alignmentEnd = seqEnd + 1;
+					item.alignmentXScore = (int32_t)item.alignmentLength() * 100 - opt.xscoreErrorCost * (int32_t)item.alignmentScore;
+					item.alignmentXScore /= 100.0;
+					if (item.alignmentFailed()) return;
+					item.seedGoodness = s.seedGoodness;
+					st.alignments.push_back(std::move(item));
+					if (opt.sloppyOptimizations)
+					{
+						auto& al = st.alignments;
+						std::sort(al.begin(), al.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
+						if (al[0].alignmentStart == 0)
+						{
+							uint64_t minGoodness = al[0].seedGoodness;
+							uint64_t contiguousEnd = al[0].alignmentEnd;
+							for (size_t k = 1; k < al.size(); k++)
+							{
+								if (al[k].alignmentStart <= contiguousEnd)
+								{
+									minGoodness = std::min(minGoodness, al[k].seedGoodness);
+									contiguousEnd = std::max(contiguousEnd, al[k].alignmentEnd);
+								}
+							}
+							if (contiguousEnd == L) st.seedScoreForEndToEndAln = minGoodness;
+						}
+					}
+				}
+				catch (const ReadAssert&)
+				{
+					st.status = READ_ASSERT;
+					st.done = true;
+				}
+			});
+			stats.hostMs += msSince(th);
+			std::vector<uint32_t> still;
+			for (uint32_t r : active)
+				if (!states[r].done) still.push_back(r);
+			active.swap(still);
+		}
+		// selection and GAF (src/Aligner.cpp:502-606)
+		th = std::chrono::steady_clock::now();
+		parallelFor(bn, opt.hostThreads, [&](size_t i) {
+			ReadState& st = states[i];
+			ReadOutcome& o = out[batchStart + i];
+			if (o.status == READ_NO_SEEDS) return;
+			o.seedsExtended = st.seedsExtended;
+			if (st.status != READ_ALIGNED)
+			{
+				o.status = st.status;
+				return;
+			}
+			std::vector<Alignment> sel;
+			try
+			{
+				if (!st.alignments.empty()) sel = selectAlignments(st.alignments, opt.minAlignmentScore, opt.multimapScoreFraction);
+				if (sel.empty())
+				{
+					o.status = READ_NO_ALIGNMENT;
+					return;
+				}
+				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
+				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentXScore > r.alignmentXScore; });
+				std::string text;
+				for (const auto& a : sel)
+				{
+					text += gafLine(g, names[batchStart + i], blens[i], a, opt.cigarMatchMismatchMerge);
+					text += '\n';
+				}
+				o.gaf = std::move(text);
+				o.status = READ_ALIGNED;
+			}
+			catch (const ReadAssert&)
+			{
+				o.status = READ_ASSERT;
+				o.gaf.clear();
+			}
+		});
+		stats.hostMs += msSince(th);
+	}
+	stats.totalMs = msSince(t0);
+}
+
+} // namespace ga

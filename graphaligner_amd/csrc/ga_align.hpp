@@ -1,0 +1,87 @@
+// Whole-read alignment on one GPU: device seed lookup, host seed ordering, the
+// seed-extension loop run as rounds of batched device extensions (every active
+// read contributes the extensions of its next seed to one launch), trace
+// fix-up, alignment selection and GAF text.
+//
+// Each read's seed loop makes exactly the decisions of GraphAligner::AlignOneWay
+// (src/GraphAligner.h:94-181); batching only interleaves independent reads.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "ga_graph.hpp"
+#include "ga_index.hpp"
+#include "ga_backend.hpp"
+
+namespace ga {
+
+// src/GraphAlignerWrapper.h:11-50 (SeedHit)
+struct SeedHit
+{
+	int nodeID = 0;                 // bigraph node id / 2
+	uint64_t nodeOffset = 0;        // offset in the bigraph node
+	uint64_t seqPos = 0;            // read position of the seed's last base
+	uint64_t matchLen = 0;
+	bool reverse = false;
+	uint64_t alignmentGraphNodeId = 0;
+	uint64_t alignmentGraphNodeOffset = 0;
+	uint64_t rawSeedGoodness = 0;
+	uint64_t seedGoodness = 0;
+	uint64_t seedClusterSize = 0;
+};
+
+struct AlignOptions
+{
+	gad::ExtParams ext;
+	int32_t xscoreErrorCost = 294;
+	double seedExtendDensity = 0.002;
+	double multimapScoreFraction = 0.9;
+	double minAlignmentScore = 0;
+	double minimizerSeedDensity = 5;
+	bool sloppyOptimizations = true;
+	uint64_t minSeedClusterSize = 1;
+	bool cigarMatchMismatchMerge = false;
+	int hostThreads = 16;
+	uint32_t readsPerBatch = 1u << 20;
+};
+
+enum ReadStatus
+{
+	READ_ALIGNED = 0,
+	READ_NO_SEEDS = 1,
+	READ_ASSERT = 2,      // a reference assertion fired; the read is dropped
+	READ_NO_ALIGNMENT = 3,
+	READ_OVERFLOW = 4     // an extension did not fit the largest device workspace
+};
+
+struct ReadOutcome
+{
+	int status = READ_NO_ALIGNMENT;
+	std::string gaf;
+	uint64_t seedsExtended = 0;
+};
+
+struct AlignStats
+{
+	double lookupMs = 0;        // device seed lookup (+ reverse complement)
+	double extendKernelMs = 0;  // device seed extension
+	double hostMs = 0;          // host seeding, fix-up, selection, GAF
+	double totalMs = 0;
+	uint64_t rounds = 0, launches = 0, extensions = 0, wordSteps = 0, seeds = 0, extendedBp = 0;
+};
+
+// MinimizerSeeder::getSeeds (src/MinimizerSeeder.cpp:521-555) from the device
+// lookup's matches of one read.
+std::vector<SeedHit> seedsFromMatches(const GraphHost& g, const MinimizerIndex& idx, const gad::SeedMatch* matches, uint32_t nmatches, uint64_t readLen, double density);
+
+// GraphAligner::orderSeedsByChaining (src/GraphAligner.h:250-312).
+void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds);
+
+// Seeds of every read as the reference's seeder + OrderSeeds produce them.
+void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs);
+
+// src/Aligner.cpp:390-606 for a batch of reads (one GAF line per selected
+// alignment, '\n'-terminated, in out[i].gaf).
+void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const std::vector<const char*>& names, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<ReadOutcome>& out, AlignStats& stats);
+
+} // namespace ga

@@ -1,0 +1,97 @@
+// What the alignment pipeline needs from a device: resident reads with their
+// index matches, and batched seed extensions over them. Context (ga_runtime.hpp)
+// is the gfx950 implementation; this header has no HIP dependency.
+#pragma once
+#include <cstdint>
+#include <vector>
+#include "ga_device_types.h"
+#include "ga_graph.hpp"
+#include "ga_index.hpp"
+
+namespace ga {
+
+// One extension: align seq[0, len) from column startOffset of split node startNode.
+// With a resident sequence base (extend's devSeqBase) the sequence is the
+// resident one at devOffset and seq is not read.
+struct ExtJob
+{
+	const char* seq;
+	uint32_t len;
+	uint32_t startNode;
+	uint32_t startOffset;
+	uint64_t devOffset = 0;
+};
+
+struct ExtBatch
+{
+	std::vector<gad::ExtResult> results;
+	std::vector<uint64_t> traceOffset; // per job, into trace
+	std::vector<uint64_t> trace;       // packed items, each job's trace end first
+	std::vector<gad::SliceSummary> summaries;
+	std::vector<uint64_t> summaryOffset;
+	double kernelMs = 0;               // summed kernel time of all launches
+	uint32_t launches = 0;
+	uint64_t wordSteps = 0;
+};
+
+// Reads resident on the device for one alignment batch: forward and
+// reverse-complement bytes, and the index matches of each read.
+struct ReadBatch
+{
+	std::vector<uint64_t> offset;  // per read, into the forward bytes
+	std::vector<uint32_t> length;
+	uint64_t bytes = 0;
+	uint32_t maxLen = 0;
+	// host copies of the lookup output: matches of read i are
+	// matches[matchOffset[i], matchOffset[i] + matchCount[i])
+	std::vector<gad::SeedMatch> matches;
+	std::vector<uint64_t> matchOffset;
+	std::vector<uint32_t> matchCount;
+	double lookupMs = 0;
+};
+
+// Workspace capacities of the first extension attempt; a job that overflows is
+// rerun with every capacity multiplied by kArenaGrowth, at most kArenaAttempts
+// attempts in all. Band items are budgeted per 64-row slice.
+constexpr int kArenaAttempts = 4;
+constexpr uint32_t kArenaGrowth = 8;
+inline gad::ArenaCaps initialArenaCaps()
+{
+	gad::ArenaCaps c;
+	c.maxSlices = 0;  // set per request from its length
+	c.maxItems = 48;
+	c.maxSlots = 1024;
+	c.maxExtras = 4096;
+	c.maxPending = 1024;
+	c.pad = 0;
+	return c;
+}
+inline void growArenaCaps(gad::ArenaCaps& c)
+{
+	c.maxItems *= kArenaGrowth;
+	c.maxSlots *= kArenaGrowth;
+	c.maxExtras *= kArenaGrowth;
+	c.maxPending *= kArenaGrowth;
+}
+
+// Trace capacity of an extension of len read bases (a trace has at most one item
+// per read base plus one per graph base).
+inline uint32_t traceCapFor(uint32_t len) { return 2 * len + 256; }
+
+class ExtendBackend
+{
+public:
+	virtual ~ExtendBackend() = default;
+	// Runs all jobs; statuses EXT_OVERFLOW are rerun with larger workspaces and
+	// only remain if the largest workspace still does not fit.
+	virtual void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase = nullptr) = 0;
+	// Makes the reads resident, computes their reverse complements and looks every
+	// read k-mer up in the index; the reads stay resident until the next call.
+	// Forward bytes of read i start at readsBase() + offset[i], its reverse
+	// complement at readsBase() + reverseBase() + offset[i].
+	virtual void loadReads(const MinimizerIndex& idx, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, ReadBatch& out) = 0;
+	virtual const char* readsBase() const = 0;
+	virtual uint64_t reverseBase() const = 0;
+};
+
+} // namespace ga

@@ -4,9 +4,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
 #include <new>
 #include <string>
+#include "ga_align.hpp"
 #include "ga_capi_internal.hpp"
 
 using namespace ga;
@@ -230,5 +232,165 @@ ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga
 		return GA_ENOMEM;
 	}
 }
+
+ga_status_t ga_index_build(const ga_graph_t* graph, const ga_params_t* params, ga_index_t** index, char* err, size_t errcap)
+{
+	if (!graph || !params || !index) return GA_EBADARG;
+	*index = nullptr;
+	if (params->minimizerLength < 1 || params->minimizerLength > 31 || params->minimizerWindowSize < params->minimizerLength)
+	{
+		copyErr(err, errcap, "minimizer length must be 1..31 and not above the window size");
+		return GA_EBADARG;
+	}
+	try
+	{
+		auto* x = new ga_index{buildMinimizerIndex(graph->host, params->minimizerLength, params->minimizerWindowSize, params->minimizerDiscardMostNumerousFraction), graph};
+		*index = x;
+		return GA_OK;
+	}
+	catch (const std::bad_alloc&)
+	{
+		copyErr(err, errcap, "out of host memory");
+		return GA_ENOMEM;
+	}
+	catch (const std::exception& e)
+	{
+		copyErr(err, errcap, e.what());
+		return GA_EBADARG;
+	}
+}
+
+void ga_index_free(ga_index_t* index) { delete index; }
+
+int64_t ga_index_max_count(const ga_index_t* index) { return index ? (int64_t)index->idx.maxCount : -1; }
+
+int64_t ga_index_num_kmers(const ga_index_t* index) { return index ? (int64_t)index->idx.numKmers : -1; }
+
+static bool readArgsOk(int64_t n, const char* const* seqs, const int64_t* lens)
+{
+	if (n < 0 || (n > 0 && (!seqs || !lens))) return false;
+	for (int64_t i = 0; i < n; i++)
+		if (lens[i] < 0 || lens[i] > (1 << 24) || (lens[i] > 0 && !seqs[i])) return false;
+	return true;
+}
+
+ga_status_t ga_seeds(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, double density, int32_t order, int64_t n, const char* const* seqs, const int64_t* lens, int64_t** seeds, int64_t* counts)
+{
+	if (!ctx || !graph || !index || !seeds || !counts || index->graph != graph || !readArgsOk(n, seqs, lens)) return GA_EBADARG;
+	*seeds = nullptr;
+	try
+	{
+		std::vector<const char*> s(seqs, seqs + n);
+		std::vector<uint32_t> l(n);
+		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
+		std::vector<std::vector<SeedHit>> out;
+		seedReads(*ctx->ctx, graph->host, index->idx, density, order != 0, s, l, out, nullptr);
+		size_t total = 0;
+		for (auto& v : out) total += v.size();
+		int64_t* buf = (int64_t*)std::malloc(std::max<size_t>(1, total) * 10 * sizeof(int64_t));
+		if (!buf) return GA_ENOMEM;
+		int64_t* o = buf;
+		for (int64_t i = 0; i < n; i++)
+		{
+			counts[i] = out[i].size();
+			for (const SeedHit& h : out[i])
+			{
+				o[0] = h.nodeID;
+				o[1] = h.nodeOffset;
+				o[2] = h.seqPos;
+				o[3] = h.matchLen;
+				o[4] = h.reverse;
+				o[5] = h.alignmentGraphNodeId;
+				o[6] = h.alignmentGraphNodeOffset;
+				o[7] = h.rawSeedGoodness;
+				o[8] = h.seedGoodness;
+				o[9] = h.seedClusterSize;
+				o += 10;
+			}
+		}
+		*seeds = buf;
+		return GA_OK;
+	}
+	catch (const HipError&)
+	{
+		return GA_EHIP;
+	}
+	catch (const std::bad_alloc&)
+	{
+		return GA_ENOMEM;
+	}
+	catch (const std::exception&)
+	{
+		return GA_EBADARG;
+	}
+}
+
+ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t n, const char* const* names, const char* const* seqs, const int64_t* lens, char** gaf, int32_t* status, ga_align_stats_t* stats)
+{
+	if (!ctx || !graph || !index || !params || !gaf || !status || index->graph != graph || !readArgsOk(n, seqs, lens) || (n > 0 && !names)) return GA_EBADARG;
+	for (int64_t i = 0; i < n; i++)
+		if (!names[i]) return GA_EBADARG;
+	*gaf = nullptr;
+	try
+	{
+		AlignOptions opt;
+		opt.ext = extParamsOf(*params);
+		opt.xscoreErrorCost = errorCostOf(*params);
+		opt.seedExtendDensity = params->seedExtendDensity;
+		opt.multimapScoreFraction = params->multimapScoreFraction;
+		opt.minAlignmentScore = params->minAlignmentScore;
+		opt.minimizerSeedDensity = params->minimizerSeedDensity;
+		opt.sloppyOptimizations = params->tryAllSeeds == 0;
+		opt.minSeedClusterSize = params->seedClusterMinSize < 0 ? 0 : (uint64_t)params->seedClusterMinSize;
+		opt.cigarMatchMismatchMerge = params->cigarMatchMismatchMerge != 0;
+		std::vector<const char*> nm(names, names + n), s(seqs, seqs + n);
+		std::vector<uint32_t> l(n);
+		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
+		std::vector<ReadOutcome> out;
+		AlignStats st;
+		alignReads(*ctx->ctx, graph->host, index->idx, opt, nm, s, l, out, st);
+		size_t total = 1;
+		for (auto& o : out) total += o.gaf.size();
+		char* buf = (char*)std::malloc(total);
+		if (!buf) return GA_ENOMEM;
+		size_t pos = 0;
+		for (int64_t i = 0; i < n; i++)
+		{
+			std::memcpy(buf + pos, out[i].gaf.data(), out[i].gaf.size());
+			pos += out[i].gaf.size();
+			status[i] = out[i].status;
+		}
+		buf[pos] = 0;
+		*gaf = buf;
+		if (stats)
+		{
+			stats->lookupMs = st.lookupMs;
+			stats->extendKernelMs = st.extendKernelMs;
+			stats->hostMs = st.hostMs;
+			stats->totalMs = st.totalMs;
+			stats->rounds = st.rounds;
+			stats->launches = st.launches;
+			stats->extensions = st.extensions;
+			stats->wordSteps = st.wordSteps;
+			stats->seeds = st.seeds;
+			stats->extendedBp = st.extendedBp;
+		}
+		return GA_OK;
+	}
+	catch (const HipError&)
+	{
+		return GA_EHIP;
+	}
+	catch (const std::bad_alloc&)
+	{
+		return GA_ENOMEM;
+	}
+	catch (const std::exception&)
+	{
+		return GA_EBADARG;
+	}
+}
+
+void ga_free(void* p) { std::free(p); }
 
 } // extern "C"

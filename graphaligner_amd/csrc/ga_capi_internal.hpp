@@ -3,11 +3,18 @@
 #include <memory>
 #include "../../include/graphaligner_amd.h"
 #include "ga_graph.hpp"
+#include "ga_index.hpp"
 #include "ga_runtime.hpp"
 
 struct ga_graph
 {
 	ga::GraphHost host;
+};
+
+struct ga_index
+{
+	ga::MinimizerIndex idx;
+	const ga_graph* graph;
 };
 
 struct ga_context

@@ -116,4 +116,21 @@ struct Item
 	uint32_t node;
 };
 
+// Minimizer table slot (k-mer -> position range); kmer == ~0 marks an empty slot.
+struct SeedTableSlot
+{
+	uint64_t kmer;
+	uint32_t start;
+	uint32_t count;
+};
+
+// One read k-mer found in the index: read position of its last base, and the
+// k-mer's range in the position array.
+struct SeedMatch
+{
+	uint32_t pos;
+	uint32_t start;
+	uint32_t count;
+};
+
 } // namespace gad

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include "ga_extend_core.h"
 #include "ga_kernels.hpp"
+#include "ga_seed_core.h"
 
 namespace gad {
 
@@ -33,6 +34,63 @@ uint64_t requestArenaBytes(uint32_t len, const ArenaCaps& caps)
 	c.maxSlices = requestSlices(len);
 	c.maxItems = requestItems(len, caps);
 	return arenaBytes(c);
+}
+
+} // namespace gad
+
+namespace gad {
+
+// One thread per read (reads are independent and their lookups are short,
+// irregular hash probes; the table stays in L2/HBM).
+__global__ void __launch_bounds__(64) seedLookupKernel(const char* __restrict__ reads, const uint64_t* __restrict__ readOff, const uint32_t* __restrict__ readLen, uint32_t nreads, const SeedTableSlot* __restrict__ table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* __restrict__ out, uint32_t* __restrict__ outCount)
+{
+	uint32_t r = blockIdx.x * 64 + threadIdx.x;
+	if (r >= nreads) return;
+	outCount[r] = lookupReadKmers(reads + readOff[r], readLen[r], table, mask, k, w, maxCount, out + readOff[r]);
+}
+
+// Reverse complement of every read, one block per read.
+__global__ void revcompKernel(const char* __restrict__ reads, char* __restrict__ rev, const uint64_t* __restrict__ readOff, const uint32_t* __restrict__ readLen, uint32_t nreads)
+{
+	uint32_t r = blockIdx.x;
+	if (r >= nreads) return;
+	uint64_t len = readLen[r];
+	const char* s = reads + readOff[r];
+	char* d = rev + readOff[r];
+	for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) d[i] = complementByte(s[len - 1 - i]);
+}
+
+hipError_t launchSeedLookup(hipStream_t stream, const char* reads, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, const SeedTableSlot* table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* out, uint32_t* outCount)
+{
+	hipLaunchKernelGGL(seedLookupKernel, dim3((nreads + 63) / 64), dim3(64), 0, stream, reads, readOff, readLen, nreads, table, mask, k, w, maxCount, out, outCount);
+	return hipGetLastError();
+}
+
+hipError_t launchRevcomp(hipStream_t stream, const char* reads, char* rev, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, uint32_t maxLen)
+{
+	(void)maxLen;
+	hipLaunchKernelGGL(revcompKernel, dim3(nreads), dim3(256), 0, stream, reads, rev, readOff, readLen, nreads);
+	return hipGetLastError();
+}
+
+} // namespace gad
+
+namespace gad {
+
+__global__ void gatherMatchesKernel(const SeedMatch* __restrict__ matches, const uint64_t* __restrict__ readOff, const uint32_t* __restrict__ count, const uint64_t* __restrict__ outOff, uint32_t nreads, SeedMatch* __restrict__ out)
+{
+	uint32_t r = blockIdx.x;
+	if (r >= nreads) return;
+	const SeedMatch* src = matches + readOff[r];
+	SeedMatch* dst = out + outOff[r];
+	uint32_t c = count[r];
+	for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) dst[i] = src[i];
+}
+
+hipError_t launchGatherMatches(hipStream_t stream, const SeedMatch* matches, const uint64_t* readOff, const uint32_t* count, const uint64_t* outOff, uint32_t nreads, SeedMatch* out)
+{
+	hipLaunchKernelGGL(gatherMatchesKernel, dim3(nreads), dim3(256), 0, stream, matches, readOff, count, outOff, nreads, out);
+	return hipGetLastError();
 }
 
 } // namespace gad

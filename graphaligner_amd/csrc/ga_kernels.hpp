@@ -12,4 +12,11 @@ hipError_t launchExtend(hipStream_t stream, uint32_t nreq, const DevGraph& g, co
 // items scale with len; the others come from caps).
 uint64_t requestArenaBytes(uint32_t len, const ArenaCaps& caps);
 
+hipError_t launchSeedLookup(hipStream_t stream, const char* reads, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, const SeedTableSlot* table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* out, uint32_t* outCount);
+
+// Copies matches[readOff[i], + count[i]) of every read i to out[outOff[i], ...).
+hipError_t launchGatherMatches(hipStream_t stream, const SeedMatch* matches, const uint64_t* readOff, const uint32_t* count, const uint64_t* outOff, uint32_t nreads, SeedMatch* out);
+
+hipError_t launchRevcomp(hipStream_t stream, const char* reads, char* rev, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, uint32_t maxLen);
+
 } // namespace gad

@@ -42,6 +42,7 @@ Context::~Context()
 {
 	(void)hipSetDevice(device_);
 	graphs_.clear();
+	indexes_.clear();
 	if (ev0_) (void)hipEventDestroy(ev0_);
 	if (ev1_) (void)hipEventDestroy(ev1_);
 	if (stream_) (void)hipStreamDestroy(stream_);
@@ -87,9 +88,88 @@ const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 	return ref->view;
 }
 
-static uint32_t traceCapFor(uint32_t len) { return 2 * len + 256; }
+const DeviceIndex& Context::deviceIndex(const MinimizerIndex& idx)
+{
+	auto it = indexes_.find(&idx);
+	if (it != indexes_.end()) return *it->second;
+	GA_HIP(hipSetDevice(device_));
+	static_assert(sizeof(KmerSlot) == sizeof(gad::SeedTableSlot), "table slot layouts must agree");
+	auto di = std::make_unique<DeviceIndex>();
+	upload(di->table, idx.table, stream_);
+	GA_HIP(hipStreamSynchronize(stream_));
+	di->mask = idx.mask;
+	auto& ref = indexes_[&idx];
+	ref = std::move(di);
+	return *ref;
+}
 
-void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out)
+void Context::loadReads(const MinimizerIndex& idx, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, ReadBatch& out)
+{
+	GA_HIP(hipSetDevice(device_));
+	if (idx.table.empty()) throw std::runtime_error("minimizer index is empty");
+	const DeviceIndex& di = deviceIndex(idx);
+	const uint32_t n = seqs.size();
+	out.offset.resize(n);
+	out.length.assign(lens.begin(), lens.end());
+	out.bytes = 0;
+	out.maxLen = 0;
+	for (uint32_t i = 0; i < n; i++)
+	{
+		out.offset[i] = out.bytes;
+		out.bytes += lens[i];
+		out.maxLen = std::max(out.maxLen, lens[i]);
+	}
+	std::vector<char> host(out.bytes + 1);
+	for (uint32_t i = 0; i < n; i++) std::memcpy(host.data() + out.offset[i], seqs[i], lens[i]);
+	revBase_ = out.bytes + 1;
+	reads_.reserve(2 * (out.bytes + 1));
+	readOff_.reserve(std::max<size_t>(1, n) * sizeof(uint64_t));
+	readLen_.reserve(std::max<size_t>(1, n) * sizeof(uint32_t));
+	matches_.reserve((out.bytes + 1) * sizeof(gad::SeedMatch));
+	matchCount_.reserve(std::max<size_t>(1, n) * sizeof(uint32_t));
+	GA_HIP(hipMemcpyAsync(reads_.ptr, host.data(), out.bytes + 1, hipMemcpyHostToDevice, stream_));
+	if (n > 0)
+	{
+		GA_HIP(hipMemcpyAsync(readOff_.ptr, out.offset.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipMemcpyAsync(readLen_.ptr, out.length.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipEventRecord(ev0_, stream_));
+		GA_HIP(gad::launchRevcomp(stream_, reads_.as<char>(), reads_.as<char>() + revBase_, readOff_.as<uint64_t>(), readLen_.as<uint32_t>(), n, out.maxLen));
+		GA_HIP(gad::launchSeedLookup(stream_, reads_.as<char>(), readOff_.as<uint64_t>(), readLen_.as<uint32_t>(), n, di.table.as<gad::SeedTableSlot>(), di.mask, idx.k, idx.w, idx.maxCount, matches_.as<gad::SeedMatch>(), matchCount_.as<uint32_t>()));
+		GA_HIP(hipEventRecord(ev1_, stream_));
+	}
+	out.matchCount.assign(n, 0);
+	if (n > 0) GA_HIP(hipMemcpyAsync(out.matchCount.data(), matchCount_.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+	GA_HIP(hipStreamSynchronize(stream_));
+	out.lookupMs = 0;
+	if (n > 0)
+	{
+		float ms = 0;
+		GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+		out.lookupMs = ms;
+	}
+	// matches of read i sit at its byte offset; gather the used prefixes into
+	// one contiguous block on the device and copy that
+	out.matchOffset.resize(n);
+	uint64_t total = 0;
+	for (uint32_t i = 0; i < n; i++)
+	{
+		if (out.matchCount[i] > lens[i]) throw HipError("seed lookup kernel returned an inconsistent match count");
+		out.matchOffset[i] = total;
+		total += out.matchCount[i];
+	}
+	out.matches.resize(total);
+	if (total > 0)
+	{
+		gathered_.reserve(total * sizeof(gad::SeedMatch));
+		matchOff_.reserve(n * sizeof(uint64_t));
+		GA_HIP(hipMemcpyAsync(matchOff_.ptr, out.matchOffset.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(gad::launchGatherMatches(stream_, matches_.as<gad::SeedMatch>(), readOff_.as<uint64_t>(), matchCount_.as<uint32_t>(), matchOff_.as<uint64_t>(), n, gathered_.as<gad::SeedMatch>()));
+		GA_HIP(hipMemcpyAsync(out.matches.data(), gathered_.ptr, total * sizeof(gad::SeedMatch), hipMemcpyDeviceToHost, stream_));
+		GA_HIP(hipStreamSynchronize(stream_));
+	}
+}
+
+void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase)
 {
 	// split into launches whose workspaces fit the memory budget
 	size_t begin = 0;
@@ -105,12 +185,12 @@ void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const st
 			end++;
 		}
 		std::vector<uint32_t> part(which.begin() + begin, which.begin() + end);
-		runLaunch(dg, p, jobs, part, caps, wantSummaries, out);
+		runLaunch(dg, p, jobs, part, caps, wantSummaries, out, devSeqBase);
 		begin = end;
 	}
 }
 
-void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out)
+void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase)
 {
 	const uint32_t n = which.size();
 	if (n == 0) return;
@@ -120,7 +200,7 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 	for (uint32_t k = 0; k < n; k++)
 	{
 		const ExtJob& j = jobs[which[k]];
-		reqs[k].seqOffset = seqBytes;
+		reqs[k].seqOffset = devSeqBase ? j.devOffset : seqBytes;
 		reqs[k].traceOffset = traceItems;
 		reqs[k].len = j.len;
 		reqs[k].startNode = j.startNode;
@@ -133,12 +213,16 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 		sumItems += j.len / 64 + 2;
 		arenaTotal += gad::requestArenaBytes(j.len, caps);
 	}
-	std::vector<char> seqHost(seqBytes + 1);
-	for (uint32_t k = 0; k < n; k++) std::memcpy(seqHost.data() + reqs[k].seqOffset, jobs[which[k]].seq, reqs[k].len);
+	std::vector<char> seqHost;
+	if (!devSeqBase)
+	{
+		seqHost.resize(seqBytes + 1);
+		for (uint32_t k = 0; k < n; k++) std::memcpy(seqHost.data() + reqs[k].seqOffset, jobs[which[k]].seq, reqs[k].len);
+	}
 	arena_.reserve(arenaTotal);
 	arenaOffsets_.reserve(n * sizeof(uint64_t));
 	reqs_.reserve(n * sizeof(gad::ExtRequest));
-	seqs_.reserve(seqHost.size());
+	if (!devSeqBase) seqs_.reserve(seqHost.size());
 	traces_.reserve(traceItems * sizeof(uint64_t));
 	results_.reserve(n * sizeof(gad::ExtResult));
 	if (wantSummaries)
@@ -149,9 +233,9 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 	}
 	GA_HIP(hipMemcpyAsync(arenaOffsets_.ptr, arenaOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
 	GA_HIP(hipMemcpyAsync(reqs_.ptr, reqs.data(), n * sizeof(gad::ExtRequest), hipMemcpyHostToDevice, stream_));
-	GA_HIP(hipMemcpyAsync(seqs_.ptr, seqHost.data(), seqHost.size(), hipMemcpyHostToDevice, stream_));
+	if (!devSeqBase) GA_HIP(hipMemcpyAsync(seqs_.ptr, seqHost.data(), seqHost.size(), hipMemcpyHostToDevice, stream_));
 	GA_HIP(hipEventRecord(ev0_, stream_));
-	GA_HIP(gad::launchExtend(stream_, n, dg, p, reqs_.as<gad::ExtRequest>(), seqs_.as<char>(), traces_.as<uint64_t>(), results_.as<gad::ExtResult>(), arena_.as<unsigned char>(), arenaOffsets_.as<uint64_t>(), caps, wantSummaries ? sums_.as<gad::SliceSummary>() : nullptr, wantSummaries ? sumOffsets_.as<uint64_t>() : nullptr));
+	GA_HIP(gad::launchExtend(stream_, n, dg, p, reqs_.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : seqs_.as<char>(), traces_.as<uint64_t>(), results_.as<gad::ExtResult>(), arena_.as<unsigned char>(), arenaOffsets_.as<uint64_t>(), caps, wantSummaries ? sums_.as<gad::SliceSummary>() : nullptr, wantSummaries ? sumOffsets_.as<uint64_t>() : nullptr));
 	GA_HIP(hipEventRecord(ev1_, stream_));
 	std::vector<gad::ExtResult> res(n);
 	GA_HIP(hipMemcpyAsync(res.data(), results_.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, stream_));
@@ -187,7 +271,7 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 	}
 }
 
-void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out)
+void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase)
 {
 	GA_HIP(hipSetDevice(device_));
 	const gad::DevGraph& dg = deviceGraph(g);
@@ -200,24 +284,15 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 	for (uint32_t i = 0; i < jobs.size(); i++) which[i] = i;
 	// longest first: the 64 threads of a wave then carry similar lengths
 	std::stable_sort(which.begin(), which.end(), [&](uint32_t a, uint32_t b) { return jobs[a].len > jobs[b].len; });
-	gad::ArenaCaps caps;
-	caps.maxSlices = 0;   // per request
-	caps.maxItems = 48;   // band items per slice
-	caps.maxSlots = 1024;
-	caps.maxExtras = 4096;
-	caps.maxPending = 1024;
-	caps.pad = 0;
-	for (int round = 0; round < 4 && !which.empty(); round++)
+	gad::ArenaCaps caps = initialArenaCaps();
+	for (int attempt = 0; attempt < kArenaAttempts && !which.empty(); attempt++)
 	{
-		runOnce(dg, p, jobs, which, caps, wantSummaries, out);
+		runOnce(dg, p, jobs, which, caps, wantSummaries, out, devSeqBase);
 		std::vector<uint32_t> again;
 		for (uint32_t id : which)
 			if (out.results[id].status == gad::EXT_OVERFLOW) again.push_back(id);
 		which.swap(again);
-		caps.maxItems *= 8;
-		caps.maxSlots *= 8;
-		caps.maxExtras *= 8;
-		caps.maxPending *= 8;
+		growArenaCaps(caps);
 	}
 }
 

@@ -8,7 +8,9 @@
 #include <unordered_map>
 #include <vector>
 #include "ga_device_types.h"
+#include "ga_backend.hpp"
 #include "ga_graph.hpp"
+#include "ga_index.hpp"
 
 namespace ga {
 
@@ -42,28 +44,13 @@ struct DeviceGraph
 	gad::DevGraph view;
 };
 
-// One extension: align seq[0, len) from column startOffset of split node startNode.
-struct ExtJob
+struct DeviceIndex
 {
-	const char* seq;
-	uint32_t len;
-	uint32_t startNode;
-	uint32_t startOffset;
+	DeviceBuffer table;
+	uint64_t mask = 0;
 };
 
-struct ExtBatch
-{
-	std::vector<gad::ExtResult> results;
-	std::vector<uint64_t> traceOffset; // per job, into trace
-	std::vector<uint64_t> trace;       // packed items, each job's trace end first
-	std::vector<gad::SliceSummary> summaries;
-	std::vector<uint64_t> summaryOffset;
-	double kernelMs = 0;               // summed kernel time of all launches
-	uint32_t launches = 0;
-	uint64_t wordSteps = 0;
-};
-
-class Context
+class Context : public ExtendBackend
 {
 public:
 	explicit Context(int device);
@@ -72,20 +59,25 @@ public:
 	int computeUnits() const { return cus_; }
 	hipStream_t stream() const { return stream_; }
 	const gad::DevGraph& deviceGraph(const GraphHost& g);
-	// Runs all jobs; statuses EXT_OVERFLOW are rerun with larger workspaces and
-	// only remain if the largest workspace still does not fit.
-	void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out);
+	void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase = nullptr) override;
+	const DeviceIndex& deviceIndex(const MinimizerIndex& idx);
+	void loadReads(const MinimizerIndex& idx, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, ReadBatch& out) override;
+	const char* readsBase() const override { return reads_.as<char>(); }
+	uint64_t reverseBase() const override { return revBase_; }
 	uint64_t arenaBudgetBytes = 32ull << 30;
 
 private:
-	void runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out);
-	void runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out);
+	void runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase);
+	void runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase);
 	int device_;
 	int cus_ = 0;
 	hipStream_t stream_ = nullptr;
 	hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 	std::unordered_map<const GraphHost*, std::unique_ptr<DeviceGraph>> graphs_;
+	std::unordered_map<const MinimizerIndex*, std::unique_ptr<DeviceIndex>> indexes_;
 	DeviceBuffer arena_, arenaOffsets_, reqs_, seqs_, traces_, results_, sums_, sumOffsets_;
+	DeviceBuffer reads_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_;
+	uint64_t revBase_ = 0;
 };
 
 } // namespace ga

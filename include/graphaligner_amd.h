@@ -103,6 +103,53 @@ typedef struct
  * take sumCap rows per extension. */
 ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, ga_slice_summary_t* summaries, int64_t sumCap, int64_t* nslices, double* kernelMs);
 
+/* Minimizer index over the graph: MinimizerSeeder's constructor and initMaxCount
+ * (src/MinimizerSeeder.cpp:299-491, 557-575) with k = params->minimizerLength,
+ * window = params->minimizerWindowSize and the most numerous
+ * params->minimizerDiscardMostNumerousFraction of k-mers left out of seeding.
+ * Built on the host; the table is copied to a device on its first use there. */
+ga_status_t ga_index_build(const ga_graph_t* graph, const ga_params_t* params, ga_index_t** index, char* err, size_t errcap);
+void ga_index_free(ga_index_t* index);
+int64_t ga_index_max_count(const ga_index_t* index);
+int64_t ga_index_num_kmers(const ga_index_t* index);
+
+/* Seeds of n reads: MinimizerSeeder::getSeeds (src/MinimizerSeeder.cpp:521-555)
+ * with the k-mer lookup on the GPU and, when order != 0,
+ * GraphAligner::orderSeedsByChaining (src/GraphAligner.h:250-312). *seeds is a
+ * buffer from malloc (free with ga_free) holding counts[0] + ... + counts[n-1]
+ * seeds, read by read, 10 int64 each in SeedHit field order
+ * (src/GraphAlignerWrapper.h:11-50): nodeID, nodeOffset, seqPos, matchLen,
+ * reverse, alignmentGraphNodeId, alignmentGraphNodeOffset, rawSeedGoodness,
+ * seedGoodness, seedClusterSize. */
+ga_status_t ga_seeds(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, double density, int32_t order, int64_t n, const char* const* seqs, const int64_t* lens, int64_t** seeds, int64_t* counts);
+
+typedef struct
+{
+	double lookupMs;       /* device seed lookup and reverse complements */
+	double extendKernelMs; /* device seed extension, summed over launches */
+	double hostMs;         /* host seed ordering, trace fix-up, selection, GAF text */
+	double totalMs;
+	int64_t rounds;        /* seed-loop rounds (one batched extension step each) */
+	int64_t launches;
+	int64_t extensions;
+	int64_t wordSteps;     /* 64-row column steps of the bit-parallel DP */
+	int64_t seeds;
+	int64_t extendedBp;    /* read bases handed to extensions */
+} ga_align_stats_t;
+
+/* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed
+ * loop, select, GAF) for n reads with the given parameters. The seed loop of
+ * each read makes the decisions of GraphAligner::AlignOneWay
+ * (src/GraphAligner.h:94-181); all reads' extensions of one loop step run in one
+ * batched GPU launch. *gaf is a NUL-terminated buffer from malloc (free with
+ * ga_free): the GAF lines (src/GraphAlignerGAFAlignment.h:37-250) of read 0,
+ * then read 1, ... status[i]: 0 aligned, 1 no seeds, 2 a reference assertion
+ * fired and the read was dropped, 3 no alignment kept, 4 an extension exceeded
+ * the largest device workspace. stats may be NULL. */
+ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t n, const char* const* names, const char* const* seqs, const int64_t* lens, char** gaf, int32_t* status, ga_align_stats_t* stats);
+
+void ga_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,275 @@
+// Host build of the whole alignment pipeline for the CPU test suite: the
+// product's orchestrator (ga_align.cpp), index (ga_index.cpp) and graph builder
+// run unchanged over a backend that executes the device code of the gfx950
+// kernels (the one-extension-per-thread extension core and the seed lookup core)
+// on host threads. Test infrastructure only; the product library never contains it.
+#define GA_THREAD_PER_EXTENSION 1
+#define GA_THREAD_HOST_TEST 1
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+#include "../../graphaligner_amd/csrc/ga_align.hpp"
+#include "../../graphaligner_amd/csrc/ga_extend_core.h"
+#include "../../graphaligner_amd/csrc/ga_seed_core.h"
+
+namespace {
+
+template <typename F>
+void forEachParallel(size_t n, F fn)
+{
+	std::atomic<size_t> next{0};
+	unsigned t = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+	std::vector<std::thread> pool;
+	for (unsigned k = 0; k < t; k++)
+		pool.emplace_back([&]() {
+			while (true)
+			{
+				size_t i = next.fetch_add(1);
+				if (i >= n) break;
+				fn(i);
+			}
+		});
+	for (auto& th : pool) th.join();
+}
+
+class HostBackend : public ga::ExtendBackend
+{
+public:
+	void extend(const ga::GraphHost& G, const gad::ExtParams& p, const std::vector<ga::ExtJob>& jobs, bool, ga::ExtBatch& out, const char* devSeqBase) override
+	{
+		gad::DevGraph g;
+		g.nodeLen = G.nodeLen.data();
+		g.seq2 = G.seq2.data();
+		g.amb4 = G.amb4.data();
+		g.comp = G.comp.data();
+		g.linearizable = G.linearizable.data();
+		g.inOff = G.inOff.data();
+		g.inAdj = G.inAdj.data();
+		g.outOff = G.outOff.data();
+		g.outAdj = G.outAdj.data();
+		g.numNodes = G.numNodes();
+		g.firstAmbiguous = G.firstAmbiguous;
+		const size_t n = jobs.size();
+		out.results.assign(n, gad::ExtResult{});
+		out.traceOffset.assign(n, 0);
+		out.summaryOffset.assign(n, 0);
+		out.trace.clear();
+		out.summaries.clear();
+		std::vector<std::vector<uint64_t>> traces(n);
+		forEachParallel(n, [&](size_t i) {
+			const ga::ExtJob& j = jobs[i];
+			gad::ArenaCaps caps = ga::initialArenaCaps();
+			for (int attempt = 0; attempt < ga::kArenaAttempts; attempt++)
+			{
+				gad::ArenaCaps c = caps;
+				c.maxSlices = gad::requestSlices(j.len);
+				c.maxItems = gad::requestItems(j.len, caps);
+				std::vector<unsigned char> arena(gad::arenaBytes(c) + 256);
+				unsigned char* base = (unsigned char*)(((uintptr_t)arena.data() + 255) & ~(uintptr_t)255);
+				gad::ExtRequest r;
+				r.seqOffset = devSeqBase ? j.devOffset : 0;
+				r.traceOffset = 0;
+				r.len = j.len;
+				r.startNode = j.startNode;
+				r.startOffset = j.startOffset;
+				r.traceCap = ga::traceCapFor(j.len);
+				traces[i].assign(r.traceCap, 0);
+				gad::ExtResult res;
+				gad::runRequest(g, p, r, devSeqBase ? devSeqBase : j.seq, traces[i].data(), base, c, nullptr, res);
+				out.results[i] = res;
+				if (res.status != gad::EXT_OVERFLOW) break;
+				ga::growArenaCaps(caps);
+			}
+			traces[i].resize(out.results[i].status == gad::EXT_OVERFLOW ? 0 : out.results[i].ntrace);
+		});
+		for (size_t i = 0; i < n; i++)
+		{
+			out.traceOffset[i] = out.trace.size();
+			out.trace.insert(out.trace.end(), traces[i].begin(), traces[i].end());
+			out.wordSteps += out.results[i].wordSteps;
+		}
+		out.launches++;
+	}
+
+	void loadReads(const ga::MinimizerIndex& idx, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, ga::ReadBatch& out) override
+	{
+		const size_t n = seqs.size();
+		out.offset.resize(n);
+		out.length.assign(lens.begin(), lens.end());
+		out.bytes = 0;
+		out.maxLen = 0;
+		for (size_t i = 0; i < n; i++)
+		{
+			out.offset[i] = out.bytes;
+			out.bytes += lens[i];
+			out.maxLen = std::max(out.maxLen, lens[i]);
+		}
+		revBase_ = out.bytes + 1;
+		reads_.assign(2 * (out.bytes + 1), 0);
+		for (size_t i = 0; i < n; i++)
+		{
+			std::memcpy(reads_.data() + out.offset[i], seqs[i], lens[i]);
+			for (uint32_t k = 0; k < lens[i]; k++) reads_[revBase_ + out.offset[i] + k] = gad::complementByte(seqs[i][lens[i] - 1 - k]);
+		}
+		std::vector<gad::SeedMatch> all(out.bytes + 1);
+		out.matchCount.assign(n, 0);
+		const gad::SeedTableSlot* table = reinterpret_cast<const gad::SeedTableSlot*>(idx.table.data());
+		forEachParallel(n, [&](size_t i) {
+			out.matchCount[i] = gad::lookupReadKmers(reads_.data() + out.offset[i], lens[i], table, idx.mask, idx.k, idx.w, idx.maxCount, all.data() + out.offset[i]);
+		});
+		out.matchOffset.resize(n);
+		uint64_t total = 0;
+		for (size_t i = 0; i < n; i++)
+		{
+			out.matchOffset[i] = total;
+			total += out.matchCount[i];
+		}
+		out.matches.resize(total);
+		for (size_t i = 0; i < n; i++) std::memcpy(out.matches.data() + out.matchOffset[i], all.data() + out.offset[i], out.matchCount[i] * sizeof(gad::SeedMatch));
+		out.lookupMs = 0;
+	}
+
+	const char* readsBase() const override { return reads_.data(); }
+	uint64_t reverseBase() const override { return revBase_; }
+
+private:
+	std::vector<char> reads_;
+	uint64_t revBase_ = 0;
+};
+
+struct EmuIndex
+{
+	ga::MinimizerIndex idx;
+	const ga::GraphHost* g;
+};
+
+} // namespace
+
+extern "C" {
+
+void* pipe_graph_load(const char* path, char* err, int cap)
+{
+	try
+	{
+		return new ga::GraphHost(ga::loadGfaGraph(path));
+	}
+	catch (const std::exception& e)
+	{
+		std::strncpy(err, e.what(), cap - 1);
+		err[cap - 1] = 0;
+	}
+	return nullptr;
+}
+
+void pipe_graph_free(void* g) { delete (ga::GraphHost*)g; }
+
+void* pipe_index_build(void* g, int k, int w, double discard)
+{
+	return new EmuIndex{ga::buildMinimizerIndex(*(ga::GraphHost*)g, k, w, discard), (ga::GraphHost*)g};
+}
+
+void pipe_index_free(void* i) { delete (EmuIndex*)i; }
+
+int64_t pipe_index_maxcount(void* i) { return ((EmuIndex*)i)->idx.maxCount; }
+
+int64_t pipe_index_num_kmers(void* i) { return ((EmuIndex*)i)->idx.numKmers; }
+
+// Seeds of one read in the layout of the oracle's orc_get_seeds.
+int pipe_get_seeds(void* ih, const char* seq, int64_t len, double density, int order, int64_t* out, int64_t cap, int64_t* n)
+{
+	auto* x = (EmuIndex*)ih;
+	HostBackend be;
+	std::vector<std::vector<ga::SeedHit>> seeds;
+	ga::seedReads(be, *x->g, x->idx, density, order != 0, {seq}, {(uint32_t)len}, seeds, nullptr);
+	*n = seeds[0].size();
+	if ((int64_t)seeds[0].size() > cap) return 1;
+	for (size_t i = 0; i < seeds[0].size(); i++)
+	{
+		const auto& h = seeds[0][i];
+		int64_t* o = out + 10 * i;
+		o[0] = h.nodeID;
+		o[1] = h.nodeOffset;
+		o[2] = h.seqPos;
+		o[3] = h.matchLen;
+		o[4] = h.reverse;
+		o[5] = h.alignmentGraphNodeId;
+		o[6] = h.alignmentGraphNodeOffset;
+		o[7] = h.rawSeedGoodness;
+		o[8] = h.seedGoodness;
+		o[9] = h.seedClusterSize;
+	}
+	return 0;
+}
+
+// Whole pipeline; arguments follow the oracle's orc_align_reads.
+struct PipeOptions
+{
+	int32_t bandwidth;
+	int32_t errorCost;
+	int32_t xdrop;
+	int32_t sloppy;
+	int64_t maxCellsPerSlice;
+	double seedExtendDensity;
+	double multimapScoreFraction;
+	double seedDensity;
+	int64_t minClusterSize;
+};
+
+int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char* const* names, const char* const* seqs, const int64_t* lens, char** gafOut, int32_t* statusOut, int64_t* statsOut)
+{
+	auto* x = (EmuIndex*)ih;
+	ga::AlignOptions o;
+	o.ext.bandwidth = po->bandwidth;
+	o.ext.errorCost = po->errorCost;
+	o.ext.xdrop = po->xdrop;
+	o.ext.pad = 0;
+	o.ext.maxCellsPerSlice = po->maxCellsPerSlice <= 0 ? INT64_MAX : po->maxCellsPerSlice;
+	o.xscoreErrorCost = po->errorCost;
+	o.seedExtendDensity = po->seedExtendDensity;
+	o.multimapScoreFraction = po->multimapScoreFraction;
+	o.minimizerSeedDensity = po->seedDensity;
+	o.sloppyOptimizations = po->sloppy != 0;
+	o.minSeedClusterSize = po->minClusterSize;
+	o.hostThreads = 4;
+	std::vector<const char*> nm(names, names + nreads), sq(seqs, seqs + nreads);
+	std::vector<uint32_t> ln(nreads);
+	for (int64_t i = 0; i < nreads; i++) ln[i] = (uint32_t)lens[i];
+	HostBackend be;
+	std::vector<ga::ReadOutcome> out;
+	ga::AlignStats st;
+	try
+	{
+		ga::alignReads(be, *x->g, x->idx, o, nm, sq, ln, out, st);
+	}
+	catch (const std::exception&)
+	{
+		return 1;
+	}
+	size_t total = 1;
+	for (auto& r : out) total += r.gaf.size();
+	char* buf = (char*)std::malloc(total);
+	size_t pos = 0;
+	for (int64_t i = 0; i < nreads; i++)
+	{
+		std::memcpy(buf + pos, out[i].gaf.data(), out[i].gaf.size());
+		pos += out[i].gaf.size();
+		statusOut[i] = out[i].status;
+	}
+	buf[pos] = 0;
+	*gafOut = buf;
+	if (statsOut)
+	{
+		statsOut[0] = st.rounds;
+		statsOut[1] = st.extensions;
+		statsOut[2] = st.seeds;
+		statsOut[3] = st.wordSteps;
+	}
+	return 0;
+}
+
+void pipe_free(void* p) { std::free(p); }
+}

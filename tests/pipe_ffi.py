@@ -1,0 +1,109 @@
+"""ctypes binding of tests/_build/libga_pipe.so: the product's host pipeline
+(index, seeding, seed loop, selection, GAF) over a backend that runs the gfx950
+device code (extension core, seed lookup core) on host threads."""
+import ctypes
+import os
+
+import numpy as np
+
+_LIB = None
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+class PipeOptions(ctypes.Structure):
+    _fields_ = [("bandwidth", ctypes.c_int32), ("errorCost", ctypes.c_int32), ("xdrop", ctypes.c_int32),
+                ("sloppy", ctypes.c_int32), ("maxCellsPerSlice", ctypes.c_int64),
+                ("seedExtendDensity", ctypes.c_double), ("multimapScoreFraction", ctypes.c_double),
+                ("seedDensity", ctypes.c_double), ("minClusterSize", ctypes.c_int64)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        L = ctypes.CDLL(os.path.join(ROOT, "tests", "_build", "libga_pipe.so"))
+        L.pipe_graph_load.restype = ctypes.c_void_p
+        L.pipe_graph_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.pipe_graph_free.argtypes = [ctypes.c_void_p]
+        L.pipe_index_build.restype = ctypes.c_void_p
+        L.pipe_index_build.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.pipe_index_free.argtypes = [ctypes.c_void_p]
+        L.pipe_index_maxcount.restype = ctypes.c_int64
+        L.pipe_index_maxcount.argtypes = [ctypes.c_void_p]
+        L.pipe_index_num_kmers.restype = ctypes.c_int64
+        L.pipe_index_num_kmers.argtypes = [ctypes.c_void_p]
+        L.pipe_get_seeds.restype = ctypes.c_int
+        L.pipe_get_seeds.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.pipe_align_reads.restype = ctypes.c_int
+        L.pipe_align_reads.argtypes = [ctypes.c_void_p, ctypes.POINTER(PipeOptions), ctypes.c_int64,
+                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
+        L.pipe_free.argtypes = [ctypes.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+class Graph:
+    def __init__(self, path):
+        err = ctypes.create_string_buffer(512)
+        self.h = lib().pipe_graph_load(path.encode(), err, 512)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pipe_graph_free(self.h)
+            self.h = None
+
+
+class Index:
+    def __init__(self, graph, k, w, discard):
+        self.graph = graph
+        self.h = lib().pipe_index_build(graph.h, k, w, discard)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pipe_index_free(self.h)
+            self.h = None
+
+    @property
+    def max_count(self):
+        return lib().pipe_index_maxcount(self.h)
+
+    @property
+    def num_kmers(self):
+        return lib().pipe_index_num_kmers(self.h)
+
+    def seeds(self, seq, density, order=False):
+        cap = max(1024, 64 * len(seq))
+        out = np.zeros(10 * cap, dtype=np.int64)
+        n = ctypes.c_int64()
+        rc = lib().pipe_get_seeds(self.h, seq.encode(), len(seq), density, int(order),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(n))
+        assert rc == 0
+        return out[: 10 * n.value].reshape(-1, 10)
+
+
+def options(oracle_params, density):
+    p = oracle_params
+    return PipeOptions(p.bandwidth, p.errorCost, p.xdrop, p.sloppy, p.maxCellsPerSlice, p.seedExtendDensity,
+                       p.multimapScoreFraction, density, p.minClusterSize)
+
+
+def align_reads(index, oracle_params, density, names, seqs):
+    n = len(seqs)
+    na = (ctypes.c_char_p * n)(*[x.encode() for x in names])
+    sa = (ctypes.c_char_p * n)(*[x.encode() for x in seqs])
+    lens = np.array([len(s) for s in seqs], dtype=np.int64)
+    out = ctypes.c_void_p()
+    status = np.zeros(n, dtype=np.int32)
+    stats = np.zeros(4, dtype=np.int64)
+    opt = options(oracle_params, density)
+    rc = lib().pipe_align_reads(index.h, ctypes.byref(opt), n, na, sa, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                ctypes.byref(out), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                stats.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    assert rc == 0
+    text = ctypes.string_at(out.value).decode()
+    lib().pipe_free(out)
+    return text, status, dict(rounds=int(stats[0]), extensions=int(stats[1]), seeds=int(stats[2]), word_steps=int(stats[3]))

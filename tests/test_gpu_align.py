@@ -1,0 +1,78 @@
+"""Whole-read alignment on gfx950 through the C ABI (ga_index_build, ga_seeds,
+ga_align_reads) against the oracle: minimizer index, seeds, per-read status and
+the GAF text byte for byte (src/Aligner.cpp:390-606), vg and dbg presets."""
+import os
+
+import pytest
+
+import oracle_ffi as o
+import graphaligner_amd as ga
+from test_oracle_golden import GOLD, README_LINE, read_fa
+from workloads import edge_reads, workload
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return ga.Context(0)
+
+
+def _both(gfa, preset):
+    sd = o.PRESET_SEEDING[preset]
+    og = o.Graph(gfa)
+    oi = o.Index(og, sd["k"], sd["w"], sd["discard"])
+    gp = ga.params_preset(preset)
+    g = ga.Graph(gfa)
+    gi = ga.Index(g, gp)
+    assert gi.max_count == oi.max_count
+    return sd, og, oi, gp, g, gi
+
+
+def test_gpu_readme_known_answer(ctx):
+    sd, og, oi, gp, g, gi = _both(os.path.join(GOLD, "readme_graph.gfa"), "vg")
+    name, seq = read_fa(os.path.join(GOLD, "readme_read.fa"))
+    print("[gpu-align] readme", flush=True)
+    text, status, _ = ga.align_reads(ctx, g, gi, gp, [name], [seq])
+    assert list(status) == [0]
+    assert text == README_LINE + "\n"
+
+
+@pytest.mark.parametrize("wl", ["mini", "mini_sv"])
+@pytest.mark.parametrize("preset", ["vg", "dbg"])
+def test_gpu_seeds_match_oracle(ctx, workdir, wl, preset):
+    gfa, names, seqs = workload(wl, workdir)
+    sd, og, oi, gp, g, gi = _both(gfa, preset)
+    en, es = edge_reads(names, seqs)
+    reads = list(seqs[:16]) + es
+    print(f"[gpu-align] seeds {wl} {preset} reads={len(reads)}", flush=True)
+    got = ga.seeds(ctx, g, gi, reads, sd["density"], order=True)
+    for s, b in zip(reads, got):
+        a = oi.seeds(s, sd["density"], True)
+        assert a.shape == b.shape and (a == b).all()
+
+
+@pytest.mark.parametrize("wl", ["mini", "mini_sv"])
+@pytest.mark.parametrize("preset", ["vg", "dbg"])
+def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset):
+    gfa, names, seqs = workload(wl, workdir)
+    sd, og, oi, gp, g, gi = _both(gfa, preset)
+    en, es = edge_reads(names, seqs)
+    names = list(names) + en
+    seqs = list(seqs) + es
+    ot, ostat, _ = o.align_reads(og, oi, o.preset_params(preset), sd["density"], names, seqs, 8)
+    print(f"[gpu-align] gaf {wl} {preset} reads={len(seqs)}", flush=True)
+    gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
+    assert list(gstat) == list(ostat)
+    assert gt == ot
+
+
+def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir):
+    gfa, names, seqs = workload("vg100k", workdir)
+    names, seqs = names[:400], seqs[:400]
+    sd, og, oi, gp, g, gi = _both(gfa, "vg")
+    ot, ostat, _ = o.align_reads(og, oi, o.preset_params("vg"), sd["density"], names, seqs, 16)
+    print(f"[gpu-align] gaf vg100k reads={len(seqs)}", flush=True)
+    gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
+    assert list(gstat) == list(ostat)
+    assert gt == ot

@@ -47,3 +47,29 @@ def extension_cases(oracle_graph, oracle_index, seqs, n, seed=7, density=10.0):
             rc = revcomp(seq)
             cases.append((rc[len(rc) - seq_pos:], node_id * 2 + (1 - rev), osize[fwd] - node_offset - 1))
     return cases[:n]
+
+
+def edge_reads(names, seqs, seed=11):
+    """Reads exercising the per-read outcomes of src/Aligner.cpp:390-606: no seeds
+    (empty, shorter than k, random), reverse strand, lowercase, N runs, IUPAC
+    letters, a letter the complement table does not know (assertion -> dropped),
+    and a chimera of two distant reads (several alignments per read)."""
+    rng = random.Random(seed)
+    base = [s for s in seqs if len(s) > 2000][:4]
+    out = []
+
+    def add(name, s):
+        out.append((name, s))
+
+    add("empty", "")
+    add("short", "ACGTACGTAC")
+    add("random", "".join(rng.choice("ACGT") for _ in range(3000)))
+    add("revcomp", revcomp(base[0]))
+    add("lower", base[1].lower())
+    s = base[2]
+    add("nrun", s[:700] + "N" * 150 + s[850:])
+    add("iupac", s[:500] + "R" + s[501:1200] + "y" + s[1201:])
+    add("badletter", s[:900] + "X" + s[901:])
+    add("chimera", base[0][:1500] + revcomp(base[3][-1500:]))
+    add("tail_only", base[1][-200:])
+    return [n for n, _ in out], [s for _, s in out]
