@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Throughput of whole-read alignment on one MI355X.
+
+Workload: the 5 Mbp synthetic variation graph (SNP and indel bubbles, one
+structural-variant bubble of 50-5,000 bp per 20 kbp: insertions, deletions,
+inversions) with 20,000 reads of log-normal length (median 12 kbp, clipped to
+1-60 kbp) at 12% error, aligned with preset -x vg. One step aligns every read:
+device seed lookup, seed ordering, the seed-extension loop with each round's
+extensions batched into one gfx950 launch, selection and GAF text. Reads are
+resident in HBM before the timed region (ga_reads_upload).
+
+Prints one JSON line (reads/s, GCUPS, roofline of the extension kernel, and the
+CPU baseline: the C++ CPU restatement of the reference run on a sample of the
+same reads with the host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# 256 CUs x 64 int32 lanes per clock x 2.4 GHz
+VALU_INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# algorithmic work of one 64-row column step of the bit-parallel DP:
+# 42 int32-equivalent VALU ops, ~6 bytes of HBM traffic (node bases, carries,
+# band item state amortised over the node's columns)
+OPS_PER_WORD_STEP = 42
+BYTES_PER_WORD_STEP = 6
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--reads", type=int, default=0, help="use only the first N reads (0: all 20,000)")
+    ap.add_argument("--warmup-reads", type=int, default=2000, help="reads aligned by each warmup step")
+    ap.add_argument("--cpu-sample", type=int, default=1500, help="reads in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores, at most 16)")
+    ap.add_argument("--workdir", default=os.environ.get("GA_BENCH_DIR", ""))
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path, and "
+              "--gpus %d is not supported (run with --gpus 1)" % args.gpus, file=sys.stderr)
+        return 2
+    import numpy as np
+    import graphaligner_amd as ga
+    from graphaligner_amd import synth
+
+    workdir = args.workdir or tempfile.mkdtemp(prefix="ga_bench_")
+    os.makedirs(workdir, exist_ok=True)
+    prefix = os.path.join(workdir, "vg5m")
+    t = time.time()
+    if not (os.path.exists(prefix + ".gfa") and os.path.exists(prefix + ".fa")):
+        synth.generate("vg5m", prefix)
+    names, seqs = synth.read_fasta(prefix + ".fa")
+    if args.reads:
+        names, seqs = names[: args.reads], seqs[: args.reads]
+    gen_s = time.time() - t
+    print(f"[bench] workload vg5m: {len(seqs)} reads, {sum(map(len, seqs)) / 1e6:.1f} Mbp ({gen_s:.1f} s)", file=sys.stderr, flush=True)
+
+    params = ga.params_preset("vg")
+    t = time.time()
+    graph = ga.Graph(prefix + ".gfa")
+    load_s = time.time() - t
+    t = time.time()
+    index = ga.Index(graph, params)
+    index_s = time.time() - t
+    ctx = ga.Context(0)
+    ga.upload_reads(ctx, seqs)
+    ctx.synchronize()
+    print(f"[bench] graph {graph.num_nodes} split nodes ({load_s:.1f} s), index {index.num_kmers} k-mers ({index_s:.1f} s)",
+          file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        n = min(len(names), args.warmup_reads) if args.warmup_reads else len(names)
+        t = time.time()
+        ga.align_resident(ctx, graph, index, params, 0, names[:n])
+        print(f"[bench] warmup {w + 1}/{args.warmup}: {n} reads {time.time() - t:.1f} s", file=sys.stderr, flush=True)
+
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    tot = dict(extendKernelMs=0.0, lookupMs=0.0, hostMs=0.0, launches=0, extensions=0, wordSteps=0, rounds=0, seeds=0, extendedBp=0)
+    gaf = status = None
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        gaf, status, st = ga.align_resident(ctx, graph, index, params, 0, names)
+        for key in tot:
+            tot[key] += st[key]
+        print(f"[bench] step {k + 1}/{args.steps}: {time.perf_counter() - ts:.2f} s, kernel {st['extendKernelMs'] / 1e3:.2f} s, "
+              f"{st['rounds']} rounds, {st['extensions']} extensions", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    steps = max(args.steps, 1)
+    reads_per_s = len(names) * args.steps / elapsed
+    gcups = 64 * tot["wordSteps"] / elapsed / 1e9
+    kernel_s = tot["extendKernelMs"] / 1e3
+    launches = max(tot["launches"], 1)
+    # the dominant kernel: seed extension. Achieved = algorithmic bytes per launch
+    # over the launch's mean duration (HIP events on the launching stream).
+    achieved_gbs = BYTES_PER_WORD_STEP * tot["wordSteps"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    valu_tops = OPS_PER_WORD_STEP * tot["wordSteps"] / kernel_s / 1e12 if kernel_s > 0 else 0.0
+    counts = np.bincount(status, minlength=5).tolist() if status is not None else []
+
+    cpu = None
+    parity = None
+    if args.cpu_sample > 0:
+        cpu, parity = cpu_baseline(prefix, names, seqs, gaf, status, args)
+
+    line = {
+        "metric": "reads/s + GCUPS on 1 MI355X (plain GPU path)",
+        "value": reads_per_s,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "gcups": gcups,
+        "config": {"workload": "vg5m: 5 Mbp variation graph with SV bubbles, 20,000 log-normal reads (median 12 kbp) at 12% error, -x vg",
+                   "reads": len(names), "read_mbp": sum(map(len, seqs)) / 1e6, "split_nodes": graph.num_nodes,
+                   "preset": "vg", "parallelism": "1 GPU"},
+        "breakdown_s_per_step": {"extend_kernel": kernel_s / steps, "seed_lookup_kernel": tot["lookupMs"] / 1e3 / steps,
+                                 "host": tot["hostMs"] / 1e3 / steps},
+        "counters_per_step": {k: tot[k] // steps for k in ("rounds", "launches", "extensions", "wordSteps", "seeds", "extendedBp")},
+        "read_status_counts": counts,
+        "roofline": {"kernel": "extendThreadKernel", "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "mean_launch_ms": tot["extendKernelMs"] / launches,
+                     "valu_int32": {"achieved_tops": valu_tops, "peak_tops": VALU_INT32_PEAK_TOPS,
+                                    "frac": valu_tops / VALU_INT32_PEAK_TOPS}},
+        "cpu_baseline": cpu,
+        "gaf_identical_to_cpu_sample": parity,
+        "setup_s": {"generate": gen_s, "graph_load": load_s, "index_build": index_s},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(prefix, names, seqs, gaf, status, args):
+    """The oracle (C++ CPU restatement of the reference path, one read per thread
+    from a shared queue as src/Aligner.cpp:777-780) on the first reads of the same
+    workload; also checks that the GPU's GAF lines for those reads are identical."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as o
+    n = min(args.cpu_sample, len(names))
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    sd = o.PRESET_SEEDING["vg"]
+    og = o.Graph(prefix + ".gfa")
+    oi = o.Index(og, sd["k"], sd["w"], sd["discard"])
+    text, ostat, secs = o.align_reads(og, oi, o.preset_params("vg"), sd["density"], names[:n], seqs[:n], threads)
+    # GPU lines of the same reads: GAF text is read-ordered, so take the lines up
+    # to the first read outside the sample
+    gpu_lines = [ln for ln in gaf.split("\n") if ln]
+    sample = set(names[:n])
+    gpu_sample = [ln for ln in gpu_lines if ln.split("\t", 1)[0] in sample]
+    cpu_lines = [ln for ln in text.split("\n") if ln]
+    same = sum(1 for a, b in zip(cpu_lines, gpu_sample) if a == b)
+    parity = {"reads": n, "lines_cpu": len(cpu_lines), "lines_gpu": len(gpu_sample), "identical_lines": same,
+              "status_identical": bool((ostat == status[:n]).all())}
+    cpu = {"value": n / secs, "unit": "reads/s", "cores": threads, "kind": "port",
+           "sample": f"first {n} reads of the same workload, oracle -O3 with asserts, {threads} threads, {secs:.1f} s"}
+    return cpu, parity
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -18,7 +18,7 @@ STATUS_NAMES = {0: "GA_OK", 1: "GA_EXT_FAILED", 2: "GA_ASSERT", 3: "GA_EOVERFLOW
 EXPORTED = ["ga_params_preset", "ga_graph_load_gfa", "ga_graph_free", "ga_graph_num_nodes", "ga_graph_num_edges",
             "ga_graph_first_ambiguous", "ga_graph_export", "ga_context_create", "ga_context_free", "ga_extend_batch",
             "ga_index_build", "ga_index_free", "ga_index_max_count", "ga_index_num_kmers", "ga_seeds",
-            "ga_align_reads", "ga_free"]
+            "ga_align_reads", "ga_reads_upload", "ga_align_resident", "ga_context_synchronize", "ga_free"]
 
 READ_STATUS_NAMES = {0: "aligned", 1: "no seeds", 2: "assertion (read dropped)", 3: "no alignment", 4: "workspace overflow"}
 
@@ -67,6 +67,7 @@ def lib():
         L.ga_graph_export.argtypes = [vp] + [vp] * 13
         L.ga_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
         L.ga_context_free.argtypes = [vp]
+        L.ga_context_synchronize.argtypes = [vp]
         L.ga_extend_batch.argtypes = [vp, vp, ctypes.POINTER(Params), i64, ctypes.POINTER(ctypes.c_char_p), vp, vp, vp, i64, vp,
                                       vp, vp, vp, ctypes.POINTER(SliceSummary), i64, vp, ctypes.POINTER(ctypes.c_double)]
         L.ga_index_build.argtypes = [vp, ctypes.POINTER(Params), ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
@@ -79,6 +80,9 @@ def lib():
                                ctypes.POINTER(vp), vp]
         L.ga_align_reads.argtypes = [vp, vp, vp, ctypes.POINTER(Params), i64, ctypes.POINTER(ctypes.c_char_p),
                                      ctypes.POINTER(ctypes.c_char_p), vp, ctypes.POINTER(vp), vp, ctypes.POINTER(AlignStats)]
+        L.ga_reads_upload.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_char_p), vp]
+        L.ga_align_resident.argtypes = [vp, vp, vp, ctypes.POINTER(Params), i64, i64, ctypes.POINTER(ctypes.c_char_p),
+                                        ctypes.POINTER(vp), vp, ctypes.POINTER(AlignStats)]
         L.ga_free.argtypes = [vp]
         _lib = L
     return _lib

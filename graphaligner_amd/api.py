@@ -76,6 +76,9 @@ class Context:
             N.lib().ga_context_free(self.h)
             self.h = None
 
+    def synchronize(self):
+        _check(N.lib().ga_context_synchronize(self.h), "ga_context_synchronize")
+
 
 def extend_batch(ctx, graph, params, seqs, bigraph_nodes, offsets, summaries=False):
     """Returns per extension (status, score, trace[n,4] end first, summaries)."""
@@ -174,3 +177,24 @@ def align_reads(ctx, graph, index, params, names, seqs):
     N.lib().ga_free(out)
     stats = {f: getattr(st, f) for f, _ in N.AlignStats._fields_}
     return text, status[:n], stats
+
+
+def upload_reads(ctx, seqs):
+    """Makes reads resident on the context's GPU (replacing the previous set)."""
+    n, arr, lens = _reads(seqs)
+    _check(N.lib().ga_reads_upload(ctx.h, n, arr, lens.ctypes.data_as(ctypes.c_void_p)), "ga_reads_upload")
+
+
+def align_resident(ctx, graph, index, params, first, names):
+    """align_reads on resident reads [first, first + len(names))."""
+    count = len(names)
+    na = (ctypes.c_char_p * max(count, 1))(*[x.encode() for x in names])
+    out = ctypes.c_void_p()
+    status = np.zeros(max(count, 1), dtype=np.int32)
+    st = N.AlignStats()
+    _check(N.lib().ga_align_resident(ctx.h, graph.h, index.h, ctypes.byref(params), first, count, na, ctypes.byref(out),
+                                     status.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)), "ga_align_resident")
+    text = ctypes.string_at(out.value).decode()
+    N.lib().ga_free(out)
+    stats = {f: getattr(st, f) for f, _ in N.AlignStats._fields_}
+    return text, status[:count], stats

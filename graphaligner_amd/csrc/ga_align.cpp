@@ -553,12 +553,13 @@ void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds)
 
 void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs)
 {
-	ReadBatch rb;
-	ctx.loadReads(idx, seqs, lens, rb);
-	if (lookupMs) *lookupMs = rb.lookupMs;
+	ctx.uploadReads(seqs, lens);
+	SeedMatches sm;
+	ctx.lookupSeeds(idx, 0, (uint32_t)seqs.size(), sm);
+	if (lookupMs) *lookupMs = sm.lookupMs;
 	out.assign(seqs.size(), {});
 	parallelFor(seqs.size(), 16, [&](size_t i) {
-		out[i] = seedsFromMatches(g, idx, rb.matches.data() + rb.matchOffset[i], rb.matchCount[i], lens[i], density);
+		out[i] = seedsFromMatches(g, idx, sm.matches.data() + sm.matchOffset[i], sm.matchCount[i], lens[i], density);
 		if (order) orderSeedsByChaining(g, out[i]);
 	});
 }
@@ -617,23 +618,33 @@ bool advance(ReadState& st, const AlignOptions& opt)
 
 } // namespace
 
-void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const std::vector<const char*>& names, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<ReadOutcome>& out, AlignStats& stats)
+void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
 {
 	auto t0 = std::chrono::steady_clock::now();
-	const size_t n = seqs.size();
-	out.assign(n, ReadOutcome{});
+	const ReadSet& rs = ctx.reads();
+	if ((uint64_t)first + count > rs.size()) throw std::runtime_error("read range outside the resident reads");
 	stats = AlignStats{};
-	for (size_t i = 0; i < n; i++)
-		if (lens[i] > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
-	for (size_t batchStart = 0; batchStart < n; batchStart += opt.readsPerBatch)
+	for (uint32_t i = 0; i < count; i++)
 	{
-		size_t batchEnd = std::min(n, batchStart + (size_t)opt.readsPerBatch);
-		std::vector<const char*> bseqs(seqs.begin() + batchStart, seqs.begin() + batchEnd);
-		std::vector<uint32_t> blens(lens.begin() + batchStart, lens.begin() + batchEnd);
-		ReadBatch rb;
-		ctx.loadReads(idx, bseqs, blens, rb);
+		out[i] = ReadOutcome{};
+		if (rs.length[first + i] > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
+	}
+	for (uint32_t batchStart = 0; batchStart < count; batchStart += opt.readsPerBatch)
+	{
+		const uint32_t bn = std::min<uint32_t>(count - batchStart, opt.readsPerBatch);
+		const uint32_t base = first + batchStart;
+		std::vector<const char*> bseqs(bn);
+		std::vector<uint32_t> blens(bn);
+		std::vector<uint64_t> boff(bn);
+		for (uint32_t i = 0; i < bn; i++)
+		{
+			bseqs[i] = rs.seq(base + i);
+			blens[i] = rs.length[base + i];
+			boff[i] = rs.offset[base + i];
+		}
+		SeedMatches rb;
+		ctx.lookupSeeds(idx, base, bn, rb);
 		stats.lookupMs += rb.lookupMs;
-		const size_t bn = bseqs.size();
 		std::vector<ReadState> states(bn);
 		auto th = std::chrono::steady_clock::now();
 		parallelFor(bn, opt.hostThreads, [&](size_t i) {
@@ -698,7 +709,7 @@ void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& id
 					auto rp = g.reversePosition(forwardId, s.nodeOffset);
 					uint32_t node = g.unitigNode(backwardId, rp.second);
 					ExtJob j{nullptr, (uint32_t)s.seqPos, node, (uint32_t)(rp.second - g.nodeOff[node])};
-					j.devOffset = ctx.reverseBase() + rb.offset[r] + (L - s.seqPos);
+					j.devOffset = ctx.reverseBase() + boff[r] + (L - s.seqPos);
 					st.bwdJob = jobs.size();
 					jobs.push_back(j);
 				}
@@ -706,7 +717,7 @@ void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& id
 				{
 					uint32_t node = g.unitigNode(forwardId, s.nodeOffset);
 					ExtJob j{nullptr, (uint32_t)(L - s.seqPos - 1), node, (uint32_t)(s.nodeOffset - g.nodeOff[node])};
-					j.devOffset = rb.offset[r] + s.seqPos + 1;
+					j.devOffset = boff[r] + s.seqPos + 1;
 					st.fwdJob = jobs.size();
 					jobs.push_back(j);
 				}
@@ -886,6 +897,17 @@ void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& id
 		});
 		stats.hostMs += msSince(th);
 	}
+	stats.totalMs = msSince(t0);
+}
+
+void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const std::vector<const char*>& names, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<ReadOutcome>& out, AlignStats& stats)
+{
+	auto t0 = std::chrono::steady_clock::now();
+	for (uint32_t l : lens)
+		if (l > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
+	ctx.uploadReads(seqs, lens);
+	out.assign(seqs.size(), ReadOutcome{});
+	alignResident(ctx, g, idx, opt, names.data(), 0, (uint32_t)seqs.size(), out.data(), stats);
 	stats.totalMs = msSince(t0);
 }
 

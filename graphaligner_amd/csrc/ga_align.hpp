@@ -80,8 +80,12 @@ void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds);
 // Seeds of every read as the reference's seeder + OrderSeeds produce them.
 void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs);
 
-// src/Aligner.cpp:390-606 for a batch of reads (one GAF line per selected
-// alignment, '\n'-terminated, in out[i].gaf).
+// src/Aligner.cpp:390-606 for resident reads [first, first + count) of the
+// backend (one GAF line per selected alignment, '\n'-terminated, in out[i].gaf;
+// names[i] names read first + i).
+void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats);
+
+// Uploads the reads, then alignResident over all of them.
 void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const std::vector<const char*>& names, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<ReadOutcome>& out, AlignStats& stats);
 
 } // namespace ga

@@ -34,16 +34,23 @@ struct ExtBatch
 	uint64_t wordSteps = 0;
 };
 
-// Reads resident on the device for one alignment batch: forward and
-// reverse-complement bytes, and the index matches of each read.
-struct ReadBatch
+// Reads resident on the device: forward bytes (and, once looked up, their
+// reverse complements) plus a host copy for the trace fix-ups and GAF text.
+struct ReadSet
 {
 	std::vector<uint64_t> offset;  // per read, into the forward bytes
 	std::vector<uint32_t> length;
+	std::vector<char> host;        // forward bytes
 	uint64_t bytes = 0;
 	uint32_t maxLen = 0;
-	// host copies of the lookup output: matches of read i are
-	// matches[matchOffset[i], matchOffset[i] + matchCount[i])
+	size_t size() const { return offset.size(); }
+	const char* seq(size_t i) const { return host.data() + offset[i]; }
+};
+
+// Index matches of a range of resident reads: matches of read first + i are
+// matches[matchOffset[i], matchOffset[i] + matchCount[i]).
+struct SeedMatches
+{
 	std::vector<gad::SeedMatch> matches;
 	std::vector<uint64_t> matchOffset;
 	std::vector<uint32_t> matchCount;
@@ -85,11 +92,14 @@ public:
 	// Runs all jobs; statuses EXT_OVERFLOW are rerun with larger workspaces and
 	// only remain if the largest workspace still does not fit.
 	virtual void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase = nullptr) = 0;
-	// Makes the reads resident, computes their reverse complements and looks every
-	// read k-mer up in the index; the reads stay resident until the next call.
-	// Forward bytes of read i start at readsBase() + offset[i], its reverse
-	// complement at readsBase() + reverseBase() + offset[i].
-	virtual void loadReads(const MinimizerIndex& idx, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, ReadBatch& out) = 0;
+	// Makes the reads resident (replacing the previous set).
+	virtual void uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens) = 0;
+	virtual const ReadSet& reads() const = 0;
+	// For resident reads [first, first + count): writes their reverse complements
+	// and looks every read k-mer up in the index.
+	virtual void lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out) = 0;
+	// Forward bytes of resident read i start at readsBase() + offset[i], its reverse
+	// complement (once looked up) at readsBase() + reverseBase() + offset[i].
 	virtual const char* readsBase() const = 0;
 	virtual uint64_t reverseBase() const = 0;
 };

@@ -325,57 +325,62 @@ ga_status_t ga_seeds(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_
 	}
 }
 
-ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t n, const char* const* names, const char* const* seqs, const int64_t* lens, char** gaf, int32_t* status, ga_align_stats_t* stats)
+} // extern "C"
+
+namespace {
+
+AlignOptions alignOptionsOf(const ga_params_t& p)
 {
-	if (!ctx || !graph || !index || !params || !gaf || !status || index->graph != graph || !readArgsOk(n, seqs, lens) || (n > 0 && !names)) return GA_EBADARG;
-	for (int64_t i = 0; i < n; i++)
-		if (!names[i]) return GA_EBADARG;
-	*gaf = nullptr;
+	AlignOptions opt;
+	opt.ext = extParamsOf(p);
+	opt.xscoreErrorCost = errorCostOf(p);
+	opt.seedExtendDensity = p.seedExtendDensity;
+	opt.multimapScoreFraction = p.multimapScoreFraction;
+	opt.minAlignmentScore = p.minAlignmentScore;
+	opt.minimizerSeedDensity = p.minimizerSeedDensity;
+	opt.sloppyOptimizations = p.tryAllSeeds == 0;
+	opt.minSeedClusterSize = p.seedClusterMinSize < 0 ? 0 : (uint64_t)p.seedClusterMinSize;
+	opt.cigarMatchMismatchMerge = p.cigarMatchMismatchMerge != 0;
+	return opt;
+}
+
+ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& st, char** gaf, int32_t* status, ga_align_stats_t* stats)
+{
+	size_t total = 1;
+	for (auto& o : out) total += o.gaf.size();
+	char* buf = (char*)std::malloc(total);
+	if (!buf) return GA_ENOMEM;
+	size_t pos = 0;
+	for (size_t i = 0; i < out.size(); i++)
+	{
+		std::memcpy(buf + pos, out[i].gaf.data(), out[i].gaf.size());
+		pos += out[i].gaf.size();
+		status[i] = out[i].status;
+	}
+	buf[pos] = 0;
+	*gaf = buf;
+	if (stats)
+	{
+		stats->lookupMs = st.lookupMs;
+		stats->extendKernelMs = st.extendKernelMs;
+		stats->hostMs = st.hostMs;
+		stats->totalMs = st.totalMs;
+		stats->rounds = st.rounds;
+		stats->launches = st.launches;
+		stats->extensions = st.extensions;
+		stats->wordSteps = st.wordSteps;
+		stats->seeds = st.seeds;
+		stats->extendedBp = st.extendedBp;
+	}
+	return GA_OK;
+}
+
+template <typename F>
+ga_status_t guarded(F fn)
+{
 	try
 	{
-		AlignOptions opt;
-		opt.ext = extParamsOf(*params);
-		opt.xscoreErrorCost = errorCostOf(*params);
-		opt.seedExtendDensity = params->seedExtendDensity;
-		opt.multimapScoreFraction = params->multimapScoreFraction;
-		opt.minAlignmentScore = params->minAlignmentScore;
-		opt.minimizerSeedDensity = params->minimizerSeedDensity;
-		opt.sloppyOptimizations = params->tryAllSeeds == 0;
-		opt.minSeedClusterSize = params->seedClusterMinSize < 0 ? 0 : (uint64_t)params->seedClusterMinSize;
-		opt.cigarMatchMismatchMerge = params->cigarMatchMismatchMerge != 0;
-		std::vector<const char*> nm(names, names + n), s(seqs, seqs + n);
-		std::vector<uint32_t> l(n);
-		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
-		std::vector<ReadOutcome> out;
-		AlignStats st;
-		alignReads(*ctx->ctx, graph->host, index->idx, opt, nm, s, l, out, st);
-		size_t total = 1;
-		for (auto& o : out) total += o.gaf.size();
-		char* buf = (char*)std::malloc(total);
-		if (!buf) return GA_ENOMEM;
-		size_t pos = 0;
-		for (int64_t i = 0; i < n; i++)
-		{
-			std::memcpy(buf + pos, out[i].gaf.data(), out[i].gaf.size());
-			pos += out[i].gaf.size();
-			status[i] = out[i].status;
-		}
-		buf[pos] = 0;
-		*gaf = buf;
-		if (stats)
-		{
-			stats->lookupMs = st.lookupMs;
-			stats->extendKernelMs = st.extendKernelMs;
-			stats->hostMs = st.hostMs;
-			stats->totalMs = st.totalMs;
-			stats->rounds = st.rounds;
-			stats->launches = st.launches;
-			stats->extensions = st.extensions;
-			stats->wordSteps = st.wordSteps;
-			stats->seeds = st.seeds;
-			stats->extendedBp = st.extendedBp;
-		}
-		return GA_OK;
+		return fn();
 	}
 	catch (const HipError&)
 	{
@@ -389,6 +394,64 @@ ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_
 	{
 		return GA_EBADARG;
 	}
+}
+
+} // namespace
+
+extern "C" {
+
+ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t n, const char* const* names, const char* const* seqs, const int64_t* lens, char** gaf, int32_t* status, ga_align_stats_t* stats)
+{
+	if (!ctx || !graph || !index || !params || !gaf || !status || index->graph != graph || !readArgsOk(n, seqs, lens) || (n > 0 && !names)) return GA_EBADARG;
+	for (int64_t i = 0; i < n; i++)
+		if (!names[i]) return GA_EBADARG;
+	*gaf = nullptr;
+	return guarded([&]() {
+		std::vector<const char*> nm(names, names + n), s(seqs, seqs + n);
+		std::vector<uint32_t> l(n);
+		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
+		std::vector<ReadOutcome> out;
+		AlignStats st;
+		alignReads(*ctx->ctx, graph->host, index->idx, alignOptionsOf(*params), nm, s, l, out, st);
+		return packOutcomes(out, st, gaf, status, stats);
+	});
+}
+
+ga_status_t ga_context_synchronize(ga_context_t* ctx)
+{
+	if (!ctx) return GA_EBADARG;
+	return guarded([&]() {
+		GA_HIP(hipSetDevice(ctx->ctx->device()));
+		GA_HIP(hipDeviceSynchronize());
+		return GA_OK;
+	});
+}
+
+ga_status_t ga_reads_upload(ga_context_t* ctx, int64_t n, const char* const* seqs, const int64_t* lens)
+{
+	if (!ctx || !readArgsOk(n, seqs, lens)) return GA_EBADARG;
+	return guarded([&]() {
+		std::vector<const char*> s(seqs, seqs + n);
+		std::vector<uint32_t> l(n);
+		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
+		ctx->ctx->uploadReads(s, l);
+		return GA_OK;
+	});
+}
+
+ga_status_t ga_align_resident(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t first, int64_t count, const char* const* names, char** gaf, int32_t* status, ga_align_stats_t* stats)
+{
+	if (!ctx || !graph || !index || !params || !gaf || !status || index->graph != graph || first < 0 || count < 0) return GA_EBADARG;
+	if ((uint64_t)(first + count) > ctx->ctx->reads().size() || (count > 0 && !names)) return GA_EBADARG;
+	for (int64_t i = 0; i < count; i++)
+		if (!names[i]) return GA_EBADARG;
+	*gaf = nullptr;
+	return guarded([&]() {
+		std::vector<ReadOutcome> out(count);
+		AlignStats st;
+		alignResident(*ctx->ctx, graph->host, index->idx, alignOptionsOf(*params), names, (uint32_t)first, (uint32_t)count, out.data(), st);
+		return packOutcomes(out, st, gaf, status, stats);
+	});
 }
 
 void ga_free(void* p) { std::free(p); }

@@ -906,7 +906,7 @@ struct Extension
 				else if (compareHin > ib) hinP = 1;
 				uint64_t Ph, Mh;
 				Slice stepped = nextSlice(Eq0, inc, hinP, hinN, Ph, Mh);
-				wordSteps++;
+				if (!Recalc) wordSteps++;
 				if (!prevExists || before(stepped) < prev.sEnd)
 				{
 					stepped.VP &= ONES ^ 1;

@@ -103,70 +103,78 @@ const DeviceIndex& Context::deviceIndex(const MinimizerIndex& idx)
 	return *ref;
 }
 
-void Context::loadReads(const MinimizerIndex& idx, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, ReadBatch& out)
+void Context::uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens)
+{
+	GA_HIP(hipSetDevice(device_));
+	const size_t n = seqs.size();
+	ReadSet& rs = reads_;
+	rs.offset.resize(n);
+	rs.length.assign(lens.begin(), lens.end());
+	rs.bytes = 0;
+	rs.maxLen = 0;
+	for (size_t i = 0; i < n; i++)
+	{
+		rs.offset[i] = rs.bytes;
+		rs.bytes += lens[i];
+		rs.maxLen = std::max(rs.maxLen, lens[i]);
+	}
+	rs.host.assign(rs.bytes + 1, 0);
+	for (size_t i = 0; i < n; i++) std::memcpy(rs.host.data() + rs.offset[i], seqs[i], lens[i]);
+	revBase_ = rs.bytes + 1;
+	readBytes_.reserve(2 * (rs.bytes + 1));
+	readOff_.reserve(std::max<size_t>(1, n) * sizeof(uint64_t));
+	readLen_.reserve(std::max<size_t>(1, n) * sizeof(uint32_t));
+	GA_HIP(hipMemcpyAsync(readBytes_.ptr, rs.host.data(), rs.bytes + 1, hipMemcpyHostToDevice, stream_));
+	if (n > 0)
+	{
+		GA_HIP(hipMemcpyAsync(readOff_.ptr, rs.offset.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipMemcpyAsync(readLen_.ptr, rs.length.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+	}
+	GA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out)
 {
 	GA_HIP(hipSetDevice(device_));
 	if (idx.table.empty()) throw std::runtime_error("minimizer index is empty");
+	const ReadSet& rs = reads_;
+	if ((uint64_t)first + count > rs.size()) throw std::runtime_error("read range outside the resident reads");
 	const DeviceIndex& di = deviceIndex(idx);
-	const uint32_t n = seqs.size();
-	out.offset.resize(n);
-	out.length.assign(lens.begin(), lens.end());
-	out.bytes = 0;
-	out.maxLen = 0;
-	for (uint32_t i = 0; i < n; i++)
-	{
-		out.offset[i] = out.bytes;
-		out.bytes += lens[i];
-		out.maxLen = std::max(out.maxLen, lens[i]);
-	}
-	std::vector<char> host(out.bytes + 1);
-	for (uint32_t i = 0; i < n; i++) std::memcpy(host.data() + out.offset[i], seqs[i], lens[i]);
-	revBase_ = out.bytes + 1;
-	reads_.reserve(2 * (out.bytes + 1));
-	readOff_.reserve(std::max<size_t>(1, n) * sizeof(uint64_t));
-	readLen_.reserve(std::max<size_t>(1, n) * sizeof(uint32_t));
-	matches_.reserve((out.bytes + 1) * sizeof(gad::SeedMatch));
-	matchCount_.reserve(std::max<size_t>(1, n) * sizeof(uint32_t));
-	GA_HIP(hipMemcpyAsync(reads_.ptr, host.data(), out.bytes + 1, hipMemcpyHostToDevice, stream_));
-	if (n > 0)
-	{
-		GA_HIP(hipMemcpyAsync(readOff_.ptr, out.offset.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
-		GA_HIP(hipMemcpyAsync(readLen_.ptr, out.length.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
-		GA_HIP(hipEventRecord(ev0_, stream_));
-		GA_HIP(gad::launchRevcomp(stream_, reads_.as<char>(), reads_.as<char>() + revBase_, readOff_.as<uint64_t>(), readLen_.as<uint32_t>(), n, out.maxLen));
-		GA_HIP(gad::launchSeedLookup(stream_, reads_.as<char>(), readOff_.as<uint64_t>(), readLen_.as<uint32_t>(), n, di.table.as<gad::SeedTableSlot>(), di.mask, idx.k, idx.w, idx.maxCount, matches_.as<gad::SeedMatch>(), matchCount_.as<uint32_t>()));
-		GA_HIP(hipEventRecord(ev1_, stream_));
-	}
-	out.matchCount.assign(n, 0);
-	if (n > 0) GA_HIP(hipMemcpyAsync(out.matchCount.data(), matchCount_.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
-	GA_HIP(hipStreamSynchronize(stream_));
+	out.matchCount.assign(count, 0);
+	out.matchOffset.assign(count, 0);
+	out.matches.clear();
 	out.lookupMs = 0;
-	if (n > 0)
-	{
-		float ms = 0;
-		GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-		out.lookupMs = ms;
-	}
-	// matches of read i sit at its byte offset; gather the used prefixes into
-	// one contiguous block on the device and copy that
-	out.matchOffset.resize(n);
+	if (count == 0) return;
+	// matches of read i are written at its byte offset (at most one per base)
+	matches_.reserve((rs.bytes + 1) * sizeof(gad::SeedMatch));
+	matchCount_.reserve(count * sizeof(uint32_t));
+	const uint64_t* off = readOff_.as<uint64_t>() + first;
+	const uint32_t* len = readLen_.as<uint32_t>() + first;
+	GA_HIP(hipEventRecord(ev0_, stream_));
+	GA_HIP(gad::launchRevcomp(stream_, readBytes_.as<char>(), readBytes_.as<char>() + revBase_, off, len, count, 0));
+	GA_HIP(gad::launchSeedLookup(stream_, readBytes_.as<char>(), off, len, count, di.table.as<gad::SeedTableSlot>(), di.mask, idx.k, idx.w, idx.maxCount, matches_.as<gad::SeedMatch>(), matchCount_.as<uint32_t>()));
+	GA_HIP(hipEventRecord(ev1_, stream_));
+	GA_HIP(hipMemcpyAsync(out.matchCount.data(), matchCount_.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+	GA_HIP(hipStreamSynchronize(stream_));
+	float ms = 0;
+	GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+	out.lookupMs = ms;
+	// gather the used prefixes into one contiguous block and copy that
 	uint64_t total = 0;
-	for (uint32_t i = 0; i < n; i++)
+	for (uint32_t i = 0; i < count; i++)
 	{
-		if (out.matchCount[i] > lens[i]) throw HipError("seed lookup kernel returned an inconsistent match count");
+		if (out.matchCount[i] > rs.length[first + i]) throw HipError("seed lookup kernel returned an inconsistent match count");
 		out.matchOffset[i] = total;
 		total += out.matchCount[i];
 	}
 	out.matches.resize(total);
-	if (total > 0)
-	{
-		gathered_.reserve(total * sizeof(gad::SeedMatch));
-		matchOff_.reserve(n * sizeof(uint64_t));
-		GA_HIP(hipMemcpyAsync(matchOff_.ptr, out.matchOffset.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
-		GA_HIP(gad::launchGatherMatches(stream_, matches_.as<gad::SeedMatch>(), readOff_.as<uint64_t>(), matchCount_.as<uint32_t>(), matchOff_.as<uint64_t>(), n, gathered_.as<gad::SeedMatch>()));
-		GA_HIP(hipMemcpyAsync(out.matches.data(), gathered_.ptr, total * sizeof(gad::SeedMatch), hipMemcpyDeviceToHost, stream_));
-		GA_HIP(hipStreamSynchronize(stream_));
-	}
+	if (total == 0) return;
+	gathered_.reserve(total * sizeof(gad::SeedMatch));
+	matchOff_.reserve(count * sizeof(uint64_t));
+	GA_HIP(hipMemcpyAsync(matchOff_.ptr, out.matchOffset.data(), count * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+	GA_HIP(gad::launchGatherMatches(stream_, matches_.as<gad::SeedMatch>(), off, matchCount_.as<uint32_t>(), matchOff_.as<uint64_t>(), count, gathered_.as<gad::SeedMatch>()));
+	GA_HIP(hipMemcpyAsync(out.matches.data(), gathered_.ptr, total * sizeof(gad::SeedMatch), hipMemcpyDeviceToHost, stream_));
+	GA_HIP(hipStreamSynchronize(stream_));
 }
 
 void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase)

@@ -61,8 +61,10 @@ public:
 	const gad::DevGraph& deviceGraph(const GraphHost& g);
 	void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase = nullptr) override;
 	const DeviceIndex& deviceIndex(const MinimizerIndex& idx);
-	void loadReads(const MinimizerIndex& idx, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, ReadBatch& out) override;
-	const char* readsBase() const override { return reads_.as<char>(); }
+	void uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens) override;
+	const ReadSet& reads() const override { return reads_; }
+	void lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out) override;
+	const char* readsBase() const override { return readBytes_.as<char>(); }
 	uint64_t reverseBase() const override { return revBase_; }
 	uint64_t arenaBudgetBytes = 32ull << 30;
 
@@ -76,7 +78,8 @@ private:
 	std::unordered_map<const GraphHost*, std::unique_ptr<DeviceGraph>> graphs_;
 	std::unordered_map<const MinimizerIndex*, std::unique_ptr<DeviceIndex>> indexes_;
 	DeviceBuffer arena_, arenaOffsets_, reqs_, seqs_, traces_, results_, sums_, sumOffsets_;
-	DeviceBuffer reads_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_;
+	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_;
+	ReadSet reads_;
 	uint64_t revBase_ = 0;
 };
 

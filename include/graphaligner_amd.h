@@ -75,6 +75,8 @@ ga_status_t ga_graph_export(const ga_graph_t* graph, int64_t* nodeLength, int64_
 /* One GPU (HIP device ordinal), one stream, resident graph copies. */
 ga_status_t ga_context_create(int device, ga_context_t** ctx, char* err, size_t errcap);
 void ga_context_free(ga_context_t* ctx);
+/* Waits until all work on the context's device has finished. */
+ga_status_t ga_context_synchronize(ga_context_t* ctx);
 
 /* Per-slice words of one extension's DP table, for parity checks. */
 typedef struct
@@ -147,6 +149,14 @@ typedef struct
  * fired and the read was dropped, 3 no alignment kept, 4 an extension exceeded
  * the largest device workspace. stats may be NULL. */
 ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t n, const char* const* names, const char* const* seqs, const int64_t* lens, char** gaf, int32_t* status, ga_align_stats_t* stats);
+
+/* Reads resident on the context's GPU: ga_reads_upload copies n reads into HBM
+ * (and keeps a host copy for trace fix-ups and GAF text), replacing the previous
+ * set; ga_align_resident then runs ga_align_reads' per-read pipeline on resident
+ * reads [first, first + count) with no host-to-device read transfer. names[i]
+ * names read first + i; gaf and status are as in ga_align_reads. */
+ga_status_t ga_reads_upload(ga_context_t* ctx, int64_t n, const char* const* seqs, const int64_t* lens);
+ga_status_t ga_align_resident(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t first, int64_t count, const char* const* names, char** gaf, int32_t* status, ga_align_stats_t* stats);
 
 void ga_free(void* p);
 

@@ -95,41 +95,48 @@ public:
 		out.launches++;
 	}
 
-	void loadReads(const ga::MinimizerIndex& idx, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, ga::ReadBatch& out) override
+	void uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens) override
 	{
 		const size_t n = seqs.size();
-		out.offset.resize(n);
-		out.length.assign(lens.begin(), lens.end());
-		out.bytes = 0;
-		out.maxLen = 0;
+		rs_.offset.resize(n);
+		rs_.length.assign(lens.begin(), lens.end());
+		rs_.bytes = 0;
+		rs_.maxLen = 0;
 		for (size_t i = 0; i < n; i++)
 		{
-			out.offset[i] = out.bytes;
-			out.bytes += lens[i];
-			out.maxLen = std::max(out.maxLen, lens[i]);
+			rs_.offset[i] = rs_.bytes;
+			rs_.bytes += lens[i];
+			rs_.maxLen = std::max(rs_.maxLen, lens[i]);
 		}
-		revBase_ = out.bytes + 1;
-		reads_.assign(2 * (out.bytes + 1), 0);
-		for (size_t i = 0; i < n; i++)
-		{
-			std::memcpy(reads_.data() + out.offset[i], seqs[i], lens[i]);
-			for (uint32_t k = 0; k < lens[i]; k++) reads_[revBase_ + out.offset[i] + k] = gad::complementByte(seqs[i][lens[i] - 1 - k]);
-		}
-		std::vector<gad::SeedMatch> all(out.bytes + 1);
-		out.matchCount.assign(n, 0);
+		rs_.host.assign(rs_.bytes + 1, 0);
+		for (size_t i = 0; i < n; i++) std::memcpy(rs_.host.data() + rs_.offset[i], seqs[i], lens[i]);
+		revBase_ = rs_.bytes + 1;
+		reads_.assign(2 * (rs_.bytes + 1), 0);
+		std::memcpy(reads_.data(), rs_.host.data(), rs_.bytes + 1);
+	}
+
+	const ga::ReadSet& reads() const override { return rs_; }
+
+	void lookupSeeds(const ga::MinimizerIndex& idx, uint32_t first, uint32_t count, ga::SeedMatches& out) override
+	{
+		std::vector<gad::SeedMatch> all(rs_.bytes + 1);
+		out.matchCount.assign(count, 0);
 		const gad::SeedTableSlot* table = reinterpret_cast<const gad::SeedTableSlot*>(idx.table.data());
-		forEachParallel(n, [&](size_t i) {
-			out.matchCount[i] = gad::lookupReadKmers(reads_.data() + out.offset[i], lens[i], table, idx.mask, idx.k, idx.w, idx.maxCount, all.data() + out.offset[i]);
+		forEachParallel(count, [&](size_t i) {
+			uint64_t off = rs_.offset[first + i];
+			uint32_t len = rs_.length[first + i];
+			for (uint32_t k = 0; k < len; k++) reads_[revBase_ + off + k] = gad::complementByte(reads_[off + len - 1 - k]);
+			out.matchCount[i] = gad::lookupReadKmers(reads_.data() + off, len, table, idx.mask, idx.k, idx.w, idx.maxCount, all.data() + off);
 		});
-		out.matchOffset.resize(n);
+		out.matchOffset.resize(count);
 		uint64_t total = 0;
-		for (size_t i = 0; i < n; i++)
+		for (size_t i = 0; i < count; i++)
 		{
 			out.matchOffset[i] = total;
 			total += out.matchCount[i];
 		}
 		out.matches.resize(total);
-		for (size_t i = 0; i < n; i++) std::memcpy(out.matches.data() + out.matchOffset[i], all.data() + out.offset[i], out.matchCount[i] * sizeof(gad::SeedMatch));
+		for (size_t i = 0; i < count; i++) std::memcpy(out.matches.data() + out.matchOffset[i], all.data() + rs_.offset[first + i], out.matchCount[i] * sizeof(gad::SeedMatch));
 		out.lookupMs = 0;
 	}
 
@@ -137,6 +144,7 @@ public:
 	uint64_t reverseBase() const override { return revBase_; }
 
 private:
+	ga::ReadSet rs_;
 	std::vector<char> reads_;
 	uint64_t revBase_ = 0;
 };
