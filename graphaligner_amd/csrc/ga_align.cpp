@@ -485,6 +485,8 @@ std::vector<SeedHit> seedsFromMatches(const GraphHost& g, const MinimizerIndex& 
 	std::vector<SeedHit> result;
 	uint64_t seedsHere = 0, allowedCount = 0;
 	for (const auto& m : order)
+		if ((uint64_t)m.start + m.count > idx.positions.size()) throw std::runtime_error("seed match outside the index's position array");
+	for (const auto& m : order)
 	{
 		uint64_t count = m.count;
 		if (seedsHere >= maxHits && count > allowedCount) break;

@@ -2,6 +2,7 @@
 #include "ga_graph.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <fstream>
 #include <functional>
 #include <map>
@@ -687,6 +688,12 @@ std::pair<int, uint64_t> GraphHost::reversePosition(int id, uint64_t offset) con
 	return {rev, size - offset - 1};
 }
 
+uint64_t nextObjectUid()
+{
+	static std::atomic<uint64_t> next{1};
+	return next.fetch_add(1);
+}
+
 GraphHost loadGfaGraph(const std::string& path)
 {
 	std::ifstream file{path};
@@ -746,6 +753,7 @@ GraphHost loadGfaGraph(const std::string& path)
 	f.chains();
 
 	GraphHost g;
+	g.uid = nextObjectUid();
 	size_t n = b.len.size();
 	g.firstAmbiguous = firstAmb;
 	g.dbgOverlap = gfa.edgeOverlap;

@@ -14,8 +14,14 @@
 
 namespace ga {
 
+// Process-unique, never reused identity of a loaded graph or built index (device
+// copies are cached by it, so a new object at a freed one's address is not
+// mistaken for it).
+uint64_t nextObjectUid();
+
 struct GraphHost
 {
+	uint64_t uid = 0;
 	// split nodes (<= 64 bp), indexed 0..N-1; ambiguous (IUPAC) nodes are numbered last
 	std::vector<uint32_t> nodeLen;
 	std::vector<int32_t> nodeId;       // bigraph node id: 2*id (+) or 2*id+1 (-)

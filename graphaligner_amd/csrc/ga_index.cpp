@@ -146,6 +146,7 @@ MinimizerIndex buildMinimizerIndex(const GraphHost& g, uint32_t k, uint32_t w, d
 {
 	if (k < 1 || k > 31 || w < k) throw std::runtime_error("minimizer length must be 1..31 and not above the window size");
 	MinimizerIndex idx;
+	idx.uid = nextObjectUid();
 	idx.k = k;
 	idx.w = w;
 	// graph minimizers are kept only from the first split piece that has an

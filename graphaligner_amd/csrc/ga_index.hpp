@@ -19,6 +19,7 @@ struct KmerSlot
 
 struct MinimizerIndex
 {
+	uint64_t uid = 0;
 	uint32_t k = 15, w = 20;
 	uint64_t maxCount = 0;
 	uint64_t mask = 0;                // table size - 1 (power of two)

@@ -57,10 +57,14 @@ static void upload(DeviceBuffer& buf, const std::vector<T>& v, hipStream_t s)
 
 const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 {
-	auto it = graphs_.find(&g);
-	if (it != graphs_.end()) return it->second->view;
+	if (g.uid == 0) throw std::runtime_error("graph has no identity (not built by loadGfaGraph)");
+	for (auto& c : graphs_)
+		if (c->uid == g.uid) return c->view;
 	GA_HIP(hipSetDevice(device_));
+	GA_HIP(hipStreamSynchronize(stream_));
+	if (graphs_.size() >= kCachedObjects) graphs_.erase(graphs_.begin());
 	auto dg = std::make_unique<DeviceGraph>();
+	dg->uid = g.uid;
 	upload(dg->nodeLen, g.nodeLen, stream_);
 	upload(dg->seq2, g.seq2, stream_);
 	upload(dg->amb4, g.amb4, stream_);
@@ -83,24 +87,26 @@ const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 	v.outAdj = dg->outAdj.as<uint32_t>();
 	v.numNodes = g.numNodes();
 	v.firstAmbiguous = g.firstAmbiguous;
-	auto& ref = graphs_[&g];
-	ref = std::move(dg);
-	return ref->view;
+	graphs_.push_back(std::move(dg));
+	return graphs_.back()->view;
 }
 
 const DeviceIndex& Context::deviceIndex(const MinimizerIndex& idx)
 {
-	auto it = indexes_.find(&idx);
-	if (it != indexes_.end()) return *it->second;
+	if (idx.uid == 0) throw std::runtime_error("index has no identity (not built by buildMinimizerIndex)");
+	for (auto& c : indexes_)
+		if (c->uid == idx.uid) return *c;
 	GA_HIP(hipSetDevice(device_));
+	GA_HIP(hipStreamSynchronize(stream_));
+	if (indexes_.size() >= kCachedObjects) indexes_.erase(indexes_.begin());
 	static_assert(sizeof(KmerSlot) == sizeof(gad::SeedTableSlot), "table slot layouts must agree");
 	auto di = std::make_unique<DeviceIndex>();
+	di->uid = idx.uid;
 	upload(di->table, idx.table, stream_);
 	GA_HIP(hipStreamSynchronize(stream_));
 	di->mask = idx.mask;
-	auto& ref = indexes_[&idx];
-	ref = std::move(di);
-	return *ref;
+	indexes_.push_back(std::move(di));
+	return *indexes_.back();
 }
 
 void Context::uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens)

@@ -40,12 +40,14 @@ struct DeviceBuffer
 
 struct DeviceGraph
 {
+	uint64_t uid = 0;
 	DeviceBuffer nodeLen, seq2, amb4, comp, lin, inOff, inAdj, outOff, outAdj;
 	gad::DevGraph view;
 };
 
 struct DeviceIndex
 {
+	uint64_t uid = 0;
 	DeviceBuffer table;
 	uint64_t mask = 0;
 };
@@ -75,8 +77,10 @@ private:
 	int cus_ = 0;
 	hipStream_t stream_ = nullptr;
 	hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-	std::unordered_map<const GraphHost*, std::unique_ptr<DeviceGraph>> graphs_;
-	std::unordered_map<const MinimizerIndex*, std::unique_ptr<DeviceIndex>> indexes_;
+	// device copies of the most recently used graphs and indexes, by uid
+	static constexpr size_t kCachedObjects = 2;
+	std::vector<std::unique_ptr<DeviceGraph>> graphs_;
+	std::vector<std::unique_ptr<DeviceIndex>> indexes_;
 	DeviceBuffer arena_, arenaOffsets_, reqs_, seqs_, traces_, results_, sums_, sumOffsets_;
 	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_;
 	ReadSet reads_;
