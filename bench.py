@@ -43,11 +43,28 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1500, help="reads in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores, at most 16)")
     ap.add_argument("--workdir", default=os.environ.get("GA_BENCH_DIR", ""))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes (traffic-counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "graphaligner_amd", "_build", "libga_calib.so"))
+    lib.ga_calib_copy.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    ms = ctypes.c_double()
+    rc = lib.ga_calib_copy(nbytes, 3, ctypes.byref(ms))
+    if rc != 0:
+        print(f"copy calibration failed ({rc})", file=sys.stderr)
+        return 1
+    print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms.value, "gbs": 2 * nbytes / ms.value / 1e6}))
+    return 0
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path, and "
               "--gpus %d is not supported (run with --gpus 1)" % args.gpus, file=sys.stderr)

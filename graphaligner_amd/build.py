@@ -33,6 +33,12 @@ def build_product():
     os.replace(tmp, os.path.join(OUT, "libgraphaligner_amd.so"))
 
 
+def build_calibration():
+    """Copy kernel of known size for calibrating the HBM traffic counters."""
+    os.makedirs(OUT, exist_ok=True)
+    _run(["hipcc"] + HIP_FLAGS + ["-o", os.path.join(OUT, "libga_calib.so"), os.path.join(CSRC, "ga_calib.hip")])
+
+
 def build_synth():
     os.makedirs(OUT, exist_ok=True)
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", os.path.join(OUT, "libga_synth.so"),
@@ -56,6 +62,7 @@ def build_test_infrastructure():
 
 def build_all():
     build_synth()
+    build_calibration()
     build_test_infrastructure()
     build_product()
 

@@ -94,3 +94,23 @@ hipError_t launchGatherMatches(hipStream_t stream, const SeedMatch* matches, con
 }
 
 } // namespace gad
+
+namespace gad {
+
+__global__ void gatherU64Kernel(const uint64_t* __restrict__ src, const uint64_t* __restrict__ srcOff, const uint32_t* __restrict__ count, const uint64_t* __restrict__ dstOff, uint32_t nranges, uint64_t* __restrict__ dst)
+{
+	uint32_t r = blockIdx.x;
+	if (r >= nranges) return;
+	const uint64_t* s = src + srcOff[r];
+	uint64_t* d = dst + dstOff[r];
+	uint32_t c = count[r];
+	for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) d[i] = s[i];
+}
+
+hipError_t launchGatherU64(hipStream_t stream, const uint64_t* src, const uint64_t* srcOff, const uint32_t* count, const uint64_t* dstOff, uint32_t nranges, uint64_t* dst)
+{
+	hipLaunchKernelGGL(gatherU64Kernel, dim3(nranges), dim3(256), 0, stream, src, srcOff, count, dstOff, nranges, dst);
+	return hipGetLastError();
+}
+
+} // namespace gad

@@ -17,6 +17,9 @@ hipError_t launchSeedLookup(hipStream_t stream, const char* reads, const uint64_
 // Copies matches[readOff[i], + count[i]) of every read i to out[outOff[i], ...).
 hipError_t launchGatherMatches(hipStream_t stream, const SeedMatch* matches, const uint64_t* readOff, const uint32_t* count, const uint64_t* outOff, uint32_t nreads, SeedMatch* out);
 
+// Copies src[srcOff[i], + count[i]) to dst[dstOff[i], ...) for every range i.
+hipError_t launchGatherU64(hipStream_t stream, const uint64_t* src, const uint64_t* srcOff, const uint32_t* count, const uint64_t* dstOff, uint32_t nranges, uint64_t* dst);
+
 hipError_t launchRevcomp(hipStream_t stream, const char* reads, char* rev, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, uint32_t maxLen);
 
 } // namespace gad

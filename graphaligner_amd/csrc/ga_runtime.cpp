@@ -253,8 +253,6 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 	GA_HIP(hipEventRecord(ev1_, stream_));
 	std::vector<gad::ExtResult> res(n);
 	GA_HIP(hipMemcpyAsync(res.data(), results_.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, stream_));
-	std::vector<uint64_t> tr(traceItems);
-	GA_HIP(hipMemcpyAsync(tr.data(), traces_.ptr, traceItems * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
 	std::vector<gad::SliceSummary> sm;
 	if (wantSummaries)
 	{
@@ -266,6 +264,12 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 	GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
 	out.kernelMs += ms;
 	out.launches++;
+	// traces: gather the used prefix of every kept job's trace region into one
+	// contiguous block on the device, then copy only that block
+	std::vector<uint64_t> srcOff, dstOff;
+	std::vector<uint32_t> cnt;
+	const uint64_t base = out.trace.size();
+	uint64_t total = 0;
 	for (uint32_t k = 0; k < n; k++)
 	{
 		uint32_t jid = which[k];
@@ -275,13 +279,34 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 		out.results[jid] = r;
 		if (r.status == gad::EXT_OVERFLOW) continue;
 		out.wordSteps += r.wordSteps;
-		out.traceOffset[jid] = out.trace.size();
-		out.trace.insert(out.trace.end(), tr.begin() + reqs[k].traceOffset, tr.begin() + reqs[k].traceOffset + r.ntrace);
+		out.traceOffset[jid] = base + total;
+		if (r.ntrace > 0)
+		{
+			srcOff.push_back(reqs[k].traceOffset);
+			dstOff.push_back(total);
+			cnt.push_back(r.ntrace);
+			total += r.ntrace;
+		}
 		if (wantSummaries)
 		{
 			out.summaryOffset[jid] = out.summaries.size();
 			out.summaries.insert(out.summaries.end(), sm.begin() + sumOff[k], sm.begin() + sumOff[k] + r.nslices);
 		}
+	}
+	out.trace.resize(base + total);
+	if (total > 0)
+	{
+		const uint32_t m = cnt.size();
+		gatherSrc_.reserve(m * sizeof(uint64_t));
+		gatherDst_.reserve(m * sizeof(uint64_t));
+		gatherCnt_.reserve(m * sizeof(uint32_t));
+		traceOut_.reserve(total * sizeof(uint64_t));
+		GA_HIP(hipMemcpyAsync(gatherSrc_.ptr, srcOff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipMemcpyAsync(gatherDst_.ptr, dstOff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipMemcpyAsync(gatherCnt_.ptr, cnt.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(gad::launchGatherU64(stream_, traces_.as<uint64_t>(), gatherSrc_.as<uint64_t>(), gatherCnt_.as<uint32_t>(), gatherDst_.as<uint64_t>(), m, traceOut_.as<uint64_t>()));
+		GA_HIP(hipMemcpyAsync(out.trace.data() + base, traceOut_.ptr, total * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+		GA_HIP(hipStreamSynchronize(stream_));
 	}
 }
 

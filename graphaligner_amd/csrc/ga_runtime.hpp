@@ -82,6 +82,7 @@ private:
 	std::vector<std::unique_ptr<DeviceGraph>> graphs_;
 	std::vector<std::unique_ptr<DeviceIndex>> indexes_;
 	DeviceBuffer arena_, arenaOffsets_, reqs_, seqs_, traces_, results_, sums_, sumOffsets_;
+	DeviceBuffer gatherSrc_, gatherDst_, gatherCnt_, traceOut_;
 	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_;
 	ReadSet reads_;
 	uint64_t revBase_ = 0;
