@@ -450,30 +450,6 @@ std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, cons
 	return s.str();
 }
 
-// Seed loop state of one read (GraphAligner::AlignOneWay, src/GraphAligner.h:94-181).
-struct ReadState
-{
-	uint32_t read = 0;
-	std::vector<SeedHit> seeds;
-	size_t next = 0;
-	std::vector<Alignment> alignments;
-	uint64_t seedScoreForEndToEndAln = 0;
-	uint64_t worstExtendedSeedScore = 0;
-	uint64_t seedsExtended = 0;
-	uint64_t extendSeeds = 0;
-	bool done = false;
-	int status = READ_ALIGNED;
-	// seed being extended this round and its device jobs (-1: none)
-	size_t pending = 0;
-	int64_t bwdJob = -1, fwdJob = -1;
-};
-
-struct Job
-{
-	uint32_t state;
-	bool backward;
-};
-
 } // namespace
 
 std::vector<SeedHit> seedsFromMatches(const GraphHost& g, const MinimizerIndex& idx, const gad::SeedMatch* matches, uint32_t nmatches, uint64_t readLen, double density)
@@ -568,54 +544,67 @@ void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx
 
 namespace {
 
-// Picks the next seed of the read's loop to extend (src/GraphAligner.h:105-140);
-// returns false when the loop is over.
-bool advance(ReadState& st, const AlignOptions& opt)
+// getAlignmentFromSeed (src/GraphAligner.h:631-692) rebuilt from the two raw
+// traces the device seed loop kept (either may be empty: that direction failed
+// or did not run): the checks of getTwoDirectionalTrace, verifyTrace, the
+// fix-ups with their score recount, the merge and the X-score.
+Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char* seq, uint64_t L, const SeedHit& s, const uint64_t* bwdRaw, uint32_t nb, const uint64_t* fwdRaw, uint32_t nf)
 {
-	while (st.next < st.seeds.size())
+	OneWay bwd, fwd;
+	const int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
+	const int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
+	const auto rp = g.reversePosition(forwardId, s.nodeOffset);
+	if (nb > 0)
 	{
-		const SeedHit& s = st.seeds[st.next];
-		if (opt.sloppyOptimizations && s.seedGoodness <= st.seedScoreForEndToEndAln) return false;
-		if (st.seedsExtended >= st.extendSeeds && s.seedGoodness < st.worstExtendedSeedScore) return false;
-		if (s.seedClusterSize < opt.minSeedClusterSize)
-		{
-			st.next++;
-			continue;
-		}
-		bool covered = false;
-		if (opt.sloppyOptimizations)
-		{
-			for (const auto& a : st.alignments)
-			{
-				if (a.alignmentStart <= s.seqPos && a.alignmentEnd >= s.seqPos && a.seedGoodness > s.seedGoodness)
-				{
-					covered = true;
-					break;
-				}
-			}
-		}
-		if (!covered)
-		{
-			for (const auto& a : st.alignments)
-			{
-				if (exactAlignmentPart(a, s))
-				{
-					covered = true;
-					break;
-				}
-			}
-		}
-		if (covered)
-		{
-			st.next++;
-			continue;
-		}
-		st.worstExtendedSeedScore = s.seedGoodness;
-		st.seedsExtended++;
-		st.pending = st.next;
-		return true;
+		std::string revPart(s.seqPos, 0);
+		for (uint64_t k = 0; k < s.seqPos; k++) revPart[k] = complementOf(seq[s.seqPos - 1 - k]);
+		bwd = decodeTrace(g, bwdRaw, nb, 0, revPart.data(), revPart.size());
 	}
-	return false;
+	if (nf > 0) fwd = decodeTrace(g, fwdRaw, nf, 0, seq + s.seqPos + 1, L - s.seqPos - 1);
+	if (!bwd.failed())
+	{
+		const TItem& b = bwd.trace.back();
+		check(b.seqPos == kNoPos && g.nodeId[b.node] == backwardId && g.nodeOff[b.node] + b.nodeOffset == rp.second);
+		std::reverse(bwd.trace.begin(), bwd.trace.end());
+	}
+	if (!fwd.failed())
+	{
+		const TItem& f = fwd.trace.back();
+		check(f.seqPos == kNoPos && g.nodeId[f.node] == forwardId && g.nodeOff[f.node] + f.nodeOffset == s.nodeOffset);
+		std::reverse(fwd.trace.begin(), fwd.trace.end());
+	}
+	if (!fwd.trace.empty()) verifyTrace(g, fwd.trace);
+	if (!bwd.trace.empty()) verifyTrace(g, bwd.trace);
+	fixReverse(g, bwd, s.seqPos - 1, seq, L);
+	fixForward(g, fwd, s.seqPos + 1, seq, L);
+	Alignment item;
+	if (fwd.failed() && bwd.failed()) return item;
+	OneWay merged = std::move(bwd);
+	if (merged.failed())
+	{
+		merged = std::move(fwd);
+	}
+	else if (!fwd.failed())
+	{
+		const TItem& x = merged.trace.back();
+		const TItem& y = fwd.trace[0];
+		check(x.node == y.node && x.nodeOffset == y.nodeOffset && x.seqPos == y.seqPos);
+		merged.trace.pop_back();
+		merged.trace.insert(merged.trace.end(), fwd.trace.begin(), fwd.trace.end());
+		merged.score += fwd.score;
+	}
+	check(!merged.trace.empty());
+	uint64_t seqStart = merged.trace[0].seqPos;
+	uint64_t seqEnd = merged.trace.back().seqPos;
+	check(seqEnd < L);
+	item.traceScore = merged.score;
+	item.trace = std::move(merged.trace);
+	item.alignmentScore = (uint64_t)(int64_t)item.traceScore;
+	item.alignmentStart = seqStart;
+	item.alignmentEnd = seqEnd + 1;
+	item.alignmentXScore = (int32_t)item.alignmentLength() * 100 - opt.xscoreErrorCost * (int32_t)item.alignmentScore;
+	item.alignmentXScore /= 100.0;
+	return item;
 }
 
 } // namespace
@@ -631,250 +620,131 @@ void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex&
 		out[i] = ReadOutcome{};
 		if (rs.length[first + i] > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
 	}
+	gad::ReadLoopParams rp;
+	rp.ext = opt.ext;
+	rp.minSeedClusterSize = opt.minSeedClusterSize;
+	rp.sloppy = opt.sloppyOptimizations ? 1 : 0;
+	rp.pad = 0;
 	for (uint32_t batchStart = 0; batchStart < count; batchStart += opt.readsPerBatch)
 	{
 		const uint32_t bn = std::min<uint32_t>(count - batchStart, opt.readsPerBatch);
 		const uint32_t base = first + batchStart;
-		std::vector<const char*> bseqs(bn);
-		std::vector<uint32_t> blens(bn);
-		std::vector<uint64_t> boff(bn);
-		for (uint32_t i = 0; i < bn; i++)
-		{
-			bseqs[i] = rs.seq(base + i);
-			blens[i] = rs.length[base + i];
-			boff[i] = rs.offset[base + i];
-		}
 		SeedMatches rb;
 		ctx.lookupSeeds(idx, base, bn, rb);
 		stats.lookupMs += rb.lookupMs;
-		std::vector<ReadState> states(bn);
 		auto th = std::chrono::steady_clock::now();
+		// seeds, seed order and the read checks that precede the seed loop
+		std::vector<std::vector<SeedHit>> seeds(bn);
+		std::vector<uint8_t> ready(bn, 0);
 		parallelFor(bn, opt.hostThreads, [&](size_t i) {
-			ReadState& st = states[i];
-			st.read = (uint32_t)i;
 			ReadOutcome& o = out[batchStart + i];
+			const char* seq = rs.seq(base + i);
+			const uint32_t L = rs.length[base + i];
 			try
 			{
-				st.seeds = seedsFromMatches(g, idx, rb.matches.data() + rb.matchOffset[i], rb.matchCount[i], blens[i], opt.minimizerSeedDensity);
-				if (st.seeds.empty())
+				seeds[i] = seedsFromMatches(g, idx, rb.matches.data() + rb.matchOffset[i], rb.matchCount[i], L, opt.minimizerSeedDensity);
+				if (seeds[i].empty())
 				{
 					o.status = READ_NO_SEEDS;
-					st.done = true;
 					return;
 				}
-				orderSeedsByChaining(g, st.seeds);
+				orderSeedsByChaining(g, seeds[i]);
 				// ReverseComplement of the read asserts on letters it does not know
-				for (uint32_t k = 0; k < blens[i]; k++) check(complementOf(bseqs[i][k]) != 0);
-				st.extendSeeds = opt.seedExtendDensity == -1 ? st.seeds.size() : (uint64_t)(opt.seedExtendDensity * blens[i] + 1);
+				for (uint32_t k = 0; k < L; k++) check(complementOf(seq[k]) != 0);
+				ready[i] = 1;
 			}
 			catch (const ReadAssert&)
 			{
-				st.status = READ_ASSERT;
-				st.done = true;
+				o.status = READ_ASSERT;
+			}
+		});
+		std::vector<ReadLoopJob> jobs;
+		std::vector<uint32_t> jobRead;
+		uint64_t totalSeeds = 0;
+		for (uint32_t i = 0; i < bn; i++)
+		{
+			stats.seeds += seeds[i].size();
+			if (!ready[i]) continue;
+			ReadLoopJob j;
+			j.read = base + i;
+			j.seedCount = (uint32_t)seeds[i].size();
+			j.seedStart = totalSeeds;
+			const uint32_t L = rs.length[base + i];
+			j.extendSeeds = opt.seedExtendDensity == -1 ? seeds[i].size() : (uint64_t)(opt.seedExtendDensity * L + 1);
+			totalSeeds += seeds[i].size();
+			jobs.push_back(j);
+			jobRead.push_back(i);
+		}
+		std::vector<gad::DevSeed> devSeeds(totalSeeds);
+		parallelFor(jobs.size(), opt.hostThreads, [&](size_t k) {
+			const uint32_t i = jobRead[k];
+			gad::DevSeed* d = devSeeds.data() + jobs[k].seedStart;
+			for (size_t q = 0; q < seeds[i].size(); q++)
+			{
+				const SeedHit& s = seeds[i][q];
+				const int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
+				const int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
+				gad::DevSeed& e = d[q];
+				e.seedGoodness = s.seedGoodness;
+				e.clusterSize = s.seedClusterSize;
+				e.seqPos = (uint32_t)s.seqPos;
+				e.nodeID = s.nodeID;
+				e.nodeOffset = (uint32_t)s.nodeOffset;
+				e.reverse = s.reverse ? 1 : 0;
+				// getInitialSliceExactPosition's split node for both directions
+				const uint32_t fn = (uint32_t)s.alignmentGraphNodeId;
+				e.fwdNode = fn;
+				e.fwdOff = (uint32_t)(s.nodeOffset - g.nodeOff[fn]);
+				const auto rpos = g.reversePosition(forwardId, s.nodeOffset);
+				const uint32_t bnode = g.unitigNode(backwardId, rpos.second);
+				e.bwdNode = bnode;
+				e.bwdOff = (uint32_t)(rpos.second - g.nodeOff[bnode]);
 			}
 		});
 		stats.hostMs += msSince(th);
-		for (auto& st : states) stats.seeds += st.seeds.size();
-		std::vector<uint32_t> active;
-		for (uint32_t i = 0; i < bn; i++)
-			if (!states[i].done) active.push_back(i);
-		while (!active.empty())
-		{
-			th = std::chrono::steady_clock::now();
-			std::vector<uint8_t> extendNow(active.size(), 0);
-			parallelFor(active.size(), opt.hostThreads, [&](size_t a) {
-				ReadState& st = states[active[a]];
-				try
-				{
-					extendNow[a] = advance(st, opt) ? 1 : 0;
-				}
-				catch (const ReadAssert&)
-				{
-					st.status = READ_ASSERT;
-				}
-				if (!extendNow[a]) st.done = true;
-			});
-			std::vector<ExtJob> jobs;
-			std::vector<uint32_t> extending;
-			for (size_t a = 0; a < active.size(); a++)
-			{
-				if (!extendNow[a]) continue;
-				uint32_t r = active[a];
-				ReadState& st = states[r];
-				const SeedHit& s = st.seeds[st.pending];
-				const uint64_t L = blens[r];
-				int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
-				int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
-				st.bwdJob = st.fwdJob = -1;
-				if (s.seqPos > 0)
-				{
-					auto rp = g.reversePosition(forwardId, s.nodeOffset);
-					uint32_t node = g.unitigNode(backwardId, rp.second);
-					ExtJob j{nullptr, (uint32_t)s.seqPos, node, (uint32_t)(rp.second - g.nodeOff[node])};
-					j.devOffset = ctx.reverseBase() + boff[r] + (L - s.seqPos);
-					st.bwdJob = jobs.size();
-					jobs.push_back(j);
-				}
-				if (s.seqPos < L - 1)
-				{
-					uint32_t node = g.unitigNode(forwardId, s.nodeOffset);
-					ExtJob j{nullptr, (uint32_t)(L - s.seqPos - 1), node, (uint32_t)(s.nodeOffset - g.nodeOff[node])};
-					j.devOffset = boff[r] + s.seqPos + 1;
-					st.fwdJob = jobs.size();
-					jobs.push_back(j);
-				}
-				extending.push_back(r);
-			}
-			stats.hostMs += msSince(th);
-			if (extending.empty()) break;
-			ExtBatch eb;
-			ctx.extend(g, opt.ext, jobs, false, eb, ctx.readsBase());
-			stats.rounds++;
-			stats.launches += eb.launches;
-			stats.extensions += jobs.size();
-			stats.extendKernelMs += eb.kernelMs;
-			stats.wordSteps += eb.wordSteps;
-			for (const auto& j : jobs) stats.extendedBp += j.len;
-			th = std::chrono::steady_clock::now();
-			parallelFor(extending.size(), opt.hostThreads, [&](size_t a) {
-				ReadState& st = states[extending[a]];
-				const uint32_t r = extending[a];
-				const char* seq = bseqs[r];
-				const uint64_t L = blens[r];
-				const SeedHit s = st.seeds[st.pending];
-				st.next = st.pending + 1;
-				try
-				{
-					// getTwoDirectionalTrace (src/GraphAligner.h:475-520)
-					OneWay bwd, fwd;
-					std::string revPart;
-					int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
-					int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
-					auto rp = g.reversePosition(forwardId, s.nodeOffset);
-					for (int dir = 0; dir < 2; dir++)
-					{
-						int64_t j = dir == 0 ? st.bwdJob : st.fwdJob;
-						if (j < 0) continue;
-						const gad::ExtResult& res = eb.results[j];
-						if (res.status == gad::EXT_OVERFLOW)
-						{
-							st.status = READ_OVERFLOW;
-							st.done = true;
-							return;
-						}
-						check(res.status != gad::EXT_ASSERT);
-						OneWay t;
-						if (res.status == gad::EXT_OK)
-						{
-							if (dir == 0)
-							{
-								revPart.resize(s.seqPos);
-								for (uint64_t k = 0; k < s.seqPos; k++) revPart[k] = complementOf(seq[s.seqPos - 1 - k]);
-								t = decodeTrace(g, eb.trace.data() + eb.traceOffset[j], res.ntrace, res.score, revPart.data(), revPart.size());
-							}
-							else
-							{
-								t = decodeTrace(g, eb.trace.data() + eb.traceOffset[j], res.ntrace, res.score, seq + s.seqPos + 1, L - s.seqPos - 1);
-							}
-							check(!t.trace.empty());
-						}
-						(dir == 0 ? bwd : fwd) = std::move(t);
-					}
-					if (!bwd.failed())
-					{
-						const TItem& b = bwd.trace.back();
-						check(b.seqPos == kNoPos && g.nodeId[b.node] == backwardId && g.nodeOff[b.node] + b.nodeOffset == rp.second);
-						std::reverse(bwd.trace.begin(), bwd.trace.end());
-					}
-					if (!fwd.failed())
-					{
-						const TItem& f = fwd.trace.back();
-						check(f.seqPos == kNoPos && g.nodeId[f.node] == forwardId && g.nodeOff[f.node] + f.nodeOffset == s.nodeOffset);
-						std::reverse(fwd.trace.begin(), fwd.trace.end());
-					}
-					// getAlignmentFromSeed (src/GraphAligner.h:631-692)
-					if (!fwd.trace.empty()) verifyTrace(g, fwd.trace);
-					if (!bwd.trace.empty()) verifyTrace(g, bwd.trace);
-					fixReverse(g, bwd, s.seqPos - 1, seq, L);
-					fixForward(g, fwd, s.seqPos + 1, seq, L);
-					if (fwd.failed() && bwd.failed()) return;
-					OneWay merged = std::move(bwd);
-					if (merged.failed())
-					{
-						merged = std::move(fwd);
-					}
-					else if (!fwd.failed())
-					{
-						const TItem& x = merged.trace.back();
-						const TItem& y = fwd.trace[0];
-						check(x.node == y.node && x.nodeOffset == y.nodeOffset && x.seqPos == y.seqPos);
-						merged.trace.pop_back();
-						merged.trace.insert(merged.trace.end(), fwd.trace.begin(), fwd.trace.end());
-						merged.score += fwd.score;
-					}
-					check(!merged.trace.empty());
-					Alignment item;
-					uint64_t seqStart = merged.trace[0].seqPos;
-					uint64_t seqEnd = merged.trace.back().seqPos;
-					check(seqEnd < L);
-					item.traceScore = merged.score;
-					item.trace = std::move(merged.trace);
-					item.alignmentScore = (uint64_t)(int64_t)item.traceScore;
-					item.alignmentStart = seqStart;
-					item.alignmentEnd = seqEnd + 1;
-					item.alignmentXScore = (int32_t)item.alignmentLength() * 100 - opt.xscoreErrorCost * (int32_t)item.alignmentScore;
-					item.alignmentXScore /= 100.0;
-					if (item.alignmentFailed()) return;
-					item.seedGoodness = s.seedGoodness;
-					st.alignments.push_back(std::move(item));
-					if (opt.sloppyOptimizations)
-					{
-						auto& al = st.alignments;
-						std::sort(al.begin(), al.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
-						if (al[0].alignmentStart == 0)
-						{
-							uint64_t minGoodness = al[0].seedGoodness;
-							uint64_t contiguousEnd = al[0].alignmentEnd;
-							for (size_t k = 1; k < al.size(); k++)
-							{
-								if (al[k].alignmentStart <= contiguousEnd)
-								{
-									minGoodness = std::min(minGoodness, al[k].seedGoodness);
-									contiguousEnd = std::max(contiguousEnd, al[k].alignmentEnd);
-								}
-							}
-							if (contiguousEnd == L) st.seedScoreForEndToEndAln = minGoodness;
-						}
-					}
-				}
-				catch (const ReadAssert&)
-				{
-					st.status = READ_ASSERT;
-					st.done = true;
-				}
-			});
-			stats.hostMs += msSince(th);
-			std::vector<uint32_t> still;
-			for (uint32_t r : active)
-				if (!states[r].done) still.push_back(r);
-			active.swap(still);
-		}
-		// selection and GAF (src/Aligner.cpp:502-606)
+		ReadLoopOutput rlo;
+		ctx.runReadLoops(g, rp, devSeeds, jobs, rlo);
+		stats.rounds += rlo.launches;
+		stats.launches += rlo.launches;
+		stats.extensions += rlo.extensions;
+		stats.extendKernelMs += rlo.kernelMs;
+		stats.wordSteps += rlo.wordSteps;
 		th = std::chrono::steady_clock::now();
-		parallelFor(bn, opt.hostThreads, [&](size_t i) {
-			ReadState& st = states[i];
+		// alignments in creation order, the sloppy re-sort after each one, then
+		// selection, sorts and GAF (src/GraphAligner.h:141-177, src/Aligner.cpp:502-606)
+		parallelFor(jobs.size(), opt.hostThreads, [&](size_t k) {
+			const uint32_t i = jobRead[k];
 			ReadOutcome& o = out[batchStart + i];
-			if (o.status == READ_NO_SEEDS) return;
-			o.seedsExtended = st.seedsExtended;
-			if (st.status != READ_ALIGNED)
+			const gad::ReadResult& r = rlo.results[k];
+			o.seedsExtended = r.seedsExtended;
+			if (r.status == gad::READLOOP_ASSERT)
 			{
-				o.status = st.status;
+				o.status = READ_ASSERT;
 				return;
 			}
-			std::vector<Alignment> sel;
+			if (r.status != gad::READLOOP_DONE)
+			{
+				o.status = READ_OVERFLOW;
+				return;
+			}
+			const char* seq = rs.seq(base + i);
+			const uint64_t L = rs.length[base + i];
 			try
 			{
-				if (!st.alignments.empty()) sel = selectAlignments(st.alignments, opt.minAlignmentScore, opt.multimapScoreFraction);
+				std::vector<Alignment> al;
+				for (uint32_t a = 0; a < r.nAlignments; a++)
+				{
+					const gad::DevAlignment& da = rlo.alns[rlo.alnOffset[k] + a];
+					const uint64_t* raw = rlo.pool.data() + rlo.poolOffset[k] + da.poolOffset;
+					const SeedHit& s = seeds[i][da.seedIndex];
+					Alignment item = buildAlignment(g, opt, seq, L, s, raw, da.nb, raw + da.nb, da.nf);
+					if (item.alignmentFailed()) continue;
+					item.seedGoodness = s.seedGoodness;
+					al.push_back(std::move(item));
+					if (opt.sloppyOptimizations) std::sort(al.begin(), al.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
+				}
+				std::vector<Alignment> sel;
+				if (!al.empty()) sel = selectAlignments(al, opt.minAlignmentScore, opt.multimapScoreFraction);
 				if (sel.empty())
 				{
 					o.status = READ_NO_ALIGNMENT;
@@ -885,7 +755,7 @@ void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex&
 				std::string text;
 				for (const auto& a : sel)
 				{
-					text += gafLine(g, names[batchStart + i], blens[i], a, opt.cigarMatchMismatchMerge);
+					text += gafLine(g, names[batchStart + i], L, a, opt.cigarMatchMismatchMerge);
 					text += '\n';
 				}
 				o.gaf = std::move(text);

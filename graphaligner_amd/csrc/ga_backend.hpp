@@ -85,6 +85,57 @@ inline void growArenaCaps(gad::ArenaCaps& c)
 // per read base plus one per graph base).
 inline uint32_t traceCapFor(uint32_t len) { return 2 * len + 256; }
 
+// One read for the device seed loop: its resident index, its ordered seeds
+// (seeds[seedStart, seedStart + seedCount)) and its extension budget.
+struct ReadLoopJob
+{
+	uint32_t read;
+	uint32_t seedCount;
+	uint64_t seedStart;
+	uint64_t extendSeeds;
+};
+
+struct ReadLoopOutput
+{
+	std::vector<gad::ReadResult> results;   // per job
+	std::vector<uint64_t> alnOffset;         // per job, into alns
+	std::vector<gad::DevAlignment> alns;     // creation order per job
+	std::vector<uint64_t> poolOffset;        // per job, into pool
+	std::vector<uint64_t> pool;              // raw trace items
+	double kernelMs = 0;
+	uint32_t launches = 0;
+	uint64_t wordSteps = 0, extensions = 0;
+};
+
+// Workspace capacities of the device seed loop; grown like the arena on overflow.
+struct ReadLoopCaps
+{
+	gad::ArenaCaps arena;
+	uint32_t poolPerBase;  // alignment trace pool items per read base
+	uint32_t poolExtra;
+	uint32_t alnCap;       // alignment records per read
+};
+inline ReadLoopCaps initialReadLoopCaps()
+{
+	ReadLoopCaps c;
+	c.arena = initialArenaCaps();
+	c.poolPerBase = 6;
+	c.poolExtra = 4096;
+	c.alnCap = 32;
+	return c;
+}
+inline void growReadLoopCaps(ReadLoopCaps& c)
+{
+	growArenaCaps(c.arena);
+	c.poolPerBase *= 4;
+	c.alnCap *= 4;
+}
+inline uint32_t poolCapFor(uint32_t len, const ReadLoopCaps& c)
+{
+	uint64_t v = (uint64_t)len * c.poolPerBase + c.poolExtra;
+	return v > 0xffffffffull ? 0xffffffffu : (uint32_t)v;
+}
+
 class ExtendBackend
 {
 public:
@@ -102,6 +153,10 @@ public:
 	// complement (once looked up) at readsBase() + reverseBase() + offset[i].
 	virtual const char* readsBase() const = 0;
 	virtual uint64_t reverseBase() const = 0;
+	// Runs the seed loop of every job's read (the reads' reverse complements must
+	// be in place, i.e. lookupSeeds ran on them); jobs whose workspaces overflow
+	// are rerun with larger ones, kArenaAttempts attempts in all.
+	virtual void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) = 0;
 };
 
 } // namespace ga

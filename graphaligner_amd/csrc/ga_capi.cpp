@@ -370,7 +370,6 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& 
 		stats->extensions = st.extensions;
 		stats->wordSteps = st.wordSteps;
 		stats->seeds = st.seeds;
-		stats->extendedBp = st.extendedBp;
 	}
 	return GA_OK;
 }

@@ -18,6 +18,9 @@ struct DevGraph
 	const uint32_t* inAdj;
 	const uint32_t* outOff;
 	const uint32_t* outAdj;
+	const int32_t* nodeId;   // bigraph node id of each split node
+	const uint32_t* nodeOff; // offset of the split node in its bigraph node
+	const uint32_t* bigSize; // length of the split node's bigraph node
 	uint32_t numNodes;
 	uint32_t firstAmbiguous;
 };
@@ -131,6 +134,78 @@ struct SeedMatch
 	uint32_t pos;
 	uint32_t start;
 	uint32_t count;
+};
+
+// One seed of the device seed loop: the SeedHit fields the loop reads
+// (src/GraphAligner.h:94-181) plus the split-node start of each extension.
+struct DevSeed
+{
+	uint64_t seedGoodness;
+	uint64_t clusterSize;
+	uint32_t seqPos;
+	int32_t nodeID;      // bigraph node id / 2
+	uint32_t nodeOffset; // offset in the bigraph node
+	uint32_t reverse;
+	uint32_t bwdNode, bwdOff; // backward extension start (split node, offset)
+	uint32_t fwdNode, fwdOff; // forward extension start
+};
+
+struct ReadLoopParams
+{
+	ExtParams ext;
+	uint64_t minSeedClusterSize;
+	uint32_t sloppy;
+	uint32_t pad;
+};
+
+// One read's seed loop on the device. Offsets locate its bytes, seeds and
+// workspaces; capacities are item / record counts.
+struct ReadJob
+{
+	uint64_t fwdOffset;   // forward bytes in the read buffer
+	uint64_t revOffset;   // reverse-complement bytes in the read buffer
+	uint64_t seedStart;   // into the seed array
+	uint64_t arenaOffset; // extension workspace (bytes)
+	uint64_t traceOffset; // two raw trace regions of traceCap items each
+	uint64_t poolOffset;  // alignment trace pool (items)
+	uint64_t alnOffset;   // alignment records
+	uint64_t extendSeeds; // seed extension budget (AlignOneWay's extendSeeds)
+	uint32_t len;
+	uint32_t seedCount;
+	uint32_t traceCap;
+	uint32_t poolCap;
+	uint32_t alnCap;
+	uint32_t pad;
+};
+
+// One alignment made by the device seed loop, in creation order: its trace is
+// the backward extension's raw trace (nb items, end first) followed by the
+// forward extension's raw trace (nf items, end first) in the read's pool.
+struct DevAlignment
+{
+	uint64_t seedGoodness;
+	uint32_t poolOffset; // relative to the read's pool
+	uint32_t seedIndex;  // in the read's ordered seeds
+	uint32_t start, end; // read interval [start, end)
+	uint32_t nb, nf;
+};
+
+enum ReadLoopStatus : int32_t
+{
+	READLOOP_DONE = 0,
+	READLOOP_ASSERT = 2,   // a reference assertion fired: the read is dropped
+	READLOOP_OVERFLOW = 4, // a workspace capacity was exceeded: rerun larger
+};
+
+struct ReadResult
+{
+	int32_t status;
+	uint32_t nAlignments;
+	uint32_t seedsExtended;
+	uint32_t poolUsed;
+	uint64_t wordSteps;
+	uint32_t extensions;
+	uint32_t overflowWhat;
 };
 
 } // namespace gad

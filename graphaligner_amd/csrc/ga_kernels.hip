@@ -1,16 +1,22 @@
 // gfx950 kernels: seed extension with one extension per thread. Requests are
-// sorted longest first so the 64 threads of a wave carry similar lengths.
+// sorted longest first so the threads of a wave carry similar lengths; a wave
+// runs lanesPerWave requests (its other lanes exit at once), so a launch with
+// few requests spreads them over many SIMDs instead of serialising them in the
+// divergent lanes of a few waves.
 #define GA_THREAD_PER_EXTENSION 1
 #include <hip/hip_runtime.h>
 #include "ga_extend_core.h"
+#include "ga_read_core.h"
 #include "ga_kernels.hpp"
 #include "ga_seed_core.h"
 
 namespace gad {
 
-__global__ void __launch_bounds__(64) extendThreadKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets)
+template <int MinWavesPerSimd>
+__global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, uint32_t lanesPerWave, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets)
 {
-	uint32_t idx = blockIdx.x * 64 + threadIdx.x;
+	if (threadIdx.x >= lanesPerWave) return;
+	uint32_t idx = blockIdx.x * lanesPerWave + threadIdx.x;
 	if (idx >= nreq) return;
 	ExtRequest r = reqs[idx];
 	ArenaCaps c = caps;
@@ -21,10 +27,46 @@ __global__ void __launch_bounds__(64) extendThreadKernel(DevGraph g, ExtParams p
 	results[idx] = out;
 }
 
-hipError_t launchExtend(hipStream_t stream, uint32_t nreq, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets)
+hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets)
 {
-	uint32_t blocks = (nreq + 63) / 64;
-	hipLaunchKernelGGL(extendThreadKernel, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
+	uint32_t blocks = (nreq + lanesPerWave - 1) / lanesPerWave;
+	// register budget variants: more resident waves per SIMD for fewer VGPRs
+	if (wavesPerSimd >= 8)
+		hipLaunchKernelGGL(extendThreadKernel<8>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+	else if (wavesPerSimd >= 4)
+		hipLaunchKernelGGL(extendThreadKernel<4>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+	else
+		hipLaunchKernelGGL(extendThreadKernel<1>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+	return hipGetLastError();
+}
+
+// One read's whole seed loop per wave (lane 0): extensions of one read follow
+// each other with no host round trip, and reads are independent, so the
+// hardware dispatcher balances the uneven per-read work across the chip.
+template <int MinWavesPerSimd>
+__global__ void __launch_bounds__(64, MinWavesPerSimd) readLoopKernel(DevGraph g, ReadLoopParams p, const ReadJob* __restrict__ jobs, uint32_t njobs, uint32_t lanesPerWave, const char* __restrict__ reads, const DevSeed* __restrict__ seeds, uint64_t* __restrict__ rawTraces, uint64_t* __restrict__ pools, DevAlignment* __restrict__ alns, ReadResult* __restrict__ results, unsigned char* __restrict__ arenas, ArenaCaps caps)
+{
+	if (threadIdx.x >= lanesPerWave) return;
+	uint32_t idx = blockIdx.x * lanesPerWave + threadIdx.x;
+	if (idx >= njobs) return;
+	const ReadJob j = jobs[idx];
+	ArenaCaps c = caps;
+	c.maxSlices = requestSlices(j.len);
+	c.maxItems = requestItems(j.len, caps);
+	ReadResult out;
+	runReadLoop(g, p, j, reads, seeds + j.seedStart, rawTraces + j.traceOffset, pools + j.poolOffset, alns + j.alnOffset, arenas + j.arenaOffset, c, out);
+	results[idx] = out;
+}
+
+hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, const ArenaCaps& caps)
+{
+	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
+	uint32_t blocks = (njobs + lanesPerWave - 1) / lanesPerWave;
+	if (wavesPerSimd >= 4)
+		hipLaunchKernelGGL(readLoopKernel<4>, dim3(blocks), dim3(64), 0, stream, g, p, jobs, njobs, lanesPerWave, reads, seeds, rawTraces, pools, alns, results, arenas, caps);
+	else
+		hipLaunchKernelGGL(readLoopKernel<1>, dim3(blocks), dim3(64), 0, stream, g, p, jobs, njobs, lanesPerWave, reads, seeds, rawTraces, pools, alns, results, arenas, caps);
 	return hipGetLastError();
 }
 

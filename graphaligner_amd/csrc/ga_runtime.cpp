@@ -2,6 +2,7 @@
 #include "ga_runtime.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include "ga_kernels.hpp"
 
@@ -23,8 +24,24 @@ void DeviceBuffer::reserve(size_t b)
 	bytes = want;
 }
 
+uint32_t Context::lanesPerWave(uint32_t nreq) const
+{
+	(void)nreq;
+	return lanesOverride_ > 0 ? lanesOverride_ : defaultLanesPerWave;
+}
+
 Context::Context(int device) : device_(device)
 {
+	if (const char* v = std::getenv("GA_LANES_PER_WAVE"))
+	{
+		int x = std::atoi(v);
+		if (x >= 1 && x <= 64) lanesOverride_ = (uint32_t)x;
+	}
+	if (const char* v = std::getenv("GA_KERNEL_WAVES_PER_SIMD"))
+	{
+		int x = std::atoi(v);
+		if (x >= 1 && x <= 8) kernelWavesPerSimd = (uint32_t)x;
+	}
 	int count = 0;
 	hipError_t e = hipGetDeviceCount(&count);
 	if (e != hipSuccess || count == 0) throw HipError("no HIP device available: the GPU path needs an MI355X (gfx950)");
@@ -74,6 +91,11 @@ const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 	upload(dg->inAdj, g.inAdj, stream_);
 	upload(dg->outOff, g.outOff, stream_);
 	upload(dg->outAdj, g.outAdj, stream_);
+	upload(dg->nodeId, g.nodeId, stream_);
+	upload(dg->nodeOff, g.nodeOff, stream_);
+	std::vector<uint32_t> bigSize(g.numNodes());
+	for (size_t i = 0; i < g.numNodes(); i++) bigSize[i] = (uint32_t)g.originalSize.at(g.nodeId[i]);
+	upload(dg->bigSize, bigSize, stream_);
 	GA_HIP(hipStreamSynchronize(stream_));
 	auto& v = dg->view;
 	v.nodeLen = dg->nodeLen.as<uint32_t>();
@@ -85,6 +107,9 @@ const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 	v.inAdj = dg->inAdj.as<uint32_t>();
 	v.outOff = dg->outOff.as<uint32_t>();
 	v.outAdj = dg->outAdj.as<uint32_t>();
+	v.nodeId = dg->nodeId.as<int32_t>();
+	v.nodeOff = dg->nodeOff.as<uint32_t>();
+	v.bigSize = dg->bigSize.as<uint32_t>();
 	v.numNodes = g.numNodes();
 	v.firstAmbiguous = g.firstAmbiguous;
 	graphs_.push_back(std::move(dg));
@@ -249,7 +274,7 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 	GA_HIP(hipMemcpyAsync(reqs_.ptr, reqs.data(), n * sizeof(gad::ExtRequest), hipMemcpyHostToDevice, stream_));
 	if (!devSeqBase) GA_HIP(hipMemcpyAsync(seqs_.ptr, seqHost.data(), seqHost.size(), hipMemcpyHostToDevice, stream_));
 	GA_HIP(hipEventRecord(ev0_, stream_));
-	GA_HIP(gad::launchExtend(stream_, n, dg, p, reqs_.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : seqs_.as<char>(), traces_.as<uint64_t>(), results_.as<gad::ExtResult>(), arena_.as<unsigned char>(), arenaOffsets_.as<uint64_t>(), caps, wantSummaries ? sums_.as<gad::SliceSummary>() : nullptr, wantSummaries ? sumOffsets_.as<uint64_t>() : nullptr));
+	GA_HIP(gad::launchExtend(stream_, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, reqs_.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : seqs_.as<char>(), traces_.as<uint64_t>(), results_.as<gad::ExtResult>(), arena_.as<unsigned char>(), arenaOffsets_.as<uint64_t>(), caps, wantSummaries ? sums_.as<gad::SliceSummary>() : nullptr, wantSummaries ? sumOffsets_.as<uint64_t>() : nullptr));
 	GA_HIP(hipEventRecord(ev1_, stream_));
 	std::vector<gad::ExtResult> res(n);
 	GA_HIP(hipMemcpyAsync(res.data(), results_.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, stream_));
@@ -333,6 +358,163 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 		which.swap(again);
 		growArenaCaps(caps);
 	}
+}
+
+static uint64_t roundUp256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
+
+static uint64_t readLoopBytes(uint32_t len, const ReadLoopCaps& caps)
+{
+	return roundUp256(gad::requestArenaBytes(len, caps.arena)) + 2ull * traceCapFor(len) * 8 + (uint64_t)poolCapFor(len, caps) * 8 + (uint64_t)caps.alnCap * sizeof(gad::DevAlignment);
+}
+
+void Context::runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out)
+{
+	static_assert(sizeof(gad::DevAlignment) == 32, "alignment records are gathered as four 64-bit words");
+	GA_HIP(hipSetDevice(device_));
+	const gad::DevGraph& dg = deviceGraph(g);
+	const size_t n = jobs.size();
+	out = ReadLoopOutput{};
+	out.results.assign(n, gad::ReadResult{});
+	out.alnOffset.assign(n, 0);
+	out.poolOffset.assign(n, 0);
+	if (n == 0) return;
+	for (const auto& j : jobs)
+	{
+		if (j.read >= reads_.size()) throw std::runtime_error("read loop job names a read that is not resident");
+		if (j.seedStart + j.seedCount > seeds.size()) throw std::runtime_error("read loop job seeds out of range");
+	}
+	rlSeeds_.reserve(std::max<size_t>(1, seeds.size()) * sizeof(gad::DevSeed));
+	if (!seeds.empty()) GA_HIP(hipMemcpyAsync(rlSeeds_.ptr, seeds.data(), seeds.size() * sizeof(gad::DevSeed), hipMemcpyHostToDevice, stream_));
+	std::vector<uint32_t> which(n);
+	for (uint32_t i = 0; i < n; i++) which[i] = i;
+	// longest reads first: the dispatcher starts the longest seed loops earliest
+	std::stable_sort(which.begin(), which.end(), [&](uint32_t a, uint32_t b) { return reads_.length[jobs[a].read] > reads_.length[jobs[b].read]; });
+	ReadLoopCaps caps = initialReadLoopCaps();
+	for (int attempt = 0; attempt < kArenaAttempts && !which.empty(); attempt++)
+	{
+		size_t begin = 0;
+		while (begin < which.size())
+		{
+			uint64_t bytes = 0;
+			size_t end = begin;
+			while (end < which.size())
+			{
+				uint64_t b = readLoopBytes(reads_.length[jobs[which[end]].read], caps);
+				if (end > begin && bytes + b > arenaBudgetBytes) break;
+				bytes += b;
+				end++;
+			}
+			std::vector<uint32_t> part(which.begin() + begin, which.begin() + end);
+			runReadLoopLaunch(dg, p, jobs, part, caps, out);
+			begin = end;
+		}
+		std::vector<uint32_t> again;
+		for (uint32_t id : which)
+			if (out.results[id].status == gad::READLOOP_OVERFLOW) again.push_back(id);
+		which.swap(again);
+		growReadLoopCaps(caps);
+	}
+}
+
+void Context::runReadLoopLaunch(const gad::DevGraph& dg, const gad::ReadLoopParams& p, const std::vector<ReadLoopJob>& jobs, const std::vector<uint32_t>& which, const ReadLoopCaps& caps, ReadLoopOutput& out)
+{
+	const uint32_t n = which.size();
+	if (n == 0) return;
+	std::vector<gad::ReadJob> rj(n);
+	uint64_t arenaTotal = 0, traceTotal = 0, poolTotal = 0, alnTotal = 0;
+	for (uint32_t k = 0; k < n; k++)
+	{
+		const ReadLoopJob& j = jobs[which[k]];
+		const uint32_t len = reads_.length[j.read];
+		gad::ReadJob& r = rj[k];
+		r.fwdOffset = reads_.offset[j.read];
+		r.revOffset = revBase_ + reads_.offset[j.read];
+		r.seedStart = j.seedStart;
+		r.arenaOffset = arenaTotal;
+		r.traceOffset = traceTotal;
+		r.poolOffset = poolTotal;
+		r.alnOffset = alnTotal;
+		r.extendSeeds = j.extendSeeds;
+		r.len = len;
+		r.seedCount = j.seedCount;
+		r.traceCap = traceCapFor(len);
+		r.poolCap = poolCapFor(len, caps);
+		r.alnCap = caps.alnCap;
+		r.pad = 0;
+		arenaTotal += roundUp256(gad::requestArenaBytes(len, caps.arena));
+		traceTotal += 2ull * r.traceCap;
+		poolTotal += r.poolCap;
+		alnTotal += r.alnCap;
+	}
+	arena_.reserve(arenaTotal);
+	rlJobs_.reserve(n * sizeof(gad::ReadJob));
+	rlTraces_.reserve(traceTotal * 8);
+	rlPool_.reserve(std::max<uint64_t>(1, poolTotal) * 8);
+	rlAlns_.reserve(alnTotal * sizeof(gad::DevAlignment));
+	rlResults_.reserve(n * sizeof(gad::ReadResult));
+	GA_HIP(hipMemcpyAsync(rlJobs_.ptr, rj.data(), n * sizeof(gad::ReadJob), hipMemcpyHostToDevice, stream_));
+	GA_HIP(hipEventRecord(ev0_, stream_));
+	GA_HIP(gad::launchReadLoops(stream_, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, rlJobs_.as<gad::ReadJob>(), readBytes_.as<char>(), rlSeeds_.as<gad::DevSeed>(), rlTraces_.as<uint64_t>(), rlPool_.as<uint64_t>(), rlAlns_.as<gad::DevAlignment>(), rlResults_.as<gad::ReadResult>(), arena_.as<unsigned char>(), caps.arena));
+	GA_HIP(hipEventRecord(ev1_, stream_));
+	std::vector<gad::ReadResult> res(n);
+	GA_HIP(hipMemcpyAsync(res.data(), rlResults_.ptr, n * sizeof(gad::ReadResult), hipMemcpyDeviceToHost, stream_));
+	GA_HIP(hipStreamSynchronize(stream_));
+	float ms = 0;
+	GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+	out.kernelMs += ms;
+	out.launches++;
+	// gather every finished read's alignment records and used pool prefix
+	std::vector<uint64_t> srcOff, dstOff;
+	std::vector<uint32_t> cnt;
+	std::vector<uint64_t> psrc, pdst;
+	std::vector<uint32_t> pcnt;
+	const uint64_t alnBase = out.alns.size(), poolBase = out.pool.size();
+	uint64_t alnUsed = 0, poolUsed = 0;
+	for (uint32_t k = 0; k < n; k++)
+	{
+		const gad::ReadResult& r = res[k];
+		const uint32_t jid = which[k];
+		if ((r.status != gad::READLOOP_DONE && r.status != gad::READLOOP_ASSERT && r.status != gad::READLOOP_OVERFLOW) || r.nAlignments > rj[k].alnCap || r.poolUsed > rj[k].poolCap)
+			throw HipError("read loop kernel returned an inconsistent result record");
+		out.results[jid] = r;
+		out.wordSteps += r.wordSteps;
+		out.extensions += r.extensions;
+		if (r.status != gad::READLOOP_DONE) continue;
+		out.alnOffset[jid] = alnBase + alnUsed;
+		out.poolOffset[jid] = poolBase + poolUsed;
+		if (r.nAlignments > 0)
+		{
+			srcOff.push_back(rj[k].alnOffset * 4);
+			dstOff.push_back(alnUsed * 4);
+			cnt.push_back(r.nAlignments * 4);
+			alnUsed += r.nAlignments;
+		}
+		if (r.poolUsed > 0)
+		{
+			psrc.push_back(rj[k].poolOffset);
+			pdst.push_back(poolUsed);
+			pcnt.push_back(r.poolUsed);
+			poolUsed += r.poolUsed;
+		}
+	}
+	out.alns.resize(alnBase + alnUsed);
+	out.pool.resize(poolBase + poolUsed);
+	auto gather = [&](const uint64_t* src, const std::vector<uint64_t>& so, const std::vector<uint64_t>& dof, const std::vector<uint32_t>& c, uint64_t total, void* dst) {
+		if (total == 0) return;
+		const uint32_t m = c.size();
+		gatherSrc_.reserve(m * sizeof(uint64_t));
+		gatherDst_.reserve(m * sizeof(uint64_t));
+		gatherCnt_.reserve(m * sizeof(uint32_t));
+		traceOut_.reserve(total * sizeof(uint64_t));
+		GA_HIP(hipMemcpyAsync(gatherSrc_.ptr, so.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipMemcpyAsync(gatherDst_.ptr, dof.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipMemcpyAsync(gatherCnt_.ptr, c.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(gad::launchGatherU64(stream_, src, gatherSrc_.as<uint64_t>(), gatherCnt_.as<uint32_t>(), gatherDst_.as<uint64_t>(), m, traceOut_.as<uint64_t>()));
+		GA_HIP(hipMemcpyAsync(dst, traceOut_.ptr, total * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+		GA_HIP(hipStreamSynchronize(stream_));
+	};
+	gather(rlAlns_.as<uint64_t>(), srcOff, dstOff, cnt, alnUsed * 4, out.alns.data() + alnBase);
+	gather(rlPool_.as<uint64_t>(), psrc, pdst, pcnt, poolUsed, out.pool.data() + poolBase);
 }
 
 } // namespace ga

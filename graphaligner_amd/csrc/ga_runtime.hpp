@@ -41,7 +41,7 @@ struct DeviceBuffer
 struct DeviceGraph
 {
 	uint64_t uid = 0;
-	DeviceBuffer nodeLen, seq2, amb4, comp, lin, inOff, inAdj, outOff, outAdj;
+	DeviceBuffer nodeLen, seq2, amb4, comp, lin, inOff, inAdj, outOff, outAdj, nodeId, nodeOff, bigSize;
 	gad::DevGraph view;
 };
 
@@ -67,11 +67,22 @@ public:
 	const ReadSet& reads() const override { return reads_; }
 	void lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out) override;
 	const char* readsBase() const override { return readBytes_.as<char>(); }
+	void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) override;
 	uint64_t reverseBase() const override { return revBase_; }
 	uint64_t arenaBudgetBytes = 32ull << 30;
+	// Extension requests per wave: the GA_LANES_PER_WAVE environment variable
+	// (1..64) if set, else defaultLanesPerWave. One request per wave measured
+	// fastest on gfx950 even for 40,000-request launches: the requests of one
+	// wave diverge almost completely, so its lanes run nearly one after another.
+	uint32_t lanesPerWave(uint32_t nreq) const;
+	uint32_t defaultLanesPerWave = 1;
+	// register-budget variant of the extension kernel (resident waves per SIMD it
+	// is compiled for: 1, 4 or 8); GA_KERNEL_WAVES_PER_SIMD overrides it
+	uint32_t kernelWavesPerSimd = 1;
 
 private:
 	void runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase);
+	void runReadLoopLaunch(const gad::DevGraph& dg, const gad::ReadLoopParams& p, const std::vector<ReadLoopJob>& jobs, const std::vector<uint32_t>& which, const ReadLoopCaps& caps, ReadLoopOutput& out);
 	void runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase);
 	int device_;
 	int cus_ = 0;
@@ -83,9 +94,11 @@ private:
 	std::vector<std::unique_ptr<DeviceIndex>> indexes_;
 	DeviceBuffer arena_, arenaOffsets_, reqs_, seqs_, traces_, results_, sums_, sumOffsets_;
 	DeviceBuffer gatherSrc_, gatherDst_, gatherCnt_, traceOut_;
+	DeviceBuffer rlSeeds_, rlJobs_, rlTraces_, rlPool_, rlAlns_, rlResults_;
 	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_;
 	ReadSet reads_;
 	uint64_t revBase_ = 0;
+	uint32_t lanesOverride_ = 0;
 };
 
 } // namespace ga

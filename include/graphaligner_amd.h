@@ -131,19 +131,18 @@ typedef struct
 	double extendKernelMs; /* device seed extension, summed over launches */
 	double hostMs;         /* host seed ordering, trace fix-up, selection, GAF text */
 	double totalMs;
-	int64_t rounds;        /* seed-loop rounds (one batched extension step each) */
+	int64_t rounds;        /* device seed-loop launches, workspace-overflow reruns included */
 	int64_t launches;
-	int64_t extensions;
-	int64_t wordSteps;     /* 64-row column steps of the bit-parallel DP */
-	int64_t seeds;
-	int64_t extendedBp;    /* read bases handed to extensions */
+	int64_t extensions;    /* one-direction seed extensions run */
+	int64_t wordSteps;     /* forward 64-row column steps of the bit-parallel DP */
+	int64_t seeds;         /* seeds found (before the seed loop's skips) */
 } ga_align_stats_t;
 
 /* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed
  * loop, select, GAF) for n reads with the given parameters. The seed loop of
  * each read makes the decisions of GraphAligner::AlignOneWay
- * (src/GraphAligner.h:94-181); all reads' extensions of one loop step run in one
- * batched GPU launch. *gaf is a NUL-terminated buffer from malloc (free with
+ * (src/GraphAligner.h:94-181); every read's whole seed loop runs on the GPU
+ * (one wavefront per read, all reads in one launch). *gaf is a NUL-terminated buffer from malloc (free with
  * ga_free): the GAF lines (src/GraphAlignerGAFAlignment.h:37-250) of read 0,
  * then read 1, ... status[i]: 0 aligned, 1 no seeds, 2 a reference assertion
  * fired and the read was dropped, 3 no alignment kept, 4 an extension exceeded

@@ -14,6 +14,7 @@
 #include <vector>
 #include "../../graphaligner_amd/csrc/ga_align.hpp"
 #include "../../graphaligner_amd/csrc/ga_extend_core.h"
+#include "../../graphaligner_amd/csrc/ga_read_core.h"
 #include "../../graphaligner_amd/csrc/ga_seed_core.h"
 
 namespace {
@@ -22,7 +23,7 @@ template <typename F>
 void forEachParallel(size_t n, F fn)
 {
 	std::atomic<size_t> next{0};
-	unsigned t = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+	unsigned t = getenv("PIPE_THREADS") ? (unsigned)atoi(getenv("PIPE_THREADS")) : std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
 	std::vector<std::thread> pool;
 	for (unsigned k = 0; k < t; k++)
 		pool.emplace_back([&]() {
@@ -39,8 +40,14 @@ void forEachParallel(size_t n, F fn)
 class HostBackend : public ga::ExtendBackend
 {
 public:
-	void extend(const ga::GraphHost& G, const gad::ExtParams& p, const std::vector<ga::ExtJob>& jobs, bool, ga::ExtBatch& out, const char* devSeqBase) override
+	gad::DevGraph view(const ga::GraphHost& G)
 	{
+		if (bigSizeUid_ != G.uid)
+		{
+			bigSize_.resize(G.numNodes());
+			for (size_t i = 0; i < G.numNodes(); i++) bigSize_[i] = (uint32_t)G.originalSize.at(G.nodeId[i]);
+			bigSizeUid_ = G.uid;
+		}
 		gad::DevGraph g;
 		g.nodeLen = G.nodeLen.data();
 		g.seq2 = G.seq2.data();
@@ -51,8 +58,75 @@ public:
 		g.inAdj = G.inAdj.data();
 		g.outOff = G.outOff.data();
 		g.outAdj = G.outAdj.data();
+		g.nodeId = G.nodeId.data();
+		g.nodeOff = G.nodeOff.data();
+		g.bigSize = bigSize_.data();
 		g.numNodes = G.numNodes();
 		g.firstAmbiguous = G.firstAmbiguous;
+		return g;
+	}
+
+	void runReadLoops(const ga::GraphHost& G, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ga::ReadLoopJob>& jobs, ga::ReadLoopOutput& out) override
+	{
+		const gad::DevGraph g = view(G);
+		const size_t n = jobs.size();
+		out = ga::ReadLoopOutput{};
+		out.results.assign(n, gad::ReadResult{});
+		out.alnOffset.assign(n, 0);
+		out.poolOffset.assign(n, 0);
+		std::vector<std::vector<gad::DevAlignment>> alns(n);
+		std::vector<std::vector<uint64_t>> pools(n);
+		forEachParallel(n, [&](size_t i) {
+			const ga::ReadLoopJob& j = jobs[i];
+			const uint32_t len = rs_.length[j.read];
+			ga::ReadLoopCaps caps = ga::initialReadLoopCaps();
+			for (int attempt = 0; attempt < ga::kArenaAttempts; attempt++)
+			{
+				gad::ReadJob r;
+				r.fwdOffset = rs_.offset[j.read];
+				r.revOffset = revBase_ + rs_.offset[j.read];
+				r.seedStart = 0;
+				r.arenaOffset = r.traceOffset = r.poolOffset = r.alnOffset = 0;
+				r.extendSeeds = j.extendSeeds;
+				r.len = len;
+				r.seedCount = j.seedCount;
+				r.traceCap = ga::traceCapFor(len);
+				r.poolCap = ga::poolCapFor(len, caps);
+				r.alnCap = caps.alnCap;
+				r.pad = 0;
+				gad::ArenaCaps c = caps.arena;
+				c.maxSlices = gad::requestSlices(len);
+				c.maxItems = gad::requestItems(len, caps.arena);
+				std::vector<unsigned char> arena(gad::arenaBytes(c) + 256);
+				unsigned char* base = (unsigned char*)(((uintptr_t)arena.data() + 255) & ~(uintptr_t)255);
+				std::vector<uint64_t> raw(2ull * r.traceCap);
+				pools[i].assign(r.poolCap, 0);
+				alns[i].assign(r.alnCap, gad::DevAlignment{});
+				gad::ReadResult res;
+				gad::runReadLoop(g, p, r, reads_.data(), seeds.data() + j.seedStart, raw.data(), pools[i].data(), alns[i].data(), base, c, res);
+				out.results[i] = res;
+				if (res.status != gad::READLOOP_OVERFLOW) break;
+				ga::growReadLoopCaps(caps);
+			}
+			const gad::ReadResult& res = out.results[i];
+			alns[i].resize(res.status == gad::READLOOP_DONE ? res.nAlignments : 0);
+			pools[i].resize(res.status == gad::READLOOP_DONE ? res.poolUsed : 0);
+		});
+		for (size_t i = 0; i < n; i++)
+		{
+			out.alnOffset[i] = out.alns.size();
+			out.alns.insert(out.alns.end(), alns[i].begin(), alns[i].end());
+			out.poolOffset[i] = out.pool.size();
+			out.pool.insert(out.pool.end(), pools[i].begin(), pools[i].end());
+			out.wordSteps += out.results[i].wordSteps;
+			out.extensions += out.results[i].extensions;
+		}
+		out.launches++;
+	}
+
+	void extend(const ga::GraphHost& G, const gad::ExtParams& p, const std::vector<ga::ExtJob>& jobs, bool, ga::ExtBatch& out, const char* devSeqBase) override
+	{
+		const gad::DevGraph g = view(G);
 		const size_t n = jobs.size();
 		out.results.assign(n, gad::ExtResult{});
 		out.traceOffset.assign(n, 0);
@@ -144,6 +218,8 @@ public:
 	uint64_t reverseBase() const override { return revBase_; }
 
 private:
+	std::vector<uint32_t> bigSize_;
+	uint64_t bigSizeUid_ = 0;
 	ga::ReadSet rs_;
 	std::vector<char> reads_;
 	uint64_t revBase_ = 0;

@@ -76,3 +76,24 @@ def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir):
     gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
     assert list(gstat) == list(ostat)
     assert gt == ot
+
+
+@pytest.mark.parametrize("wl", ["mini", "mini_sv"])
+@pytest.mark.parametrize("preset", ["vg", "dbg"])
+def test_gpu_gaf_all_seeds_cluster2(ctx, workdir, wl, preset):
+    """--try-all-seeds (no sloppy shortcuts) with --seeds-clustersize 2."""
+    gfa, names, seqs = workload(wl, workdir)
+    sd, og, oi, gp, g, gi = _both(gfa, preset)
+    en, es = edge_reads(names, seqs)
+    names = list(names) + en
+    seqs = list(seqs) + es
+    op = o.preset_params(preset)
+    op.sloppy = 0
+    op.minClusterSize = 2
+    gp.tryAllSeeds = 1
+    gp.seedClusterMinSize = 2
+    ot, ostat, _ = o.align_reads(og, oi, op, sd["density"], names, seqs, 8)
+    print(f"[gpu-align] all seeds {wl} {preset} reads={len(seqs)}", flush=True)
+    gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
+    assert list(gstat) == list(ostat)
+    assert gt == ot
