@@ -106,7 +106,7 @@ def main():
 
     ctx.synchronize()
     t0 = time.perf_counter()
-    tot = dict(extendKernelMs=0.0, lookupMs=0.0, hostMs=0.0, launches=0, extensions=0, wordSteps=0, rounds=0, seeds=0)
+    tot = dict(extendKernelMs=0.0, lookupMs=0.0, hostMs=0.0, launches=0, extensions=0, wordSteps=0, rounds=0, seeds=0, reruns=0)
     gaf = status = None
     for k in range(args.steps):
         ts = time.perf_counter()
@@ -114,7 +114,7 @@ def main():
         for key in tot:
             tot[key] += st[key]
         print(f"[bench] step {k + 1}/{args.steps}: {time.perf_counter() - ts:.2f} s, kernel {st['extendKernelMs'] / 1e3:.2f} s, "
-              f"{st['launches']} launches, {st['extensions']} extensions", file=sys.stderr, flush=True)
+              f"{st['launches']} launches, {st['extensions']} extensions, {st['reruns']} reruns", file=sys.stderr, flush=True)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
 
@@ -153,7 +153,7 @@ def main():
                    "preset": "vg", "parallelism": "1 GPU"},
         "breakdown_s_per_step": {"extend_kernel": kernel_s / steps, "seed_lookup_kernel": tot["lookupMs"] / 1e3 / steps,
                                  "host": tot["hostMs"] / 1e3 / steps},
-        "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "wordSteps", "seeds")},
+        "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "wordSteps", "seeds", "reruns")},
         "read_status_counts": counts,
         "roofline": {"kernel": "extendThreadKernel", "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,

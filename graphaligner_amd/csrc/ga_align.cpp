@@ -709,6 +709,7 @@ void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex&
 		stats.extensions += rlo.extensions;
 		stats.extendKernelMs += rlo.kernelMs;
 		stats.wordSteps += rlo.wordSteps;
+		stats.reruns += rlo.reruns;
 		th = std::chrono::steady_clock::now();
 		// alignments in creation order, the sloppy re-sort after each one, then
 		// selection, sorts and GAF (src/GraphAligner.h:141-177, src/Aligner.cpp:502-606)

@@ -67,7 +67,7 @@ struct AlignStats
 	double extendKernelMs = 0;  // device seed extension
 	double hostMs = 0;          // host seeding, fix-up, selection, GAF
 	double totalMs = 0;
-	uint64_t rounds = 0, launches = 0, extensions = 0, wordSteps = 0, seeds = 0;
+	uint64_t rounds = 0, launches = 0, extensions = 0, wordSteps = 0, seeds = 0, reruns = 0;
 };
 
 // MinimizerSeeder::getSeeds (src/MinimizerSeeder.cpp:521-555) from the device

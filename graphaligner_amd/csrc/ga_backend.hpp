@@ -105,6 +105,7 @@ struct ReadLoopOutput
 	double kernelMs = 0;
 	uint32_t launches = 0;
 	uint64_t wordSteps = 0, extensions = 0;
+	uint64_t reruns = 0;                     // jobs rerun with larger workspaces
 };
 
 // Workspace capacities of the device seed loop; grown like the arena on overflow.
@@ -119,9 +120,13 @@ inline ReadLoopCaps initialReadLoopCaps()
 {
 	ReadLoopCaps c;
 	c.arena = initialArenaCaps();
-	c.poolPerBase = 6;
+	// a read's whole seed loop shares one workspace, and an overflow reruns the
+	// whole read: start wider than a single extension's first attempt
+	c.arena.maxItems *= 2;
+	c.arena.maxExtras *= 2;
+	c.poolPerBase = 12;
 	c.poolExtra = 4096;
-	c.alnCap = 32;
+	c.alnCap = 64;
 	return c;
 }
 inline void growReadLoopCaps(ReadLoopCaps& c)

@@ -370,6 +370,7 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& 
 		stats->extensions = st.extensions;
 		stats->wordSteps = st.wordSteps;
 		stats->seeds = st.seeds;
+		stats->reruns = st.reruns;
 	}
 	return GA_OK;
 }

@@ -813,6 +813,48 @@ struct Extension
 		if (set & 8) r |= eq.m[3];
 		return r;
 	}
+	// A node's bases held in registers for its column loop: two 2-bit chunks, or
+	// the four base masks of an IUPAC node.
+	struct NodeSeq
+	{
+		uint64_t w[4];
+		bool amb;
+	};
+	GA_HD NodeSeq loadNodeSeq(uint32_t n) const
+	{
+		NodeSeq s;
+		s.amb = n >= g.firstAmbiguous;
+		if (!s.amb)
+		{
+			s.w[0] = g.seq2[2 * n];
+			s.w[1] = g.seq2[2 * n + 1];
+			s.w[2] = s.w[3] = 0;
+		}
+		else
+		{
+			const uint64_t* m = g.amb4 + 4 * (uint64_t)(n - g.firstAmbiguous);
+			s.w[0] = m[0];
+			s.w[1] = m[1];
+			s.w[2] = m[2];
+			s.w[3] = m[3];
+		}
+		return s;
+	}
+	// eqAt from registers
+	GA_HD uint64_t eqOf(const Eq4& eq, const NodeSeq& s, uint32_t pos) const
+	{
+		if (!s.amb)
+		{
+			uint64_t w = pos < 32 ? s.w[0] : s.w[1];
+			return eqSelect(eq, (uint32_t)(w >> ((pos & 31) * 2)) & 3);
+		}
+		uint64_t r = 0;
+		if ((s.w[0] >> pos) & 1) r |= eq.m[0];
+		if ((s.w[1] >> pos) & 1) r |= eq.m[1];
+		if ((s.w[2] >> pos) & 1) r |= eq.m[2];
+		if ((s.w[3] >> pos) & 1) r |= eq.m[3];
+		return r;
+	}
 	// characterMatch(read char, graph char) (src/GraphAlignerCommon.h:176-198)
 	GA_HD bool charMatch(int32_t seqPos, uint32_t node, uint32_t off) const
 	{
@@ -877,7 +919,8 @@ struct Extension
 		res.maxX = SMIN;
 		res.cells = 0;
 		uint32_t L = nodeLen(node);
-		uint64_t Eq0 = eqAt(eq, node, 0);
+		const NodeSeq nseq = loadNodeSeq(node);
+		uint64_t Eq0 = eqOf(eq, nseq, 0);
 		Slice ws = mkslice(0, 0, 0);
 		bool hasWs = false, hasSkipless = false;
 		if (Recalc)
@@ -1029,7 +1072,7 @@ struct Extension
 		}
 		for (uint32_t pos = 1; pos < L; pos++)
 		{
-			uint64_t Eq = eqAt(eq, node, pos) & forceEq;
+			uint64_t Eq = eqOf(eq, nseq, pos) & forceEq;
 			uint64_t Ph, Mh;
 			Slice nw = nextSlice(Eq, ws, (fixedHP >> pos) & 1, (fixedHN >> pos) & 1, Ph, Mh);
 			if ((forceMask >> pos) & 1)

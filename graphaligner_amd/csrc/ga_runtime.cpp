@@ -412,6 +412,7 @@ void Context::runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, con
 		for (uint32_t id : which)
 			if (out.results[id].status == gad::READLOOP_OVERFLOW) again.push_back(id);
 		which.swap(again);
+		if (attempt + 1 < kArenaAttempts) out.reruns += which.size();
 		growReadLoopCaps(caps);
 	}
 }

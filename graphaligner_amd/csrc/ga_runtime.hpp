@@ -69,7 +69,8 @@ public:
 	const char* readsBase() const override { return readBytes_.as<char>(); }
 	void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) override;
 	uint64_t reverseBase() const override { return revBase_; }
-	uint64_t arenaBudgetBytes = 32ull << 30;
+	// device workspace per launch (MI355X has 288 GB of HBM)
+	uint64_t arenaBudgetBytes = 160ull << 30;
 	// Extension requests per wave: the GA_LANES_PER_WAVE environment variable
 	// (1..64) if set, else defaultLanesPerWave. One request per wave measured
 	// fastest on gfx950 even for 40,000-request launches: the requests of one

@@ -136,6 +136,7 @@ typedef struct
 	int64_t extensions;    /* one-direction seed extensions run */
 	int64_t wordSteps;     /* forward 64-row column steps of the bit-parallel DP */
 	int64_t seeds;         /* seeds found (before the seed loop's skips) */
+	int64_t reruns;        /* reads rerun with larger device workspaces */
 } ga_align_stats_t;
 
 /* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed
