@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1500, help="reads in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores, at most 16)")
     ap.add_argument("--workdir", default=os.environ.get("GA_BENCH_DIR", ""))
+    ap.add_argument("--device-seed-loop", action="store_true", help="run each read's whole seed loop on the GPU")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (traffic-counter calibration) and exit")
     return ap.parse_args()
@@ -86,6 +87,7 @@ def main():
     print(f"[bench] workload vg5m: {len(seqs)} reads, {sum(map(len, seqs)) / 1e6:.1f} Mbp ({gen_s:.1f} s)", file=sys.stderr, flush=True)
 
     params = ga.params_preset("vg")
+    params.deviceSeedLoop = 1 if args.device_seed_loop else 0
     t = time.time()
     graph = ga.Graph(prefix + ".gfa")
     load_s = time.time() - t
@@ -150,7 +152,8 @@ def main():
         "gcups": gcups,
         "config": {"workload": "vg5m: 5 Mbp variation graph with SV bubbles, 20,000 log-normal reads (median 12 kbp) at 12% error, -x vg",
                    "reads": len(names), "read_mbp": sum(map(len, seqs)) / 1e6, "split_nodes": graph.num_nodes,
-                   "preset": "vg", "parallelism": "1 GPU"},
+                   "preset": "vg", "parallelism": "1 GPU",
+                   "seed_loop": "device" if args.device_seed_loop else "rounds"},
         "breakdown_s_per_step": {"extend_kernel": kernel_s / steps, "seed_lookup_kernel": tot["lookupMs"] / 1e3 / steps,
                                  "host": tot["hostMs"] / 1e3 / steps},
         "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "wordSteps", "seeds", "reruns")},

@@ -30,7 +30,7 @@ class Params(ctypes.Structure):
                 ("multimapScoreFraction", ctypes.c_double), ("minAlignmentScore", ctypes.c_double),
                 ("minimizerLength", ctypes.c_int32), ("minimizerWindowSize", ctypes.c_int32),
                 ("minimizerSeedDensity", ctypes.c_double), ("minimizerDiscardMostNumerousFraction", ctypes.c_double),
-                ("cigarMatchMismatchMerge", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("cigarMatchMismatchMerge", ctypes.c_int32), ("deviceSeedLoop", ctypes.c_int32)]
 
 
 class SliceSummary(ctypes.Structure):

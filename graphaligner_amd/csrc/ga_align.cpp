@@ -450,6 +450,30 @@ std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, cons
 	return s.str();
 }
 
+// Seed loop state of one read (GraphAligner::AlignOneWay, src/GraphAligner.h:94-181).
+struct ReadState
+{
+	uint32_t read = 0;
+	std::vector<SeedHit> seeds;
+	size_t next = 0;
+	std::vector<Alignment> alignments;
+	uint64_t seedScoreForEndToEndAln = 0;
+	uint64_t worstExtendedSeedScore = 0;
+	uint64_t seedsExtended = 0;
+	uint64_t extendSeeds = 0;
+	bool done = false;
+	int status = READ_ALIGNED;
+	// seed being extended this round and its device jobs (-1: none)
+	size_t pending = 0;
+	int64_t bwdJob = -1, fwdJob = -1;
+};
+
+struct Job
+{
+	uint32_t state;
+	bool backward;
+};
+
 } // namespace
 
 std::vector<SeedHit> seedsFromMatches(const GraphHost& g, const MinimizerIndex& idx, const gad::SeedMatch* matches, uint32_t nmatches, uint64_t readLen, double density)
@@ -609,7 +633,7 @@ Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char
 
 } // namespace
 
-void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
+void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
 {
 	auto t0 = std::chrono::steady_clock::now();
 	const ReadSet& rs = ctx.reads();
@@ -771,6 +795,289 @@ void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex&
 		stats.hostMs += msSince(th);
 	}
 	stats.totalMs = msSince(t0);
+}
+
+namespace {
+
+// Picks the next seed of the read's loop to extend (src/GraphAligner.h:105-140);
+// returns false when the loop is over.
+bool advance(ReadState& st, const AlignOptions& opt)
+{
+	while (st.next < st.seeds.size())
+	{
+		const SeedHit& s = st.seeds[st.next];
+		if (opt.sloppyOptimizations && s.seedGoodness <= st.seedScoreForEndToEndAln) return false;
+		if (st.seedsExtended >= st.extendSeeds && s.seedGoodness < st.worstExtendedSeedScore) return false;
+		if (s.seedClusterSize < opt.minSeedClusterSize)
+		{
+			st.next++;
+			continue;
+		}
+		bool covered = false;
+		if (opt.sloppyOptimizations)
+		{
+			for (const auto& a : st.alignments)
+			{
+				if (a.alignmentStart <= s.seqPos && a.alignmentEnd >= s.seqPos && a.seedGoodness > s.seedGoodness)
+				{
+					covered = true;
+					break;
+				}
+			}
+		}
+		if (!covered)
+		{
+			for (const auto& a : st.alignments)
+			{
+				if (exactAlignmentPart(a, s))
+				{
+					covered = true;
+					break;
+				}
+			}
+		}
+		if (covered)
+		{
+			st.next++;
+			continue;
+		}
+		st.worstExtendedSeedScore = s.seedGoodness;
+		st.seedsExtended++;
+		st.pending = st.next;
+		return true;
+	}
+	return false;
+}
+
+} // namespace
+
+void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
+{
+	auto t0 = std::chrono::steady_clock::now();
+	const ReadSet& rs = ctx.reads();
+	if ((uint64_t)first + count > rs.size()) throw std::runtime_error("read range outside the resident reads");
+	stats = AlignStats{};
+	for (uint32_t i = 0; i < count; i++)
+	{
+		out[i] = ReadOutcome{};
+		if (rs.length[first + i] > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
+	}
+	for (uint32_t batchStart = 0; batchStart < count; batchStart += opt.readsPerBatch)
+	{
+		const uint32_t bn = std::min<uint32_t>(count - batchStart, opt.readsPerBatch);
+		const uint32_t base = first + batchStart;
+		std::vector<const char*> bseqs(bn);
+		std::vector<uint32_t> blens(bn);
+		std::vector<uint64_t> boff(bn);
+		for (uint32_t i = 0; i < bn; i++)
+		{
+			bseqs[i] = rs.seq(base + i);
+			blens[i] = rs.length[base + i];
+			boff[i] = rs.offset[base + i];
+		}
+		SeedMatches rb;
+		ctx.lookupSeeds(idx, base, bn, rb);
+		stats.lookupMs += rb.lookupMs;
+		std::vector<ReadState> states(bn);
+		auto th = std::chrono::steady_clock::now();
+		parallelFor(bn, opt.hostThreads, [&](size_t i) {
+			ReadState& st = states[i];
+			st.read = (uint32_t)i;
+			ReadOutcome& o = out[batchStart + i];
+			try
+			{
+				st.seeds = seedsFromMatches(g, idx, rb.matches.data() + rb.matchOffset[i], rb.matchCount[i], blens[i], opt.minimizerSeedDensity);
+				if (st.seeds.empty())
+				{
+					o.status = READ_NO_SEEDS;
+					st.done = true;
+					return;
+				}
+				orderSeedsByChaining(g, st.seeds);
+				// ReverseComplement of the read asserts on letters it does not know
+				for (uint32_t k = 0; k < blens[i]; k++) check(complementOf(bseqs[i][k]) != 0);
+				st.extendSeeds = opt.seedExtendDensity == -1 ? st.seeds.size() : (uint64_t)(opt.seedExtendDensity * blens[i] + 1);
+			}
+			catch (const ReadAssert&)
+			{
+				st.status = READ_ASSERT;
+				st.done = true;
+			}
+		});
+		stats.hostMs += msSince(th);
+		for (auto& st : states) stats.seeds += st.seeds.size();
+		std::vector<uint32_t> active;
+		for (uint32_t i = 0; i < bn; i++)
+			if (!states[i].done) active.push_back(i);
+		while (!active.empty())
+		{
+			th = std::chrono::steady_clock::now();
+			std::vector<uint8_t> extendNow(active.size(), 0);
+			parallelFor(active.size(), opt.hostThreads, [&](size_t a) {
+				ReadState& st = states[active[a]];
+				try
+				{
+					extendNow[a] = advance(st, opt) ? 1 : 0;
+				}
+				catch (const ReadAssert&)
+				{
+					st.status = READ_ASSERT;
+				}
+				if (!extendNow[a]) st.done = true;
+			});
+			std::vector<ExtJob> jobs;
+			std::vector<uint32_t> extending;
+			for (size_t a = 0; a < active.size(); a++)
+			{
+				if (!extendNow[a]) continue;
+				uint32_t r = active[a];
+				ReadState& st = states[r];
+				const SeedHit& s = st.seeds[st.pending];
+				const uint64_t L = blens[r];
+				int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
+				int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
+				st.bwdJob = st.fwdJob = -1;
+				if (s.seqPos > 0)
+				{
+					auto rp = g.reversePosition(forwardId, s.nodeOffset);
+					uint32_t node = g.unitigNode(backwardId, rp.second);
+					ExtJob j{nullptr, (uint32_t)s.seqPos, node, (uint32_t)(rp.second - g.nodeOff[node])};
+					j.devOffset = ctx.reverseBase() + boff[r] + (L - s.seqPos);
+					st.bwdJob = jobs.size();
+					jobs.push_back(j);
+				}
+				if (s.seqPos < L - 1)
+				{
+					uint32_t node = g.unitigNode(forwardId, s.nodeOffset);
+					ExtJob j{nullptr, (uint32_t)(L - s.seqPos - 1), node, (uint32_t)(s.nodeOffset - g.nodeOff[node])};
+					j.devOffset = boff[r] + s.seqPos + 1;
+					st.fwdJob = jobs.size();
+					jobs.push_back(j);
+				}
+				extending.push_back(r);
+			}
+			stats.hostMs += msSince(th);
+			if (extending.empty()) break;
+			ExtBatch eb;
+			ctx.extend(g, opt.ext, jobs, false, eb, ctx.readsBase());
+			stats.rounds++;
+			stats.launches += eb.launches;
+			stats.reruns += eb.reruns;
+			stats.extensions += jobs.size();
+			stats.extendKernelMs += eb.kernelMs;
+			stats.wordSteps += eb.wordSteps;
+			th = std::chrono::steady_clock::now();
+			parallelFor(extending.size(), opt.hostThreads, [&](size_t a) {
+				ReadState& st = states[extending[a]];
+				const uint32_t r = extending[a];
+				const char* seq = bseqs[r];
+				const uint64_t L = blens[r];
+				const SeedHit s = st.seeds[st.pending];
+				st.next = st.pending + 1;
+				try
+				{
+					const uint64_t* raws[2] = {nullptr, nullptr};
+					uint32_t counts[2] = {0, 0};
+					for (int dir = 0; dir < 2; dir++)
+					{
+						int64_t j = dir == 0 ? st.bwdJob : st.fwdJob;
+						if (j < 0) continue;
+						const gad::ExtResult& res = eb.results[j];
+						if (res.status == gad::EXT_OVERFLOW)
+						{
+							st.status = READ_OVERFLOW;
+							st.done = true;
+							return;
+						}
+						check(res.status != gad::EXT_ASSERT);
+						if (res.status != gad::EXT_OK) continue;
+						check(res.ntrace > 0);
+						raws[dir] = eb.trace.data() + eb.traceOffset[j];
+						counts[dir] = res.ntrace;
+					}
+					Alignment item = buildAlignment(g, opt, seq, L, s, raws[0], counts[0], raws[1], counts[1]);
+					if (item.alignmentFailed()) return;
+					item.seedGoodness = s.seedGoodness;
+					st.alignments.push_back(std::move(item));
+					if (opt.sloppyOptimizations)
+					{
+						auto& al = st.alignments;
+						std::sort(al.begin(), al.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
+						if (al[0].alignmentStart == 0)
+						{
+							uint64_t minGoodness = al[0].seedGoodness;
+							uint64_t contiguousEnd = al[0].alignmentEnd;
+							for (size_t k = 1; k < al.size(); k++)
+							{
+								if (al[k].alignmentStart <= contiguousEnd)
+								{
+									minGoodness = std::min(minGoodness, al[k].seedGoodness);
+									contiguousEnd = std::max(contiguousEnd, al[k].alignmentEnd);
+								}
+							}
+							if (contiguousEnd == L) st.seedScoreForEndToEndAln = minGoodness;
+						}
+					}
+				}
+				catch (const ReadAssert&)
+				{
+					st.status = READ_ASSERT;
+					st.done = true;
+				}
+			});
+			stats.hostMs += msSince(th);
+			std::vector<uint32_t> still;
+			for (uint32_t r : active)
+				if (!states[r].done) still.push_back(r);
+			active.swap(still);
+		}
+		// selection and GAF (src/Aligner.cpp:502-606)
+		th = std::chrono::steady_clock::now();
+		parallelFor(bn, opt.hostThreads, [&](size_t i) {
+			ReadState& st = states[i];
+			ReadOutcome& o = out[batchStart + i];
+			if (o.status == READ_NO_SEEDS) return;
+			o.seedsExtended = st.seedsExtended;
+			if (st.status != READ_ALIGNED)
+			{
+				o.status = st.status;
+				return;
+			}
+			std::vector<Alignment> sel;
+			try
+			{
+				if (!st.alignments.empty()) sel = selectAlignments(st.alignments, opt.minAlignmentScore, opt.multimapScoreFraction);
+				if (sel.empty())
+				{
+					o.status = READ_NO_ALIGNMENT;
+					return;
+				}
+				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
+				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentXScore > r.alignmentXScore; });
+				std::string text;
+				for (const auto& a : sel)
+				{
+					text += gafLine(g, names[batchStart + i], blens[i], a, opt.cigarMatchMismatchMerge);
+					text += '\n';
+				}
+				o.gaf = std::move(text);
+				o.status = READ_ALIGNED;
+			}
+			catch (const ReadAssert&)
+			{
+				o.status = READ_ASSERT;
+				o.gaf.clear();
+			}
+		});
+		stats.hostMs += msSince(th);
+	}
+	stats.totalMs = msSince(t0);
+}
+
+void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
+{
+	if (opt.deviceSeedLoop) alignResidentDeviceLoop(ctx, g, idx, opt, names, first, count, out, stats);
+	else alignResidentRounds(ctx, g, idx, opt, names, first, count, out, stats);
 }
 
 void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const std::vector<const char*>& names, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<ReadOutcome>& out, AlignStats& stats)

@@ -43,6 +43,10 @@ struct AlignOptions
 	bool cigarMatchMismatchMerge = false;
 	int hostThreads = 16;
 	uint32_t readsPerBatch = 1u << 20;
+	// false: the seed loops advance in rounds, each round extending the next seed
+	// of every unfinished read in one launch (both directions as separate jobs);
+	// true: every read's whole seed loop runs on the device in one launch
+	bool deviceSeedLoop = false;
 };
 
 enum ReadStatus

@@ -32,6 +32,7 @@ struct ExtBatch
 	double kernelMs = 0;               // summed kernel time of all launches
 	uint32_t launches = 0;
 	uint64_t wordSteps = 0;
+	uint64_t reruns = 0;               // jobs rerun with larger workspaces
 };
 
 // Reads resident on the device: forward bytes (and, once looked up, their

@@ -341,6 +341,7 @@ AlignOptions alignOptionsOf(const ga_params_t& p)
 	opt.sloppyOptimizations = p.tryAllSeeds == 0;
 	opt.minSeedClusterSize = p.seedClusterMinSize < 0 ? 0 : (uint64_t)p.seedClusterMinSize;
 	opt.cigarMatchMismatchMerge = p.cigarMatchMismatchMerge != 0;
+	opt.deviceSeedLoop = p.deviceSeedLoop != 0;
 	return opt;
 }
 

@@ -356,6 +356,7 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 		for (uint32_t id : which)
 			if (out.results[id].status == gad::EXT_OVERFLOW) again.push_back(id);
 		which.swap(again);
+		if (attempt + 1 < kArenaAttempts) out.reruns += which.size();
 		growArenaCaps(caps);
 	}
 }

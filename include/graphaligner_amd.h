@@ -52,7 +52,11 @@ typedef struct
 	double minimizerSeedDensity;     /* --seeds-minimizer-density; -1 = all */
 	double minimizerDiscardMostNumerousFraction; /* --seeds-minimizer-ignore-frequent */
 	int32_t cigarMatchMismatchMerge; /* --cigar-match-mismatch */
-	int32_t pad;
+	/* Execution only (no reference counterpart, results are identical): 0 runs
+	 * the seed loops in rounds, every round extending the next seed of each
+	 * unfinished read in one GPU launch; 1 runs each read's whole seed loop on
+	 * the GPU, all reads in one launch. */
+	int32_t deviceSeedLoop;
 } ga_params_t;
 
 /* Fills params with the defaults plus preset "vg" or "dbg" (-x), as

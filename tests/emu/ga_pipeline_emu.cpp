@@ -134,6 +134,7 @@ public:
 		out.trace.clear();
 		out.summaries.clear();
 		std::vector<std::vector<uint64_t>> traces(n);
+		std::atomic<uint64_t> reruns{0};
 		forEachParallel(n, [&](size_t i) {
 			const ga::ExtJob& j = jobs[i];
 			gad::ArenaCaps caps = ga::initialArenaCaps();
@@ -156,6 +157,7 @@ public:
 				gad::runRequest(g, p, r, devSeqBase ? devSeqBase : j.seq, traces[i].data(), base, c, nullptr, res);
 				out.results[i] = res;
 				if (res.status != gad::EXT_OVERFLOW) break;
+				if (attempt + 1 < ga::kArenaAttempts) reruns++;
 				ga::growArenaCaps(caps);
 			}
 			traces[i].resize(out.results[i].status == gad::EXT_OVERFLOW ? 0 : out.results[i].ntrace);
@@ -167,6 +169,7 @@ public:
 			out.wordSteps += out.results[i].wordSteps;
 		}
 		out.launches++;
+		out.reruns = reruns.load();
 	}
 
 	void uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens) override
@@ -301,6 +304,8 @@ struct PipeOptions
 	double multimapScoreFraction;
 	double seedDensity;
 	int64_t minClusterSize;
+	int32_t deviceLoop;
+	int32_t pad;
 };
 
 int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char* const* names, const char* const* seqs, const int64_t* lens, char** gafOut, int32_t* statusOut, int64_t* statsOut)
@@ -319,6 +324,7 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 	o.sloppyOptimizations = po->sloppy != 0;
 	o.minSeedClusterSize = po->minClusterSize;
 	o.hostThreads = 4;
+	o.deviceSeedLoop = po->deviceLoop != 0;
 	std::vector<const char*> nm(names, names + nreads), sq(seqs, seqs + nreads);
 	std::vector<uint32_t> ln(nreads);
 	for (int64_t i = 0; i < nreads; i++) ln[i] = (uint32_t)lens[i];

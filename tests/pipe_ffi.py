@@ -14,7 +14,8 @@ class PipeOptions(ctypes.Structure):
     _fields_ = [("bandwidth", ctypes.c_int32), ("errorCost", ctypes.c_int32), ("xdrop", ctypes.c_int32),
                 ("sloppy", ctypes.c_int32), ("maxCellsPerSlice", ctypes.c_int64),
                 ("seedExtendDensity", ctypes.c_double), ("multimapScoreFraction", ctypes.c_double),
-                ("seedDensity", ctypes.c_double), ("minClusterSize", ctypes.c_int64)]
+                ("seedDensity", ctypes.c_double), ("minClusterSize", ctypes.c_int64),
+                ("deviceLoop", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 def lib():
@@ -85,13 +86,13 @@ class Index:
         return out[: 10 * n.value].reshape(-1, 10)
 
 
-def options(oracle_params, density):
+def options(oracle_params, density, device_loop=False):
     p = oracle_params
     return PipeOptions(p.bandwidth, p.errorCost, p.xdrop, p.sloppy, p.maxCellsPerSlice, p.seedExtendDensity,
-                       p.multimapScoreFraction, density, p.minClusterSize)
+                       p.multimapScoreFraction, density, p.minClusterSize, int(device_loop), 0)
 
 
-def align_reads(index, oracle_params, density, names, seqs):
+def align_reads(index, oracle_params, density, names, seqs, device_loop=False):
     n = len(seqs)
     na = (ctypes.c_char_p * n)(*[x.encode() for x in names])
     sa = (ctypes.c_char_p * n)(*[x.encode() for x in seqs])
@@ -99,7 +100,7 @@ def align_reads(index, oracle_params, density, names, seqs):
     out = ctypes.c_void_p()
     status = np.zeros(n, dtype=np.int32)
     stats = np.zeros(4, dtype=np.int64)
-    opt = options(oracle_params, density)
+    opt = options(oracle_params, density, device_loop)
     rc = lib().pipe_align_reads(index.h, ctypes.byref(opt), n, na, sa, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                 ctypes.byref(out), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                 stats.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))

@@ -52,11 +52,13 @@ def test_gpu_seeds_match_oracle(ctx, workdir, wl, preset):
         assert a.shape == b.shape and (a == b).all()
 
 
+@pytest.mark.parametrize("device_loop", [0, 1], ids=["rounds", "device_loop"])
 @pytest.mark.parametrize("wl", ["mini", "mini_sv"])
 @pytest.mark.parametrize("preset", ["vg", "dbg"])
-def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset):
+def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset, device_loop):
     gfa, names, seqs = workload(wl, workdir)
     sd, og, oi, gp, g, gi = _both(gfa, preset)
+    gp.deviceSeedLoop = device_loop
     en, es = edge_reads(names, seqs)
     names = list(names) + en
     seqs = list(seqs) + es
@@ -67,10 +69,12 @@ def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset):
     assert gt == ot
 
 
-def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir):
+@pytest.mark.parametrize("device_loop", [0, 1], ids=["rounds", "device_loop"])
+def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir, device_loop):
     gfa, names, seqs = workload("vg100k", workdir)
     names, seqs = names[:400], seqs[:400]
     sd, og, oi, gp, g, gi = _both(gfa, "vg")
+    gp.deviceSeedLoop = device_loop
     ot, ostat, _ = o.align_reads(og, oi, o.preset_params("vg"), sd["density"], names, seqs, 16)
     print(f"[gpu-align] gaf vg100k reads={len(seqs)}", flush=True)
     gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
@@ -78,9 +82,10 @@ def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir):
     assert gt == ot
 
 
+@pytest.mark.parametrize("device_loop", [0, 1], ids=["rounds", "device_loop"])
 @pytest.mark.parametrize("wl", ["mini", "mini_sv"])
 @pytest.mark.parametrize("preset", ["vg", "dbg"])
-def test_gpu_gaf_all_seeds_cluster2(ctx, workdir, wl, preset):
+def test_gpu_gaf_all_seeds_cluster2(ctx, workdir, wl, preset, device_loop):
     """--try-all-seeds (no sloppy shortcuts) with --seeds-clustersize 2."""
     gfa, names, seqs = workload(wl, workdir)
     sd, og, oi, gp, g, gi = _both(gfa, preset)
@@ -92,6 +97,7 @@ def test_gpu_gaf_all_seeds_cluster2(ctx, workdir, wl, preset):
     op.minClusterSize = 2
     gp.tryAllSeeds = 1
     gp.seedClusterMinSize = 2
+    gp.deviceSeedLoop = device_loop
     ot, ostat, _ = o.align_reads(og, oi, op, sd["density"], names, seqs, 8)
     print(f"[gpu-align] all seeds {wl} {preset} reads={len(seqs)}", flush=True)
     gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
