@@ -15,6 +15,21 @@
 //    -> the key with the smallest count is the one left out.
 // Everything else (libstdc++ unordered_map / std::sort / std::priority_queue
 // behaviour) is reproduced by using the same standard containers and algorithms.
+//
+// How each part is pinned (tests/test_oracle_golden.py, tests/test_oracle_pinning.py):
+//  * WordSlice primitives and the column step: brute force over random valid
+//    slices (cellwise min, exact per-row comparison, the O(nm) recurrence, the
+//    X-score candidate rule);
+//  * slice DP (calculateSlice, getXdropSlices with band and X-drop off): every
+//    slice's minimum equals a textbook edit-distance DP over the graph;
+//  * traceback: traces are graph paths whose edit count is never below the DP
+//    score and equals it except on the reference's out-of-band fallbacks;
+//  * the whole read path: the reference's README known answer (test/graph.gfa,
+//    test/read.fa), GAF line byte for byte.
+// Parity unpinned: the three substitutions above (phmap iteration order, heap
+// ties inside strongly connected components, the BBHash key left out by
+// initMaxCount) and the backtrace tie-breaks of the out-of-band fallbacks; no
+// reference output covers them and the reference cannot be built here.
 #pragma once
 #include <algorithm>
 #include <array>
