@@ -31,6 +31,11 @@ VALU_INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 # band item state amortised over the node's columns)
 OPS_PER_WORD_STEP = 42
 BYTES_PER_WORD_STEP = 6
+# HBM bytes per extension launch of the default run of this workload, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy kernel's readings
+# (profiles/extend_vg5m.md); None when the default path changes and is not yet
+# re-measured
+MEASURED_TRAFFIC_PER_LAUNCH = 2.162e9
 
 
 def parse():
@@ -40,7 +45,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reads", type=int, default=0, help="use only the first N reads (0: all 20,000)")
     ap.add_argument("--warmup-reads", type=int, default=2000, help="reads aligned by each warmup step")
-    ap.add_argument("--cpu-sample", type=int, default=1500, help="reads in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=3000, help="reads in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores, at most 16)")
     ap.add_argument("--workdir", default=os.environ.get("GA_BENCH_DIR", ""))
     ap.add_argument("--device-seed-loop", action="store_true", help="run each read's whole seed loop on the GPU")
@@ -95,8 +100,10 @@ def main():
     index = ga.Index(graph, params)
     index_s = time.time() - t
     ctx = ga.Context(0)
+    t = time.time()
     ga.upload_reads(ctx, seqs)
     ctx.synchronize()
+    upload_s = time.time() - t
     print(f"[bench] graph {graph.num_nodes} split nodes ({load_s:.1f} s), index {index.num_kmers} k-mers ({index_s:.1f} s)",
           file=sys.stderr, flush=True)
 
@@ -159,13 +166,15 @@ def main():
         "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "wordSteps", "seeds", "reruns")},
         "read_status_counts": counts,
         "roofline": {"kernel": "extendThreadKernel", "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": None if args.device_seed_loop or args.reads else MEASURED_TRAFFIC_PER_LAUNCH,
                      "mean_launch_ms": tot["extendKernelMs"] / launches,
                      "valu_int32": {"achieved_tops": valu_tops, "peak_tops": VALU_INT32_PEAK_TOPS,
                                     "frac": valu_tops / VALU_INT32_PEAK_TOPS}},
         "cpu_baseline": cpu,
         "gaf_identical_to_cpu_sample": parity,
-        "setup_s": {"generate": gen_s, "graph_load": load_s, "index_build": index_s},
+        "setup_s": {"generate": gen_s, "graph_load": load_s, "index_build": index_s, "read_upload": upload_s},
+        "pcie_inclusive_reads_per_s": len(names) * args.steps / (elapsed + args.steps * upload_s),
     }
     print(json.dumps(line), flush=True)
     return 0
