@@ -45,7 +45,7 @@ class AlignStats(ctypes.Structure):
     _fields_ = [("lookupMs", ctypes.c_double), ("extendKernelMs", ctypes.c_double), ("hostMs", ctypes.c_double),
                 ("totalMs", ctypes.c_double), ("rounds", ctypes.c_int64), ("launches", ctypes.c_int64),
                 ("extensions", ctypes.c_int64), ("wordSteps", ctypes.c_int64), ("seeds", ctypes.c_int64),
-                ("reruns", ctypes.c_int64)]
+                ("reruns", ctypes.c_int64), ("lookupCallMs", ctypes.c_double), ("extendCallMs", ctypes.c_double)]
 
 
 _lib = None

@@ -654,7 +654,9 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 		const uint32_t bn = std::min<uint32_t>(count - batchStart, opt.readsPerBatch);
 		const uint32_t base = first + batchStart;
 		SeedMatches rb;
+		auto tl = std::chrono::steady_clock::now();
 		ctx.lookupSeeds(idx, base, bn, rb);
+		stats.lookupCallMs += msSince(tl);
 		stats.lookupMs += rb.lookupMs;
 		auto th = std::chrono::steady_clock::now();
 		// seeds, seed order and the read checks that precede the seed loop
@@ -727,7 +729,9 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 		});
 		stats.hostMs += msSince(th);
 		ReadLoopOutput rlo;
+		auto te = std::chrono::steady_clock::now();
 		ctx.runReadLoops(g, rp, devSeeds, jobs, rlo);
+		stats.extendCallMs += msSince(te);
 		stats.rounds += rlo.launches;
 		stats.launches += rlo.launches;
 		stats.extensions += rlo.extensions;
@@ -876,7 +880,9 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 			boff[i] = rs.offset[base + i];
 		}
 		SeedMatches rb;
+		auto tl = std::chrono::steady_clock::now();
 		ctx.lookupSeeds(idx, base, bn, rb);
+		stats.lookupCallMs += msSince(tl);
 		stats.lookupMs += rb.lookupMs;
 		std::vector<ReadState> states(bn);
 		auto th = std::chrono::steady_clock::now();
@@ -959,7 +965,9 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 			stats.hostMs += msSince(th);
 			if (extending.empty()) break;
 			ExtBatch eb;
+			auto te = std::chrono::steady_clock::now();
 			ctx.extend(g, opt.ext, jobs, false, eb, ctx.readsBase());
+			stats.extendCallMs += msSince(te);
 			stats.rounds++;
 			stats.launches += eb.launches;
 			stats.reruns += eb.reruns;
@@ -978,6 +986,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				{
 					const uint64_t* raws[2] = {nullptr, nullptr};
 					uint32_t counts[2] = {0, 0};
+					thread_local std::vector<uint64_t> items[2];
 					for (int dir = 0; dir < 2; dir++)
 					{
 						int64_t j = dir == 0 ? st.bwdJob : st.fwdJob;
@@ -992,7 +1001,8 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 						check(res.status != gad::EXT_ASSERT);
 						if (res.status != gad::EXT_OK) continue;
 						check(res.ntrace > 0);
-						raws[dir] = eb.trace.data() + eb.traceOffset[j];
+						eb.items(g, j, items[dir]);
+						raws[dir] = items[dir].data();
 						counts[dir] = res.ntrace;
 					}
 					Alignment item = buildAlignment(g, opt, seq, L, s, raws[0], counts[0], raws[1], counts[1]);

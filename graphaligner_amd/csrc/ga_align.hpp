@@ -72,6 +72,8 @@ struct AlignStats
 	double hostMs = 0;          // host seeding, fix-up, selection, GAF
 	double totalMs = 0;
 	uint64_t rounds = 0, launches = 0, extensions = 0, wordSteps = 0, seeds = 0, reruns = 0;
+	double lookupCallMs = 0;    // wall time of the seed-lookup calls
+	double extendCallMs = 0;    // wall time of the extension calls (kernels, set-up, copies)
 };
 
 // MinimizerSeeder::getSeeds (src/MinimizerSeeder.cpp:521-555) from the device

@@ -2,9 +2,13 @@
 // index matches, and batched seed extensions over them. Context (ga_runtime.hpp)
 // is the gfx950 implementation; this header has no HIP dependency.
 #pragma once
+#include <algorithm>
 #include <cstdint>
+#include <stdexcept>
+#include <utility>
 #include <vector>
 #include "ga_device_types.h"
+#include "ga_trace_codes.h"
 #include "ga_graph.hpp"
 #include "ga_index.hpp"
 
@@ -22,18 +26,72 @@ struct ExtJob
 	uint64_t devOffset = 0;
 };
 
+// Traces come back in the compact form of ga_trace_codes.h: one code byte per
+// item, plus the escaped items whole.
 struct ExtBatch
 {
 	std::vector<gad::ExtResult> results;
-	std::vector<uint64_t> traceOffset; // per job, into trace
-	std::vector<uint64_t> trace;       // packed items, each job's trace end first
+	std::vector<uint64_t> traceOffset; // per job, into codes
+	const uint8_t* codes = nullptr;    // owned by the backend; valid until its next call
+	std::vector<std::pair<uint64_t, uint64_t>> escapes; // (index into codes, packed item), by index
 	std::vector<gad::SliceSummary> summaries;
 	std::vector<uint64_t> summaryOffset;
 	double kernelMs = 0;               // summed kernel time of all launches
 	uint32_t launches = 0;
 	uint64_t wordSteps = 0;
 	uint64_t reruns = 0;               // jobs rerun with larger workspaces
+
+	// packed items of job j's trace (results[j].ntrace of them, end first)
+	void items(const GraphHost& g, size_t j, std::vector<uint64_t>& out) const
+	{
+		const uint32_t n = results[j].ntrace;
+		out.resize(n);
+		if (n == 0) return;
+		const gad::TraceGraph tg{g.nodeLen.data(), g.inOff.data(), g.inAdj.data()};
+		const uint64_t base = traceOffset[j];
+		const uint8_t* c = codes + base;
+		auto e = std::lower_bound(escapes.begin(), escapes.end(), std::make_pair(base, (uint64_t)0));
+		uint64_t prev = 0;
+		for (uint32_t k = 0; k < n; k++)
+		{
+			if (c[k] == gad::kTraceEscape)
+			{
+				if (e == escapes.end() || e->first != base + k) throw std::runtime_error("trace codes: escaped item missing");
+				prev = e->second;
+				++e;
+			}
+			else
+			{
+				if (k == 0) throw std::runtime_error("trace codes: first item not escaped");
+				prev = gad::traceDecode(tg, prev, c[k]);
+			}
+			out[k] = prev;
+		}
+	}
 };
+
+// Encodes traces[j] (packed items, end first) of every job into out's codes
+// (stored in codeStore) and escapes, with out.traceOffset; the host form of the
+// device encoder, for backends that compute traces on the host.
+inline void encodeTraces(const GraphHost& g, const std::vector<std::vector<uint64_t>>& traces, std::vector<uint8_t>& codeStore, ExtBatch& out)
+{
+	const gad::TraceGraph tg{g.nodeLen.data(), g.inOff.data(), g.inAdj.data()};
+	codeStore.clear();
+	out.escapes.clear();
+	out.traceOffset.assign(traces.size(), 0);
+	for (size_t j = 0; j < traces.size(); j++)
+	{
+		out.traceOffset[j] = codeStore.size();
+		const std::vector<uint64_t>& t = traces[j];
+		for (size_t k = 0; k < t.size(); k++)
+		{
+			uint32_t code = k == 0 ? gad::kTraceEscape : gad::traceCode(tg, t[k - 1], t[k]);
+			if (code == gad::kTraceEscape) out.escapes.emplace_back(codeStore.size(), t[k]);
+			codeStore.push_back((uint8_t)code);
+		}
+	}
+	out.codes = codeStore.data();
+}
 
 // Reads resident on the device: forward bytes (and, once looked up, their
 // reverse complements) plus a host copy for the trace fix-ups and GAF text.

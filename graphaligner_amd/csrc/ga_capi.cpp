@@ -198,6 +198,7 @@ ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga
 		ExtBatch out;
 		ctx->ctx->extend(g, extParamsOf(*params), jobs, summaries != nullptr, out);
 		ga_status_t rc = GA_OK;
+		std::vector<uint64_t> items;
 		for (int64_t i = 0; i < n; i++)
 		{
 			const auto& r = out.results[i];
@@ -207,9 +208,10 @@ ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga
 			ntrace[i] = r.ntrace;
 			if (nslices) nslices[i] = r.nslices;
 			int64_t* t = traces + 4 * i * traceCap;
+			out.items(g, i, items);
 			for (uint32_t k = 0; k < r.ntrace; k++)
 			{
-				uint64_t v = out.trace[out.traceOffset[i] + k];
+				uint64_t v = items[k];
 				uint32_t hi = v >> 32;
 				t[4 * k] = (uint32_t)v;
 				t[4 * k + 1] = (hi >> 1) & 63;
@@ -372,6 +374,8 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& 
 		stats->wordSteps = st.wordSteps;
 		stats->seeds = st.seeds;
 		stats->reruns = st.reruns;
+		stats->lookupCallMs = st.lookupCallMs;
+		stats->extendCallMs = st.extendCallMs;
 	}
 	return GA_OK;
 }

@@ -37,7 +37,10 @@
 #define GA_HD inline
 #define GA_HHD inline
 #elif defined(GA_THREAD_PER_EXTENSION)
-// gfx950 build with one extension per thread: no cross-lane operations
+// gfx950 build with one extension per thread: no cross-lane operations. Every
+// routine is inlined: an out-of-line call would take the extension's state by
+// pointer into scratch memory and its arguments in vector registers, losing the
+// single-lane kernel's uniform (scalar) values and raising the register count
 #include <hip/hip_runtime.h>
 #define GA_HD __device__ __forceinline__
 #define GA_HHD __host__ __device__ inline
@@ -47,7 +50,28 @@
 #define GA_HHD __host__ __device__ inline
 #endif
 
+#ifndef GA_COLD
+#define GA_COLD GA_HD
+#endif
+
 namespace gad {
+
+// The gfx950 extension kernels run one lane per workgroup. Reading a value back
+// from the first (only) lane places it in scalar registers, so the column loop's
+// 64-bit word arithmetic and its branches run on the scalar unit. Identity in
+// every other build.
+#if defined(GA_THREAD_PER_EXTENSION) && !defined(GA_THREAD_HOST_TEST)
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int32_t uni32s(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v)
+{
+	return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) | (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+#else
+GA_HD uint32_t uni32(uint32_t v) { return v; }
+GA_HD int32_t uni32s(int32_t v) { return v; }
+GA_HD uint64_t uni64(uint64_t v) { return v; }
+#endif
 
 constexpr int32_t SMAX = 0x7fffffff;
 constexpr int32_t SMIN = (int32_t)0x80000000;
@@ -171,7 +195,7 @@ GA_HD uint32_t readBaseSet(unsigned char c)
 
 // differenceMasksBitTwiddle (src/WordSlice.h:566-664): rows where left < right and
 // rows where right < left, given scoreDifference = right.before - left.before
-GA_HD void diffMasks(uint64_t leftVP, uint64_t leftVN, uint64_t rightVP, uint64_t rightVN, int scoreDifference, uint64_t& leftSmaller, uint64_t& rightSmaller)
+GA_COLD void diffMasks(uint64_t leftVP, uint64_t leftVN, uint64_t rightVP, uint64_t rightVN, int scoreDifference, uint64_t& leftSmaller, uint64_t& rightSmaller)
 {
 	leftSmaller = 0;
 	rightSmaller = 0;
@@ -774,6 +798,18 @@ struct Extension
 	uint64_t* colVP;
 	uint64_t* colVN;
 	int32_t* colEnd;
+	// node -> item index of every kept slice: one open-addressing table per slice
+	// (entries node << 32 | item, ~0 empty), sized to a power of two at least
+	// twice the slice's item count; its first entry is the header's pad field
+	uint64_t* sliceTab;
+	uint32_t sliceTabCap, sliceTabUsed;
+	// node -> slot of the slice being computed: entries stamp << 32 | node, with
+	// the slot beside them; entries of earlier slices carry older stamps and
+	// count as empty, so the table is never cleared
+	uint64_t* slotTabKey;
+	uint32_t* slotTabVal;
+	uint32_t slotTabMask;
+	uint32_t slotStamp;
 	// outputs
 	uint64_t* trace;
 	uint32_t traceCap;
@@ -871,7 +907,32 @@ struct Extension
 	}
 
 	// --- band queue (ComponentPriorityQueue with insertion-order ties) ---
-	GA_HD int32_t findSlot(uint32_t node) const { return lpFind(slotNode, nSlots, node); }
+	GA_HD static uint32_t nodeHash(uint32_t node) { return (node * 2654435761u) ^ (node >> 15); }
+	GA_HD int32_t findSlot(uint32_t node) const
+	{
+		const uint64_t want = ((uint64_t)slotStamp << 32) | node;
+		for (uint32_t i = nodeHash(node) & slotTabMask, probes = 0; probes <= slotTabMask; i = (i + 1) & slotTabMask, probes++)
+		{
+			uint64_t k = slotTabKey[i];
+			if ((uint32_t)(k >> 32) != slotStamp) return -1;
+			if (k == want) return (int32_t)slotTabVal[i];
+		}
+		return -1;
+	}
+	GA_HD void addSlotIndex(uint32_t node, uint32_t slot)
+	{
+		const uint64_t want = ((uint64_t)slotStamp << 32) | node;
+		for (uint32_t i = nodeHash(node) & slotTabMask, probes = 0; probes <= slotTabMask; i = (i + 1) & slotTabMask, probes++)
+		{
+			if ((uint32_t)(slotTabKey[i] >> 32) != slotStamp)
+			{
+				slotTabKey[i] = want;
+				slotTabVal[i] = slot;
+				return;
+			}
+		}
+		fail(EXT_OVERFLOW, 10);
+	}
 
 	GA_HD void queueInsert(uint32_t comp, int32_t score, uint32_t target, const Slice& s, bool skipFirst)
 	{
@@ -880,6 +941,7 @@ struct Extension
 		{
 			if (nSlots >= caps.maxSlots) { fail(EXT_OVERFLOW, 2); return; }
 			slot = nSlots++;
+			addSlotIndex(target, (uint32_t)slot);
 			slotNode[slot] = target;
 			slotItem[slot] = -1;
 			slotHead[slot] = NIL;
@@ -1065,6 +1127,16 @@ struct Extension
 		if (Recalc) colSet(*cols, 0, ws);
 		uint64_t forceEq = prevExists ? ONES : (ONES ^ 1);
 		uint64_t HP = 0, HN = 0;
+		ws = mkslice(uni64(ws.VP), uni64(ws.VN), uni32s(ws.end));
+		forceMask = uni64(forceMask);
+		fixedHP = uni64(fixedHP);
+		fixedHN = uni64(fixedHN);
+		forceEq = uni64(forceEq);
+		L = uni32(L);
+		Eq4 eqU;
+		for (int b = 0; b < 4; b++) eqU.m[b] = uni64(eq.m[b]);
+		NodeSeq nseqU = nseq;
+		for (int b = 0; b < 4; b++) nseqU.w[b] = uni64(nseq.w[b]);
 		LaneAcc acc;
 		if (!Recalc)
 		{
@@ -1072,7 +1144,7 @@ struct Extension
 		}
 		for (uint32_t pos = 1; pos < L; pos++)
 		{
-			uint64_t Eq = eqOf(eq, nseq, pos) & forceEq;
+			uint64_t Eq = eqOf(eqU, nseqU, pos) & forceEq;
 			uint64_t Ph, Mh;
 			Slice nw = nextSlice(Eq, ws, (fixedHP >> pos) & 1, (fixedHN >> pos) & 1, Ph, Mh);
 			if ((forceMask >> pos) & 1)
@@ -1130,11 +1202,49 @@ struct Extension
 
 	GA_HD Item loadItem(uint32_t i) const { return items[i]; }
 	GA_HD void storeItem(uint32_t i, const Item& it) { items[i] = it; }
+	GA_HD static uint32_t sliceTabSize(uint32_t count)
+	{
+		uint32_t t = 4;
+		while (t < 2 * count) t <<= 1;
+		return t;
+	}
 	GA_HD int32_t findInSlice(uint32_t sliceIdx, uint32_t node) const
 	{
 		const SliceHdr& h = hdr[sliceIdx];
-		int32_t k = lpFind(itemNode + h.itemStart, h.itemCount, node);
-		return k < 0 ? -1 : (int32_t)(h.itemStart + k);
+		if (h.itemCount == 0) return -1;
+		const uint32_t mask = sliceTabSize(h.itemCount) - 1;
+		const uint64_t* t = sliceTab + h.pad;
+		for (uint32_t i = nodeHash(node) & mask, probes = 0; probes <= mask; i = (i + 1) & mask, probes++)
+		{
+			uint64_t e = t[i];
+			if (e == ~(uint64_t)0) return -1;
+			if ((uint32_t)(e >> 32) == node) return (int32_t)(uint32_t)e;
+		}
+		return -1;
+	}
+	// indexes the items of kept slice sliceIdx for findInSlice
+	GA_HD void buildSliceTable(uint32_t sliceIdx)
+	{
+		SliceHdr& h = hdr[sliceIdx];
+		const uint32_t size = sliceTabSize(h.itemCount);
+		if (sliceTabUsed + size > sliceTabCap)
+		{
+			fail(EXT_OVERFLOW, 9);
+			return;
+		}
+		h.pad = sliceTabUsed;
+		uint64_t* t = sliceTab + sliceTabUsed;
+		sliceTabUsed += size;
+		for (uint32_t i = 0; i < size; i++) t[i] = ~(uint64_t)0;
+		const uint32_t mask = size - 1;
+		for (uint32_t k = 0; k < h.itemCount; k++)
+		{
+			uint32_t item = h.itemStart + k;
+			uint32_t node = itemNode[item];
+			uint32_t i = nodeHash(node) & mask;
+			while (t[i] != ~(uint64_t)0) i = (i + 1) & mask;
+			t[i] = ((uint64_t)node << 32) | item;
+		}
 	}
 	GA_HD static Item defaultPrev()
 	{
@@ -1178,6 +1288,8 @@ struct Extension
 		hdr[0] = h0;
 		nItems = 1;
 		kept = 1;
+		buildSliceTable(0);
+		if (status != EXT_OK) return;
 		int32_t bestX = 0;
 		const int32_t b = p.bandwidth;
 		uint32_t numSlices = (len + 63) / 64;
@@ -1199,6 +1311,7 @@ struct Extension
 			}
 			const uint32_t sliceItemStart = nItems;
 			nSlots = 0;
+			slotStamp++;
 			nEx = 0;
 			nPend = 0;
 			seqCounter = 0;
@@ -1350,6 +1463,7 @@ struct Extension
 			}
 			hdr[kept] = hn;
 			kept++;
+			buildSliceTable(kept - 1);
 			if (hn.itemCount > maxBand) maxBand = hn.itemCount;
 		}
 	}
@@ -1372,7 +1486,7 @@ struct Extension
 		return mkpos((uint32_t)v, (hi >> 1) & 63, (int32_t)(hi >> 7) - 1);
 	}
 	// src/GraphAlignerBitvectorCommon.h:760-768
-	GA_HD void checkCircularity()
+	GA_COLD void checkCircularity()
 	{
 		for (uint32_t i = ntrace - 1; i-- > 0;)
 		{
@@ -1635,7 +1749,7 @@ struct Extension
 
 	// getReverseTraceFromTableExactEndPos + getReverseTraceFromTable
 	// (src/GraphAlignerBitvectorCommon.h:496-758), writing trace items end first.
-	GA_HD int32_t traceback()
+	GA_COLD int32_t traceback()
 	{
 		uint32_t bestIndex = 1;
 		for (uint32_t i = 1; i < kept; i++)
@@ -1697,6 +1811,7 @@ struct Extension
 		lpEq(seq, len, (currentSlice - 1) * 64, eq);
 		int32_t lastScore = SMAX - 1;
 		uint32_t steps = 0;
+		NodeSeq curSeq = loadNodeSeq(startPos.node);
 		while (back.seq != -1 && status == EXT_OK)
 		{
 			if (++steps > traceCap)
@@ -1718,6 +1833,7 @@ struct Extension
 				}
 				recalc(currentSlice, currentNode, eq, cols);
 				if (status != EXT_OK) break;
+				curSeq = loadNodeSeq(currentNode);
 			}
 			const SliceHdr& ch = hdr[currentSlice];
 			const SliceHdr& ph = hdr[currentSlice - 1];
@@ -1784,44 +1900,51 @@ struct Extension
 				checkCircularity();
 				continue;
 			}
-			// pickBacktraceInside (src/GraphAlignerBitvectorCommon.h:770-815)
+			// pickBacktraceInside (src/GraphAlignerBitvectorCommon.h:770-815); the two
+			// columns in use stay in registers until the walk leaves one of them
 			uint32_t hori = bk.off;
 			int32_t vert = bk.seq - cj;
-			while (hori > 0 && vert > 0 && status == EXT_OK)
+			if (hori > 0 && vert > 0)
 			{
 				Slice ch0 = colGet(cols, hori);
 				Slice ch1 = colGet(cols, hori - 1);
-				int32_t here = valueAt(ch0, vert);
-				int32_t vertical = valueAt(ch0, vert - 1);
-				int32_t horizontal = valueAt(ch1, vert);
-				int32_t diagonal = valueAt(ch1, vert - 1);
-				bool eqc = charMatch(vert + cj, bk.node, hori);
-				if (vertical < here - 1 || horizontal < here - 1 || diagonal < here - (eqc ? 0 : 1))
+				// characterMatch of read row vert and the node's base at hori, read
+				// from the slice's match masks (rows past the read end match nothing)
+				uint64_t eqm = eqOf(eq, curSeq, hori);
+				while (hori > 0 && vert > 0 && status == EXT_OK)
 				{
-					fail(EXT_ASSERT);
-					break;
-				}
-				if (vertical == here - 1)
-				{
-					vert--;
-					pushTrace(mkpos(bk.node, hori, vert + cj), false);
-					continue;
-				}
-				if (diagonal == here - (eqc ? 0 : 1))
-				{
+					int32_t here = valueAt(ch0, vert);
+					int32_t vertical = valueAt(ch0, vert - 1);
+					int32_t horizontal = valueAt(ch1, vert);
+					int32_t diagonal = valueAt(ch1, vert - 1);
+					bool eqc = ((eqm >> vert) & 1) != 0;
+					if (vertical < here - 1 || horizontal < here - 1 || diagonal < here - (eqc ? 0 : 1))
+					{
+						fail(EXT_ASSERT);
+						break;
+					}
+					if (vertical == here - 1)
+					{
+						vert--;
+						pushTrace(mkpos(bk.node, hori, vert + cj), false);
+						continue;
+					}
+					if (diagonal == here - (eqc ? 0 : 1))
+						vert--;
+					else if (horizontal != here - 1)
+					{
+						fail(EXT_ASSERT);
+						break;
+					}
 					hori--;
-					vert--;
 					pushTrace(mkpos(bk.node, hori, vert + cj), false);
-					continue;
+					if (hori > 0)
+					{
+						ch0 = ch1;
+						ch1 = colGet(cols, hori - 1);
+						eqm = eqOf(eq, curSeq, hori);
+					}
 				}
-				if (horizontal == here - 1)
-				{
-					hori--;
-					pushTrace(mkpos(bk.node, hori, vert + cj), false);
-					continue;
-				}
-				fail(EXT_ASSERT);
-				break;
 			}
 		}
 		if (status != EXT_OK) return 0;
@@ -1885,6 +2008,9 @@ struct Extension
 		maxBand = 0;
 		ntrace = 0;
 		nItems = kept = nSlots = nEx = nPend = seqCounter = 0;
+		sliceTabUsed = 0;
+		slotStamp = 0;
+		for (uint32_t i = 0; i <= slotTabMask; i++) slotTabKey[i] = 0;
 		back = mkpos(NIL, 0, 0);
 		forward();
 		if (summaries)
@@ -1928,6 +2054,16 @@ struct Extension
 };
 
 GA_HHD uint64_t alignUp(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
+// Entries of the per-slice node tables: each slice's table has at most
+// 4 * itemCount entries and at least 4.
+GA_HHD uint64_t sliceTabEntries(const ArenaCaps& c) { return 4 * (uint64_t)c.maxItems + 4 * (uint64_t)c.maxSlices; }
+// Entries of the slot table: a power of two at least twice the slot budget.
+GA_HHD uint32_t slotTabSize(const ArenaCaps& c)
+{
+	uint32_t t = 4;
+	while (t < 2 * c.maxSlots) t <<= 1;
+	return t;
+}
 
 // Bytes of one wave's workspace for the given capacities.
 GA_HHD uint64_t arenaBytes(const ArenaCaps& c)
@@ -1937,6 +2073,8 @@ GA_HHD uint64_t arenaBytes(const ArenaCaps& c)
 	b += alignUp((uint64_t)c.maxItems * sizeof(Item));
 	b += alignUp((uint64_t)c.maxItems * 4);
 	b += 5 * alignUp((uint64_t)c.maxSlots * 4);
+	b += alignUp(sliceTabEntries(c) * 8);
+	b += alignUp((uint64_t)slotTabSize(c) * 8) + alignUp((uint64_t)slotTabSize(c) * 4);
 	b += 2 * alignUp((uint64_t)c.maxExtras * 8) + 2 * alignUp((uint64_t)c.maxExtras * 4);
 	b += alignUp((uint64_t)c.maxPending * 8) + 2 * alignUp((uint64_t)c.maxPending * 4);
 	b += 2 * alignUp(64 * 8) + alignUp(64 * 4);
@@ -1960,6 +2098,11 @@ GA_HD void carveArena(Extension& e, unsigned char* base, const ArenaCaps& c)
 	e.slotHead = (uint32_t*)take((uint64_t)c.maxSlots * 4);
 	e.slotTail = (uint32_t*)take((uint64_t)c.maxSlots * 4);
 	e.slotActive = (uint32_t*)take((uint64_t)c.maxSlots * 4);
+	e.sliceTab = (uint64_t*)take(sliceTabEntries(c) * 8);
+	e.sliceTabCap = (uint32_t)sliceTabEntries(c);
+	e.slotTabKey = (uint64_t*)take((uint64_t)slotTabSize(c) * 8);
+	e.slotTabVal = (uint32_t*)take((uint64_t)slotTabSize(c) * 4);
+	e.slotTabMask = slotTabSize(c) - 1;
 	e.exVP = (uint64_t*)take((uint64_t)c.maxExtras * 8);
 	e.exVN = (uint64_t*)take((uint64_t)c.maxExtras * 8);
 	e.exEnd = (int32_t*)take((uint64_t)c.maxExtras * 4);
@@ -1978,10 +2121,19 @@ GA_HHD uint32_t requestSlices(uint32_t len) { return len / 64 + 3; }
 GA_HHD uint32_t requestItems(uint32_t len, const ArenaCaps& caps) { return requestSlices(len) * caps.maxItems + 1024; }
 
 // Runs one request on the calling wave.
-GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out)
+// colStore, when given, holds the traceback's column store (64 VP words, 64 VN
+// words, 64 scores) in place of the arena's copy: the gfx950 kernels pass an
+// LDS array, which the backtrace reads at every step.
+GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out, uint64_t* colStore = nullptr)
 {
 	Extension e;
 	carveArena(e, arena, caps);
+	if (colStore)
+	{
+		e.colVP = colStore;
+		e.colVN = colStore + 64;
+		e.colEnd = (int32_t*)(colStore + 128);
+	}
 	e.g = g;
 	e.p = p;
 	e.seq = seqbuf + r.seqOffset;

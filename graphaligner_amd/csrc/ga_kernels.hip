@@ -9,6 +9,7 @@
 #include "ga_read_core.h"
 #include "ga_kernels.hpp"
 #include "ga_seed_core.h"
+#include "ga_trace_codes.h"
 
 namespace gad {
 
@@ -27,9 +28,32 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGra
 	results[idx] = out;
 }
 
+// One request per single-thread workgroup: the compiler knows a single lane
+// runs, so every value is wavefront-uniform and the extension's control flow and
+// most of its 64-bit word arithmetic run on the scalar unit, with no exec-mask
+// handling for its data-dependent branches.
+__global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets)
+{
+	__shared__ uint64_t colStore[64 * 2 + 32];
+	uint32_t idx = blockIdx.x;
+	if (idx >= nreq) return;
+	ExtRequest r = reqs[idx];
+	ArenaCaps c = caps;
+	c.maxSlices = requestSlices(r.len);
+	c.maxItems = requestItems(r.len, caps);
+	ExtResult out;
+	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, colStore);
+	results[idx] = out;
+}
+
 hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets)
 {
 	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
+	if (lanesPerWave == 1 && wavesPerSimd <= 1)
+	{
+		hipLaunchKernelGGL(extendSingleLaneKernel, dim3(nreq), dim3(1), 0, stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+		return hipGetLastError();
+	}
 	uint32_t blocks = (nreq + lanesPerWave - 1) / lanesPerWave;
 	// register budget variants: more resident waves per SIMD for fewer VGPRs
 	if (wavesPerSimd >= 8)
@@ -59,9 +83,28 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) readLoopKernel(DevGraph g
 	results[idx] = out;
 }
 
+__global__ void __launch_bounds__(1) readLoopSingleLaneKernel(DevGraph g, ReadLoopParams p, const ReadJob* __restrict__ jobs, uint32_t njobs, const char* __restrict__ reads, const DevSeed* __restrict__ seeds, uint64_t* __restrict__ rawTraces, uint64_t* __restrict__ pools, DevAlignment* __restrict__ alns, ReadResult* __restrict__ results, unsigned char* __restrict__ arenas, ArenaCaps caps)
+{
+	__shared__ uint64_t colStore[64 * 2 + 32];
+	uint32_t idx = blockIdx.x;
+	if (idx >= njobs) return;
+	const ReadJob j = jobs[idx];
+	ArenaCaps c = caps;
+	c.maxSlices = requestSlices(j.len);
+	c.maxItems = requestItems(j.len, caps);
+	ReadResult out;
+	runReadLoop(g, p, j, reads, seeds + j.seedStart, rawTraces + j.traceOffset, pools + j.poolOffset, alns + j.alnOffset, arenas + j.arenaOffset, c, out, colStore);
+	results[idx] = out;
+}
+
 hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, const ArenaCaps& caps)
 {
 	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
+	if (lanesPerWave == 1 && wavesPerSimd <= 1)
+	{
+		hipLaunchKernelGGL(readLoopSingleLaneKernel, dim3(njobs), dim3(1), 0, stream, g, p, jobs, njobs, reads, seeds, rawTraces, pools, alns, results, arenas, caps);
+		return hipGetLastError();
+	}
 	uint32_t blocks = (njobs + lanesPerWave - 1) / lanesPerWave;
 	if (wavesPerSimd >= 4)
 		hipLaunchKernelGGL(readLoopKernel<4>, dim3(blocks), dim3(64), 0, stream, g, p, jobs, njobs, lanesPerWave, reads, seeds, rawTraces, pools, alns, results, arenas, caps);
@@ -76,6 +119,39 @@ uint64_t requestArenaBytes(uint32_t len, const ArenaCaps& caps)
 	c.maxSlices = requestSlices(len);
 	c.maxItems = requestItems(len, caps);
 	return arenaBytes(c);
+}
+
+// One workgroup per trace range, one item per thread and step: the code of item i
+// needs only items i - 1 and i, so the range is encoded in parallel.
+__global__ void traceCodesKernel(TraceGraph tg, const uint64_t* __restrict__ src, const uint64_t* __restrict__ srcOff, const uint32_t* __restrict__ count, const uint64_t* __restrict__ dstOff, uint32_t nranges, uint8_t* __restrict__ codes, uint64_t* __restrict__ escapes, uint32_t* __restrict__ escapeCount, uint64_t escapeCap)
+{
+	uint32_t r = blockIdx.x;
+	if (r >= nranges) return;
+	const uint64_t* s = src + srcOff[r];
+	const uint64_t d = dstOff[r];
+	const uint32_t c = count[r];
+	for (uint32_t i = threadIdx.x; i < c; i += blockDim.x)
+	{
+		const uint64_t cur = s[i];
+		const uint32_t code = i == 0 ? kTraceEscape : traceCode(tg, s[i - 1], cur);
+		codes[d + i] = (uint8_t)code;
+		if (code == kTraceEscape)
+		{
+			uint32_t slot = atomicAdd(escapeCount, 1u);
+			if (slot < escapeCap)
+			{
+				escapes[2 * (uint64_t)slot] = d + i;
+				escapes[2 * (uint64_t)slot + 1] = cur;
+			}
+		}
+	}
+}
+
+hipError_t launchTraceCodes(hipStream_t stream, const DevGraph& g, const uint64_t* src, const uint64_t* srcOff, const uint32_t* count, const uint64_t* dstOff, uint32_t nranges, uint8_t* codes, uint64_t* escapes, uint32_t* escapeCount, uint64_t escapeCap)
+{
+	TraceGraph tg{g.nodeLen, g.inOff, g.inAdj};
+	hipLaunchKernelGGL(traceCodesKernel, dim3(nranges), dim3(256), 0, stream, tg, src, srcOff, count, dstOff, nranges, codes, escapes, escapeCount, escapeCap);
+	return hipGetLastError();
 }
 
 } // namespace gad

@@ -26,6 +26,12 @@ hipError_t launchGatherMatches(hipStream_t stream, const SeedMatch* matches, con
 // Copies src[srcOff[i], + count[i]) to dst[dstOff[i], ...) for every range i.
 hipError_t launchGatherU64(hipStream_t stream, const uint64_t* src, const uint64_t* srcOff, const uint32_t* count, const uint64_t* dstOff, uint32_t nranges, uint64_t* dst);
 
+// Compact copy of the trace ranges src[srcOff[i], + count[i]): one code byte per
+// item (ga_trace_codes.h) to codes[dstOff[i], ...); every escaped item is
+// appended to escapes as the pair (index into codes, packed item), counted in
+// *escapeCount (which the caller zeroes; escapeCap pairs fit).
+hipError_t launchTraceCodes(hipStream_t stream, const DevGraph& g, const uint64_t* src, const uint64_t* srcOff, const uint32_t* count, const uint64_t* dstOff, uint32_t nranges, uint8_t* codes, uint64_t* escapes, uint32_t* escapeCount, uint64_t escapeCap);
+
 hipError_t launchRevcomp(hipStream_t stream, const char* reads, char* rev, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, uint32_t maxLen);
 
 } // namespace gad

@@ -24,6 +24,22 @@ void DeviceBuffer::reserve(size_t b)
 	bytes = want;
 }
 
+PinnedBuffer::~PinnedBuffer()
+{
+	if (ptr) (void)hipHostFree(ptr);
+}
+
+void PinnedBuffer::reserve(size_t b)
+{
+	if (b <= bytes && ptr) return;
+	if (ptr) GA_HIP(hipHostFree(ptr));
+	ptr = nullptr;
+	bytes = 0;
+	size_t want = std::max<size_t>(b, 4096);
+	GA_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+	bytes = want;
+}
+
 uint32_t Context::lanesPerWave(uint32_t nreq) const
 {
 	(void)nreq;
@@ -289,11 +305,12 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 	GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
 	out.kernelMs += ms;
 	out.launches++;
-	// traces: gather the used prefix of every kept job's trace region into one
-	// contiguous block on the device, then copy only that block
+	// traces: encode the used prefix of every kept job's trace region into one
+	// contiguous block of code bytes on the device, then copy only that block
+	// and the escaped items
 	std::vector<uint64_t> srcOff, dstOff;
 	std::vector<uint32_t> cnt;
-	const uint64_t base = out.trace.size();
+	const uint64_t base = hostCodesUsed_;
 	uint64_t total = 0;
 	for (uint32_t k = 0; k < n; k++)
 	{
@@ -318,20 +335,34 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 			out.summaries.insert(out.summaries.end(), sm.begin() + sumOff[k], sm.begin() + sumOff[k] + r.nslices);
 		}
 	}
-	out.trace.resize(base + total);
+	if (base + total > hostCodes_.bytes) throw HipError("trace codes exceed the host staging buffer");
+	hostCodesUsed_ = base + total;
 	if (total > 0)
 	{
 		const uint32_t m = cnt.size();
 		gatherSrc_.reserve(m * sizeof(uint64_t));
 		gatherDst_.reserve(m * sizeof(uint64_t));
 		gatherCnt_.reserve(m * sizeof(uint32_t));
-		traceOut_.reserve(total * sizeof(uint64_t));
+		traceCodes_.reserve(total);
+		traceEscapes_.reserve(total * 2 * sizeof(uint64_t));
+		escapeCount_.reserve(sizeof(uint32_t));
 		GA_HIP(hipMemcpyAsync(gatherSrc_.ptr, srcOff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
 		GA_HIP(hipMemcpyAsync(gatherDst_.ptr, dstOff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
 		GA_HIP(hipMemcpyAsync(gatherCnt_.ptr, cnt.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
-		GA_HIP(gad::launchGatherU64(stream_, traces_.as<uint64_t>(), gatherSrc_.as<uint64_t>(), gatherCnt_.as<uint32_t>(), gatherDst_.as<uint64_t>(), m, traceOut_.as<uint64_t>()));
-		GA_HIP(hipMemcpyAsync(out.trace.data() + base, traceOut_.ptr, total * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+		GA_HIP(hipMemsetAsync(escapeCount_.ptr, 0, sizeof(uint32_t), stream_));
+		GA_HIP(gad::launchTraceCodes(stream_, dg, traces_.as<uint64_t>(), gatherSrc_.as<uint64_t>(), gatherCnt_.as<uint32_t>(), gatherDst_.as<uint64_t>(), m, traceCodes_.as<uint8_t>(), traceEscapes_.as<uint64_t>(), escapeCount_.as<uint32_t>(), total));
+		uint32_t nesc = 0;
+		GA_HIP(hipMemcpyAsync(&nesc, escapeCount_.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+		GA_HIP(hipMemcpyAsync(hostCodes_.as<uint8_t>() + base, traceCodes_.ptr, total, hipMemcpyDeviceToHost, stream_));
 		GA_HIP(hipStreamSynchronize(stream_));
+		if (nesc > total) throw HipError("trace encoder returned an inconsistent escape count");
+		std::vector<uint64_t> esc(2 * (size_t)nesc);
+		if (nesc > 0)
+		{
+			GA_HIP(hipMemcpyAsync(esc.data(), traceEscapes_.ptr, esc.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+			GA_HIP(hipStreamSynchronize(stream_));
+		}
+		for (uint32_t k = 0; k < nesc; k++) out.escapes.emplace_back(base + esc[2 * k], esc[2 * k + 1]);
 	}
 }
 
@@ -342,8 +373,15 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 	out.results.assign(jobs.size(), gad::ExtResult{});
 	out.traceOffset.assign(jobs.size(), 0);
 	out.summaryOffset.assign(jobs.size(), 0);
-	out.trace.clear();
+	out.escapes.clear();
 	out.summaries.clear();
+	// every job's codes land once (overflowed attempts copy none), so the sum of
+	// the trace capacities bounds this call's code bytes
+	uint64_t codeBound = 0;
+	for (const ExtJob& j : jobs) codeBound += traceCapFor(j.len);
+	hostCodes_.reserve(codeBound);
+	hostCodesUsed_ = 0;
+	out.codes = hostCodes_.as<uint8_t>();
 	std::vector<uint32_t> which(jobs.size());
 	for (uint32_t i = 0; i < jobs.size(); i++) which[i] = i;
 	// longest first: the 64 threads of a wave then carry similar lengths
@@ -359,6 +397,7 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 		if (attempt + 1 < kArenaAttempts) out.reruns += which.size();
 		growArenaCaps(caps);
 	}
+	std::sort(out.escapes.begin(), out.escapes.end());
 }
 
 static uint64_t roundUp256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }

@@ -38,6 +38,21 @@ struct DeviceBuffer
 	T* as() const { return (T*)ptr; }
 };
 
+// Page-locked host memory (hipHostMalloc) for device-to-host copies at the full
+// link rate.
+struct PinnedBuffer
+{
+	void* ptr = nullptr;
+	size_t bytes = 0;
+	PinnedBuffer() = default;
+	PinnedBuffer(const PinnedBuffer&) = delete;
+	PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+	~PinnedBuffer();
+	void reserve(size_t b); // grows, never shrinks; contents are not kept
+	template <typename T>
+	T* as() const { return (T*)ptr; }
+};
+
 struct DeviceGraph
 {
 	uint64_t uid = 0;
@@ -94,7 +109,9 @@ private:
 	std::vector<std::unique_ptr<DeviceGraph>> graphs_;
 	std::vector<std::unique_ptr<DeviceIndex>> indexes_;
 	DeviceBuffer arena_, arenaOffsets_, reqs_, seqs_, traces_, results_, sums_, sumOffsets_;
-	DeviceBuffer gatherSrc_, gatherDst_, gatherCnt_, traceOut_;
+	DeviceBuffer gatherSrc_, gatherDst_, gatherCnt_, traceOut_, traceCodes_, traceEscapes_, escapeCount_;
+	PinnedBuffer hostCodes_;  // compact traces of the current extend call
+	uint64_t hostCodesUsed_ = 0;
 	DeviceBuffer rlSeeds_, rlJobs_, rlTraces_, rlPool_, rlAlns_, rlResults_;
 	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_;
 	ReadSet reads_;

@@ -141,6 +141,8 @@ typedef struct
 	int64_t wordSteps;     /* forward 64-row column steps of the bit-parallel DP */
 	int64_t seeds;         /* seeds found (before the seed loop's skips) */
 	int64_t reruns;        /* reads rerun with larger device workspaces */
+	double lookupCallMs;   /* wall time of the seed-lookup calls: kernels, copies and match gathering */
+	double extendCallMs;   /* wall time of the extension calls: kernels, workspace set-up and copies */
 } ga_align_stats_t;
 
 /* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed

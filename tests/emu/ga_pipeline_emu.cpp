@@ -131,7 +131,6 @@ public:
 		out.results.assign(n, gad::ExtResult{});
 		out.traceOffset.assign(n, 0);
 		out.summaryOffset.assign(n, 0);
-		out.trace.clear();
 		out.summaries.clear();
 		std::vector<std::vector<uint64_t>> traces(n);
 		std::atomic<uint64_t> reruns{0};
@@ -162,12 +161,8 @@ public:
 			}
 			traces[i].resize(out.results[i].status == gad::EXT_OVERFLOW ? 0 : out.results[i].ntrace);
 		});
-		for (size_t i = 0; i < n; i++)
-		{
-			out.traceOffset[i] = out.trace.size();
-			out.trace.insert(out.trace.end(), traces[i].begin(), traces[i].end());
-			out.wordSteps += out.results[i].wordSteps;
-		}
+		for (size_t i = 0; i < n; i++) out.wordSteps += out.results[i].wordSteps;
+		ga::encodeTraces(G, traces, codeStore_, out);
 		out.launches++;
 		out.reruns = reruns.load();
 	}
@@ -221,6 +216,7 @@ public:
 	uint64_t reverseBase() const override { return revBase_; }
 
 private:
+	std::vector<uint8_t> codeStore_;
 	std::vector<uint32_t> bigSize_;
 	uint64_t bigSizeUid_ = 0;
 	ga::ReadSet rs_;
