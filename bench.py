@@ -115,7 +115,8 @@ def main():
 
     ctx.synchronize()
     t0 = time.perf_counter()
-    tot = dict(extendKernelMs=0.0, lookupMs=0.0, hostMs=0.0, extendCallMs=0.0, lookupCallMs=0.0, launches=0, extensions=0, wordSteps=0, rounds=0, seeds=0, reruns=0)
+    tot = dict(extendKernelMs=0.0, lookupMs=0.0, hostMs=0.0, extendCallMs=0.0, lookupCallMs=0.0, totalMs=0.0,
+               seedingMs=0.0, pickMs=0.0, replayMs=0.0, gafMs=0.0, unusedExtensions=0, launches=0, extensions=0, wordSteps=0, rounds=0, seeds=0, reruns=0)
     gaf = status = None
     for k in range(args.steps):
         ts = time.perf_counter()
@@ -163,8 +164,11 @@ def main():
                    "seed_loop": "device" if args.device_seed_loop else "rounds"},
         "breakdown_s_per_step": {"extend_kernel": kernel_s / steps, "extend_calls": tot["extendCallMs"] / 1e3 / steps,
                                  "seed_lookup_kernel": tot["lookupMs"] / 1e3 / steps,
-                                 "seed_lookup_calls": tot["lookupCallMs"] / 1e3 / steps, "host": tot["hostMs"] / 1e3 / steps},
-        "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "wordSteps", "seeds", "reruns")},
+                                 "seed_lookup_calls": tot["lookupCallMs"] / 1e3 / steps, "host": tot["hostMs"] / 1e3 / steps,
+                                 "host_seeding": tot["seedingMs"] / 1e3 / steps, "host_pick": tot["pickMs"] / 1e3 / steps,
+                                 "host_replay": tot["replayMs"] / 1e3 / steps, "host_gaf": tot["gafMs"] / 1e3 / steps,
+                                 "align_call": tot["totalMs"] / 1e3 / steps},
+        "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "unusedExtensions", "wordSteps", "seeds", "reruns")},
         "read_status_counts": counts,
         "roofline": {"kernel": "extendThreadKernel", "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,

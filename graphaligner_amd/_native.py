@@ -30,7 +30,8 @@ class Params(ctypes.Structure):
                 ("multimapScoreFraction", ctypes.c_double), ("minAlignmentScore", ctypes.c_double),
                 ("minimizerLength", ctypes.c_int32), ("minimizerWindowSize", ctypes.c_int32),
                 ("minimizerSeedDensity", ctypes.c_double), ("minimizerDiscardMostNumerousFraction", ctypes.c_double),
-                ("cigarMatchMismatchMerge", ctypes.c_int32), ("deviceSeedLoop", ctypes.c_int32)]
+                ("cigarMatchMismatchMerge", ctypes.c_int32), ("deviceSeedLoop", ctypes.c_int32),
+                ("maxSpeculativeSeeds", ctypes.c_int32)]
 
 
 class SliceSummary(ctypes.Structure):
@@ -45,7 +46,9 @@ class AlignStats(ctypes.Structure):
     _fields_ = [("lookupMs", ctypes.c_double), ("extendKernelMs", ctypes.c_double), ("hostMs", ctypes.c_double),
                 ("totalMs", ctypes.c_double), ("rounds", ctypes.c_int64), ("launches", ctypes.c_int64),
                 ("extensions", ctypes.c_int64), ("wordSteps", ctypes.c_int64), ("seeds", ctypes.c_int64),
-                ("reruns", ctypes.c_int64), ("lookupCallMs", ctypes.c_double), ("extendCallMs", ctypes.c_double)]
+                ("reruns", ctypes.c_int64), ("lookupCallMs", ctypes.c_double), ("extendCallMs", ctypes.c_double),
+                ("unusedExtensions", ctypes.c_int64), ("seedingMs", ctypes.c_double), ("pickMs", ctypes.c_double),
+                ("replayMs", ctypes.c_double), ("gafMs", ctypes.c_double)]
 
 
 _lib = None

@@ -5,8 +5,11 @@
 #include <atomic>
 #include <chrono>
 #include <climits>
+#include <condition_variable>
+#include <exception>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <thread>
@@ -263,12 +266,12 @@ void fixReverse(const GraphHost& g, OneWay& t, uint64_t end, const char* seq, ui
 	for (TItem& it : t.trace)
 	{
 		it.seqPos = end - it.seqPos;
+		// GraphHost::reversePosition with the split node's bigraph size
 		uint64_t split = it.node;
 		int id = g.nodeId[split];
 		uint64_t offset = g.nodeOff[split] + it.nodeOffset;
-		auto rp = g.reversePosition(id, offset);
-		it.node = (uint64_t)(int64_t)rp.first;
-		it.nodeOffset = rp.second;
+		it.node = (uint64_t)(int64_t)((id % 2 == 0) ? (id / 2) * 2 + 1 : (id / 2) * 2);
+		it.nodeOffset = g.splitBigSize[split] - offset - 1;
 		check(it.seqPos < len);
 		it.seqChar = seq[it.seqPos];
 		char c = complementOf(it.graphChar);
@@ -451,6 +454,13 @@ std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, cons
 }
 
 // Seed loop state of one read (GraphAligner::AlignOneWay, src/GraphAligner.h:94-181).
+// One seed of a round with its device jobs (-1: none).
+struct Spec
+{
+	size_t seed;
+	int64_t bwdJob, fwdJob;
+};
+
 struct ReadState
 {
 	uint32_t read = 0;
@@ -463,9 +473,15 @@ struct ReadState
 	uint64_t extendSeeds = 0;
 	bool done = false;
 	int status = READ_ALIGNED;
-	// seed being extended this round and its device jobs (-1: none)
+	// seed the loop picked last and, in the host-driven rounds, whether it is
+	// still to be extended
 	size_t pending = 0;
 	int64_t bwdJob = -1, fwdJob = -1;
+	bool pendingReady = false;
+	// seeds extended in the current round (the picked one first) and the read's
+	// rounds so far
+	std::vector<Spec> specs;
+	uint32_t roundsRun = 0;
 };
 
 struct Job
@@ -803,18 +819,34 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 
 namespace {
 
-// Picks the next seed of the read's loop to extend (src/GraphAligner.h:105-140);
-// returns false when the loop is over.
-bool advance(ReadState& st, const AlignOptions& opt)
+// Position of a read's seed loop: the next seed to look at and the counters the
+// loop's stop rule reads.
+struct LoopCursor
 {
-	while (st.next < st.seeds.size())
+	size_t next;
+	size_t pending;
+	uint64_t worstExtendedSeedScore;
+	uint64_t seedsExtended;
+};
+
+LoopCursor cursorOf(const ReadState& st)
+{
+	return LoopCursor{st.next, st.pending, st.worstExtendedSeedScore, st.seedsExtended};
+}
+
+// Picks the next seed of the read's loop to extend (src/GraphAligner.h:105-140)
+// against the read's alignments so far, moving the cursor; returns false when
+// the loop is over.
+bool pickNext(const ReadState& st, LoopCursor& c, const AlignOptions& opt)
+{
+	while (c.next < st.seeds.size())
 	{
-		const SeedHit& s = st.seeds[st.next];
+		const SeedHit& s = st.seeds[c.next];
 		if (opt.sloppyOptimizations && s.seedGoodness <= st.seedScoreForEndToEndAln) return false;
-		if (st.seedsExtended >= st.extendSeeds && s.seedGoodness < st.worstExtendedSeedScore) return false;
+		if (c.seedsExtended >= st.extendSeeds && s.seedGoodness < c.worstExtendedSeedScore) return false;
 		if (s.seedClusterSize < opt.minSeedClusterSize)
 		{
-			st.next++;
+			c.next++;
 			continue;
 		}
 		bool covered = false;
@@ -842,15 +874,56 @@ bool advance(ReadState& st, const AlignOptions& opt)
 		}
 		if (covered)
 		{
-			st.next++;
+			c.next++;
 			continue;
 		}
-		st.worstExtendedSeedScore = s.seedGoodness;
-		st.seedsExtended++;
-		st.pending = st.next;
+		c.worstExtendedSeedScore = s.seedGoodness;
+		c.seedsExtended++;
+		c.pending = c.next;
 		return true;
 	}
 	return false;
+}
+
+bool advance(ReadState& st, const AlignOptions& opt)
+{
+	LoopCursor c = cursorOf(st);
+	bool more = pickNext(st, c, opt);
+	st.next = c.next;
+	st.pending = c.pending;
+	st.worstExtendedSeedScore = c.worstExtendedSeedScore;
+	st.seedsExtended = c.seedsExtended;
+	return more;
+}
+
+// The sloppy loop's end-to-end rule after an alignment is added
+// (src/GraphAligner.h:160-176): when alignments chained by overlap span the
+// whole read, later seeds no better than the chain's worst seed are not tried.
+void updateEndToEnd(ReadState& st, uint64_t readLen)
+{
+	auto& al = st.alignments;
+	std::sort(al.begin(), al.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
+	if (al[0].alignmentStart != 0) return;
+	uint64_t minGoodness = al[0].seedGoodness;
+	uint64_t contiguousEnd = al[0].alignmentEnd;
+	for (size_t k = 1; k < al.size(); k++)
+	{
+		if (al[k].alignmentStart <= contiguousEnd)
+		{
+			minGoodness = std::min(minGoodness, al[k].seedGoodness);
+			contiguousEnd = std::max(contiguousEnd, al[k].alignmentEnd);
+		}
+	}
+	if (contiguousEnd == readLen) st.seedScoreForEndToEndAln = minGoodness;
+}
+
+// Seeds extended together in a read's round r (0-based): the first round only
+// the picked seed, then twice as many each round, up to maxSeeds.
+uint32_t speculationWidth(uint32_t round, uint32_t maxSeeds)
+{
+	if (round == 0 || maxSeeds <= 1) return 1;
+	uint64_t w = (uint64_t)1 << std::min<uint32_t>(round, 20);
+	return (uint32_t)std::min<uint64_t>(w, maxSeeds);
 }
 
 } // namespace
@@ -910,136 +983,258 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				st.done = true;
 			}
 		});
-		stats.hostMs += msSince(th);
+		{ double ms = msSince(th); stats.hostMs += ms; stats.seedingMs += ms; }
 		for (auto& st : states) stats.seeds += st.seeds.size();
-		std::vector<uint32_t> active;
+		// the seed loops of G groups of reads (G = the backend's extension lanes, at
+		// most opt.pipelineGroups) run in G host threads, each on its own lane,
+		// so one group's host replay overlaps the other groups' kernels
+		std::vector<uint32_t> order;
 		for (uint32_t i = 0; i < bn; i++)
-			if (!states[i].done) active.push_back(i);
-		while (!active.empty())
+			if (!states[i].done) order.push_back(i);
+		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return blens[a] > blens[b]; });
+		const uint32_t groups = std::max<uint32_t>(1, std::min<uint32_t>({ctx.extendLanes(), opt.pipelineGroups, (uint32_t)std::max<size_t>(1, order.size())}));
+		std::vector<std::vector<uint32_t>> groupReads(groups);
+		for (size_t k = 0; k < order.size(); k++) groupReads[k % groups].push_back(order[k]);
+		for (auto& gr : groupReads) std::sort(gr.begin(), gr.end());
+		std::vector<AlignStats> groupStats(groups);
+		std::vector<std::exception_ptr> groupErrors(groups);
+		const int threads = opt.hostThreads;
+		// group k's first launch waits until k earlier first launches have
+		// returned: the groups start out of phase, so while one group's host
+		// work runs another's kernel has the GPU
+		std::mutex phaseMutex;
+		std::condition_variable phaseCv;
+		uint32_t firstLaunchesDone = 0;
+		struct FirstLaunch
 		{
-			th = std::chrono::steady_clock::now();
-			std::vector<uint8_t> extendNow(active.size(), 0);
-			parallelFor(active.size(), opt.hostThreads, [&](size_t a) {
-				ReadState& st = states[active[a]];
-				try
-				{
-					extendNow[a] = advance(st, opt) ? 1 : 0;
-				}
-				catch (const ReadAssert&)
-				{
-					st.status = READ_ASSERT;
-				}
-				if (!extendNow[a]) st.done = true;
-			});
-			std::vector<ExtJob> jobs;
-			std::vector<uint32_t> extending;
-			for (size_t a = 0; a < active.size(); a++)
+			std::mutex& m;
+			std::condition_variable& cv;
+			uint32_t& done;
+			bool pending = true;
+			void signal()
 			{
-				if (!extendNow[a]) continue;
-				uint32_t r = active[a];
-				ReadState& st = states[r];
-				const SeedHit& s = st.seeds[st.pending];
-				const uint64_t L = blens[r];
-				int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
-				int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
-				st.bwdJob = st.fwdJob = -1;
-				if (s.seqPos > 0)
+				if (!pending) return;
+				pending = false;
 				{
-					auto rp = g.reversePosition(forwardId, s.nodeOffset);
-					uint32_t node = g.unitigNode(backwardId, rp.second);
-					ExtJob j{nullptr, (uint32_t)s.seqPos, node, (uint32_t)(rp.second - g.nodeOff[node])};
-					j.devOffset = ctx.reverseBase() + boff[r] + (L - s.seqPos);
-					st.bwdJob = jobs.size();
-					jobs.push_back(j);
+					std::lock_guard<std::mutex> lock(m);
+					done++;
 				}
-				if (s.seqPos < L - 1)
-				{
-					uint32_t node = g.unitigNode(forwardId, s.nodeOffset);
-					ExtJob j{nullptr, (uint32_t)(L - s.seqPos - 1), node, (uint32_t)(s.nodeOffset - g.nodeOff[node])};
-					j.devOffset = boff[r] + s.seqPos + 1;
-					st.fwdJob = jobs.size();
-					jobs.push_back(j);
-				}
-				extending.push_back(r);
+				cv.notify_all();
 			}
-			stats.hostMs += msSince(th);
-			if (extending.empty()) break;
-			ExtBatch eb;
-			auto te = std::chrono::steady_clock::now();
-			ctx.extend(g, opt.ext, jobs, false, eb, ctx.readsBase());
-			stats.extendCallMs += msSince(te);
-			stats.rounds++;
-			stats.launches += eb.launches;
-			stats.reruns += eb.reruns;
-			stats.extensions += jobs.size();
-			stats.extendKernelMs += eb.kernelMs;
-			stats.wordSteps += eb.wordSteps;
-			th = std::chrono::steady_clock::now();
-			parallelFor(extending.size(), opt.hostThreads, [&](size_t a) {
-				ReadState& st = states[extending[a]];
-				const uint32_t r = extending[a];
-				const char* seq = bseqs[r];
-				const uint64_t L = blens[r];
-				const SeedHit s = st.seeds[st.pending];
-				st.next = st.pending + 1;
-				try
-				{
-					const uint64_t* raws[2] = {nullptr, nullptr};
-					uint32_t counts[2] = {0, 0};
-					thread_local std::vector<uint64_t> items[2];
-					for (int dir = 0; dir < 2; dir++)
+			~FirstLaunch() { signal(); }
+		};
+		auto runGroup = [&](uint32_t lane) {
+			AlignStats& gs = groupStats[lane];
+			FirstLaunch firstLaunch{phaseMutex, phaseCv, firstLaunchesDone};
+			std::vector<uint32_t> active = groupReads[lane];
+			while (!active.empty())
+			{
+				auto th = std::chrono::steady_clock::now();
+				// each read's next seed (the one its loop picks now), then, from its
+				// second round on, the seeds its loop would pick after it if those
+				// alignments covered nothing: extended in the same launch and replayed
+				parallelFor(active.size(), threads, [&](size_t a) {
+					ReadState& st = states[active[a]];
+					st.specs.clear();
+					try
 					{
-						int64_t j = dir == 0 ? st.bwdJob : st.fwdJob;
-						if (j < 0) continue;
-						const gad::ExtResult& res = eb.results[j];
-						if (res.status == gad::EXT_OVERFLOW)
+						if (!st.pendingReady && !advance(st, opt))
 						{
-							st.status = READ_OVERFLOW;
 							st.done = true;
 							return;
 						}
-						check(res.status != gad::EXT_ASSERT);
-						if (res.status != gad::EXT_OK) continue;
-						check(res.ntrace > 0);
-						eb.items(g, j, items[dir]);
-						raws[dir] = items[dir].data();
-						counts[dir] = res.ntrace;
-					}
-					Alignment item = buildAlignment(g, opt, seq, L, s, raws[0], counts[0], raws[1], counts[1]);
-					if (item.alignmentFailed()) return;
-					item.seedGoodness = s.seedGoodness;
-					st.alignments.push_back(std::move(item));
-					if (opt.sloppyOptimizations)
-					{
-						auto& al = st.alignments;
-						std::sort(al.begin(), al.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
-						if (al[0].alignmentStart == 0)
+						st.pendingReady = false;
+						st.specs.push_back(Spec{st.pending, -1, -1});
+						const uint32_t k = speculationWidth(st.roundsRun, opt.maxSpeculativeSeeds);
+						if (k > 1)
 						{
-							uint64_t minGoodness = al[0].seedGoodness;
-							uint64_t contiguousEnd = al[0].alignmentEnd;
-							for (size_t k = 1; k < al.size(); k++)
+							LoopCursor c = cursorOf(st);
+							c.next = st.pending + 1;
+							while (st.specs.size() < k && pickNext(st, c, opt))
 							{
-								if (al[k].alignmentStart <= contiguousEnd)
-								{
-									minGoodness = std::min(minGoodness, al[k].seedGoodness);
-									contiguousEnd = std::max(contiguousEnd, al[k].alignmentEnd);
-								}
+								st.specs.push_back(Spec{c.pending, -1, -1});
+								c.next = c.pending + 1;
 							}
-							if (contiguousEnd == L) st.seedScoreForEndToEndAln = minGoodness;
+						}
+						st.roundsRun++;
+					}
+					catch (const ReadAssert&)
+					{
+						st.status = READ_ASSERT;
+						st.done = true;
+					}
+				});
+				std::vector<ExtJob> jobs;
+				std::vector<uint32_t> extending;
+				for (uint32_t r : active)
+				{
+					ReadState& st = states[r];
+					if (st.done) continue;
+					const uint64_t L = blens[r];
+					for (Spec& sp : st.specs)
+					{
+						const SeedHit& s = st.seeds[sp.seed];
+						int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
+						int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
+						if (s.seqPos > 0)
+						{
+							auto rp = g.reversePosition(forwardId, s.nodeOffset);
+							uint32_t node = g.unitigNode(backwardId, rp.second);
+							ExtJob j{nullptr, (uint32_t)s.seqPos, node, (uint32_t)(rp.second - g.nodeOff[node])};
+							j.devOffset = ctx.reverseBase() + boff[r] + (L - s.seqPos);
+							sp.bwdJob = jobs.size();
+							jobs.push_back(j);
+						}
+						if (s.seqPos < L - 1)
+						{
+							uint32_t node = g.unitigNode(forwardId, s.nodeOffset);
+							ExtJob j{nullptr, (uint32_t)(L - s.seqPos - 1), node, (uint32_t)(s.nodeOffset - g.nodeOff[node])};
+							j.devOffset = boff[r] + s.seqPos + 1;
+							sp.fwdJob = jobs.size();
+							jobs.push_back(j);
 						}
 					}
+					extending.push_back(r);
 				}
-				catch (const ReadAssert&)
+				{ double ms = msSince(th); gs.hostMs += ms; gs.pickMs += ms; }
+				if (extending.empty()) break;
+				if (firstLaunch.pending && lane > 0)
 				{
-					st.status = READ_ASSERT;
-					st.done = true;
+					std::unique_lock<std::mutex> lock(phaseMutex);
+					phaseCv.wait(lock, [&]() { return firstLaunchesDone >= lane; });
 				}
-			});
-			stats.hostMs += msSince(th);
-			std::vector<uint32_t> still;
-			for (uint32_t r : active)
-				if (!states[r].done) still.push_back(r);
-			active.swap(still);
+				ExtBatch eb;
+				auto te = std::chrono::steady_clock::now();
+				ctx.extend(g, opt.ext, jobs, false, eb, ctx.readsBase(), lane);
+				gs.extendCallMs += msSince(te);
+				firstLaunch.signal();
+				gs.rounds++;
+				gs.launches += eb.launches;
+				gs.reruns += eb.reruns;
+				gs.extensions += jobs.size();
+				gs.extendKernelMs += eb.kernelMs;
+				gs.wordSteps += eb.wordSteps;
+				th = std::chrono::steady_clock::now();
+				std::atomic<uint64_t> unused{0};
+				parallelFor(extending.size(), threads, [&](size_t a) {
+					ReadState& st = states[extending[a]];
+					const uint32_t r = extending[a];
+					const char* seq = bseqs[r];
+					const uint64_t L = blens[r];
+					thread_local std::vector<uint64_t> items[2];
+					size_t si = 0;
+					try
+					{
+						// replay of the loop: the seed it picks is always one of this
+						// round's (the first, then any later one the alignments so far
+						// leave uncovered) or a seed for the next round
+						while (true)
+						{
+							const Spec& sp = st.specs[si];
+							const SeedHit s = st.seeds[sp.seed];
+							st.next = sp.seed + 1;
+							const uint64_t* raws[2] = {nullptr, nullptr};
+							uint32_t counts[2] = {0, 0};
+							for (int dir = 0; dir < 2; dir++)
+							{
+								int64_t j = dir == 0 ? sp.bwdJob : sp.fwdJob;
+								if (j < 0) continue;
+								const gad::ExtResult& res = eb.results[j];
+								if (res.status == gad::EXT_OVERFLOW)
+								{
+									st.status = READ_OVERFLOW;
+									st.done = true;
+									return;
+								}
+								check(res.status != gad::EXT_ASSERT);
+								if (res.status != gad::EXT_OK) continue;
+								check(res.ntrace > 0);
+								eb.items(g, j, items[dir]);
+								raws[dir] = items[dir].data();
+								counts[dir] = res.ntrace;
+							}
+							Alignment item = buildAlignment(g, opt, seq, L, s, raws[0], counts[0], raws[1], counts[1]);
+							if (!item.alignmentFailed())
+							{
+								item.seedGoodness = s.seedGoodness;
+								st.alignments.push_back(std::move(item));
+								if (opt.sloppyOptimizations) updateEndToEnd(st, L);
+							}
+							if (!advance(st, opt))
+							{
+								unused += st.specs.size() - si - 1;
+								st.done = true;
+								break;
+							}
+							size_t k = si + 1;
+							while (k < st.specs.size() && st.specs[k].seed < st.pending) k++;
+							if (k < st.specs.size() && st.specs[k].seed == st.pending)
+							{
+								unused += k - si - 1;
+								si = k;
+								continue;
+							}
+							unused += st.specs.size() - si - 1;
+							st.pendingReady = true;
+							break;
+						}
+					}
+					catch (const ReadAssert&)
+					{
+						st.status = READ_ASSERT;
+						st.done = true;
+					}
+				});
+				gs.unusedExtensions += unused.load();
+				{ double ms = msSince(th); gs.hostMs += ms; gs.replayMs += ms; }
+				std::vector<uint32_t> still;
+				for (uint32_t r : active)
+					if (!states[r].done) still.push_back(r);
+				active.swap(still);
+			}
+		};
+		if (groups == 1) runGroup(0);
+		else
+		{
+			std::vector<std::thread> pool;
+			for (uint32_t lane = 0; lane < groups; lane++)
+				pool.emplace_back([&, lane]() {
+					try
+					{
+						runGroup(lane);
+					}
+					catch (...)
+					{
+						groupErrors[lane] = std::current_exception();
+					}
+				});
+			for (auto& t : pool) t.join();
+			for (auto& e : groupErrors)
+				if (e) std::rethrow_exception(e);
+		}
+		for (const AlignStats& gs : groupStats)
+		{
+			stats.rounds = std::max(stats.rounds, gs.rounds);
+			stats.launches += gs.launches;
+			stats.reruns += gs.reruns;
+			stats.extensions += gs.extensions;
+			stats.extendKernelMs += gs.extendKernelMs;
+			stats.wordSteps += gs.wordSteps;
+			stats.extendCallMs = std::max(stats.extendCallMs, gs.extendCallMs);
+			stats.unusedExtensions += gs.unusedExtensions;
+		}
+		// the groups' host phases overlap: count the longest group's
+		{
+			double hostMax = 0, pickMax = 0, replayMax = 0;
+			for (const AlignStats& gs : groupStats)
+			{
+				hostMax = std::max(hostMax, gs.hostMs);
+				pickMax = std::max(pickMax, gs.pickMs);
+				replayMax = std::max(replayMax, gs.replayMs);
+			}
+			stats.hostMs += hostMax;
+			stats.pickMs += pickMax;
+			stats.replayMs += replayMax;
 		}
 		// selection and GAF (src/Aligner.cpp:502-606)
 		th = std::chrono::steady_clock::now();
@@ -1079,7 +1274,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				o.gaf.clear();
 			}
 		});
-		stats.hostMs += msSince(th);
+		{ double ms = msSince(th); stats.hostMs += ms; stats.gafMs += ms; }
 	}
 	stats.totalMs = msSince(t0);
 }

@@ -47,6 +47,16 @@ struct AlignOptions
 	// of every unfinished read in one launch (both directions as separate jobs);
 	// true: every read's whole seed loop runs on the device in one launch
 	bool deviceSeedLoop = false;
+	// host-driven rounds: a read's round r > 0 also extends up to 2^r - 1 of the
+	// seeds its loop would try next (at most maxSpeculativeSeeds seeds in all);
+	// the loop's decisions are then replayed in order, so results are those of
+	// one seed at a time and extensions of seeds the replay skips are unused.
+	// Fewer rounds means fewer launches waiting on their longest extension.
+	uint32_t maxSpeculativeSeeds = 16;
+	// host-driven rounds: the reads are dealt (longest first) into this many
+	// groups, at most the backend's extension lanes, whose seed loops run in
+	// separate host threads so one group's host work overlaps another's kernels
+	uint32_t pipelineGroups = 2;
 };
 
 enum ReadStatus
@@ -74,6 +84,11 @@ struct AlignStats
 	uint64_t rounds = 0, launches = 0, extensions = 0, wordSteps = 0, seeds = 0, reruns = 0;
 	double lookupCallMs = 0;    // wall time of the seed-lookup calls
 	double extendCallMs = 0;    // wall time of the extension calls (kernels, set-up, copies)
+	uint64_t unusedExtensions = 0; // speculative extensions the replayed loops skipped
+	// parts of hostMs in the host-driven rounds: seeds from matches and their
+	// order; picking each round's seeds and building jobs; replaying the loops
+	// (trace decode, fix-ups, alignments); selection and GAF text
+	double seedingMs = 0, pickMs = 0, replayMs = 0, gafMs = 0;
 };
 
 // MinimizerSeeder::getSeeds (src/MinimizerSeeder.cpp:521-555) from the device

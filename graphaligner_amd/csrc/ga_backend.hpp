@@ -204,9 +204,12 @@ class ExtendBackend
 {
 public:
 	virtual ~ExtendBackend() = default;
+	// Extension calls on different lanes (0 .. extendLanes() - 1) may run at the
+	// same time from different threads; calls on one lane may not.
+	virtual uint32_t extendLanes() const { return 1; }
 	// Runs all jobs; statuses EXT_OVERFLOW are rerun with larger workspaces and
 	// only remain if the largest workspace still does not fit.
-	virtual void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase = nullptr) = 0;
+	virtual void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase = nullptr, uint32_t lane = 0) = 0;
 	// Makes the reads resident (replacing the previous set).
 	virtual void uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens) = 0;
 	virtual const ReadSet& reads() const = 0;

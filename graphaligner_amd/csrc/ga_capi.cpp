@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <cstdlib>
 #include <memory>
 #include <new>
@@ -71,6 +72,7 @@ ga_status_t ga_params_preset(const char* preset, ga_params_t* p)
 	p->minimizerSeedDensity = 0;
 	p->minimizerDiscardMostNumerousFraction = 0.0002;
 	p->cigarMatchMismatchMerge = 0;
+	p->maxSpeculativeSeeds = 16;
 	if (!preset) return GA_OK;
 	std::string s = preset;
 	if (s == "dbg")
@@ -344,23 +346,30 @@ AlignOptions alignOptionsOf(const ga_params_t& p)
 	opt.minSeedClusterSize = p.seedClusterMinSize < 0 ? 0 : (uint64_t)p.seedClusterMinSize;
 	opt.cigarMatchMismatchMerge = p.cigarMatchMismatchMerge != 0;
 	opt.deviceSeedLoop = p.deviceSeedLoop != 0;
+	opt.maxSpeculativeSeeds = p.maxSpeculativeSeeds < 1 ? 1 : (uint32_t)p.maxSpeculativeSeeds;
 	return opt;
 }
 
 ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& st, char** gaf, int32_t* status, ga_align_stats_t* stats)
 {
-	size_t total = 1;
-	for (auto& o : out) total += o.gaf.size();
-	char* buf = (char*)std::malloc(total);
-	if (!buf) return GA_ENOMEM;
-	size_t pos = 0;
+	std::vector<size_t> at(out.size() + 1, 0);
 	for (size_t i = 0; i < out.size(); i++)
 	{
-		std::memcpy(buf + pos, out[i].gaf.data(), out[i].gaf.size());
-		pos += out[i].gaf.size();
+		at[i + 1] = at[i] + out[i].gaf.size();
 		status[i] = out[i].status;
 	}
-	buf[pos] = 0;
+	char* buf = (char*)std::malloc(at.back() + 1);
+	if (!buf) return GA_ENOMEM;
+	// the text runs to hundreds of MB for long reads: copy it (and take the
+	// fresh pages' first-touch faults) on several threads
+	const size_t parts = std::min<size_t>(16, std::max<size_t>(1, at.back() >> 24));
+	std::vector<std::thread> pool;
+	for (size_t t = 0; t < parts; t++)
+		pool.emplace_back([&, t]() {
+			for (size_t i = t; i < out.size(); i += parts) std::memcpy(buf + at[i], out[i].gaf.data(), out[i].gaf.size());
+		});
+	for (auto& th : pool) th.join();
+	buf[at.back()] = 0;
 	*gaf = buf;
 	if (stats)
 	{
@@ -376,6 +385,11 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& 
 		stats->reruns = st.reruns;
 		stats->lookupCallMs = st.lookupCallMs;
 		stats->extendCallMs = st.extendCallMs;
+		stats->unusedExtensions = st.unusedExtensions;
+		stats->seedingMs = st.seedingMs;
+		stats->pickMs = st.pickMs;
+		stats->replayMs = st.replayMs;
+		stats->gafMs = st.gafMs;
 	}
 	return GA_OK;
 }

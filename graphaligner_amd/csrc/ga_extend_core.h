@@ -164,7 +164,7 @@ GA_HD uint64_t maxXCandidates(const Slice& s, int cells, uint64_t extraMask)
 
 // 4-bit base set of a read character (bit0 A, bit1 C, bit2 G, bit3 T), 16 for a
 // character that makes the reference's characterMatch assert (src/GraphAlignerCommon.h:273-276).
-GA_HD uint32_t readBaseSet(unsigned char c)
+GA_HHD uint32_t readBaseSet(unsigned char c)
 {
 	switch (c)
 	{
@@ -186,6 +186,45 @@ GA_HD uint32_t readBaseSet(unsigned char c)
 		case '-': return 0;
 	}
 	return 16;
+}
+
+// Match-mask planes of a byte string, for the per-slice masks of getEqVector
+// (src/GraphAlignerBitvectorCommon.h:279-318) without 64 byte reads per slice:
+// for every 64-position block b, planes[kPlaneWords * b + k] has bit l set when
+// the letter at 64 b + l admits base k (A, C, G, T; readBaseSet) for k < 4, and
+// when readBaseSet rejects it for k = 4. One block past the string's last is
+// kept (zero) for the shifted read of a slice starting in the last block.
+constexpr uint32_t kPlaneWords = 5;
+GA_HHD uint64_t planeBlockCount(uint64_t bytes) { return bytes / 64 + 2; }
+GA_HHD void planeBlock(const char* s, uint64_t n, uint64_t block, uint64_t* out)
+{
+	uint64_t m[kPlaneWords] = {0, 0, 0, 0, 0};
+	for (uint32_t l = 0; l < 64; l++)
+	{
+		const uint64_t x = block * 64 + l;
+		if (x >= n) break;
+		const uint32_t set = readBaseSet((unsigned char)s[x]);
+		const uint64_t bit = (uint64_t)1 << l;
+		if (set == 16) m[4] |= bit;
+		if (set & 1) m[0] |= bit;
+		if (set & 2) m[1] |= bit;
+		if (set & 4) m[2] |= bit;
+		if (set & 8) m[3] |= bit;
+	}
+	for (uint32_t k = 0; k < kPlaneWords; k++) out[k] = m[k];
+}
+// Masks of rows x0 .. x0 + 63 (rows at or past x0 + rows match nothing); true
+// when one of the rows holds a letter readBaseSet rejects.
+GA_HD bool eqFromPlanes(const uint64_t* planes, uint64_t x0, uint32_t rows, Eq4& eq)
+{
+	const uint64_t* a = planes + (x0 >> 6) * kPlaneWords;
+	const uint64_t* b = a + kPlaneWords;
+	const uint32_t sh = (uint32_t)(x0 & 63);
+	const uint64_t keep = rows >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << rows) - 1);
+	uint64_t v[kPlaneWords];
+	for (uint32_t k = 0; k < kPlaneWords; k++) v[k] = (sh == 0 ? a[k] : ((a[k] >> sh) | (b[k] << (64 - sh)))) & keep;
+	for (uint32_t k = 0; k < 4; k++) eq.m[k] = v[k];
+	return v[4] != 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -778,6 +817,10 @@ struct Extension
 	uint32_t len;
 	uint32_t startNode;
 	uint32_t startOffset;
+	// match-mask planes of the buffer seq points into (planeBlock) and seq's
+	// offset there; without planes the masks are read from seq byte by byte
+	const uint64_t* planes;
+	uint64_t seqBase;
 	// workspace
 	ArenaCaps caps;
 	SliceHdr* hdr;
@@ -892,6 +935,12 @@ struct Extension
 		return r;
 	}
 	// characterMatch(read char, graph char) (src/GraphAlignerCommon.h:176-198)
+	// match masks of rows j .. j + 63 (getEqVector); true on a letter readBaseSet rejects
+	GA_HD bool sliceEq(uint32_t j, Eq4& eq) const
+	{
+		if (planes) return eqFromPlanes(planes, seqBase + j, len - j, eq);
+		return lpEq(seq, len, j, eq);
+	}
 	GA_HD bool charMatch(int32_t seqPos, uint32_t node, uint32_t off) const
 	{
 		uint32_t rs = readBaseSet((unsigned char)seq[seqPos]);
@@ -1304,7 +1353,7 @@ struct Extension
 			const SliceHdr P = hdr[kept - 1];
 			const int32_t prevQuit = P.minScore + P.bandwidth;
 			Eq4 eq;
-			if (lpEq(seq, len, (uint32_t)j, eq))
+			if (sliceEq((uint32_t)j, eq))
 			{
 				fail(EXT_ASSERT);
 				return;
@@ -1758,7 +1807,7 @@ struct Extension
 		int32_t score = hdr[bestIndex].maxExactEndposScore;
 		int32_t j = (int32_t)((bestIndex - 1) * 64);
 		Eq4 eq;
-		lpEq(seq, len, (uint32_t)j, eq);
+		sliceEq((uint32_t)j, eq);
 		ColumnStore cols;
 		bindColumns(cols, colVP, colVN, colEnd);
 		recalc(bestIndex, node, eq, cols);
@@ -1808,7 +1857,7 @@ struct Extension
 		pushTrace(startPos, false);
 		uint32_t currentSlice = (uint32_t)(startPos.seq / 64 + 1);
 		uint32_t currentNode = NIL;
-		lpEq(seq, len, (currentSlice - 1) * 64, eq);
+		sliceEq((currentSlice - 1) * 64, eq);
 		int32_t lastScore = SMAX - 1;
 		uint32_t steps = 0;
 		NodeSeq curSeq = loadNodeSeq(startPos.node);
@@ -1823,7 +1872,7 @@ struct Extension
 			uint32_t newNode = back.node;
 			if (newSlice != currentSlice || newNode != currentNode)
 			{
-				if (newSlice != currentSlice) lpEq(seq, len, (newSlice - 1) * 64, eq);
+				if (newSlice != currentSlice) sliceEq((newSlice - 1) * 64, eq);
 				currentSlice = newSlice;
 				currentNode = newNode;
 				if (currentSlice == 0)
@@ -1911,12 +1960,15 @@ struct Extension
 				// characterMatch of read row vert and the node's base at hori, read
 				// from the slice's match masks (rows past the read end match nothing)
 				uint64_t eqm = eqOf(eq, curSeq, hori);
+				// scores of the cell and of its left neighbour, carried along the
+				// walk; the cells one row up differ by the columns' vertical
+				// deltas at row vert (valueAt(s, r - 1) = valueAt(s, r) + VN_r - VP_r)
+				int32_t here = valueAt(ch0, vert);
+				int32_t horizontal = valueAt(ch1, vert);
 				while (hori > 0 && vert > 0 && status == EXT_OK)
 				{
-					int32_t here = valueAt(ch0, vert);
-					int32_t vertical = valueAt(ch0, vert - 1);
-					int32_t horizontal = valueAt(ch1, vert);
-					int32_t diagonal = valueAt(ch1, vert - 1);
+					int32_t vertical = here + (int32_t)((ch0.VN >> vert) & 1) - (int32_t)((ch0.VP >> vert) & 1);
+					int32_t diagonal = horizontal + (int32_t)((ch1.VN >> vert) & 1) - (int32_t)((ch1.VP >> vert) & 1);
 					bool eqc = ((eqm >> vert) & 1) != 0;
 					if (vertical < here - 1 || horizontal < here - 1 || diagonal < here - (eqc ? 0 : 1))
 					{
@@ -1927,15 +1979,22 @@ struct Extension
 					{
 						vert--;
 						pushTrace(mkpos(bk.node, hori, vert + cj), false);
+						here = vertical;
+						horizontal = diagonal;
 						continue;
 					}
 					if (diagonal == here - (eqc ? 0 : 1))
+					{
 						vert--;
+						here = diagonal;
+					}
 					else if (horizontal != here - 1)
 					{
 						fail(EXT_ASSERT);
 						break;
 					}
+					else
+						here = horizontal;
 					hori--;
 					pushTrace(mkpos(bk.node, hori, vert + cj), false);
 					if (hori > 0)
@@ -1943,6 +2002,7 @@ struct Extension
 						ch0 = ch1;
 						ch1 = colGet(cols, hori - 1);
 						eqm = eqOf(eq, curSeq, hori);
+						horizontal = valueAt(ch1, vert);
 					}
 				}
 			}
@@ -2124,7 +2184,8 @@ GA_HHD uint32_t requestItems(uint32_t len, const ArenaCaps& caps) { return reque
 // colStore, when given, holds the traceback's column store (64 VP words, 64 VN
 // words, 64 scores) in place of the arena's copy: the gfx950 kernels pass an
 // LDS array, which the backtrace reads at every step.
-GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out, uint64_t* colStore = nullptr)
+// planes, when given, are the match-mask planes of seqbuf (planeBlock).
+GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out, uint64_t* colStore = nullptr, const uint64_t* planes = nullptr)
 {
 	Extension e;
 	carveArena(e, arena, caps);
@@ -2137,6 +2198,8 @@ GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r
 	e.g = g;
 	e.p = p;
 	e.seq = seqbuf + r.seqOffset;
+	e.planes = planes;
+	e.seqBase = r.seqOffset;
 	e.len = r.len;
 	e.startNode = r.startNode;
 	e.startOffset = r.startOffset;

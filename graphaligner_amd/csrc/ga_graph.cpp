@@ -793,6 +793,8 @@ GraphHost loadGfaGraph(const std::string& path)
 	}
 	g.originalSize.insert(b.osize.begin(), b.osize.end());
 	g.originalName = b.oname;
+	g.splitBigSize.resize(n);
+	for (size_t i = 0; i < n; i++) g.splitBigSize[i] = g.originalSize.at(g.nodeId[i]);
 	return g;
 }
 

@@ -42,6 +42,7 @@ struct GraphHost
 	std::unordered_map<int, uint64_t> originalSize;
 	std::unordered_map<int, std::string> originalName;
 	std::vector<int> lookupOrder;                         // bigraph ids in the lookup map's iteration order
+	std::vector<uint64_t> splitBigSize;                   // per split node: length of its bigraph node
 
 	size_t numNodes() const { return nodeLen.size(); }
 	char nodeChar(uint32_t node, uint32_t pos) const;

@@ -14,7 +14,7 @@
 namespace gad {
 
 template <int MinWavesPerSimd>
-__global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, uint32_t lanesPerWave, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets)
+__global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, uint32_t lanesPerWave, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets, const uint64_t* __restrict__ planes)
 {
 	if (threadIdx.x >= lanesPerWave) return;
 	uint32_t idx = blockIdx.x * lanesPerWave + threadIdx.x;
@@ -24,7 +24,7 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGra
 	c.maxSlices = requestSlices(r.len);
 	c.maxItems = requestItems(r.len, caps);
 	ExtResult out;
-	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out);
+	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, nullptr, planes);
 	results[idx] = out;
 }
 
@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGra
 // runs, so every value is wavefront-uniform and the extension's control flow and
 // most of its 64-bit word arithmetic run on the scalar unit, with no exec-mask
 // handling for its data-dependent branches.
-__global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets)
+__global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets, const uint64_t* __restrict__ planes)
 {
 	__shared__ uint64_t colStore[64 * 2 + 32];
 	uint32_t idx = blockIdx.x;
@@ -42,26 +42,26 @@ __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParam
 	c.maxSlices = requestSlices(r.len);
 	c.maxItems = requestItems(r.len, caps);
 	ExtResult out;
-	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, colStore);
+	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, colStore, planes);
 	results[idx] = out;
 }
 
-hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets)
+hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets, const uint64_t* planes)
 {
 	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
 	if (lanesPerWave == 1 && wavesPerSimd <= 1)
 	{
-		hipLaunchKernelGGL(extendSingleLaneKernel, dim3(nreq), dim3(1), 0, stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+		hipLaunchKernelGGL(extendSingleLaneKernel, dim3(nreq), dim3(1), 0, stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 		return hipGetLastError();
 	}
 	uint32_t blocks = (nreq + lanesPerWave - 1) / lanesPerWave;
 	// register budget variants: more resident waves per SIMD for fewer VGPRs
 	if (wavesPerSimd >= 8)
-		hipLaunchKernelGGL(extendThreadKernel<8>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+		hipLaunchKernelGGL(extendThreadKernel<8>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 	else if (wavesPerSimd >= 4)
-		hipLaunchKernelGGL(extendThreadKernel<4>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+		hipLaunchKernelGGL(extendThreadKernel<4>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 	else
-		hipLaunchKernelGGL(extendThreadKernel<1>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets);
+		hipLaunchKernelGGL(extendThreadKernel<1>, dim3(blocks), dim3(64), 0, stream, g, p, reqs, nreq, lanesPerWave, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 	return hipGetLastError();
 }
 
@@ -69,7 +69,7 @@ hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave
 // each other with no host round trip, and reads are independent, so the
 // hardware dispatcher balances the uneven per-read work across the chip.
 template <int MinWavesPerSimd>
-__global__ void __launch_bounds__(64, MinWavesPerSimd) readLoopKernel(DevGraph g, ReadLoopParams p, const ReadJob* __restrict__ jobs, uint32_t njobs, uint32_t lanesPerWave, const char* __restrict__ reads, const DevSeed* __restrict__ seeds, uint64_t* __restrict__ rawTraces, uint64_t* __restrict__ pools, DevAlignment* __restrict__ alns, ReadResult* __restrict__ results, unsigned char* __restrict__ arenas, ArenaCaps caps)
+__global__ void __launch_bounds__(64, MinWavesPerSimd) readLoopKernel(DevGraph g, ReadLoopParams p, const ReadJob* __restrict__ jobs, uint32_t njobs, uint32_t lanesPerWave, const char* __restrict__ reads, const DevSeed* __restrict__ seeds, uint64_t* __restrict__ rawTraces, uint64_t* __restrict__ pools, DevAlignment* __restrict__ alns, ReadResult* __restrict__ results, unsigned char* __restrict__ arenas, ArenaCaps caps, const uint64_t* __restrict__ planes)
 {
 	if (threadIdx.x >= lanesPerWave) return;
 	uint32_t idx = blockIdx.x * lanesPerWave + threadIdx.x;
@@ -79,11 +79,11 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) readLoopKernel(DevGraph g
 	c.maxSlices = requestSlices(j.len);
 	c.maxItems = requestItems(j.len, caps);
 	ReadResult out;
-	runReadLoop(g, p, j, reads, seeds + j.seedStart, rawTraces + j.traceOffset, pools + j.poolOffset, alns + j.alnOffset, arenas + j.arenaOffset, c, out);
+	runReadLoop(g, p, j, reads, seeds + j.seedStart, rawTraces + j.traceOffset, pools + j.poolOffset, alns + j.alnOffset, arenas + j.arenaOffset, c, out, nullptr, planes);
 	results[idx] = out;
 }
 
-__global__ void __launch_bounds__(1) readLoopSingleLaneKernel(DevGraph g, ReadLoopParams p, const ReadJob* __restrict__ jobs, uint32_t njobs, const char* __restrict__ reads, const DevSeed* __restrict__ seeds, uint64_t* __restrict__ rawTraces, uint64_t* __restrict__ pools, DevAlignment* __restrict__ alns, ReadResult* __restrict__ results, unsigned char* __restrict__ arenas, ArenaCaps caps)
+__global__ void __launch_bounds__(1) readLoopSingleLaneKernel(DevGraph g, ReadLoopParams p, const ReadJob* __restrict__ jobs, uint32_t njobs, const char* __restrict__ reads, const DevSeed* __restrict__ seeds, uint64_t* __restrict__ rawTraces, uint64_t* __restrict__ pools, DevAlignment* __restrict__ alns, ReadResult* __restrict__ results, unsigned char* __restrict__ arenas, ArenaCaps caps, const uint64_t* __restrict__ planes)
 {
 	__shared__ uint64_t colStore[64 * 2 + 32];
 	uint32_t idx = blockIdx.x;
@@ -93,23 +93,42 @@ __global__ void __launch_bounds__(1) readLoopSingleLaneKernel(DevGraph g, ReadLo
 	c.maxSlices = requestSlices(j.len);
 	c.maxItems = requestItems(j.len, caps);
 	ReadResult out;
-	runReadLoop(g, p, j, reads, seeds + j.seedStart, rawTraces + j.traceOffset, pools + j.poolOffset, alns + j.alnOffset, arenas + j.arenaOffset, c, out, colStore);
+	runReadLoop(g, p, j, reads, seeds + j.seedStart, rawTraces + j.traceOffset, pools + j.poolOffset, alns + j.alnOffset, arenas + j.arenaOffset, c, out, colStore, planes);
 	results[idx] = out;
 }
 
-hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, const ArenaCaps& caps)
+hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, const ArenaCaps& caps, const uint64_t* planes)
 {
 	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
 	if (lanesPerWave == 1 && wavesPerSimd <= 1)
 	{
-		hipLaunchKernelGGL(readLoopSingleLaneKernel, dim3(njobs), dim3(1), 0, stream, g, p, jobs, njobs, reads, seeds, rawTraces, pools, alns, results, arenas, caps);
+		hipLaunchKernelGGL(readLoopSingleLaneKernel, dim3(njobs), dim3(1), 0, stream, g, p, jobs, njobs, reads, seeds, rawTraces, pools, alns, results, arenas, caps, planes);
 		return hipGetLastError();
 	}
 	uint32_t blocks = (njobs + lanesPerWave - 1) / lanesPerWave;
 	if (wavesPerSimd >= 4)
-		hipLaunchKernelGGL(readLoopKernel<4>, dim3(blocks), dim3(64), 0, stream, g, p, jobs, njobs, lanesPerWave, reads, seeds, rawTraces, pools, alns, results, arenas, caps);
+		hipLaunchKernelGGL(readLoopKernel<4>, dim3(blocks), dim3(64), 0, stream, g, p, jobs, njobs, lanesPerWave, reads, seeds, rawTraces, pools, alns, results, arenas, caps, planes);
 	else
-		hipLaunchKernelGGL(readLoopKernel<1>, dim3(blocks), dim3(64), 0, stream, g, p, jobs, njobs, lanesPerWave, reads, seeds, rawTraces, pools, alns, results, arenas, caps);
+		hipLaunchKernelGGL(readLoopKernel<1>, dim3(blocks), dim3(64), 0, stream, g, p, jobs, njobs, lanesPerWave, reads, seeds, rawTraces, pools, alns, results, arenas, caps, planes);
+	return hipGetLastError();
+}
+
+// One thread per 64-position block of the byte string.
+__global__ void eqPlanesKernel(const char* __restrict__ bytes, uint64_t n, uint64_t blocks, uint64_t* __restrict__ planes)
+{
+	const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= blocks) return;
+	planeBlock(bytes, n, b, planes + b * kPlaneWords);
+}
+
+uint64_t eqPlaneWords(uint64_t n) { return planeBlockCount(n) * kPlaneWords; }
+
+hipError_t launchEqPlanes(hipStream_t stream, const char* bytes, uint64_t n, uint64_t* planes)
+{
+	const uint64_t blocks = planeBlockCount(n);
+	const uint64_t grid = (blocks + 255) / 256;
+	if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(eqPlanesKernel, dim3((uint32_t)grid), dim3(256), 0, stream, bytes, n, blocks, planes);
 	return hipGetLastError();
 }
 

@@ -8,11 +8,16 @@ namespace gad {
 // One thread per request, lanesPerWave (1..64) requests per wave, compiled for
 // at least wavesPerSimd (1, 4 or 8) resident waves per SIMD; arenaOffsets[i]
 // locates request i's workspace.
-hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets);
+hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets, const uint64_t* planes);
 
 // One read's seed loop per wave (lane 0 unless lanesPerWave > 1): jobs[i] locates
 // read i's bytes, seeds and workspaces; results[i] receives its outcome.
-hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, const ArenaCaps& caps);
+hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, const ArenaCaps& caps, const uint64_t* planes);
+
+// Match-mask planes (planeBlock) of bytes[0, n): planeBlockCount(n) blocks of
+// kPlaneWords words. The extension kernels take planes of their sequence buffer.
+hipError_t launchEqPlanes(hipStream_t stream, const char* bytes, uint64_t n, uint64_t* planes);
+uint64_t eqPlaneWords(uint64_t n);
 
 // Workspace bytes of one request of length len (capacities for slices and band
 // items scale with len; the others come from caps).

@@ -69,11 +69,27 @@ Context::Context(int device) : device_(device)
 	GA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 	GA_HIP(hipEventCreate(&ev0_));
 	GA_HIP(hipEventCreate(&ev1_));
+	for (size_t k = 0; k < kExtendLanes; k++)
+	{
+		auto L = std::make_unique<ExtendLane>();
+		GA_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+		GA_HIP(hipEventCreate(&L->ev0));
+		GA_HIP(hipEventCreate(&L->ev1));
+		lanes_.push_back(std::move(L));
+	}
+}
+
+ExtendLane::~ExtendLane()
+{
+	if (ev0) (void)hipEventDestroy(ev0);
+	if (ev1) (void)hipEventDestroy(ev1);
+	if (stream) (void)hipStreamDestroy(stream);
 }
 
 Context::~Context()
 {
 	(void)hipSetDevice(device_);
+	lanes_.clear();
 	graphs_.clear();
 	indexes_.clear();
 	if (ev0_) (void)hipEventDestroy(ev0_);
@@ -91,6 +107,7 @@ static void upload(DeviceBuffer& buf, const std::vector<T>& v, hipStream_t s)
 const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 {
 	if (g.uid == 0) throw std::runtime_error("graph has no identity (not built by loadGfaGraph)");
+	std::lock_guard<std::mutex> lock(cacheMutex_);
 	for (auto& c : graphs_)
 		if (c->uid == g.uid) return c->view;
 	GA_HIP(hipSetDevice(device_));
@@ -110,7 +127,7 @@ const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 	upload(dg->nodeId, g.nodeId, stream_);
 	upload(dg->nodeOff, g.nodeOff, stream_);
 	std::vector<uint32_t> bigSize(g.numNodes());
-	for (size_t i = 0; i < g.numNodes(); i++) bigSize[i] = (uint32_t)g.originalSize.at(g.nodeId[i]);
+	for (size_t i = 0; i < g.numNodes(); i++) bigSize[i] = (uint32_t)g.splitBigSize[i];
 	upload(dg->bigSize, bigSize, stream_);
 	GA_HIP(hipStreamSynchronize(stream_));
 	auto& v = dg->view;
@@ -135,6 +152,7 @@ const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 const DeviceIndex& Context::deviceIndex(const MinimizerIndex& idx)
 {
 	if (idx.uid == 0) throw std::runtime_error("index has no identity (not built by buildMinimizerIndex)");
+	std::lock_guard<std::mutex> lock(cacheMutex_);
 	for (auto& c : indexes_)
 		if (c->uid == idx.uid) return *c;
 	GA_HIP(hipSetDevice(device_));
@@ -148,6 +166,26 @@ const DeviceIndex& Context::deviceIndex(const MinimizerIndex& idx)
 	di->mask = idx.mask;
 	indexes_.push_back(std::move(di));
 	return *indexes_.back();
+}
+
+void Context::computePlanes()
+{
+	const uint64_t n = 2 * (reads_.bytes + 1);
+	readPlanes_.reserve(gad::eqPlaneWords(n) * sizeof(uint64_t));
+	GA_HIP(gad::launchEqPlanes(stream_, readBytes_.as<char>(), n, readPlanes_.as<uint64_t>()));
+	planesReady_ = true;
+}
+
+const uint64_t* Context::residentPlanes(const char* devSeqBase)
+{
+	if (devSeqBase != readBytes_.as<char>()) throw std::runtime_error("device sequences are not the context's resident reads");
+	std::lock_guard<std::mutex> lock(cacheMutex_);
+	if (!planesReady_)
+	{
+		computePlanes();
+		GA_HIP(hipStreamSynchronize(stream_));
+	}
+	return readPlanes_.as<uint64_t>();
 }
 
 void Context::uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens)
@@ -169,6 +207,7 @@ void Context::uploadReads(const std::vector<const char*>& seqs, const std::vecto
 	for (size_t i = 0; i < n; i++) std::memcpy(rs.host.data() + rs.offset[i], seqs[i], lens[i]);
 	revBase_ = rs.bytes + 1;
 	readBytes_.reserve(2 * (rs.bytes + 1));
+	planesReady_ = false;
 	readOff_.reserve(std::max<size_t>(1, n) * sizeof(uint64_t));
 	readLen_.reserve(std::max<size_t>(1, n) * sizeof(uint32_t));
 	GA_HIP(hipMemcpyAsync(readBytes_.ptr, rs.host.data(), rs.bytes + 1, hipMemcpyHostToDevice, stream_));
@@ -200,6 +239,11 @@ void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t co
 	GA_HIP(hipEventRecord(ev0_, stream_));
 	GA_HIP(gad::launchRevcomp(stream_, readBytes_.as<char>(), readBytes_.as<char>() + revBase_, off, len, count, 0));
 	GA_HIP(gad::launchSeedLookup(stream_, readBytes_.as<char>(), off, len, count, di.table.as<gad::SeedTableSlot>(), di.mask, idx.k, idx.w, idx.maxCount, matches_.as<gad::SeedMatch>(), matchCount_.as<uint32_t>()));
+	{
+		// the reverse complements just written change the buffer's planes
+		std::lock_guard<std::mutex> lock(cacheMutex_);
+		computePlanes();
+	}
 	GA_HIP(hipEventRecord(ev1_, stream_));
 	GA_HIP(hipMemcpyAsync(out.matchCount.data(), matchCount_.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
 	GA_HIP(hipStreamSynchronize(stream_));
@@ -224,7 +268,7 @@ void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t co
 	GA_HIP(hipStreamSynchronize(stream_));
 }
 
-void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase)
+void Context::runOnce(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase, const uint64_t* devPlanes)
 {
 	// split into launches whose workspaces fit the memory budget
 	size_t begin = 0;
@@ -235,17 +279,17 @@ void Context::runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const st
 		while (end < which.size())
 		{
 			uint64_t b = gad::requestArenaBytes(jobs[which[end]].len, caps);
-			if (end > begin && bytes + b > arenaBudgetBytes) break;
+			if (end > begin && bytes + b > arenaBudgetBytes / lanes_.size()) break;
 			bytes += b;
 			end++;
 		}
 		std::vector<uint32_t> part(which.begin() + begin, which.begin() + end);
-		runLaunch(dg, p, jobs, part, caps, wantSummaries, out, devSeqBase);
+		runLaunch(L, dg, p, jobs, part, caps, wantSummaries, out, devSeqBase, devPlanes);
 		begin = end;
 	}
 }
 
-void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase)
+void Context::runLaunch(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase, const uint64_t* devPlanes)
 {
 	const uint32_t n = which.size();
 	if (n == 0) return;
@@ -274,35 +318,43 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 		seqHost.resize(seqBytes + 1);
 		for (uint32_t k = 0; k < n; k++) std::memcpy(seqHost.data() + reqs[k].seqOffset, jobs[which[k]].seq, reqs[k].len);
 	}
-	arena_.reserve(arenaTotal);
-	arenaOffsets_.reserve(n * sizeof(uint64_t));
-	reqs_.reserve(n * sizeof(gad::ExtRequest));
-	if (!devSeqBase) seqs_.reserve(seqHost.size());
-	traces_.reserve(traceItems * sizeof(uint64_t));
-	results_.reserve(n * sizeof(gad::ExtResult));
+	L.arena.reserve(arenaTotal);
+	L.arenaOffsets.reserve(n * sizeof(uint64_t));
+	L.reqs.reserve(n * sizeof(gad::ExtRequest));
+	if (!devSeqBase) L.seqs.reserve(seqHost.size());
+	L.traces.reserve(traceItems * sizeof(uint64_t));
+	L.results.reserve(n * sizeof(gad::ExtResult));
 	if (wantSummaries)
 	{
-		sums_.reserve(sumItems * sizeof(gad::SliceSummary));
-		sumOffsets_.reserve(n * sizeof(uint64_t));
-		GA_HIP(hipMemcpyAsync(sumOffsets_.ptr, sumOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		L.sums.reserve(sumItems * sizeof(gad::SliceSummary));
+		L.sumOffsets.reserve(n * sizeof(uint64_t));
+		GA_HIP(hipMemcpyAsync(L.sumOffsets.ptr, sumOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, L.stream));
 	}
-	GA_HIP(hipMemcpyAsync(arenaOffsets_.ptr, arenaOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
-	GA_HIP(hipMemcpyAsync(reqs_.ptr, reqs.data(), n * sizeof(gad::ExtRequest), hipMemcpyHostToDevice, stream_));
-	if (!devSeqBase) GA_HIP(hipMemcpyAsync(seqs_.ptr, seqHost.data(), seqHost.size(), hipMemcpyHostToDevice, stream_));
-	GA_HIP(hipEventRecord(ev0_, stream_));
-	GA_HIP(gad::launchExtend(stream_, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, reqs_.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : seqs_.as<char>(), traces_.as<uint64_t>(), results_.as<gad::ExtResult>(), arena_.as<unsigned char>(), arenaOffsets_.as<uint64_t>(), caps, wantSummaries ? sums_.as<gad::SliceSummary>() : nullptr, wantSummaries ? sumOffsets_.as<uint64_t>() : nullptr));
-	GA_HIP(hipEventRecord(ev1_, stream_));
+	GA_HIP(hipMemcpyAsync(L.arenaOffsets.ptr, arenaOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, L.stream));
+	GA_HIP(hipMemcpyAsync(L.reqs.ptr, reqs.data(), n * sizeof(gad::ExtRequest), hipMemcpyHostToDevice, L.stream));
+	const uint64_t* planes = devPlanes;
+	if (!devSeqBase)
+	{
+		GA_HIP(hipMemcpyAsync(L.seqs.ptr, seqHost.data(), seqHost.size(), hipMemcpyHostToDevice, L.stream));
+		L.planes.reserve(gad::eqPlaneWords(seqHost.size()) * sizeof(uint64_t));
+		GA_HIP(gad::launchEqPlanes(L.stream, L.seqs.as<char>(), seqHost.size(), L.planes.as<uint64_t>()));
+		planes = L.planes.as<uint64_t>();
+	}
+	if (!planes) throw std::runtime_error("extension launch without match-mask planes");
+	GA_HIP(hipEventRecord(L.ev0, L.stream));
+	GA_HIP(gad::launchExtend(L.stream, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, L.reqs.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : L.seqs.as<char>(), L.traces.as<uint64_t>(), L.results.as<gad::ExtResult>(), L.arena.as<unsigned char>(), L.arenaOffsets.as<uint64_t>(), caps, wantSummaries ? L.sums.as<gad::SliceSummary>() : nullptr, wantSummaries ? L.sumOffsets.as<uint64_t>() : nullptr, planes));
+	GA_HIP(hipEventRecord(L.ev1, L.stream));
 	std::vector<gad::ExtResult> res(n);
-	GA_HIP(hipMemcpyAsync(res.data(), results_.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, stream_));
+	GA_HIP(hipMemcpyAsync(res.data(), L.results.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, L.stream));
 	std::vector<gad::SliceSummary> sm;
 	if (wantSummaries)
 	{
 		sm.resize(sumItems);
-		GA_HIP(hipMemcpyAsync(sm.data(), sums_.ptr, sumItems * sizeof(gad::SliceSummary), hipMemcpyDeviceToHost, stream_));
+		GA_HIP(hipMemcpyAsync(sm.data(), L.sums.ptr, sumItems * sizeof(gad::SliceSummary), hipMemcpyDeviceToHost, L.stream));
 	}
-	GA_HIP(hipStreamSynchronize(stream_));
+	GA_HIP(hipStreamSynchronize(L.stream));
 	float ms = 0;
-	GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+	GA_HIP(hipEventElapsedTime(&ms, L.ev0, L.ev1));
 	out.kernelMs += ms;
 	out.launches++;
 	// traces: encode the used prefix of every kept job's trace region into one
@@ -310,7 +362,7 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 	// and the escaped items
 	std::vector<uint64_t> srcOff, dstOff;
 	std::vector<uint32_t> cnt;
-	const uint64_t base = hostCodesUsed_;
+	const uint64_t base = L.hostCodesUsed;
 	uint64_t total = 0;
 	for (uint32_t k = 0; k < n; k++)
 	{
@@ -335,41 +387,44 @@ void Context::runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const 
 			out.summaries.insert(out.summaries.end(), sm.begin() + sumOff[k], sm.begin() + sumOff[k] + r.nslices);
 		}
 	}
-	if (base + total > hostCodes_.bytes) throw HipError("trace codes exceed the host staging buffer");
-	hostCodesUsed_ = base + total;
+	if (base + total > L.hostCodes.bytes) throw HipError("trace codes exceed the host staging buffer");
+	L.hostCodesUsed = base + total;
 	if (total > 0)
 	{
 		const uint32_t m = cnt.size();
-		gatherSrc_.reserve(m * sizeof(uint64_t));
-		gatherDst_.reserve(m * sizeof(uint64_t));
-		gatherCnt_.reserve(m * sizeof(uint32_t));
-		traceCodes_.reserve(total);
-		traceEscapes_.reserve(total * 2 * sizeof(uint64_t));
-		escapeCount_.reserve(sizeof(uint32_t));
-		GA_HIP(hipMemcpyAsync(gatherSrc_.ptr, srcOff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
-		GA_HIP(hipMemcpyAsync(gatherDst_.ptr, dstOff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
-		GA_HIP(hipMemcpyAsync(gatherCnt_.ptr, cnt.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
-		GA_HIP(hipMemsetAsync(escapeCount_.ptr, 0, sizeof(uint32_t), stream_));
-		GA_HIP(gad::launchTraceCodes(stream_, dg, traces_.as<uint64_t>(), gatherSrc_.as<uint64_t>(), gatherCnt_.as<uint32_t>(), gatherDst_.as<uint64_t>(), m, traceCodes_.as<uint8_t>(), traceEscapes_.as<uint64_t>(), escapeCount_.as<uint32_t>(), total));
+		L.gatherSrc.reserve(m * sizeof(uint64_t));
+		L.gatherDst.reserve(m * sizeof(uint64_t));
+		L.gatherCnt.reserve(m * sizeof(uint32_t));
+		L.traceCodes.reserve(total);
+		L.traceEscapes.reserve(total * 2 * sizeof(uint64_t));
+		L.escapeCount.reserve(sizeof(uint32_t));
+		GA_HIP(hipMemcpyAsync(L.gatherSrc.ptr, srcOff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, L.stream));
+		GA_HIP(hipMemcpyAsync(L.gatherDst.ptr, dstOff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, L.stream));
+		GA_HIP(hipMemcpyAsync(L.gatherCnt.ptr, cnt.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, L.stream));
+		GA_HIP(hipMemsetAsync(L.escapeCount.ptr, 0, sizeof(uint32_t), L.stream));
+		GA_HIP(gad::launchTraceCodes(L.stream, dg, L.traces.as<uint64_t>(), L.gatherSrc.as<uint64_t>(), L.gatherCnt.as<uint32_t>(), L.gatherDst.as<uint64_t>(), m, L.traceCodes.as<uint8_t>(), L.traceEscapes.as<uint64_t>(), L.escapeCount.as<uint32_t>(), total));
 		uint32_t nesc = 0;
-		GA_HIP(hipMemcpyAsync(&nesc, escapeCount_.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
-		GA_HIP(hipMemcpyAsync(hostCodes_.as<uint8_t>() + base, traceCodes_.ptr, total, hipMemcpyDeviceToHost, stream_));
-		GA_HIP(hipStreamSynchronize(stream_));
+		GA_HIP(hipMemcpyAsync(&nesc, L.escapeCount.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, L.stream));
+		GA_HIP(hipMemcpyAsync(L.hostCodes.as<uint8_t>() + base, L.traceCodes.ptr, total, hipMemcpyDeviceToHost, L.stream));
+		GA_HIP(hipStreamSynchronize(L.stream));
 		if (nesc > total) throw HipError("trace encoder returned an inconsistent escape count");
 		std::vector<uint64_t> esc(2 * (size_t)nesc);
 		if (nesc > 0)
 		{
-			GA_HIP(hipMemcpyAsync(esc.data(), traceEscapes_.ptr, esc.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
-			GA_HIP(hipStreamSynchronize(stream_));
+			GA_HIP(hipMemcpyAsync(esc.data(), L.traceEscapes.ptr, esc.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, L.stream));
+			GA_HIP(hipStreamSynchronize(L.stream));
 		}
 		for (uint32_t k = 0; k < nesc; k++) out.escapes.emplace_back(base + esc[2 * k], esc[2 * k + 1]);
 	}
 }
 
-void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase)
+void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase, uint32_t lane)
 {
+	if (lane >= lanes_.size()) throw std::runtime_error("extension lane out of range");
+	ExtendLane& L = *lanes_[lane];
 	GA_HIP(hipSetDevice(device_));
 	const gad::DevGraph& dg = deviceGraph(g);
+	const uint64_t* devPlanes = devSeqBase ? residentPlanes(devSeqBase) : nullptr;
 	out.results.assign(jobs.size(), gad::ExtResult{});
 	out.traceOffset.assign(jobs.size(), 0);
 	out.summaryOffset.assign(jobs.size(), 0);
@@ -379,9 +434,9 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 	// the trace capacities bounds this call's code bytes
 	uint64_t codeBound = 0;
 	for (const ExtJob& j : jobs) codeBound += traceCapFor(j.len);
-	hostCodes_.reserve(codeBound);
-	hostCodesUsed_ = 0;
-	out.codes = hostCodes_.as<uint8_t>();
+	L.hostCodes.reserve(codeBound);
+	L.hostCodesUsed = 0;
+	out.codes = L.hostCodes.as<uint8_t>();
 	std::vector<uint32_t> which(jobs.size());
 	for (uint32_t i = 0; i < jobs.size(); i++) which[i] = i;
 	// longest first: the 64 threads of a wave then carry similar lengths
@@ -389,7 +444,7 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 	gad::ArenaCaps caps = initialArenaCaps();
 	for (int attempt = 0; attempt < kArenaAttempts && !which.empty(); attempt++)
 	{
-		runOnce(dg, p, jobs, which, caps, wantSummaries, out, devSeqBase);
+		runOnce(L, dg, p, jobs, which, caps, wantSummaries, out, devSeqBase, devPlanes);
 		std::vector<uint32_t> again;
 		for (uint32_t id : which)
 			if (out.results[id].status == gad::EXT_OVERFLOW) again.push_back(id);
@@ -495,7 +550,7 @@ void Context::runReadLoopLaunch(const gad::DevGraph& dg, const gad::ReadLoopPara
 	rlResults_.reserve(n * sizeof(gad::ReadResult));
 	GA_HIP(hipMemcpyAsync(rlJobs_.ptr, rj.data(), n * sizeof(gad::ReadJob), hipMemcpyHostToDevice, stream_));
 	GA_HIP(hipEventRecord(ev0_, stream_));
-	GA_HIP(gad::launchReadLoops(stream_, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, rlJobs_.as<gad::ReadJob>(), readBytes_.as<char>(), rlSeeds_.as<gad::DevSeed>(), rlTraces_.as<uint64_t>(), rlPool_.as<uint64_t>(), rlAlns_.as<gad::DevAlignment>(), rlResults_.as<gad::ReadResult>(), arena_.as<unsigned char>(), caps.arena));
+	GA_HIP(gad::launchReadLoops(stream_, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, rlJobs_.as<gad::ReadJob>(), readBytes_.as<char>(), rlSeeds_.as<gad::DevSeed>(), rlTraces_.as<uint64_t>(), rlPool_.as<uint64_t>(), rlAlns_.as<gad::DevAlignment>(), rlResults_.as<gad::ReadResult>(), arena_.as<unsigned char>(), caps.arena, residentPlanes(readBytes_.as<char>())));
 	GA_HIP(hipEventRecord(ev1_, stream_));
 	std::vector<gad::ReadResult> res(n);
 	GA_HIP(hipMemcpyAsync(res.data(), rlResults_.ptr, n * sizeof(gad::ReadResult), hipMemcpyDeviceToHost, stream_));

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -53,6 +54,23 @@ struct PinnedBuffer
 	T* as() const { return (T*)ptr; }
 };
 
+// Stream, events and workspaces of one extension pipeline: extend calls on
+// different lanes may run at the same time from different host threads, and
+// their launches overlap on the GPU.
+struct ExtendLane
+{
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	DeviceBuffer arena, arenaOffsets, reqs, seqs, traces, results, sums, sumOffsets;
+	DeviceBuffer gatherSrc, gatherDst, gatherCnt, traceCodes, traceEscapes, escapeCount, planes;
+	PinnedBuffer hostCodes; // compact traces of the lane's current extend call
+	uint64_t hostCodesUsed = 0;
+	ExtendLane() = default;
+	ExtendLane(const ExtendLane&) = delete;
+	ExtendLane& operator=(const ExtendLane&) = delete;
+	~ExtendLane();
+};
+
 struct DeviceGraph
 {
 	uint64_t uid = 0;
@@ -76,7 +94,8 @@ public:
 	int computeUnits() const { return cus_; }
 	hipStream_t stream() const { return stream_; }
 	const gad::DevGraph& deviceGraph(const GraphHost& g);
-	void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase = nullptr) override;
+	void extend(const GraphHost& g, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, bool wantSummaries, ExtBatch& out, const char* devSeqBase = nullptr, uint32_t lane = 0) override;
+	uint32_t extendLanes() const override { return (uint32_t)lanes_.size(); }
 	const DeviceIndex& deviceIndex(const MinimizerIndex& idx);
 	void uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens) override;
 	const ReadSet& reads() const override { return reads_; }
@@ -84,7 +103,9 @@ public:
 	const char* readsBase() const override { return readBytes_.as<char>(); }
 	void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) override;
 	uint64_t reverseBase() const override { return revBase_; }
-	// device workspace per launch (MI355X has 288 GB of HBM)
+	// device workspace of the launches in flight (MI355X has 288 GB of HBM): a
+	// device seed-loop launch may use all of it, an extension launch its lane's
+	// share
 	uint64_t arenaBudgetBytes = 160ull << 30;
 	// Extension requests per wave: the GA_LANES_PER_WAVE environment variable
 	// (1..64) if set, else defaultLanesPerWave. One request per wave measured
@@ -97,9 +118,13 @@ public:
 	uint32_t kernelWavesPerSimd = 1;
 
 private:
-	void runOnce(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase);
+	void runOnce(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase, const uint64_t* devPlanes);
 	void runReadLoopLaunch(const gad::DevGraph& dg, const gad::ReadLoopParams& p, const std::vector<ReadLoopJob>& jobs, const std::vector<uint32_t>& which, const ReadLoopCaps& caps, ReadLoopOutput& out);
-	void runLaunch(const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase);
+	void runLaunch(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase, const uint64_t* devPlanes);
+	// match-mask planes of the resident reads (forward and reverse bytes);
+	// computePlanes runs with cacheMutex_ held
+	const uint64_t* residentPlanes(const char* devSeqBase);
+	void computePlanes();
 	int device_;
 	int cus_ = 0;
 	hipStream_t stream_ = nullptr;
@@ -108,12 +133,13 @@ private:
 	static constexpr size_t kCachedObjects = 2;
 	std::vector<std::unique_ptr<DeviceGraph>> graphs_;
 	std::vector<std::unique_ptr<DeviceIndex>> indexes_;
-	DeviceBuffer arena_, arenaOffsets_, reqs_, seqs_, traces_, results_, sums_, sumOffsets_;
-	DeviceBuffer gatherSrc_, gatherDst_, gatherCnt_, traceOut_, traceCodes_, traceEscapes_, escapeCount_;
-	PinnedBuffer hostCodes_;  // compact traces of the current extend call
-	uint64_t hostCodesUsed_ = 0;
+	static constexpr size_t kExtendLanes = 2;
+	std::vector<std::unique_ptr<ExtendLane>> lanes_;
+	std::mutex cacheMutex_; // graph and index caches, shared by the lanes
+	DeviceBuffer arena_, gatherSrc_, gatherDst_, gatherCnt_, traceOut_; // device seed loop
 	DeviceBuffer rlSeeds_, rlJobs_, rlTraces_, rlPool_, rlAlns_, rlResults_;
-	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_;
+	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_, readPlanes_;
+	bool planesReady_ = false;
 	ReadSet reads_;
 	uint64_t revBase_ = 0;
 	uint32_t lanesOverride_ = 0;

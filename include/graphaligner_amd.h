@@ -57,6 +57,11 @@ typedef struct
 	 * unfinished read in one GPU launch; 1 runs each read's whole seed loop on
 	 * the GPU, all reads in one launch. */
 	int32_t deviceSeedLoop;
+	/* Execution only, rounds mode: from a read's second round on, round r also
+	 * extends up to 2^r - 1 of the seeds its loop would try next, at most this
+	 * many seeds in all (1: one seed per round). The loop's decisions are
+	 * replayed in order, so results do not depend on it. Presets set 16. */
+	int32_t maxSpeculativeSeeds;
 } ga_params_t;
 
 /* Fills params with the defaults plus preset "vg" or "dbg" (-x), as
@@ -143,6 +148,11 @@ typedef struct
 	int64_t reruns;        /* reads rerun with larger device workspaces */
 	double lookupCallMs;   /* wall time of the seed-lookup calls: kernels, copies and match gathering */
 	double extendCallMs;   /* wall time of the extension calls: kernels, workspace set-up and copies */
+	int64_t unusedExtensions; /* extensions run ahead for later seeds that the seed loop then skipped */
+	double seedingMs;      /* parts of hostMs (rounds mode): seeds from matches and their order, */
+	double pickMs;         /* picking each round's seeds, */
+	double replayMs;       /* replaying the seed loops on the results (trace fix-ups, alignments), */
+	double gafMs;          /* selection and GAF text */
 } ga_align_stats_t;
 
 /* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed

@@ -6,6 +6,7 @@
 #define GA_THREAD_PER_EXTENSION 1
 #define GA_THREAD_HOST_TEST 1
 #include <atomic>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -42,6 +43,7 @@ class HostBackend : public ga::ExtendBackend
 public:
 	gad::DevGraph view(const ga::GraphHost& G)
 	{
+		std::lock_guard<std::mutex> lock(viewMutex_);
 		if (bigSizeUid_ != G.uid)
 		{
 			bigSize_.resize(G.numNodes());
@@ -103,7 +105,7 @@ public:
 				pools[i].assign(r.poolCap, 0);
 				alns[i].assign(r.alnCap, gad::DevAlignment{});
 				gad::ReadResult res;
-				gad::runReadLoop(g, p, r, reads_.data(), seeds.data() + j.seedStart, raw.data(), pools[i].data(), alns[i].data(), base, c, res);
+				gad::runReadLoop(g, p, r, reads_.data(), seeds.data() + j.seedStart, raw.data(), pools[i].data(), alns[i].data(), base, c, res, nullptr, planes_.data());
 				out.results[i] = res;
 				if (res.status != gad::READLOOP_OVERFLOW) break;
 				ga::growReadLoopCaps(caps);
@@ -124,7 +126,9 @@ public:
 		out.launches++;
 	}
 
-	void extend(const ga::GraphHost& G, const gad::ExtParams& p, const std::vector<ga::ExtJob>& jobs, bool, ga::ExtBatch& out, const char* devSeqBase) override
+	uint32_t extendLanes() const override { return 2; }
+
+	void extend(const ga::GraphHost& G, const gad::ExtParams& p, const std::vector<ga::ExtJob>& jobs, bool, ga::ExtBatch& out, const char* devSeqBase, uint32_t lane) override
 	{
 		const gad::DevGraph g = view(G);
 		const size_t n = jobs.size();
@@ -134,9 +138,18 @@ public:
 		out.summaries.clear();
 		std::vector<std::vector<uint64_t>> traces(n);
 		std::atomic<uint64_t> reruns{0};
+		if (devSeqBase && devSeqBase != reads_.data()) throw std::runtime_error("device sequences are not the resident reads");
 		forEachParallel(n, [&](size_t i) {
 			const ga::ExtJob& j = jobs[i];
 			gad::ArenaCaps caps = ga::initialArenaCaps();
+			// planes of the resident bytes, or of the job's own sequence
+			std::vector<uint64_t> ownPlanes;
+			if (!devSeqBase)
+			{
+				ownPlanes.assign(gad::planeBlockCount(j.len) * gad::kPlaneWords, 0);
+				for (uint64_t b = 0; b < gad::planeBlockCount(j.len); b++) gad::planeBlock(j.seq, j.len, b, ownPlanes.data() + b * gad::kPlaneWords);
+			}
+			const uint64_t* planes = devSeqBase ? planes_.data() : ownPlanes.data();
 			for (int attempt = 0; attempt < ga::kArenaAttempts; attempt++)
 			{
 				gad::ArenaCaps c = caps;
@@ -153,7 +166,7 @@ public:
 				r.traceCap = ga::traceCapFor(j.len);
 				traces[i].assign(r.traceCap, 0);
 				gad::ExtResult res;
-				gad::runRequest(g, p, r, devSeqBase ? devSeqBase : j.seq, traces[i].data(), base, c, nullptr, res);
+				gad::runRequest(g, p, r, devSeqBase ? devSeqBase : j.seq, traces[i].data(), base, c, nullptr, res, nullptr, planes);
 				out.results[i] = res;
 				if (res.status != gad::EXT_OVERFLOW) break;
 				if (attempt + 1 < ga::kArenaAttempts) reruns++;
@@ -162,7 +175,7 @@ public:
 			traces[i].resize(out.results[i].status == gad::EXT_OVERFLOW ? 0 : out.results[i].ntrace);
 		});
 		for (size_t i = 0; i < n; i++) out.wordSteps += out.results[i].wordSteps;
-		ga::encodeTraces(G, traces, codeStore_, out);
+		ga::encodeTraces(G, traces, codeStore_[lane], out);
 		out.launches++;
 		out.reruns = reruns.load();
 	}
@@ -185,6 +198,15 @@ public:
 		revBase_ = rs_.bytes + 1;
 		reads_.assign(2 * (rs_.bytes + 1), 0);
 		std::memcpy(reads_.data(), rs_.host.data(), rs_.bytes + 1);
+		computePlanes();
+	}
+
+	// match-mask planes of the resident bytes, as the device keeps them
+	void computePlanes()
+	{
+		const uint64_t blocks = gad::planeBlockCount(reads_.size());
+		planes_.assign(blocks * gad::kPlaneWords, 0);
+		forEachParallel(blocks, [&](size_t b) { gad::planeBlock(reads_.data(), reads_.size(), b, planes_.data() + b * gad::kPlaneWords); });
 	}
 
 	const ga::ReadSet& reads() const override { return rs_; }
@@ -210,17 +232,20 @@ public:
 		out.matches.resize(total);
 		for (size_t i = 0; i < count; i++) std::memcpy(out.matches.data() + out.matchOffset[i], all.data() + rs_.offset[first + i], out.matchCount[i] * sizeof(gad::SeedMatch));
 		out.lookupMs = 0;
+		computePlanes();
 	}
 
 	const char* readsBase() const override { return reads_.data(); }
 	uint64_t reverseBase() const override { return revBase_; }
 
 private:
-	std::vector<uint8_t> codeStore_;
+	std::vector<uint8_t> codeStore_[2];
+	std::mutex viewMutex_;
 	std::vector<uint32_t> bigSize_;
 	uint64_t bigSizeUid_ = 0;
 	ga::ReadSet rs_;
 	std::vector<char> reads_;
+	std::vector<uint64_t> planes_;
 	uint64_t revBase_ = 0;
 };
 
@@ -301,7 +326,7 @@ struct PipeOptions
 	double seedDensity;
 	int64_t minClusterSize;
 	int32_t deviceLoop;
-	int32_t pad;
+	int32_t maxSpeculativeSeeds;
 };
 
 int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char* const* names, const char* const* seqs, const int64_t* lens, char** gafOut, int32_t* statusOut, int64_t* statsOut)
@@ -321,6 +346,7 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 	o.minSeedClusterSize = po->minClusterSize;
 	o.hostThreads = 4;
 	o.deviceSeedLoop = po->deviceLoop != 0;
+	o.maxSpeculativeSeeds = po->maxSpeculativeSeeds < 1 ? 1 : (uint32_t)po->maxSpeculativeSeeds;
 	std::vector<const char*> nm(names, names + nreads), sq(seqs, seqs + nreads);
 	std::vector<uint32_t> ln(nreads);
 	for (int64_t i = 0; i < nreads; i++) ln[i] = (uint32_t)lens[i];
@@ -353,6 +379,8 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 		statsOut[1] = st.extensions;
 		statsOut[2] = st.seeds;
 		statsOut[3] = st.wordSteps;
+		statsOut[4] = st.unusedExtensions;
+		statsOut[5] = st.launches;
 	}
 	return 0;
 }

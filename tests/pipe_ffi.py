@@ -15,7 +15,7 @@ class PipeOptions(ctypes.Structure):
                 ("sloppy", ctypes.c_int32), ("maxCellsPerSlice", ctypes.c_int64),
                 ("seedExtendDensity", ctypes.c_double), ("multimapScoreFraction", ctypes.c_double),
                 ("seedDensity", ctypes.c_double), ("minClusterSize", ctypes.c_int64),
-                ("deviceLoop", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("deviceLoop", ctypes.c_int32), ("maxSpeculativeSeeds", ctypes.c_int32)]
 
 
 def lib():
@@ -86,25 +86,32 @@ class Index:
         return out[: 10 * n.value].reshape(-1, 10)
 
 
-def options(oracle_params, density, device_loop=False):
+def options(oracle_params, density, device_loop=False, speculative=16):
     p = oracle_params
     return PipeOptions(p.bandwidth, p.errorCost, p.xdrop, p.sloppy, p.maxCellsPerSlice, p.seedExtendDensity,
-                       p.multimapScoreFraction, density, p.minClusterSize, int(device_loop), 0)
+                       p.multimapScoreFraction, density, p.minClusterSize, int(device_loop), speculative)
 
 
-def align_reads(index, oracle_params, density, names, seqs, device_loop=False):
+# execution modes of the seed loop: host-driven rounds one seed at a time, rounds
+# with speculative extension of later seeds, the whole loop on the device
+MODES = {"rounds": dict(device_loop=False, speculative=1), "rounds_spec": dict(device_loop=False, speculative=16),
+         "device_loop": dict(device_loop=True, speculative=1)}
+
+
+def align_reads(index, oracle_params, density, names, seqs, device_loop=False, speculative=16):
     n = len(seqs)
     na = (ctypes.c_char_p * n)(*[x.encode() for x in names])
     sa = (ctypes.c_char_p * n)(*[x.encode() for x in seqs])
     lens = np.array([len(s) for s in seqs], dtype=np.int64)
     out = ctypes.c_void_p()
     status = np.zeros(n, dtype=np.int32)
-    stats = np.zeros(4, dtype=np.int64)
-    opt = options(oracle_params, density, device_loop)
+    stats = np.zeros(6, dtype=np.int64)
+    opt = options(oracle_params, density, device_loop, speculative)
     rc = lib().pipe_align_reads(index.h, ctypes.byref(opt), n, na, sa, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                 ctypes.byref(out), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                 stats.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
     assert rc == 0
     text = ctypes.string_at(out.value).decode()
     lib().pipe_free(out)
-    return text, status, dict(rounds=int(stats[0]), extensions=int(stats[1]), seeds=int(stats[2]), word_steps=int(stats[3]))
+    return text, status, dict(rounds=int(stats[0]), extensions=int(stats[1]), seeds=int(stats[2]), word_steps=int(stats[3]),
+                              unused=int(stats[4]), launches=int(stats[5]))

@@ -52,13 +52,22 @@ def test_gpu_seeds_match_oracle(ctx, workdir, wl, preset):
         assert a.shape == b.shape and (a == b).all()
 
 
-@pytest.mark.parametrize("device_loop", [0, 1], ids=["rounds", "device_loop"])
+# execution modes of the seed loop: host-driven rounds one seed at a time, rounds
+# with speculative extension of later seeds, the whole loop on the device
+MODES = {"rounds": (0, 1), "rounds_spec": (0, 16), "device_loop": (1, 1)}
+
+
+def _set_mode(gp, mode):
+    gp.deviceSeedLoop, gp.maxSpeculativeSeeds = MODES[mode]
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
 @pytest.mark.parametrize("wl", ["mini", "mini_sv"])
 @pytest.mark.parametrize("preset", ["vg", "dbg"])
-def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset, device_loop):
+def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset, mode):
     gfa, names, seqs = workload(wl, workdir)
     sd, og, oi, gp, g, gi = _both(gfa, preset)
-    gp.deviceSeedLoop = device_loop
+    _set_mode(gp, mode)
     en, es = edge_reads(names, seqs)
     names = list(names) + en
     seqs = list(seqs) + es
@@ -69,12 +78,12 @@ def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset, device_loop):
     assert gt == ot
 
 
-@pytest.mark.parametrize("device_loop", [0, 1], ids=["rounds", "device_loop"])
-def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir, device_loop):
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir, mode):
     gfa, names, seqs = workload("vg100k", workdir)
     names, seqs = names[:400], seqs[:400]
     sd, og, oi, gp, g, gi = _both(gfa, "vg")
-    gp.deviceSeedLoop = device_loop
+    _set_mode(gp, mode)
     ot, ostat, _ = o.align_reads(og, oi, o.preset_params("vg"), sd["density"], names, seqs, 16)
     print(f"[gpu-align] gaf vg100k reads={len(seqs)}", flush=True)
     gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
@@ -82,10 +91,10 @@ def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir, device_loop):
     assert gt == ot
 
 
-@pytest.mark.parametrize("device_loop", [0, 1], ids=["rounds", "device_loop"])
+@pytest.mark.parametrize("mode", sorted(MODES))
 @pytest.mark.parametrize("wl", ["mini", "mini_sv"])
 @pytest.mark.parametrize("preset", ["vg", "dbg"])
-def test_gpu_gaf_all_seeds_cluster2(ctx, workdir, wl, preset, device_loop):
+def test_gpu_gaf_all_seeds_cluster2(ctx, workdir, wl, preset, mode):
     """--try-all-seeds (no sloppy shortcuts) with --seeds-clustersize 2."""
     gfa, names, seqs = workload(wl, workdir)
     sd, og, oi, gp, g, gi = _both(gfa, preset)
@@ -97,7 +106,7 @@ def test_gpu_gaf_all_seeds_cluster2(ctx, workdir, wl, preset, device_loop):
     op.minClusterSize = 2
     gp.tryAllSeeds = 1
     gp.seedClusterMinSize = 2
-    gp.deviceSeedLoop = device_loop
+    _set_mode(gp, mode)
     ot, ostat, _ = o.align_reads(og, oi, op, sd["density"], names, seqs, 8)
     print(f"[gpu-align] all seeds {wl} {preset} reads={len(seqs)}", flush=True)
     gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)

@@ -51,10 +51,10 @@ def lib_num_kmers(oracle_index):
     return O.lib().orc_index_num_kmers(oracle_index.h)
 
 
-@pytest.mark.parametrize("device_loop", [False, True], ids=["rounds", "device_loop"])
+@pytest.mark.parametrize("mode", sorted(P.MODES))
 @pytest.mark.parametrize("wl", ["mini", "mini_sv"])
 @pytest.mark.parametrize("preset", ["vg", "dbg"])
-def test_gaf_identical(workdir, wl, preset, device_loop):
+def test_gaf_identical(workdir, wl, preset, mode):
     gfa, names, seqs = W.workload(wl, workdir)
     sd, og, oi, pg, pi = _pair(gfa, preset)
     en, es = W.edge_reads(names, seqs)
@@ -62,29 +62,29 @@ def test_gaf_identical(workdir, wl, preset, device_loop):
     seqs = list(seqs) + es
     params = O.preset_params(preset)
     ot, ostat, _ = O.align_reads(og, oi, params, sd["density"], names, seqs, 4)
-    pt, pstat, st = P.align_reads(pi, params, sd["density"], names, seqs, device_loop)
+    pt, pstat, st = P.align_reads(pi, params, sd["density"], names, seqs, **P.MODES[mode])
     assert list(pstat) == list(ostat)
     assert set(ostat.tolist()) >= {0, 1, 2}
     assert pt == ot
     assert st["extensions"] >= int((pstat == 0).sum())
 
 
-@pytest.mark.parametrize("device_loop", [False, True], ids=["rounds", "device_loop"])
-def test_gaf_identical_vg100k_sample(workdir, device_loop):
+@pytest.mark.parametrize("mode", sorted(P.MODES))
+def test_gaf_identical_vg100k_sample(workdir, mode):
     gfa, names, seqs = W.workload("vg100k", workdir)
     names, seqs = names[:120], seqs[:120]
     sd, og, oi, pg, pi = _pair(gfa, "vg")
     params = O.preset_params("vg")
     ot, ostat, _ = O.align_reads(og, oi, params, sd["density"], names, seqs, 8)
-    pt, pstat, _ = P.align_reads(pi, params, sd["density"], names, seqs, device_loop)
+    pt, pstat, _ = P.align_reads(pi, params, sd["density"], names, seqs, **P.MODES[mode])
     assert list(pstat) == list(ostat)
     assert pt == ot
 
 
-@pytest.mark.parametrize("device_loop", [False, True], ids=["rounds", "device_loop"])
+@pytest.mark.parametrize("mode", sorted(P.MODES))
 @pytest.mark.parametrize("wl", ["mini", "mini_sv"])
 @pytest.mark.parametrize("preset", ["vg", "dbg"])
-def test_gaf_identical_all_seeds_cluster2(workdir, wl, preset, device_loop):
+def test_gaf_identical_all_seeds_cluster2(workdir, wl, preset, mode):
     """--try-all-seeds (no sloppy shortcuts) with --seeds-clustersize 2."""
     gfa, names, seqs = W.workload(wl, workdir)
     sd, og, oi, pg, pi = _pair(gfa, preset)
@@ -95,6 +95,6 @@ def test_gaf_identical_all_seeds_cluster2(workdir, wl, preset, device_loop):
     params.sloppy = 0
     params.minClusterSize = 2
     ot, ostat, _ = O.align_reads(og, oi, params, sd["density"], names, seqs, 4)
-    pt, pstat, _ = P.align_reads(pi, params, sd["density"], names, seqs, device_loop)
+    pt, pstat, _ = P.align_reads(pi, params, sd["density"], names, seqs, **P.MODES[mode])
     assert list(pstat) == list(ostat)
     assert pt == ot
