@@ -29,7 +29,6 @@ inline void check(bool c)
 	if (!c) throw ReadAssert{};
 }
 
-constexpr int32_t kScoreMax = INT32_MAX;
 constexpr uint64_t kNoPos = ~0ull;
 
 // Bases an IUPAC letter stands for (A 1, C 2, G 4, T/U 8); '-' stands for none,
@@ -123,13 +122,6 @@ struct TItem
 	char graphChar;
 };
 
-struct OneWay
-{
-	std::vector<TItem> trace;
-	int32_t score = kScoreMax;
-	bool failed() const { return score == kScoreMax; }
-};
-
 struct Alignment
 {
 	std::vector<TItem> trace;
@@ -173,112 +165,57 @@ double msSince(std::chrono::steady_clock::time_point t0)
 	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// Decodes one device trace (packed items, end first) into items with the
-// characters OnewayTrace holds right after the traceback.
-OneWay decodeTrace(const GraphHost& g, const uint64_t* packed, uint32_t n, int32_t score, const char* part, uint64_t partLen)
+// One device trace item (packTrace): split node, offset in it, read position
+// (kNoPos for the seed column, stored as -1) and the node-switch flag.
+struct RawStep
 {
-	OneWay t;
-	t.score = score;
-	t.trace.resize(n);
-	for (uint32_t k = 0; k < n; k++)
-	{
-		uint64_t v = packed[k];
-		uint32_t hi = (uint32_t)(v >> 32);
-		TItem& it = t.trace[k];
-		it.node = (uint32_t)v;
-		it.nodeOffset = (hi >> 1) & 63;
-		it.seqPos = (uint64_t)((int64_t)(hi >> 7) - 1);
-		it.nodeSwitch = (hi & 1) != 0;
-		it.seqChar = it.seqPos < partLen ? part[it.seqPos] : '-';
-		it.graphChar = g.nodeChar((uint32_t)it.node, (uint32_t)it.nodeOffset);
-	}
-	return t;
+	uint32_t node;
+	uint32_t off;
+	uint64_t seqPos;
+	bool sw;
+};
+
+RawStep unpackRaw(uint64_t v)
+{
+	const uint32_t hi = (uint32_t)(v >> 32);
+	RawStep r;
+	r.node = (uint32_t)v;
+	r.off = (hi >> 1) & 63;
+	r.seqPos = (uint64_t)((int64_t)(hi >> 7) - 1);
+	r.sw = (hi & 1) != 0;
+	return r;
 }
 
-// GraphAligner::verifyTrace (src/GraphAligner.h:746-766) on split-node positions.
-void verifyTrace(const GraphHost& g, const std::vector<TItem>& trace)
+// GraphAligner::verifyTrace (src/GraphAligner.h:746-766) for one step o -> n of a
+// split-node trace in alignment order.
+void verifyStep(const GraphHost& g, const RawStep& o, const RawStep& n)
 {
-	for (size_t i = 1; i < trace.size(); i++)
+	check(n.seqPos != o.seqPos || n.node != o.node || n.off != o.off);
+	const bool seqStep = n.seqPos == o.seqPos || n.seqPos == o.seqPos + 1;
+	if (o.node == n.node)
 	{
-		const TItem& o = trace[i - 1];
-		const TItem& n = trace[i];
-		check(n.seqPos != o.seqPos || n.node != o.node || n.nodeOffset != o.nodeOffset);
-		bool seqStep = n.seqPos == o.seqPos || n.seqPos == o.seqPos + 1;
-		if (o.node == n.node)
-		{
-			bool inside = (n.nodeOffset == o.nodeOffset || n.nodeOffset == o.nodeOffset + 1) && seqStep;
-			bool wrap = o.nodeOffset == g.nodeLen[n.node] - 1 && n.nodeOffset == 0 && seqStep;
-			check(inside || wrap);
-			continue;
-		}
-		bool edge = false;
-		for (uint32_t e = g.outOff[o.node]; e < g.outOff[o.node + 1]; e++)
-			if (g.outAdj[e] == n.node) edge = true;
-		check(edge);
-		check(seqStep);
-		check(o.nodeOffset == g.nodeLen[o.node] - 1);
-		check(n.nodeOffset == 0);
+		const bool inside = (n.off == o.off || n.off == o.off + 1) && seqStep;
+		const bool wrap = o.off == g.nodeLen[n.node] - 1 && n.off == 0 && seqStep;
+		check(inside || wrap);
+		return;
 	}
+	bool edge = false;
+	for (uint32_t e = g.outOff[o.node]; e < g.outOff[o.node + 1]; e++)
+		if (g.outAdj[e] == n.node) edge = true;
+	check(edge);
+	check(seqStep);
+	check(o.off == g.nodeLen[o.node] - 1);
+	check(n.off == 0);
 }
 
-// Edit score of a fixed-up trace (src/GraphAligner.h:540-557).
-int32_t traceEditScore(const std::vector<TItem>& t)
+// One term of the fixed-up trace's edit score (src/GraphAligner.h:540-557): item
+// t after item p, or the first item of its direction when p is null.
+int32_t editTerm(const TItem* p, const TItem& t)
 {
-	int32_t score = 0;
-	for (size_t i = 0; i < t.size(); i++)
-	{
-		if (i == 0)
-		{
-			if (!characterMatch(t[0].seqChar, t[0].graphChar)) score += 1;
-			continue;
-		}
-		if (t[i].seqPos == t[i - 1].seqPos) score += 1;
-		else if (!t[i - 1].nodeSwitch && t[i].node == t[i - 1].node && t[i].nodeOffset == t[i - 1].nodeOffset) score += 1;
-		else if (!characterMatch(t[i].seqChar, t[i].graphChar)) score += 1;
-	}
-	return score;
-}
-
-// GraphAligner::fixForwardTraceSeqPos (src/GraphAligner.h:522-559)
-void fixForward(const GraphHost& g, OneWay& t, uint64_t start, const char* seq, uint64_t len)
-{
-	if (t.trace.empty()) return;
-	for (TItem& it : t.trace)
-	{
-		it.seqPos += start;
-		uint64_t split = it.node;
-		it.node = (uint64_t)(int64_t)g.nodeId[split];
-		it.nodeOffset += g.nodeOff[split];
-		check(it.seqPos < len);
-		if (it.seqPos == start - 1) it.seqChar = seq[it.seqPos];
-		check(it.seqChar == seq[it.seqPos]);
-	}
-	t.score = traceEditScore(t.trace);
-}
-
-// GraphAligner::fixReverseTraceSeqPosAndOrder (src/GraphAligner.h:562-601)
-void fixReverse(const GraphHost& g, OneWay& t, uint64_t end, const char* seq, uint64_t len)
-{
-	if (t.trace.empty()) return;
-	std::reverse(t.trace.begin(), t.trace.end());
-	for (size_t i = 0; i + 1 < t.trace.size(); i++) t.trace[i].nodeSwitch = t.trace[i + 1].nodeSwitch;
-	t.trace.back().nodeSwitch = false;
-	for (TItem& it : t.trace)
-	{
-		it.seqPos = end - it.seqPos;
-		// GraphHost::reversePosition with the split node's bigraph size
-		uint64_t split = it.node;
-		int id = g.nodeId[split];
-		uint64_t offset = g.nodeOff[split] + it.nodeOffset;
-		it.node = (uint64_t)(int64_t)((id % 2 == 0) ? (id / 2) * 2 + 1 : (id / 2) * 2);
-		it.nodeOffset = g.splitBigSize[split] - offset - 1;
-		check(it.seqPos < len);
-		it.seqChar = seq[it.seqPos];
-		char c = complementOf(it.graphChar);
-		check(c != 0);
-		it.graphChar = c;
-	}
-	t.score = traceEditScore(t.trace);
+	if (!p) return characterMatch(t.seqChar, t.graphChar) ? 0 : 1;
+	if (t.seqPos == p->seqPos) return 1;
+	if (!p->nodeSwitch && t.node == p->node && t.nodeOffset == p->nodeOffset) return 1;
+	return characterMatch(t.seqChar, t.graphChar) ? 0 : 1;
 }
 
 // GraphAligner::exactAlignmentPart (src/GraphAligner.h:403-457)
@@ -372,11 +309,35 @@ std::vector<Alignment> selectAlignments(std::vector<Alignment>& all, double minS
 	return result;
 }
 
-void addCigar(std::ostringstream& s, uint64_t n, int type)
+// decimal text of v, as std::ostream prints it, appended to s
+void appendUint(std::string& s, uint64_t v)
+{
+	char buf[20];
+	int n = 0;
+	do
+	{
+		buf[n++] = (char)('0' + v % 10);
+		v /= 10;
+	} while (v != 0);
+	while (n > 0) s.push_back(buf[--n]);
+}
+
+void appendInt(std::string& s, int64_t v)
+{
+	if (v < 0)
+	{
+		s.push_back('-');
+		appendUint(s, (uint64_t)0 - (uint64_t)v);
+	}
+	else appendUint(s, (uint64_t)v);
+}
+
+void addCigar(std::string& s, uint64_t n, int type)
 {
 	static const char ops[] = {'=', 'X', 'M', 'I', 'D'};
 	if (n == 0 || type < 0 || type > 4) return;
-	s << n << ops[type];
+	appendUint(s, n);
+	s.push_back(ops[type]);
 }
 
 // GAF line of one alignment (src/GraphAlignerGAFAlignment.h:37-250).
@@ -384,17 +345,18 @@ std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, cons
 {
 	enum { Match = 0, Mismatch = 1, MatchOrMismatch = 2, Insertion = 3, Deletion = 4, Empty = 5 };
 	const auto& t = a.trace;
-	std::ostringstream cigar, path;
+	std::string cigar, path;
+	cigar.reserve(t.size() / 2 + 16);
 	uint64_t pathLen = 0, matches = 0, mismatches = 0, deletions = 0, insertions = 0, editLength = 0;
 	int curId = (int)(int64_t)t[0].node;
 	bool curRev = (t[0].node % 2) == 1;
 	uint64_t curOff = t[0].nodeOffset;
 	int currentEdit = Empty;
 	auto addNode = [&](int id, bool rev) {
-		path << (rev ? "<" : ">");
+		path.push_back(rev ? '<' : '>');
 		const std::string& nm = g.originalName.at(id);
-		if (nm.empty()) path << id / 2;
-		else path << nm;
+		if (nm.empty()) appendInt(path, id / 2);
+		else path += nm;
 	};
 	bool firstMatch = characterMatch(t[0].seqChar, t[0].graphChar);
 	currentEdit = merge ? MatchOrMismatch : (firstMatch ? Match : Mismatch);
@@ -443,17 +405,42 @@ std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, cons
 	addCigar(cigar, editLength, currentEdit);
 	uint64_t pathEnd = pathLen - (g.originalSize.at((int)(int64_t)t.back().node) - 1 - t.back().nodeOffset);
 	double total = (double)(matches + mismatches + deletions + insertions);
-	std::ostringstream s;
-	s << name << "\t" << readLen << "\t" << t[0].seqPos << "\t" << (t.back().seqPos + 1) << "\t+\t" << path.str() << "\t" << pathLen << "\t" << t[0].nodeOffset << "\t" << pathEnd << "\t" << matches << "\t" << t.size() << "\t" << a.mappingQuality;
-	s << "\tNM:i:" << (mismatches + deletions + insertions);
-	if (a.alignmentXScore != -1) s << "\tAS:f:" << a.alignmentXScore;
-	s << "\tdv:f:" << 1.0 - ((double)matches / total);
-	s << "\tid:f:" << ((double)matches / total);
-	s << "\tcg:Z:" << cigar.str();
-	return s.str();
+	std::string s;
+	s.reserve(cigar.size() + path.size() + std::strlen(name) + 192);
+	s += name;
+	s += '\t';
+	appendUint(s, readLen);
+	s += '\t';
+	appendUint(s, t[0].seqPos);
+	s += '\t';
+	appendUint(s, t.back().seqPos + 1);
+	s += "\t+\t";
+	s += path;
+	s += '\t';
+	appendUint(s, pathLen);
+	s += '\t';
+	appendUint(s, t[0].nodeOffset);
+	s += '\t';
+	appendUint(s, pathEnd);
+	s += '\t';
+	appendUint(s, matches);
+	s += '\t';
+	appendUint(s, t.size());
+	s += '\t';
+	appendInt(s, a.mappingQuality);
+	s += "\tNM:i:";
+	appendUint(s, mismatches + deletions + insertions);
+	// floating-point fields in std::ostream's default format
+	std::ostringstream f;
+	if (a.alignmentXScore != -1) f << "\tAS:f:" << a.alignmentXScore;
+	f << "\tdv:f:" << 1.0 - ((double)matches / total);
+	f << "\tid:f:" << ((double)matches / total);
+	s += f.str();
+	s += "\tcg:Z:";
+	s += cigar;
+	return s;
 }
 
-// Seed loop state of one read (GraphAligner::AlignOneWay, src/GraphAligner.h:94-181).
 // One seed of a round with its device jobs (-1: none).
 struct Spec
 {
@@ -461,6 +448,7 @@ struct Spec
 	int64_t bwdJob, fwdJob;
 };
 
+// Seed loop state of one read (GraphAligner::AlignOneWay, src/GraphAligner.h:94-181).
 struct ReadState
 {
 	uint32_t read = 0;
@@ -590,55 +578,92 @@ namespace {
 // fix-ups with their score recount, the merge and the X-score.
 Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char* seq, uint64_t L, const SeedHit& s, const uint64_t* bwdRaw, uint32_t nb, const uint64_t* fwdRaw, uint32_t nf)
 {
-	OneWay bwd, fwd;
+	// One pass per direction builds the merged trace directly: the backward trace
+	// (raw order = read order after fixReverseTraceSeqPosAndOrder, its node-switch
+	// flags moved one item back) at [0, nb), then the forward trace (raw order
+	// reversed, fixForwardTraceSeqPos) from nb - 1 on, its first item replacing the
+	// backward trace's last (getTwoDirectionalTrace's merge). Both traces are
+	// checked step by step in alignment order as verifyTrace does.
 	const int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
 	const int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
 	const auto rp = g.reversePosition(forwardId, s.nodeOffset);
+	Alignment item;
+	if (nb == 0 && nf == 0) return item;
 	if (nb > 0)
 	{
-		std::string revPart(s.seqPos, 0);
-		for (uint64_t k = 0; k < s.seqPos; k++) revPart[k] = complementOf(seq[s.seqPos - 1 - k]);
-		bwd = decodeTrace(g, bwdRaw, nb, 0, revPart.data(), revPart.size());
+		const RawStep b = unpackRaw(bwdRaw[nb - 1]);
+		check(b.seqPos == kNoPos && g.nodeId[b.node] == backwardId && g.nodeOff[b.node] + b.off == rp.second);
 	}
-	if (nf > 0) fwd = decodeTrace(g, fwdRaw, nf, 0, seq + s.seqPos + 1, L - s.seqPos - 1);
-	if (!bwd.failed())
+	if (nf > 0)
 	{
-		const TItem& b = bwd.trace.back();
-		check(b.seqPos == kNoPos && g.nodeId[b.node] == backwardId && g.nodeOff[b.node] + b.nodeOffset == rp.second);
-		std::reverse(bwd.trace.begin(), bwd.trace.end());
+		const RawStep f = unpackRaw(fwdRaw[nf - 1]);
+		check(f.seqPos == kNoPos && g.nodeId[f.node] == forwardId && g.nodeOff[f.node] + f.off == s.nodeOffset);
 	}
-	if (!fwd.failed())
+	std::vector<TItem>& out = item.trace;
+	out.resize((size_t)nb + nf - (nb > 0 && nf > 0 ? 1 : 0));
+	int32_t score = 0;
+	if (nb > 0)
 	{
-		const TItem& f = fwd.trace.back();
-		check(f.seqPos == kNoPos && g.nodeId[f.node] == forwardId && g.nodeOff[f.node] + f.nodeOffset == s.nodeOffset);
-		std::reverse(fwd.trace.begin(), fwd.trace.end());
+		const uint64_t end = s.seqPos - 1;
+		RawStep next = unpackRaw(bwdRaw[0]);
+		for (uint32_t k = 0; k < nb; k++)
+		{
+			const RawStep cur = next;
+			if (k + 1 < nb)
+			{
+				next = unpackRaw(bwdRaw[k + 1]);
+				verifyStep(g, next, cur);
+			}
+			TItem& t = out[k];
+			// GraphHost::reversePosition with the split node's bigraph size
+			const int id = g.nodeId[cur.node];
+			const uint64_t offset = g.nodeOff[cur.node] + cur.off;
+			t.node = (uint64_t)(int64_t)((id % 2 == 0) ? (id / 2) * 2 + 1 : (id / 2) * 2);
+			t.nodeOffset = g.splitBigSize[cur.node] - offset - 1;
+			t.seqPos = end - cur.seqPos;
+			t.nodeSwitch = k + 1 < nb ? next.sw : false;
+			check(t.seqPos < L);
+			t.seqChar = seq[t.seqPos];
+			const char c = complementOf(g.nodeChar(cur.node, cur.off));
+			check(c != 0);
+			t.graphChar = c;
+			score += editTerm(k == 0 ? nullptr : &out[k - 1], t);
+		}
 	}
-	if (!fwd.trace.empty()) verifyTrace(g, fwd.trace);
-	if (!bwd.trace.empty()) verifyTrace(g, bwd.trace);
-	fixReverse(g, bwd, s.seqPos - 1, seq, L);
-	fixForward(g, fwd, s.seqPos + 1, seq, L);
-	Alignment item;
-	if (fwd.failed() && bwd.failed()) return item;
-	OneWay merged = std::move(bwd);
-	if (merged.failed())
+	if (nf > 0)
 	{
-		merged = std::move(fwd);
+		const uint64_t start = s.seqPos + 1;
+		const size_t base = nb > 0 ? nb - 1 : 0;
+		RawStep prevRaw{};
+		TItem prev{};
+		for (uint32_t i = 0; i < nf; i++)
+		{
+			const RawStep cur = unpackRaw(fwdRaw[nf - 1 - i]);
+			if (i > 0) verifyStep(g, prevRaw, cur);
+			TItem t;
+			t.node = (uint64_t)(int64_t)g.nodeId[cur.node];
+			t.nodeOffset = cur.off + g.nodeOff[cur.node];
+			t.seqPos = cur.seqPos + start;
+			t.nodeSwitch = cur.sw;
+			check(t.seqPos < L);
+			t.seqChar = seq[t.seqPos];
+			t.graphChar = g.nodeChar(cur.node, cur.off);
+			score += editTerm(i == 0 ? nullptr : &prev, t);
+			if (i == 0 && nb > 0)
+			{
+				const TItem& x = out[base];
+				check(x.node == t.node && x.nodeOffset == t.nodeOffset && x.seqPos == t.seqPos);
+			}
+			out[base + i] = t;
+			prev = t;
+			prevRaw = cur;
+		}
 	}
-	else if (!fwd.failed())
-	{
-		const TItem& x = merged.trace.back();
-		const TItem& y = fwd.trace[0];
-		check(x.node == y.node && x.nodeOffset == y.nodeOffset && x.seqPos == y.seqPos);
-		merged.trace.pop_back();
-		merged.trace.insert(merged.trace.end(), fwd.trace.begin(), fwd.trace.end());
-		merged.score += fwd.score;
-	}
-	check(!merged.trace.empty());
-	uint64_t seqStart = merged.trace[0].seqPos;
-	uint64_t seqEnd = merged.trace.back().seqPos;
+	check(!out.empty());
+	uint64_t seqStart = out[0].seqPos;
+	uint64_t seqEnd = out.back().seqPos;
 	check(seqEnd < L);
-	item.traceScore = merged.score;
-	item.trace = std::move(merged.trace);
+	item.traceScore = score;
 	item.alignmentScore = (uint64_t)(int64_t)item.traceScore;
 	item.alignmentStart = seqStart;
 	item.alignmentEnd = seqEnd + 1;
