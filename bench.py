@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reads", type=int, default=0, help="use only the first N reads (0: all 20,000)")
-    ap.add_argument("--warmup-reads", type=int, default=2000, help="reads aligned by each warmup step")
+    ap.add_argument("--warmup-reads", type=int, default=0, help="reads aligned by each warmup step (0: all, as a timed step)")
     ap.add_argument("--cpu-sample", type=int, default=3000, help="reads in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores, at most 16)")
     ap.add_argument("--workdir", default=os.environ.get("GA_BENCH_DIR", ""))

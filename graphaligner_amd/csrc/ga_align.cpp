@@ -7,6 +7,9 @@
 #include <climits>
 #include <condition_variable>
 #include <exception>
+#include <functional>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -466,6 +469,7 @@ struct ReadState
 	size_t pending = 0;
 	int64_t bwdJob = -1, fwdJob = -1;
 	bool pendingReady = false;
+	bool finished = false; // selection and GAF done
 	// seeds extended in the current round (the picked one first) and the read's
 	// rounds so far
 	std::vector<Spec> specs;
@@ -943,11 +947,15 @@ void updateEndToEnd(ReadState& st, uint64_t readLen)
 }
 
 // Seeds extended together in a read's round r (0-based): the first round only
-// the picked seed, then twice as many each round, up to maxSeeds.
-uint32_t speculationWidth(uint32_t round, uint32_t maxSeeds)
+// the picked seed; later rounds four times as many as the round before, or more when
+// few reads remain (up to jobBudget extension jobs in the launch, two per seed:
+// a launch lasts as long as its longest extension, so jobs that fill idle
+// wavefront slots cost little), never more than maxSeeds.
+uint32_t speculationWidth(uint32_t round, uint32_t maxSeeds, size_t activeReads, uint32_t jobBudget)
 {
 	if (round == 0 || maxSeeds <= 1) return 1;
-	uint64_t w = (uint64_t)1 << std::min<uint32_t>(round, 20);
+	uint64_t w = (uint64_t)1 << std::min<uint32_t>(2 * round, 20);
+	if (activeReads > 0) w = std::max<uint64_t>(w, jobBudget / (2 * (uint64_t)activeReads));
 	return (uint32_t)std::min<uint64_t>(w, maxSeeds);
 }
 
@@ -956,6 +964,8 @@ uint32_t speculationWidth(uint32_t round, uint32_t maxSeeds)
 void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
 {
 	auto t0 = std::chrono::steady_clock::now();
+	// GA_ROUND_LOG=1 prints one line per round and group to stderr
+	const bool roundLog = std::getenv("GA_ROUND_LOG") != nullptr;
 	const ReadSet& rs = ctx.reads();
 	if ((uint64_t)first + count > rs.size()) throw std::runtime_error("read range outside the resident reads");
 	stats = AlignStats{};
@@ -1010,6 +1020,49 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		});
 		{ double ms = msSince(th); stats.hostMs += ms; stats.seedingMs += ms; }
 		for (auto& st : states) stats.seeds += st.seeds.size();
+		// selection and GAF of one read (src/Aligner.cpp:502-606); its alignments
+		// are released afterwards
+		auto selectAndWrite = [&](size_t i) {
+			ReadState& st = states[i];
+			ReadOutcome& o = out[batchStart + i];
+			if (o.status == READ_NO_SEEDS) return;
+			o.seedsExtended = st.seedsExtended;
+			if (st.status != READ_ALIGNED)
+			{
+				o.status = st.status;
+				return;
+			}
+			std::vector<Alignment> sel;
+			try
+			{
+				if (!st.alignments.empty()) sel = selectAlignments(st.alignments, opt.minAlignmentScore, opt.multimapScoreFraction);
+				if (sel.empty())
+				{
+					o.status = READ_NO_ALIGNMENT;
+					return;
+				}
+				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
+				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentXScore > r.alignmentXScore; });
+				std::string text;
+				for (const auto& a : sel)
+				{
+					text += gafLine(g, names[batchStart + i], blens[i], a, opt.cigarMatchMismatchMerge);
+					text += '\n';
+				}
+				o.gaf = std::move(text);
+				o.status = READ_ALIGNED;
+			}
+			catch (const ReadAssert&)
+			{
+				o.status = READ_ASSERT;
+				o.gaf.clear();
+			}
+		};
+		auto finishRead = [&](size_t i) {
+			selectAndWrite(i);
+			states[i].finished = true;
+			std::vector<Alignment>().swap(states[i].alignments);
+		};
 		// the seed loops of G groups of reads (G = the backend's extension lanes, at
 		// most opt.pipelineGroups) run in G host threads, each on its own lane,
 		// so one group's host replay overlaps the other groups' kernels
@@ -1024,9 +1077,9 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		std::vector<AlignStats> groupStats(groups);
 		std::vector<std::exception_ptr> groupErrors(groups);
 		const int threads = opt.hostThreads;
-		// group k's first launch waits until k earlier first launches have
-		// returned: the groups start out of phase, so while one group's host
-		// work runs another's kernel has the GPU
+		// the other groups' first launches wait until group 0's has returned: the
+		// groups start out of phase, so while one group's host work runs another's
+		// kernel has the GPU
 		std::mutex phaseMutex;
 		std::condition_variable phaseCv;
 		uint32_t firstLaunchesDone = 0;
@@ -1048,8 +1101,46 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 			}
 			~FirstLaunch() { signal(); }
 		};
+		// runs finishRead over a list of reads on a thread of its own, one list at a time
+		struct Finisher
+		{
+			std::function<void(size_t)> fn;
+			int threads;
+			double& ms;
+			std::thread worker;
+			std::vector<uint32_t> reads;
+			std::exception_ptr error;
+			void join()
+			{
+				if (worker.joinable()) worker.join();
+				if (error) std::rethrow_exception(error);
+			}
+			void run(std::vector<uint32_t>&& next)
+			{
+				join();
+				if (next.empty()) return;
+				reads = std::move(next);
+				worker = std::thread([this]() {
+					try
+					{
+						auto t = std::chrono::steady_clock::now();
+						parallelFor(reads.size(), threads, [&](size_t k) { fn(reads[k]); });
+						ms += msSince(t);
+					}
+					catch (...)
+					{
+						error = std::current_exception();
+					}
+				});
+			}
+			~Finisher()
+			{
+				if (worker.joinable()) worker.join();
+			}
+		};
 		auto runGroup = [&](uint32_t lane) {
 			AlignStats& gs = groupStats[lane];
+			Finisher finisher{finishRead, threads, gs.gafMs};
 			FirstLaunch firstLaunch{phaseMutex, phaseCv, firstLaunchesDone};
 			std::vector<uint32_t> active = groupReads[lane];
 			while (!active.empty())
@@ -1070,7 +1161,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 						}
 						st.pendingReady = false;
 						st.specs.push_back(Spec{st.pending, -1, -1});
-						const uint32_t k = speculationWidth(st.roundsRun, opt.maxSpeculativeSeeds);
+						const uint32_t k = speculationWidth(st.roundsRun, opt.maxSpeculativeSeeds, active.size(), opt.speculationJobBudget);
 						if (k > 1)
 						{
 							LoopCursor c = cursorOf(st);
@@ -1126,7 +1217,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				if (firstLaunch.pending && lane > 0)
 				{
 					std::unique_lock<std::mutex> lock(phaseMutex);
-					phaseCv.wait(lock, [&]() { return firstLaunchesDone >= lane; });
+					phaseCv.wait(lock, [&]() { return firstLaunchesDone >= 1; });
 				}
 				ExtBatch eb;
 				auto te = std::chrono::steady_clock::now();
@@ -1211,12 +1302,23 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 					}
 				});
 				gs.unusedExtensions += unused.load();
-				{ double ms = msSince(th); gs.hostMs += ms; gs.replayMs += ms; }
-				std::vector<uint32_t> still;
+				{
+					double ms = msSince(th);
+					gs.hostMs += ms;
+					gs.replayMs += ms;
+					if (roundLog)
+						std::fprintf(stderr, "[rounds] group %u round %llu: %zu reads, %zu jobs, extend %.1f ms (kernel %.1f ms), replay %.1f ms, ends at %.1f ms\n", lane,
+						             (unsigned long long)gs.rounds, extending.size(), jobs.size(), msSince(te) - ms, eb.kernelMs, ms, msSince(t0));
+				}
+				std::vector<uint32_t> still, ended;
 				for (uint32_t r : active)
-					if (!states[r].done) still.push_back(r);
+					(states[r].done ? ended : still).push_back(r);
 				active.swap(still);
+				// reads whose loops ended are written out by a helper thread while
+				// the group's next round runs
+				finisher.run(std::move(ended));
 			}
+			finisher.join();
 		};
 		if (groups == 1) runGroup(0);
 		else
@@ -1250,54 +1352,23 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		}
 		// the groups' host phases overlap: count the longest group's
 		{
-			double hostMax = 0, pickMax = 0, replayMax = 0;
+			double hostMax = 0, pickMax = 0, replayMax = 0, gafMax = 0;
 			for (const AlignStats& gs : groupStats)
 			{
 				hostMax = std::max(hostMax, gs.hostMs);
 				pickMax = std::max(pickMax, gs.pickMs);
 				replayMax = std::max(replayMax, gs.replayMs);
+				gafMax = std::max(gafMax, gs.gafMs);
 			}
 			stats.hostMs += hostMax;
 			stats.pickMs += pickMax;
 			stats.replayMs += replayMax;
+			stats.gafMs += gafMax;
 		}
-		// selection and GAF (src/Aligner.cpp:502-606)
+		// selection and GAF of the reads the seed loops did not reach
 		th = std::chrono::steady_clock::now();
 		parallelFor(bn, opt.hostThreads, [&](size_t i) {
-			ReadState& st = states[i];
-			ReadOutcome& o = out[batchStart + i];
-			if (o.status == READ_NO_SEEDS) return;
-			o.seedsExtended = st.seedsExtended;
-			if (st.status != READ_ALIGNED)
-			{
-				o.status = st.status;
-				return;
-			}
-			std::vector<Alignment> sel;
-			try
-			{
-				if (!st.alignments.empty()) sel = selectAlignments(st.alignments, opt.minAlignmentScore, opt.multimapScoreFraction);
-				if (sel.empty())
-				{
-					o.status = READ_NO_ALIGNMENT;
-					return;
-				}
-				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
-				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentXScore > r.alignmentXScore; });
-				std::string text;
-				for (const auto& a : sel)
-				{
-					text += gafLine(g, names[batchStart + i], blens[i], a, opt.cigarMatchMismatchMerge);
-					text += '\n';
-				}
-				o.gaf = std::move(text);
-				o.status = READ_ALIGNED;
-			}
-			catch (const ReadAssert&)
-			{
-				o.status = READ_ASSERT;
-				o.gaf.clear();
-			}
+			if (!states[i].finished) finishRead(i);
 		});
 		{ double ms = msSince(th); stats.hostMs += ms; stats.gafMs += ms; }
 	}

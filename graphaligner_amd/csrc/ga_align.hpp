@@ -47,12 +47,16 @@ struct AlignOptions
 	// of every unfinished read in one launch (both directions as separate jobs);
 	// true: every read's whole seed loop runs on the device in one launch
 	bool deviceSeedLoop = false;
-	// host-driven rounds: a read's round r > 0 also extends up to 2^r - 1 of the
-	// seeds its loop would try next (at most maxSpeculativeSeeds seeds in all);
+	// host-driven rounds: a read's round r > 0 also extends up to 4^r - 1 of the
+	// seeds its loop would try next (at most maxSpeculativeSeeds seeds in all,
+	// more than 4^r when few reads remain, see speculationJobBudget);
 	// the loop's decisions are then replayed in order, so results are those of
 	// one seed at a time and extensions of seeds the replay skips are unused.
 	// Fewer rounds means fewer launches waiting on their longest extension.
-	uint32_t maxSpeculativeSeeds = 16;
+	uint32_t maxSpeculativeSeeds = 64;
+	// and, from the second round on, up to this many extension jobs per launch
+	// when few reads remain (roughly the wavefronts one launch keeps resident)
+	uint32_t speculationJobBudget = 3072;
 	// host-driven rounds: the reads are dealt (longest first) into this many
 	// groups, at most the backend's extension lanes, whose seed loops run in
 	// separate host threads so one group's host work overlaps another's kernels

@@ -72,7 +72,7 @@ ga_status_t ga_params_preset(const char* preset, ga_params_t* p)
 	p->minimizerSeedDensity = 0;
 	p->minimizerDiscardMostNumerousFraction = 0.0002;
 	p->cigarMatchMismatchMerge = 0;
-	p->maxSpeculativeSeeds = 16;
+	p->maxSpeculativeSeeds = 64;
 	if (!preset) return GA_OK;
 	std::string s = preset;
 	if (s == "dbg")
@@ -347,6 +347,11 @@ AlignOptions alignOptionsOf(const ga_params_t& p)
 	opt.cigarMatchMismatchMerge = p.cigarMatchMismatchMerge != 0;
 	opt.deviceSeedLoop = p.deviceSeedLoop != 0;
 	opt.maxSpeculativeSeeds = p.maxSpeculativeSeeds < 1 ? 1 : (uint32_t)p.maxSpeculativeSeeds;
+	if (const char* v = std::getenv("GA_PIPELINE_GROUPS"))
+	{
+		int x = std::atoi(v);
+		if (x >= 1 && x <= 64) opt.pipelineGroups = (uint32_t)x;
+	}
 	return opt;
 }
 

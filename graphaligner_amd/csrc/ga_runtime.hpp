@@ -133,7 +133,7 @@ private:
 	static constexpr size_t kCachedObjects = 2;
 	std::vector<std::unique_ptr<DeviceGraph>> graphs_;
 	std::vector<std::unique_ptr<DeviceIndex>> indexes_;
-	static constexpr size_t kExtendLanes = 2;
+	static constexpr size_t kExtendLanes = 4;
 	std::vector<std::unique_ptr<ExtendLane>> lanes_;
 	std::mutex cacheMutex_; // graph and index caches, shared by the lanes
 	DeviceBuffer arena_, gatherSrc_, gatherDst_, gatherCnt_, traceOut_; // device seed loop

@@ -58,9 +58,10 @@ typedef struct
 	 * the GPU, all reads in one launch. */
 	int32_t deviceSeedLoop;
 	/* Execution only, rounds mode: from a read's second round on, round r also
-	 * extends up to 2^r - 1 of the seeds its loop would try next, at most this
-	 * many seeds in all (1: one seed per round). The loop's decisions are
-	 * replayed in order, so results do not depend on it. Presets set 16. */
+	 * extends up to 4^r - 1 of the seeds its loop would try next (more when few
+	 * reads remain), at most this many seeds in all (1: one seed per round). The
+	 * loop's decisions are replayed in order, so results do not depend on it.
+	 * Presets set 64. */
 	int32_t maxSpeculativeSeeds;
 } ga_params_t;
 

@@ -86,7 +86,7 @@ class Index:
         return out[: 10 * n.value].reshape(-1, 10)
 
 
-def options(oracle_params, density, device_loop=False, speculative=16):
+def options(oracle_params, density, device_loop=False, speculative=64):
     p = oracle_params
     return PipeOptions(p.bandwidth, p.errorCost, p.xdrop, p.sloppy, p.maxCellsPerSlice, p.seedExtendDensity,
                        p.multimapScoreFraction, density, p.minClusterSize, int(device_loop), speculative)
@@ -94,11 +94,11 @@ def options(oracle_params, density, device_loop=False, speculative=16):
 
 # execution modes of the seed loop: host-driven rounds one seed at a time, rounds
 # with speculative extension of later seeds, the whole loop on the device
-MODES = {"rounds": dict(device_loop=False, speculative=1), "rounds_spec": dict(device_loop=False, speculative=16),
+MODES = {"rounds": dict(device_loop=False, speculative=1), "rounds_spec": dict(device_loop=False, speculative=64),
          "device_loop": dict(device_loop=True, speculative=1)}
 
 
-def align_reads(index, oracle_params, density, names, seqs, device_loop=False, speculative=16):
+def align_reads(index, oracle_params, density, names, seqs, device_loop=False, speculative=64):
     n = len(seqs)
     na = (ctypes.c_char_p * n)(*[x.encode() for x in names])
     sa = (ctypes.c_char_p * n)(*[x.encode() for x in seqs])

@@ -54,7 +54,7 @@ def test_gpu_seeds_match_oracle(ctx, workdir, wl, preset):
 
 # execution modes of the seed loop: host-driven rounds one seed at a time, rounds
 # with speculative extension of later seeds, the whole loop on the device
-MODES = {"rounds": (0, 1), "rounds_spec": (0, 16), "device_loop": (1, 1)}
+MODES = {"rounds": (0, 1), "rounds_spec": (0, 64), "device_loop": (1, 1)}
 
 
 def _set_mode(gp, mode):
