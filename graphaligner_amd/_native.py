@@ -48,7 +48,7 @@ class AlignStats(ctypes.Structure):
                 ("extensions", ctypes.c_int64), ("wordSteps", ctypes.c_int64), ("seeds", ctypes.c_int64),
                 ("reruns", ctypes.c_int64), ("lookupCallMs", ctypes.c_double), ("extendCallMs", ctypes.c_double),
                 ("unusedExtensions", ctypes.c_int64), ("seedingMs", ctypes.c_double), ("pickMs", ctypes.c_double),
-                ("replayMs", ctypes.c_double), ("gafMs", ctypes.c_double)]
+                ("replayMs", ctypes.c_double), ("gafMs", ctypes.c_double), ("gafBytes", ctypes.c_int64)]
 
 
 _lib = None

@@ -163,6 +163,19 @@ def seeds(ctx, graph, index, seqs, density, order=True):
     return res
 
 
+_decode_utf8 = ctypes.pythonapi.PyUnicode_DecodeUTF8
+_decode_utf8.restype = ctypes.py_object
+_decode_utf8.argtypes = [ctypes.c_void_p, ctypes.c_ssize_t, ctypes.c_char_p]
+
+
+def _gaf_text(out, st):
+    """The library's GAF buffer as str (decoded in place, one copy), then freed."""
+    try:
+        return _decode_utf8(out.value, st.gafBytes, b"strict")
+    finally:
+        N.lib().ga_free(out)
+
+
 def align_reads(ctx, graph, index, params, names, seqs):
     """GAF text of all reads (lines of read 0 first), per-read status, stats dict."""
     n, arr, lens = _reads(seqs)
@@ -173,8 +186,7 @@ def align_reads(ctx, graph, index, params, names, seqs):
     P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
     _check(N.lib().ga_align_reads(ctx.h, graph.h, index.h, ctypes.byref(params), n, na, arr, P(lens), ctypes.byref(out),
                                   P(status), ctypes.byref(st)), "ga_align_reads")
-    text = ctypes.string_at(out.value).decode()
-    N.lib().ga_free(out)
+    text = _gaf_text(out, st)
     stats = {f: getattr(st, f) for f, _ in N.AlignStats._fields_}
     return text, status[:n], stats
 
@@ -194,7 +206,6 @@ def align_resident(ctx, graph, index, params, first, names):
     st = N.AlignStats()
     _check(N.lib().ga_align_resident(ctx.h, graph.h, index.h, ctypes.byref(params), first, count, na, ctypes.byref(out),
                                      status.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)), "ga_align_resident")
-    text = ctypes.string_at(out.value).decode()
-    N.lib().ga_free(out)
+    text = _gaf_text(out, st)
     stats = {f: getattr(st, f) for f, _ in N.AlignStats._fields_}
     return text, status[:count], stats

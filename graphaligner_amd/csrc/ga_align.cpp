@@ -123,6 +123,8 @@ struct TItem
 	bool nodeSwitch;
 	char seqChar;
 	char graphChar;
+	// left uninitialised: traces are sized first and then filled item by item
+	TItem() {}
 };
 
 struct Alignment
@@ -580,55 +582,128 @@ namespace {
 // traces the device seed loop kept (either may be empty: that direction failed
 // or did not run): the checks of getTwoDirectionalTrace, verifyTrace, the
 // fix-ups with their score recount, the merge and the X-score.
-Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char* seq, uint64_t L, const SeedHit& s, const uint64_t* bwdRaw, uint32_t nb, const uint64_t* fwdRaw, uint32_t nf)
+// Sequential readers of one direction's trace (end first): packed items as the
+// device wrote them, or the compact codes of ga_trace_codes.h.
+struct PackedReader
 {
-	// One pass per direction builds the merged trace directly: the backward trace
-	// (raw order = read order after fixReverseTraceSeqPosAndOrder, its node-switch
-	// flags moved one item back) at [0, nb), then the forward trace (raw order
-	// reversed, fixForwardTraceSeqPos) from nb - 1 on, its first item replacing the
-	// backward trace's last (getTwoDirectionalTrace's merge). Both traces are
-	// checked step by step in alignment order as verifyTrace does.
+	const uint64_t* p;
+	uint64_t next() { return *p++; }
+};
+
+struct CodeReader
+{
+	const uint8_t* codes;
+	const std::pair<uint64_t, uint64_t>* esc;
+	const std::pair<uint64_t, uint64_t>* escEnd;
+	uint64_t index;
+	uint64_t prev;
+	gad::TraceGraph tg;
+	CodeReader(const GraphHost& g, const ExtBatch& b, size_t job)
+	    : codes(b.codes), esc(nullptr), escEnd(b.escapes.data() + b.escapes.size()), index(b.traceOffset[job]), prev(0), tg{g.nodeLen.data(), g.inOff.data(), g.inAdj.data()}
+	{
+		esc = b.escapes.data() + (std::lower_bound(b.escapes.begin(), b.escapes.end(), std::make_pair(index, (uint64_t)0)) - b.escapes.begin());
+	}
+	uint64_t next()
+	{
+		const uint32_t c = codes[index];
+		if (c == gad::kTraceEscape)
+		{
+			if (esc == escEnd || esc->first != index) throw std::runtime_error("trace codes: escaped item missing");
+			prev = esc->second;
+			++esc;
+		}
+		else prev = gad::traceDecode(tg, prev, c);
+		index++;
+		return prev;
+	}
+};
+
+// What the trace rebuild needs of one split node, refreshed when the trace
+// enters another node: its bigraph id and offset base in either direction and
+// its bases.
+struct NodeView
+{
+	uint32_t split = UINT32_MAX;
+	uint64_t bigNode = 0;
+	uint64_t offBase = 0;
+	uint64_t w0 = 0, w1 = 0;
+	bool amb = false;
+	// reverse: the bigraph node of the other strand, offsets counted from its start
+	void load(const GraphHost& g, uint32_t n, bool reverse)
+	{
+		split = n;
+		const int id = g.nodeId[n];
+		if (reverse)
+		{
+			bigNode = (uint64_t)(int64_t)((id % 2 == 0) ? (id / 2) * 2 + 1 : (id / 2) * 2);
+			offBase = g.splitBigSize[n] - g.nodeOff[n] - 1;
+		}
+		else
+		{
+			bigNode = (uint64_t)(int64_t)id;
+			offBase = g.nodeOff[n];
+		}
+		amb = n >= g.firstAmbiguous;
+		if (!amb)
+		{
+			w0 = g.seq2[2 * (uint64_t)n];
+			w1 = g.seq2[2 * (uint64_t)n + 1];
+		}
+	}
+	char base(const GraphHost& g, uint32_t off) const
+	{
+		if (amb) return g.nodeChar(split, off);
+		return "ACGT"[((off < 32 ? w0 : w1) >> ((off & 31) * 2)) & 3];
+	}
+};
+
+inline void verifyFast(const GraphHost& g, const RawStep& o, const RawStep& n)
+{
+	if (o.node == n.node && n.off == o.off + 1 && (n.seqPos == o.seqPos || n.seqPos == o.seqPos + 1)) return;
+	verifyStep(g, o, n);
+}
+
+template <typename Reader>
+Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char* seq, uint64_t L, const SeedHit& s, Reader bwd, uint32_t nb, Reader fwd, uint32_t nf)
+{
+	// One pass per direction, in the order the traces are stored, builds the
+	// merged trace directly: the backward trace (its order is read order after
+	// fixReverseTraceSeqPosAndOrder, node-switch flags moved one item back) at
+	// [0, nb), then the forward trace reversed (fixForwardTraceSeqPos) from
+	// nb - 1 on, its first item replacing the backward trace's last
+	// (getTwoDirectionalTrace's merge). Every step is checked as verifyTrace does
+	// and the edit scores are summed term by term.
 	const int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
 	const int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
 	const auto rp = g.reversePosition(forwardId, s.nodeOffset);
 	Alignment item;
 	if (nb == 0 && nf == 0) return item;
-	if (nb > 0)
-	{
-		const RawStep b = unpackRaw(bwdRaw[nb - 1]);
-		check(b.seqPos == kNoPos && g.nodeId[b.node] == backwardId && g.nodeOff[b.node] + b.off == rp.second);
-	}
-	if (nf > 0)
-	{
-		const RawStep f = unpackRaw(fwdRaw[nf - 1]);
-		check(f.seqPos == kNoPos && g.nodeId[f.node] == forwardId && g.nodeOff[f.node] + f.off == s.nodeOffset);
-	}
 	std::vector<TItem>& out = item.trace;
 	out.resize((size_t)nb + nf - (nb > 0 && nf > 0 ? 1 : 0));
 	int32_t score = 0;
+	NodeView view;
 	if (nb > 0)
 	{
 		const uint64_t end = s.seqPos - 1;
-		RawStep next = unpackRaw(bwdRaw[0]);
+		RawStep next = unpackRaw(bwd.next());
 		for (uint32_t k = 0; k < nb; k++)
 		{
 			const RawStep cur = next;
 			if (k + 1 < nb)
 			{
-				next = unpackRaw(bwdRaw[k + 1]);
-				verifyStep(g, next, cur);
+				next = unpackRaw(bwd.next());
+				verifyFast(g, next, cur);
 			}
+			else check(cur.seqPos == kNoPos && g.nodeId[cur.node] == backwardId && g.nodeOff[cur.node] + cur.off == rp.second);
+			if (cur.node != view.split) view.load(g, cur.node, true);
 			TItem& t = out[k];
-			// GraphHost::reversePosition with the split node's bigraph size
-			const int id = g.nodeId[cur.node];
-			const uint64_t offset = g.nodeOff[cur.node] + cur.off;
-			t.node = (uint64_t)(int64_t)((id % 2 == 0) ? (id / 2) * 2 + 1 : (id / 2) * 2);
-			t.nodeOffset = g.splitBigSize[cur.node] - offset - 1;
+			t.node = view.bigNode;
+			t.nodeOffset = view.offBase - cur.off;
 			t.seqPos = end - cur.seqPos;
 			t.nodeSwitch = k + 1 < nb ? next.sw : false;
 			check(t.seqPos < L);
 			t.seqChar = seq[t.seqPos];
-			const char c = complementOf(g.nodeChar(cur.node, cur.off));
+			const char c = complementOf(view.base(g, cur.off));
 			check(c != 0);
 			t.graphChar = c;
 			score += editTerm(k == 0 ? nullptr : &out[k - 1], t);
@@ -637,29 +712,37 @@ Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char
 	if (nf > 0)
 	{
 		const uint64_t start = s.seqPos + 1;
-		const size_t base = nb > 0 ? nb - 1 : 0;
+		const size_t last = (nb > 0 ? nb - 1 : 0) + nf - 1; // where raw item 0 goes
 		RawStep prevRaw{};
-		TItem prev{};
-		for (uint32_t i = 0; i < nf; i++)
+		TItem prevT{};
+		view.split = UINT32_MAX;
+		for (uint32_t k = 0; k < nf; k++)
 		{
-			const RawStep cur = unpackRaw(fwdRaw[nf - 1 - i]);
-			if (i > 0) verifyStep(g, prevRaw, cur);
+			const RawStep cur = unpackRaw(fwd.next());
+			if (k > 0) verifyFast(g, cur, prevRaw);
+			if (cur.node != view.split) view.load(g, cur.node, false);
 			TItem t;
-			t.node = (uint64_t)(int64_t)g.nodeId[cur.node];
-			t.nodeOffset = cur.off + g.nodeOff[cur.node];
+			t.node = view.bigNode;
+			t.nodeOffset = view.offBase + cur.off;
 			t.seqPos = cur.seqPos + start;
 			t.nodeSwitch = cur.sw;
 			check(t.seqPos < L);
 			t.seqChar = seq[t.seqPos];
-			t.graphChar = g.nodeChar(cur.node, cur.off);
-			score += editTerm(i == 0 ? nullptr : &prev, t);
-			if (i == 0 && nb > 0)
+			t.graphChar = view.base(g, cur.off);
+			// the term of the step from this item to the one after it in read order
+			if (k > 0) score += editTerm(&t, prevT);
+			if (k + 1 == nf)
 			{
-				const TItem& x = out[base];
-				check(x.node == t.node && x.nodeOffset == t.nodeOffset && x.seqPos == t.seqPos);
+				check(cur.seqPos == kNoPos && g.nodeId[cur.node] == forwardId && g.nodeOff[cur.node] + cur.off == s.nodeOffset);
+				score += editTerm(nullptr, t);
+				if (nb > 0)
+				{
+					const TItem& x = out[nb - 1];
+					check(x.node == t.node && x.nodeOffset == t.nodeOffset && x.seqPos == t.seqPos);
+				}
 			}
-			out[base + i] = t;
-			prev = t;
+			out[last - k] = t;
+			prevT = t;
 			prevRaw = cur;
 		}
 	}
@@ -811,7 +894,7 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 					const gad::DevAlignment& da = rlo.alns[rlo.alnOffset[k] + a];
 					const uint64_t* raw = rlo.pool.data() + rlo.poolOffset[k] + da.poolOffset;
 					const SeedHit& s = seeds[i][da.seedIndex];
-					Alignment item = buildAlignment(g, opt, seq, L, s, raw, da.nb, raw + da.nb, da.nf);
+					Alignment item = buildAlignment(g, opt, seq, L, s, PackedReader{raw}, da.nb, PackedReader{raw + da.nb}, da.nf);
 					if (item.alignmentFailed()) continue;
 					item.seedGoodness = s.seedGoodness;
 					al.push_back(std::move(item));
@@ -1237,7 +1320,6 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 					const uint32_t r = extending[a];
 					const char* seq = bseqs[r];
 					const uint64_t L = blens[r];
-					thread_local std::vector<uint64_t> items[2];
 					size_t si = 0;
 					try
 					{
@@ -1249,7 +1331,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 							const Spec& sp = st.specs[si];
 							const SeedHit s = st.seeds[sp.seed];
 							st.next = sp.seed + 1;
-							const uint64_t* raws[2] = {nullptr, nullptr};
+							int64_t jobOf[2] = {-1, -1};
 							uint32_t counts[2] = {0, 0};
 							for (int dir = 0; dir < 2; dir++)
 							{
@@ -1265,11 +1347,10 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 								check(res.status != gad::EXT_ASSERT);
 								if (res.status != gad::EXT_OK) continue;
 								check(res.ntrace > 0);
-								eb.items(g, j, items[dir]);
-								raws[dir] = items[dir].data();
+								jobOf[dir] = j;
 								counts[dir] = res.ntrace;
 							}
-							Alignment item = buildAlignment(g, opt, seq, L, s, raws[0], counts[0], raws[1], counts[1]);
+							Alignment item = buildAlignment(g, opt, seq, L, s, CodeReader(g, eb, jobOf[0] < 0 ? 0 : jobOf[0]), counts[0], CodeReader(g, eb, jobOf[1] < 0 ? 0 : jobOf[1]), counts[1]);
 							if (!item.alignmentFailed())
 							{
 								item.seedGoodness = s.seedGoodness;

@@ -395,6 +395,7 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& 
 		stats->pickMs = st.pickMs;
 		stats->replayMs = st.replayMs;
 		stats->gafMs = st.gafMs;
+		stats->gafBytes = (int64_t)at.back();
 	}
 	return GA_OK;
 }

@@ -94,11 +94,15 @@ struct Eq4
 {
 	uint64_t m[4];
 };
+// mask of the base with 2-bit code (A0 C1 G2 T3), by bit arithmetic: a select
+// between array elements may be compiled into an indexed load from memory
 GA_HD uint64_t eqSelect(const Eq4& e, uint32_t code)
 {
-	uint64_t lo = (code & 1) ? e.m[1] : e.m[0];
-	uint64_t hi = (code & 1) ? e.m[3] : e.m[2];
-	return (code & 2) ? hi : lo;
+	const uint64_t b0 = (uint64_t)0 - (uint64_t)(code & 1);
+	const uint64_t b1 = (uint64_t)0 - (uint64_t)((code >> 1) & 1);
+	const uint64_t lo = e.m[0] ^ ((e.m[0] ^ e.m[1]) & b0);
+	const uint64_t hi = e.m[2] ^ ((e.m[2] ^ e.m[3]) & b0);
+	return lo ^ ((lo ^ hi) & b1);
 }
 
 GA_HD int32_t popc(uint64_t x) { return __builtin_popcountll(x); }

@@ -154,6 +154,7 @@ typedef struct
 	double pickMs;         /* picking each round's seeds, */
 	double replayMs;       /* replaying the seed loops on the results (trace fix-ups, alignments), */
 	double gafMs;          /* selection and GAF text */
+	int64_t gafBytes;      /* length of the returned GAF text (without its NUL) */
 } ga_align_stats_t;
 
 /* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed
