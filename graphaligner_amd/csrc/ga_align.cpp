@@ -170,6 +170,16 @@ double msSince(std::chrono::steady_clock::time_point t0)
 	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Clears an outcome for a new alignment, keeping its text buffer: callers that
+// align batch after batch reuse the buffers instead of releasing hundreds of MB
+// of text per batch.
+void resetOutcome(ReadOutcome& o)
+{
+	o.status = READ_NO_ALIGNMENT;
+	o.gaf.clear();
+	o.seedsExtended = 0;
+}
+
 // One device trace item (packTrace): split node, offset in it, read position
 // (kNoPos for the seed column, stored as -1) and the node-switch flag.
 struct RawStep
@@ -346,12 +356,15 @@ void addCigar(std::string& s, uint64_t n, int type)
 }
 
 // GAF line of one alignment (src/GraphAlignerGAFAlignment.h:37-250).
-std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, const Alignment& a, bool merge)
+// Appends the GAF line (without its newline) to s.
+void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_t readLen, const Alignment& a, bool merge)
 {
 	enum { Match = 0, Mismatch = 1, MatchOrMismatch = 2, Insertion = 3, Deletion = 4, Empty = 5 };
 	const auto& t = a.trace;
-	std::string cigar, path;
-	cigar.reserve(t.size() / 2 + 16);
+	// scratch kept per thread: the line's parts are built, then appended
+	thread_local std::string cigar, path;
+	cigar.clear();
+	path.clear();
 	uint64_t pathLen = 0, matches = 0, mismatches = 0, deletions = 0, insertions = 0, editLength = 0;
 	int curId = (int)(int64_t)t[0].node;
 	bool curRev = (t[0].node % 2) == 1;
@@ -410,8 +423,7 @@ std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, cons
 	addCigar(cigar, editLength, currentEdit);
 	uint64_t pathEnd = pathLen - (g.originalSize.at((int)(int64_t)t.back().node) - 1 - t.back().nodeOffset);
 	double total = (double)(matches + mismatches + deletions + insertions);
-	std::string s;
-	s.reserve(cigar.size() + path.size() + std::strlen(name) + 192);
+	s.reserve(s.size() + cigar.size() + path.size() + std::strlen(name) + 192);
 	s += name;
 	s += '\t';
 	appendUint(s, readLen);
@@ -443,7 +455,6 @@ std::string gafLine(const GraphHost& g, const char* name, uint64_t readLen, cons
 	s += f.str();
 	s += "\tcg:Z:";
 	s += cigar;
-	return s;
 }
 
 // One seed of a round with its device jobs (-1: none).
@@ -769,7 +780,7 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 	stats = AlignStats{};
 	for (uint32_t i = 0; i < count; i++)
 	{
-		out[i] = ReadOutcome{};
+		resetOutcome(out[i]);
 		if (rs.length[first + i] > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
 	}
 	gad::ReadLoopParams rp;
@@ -909,13 +920,12 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 				}
 				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
 				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentXScore > r.alignmentXScore; });
-				std::string text;
+				o.gaf.clear();
 				for (const auto& a : sel)
 				{
-					text += gafLine(g, names[batchStart + i], L, a, opt.cigarMatchMismatchMerge);
-					text += '\n';
+					appendGafLine(o.gaf, g, names[batchStart + i], L, a, opt.cigarMatchMismatchMerge);
+					o.gaf += '\n';
 				}
-				o.gaf = std::move(text);
 				o.status = READ_ALIGNED;
 			}
 			catch (const ReadAssert&)
@@ -1054,7 +1064,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 	stats = AlignStats{};
 	for (uint32_t i = 0; i < count; i++)
 	{
-		out[i] = ReadOutcome{};
+		resetOutcome(out[i]);
 		if (rs.length[first + i] > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
 	}
 	for (uint32_t batchStart = 0; batchStart < count; batchStart += opt.readsPerBatch)
@@ -1126,13 +1136,12 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				}
 				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
 				std::sort(sel.begin(), sel.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentXScore > r.alignmentXScore; });
-				std::string text;
+				o.gaf.clear();
 				for (const auto& a : sel)
 				{
-					text += gafLine(g, names[batchStart + i], blens[i], a, opt.cigarMatchMismatchMerge);
-					text += '\n';
+					appendGafLine(o.gaf, g, names[batchStart + i], blens[i], a, opt.cigarMatchMismatchMerge);
+					o.gaf += '\n';
 				}
-				o.gaf = std::move(text);
 				o.status = READ_ALIGNED;
 			}
 			catch (const ReadAssert&)
@@ -1454,6 +1463,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		{ double ms = msSince(th); stats.hostMs += ms; stats.gafMs += ms; }
 	}
 	stats.totalMs = msSince(t0);
+	if (roundLog) std::fprintf(stderr, "[rounds] done at %.1f ms\n", stats.totalMs);
 }
 
 void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
@@ -1468,7 +1478,7 @@ void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& id
 	for (uint32_t l : lens)
 		if (l > (1u << 24)) throw std::runtime_error("reads longer than 16 Mbp are not supported");
 	ctx.uploadReads(seqs, lens);
-	out.assign(seqs.size(), ReadOutcome{});
+	out.resize(seqs.size());
 	alignResident(ctx, g, idx, opt, names.data(), 0, (uint32_t)seqs.size(), out.data(), stats);
 	stats.totalMs = msSince(t0);
 }

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <chrono>
 #include <thread>
 #include <cstdlib>
 #include <memory>
@@ -355,10 +357,11 @@ AlignOptions alignOptionsOf(const ga_params_t& p)
 	return opt;
 }
 
-ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& st, char** gaf, int32_t* status, ga_align_stats_t* stats)
+// The first n outcomes as one NUL-terminated buffer from malloc, plus statuses.
+ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, size_t n, const AlignStats& st, char** gaf, int32_t* status, ga_align_stats_t* stats)
 {
-	std::vector<size_t> at(out.size() + 1, 0);
-	for (size_t i = 0; i < out.size(); i++)
+	std::vector<size_t> at(n + 1, 0);
+	for (size_t i = 0; i < n; i++)
 	{
 		at[i + 1] = at[i] + out[i].gaf.size();
 		status[i] = out[i].status;
@@ -371,7 +374,7 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, const AlignStats& 
 	std::vector<std::thread> pool;
 	for (size_t t = 0; t < parts; t++)
 		pool.emplace_back([&, t]() {
-			for (size_t i = t; i < out.size(); i += parts) std::memcpy(buf + at[i], out[i].gaf.data(), out[i].gaf.size());
+			for (size_t i = t; i < n; i += parts) std::memcpy(buf + at[i], out[i].gaf.data(), out[i].gaf.size());
 		});
 	for (auto& th : pool) th.join();
 	buf[at.back()] = 0;
@@ -435,10 +438,10 @@ ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_
 		std::vector<const char*> nm(names, names + n), s(seqs, seqs + n);
 		std::vector<uint32_t> l(n);
 		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
-		std::vector<ReadOutcome> out;
+		std::vector<ReadOutcome>& out = ctx->outcomes;
 		AlignStats st;
 		alignReads(*ctx->ctx, graph->host, index->idx, alignOptionsOf(*params), nm, s, l, out, st);
-		return packOutcomes(out, st, gaf, status, stats);
+		return packOutcomes(out, (size_t)n, st, gaf, status, stats);
 	});
 }
 
@@ -472,10 +475,17 @@ ga_status_t ga_align_resident(ga_context_t* ctx, const ga_graph_t* graph, const 
 		if (!names[i]) return GA_EBADARG;
 	*gaf = nullptr;
 	return guarded([&]() {
-		std::vector<ReadOutcome> out(count);
+		const bool log = std::getenv("GA_ROUND_LOG") != nullptr;
+		auto t0 = std::chrono::steady_clock::now();
+		auto since = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+		std::vector<ReadOutcome>& out = ctx->outcomes;
+		if (out.size() < (size_t)count) out.resize(count);
 		AlignStats st;
 		alignResident(*ctx->ctx, graph->host, index->idx, alignOptionsOf(*params), names, (uint32_t)first, (uint32_t)count, out.data(), st);
-		return packOutcomes(out, st, gaf, status, stats);
+		if (log) std::fprintf(stderr, "[capi] aligned at %.1f ms\n", since());
+		ga_status_t rc = packOutcomes(out, (size_t)count, st, gaf, status, stats);
+		if (log) std::fprintf(stderr, "[capi] packed at %.1f ms\n", since());
+		return rc;
 	});
 }
 

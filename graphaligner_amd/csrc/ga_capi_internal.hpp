@@ -2,6 +2,7 @@
 #pragma once
 #include <memory>
 #include "../../include/graphaligner_amd.h"
+#include "ga_align.hpp"
 #include "ga_graph.hpp"
 #include "ga_index.hpp"
 #include "ga_runtime.hpp"
@@ -20,6 +21,8 @@ struct ga_index
 struct ga_context
 {
 	std::unique_ptr<ga::Context> ctx;
+	// per-read outcomes of the last call, kept so their text buffers are reused
+	std::vector<ga::ReadOutcome> outcomes;
 };
 
 namespace ga {
