@@ -179,11 +179,56 @@ namespace gad {
 
 // One thread per read (reads are independent and their lookups are short,
 // irregular hash probes; the table stays in L2/HBM).
-__global__ void __launch_bounds__(64) seedLookupKernel(const char* __restrict__ reads, const uint64_t* __restrict__ readOff, const uint32_t* __restrict__ readLen, uint32_t nreads, const SeedTableSlot* __restrict__ table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* __restrict__ out, uint32_t* __restrict__ outCount)
+// One workgroup of 256 threads per read. Positions go in chunks of 256, one per
+// thread: each thread decides whether iterateKmers reports its k-mer
+// (kmerReported), looks it up, and the workgroup writes the chunk's matches in
+// read order (ballot prefix counts within each wavefront, wavefront totals
+// through LDS), as lookupReadKmers writes them one by one.
+__global__ void __launch_bounds__(256) seedLookupKernel(const char* __restrict__ reads, const uint64_t* __restrict__ readOff, const uint32_t* __restrict__ readLen, uint32_t nreads, const SeedTableSlot* __restrict__ table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* __restrict__ out, uint32_t* __restrict__ outCount)
 {
-	uint32_t r = blockIdx.x * 64 + threadIdx.x;
+	__shared__ uint32_t waveHits[4];
+	__shared__ uint32_t written;
+	const uint32_t r = blockIdx.x;
 	if (r >= nreads) return;
-	outCount[r] = lookupReadKmers(reads + readOff[r], readLen[r], table, mask, k, w, maxCount, out + readOff[r]);
+	const char* s = reads + readOff[r];
+	const uint64_t len = readLen[r];
+	SeedMatch* o = out + readOff[r];
+	const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	if (threadIdx.x == 0) written = 0;
+	__syncthreads();
+	for (uint64_t c = 0; c < len; c += 256)
+	{
+		const uint64_t i = c + threadIdx.x;
+		bool hit = false;
+		SeedMatch m;
+		m.pos = m.start = m.count = 0;
+		if (i < len)
+		{
+			uint64_t kmer;
+			if (kmerReported(s, i, k, w, kmer))
+			{
+				const SeedTableSlot* t = seedTableFind(table, mask, kmer);
+				if (t && t->count < maxCount)
+				{
+					hit = true;
+					m.pos = (uint32_t)i;
+					m.start = t->start;
+					m.count = t->count;
+				}
+			}
+		}
+		const uint64_t ballot = __ballot(hit);
+		const uint32_t before = (uint32_t)__popcll(ballot & (((uint64_t)1 << lane) - 1));
+		if (lane == 0) waveHits[wave] = (uint32_t)__popcll(ballot);
+		__syncthreads();
+		uint32_t at = written;
+		for (uint32_t v = 0; v < wave; v++) at += waveHits[v];
+		if (hit) o[at + before] = m;
+		__syncthreads();
+		if (threadIdx.x == 0) written += waveHits[0] + waveHits[1] + waveHits[2] + waveHits[3];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) outCount[r] = written;
 }
 
 // Reverse complement of every read, one block per read.
@@ -199,7 +244,7 @@ __global__ void revcompKernel(const char* __restrict__ reads, char* __restrict__
 
 hipError_t launchSeedLookup(hipStream_t stream, const char* reads, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, const SeedTableSlot* table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* out, uint32_t* outCount)
 {
-	hipLaunchKernelGGL(seedLookupKernel, dim3((nreads + 63) / 64), dim3(64), 0, stream, reads, readOff, readLen, nreads, table, mask, k, w, maxCount, out, outCount);
+	hipLaunchKernelGGL(seedLookupKernel, dim3(nreads), dim3(256), 0, stream, reads, readOff, readLen, nreads, table, mask, k, w, maxCount, out, outCount);
 	return hipGetLastError();
 }
 

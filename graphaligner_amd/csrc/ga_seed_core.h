@@ -37,81 +37,64 @@ GA_SEED_HD uint64_t seedKmerHash(uint64_t key)
 	return key;
 }
 
-// Looks up every read k-mer that the seeder's iterateKmers reports
-// (src/MinimizerSeeder.cpp:60-102: k-mers are restarted after a non-ACGT letter,
-// and a repeat of the previous k-mer is reported again only once it lies a full
-// window of w - k + 1 positions later). K-mers present in the table with fewer
-// than maxCount positions are written to out in read order; returns their number.
+// Whether the seeder's iterateKmers (src/MinimizerSeeder.cpp:60-102) reports the
+// k-mer ending at read position i, and that k-mer. iterateKmers walks each run of
+// ACGT letters (it restarts after any other letter) and reports a k-mer unless
+// it equals the last reported one and lies less than a window of w - k + 1
+// positions after it. Consecutive k-mers are equal only inside a run of one base
+// longer than k, so the rule is a function of the position: k-mers are reported
+// where they differ from the previous one, and inside such a run at its first
+// k-mer and every w - k + 1 positions after it. Each position can thus be
+// decided on its own, as the block-parallel kernel does.
+GA_SEED_HD bool kmerReported(const char* s, uint64_t i, uint32_t k, uint32_t w, uint64_t& kmer)
+{
+	if (i + 1 < k) return false;
+	kmer = 0;
+	for (uint64_t p = i + 1 - k; p <= i; p++)
+	{
+		const int c = readBaseCode((unsigned char)s[p]);
+		if (c < 0) return false;
+		kmer = (kmer << 2) | (uint64_t)c;
+	}
+	// length of the run of one base ending at i
+	const int c = readBaseCode((unsigned char)s[i]);
+	uint64_t h = 1;
+	while (h <= i && readBaseCode((unsigned char)s[i - h]) == c) h++;
+	if (h <= k) return true;
+	const uint64_t realWindow = w - k + 1;
+	return (h - k) % realWindow == 0;
+}
+
+// Table slot of kmer (open addressing in seedKmerHash order), or null.
+GA_SEED_HD const SeedTableSlot* seedTableFind(const SeedTableSlot* table, uint64_t mask, uint64_t kmer)
+{
+	uint64_t slot = seedKmerHash(kmer) & mask;
+	while (true)
+	{
+		const uint64_t key = table[slot].kmer;
+		if (key == kmer) return &table[slot];
+		if (key == ~0ull) return nullptr;
+		slot = (slot + 1) & mask;
+	}
+}
+
+// Looks up every read k-mer that iterateKmers reports; k-mers present in the
+// table with fewer than maxCount positions are written to out in read order.
+// Returns their number.
 GA_SEED_HD uint32_t lookupReadKmers(const char* s, uint64_t len, const SeedTableSlot* table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* out)
 {
 	uint32_t n = 0;
-	if (len < k) return 0;
-	const uint64_t realWindow = w - k + 1;
-	const uint64_t kmask = ~(~0ull << (2 * k));
-	uint64_t offset = 0;
-	while (true)
+	for (uint64_t i = 0; i < len; i++)
 	{
-		while (offset < len && readBaseCode((unsigned char)s[offset]) < 0) offset++;
-		if (offset + k > len) break;
-		uint64_t kmer = 0;
-		uint64_t bad = len;
-		for (uint64_t i = 0; i < k; i++)
-		{
-			int c = readBaseCode((unsigned char)s[offset + i]);
-			if (c < 0)
-			{
-				bad = i;
-				break;
-			}
-			kmer = (kmer << 2) | (uint64_t)c;
-		}
-		if (bad != len)
-		{
-			offset += bad;
-			continue;
-		}
-		uint64_t lastKmer = 0, lastPos = 0;
-		bool restart = false;
-		for (uint64_t i = k - 1; offset + i < len; i++)
-		{
-			if (i >= k)
-			{
-				int c = readBaseCode((unsigned char)s[offset + i]);
-				if (c < 0)
-				{
-					offset += i;
-					restart = true;
-					break;
-				}
-				kmer = ((kmer << 2) & kmask) | (uint64_t)c;
-			}
-			if (i == k - 1 || lastKmer != kmer || lastPos <= offset + i - realWindow)
-			{
-				lastKmer = kmer;
-				lastPos = offset + i;
-				uint64_t slot = seedKmerHash(kmer) & mask;
-				while (true)
-				{
-					uint64_t key = table[slot].kmer;
-					if (key == kmer)
-					{
-						uint32_t c = table[slot].count;
-						if (c < maxCount)
-						{
-							SeedMatch m;
-							m.pos = (uint32_t)(offset + i);
-							m.start = table[slot].start;
-							m.count = c;
-							out[n++] = m;
-						}
-						break;
-					}
-					if (key == ~0ull) break;
-					slot = (slot + 1) & mask;
-				}
-			}
-		}
-		if (!restart) break;
+		uint64_t kmer;
+		if (!kmerReported(s, i, k, w, kmer)) continue;
+		const SeedTableSlot* t = seedTableFind(table, mask, kmer);
+		if (!t || t->count >= maxCount) continue;
+		SeedMatch m;
+		m.pos = (uint32_t)i;
+		m.start = t->start;
+		m.count = t->count;
+		out[n++] = m;
 	}
 	return n;
 }

@@ -162,8 +162,8 @@ class Index:
     def max_count(self):
         return lib().orc_index_maxcount(self.h)
 
-    def seeds(self, seq, density, order=False):
-        cap = max(1024, 64 * len(seq))
+    def seeds(self, seq, density, order=False, cap_per_base=64):
+        cap = max(1024, cap_per_base * len(seq))
         out = np.zeros(10 * cap, dtype=np.int64)
         n = ctypes.c_int64()
         rc = lib().orc_get_seeds(self.graph.h, self.h, seq.encode(), len(seq), density, int(order),

@@ -112,3 +112,16 @@ def test_gpu_gaf_all_seeds_cluster2(ctx, workdir, wl, preset, mode):
     gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
     assert list(gstat) == list(ostat)
     assert gt == ot
+
+
+@pytest.mark.parametrize("preset", ["vg", "dbg"])
+def test_gpu_seeds_of_single_base_runs(ctx, workdir, preset):
+    """The block-parallel lookup's per-position repeat rule on single-base runs."""
+    from test_pipeline_parity import _homopolymer_workload
+    gfa, reads = _homopolymer_workload(workdir)
+    sd, og, oi, gp, g, gi = _both(gfa, preset)
+    print(f"[gpu-align] single-base-run seeds {preset} reads={len(reads)}", flush=True)
+    got = ga.seeds(ctx, g, gi, reads, sd["density"], order=True)
+    for s, b in zip(reads, got):
+        a = oi.seeds(s, sd["density"], True, cap_per_base=512)
+        assert a.shape == b.shape and (a == b).all()

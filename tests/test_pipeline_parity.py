@@ -98,3 +98,55 @@ def test_gaf_identical_all_seeds_cluster2(workdir, wl, preset, mode):
     pt, pstat, _ = P.align_reads(pi, params, sd["density"], names, seqs, **P.MODES[mode])
     assert list(pstat) == list(ostat)
     assert pt == ot
+
+
+def _homopolymer_workload(workdir):
+    """A small graph whose sequence holds single-base runs of 12-60 bp, and reads
+    with runs of every length around k and the window (w - k + 1), lowercase and
+    N-interrupted runs: the cases of iterateKmers' repeat rule."""
+    import random
+    rng = random.Random(11)
+    parts = []
+    for r in range(40):
+        parts.append("".join(rng.choice("ACGT") for _ in range(rng.randrange(30, 90))))
+        if r % 5 == 0:
+            parts.append(rng.choice("ACGT") * rng.randrange(12, 60))
+    genome = "".join(parts)
+    path = os.path.join(workdir, "homopolymer.gfa")
+    chunks = [genome[i:i + 700] for i in range(0, len(genome), 700)]
+    with open(path, "w") as f:
+        f.write("H\tVN:Z:1.0\n")
+        for i, c in enumerate(chunks):
+            f.write(f"S\t{i + 1}\t{c}\n")
+        for i in range(len(chunks) - 1):
+            f.write(f"L\t{i + 1}\t+\t{i + 2}\t+\t0M\n")
+    reads = []
+    for run in list(range(10, 40)) + [45, 50, 57, 64, 80, 101, 150]:
+        for base in "ACGT":
+            left = "".join(rng.choice("ACGT") for _ in range(40))
+            right = "".join(rng.choice("ACGT") for _ in range(40))
+            reads.append(left + base * run + right)
+    for run in (30, 60, 90):
+        reads.append("GATTACA" * 5 + "a" * run + "CCGTA" * 6)
+        reads.append("TTGCA" * 6 + "C" * (run // 2) + "N" + "C" * (run // 2) + "AGGT" * 8)
+        reads.append("ACGT" * 5 + "G" * run + "n" + "g" * run + "TACG" * 5)
+    # pieces of the genome itself, runs included
+    for _ in range(40):
+        a = rng.randrange(0, len(genome) - 400)
+        reads.append(genome[a:a + rng.randrange(100, 400)])
+    return path, reads
+
+
+@pytest.mark.parametrize("preset", ["vg", "dbg"])
+def test_seeds_of_single_base_runs(workdir, preset):
+    gfa, reads = _homopolymer_workload(workdir)
+    sd, og, oi, pg, pi = _pair(gfa, preset)
+    nonempty = 0
+    for s in reads:
+        for order in (False, True):
+            a = oi.seeds(s, sd["density"], order, cap_per_base=512)
+            b = pi.seeds(s, sd["density"], order)
+            assert a.shape == b.shape
+            assert (a == b).all()
+            nonempty += len(a) > 0
+    assert nonempty > len(reads)
