@@ -5,9 +5,11 @@ Workload: the 5 Mbp synthetic variation graph (SNP and indel bubbles, one
 structural-variant bubble of 50-5,000 bp per 20 kbp: insertions, deletions,
 inversions) with 20,000 reads of log-normal length (median 12 kbp, clipped to
 1-60 kbp) at 12% error, aligned with preset -x vg. One step aligns every read:
-device seed lookup, seed ordering, the seed-extension loop with each round's
-extensions batched into one gfx950 launch, selection and GAF text. Reads are
-resident in HBM before the timed region (ga_reads_upload).
+device seed lookup, seed ordering, the seed-extension loops (two read groups on
+two streams, each round's extensions of a group batched into one gfx950 launch,
+later rounds extending several seeds per read ahead and replaying the loop),
+selection and GAF text. Reads are resident in HBM before the timed region
+(ga_reads_upload).
 
 Prints one JSON line (reads/s, GCUPS, roofline of the extension kernel, and the
 CPU baseline: the C++ CPU restatement of the reference run on a sample of the
@@ -31,11 +33,10 @@ VALU_INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 # band item state amortised over the node's columns)
 OPS_PER_WORD_STEP = 42
 BYTES_PER_WORD_STEP = 6
-# HBM bytes per extension launch of the default run of this workload, from
-# rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy kernel's readings
-# (profiles/extend_vg5m.md); None when the default path changes and is not yet
-# re-measured
-MEASURED_TRAFFIC_PER_LAUNCH = 2.162e9
+# HBM bytes per extension launch (mean) of the default run of this workload,
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy kernel's
+# readings (profiles/extend_vg5m.md)
+MEASURED_TRAFFIC_PER_LAUNCH = 8.65e9
 
 
 def parse():
@@ -170,7 +171,7 @@ def main():
                                  "align_call": tot["totalMs"] / 1e3 / steps},
         "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "unusedExtensions", "wordSteps", "seeds", "reruns")},
         "read_status_counts": counts,
-        "roofline": {"kernel": "extendThreadKernel", "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
+        "roofline": {"kernel": "extendSingleLaneKernel", "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
                      "traffic": None if args.device_seed_loop or args.reads else MEASURED_TRAFFIC_PER_LAUNCH,
                      "mean_launch_ms": tot["extendKernelMs"] / launches,
