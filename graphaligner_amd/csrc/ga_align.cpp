@@ -60,32 +60,30 @@ int baseSet(char c)
 	return -1;
 }
 
-inline bool graphHasBase(char g, int base)
+// baseSet of every byte, for the per-item checks of the trace rebuild and GAF
+struct BaseSetTable
 {
-	int s = baseSet(g);
-	check(s >= 0);
-	return (s & base) != 0;
-}
+	int8_t v[256];
+	BaseSetTable()
+	{
+		for (int c = 0; c < 256; c++) v[c] = (int8_t)baseSet((char)c);
+	}
+};
+const BaseSetTable kBaseSets;
 
 // src/GraphAlignerCommon.h:176-279 (characterMatch / ambiguousMatch): an exact
 // read base matches any graph letter standing for it; otherwise the first base
-// (in A, C, G, T order) both letters stand for decides.
-bool characterMatch(char s, char g)
+// (in A, C, G, T order) both letters stand for decides, so the letters match when
+// their base sets meet. A letter outside the tables fails the reference's check.
+inline bool characterMatch(char s, char g)
 {
 	if (s == g) return true;
-	switch (s)
-	{
-		case 'a': case 'A': return graphHasBase(g, 1);
-		case 'c': case 'C': return graphHasBase(g, 2);
-		case 'g': case 'G': return graphHasBase(g, 4);
-		case 't': case 'T': return graphHasBase(g, 8);
-		case '-': return false;
-	}
-	int ss = baseSet(s);
+	if (s == '-') return false;
+	const int ss = kBaseSets.v[(unsigned char)s];
 	check(ss >= 0);
-	for (int b = 1; b <= 8; b <<= 1)
-		if ((ss & b) && graphHasBase(g, b)) return true;
-	return false;
+	const int gs = kBaseSets.v[(unsigned char)g];
+	check(gs >= 0);
+	return (ss & gs) != 0;
 }
 
 // CommonUtils::Complement (src/CommonUtils.cpp:67-97); 0 for letters outside the table.
@@ -666,6 +664,12 @@ struct NodeView
 		if (amb) return g.nodeChar(split, off);
 		return "ACGT"[((off < 32 ? w0 : w1) >> ((off & 31) * 2)) & 3];
 	}
+	// complementOf(base(g, off)); 0 for a letter without a complement
+	char complementBase(const GraphHost& g, uint32_t off) const
+	{
+		if (amb) return complementOf(g.nodeChar(split, off));
+		return "TGCA"[((off < 32 ? w0 : w1) >> ((off & 31) * 2)) & 3];
+	}
 };
 
 inline void verifyFast(const GraphHost& g, const RawStep& o, const RawStep& n)
@@ -714,7 +718,7 @@ Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char
 			t.nodeSwitch = k + 1 < nb ? next.sw : false;
 			check(t.seqPos < L);
 			t.seqChar = seq[t.seqPos];
-			const char c = complementOf(view.base(g, cur.off));
+			const char c = view.complementBase(g, cur.off);
 			check(c != 0);
 			t.graphChar = c;
 			score += editTerm(k == 0 ? nullptr : &out[k - 1], t);
@@ -1164,7 +1168,22 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return blens[a] > blens[b]; });
 		const uint32_t groups = std::max<uint32_t>(1, std::min<uint32_t>({ctx.extendLanes(), opt.pipelineGroups, (uint32_t)std::max<size_t>(1, order.size())}));
 		std::vector<std::vector<uint32_t>> groupReads(groups);
-		for (size_t k = 0; k < order.size(); k++) groupReads[k % groups].push_back(order[k]);
+		if (opt.groupByLength)
+		{
+			// contiguous length classes with equal shares of the bases: the longest
+			// reads' group has few reads and long launches, the others many reads
+			// and short launches, so the groups fall out of step by themselves
+			uint64_t total = 0, acc = 0;
+			for (uint32_t r : order) total += blens[r];
+			for (uint32_t r : order)
+			{
+				const uint32_t grp = (uint32_t)std::min<uint64_t>(groups - 1, acc * groups / std::max<uint64_t>(1, total));
+				groupReads[grp].push_back(r);
+				acc += blens[r];
+			}
+		}
+		else
+			for (size_t k = 0; k < order.size(); k++) groupReads[k % groups].push_back(order[k]);
 		for (auto& gr : groupReads) std::sort(gr.begin(), gr.end());
 		std::vector<AlignStats> groupStats(groups);
 		std::vector<std::exception_ptr> groupErrors(groups);
@@ -1306,7 +1325,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				}
 				{ double ms = msSince(th); gs.hostMs += ms; gs.pickMs += ms; }
 				if (extending.empty()) break;
-				if (firstLaunch.pending && lane > 0)
+				if (firstLaunch.pending && lane > 0 && !opt.groupByLength)
 				{
 					std::unique_lock<std::mutex> lock(phaseMutex);
 					phaseCv.wait(lock, [&]() { return firstLaunchesDone >= 1; });

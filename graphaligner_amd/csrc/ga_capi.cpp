@@ -354,6 +354,7 @@ AlignOptions alignOptionsOf(const ga_params_t& p)
 		int x = std::atoi(v);
 		if (x >= 1 && x <= 64) opt.pipelineGroups = (uint32_t)x;
 	}
+	if (const char* v = std::getenv("GA_GROUP_BY_LENGTH")) opt.groupByLength = std::atoi(v) != 0;
 	return opt;
 }
 
