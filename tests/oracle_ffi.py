@@ -9,7 +9,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(os.path.dirname(_HERE), "oracle", "_build", "libga_oracle.so")
+_BUILD = os.path.join(os.path.dirname(_HERE), "oracle", "_build")
+# the reference's assertions on (as it ships) or compiled out (-DNDEBUG, its
+# faster variant: used only to time the CPU baseline)
+_LIB_PATHS = {False: os.path.join(_BUILD, "libga_oracle.so"), True: os.path.join(_BUILD, "libga_oracle_ndebug.so")}
 
 
 class OrcParams(ctypes.Structure):
@@ -40,15 +43,15 @@ class SliceSummary(ctypes.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(ndebug=False):
+    if ndebug not in _libs:
+        path = _LIB_PATHS[ndebug]
+        if not os.path.exists(path):
             raise RuntimeError("oracle library not built: run make -C oracle")
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         i64p = ctypes.POINTER(ctypes.c_int64)
         L.orc_graph_load.restype = vp
@@ -87,8 +90,8 @@ def lib():
                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.orc_slice_summary_size.restype = ctypes.c_int64
         assert L.orc_slice_summary_size() == ctypes.sizeof(SliceSummary)
-        _lib = L
-    return _lib
+        _libs[ndebug] = L
+    return _libs[ndebug]
 
 
 def preset_params(preset="vg"):
@@ -117,19 +120,20 @@ PRESET_SEEDING = {"vg": dict(k=15, w=20, density=10.0, discard=0.001),
 
 
 class Graph:
-    def __init__(self, path):
+    def __init__(self, path, ndebug=False):
+        self.L = lib(ndebug)
         err = ctypes.create_string_buffer(512)
-        self.h = lib().orc_graph_load(path.encode(), err, 512)
+        self.h = self.L.orc_graph_load(path.encode(), err, 512)
         if not self.h:
             raise RuntimeError("oracle graph load failed: " + err.value.decode())
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_graph_free(self.h)
+            self.L.orc_graph_free(self.h)
             self.h = None
 
     def export(self):
-        L = lib()
+        L = self.L
         n = L.orc_graph_num_nodes(self.h)
         e = L.orc_graph_num_edges(self.h)
         a = {k: np.zeros(n, dtype=np.int64) for k in ["nodeLength", "nodeIDs", "nodeOffset", "comp", "chain", "chainPos"]}
@@ -151,22 +155,23 @@ class Graph:
 class Index:
     def __init__(self, graph, k, w, discard):
         self.graph = graph
-        self.h = lib().orc_index_build(graph.h, k, w, discard)
+        self.L = graph.L
+        self.h = self.L.orc_index_build(graph.h, k, w, discard)
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_index_free(self.h)
+            self.L.orc_index_free(self.h)
             self.h = None
 
     @property
     def max_count(self):
-        return lib().orc_index_maxcount(self.h)
+        return self.L.orc_index_maxcount(self.h)
 
     def seeds(self, seq, density, order=False, cap_per_base=64):
         cap = max(1024, cap_per_base * len(seq))
         out = np.zeros(10 * cap, dtype=np.int64)
         n = ctypes.c_int64()
-        rc = lib().orc_get_seeds(self.graph.h, self.h, seq.encode(), len(seq), density, int(order),
+        rc = self.L.orc_get_seeds(self.graph.h, self.h, seq.encode(), len(seq), density, int(order),
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(n))
         assert rc == 0
         return out[: 10 * n.value].reshape(-1, 10)
@@ -181,7 +186,7 @@ def extend(graph, params, seq, bigraph_node, offset):
     scap = len(seq) // 64 + 4
     sums = (SliceSummary * scap)()
     ns = ctypes.c_int64()
-    rc = lib().orc_extend(graph.h, ctypes.byref(params), seq.encode(), len(seq), bigraph_node, offset,
+    rc = graph.L.orc_extend(graph.h, ctypes.byref(params), seq.encode(), len(seq), bigraph_node, offset,
                           trace.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(nt), ctypes.byref(score),
                           sums, scap, ctypes.byref(ns))
     rows = [(s.j, s.minScore, s.minScoreNode, s.minScoreNodeOffset, s.maxExactEndposScore, s.maxExactEndposNode,
@@ -198,9 +203,9 @@ def align_reads(graph, index, params, density, names, seqs, nthreads=1):
     lens = np.array([len(s) for s in seqs], dtype=np.int64)
     out = ctypes.c_void_p()
     status = np.zeros(n, dtype=np.int32)
-    secs = lib().orc_align_reads(graph.h, index.h, ctypes.byref(params), density, n, na, sa,
+    secs = graph.L.orc_align_reads(graph.h, index.h, ctypes.byref(params), density, n, na, sa,
                                  lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nthreads, ctypes.byref(out),
                                  status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
     text = ctypes.string_at(out.value).decode()
-    lib().orc_free(out)
+    graph.L.orc_free(out)
     return text, status, secs
