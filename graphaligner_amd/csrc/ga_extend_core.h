@@ -1025,8 +1025,11 @@ struct Extension
 	// --- one node's columns for one slice (calculateNodeInner) ---
 	// Forward mode (Recalc = false) with early leave; recalc mode stores every
 	// column in `cols` and leaves the X-scores out.
+	// xFloor: the slice's best X-score so far. The node's maxX is exact whenever
+	// it exceeds xFloor (the only case where the slice uses it); columns whose
+	// X-scores cannot exceed max(xFloor, column 0's) are not scanned.
 	template <bool Recalc>
-	GA_HD NodeResult calcNode(uint32_t node, Item& it, bool exists, const Eq4& eq, const Item& prev, bool prevExists, uint32_t firstExtra, const Slice& recalcIncoming, int32_t seqOffset, bool prevHasInNeighbor, ColumnStore* cols)
+	GA_HD NodeResult calcNode(uint32_t node, Item& it, bool exists, const Eq4& eq, const Item& prev, bool prevExists, uint32_t firstExtra, const Slice& recalcIncoming, int32_t seqOffset, bool prevHasInNeighbor, ColumnStore* cols, int32_t xFloor = SMIN)
 	{
 		NodeResult res;
 		res.minScore = SMAX;
@@ -1191,6 +1194,9 @@ struct Extension
 		NodeSeq nseqU = nseq;
 		for (int b = 0; b < 4; b++) nseqU.w[b] = uni64(nseq.w[b]);
 		LaneAcc acc;
+		// bound of a column's X-scores: every row scores at least before - popc(VN)
+		const int64_t xTop = ((int64_t)seqOffset + 64) * 100;
+		const int64_t xSkip = (int64_t)imax(xFloor, res.maxX);
 		if (!Recalc)
 		{
 			accInit(acc);
@@ -1212,7 +1218,8 @@ struct Extension
 				res.minScoreNodeOffset = pos;
 			}
 			if (Recalc) colSet(*cols, pos, ws);
-			else accAdd(acc, ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, Mh));
+			else if (xTop - (int64_t)p.errorCost * (before(ws) - popc(ws.VN)) > xSkip)
+				accAdd(acc, ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, Mh));
 			HP |= (Ph >> 63) << pos;
 			HN |= (Mh >> 63) << pos;
 		}
@@ -1437,7 +1444,7 @@ struct Extension
 				Item prevIt = pq >= 0 ? loadItem(pq) : defaultPrev();
 				bool prevHasIn = false;
 				if (exists && (g.inOff[node + 1] - g.inOff[node]) == 1) prevHasIn = findInSlice(kept - 1, g.inAdj[g.inOff[node]]) >= 0;
-				NodeResult nc = calcNode<false>(node, it, exists, eq, prevIt, pq >= 0, slotHead[slot], mkslice(0, 0, 0), j, prevHasIn, nullptr);
+				NodeResult nc = calcNode<false>(node, it, exists, eq, prevIt, pq >= 0, slotHead[slot], mkslice(0, 0, 0), j, prevHasIn, nullptr, rMaxX);
 				nodeCalcs++;
 				if (status != EXT_OK) return;
 				// pop: the node leaves the queue and its queued edges are dropped
