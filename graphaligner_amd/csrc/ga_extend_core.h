@@ -442,13 +442,39 @@ GA_HD uint64_t lpDigest(const Item* items, uint32_t start, uint32_t count)
 	for (uint32_t i = 0; i < count; i++) h += itemHash(items[start + i]);
 	return h;
 }
-// a node's column slices, in the thread's workspace
+// a node's column slices, in the thread's workspace or in LDS
 struct ColumnStore
 {
 	uint64_t* vp;
 	uint64_t* vn;
 	int32_t* end;
+	bool lds;
 };
+#if !defined(GA_THREAD_HOST_TEST)
+// An LDS column store is accessed through LDS-typed pointers. Flat accesses to
+// it would count on the vector-memory counter, so that every column read of the
+// backtrace walk would also wait for all trace stores still in flight.
+typedef __attribute__((address_space(3))) uint64_t LdsU64;
+typedef __attribute__((address_space(3))) int32_t LdsI32;
+GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
+{
+	if (c.lds)
+	{
+		((LdsU64*)c.vp)[col] = s.VP;
+		((LdsU64*)c.vn)[col] = s.VN;
+		((LdsI32*)c.end)[col] = s.end;
+		return;
+	}
+	c.vp[col] = s.VP;
+	c.vn[col] = s.VN;
+	c.end[col] = s.end;
+}
+GA_HD Slice colGet(const ColumnStore& c, int col)
+{
+	if (c.lds) return mkslice(((LdsU64*)c.vp)[col], ((LdsU64*)c.vn)[col], ((LdsI32*)c.end)[col]);
+	return mkslice(c.vp[col], c.vn[col], c.end[col]);
+}
+#else
 GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
 {
 	c.vp[col] = s.VP;
@@ -456,11 +482,13 @@ GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
 	c.end[col] = s.end;
 }
 GA_HD Slice colGet(const ColumnStore& c, int col) { return mkslice(c.vp[col], c.vn[col], c.end[col]); }
-GA_HD void bindColumns(ColumnStore& c, uint64_t* vp, uint64_t* vn, int32_t* end)
+#endif
+GA_HD void bindColumns(ColumnStore& c, uint64_t* vp, uint64_t* vn, int32_t* end, bool lds)
 {
 	c.vp = vp;
 	c.vn = vn;
 	c.end = end;
+	c.lds = lds;
 }
 
 #elif !defined(GA_WAVE_EMULATION)
@@ -651,7 +679,7 @@ GA_HD Slice colGet(const ColumnStore& c, int col)
 	s.end = __shfl(c.end, col);
 	return s;
 }
-GA_HD void bindColumns(ColumnStore&, uint64_t*, uint64_t*, int32_t*) {}
+GA_HD void bindColumns(ColumnStore&, uint64_t*, uint64_t*, int32_t*, bool) {}
 
 #else // GA_WAVE_EMULATION: the same helpers as loops over 64 lanes
 
@@ -771,7 +799,7 @@ struct ColumnStore
 };
 inline void colSet(ColumnStore& c, int col, const Slice& s) { c.c[col] = s; }
 inline Slice colGet(const ColumnStore& c, int col) { return c.c[col]; }
-inline void bindColumns(ColumnStore&, uint64_t*, uint64_t*, int32_t*) {}
+inline void bindColumns(ColumnStore&, uint64_t*, uint64_t*, int32_t*, bool) {}
 
 #endif
 
@@ -845,6 +873,7 @@ struct Extension
 	uint64_t* colVP;
 	uint64_t* colVN;
 	int32_t* colEnd;
+	bool colsInLds; // colVP/colVN/colEnd point into LDS
 	// node -> item index of every kept slice: one open-addressing table per slice
 	// (entries node << 32 | item, ~0 empty), sized to a power of two at least
 	// twice the slice's item count; its first entry is the header's pad field
@@ -1820,7 +1849,7 @@ struct Extension
 		Eq4 eq;
 		sliceEq((uint32_t)j, eq);
 		ColumnStore cols;
-		bindColumns(cols, colVP, colVN, colEnd);
+		bindColumns(cols, colVP, colVN, colEnd, colsInLds);
 		recalc(bestIndex, node, eq, cols);
 		if (status != EXT_OK) return 0;
 		int32_t cells = imin(64, (int32_t)len - j);
@@ -2184,6 +2213,7 @@ GA_HD void carveArena(Extension& e, unsigned char* base, const ArenaCaps& c)
 	e.colVP = (uint64_t*)take(64 * 8);
 	e.colVN = (uint64_t*)take(64 * 8);
 	e.colEnd = (int32_t*)take(64 * 4);
+	e.colsInLds = false;
 }
 
 // Per-request capacities that scale with the read length. caps.maxItems is the
@@ -2193,19 +2223,21 @@ GA_HHD uint32_t requestItems(uint32_t len, const ArenaCaps& caps) { return reque
 
 // Runs one request on the calling wave.
 // colStore, when given, holds the traceback's column store (64 VP words, 64 VN
-// words, 64 scores) in place of the arena's copy: the gfx950 kernels pass an
-// LDS array, which the backtrace reads at every step.
+// words, 64 scores) in place of the arena's copy; the backtrace reads it at
+// every step. colStoreLds says it is an LDS array (a compile-time constant at
+// the kernels' call sites, so only one access path is built); it requires colStore.
 // planes, when given, are the match-mask planes of seqbuf (planeBlock).
-GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out, uint64_t* colStore = nullptr, const uint64_t* planes = nullptr)
+GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out, uint64_t* colStore = nullptr, const uint64_t* planes = nullptr, bool colStoreLds = false)
 {
 	Extension e;
 	carveArena(e, arena, caps);
-	if (colStore)
+	if (colStore || colStoreLds)
 	{
 		e.colVP = colStore;
 		e.colVN = colStore + 64;
 		e.colEnd = (int32_t*)(colStore + 128);
 	}
+	e.colsInLds = colStoreLds;
 	e.g = g;
 	e.p = p;
 	e.seq = seqbuf + r.seqOffset;

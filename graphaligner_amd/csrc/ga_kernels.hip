@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParam
 	c.maxSlices = requestSlices(r.len);
 	c.maxItems = requestItems(r.len, caps);
 	ExtResult out;
-	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, colStore, planes);
+	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, colStore, planes, true);
 	results[idx] = out;
 }
 
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(1) readLoopSingleLaneKernel(DevGraph g, ReadLo
 	c.maxSlices = requestSlices(j.len);
 	c.maxItems = requestItems(j.len, caps);
 	ReadResult out;
-	runReadLoop(g, p, j, reads, seeds + j.seedStart, rawTraces + j.traceOffset, pools + j.poolOffset, alns + j.alnOffset, arenas + j.arenaOffset, c, out, colStore, planes);
+	runReadLoop(g, p, j, reads, seeds + j.seedStart, rawTraces + j.traceOffset, pools + j.poolOffset, alns + j.alnOffset, arenas + j.arenaOffset, c, out, colStore, planes, true);
 	results[idx] = out;
 }
 
