@@ -1416,8 +1416,43 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 					gs.hostMs += ms;
 					gs.replayMs += ms;
 					if (roundLog)
-						std::fprintf(stderr, "[rounds] group %u round %llu: %zu reads, %zu jobs, extend %.1f ms (kernel %.1f ms), replay %.1f ms, ends at %.1f ms\n", lane,
-						             (unsigned long long)gs.rounds, extending.size(), jobs.size(), msSince(te) - ms, eb.kernelMs, ms, msSince(t0));
+					{
+						// the launch's heaviest extension (forward word steps), which
+						// bounds its duration
+						size_t top = 0;
+						for (size_t j = 1; j < jobs.size(); j++)
+							if (eb.results[j].wordSteps > eb.results[top].wordSteps) top = j;
+						std::fprintf(stderr, "[rounds] group %u round %llu: %zu reads, %zu jobs, extend %.1f ms (kernel %.1f ms), replay %.1f ms, ends at %.1f ms; heaviest: len %u, %llu word steps, %llu node calcs, %u slices, band %u, trace %u\n", lane,
+						             (unsigned long long)gs.rounds, extending.size(), jobs.size(), msSince(te) - ms, eb.kernelMs, ms, msSince(t0),
+						             jobs.empty() ? 0u : jobs[top].len, jobs.empty() ? 0ull : (unsigned long long)eb.results[top].wordSteps,
+						             jobs.empty() ? 0ull : (unsigned long long)eb.results[top].nodeCalcs, jobs.empty() ? 0u : eb.results[top].nslices,
+						             jobs.empty() ? 0u : eb.results[top].maxBand, jobs.empty() ? 0u : eb.results[top].ntrace);
+						// GA_HEAVY_DUMP=path appends it as "sequence node offset" (graph node
+						// id, offset in the node) for offline timing of single extensions
+						if (const char* dump = std::getenv("GA_HEAVY_DUMP"); dump && !jobs.empty())
+						{
+							std::lock_guard<std::mutex> lock(phaseMutex);
+							if (FILE* f = std::fopen(dump, "a"))
+							{
+								const ExtJob& jt = jobs[top];
+								std::string seq(jt.len, 'N');
+								if (jt.devOffset < ctx.reverseBase())
+									seq.assign(rs.host.data() + jt.devOffset, jt.len);
+								else
+								{
+									// reverse strand: the reverse complement buffer has the
+									// forward layout
+									const uint64_t off = jt.devOffset - ctx.reverseBase();
+									const size_t r = std::upper_bound(rs.offset.begin(), rs.offset.end(), off) - rs.offset.begin() - 1;
+									const uint64_t k = off - rs.offset[r], L = rs.length[r];
+									for (uint32_t i = 0; i < jt.len; i++) seq[i] = complementOf(rs.seq(r)[L - 1 - (k + i)]);
+								}
+								std::fprintf(f, "%s %d %llu\n", seq.c_str(), g.nodeId[jt.startNode],
+								             (unsigned long long)(g.nodeOff[jt.startNode] + jt.startOffset));
+								std::fclose(f);
+							}
+						}
+					}
 				}
 				std::vector<uint32_t> still, ended;
 				for (uint32_t r : active)

@@ -81,6 +81,8 @@ constexpr uint32_t NIL = 0xffffffffu;
 constexpr uint32_t SKIPFIRST = 0x80000000u;
 constexpr uint32_t LISTEND = 0x7fffffffu; // end of a queued-edge list (low 31 bits of exNext)
 constexpr uint32_t kMaxPopsPerSlice = 1u << 20;
+// a node's columns for the backtrace: five 64-bit words per column and 64 scores
+constexpr uint32_t kColumnWords = 64 * 5 + 32;
 
 struct Slice
 {
@@ -124,6 +126,21 @@ GA_HD int32_t valueAt(const Slice& s, int row)
 {
 	uint64_t mask = row < 63 ? (ONES << (row + 1)) : 0;
 	return s.end + popc(s.VN & mask) - popc(s.VP & mask);
+}
+// a column's step data for the backtrace: the horizontal deltas of the step
+// that produced it (Ph: row r scores one more than in the column before, Mh: one
+// less) and its match mask
+struct ColStep
+{
+	uint64_t ph, mh, eq;
+};
+GA_HD ColStep mkstep(uint64_t ph, uint64_t mh, uint64_t eq)
+{
+	ColStep t;
+	t.ph = ph;
+	t.mh = mh;
+	t.eq = eq;
+	return t;
 }
 // getSourceSliceFromScore (src/GraphAlignerBitvectorCommon.h:1056-1060)
 GA_HD Slice sourceSlice(int32_t score) { return mkslice(ONES, 0, score + 64); }
@@ -442,11 +459,16 @@ GA_HD uint64_t lpDigest(const Item* items, uint32_t start, uint32_t count)
 	for (uint32_t i = 0; i < count; i++) h += itemHash(items[start + i]);
 	return h;
 }
-// a node's column slices, in the thread's workspace or in LDS
+// a node's column slices, in the thread's workspace or in LDS; with each column
+// c >= 1 also the horizontal deltas (Ph, Mh) of the step from column c - 1 and
+// the column's match mask
 struct ColumnStore
 {
 	uint64_t* vp;
 	uint64_t* vn;
+	uint64_t* ph;
+	uint64_t* mh;
+	uint64_t* eq;
 	int32_t* end;
 	bool lds;
 };
@@ -474,6 +496,24 @@ GA_HD Slice colGet(const ColumnStore& c, int col)
 	if (c.lds) return mkslice(((LdsU64*)c.vp)[col], ((LdsU64*)c.vn)[col], ((LdsI32*)c.end)[col]);
 	return mkslice(c.vp[col], c.vn[col], c.end[col]);
 }
+GA_HD void colSetStep(ColumnStore& c, int col, uint64_t ph, uint64_t mh, uint64_t eq)
+{
+	if (c.lds)
+	{
+		((LdsU64*)c.ph)[col] = ph;
+		((LdsU64*)c.mh)[col] = mh;
+		((LdsU64*)c.eq)[col] = eq;
+		return;
+	}
+	c.ph[col] = ph;
+	c.mh[col] = mh;
+	c.eq[col] = eq;
+}
+GA_HD ColStep colGetStep(const ColumnStore& c, int col)
+{
+	if (c.lds) return mkstep(((LdsU64*)c.ph)[col], ((LdsU64*)c.mh)[col], ((LdsU64*)c.eq)[col]);
+	return mkstep(c.ph[col], c.mh[col], c.eq[col]);
+}
 #else
 GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
 {
@@ -482,11 +522,21 @@ GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
 	c.end[col] = s.end;
 }
 GA_HD Slice colGet(const ColumnStore& c, int col) { return mkslice(c.vp[col], c.vn[col], c.end[col]); }
-#endif
-GA_HD void bindColumns(ColumnStore& c, uint64_t* vp, uint64_t* vn, int32_t* end, bool lds)
+GA_HD void colSetStep(ColumnStore& c, int col, uint64_t ph, uint64_t mh, uint64_t eq)
 {
-	c.vp = vp;
-	c.vn = vn;
+	c.ph[col] = ph;
+	c.mh[col] = mh;
+	c.eq[col] = eq;
+}
+GA_HD ColStep colGetStep(const ColumnStore& c, int col) { return mkstep(c.ph[col], c.mh[col], c.eq[col]); }
+#endif
+GA_HD void bindColumns(ColumnStore& c, uint64_t* words, int32_t* end, bool lds)
+{
+	c.vp = words;
+	c.vn = words + 64;
+	c.ph = words + 128;
+	c.mh = words + 192;
+	c.eq = words + 256;
 	c.end = end;
 	c.lds = lds;
 }
@@ -659,7 +709,7 @@ GA_HD uint64_t lpDigest(const Item* items, uint32_t start, uint32_t count)
 // the node's column slices, column c held by lane c
 struct ColumnStore
 {
-	uint64_t vp, vn;
+	uint64_t vp, vn, ph, mh, eq;
 	int32_t end;
 };
 GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
@@ -671,6 +721,16 @@ GA_HD void colSet(ColumnStore& c, int col, const Slice& s)
 		c.end = s.end;
 	}
 }
+GA_HD void colSetStep(ColumnStore& c, int col, uint64_t ph, uint64_t mh, uint64_t eq)
+{
+	if (laneId() == col)
+	{
+		c.ph = ph;
+		c.mh = mh;
+		c.eq = eq;
+	}
+}
+GA_HD ColStep colGetStep(const ColumnStore& c, int col) { return mkstep(__shfl(c.ph, col), __shfl(c.mh, col), __shfl(c.eq, col)); }
 GA_HD Slice colGet(const ColumnStore& c, int col)
 {
 	Slice s;
@@ -679,7 +739,7 @@ GA_HD Slice colGet(const ColumnStore& c, int col)
 	s.end = __shfl(c.end, col);
 	return s;
 }
-GA_HD void bindColumns(ColumnStore&, uint64_t*, uint64_t*, int32_t*, bool) {}
+GA_HD void bindColumns(ColumnStore&, uint64_t*, int32_t*, bool) {}
 
 #else // GA_WAVE_EMULATION: the same helpers as loops over 64 lanes
 
@@ -796,10 +856,13 @@ inline uint64_t lpDigest(const Item* items, uint32_t start, uint32_t count)
 struct ColumnStore
 {
 	Slice c[64];
+	ColStep st[64];
 };
 inline void colSet(ColumnStore& c, int col, const Slice& s) { c.c[col] = s; }
 inline Slice colGet(const ColumnStore& c, int col) { return c.c[col]; }
-inline void bindColumns(ColumnStore&, uint64_t*, uint64_t*, int32_t*, bool) {}
+inline void colSetStep(ColumnStore& c, int col, uint64_t ph, uint64_t mh, uint64_t eq) { c.st[col] = mkstep(ph, mh, eq); }
+inline ColStep colGetStep(const ColumnStore& c, int col) { return c.st[col]; }
+inline void bindColumns(ColumnStore&, uint64_t*, int32_t*, bool) {}
 
 #endif
 
@@ -870,10 +933,9 @@ struct Extension
 	uint64_t* pendKey;
 	uint32_t* pendSeq;
 	uint32_t* pendSlot;
-	uint64_t* colVP;
-	uint64_t* colVN;
+	uint64_t* colWords; // kColumnWords: VP, VN, Ph, Mh, Eq of 64 columns, then 64 scores
 	int32_t* colEnd;
-	bool colsInLds; // colVP/colVN/colEnd point into LDS
+	bool colsInLds; // colWords/colEnd point into LDS
 	// node -> item index of every kept slice: one open-addressing table per slice
 	// (entries node << 32 | item, ~0 empty), sized to a power of two at least
 	// twice the slice's item count; its first entry is the header's pad field
@@ -1246,7 +1308,11 @@ struct Extension
 				res.minScore = ws.end;
 				res.minScoreNodeOffset = pos;
 			}
-			if (Recalc) colSet(*cols, pos, ws);
+			if (Recalc)
+			{
+				colSet(*cols, pos, ws);
+				colSetStep(*cols, pos, Ph, Mh, Eq);
+			}
 			else if (xTop - (int64_t)p.errorCost * (before(ws) - popc(ws.VN)) > xSkip)
 				accAdd(acc, ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, Mh));
 			HP |= (Ph >> 63) << pos;
@@ -1849,7 +1915,7 @@ struct Extension
 		Eq4 eq;
 		sliceEq((uint32_t)j, eq);
 		ColumnStore cols;
-		bindColumns(cols, colVP, colVN, colEnd, colsInLds);
+		bindColumns(cols, colWords, colEnd, colsInLds);
 		recalc(bestIndex, node, eq, cols);
 		if (status != EXT_OK) return 0;
 		int32_t cells = imin(64, (int32_t)len - j);
@@ -1900,7 +1966,6 @@ struct Extension
 		sliceEq((currentSlice - 1) * 64, eq);
 		int32_t lastScore = SMAX - 1;
 		uint32_t steps = 0;
-		NodeSeq curSeq = loadNodeSeq(startPos.node);
 		while (back.seq != -1 && status == EXT_OK)
 		{
 			if (++steps > traceCap)
@@ -1922,7 +1987,6 @@ struct Extension
 				}
 				recalc(currentSlice, currentNode, eq, cols);
 				if (status != EXT_OK) break;
-				curSeq = loadNodeSeq(currentNode);
 			}
 			const SliceHdr& ch = hdr[currentSlice];
 			const SliceHdr& ph = hdr[currentSlice - 1];
@@ -1989,61 +2053,53 @@ struct Extension
 				checkCircularity();
 				continue;
 			}
-			// pickBacktraceInside (src/GraphAlignerBitvectorCommon.h:770-815); the two
-			// columns in use stay in registers until the walk leaves one of them
+			// pickBacktraceInside (src/GraphAlignerBitvectorCommon.h:770-815) on the
+			// columns' delta bits. With here the score of cell (c, r), vd_c(r) the
+			// column's vertical delta (VP/VN bit) and hd_c(r) the horizontal delta of
+			// the step from column c - 1 (Ph/Mh bit):
+			//   vertical   = here - vd_c(r)
+			//   horizontal = here - hd_c(r)
+			//   diagonal   = here - hd_c(r) - vd_{c-1}(r)
+			// so every choice of the walk is a test of a few bits at row r.
 			uint32_t hori = bk.off;
 			int32_t vert = bk.seq - cj;
 			if (hori > 0 && vert > 0)
 			{
-				Slice ch0 = colGet(cols, hori);
-				Slice ch1 = colGet(cols, hori - 1);
-				// characterMatch of read row vert and the node's base at hori, read
-				// from the slice's match masks (rows past the read end match nothing)
-				uint64_t eqm = eqOf(eq, curSeq, hori);
-				// scores of the cell and of its left neighbour, carried along the
-				// walk; the cells one row up differ by the columns' vertical
-				// deltas at row vert (valueAt(s, r - 1) = valueAt(s, r) + VN_r - VP_r)
-				int32_t here = valueAt(ch0, vert);
-				int32_t horizontal = valueAt(ch1, vert);
-				while (hori > 0 && vert > 0 && status == EXT_OK)
+				Slice cc = colGet(cols, hori);
+				Slice cp = colGet(cols, hori - 1);
+				ColStep st = colGetStep(cols, hori);
+				while (status == EXT_OK)
 				{
-					int32_t vertical = here + (int32_t)((ch0.VN >> vert) & 1) - (int32_t)((ch0.VP >> vert) & 1);
-					int32_t diagonal = horizontal + (int32_t)((ch1.VN >> vert) & 1) - (int32_t)((ch1.VP >> vert) & 1);
-					bool eqc = ((eqm >> vert) & 1) != 0;
-					if (vertical < here - 1 || horizontal < here - 1 || diagonal < here - (eqc ? 0 : 1))
+					const uint64_t bit = (uint64_t)1 << vert;
+					const int32_t hd = (int32_t)((st.ph & bit) != 0) - (int32_t)((st.mh & bit) != 0);
+					const int32_t hdv = hd + (int32_t)((cp.VP & bit) != 0) - (int32_t)((cp.VN & bit) != 0);
+					// diagonal == here - (characterMatch ? 0 : 1) when hdv == want
+					const int32_t want = (st.eq & bit) ? 0 : 1;
+					// vertical and horizontal never score below here - 1 (|delta| <= 1)
+					if (hdv > want)
 					{
 						fail(EXT_ASSERT);
 						break;
 					}
-					if (vertical == here - 1)
+					if (cc.VP & bit)
 					{
 						vert--;
 						pushTrace(mkpos(bk.node, hori, vert + cj), false);
-						here = vertical;
-						horizontal = diagonal;
+						if (vert == 0) break;
 						continue;
 					}
-					if (diagonal == here - (eqc ? 0 : 1))
-					{
-						vert--;
-						here = diagonal;
-					}
-					else if (horizontal != here - 1)
+					if (hdv == want) vert--;
+					else if (hd != 1)
 					{
 						fail(EXT_ASSERT);
 						break;
 					}
-					else
-						here = horizontal;
 					hori--;
 					pushTrace(mkpos(bk.node, hori, vert + cj), false);
-					if (hori > 0)
-					{
-						ch0 = ch1;
-						ch1 = colGet(cols, hori - 1);
-						eqm = eqOf(eq, curSeq, hori);
-						horizontal = valueAt(ch1, vert);
-					}
+					if (hori == 0 || vert == 0) break;
+					cc = cp;
+					cp = colGet(cols, hori - 1);
+					st = colGetStep(cols, hori);
 				}
 			}
 		}
@@ -2177,7 +2233,7 @@ GA_HHD uint64_t arenaBytes(const ArenaCaps& c)
 	b += alignUp((uint64_t)slotTabSize(c) * 8) + alignUp((uint64_t)slotTabSize(c) * 4);
 	b += 2 * alignUp((uint64_t)c.maxExtras * 8) + 2 * alignUp((uint64_t)c.maxExtras * 4);
 	b += alignUp((uint64_t)c.maxPending * 8) + 2 * alignUp((uint64_t)c.maxPending * 4);
-	b += 2 * alignUp(64 * 8) + alignUp(64 * 4);
+	b += alignUp(kColumnWords * 8);
 	return b;
 }
 
@@ -2210,9 +2266,8 @@ GA_HD void carveArena(Extension& e, unsigned char* base, const ArenaCaps& c)
 	e.pendKey = (uint64_t*)take((uint64_t)c.maxPending * 8);
 	e.pendSeq = (uint32_t*)take((uint64_t)c.maxPending * 4);
 	e.pendSlot = (uint32_t*)take((uint64_t)c.maxPending * 4);
-	e.colVP = (uint64_t*)take(64 * 8);
-	e.colVN = (uint64_t*)take(64 * 8);
-	e.colEnd = (int32_t*)take(64 * 4);
+	e.colWords = (uint64_t*)take(kColumnWords * 8);
+	e.colEnd = (int32_t*)(e.colWords + 320);
 	e.colsInLds = false;
 }
 
@@ -2222,8 +2277,8 @@ GA_HHD uint32_t requestSlices(uint32_t len) { return len / 64 + 3; }
 GA_HHD uint32_t requestItems(uint32_t len, const ArenaCaps& caps) { return requestSlices(len) * caps.maxItems + 1024; }
 
 // Runs one request on the calling wave.
-// colStore, when given, holds the traceback's column store (64 VP words, 64 VN
-// words, 64 scores) in place of the arena's copy; the backtrace reads it at
+// colStore, when given, holds the traceback's column store (kColumnWords
+// words) in place of the arena's copy; the backtrace reads it at
 // every step. colStoreLds says it is an LDS array (a compile-time constant at
 // the kernels' call sites, so only one access path is built); it requires colStore.
 // planes, when given, are the match-mask planes of seqbuf (planeBlock).
@@ -2233,9 +2288,8 @@ GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r
 	carveArena(e, arena, caps);
 	if (colStore || colStoreLds)
 	{
-		e.colVP = colStore;
-		e.colVN = colStore + 64;
-		e.colEnd = (int32_t*)(colStore + 128);
+		e.colWords = colStore;
+		e.colEnd = (int32_t*)(colStore + 320);
 	}
 	e.colsInLds = colStoreLds;
 	e.g = g;

@@ -5,6 +5,7 @@
 // divergent lanes of a few waves.
 #define GA_THREAD_PER_EXTENSION 1
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 #include "ga_extend_core.h"
 #include "ga_read_core.h"
 #include "ga_kernels.hpp"
@@ -34,7 +35,7 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGra
 // handling for its data-dependent branches.
 __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets, const uint64_t* __restrict__ planes)
 {
-	__shared__ uint64_t colStore[64 * 2 + 32];
+	__shared__ uint64_t colStore[kColumnWords];
 	uint32_t idx = blockIdx.x;
 	if (idx >= nreq) return;
 	ExtRequest r = reqs[idx];
@@ -46,12 +47,35 @@ __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParam
 	results[idx] = out;
 }
 
+// LDS that a single-lane workgroup reserves beyond its own so that at most
+// kSingleLaneWavesPerSimd of them share a SIMD (0: no cap; GA_EXT_WAVES_PER_SIMD
+// overrides). A launch lasts as long as its longest extension, which runs
+// slower the more waves share its SIMD's issue slots; five was the best
+// measured setting on MI355X for the 5 Mbp variation-graph workload (the kernel
+// needs few enough registers for seven).
+constexpr uint32_t kSingleLaneWavesPerSimd = 5;
+static uint32_t singleLaneExtraLds()
+{
+	static const uint32_t extra = []() -> uint32_t {
+		uint32_t waves = kSingleLaneWavesPerSimd;
+		if (const char* v = std::getenv("GA_EXT_WAVES_PER_SIMD")) waves = (uint32_t)std::strtoul(v, nullptr, 10);
+		if (waves == 0) return 0;
+		int dev = 0, ldsPerCu = 0;
+		if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ldsPerCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess) return 0;
+		// four SIMDs per CU; whole 512-byte LDS allocation blocks
+		const uint32_t perGroup = ((uint32_t)ldsPerCu / (4 * waves)) & ~511u;
+		const uint32_t own = kColumnWords * 8;
+		return perGroup > own ? perGroup - own : 0;
+	}();
+	return extra;
+}
+
 hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets, const uint64_t* planes)
 {
 	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
 	if (lanesPerWave == 1 && wavesPerSimd <= 1)
 	{
-		hipLaunchKernelGGL(extendSingleLaneKernel, dim3(nreq), dim3(1), 0, stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
+		hipLaunchKernelGGL(extendSingleLaneKernel, dim3(nreq), dim3(1), singleLaneExtraLds(), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 		return hipGetLastError();
 	}
 	uint32_t blocks = (nreq + lanesPerWave - 1) / lanesPerWave;
@@ -85,7 +109,7 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) readLoopKernel(DevGraph g
 
 __global__ void __launch_bounds__(1) readLoopSingleLaneKernel(DevGraph g, ReadLoopParams p, const ReadJob* __restrict__ jobs, uint32_t njobs, const char* __restrict__ reads, const DevSeed* __restrict__ seeds, uint64_t* __restrict__ rawTraces, uint64_t* __restrict__ pools, DevAlignment* __restrict__ alns, ReadResult* __restrict__ results, unsigned char* __restrict__ arenas, ArenaCaps caps, const uint64_t* __restrict__ planes)
 {
-	__shared__ uint64_t colStore[64 * 2 + 32];
+	__shared__ uint64_t colStore[kColumnWords];
 	uint32_t idx = blockIdx.x;
 	if (idx >= njobs) return;
 	const ReadJob j = jobs[idx];
