@@ -1419,9 +1419,21 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 					{
 						// the launch's heaviest extension (forward word steps), which
 						// bounds its duration
-						size_t top = 0;
-						for (size_t j = 1; j < jobs.size(); j++)
+						size_t top = 0, last = 0;
+						uint64_t c0 = ~0ull;
+						for (size_t j = 0; j < jobs.size(); j++)
+						{
 							if (eb.results[j].wordSteps > eb.results[top].wordSteps) top = j;
+							if (eb.results[j].clockEnd > eb.results[last].clockEnd) last = j;
+							c0 = std::min<uint64_t>(c0, eb.results[j].clockStart);
+						}
+						if (!jobs.empty() && eb.results[top].clockEnd)
+						{
+							// device clock ticks of 10 ns
+							auto rel = [&](uint64_t t) { return (double)(t - c0) * 1e-5; };
+							std::fprintf(stderr, "[rounds]   clocks (ms from the first start): heaviest %.1f-%.1f, last to end: len %u, %.1f-%.1f\n",
+							             rel(eb.results[top].clockStart), rel(eb.results[top].clockEnd), jobs[last].len, rel(eb.results[last].clockStart), rel(eb.results[last].clockEnd));
+						}
 						std::fprintf(stderr, "[rounds] group %u round %llu: %zu reads, %zu jobs, extend %.1f ms (kernel %.1f ms), replay %.1f ms, ends at %.1f ms; heaviest: len %u, %llu word steps, %llu node calcs, %u slices, band %u, trace %u\n", lane,
 						             (unsigned long long)gs.rounds, extending.size(), jobs.size(), msSince(te) - ms, eb.kernelMs, ms, msSince(t0),
 						             jobs.empty() ? 0u : jobs[top].len, jobs.empty() ? 0ull : (unsigned long long)eb.results[top].wordSteps,

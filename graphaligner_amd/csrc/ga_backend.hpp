@@ -126,9 +126,11 @@ inline gad::ArenaCaps initialArenaCaps()
 	gad::ArenaCaps c;
 	c.maxSlices = 0;  // set per request from its length
 	c.maxItems = 48;
-	c.maxSlots = 1024;
-	c.maxExtras = 4096;
-	c.maxPending = 1024;
+	// the band queue of a first attempt fits the single-lane kernel's LDS copy;
+	// reruns grow past it
+	c.maxSlots = gad::kLdsSlots;
+	c.maxExtras = gad::kLdsExtras;
+	c.maxPending = gad::kLdsPending;
 	c.pad = 0;
 	return c;
 }
@@ -182,7 +184,9 @@ inline ReadLoopCaps initialReadLoopCaps()
 	// a read's whole seed loop shares one workspace, and an overflow reruns the
 	// whole read: start wider than a single extension's first attempt
 	c.arena.maxItems *= 2;
-	c.arena.maxExtras *= 2;
+	c.arena.maxSlots = 1024;
+	c.arena.maxExtras = 8192;
+	c.arena.maxPending = 1024;
 	c.poolPerBase = 12;
 	c.poolExtra = 4096;
 	c.alnCap = 64;

@@ -64,6 +64,10 @@ struct ExtResult
 	uint64_t nodeCalcs;   // forward node calculations
 	uint32_t maxBand;     // largest band (nodes) of a kept slice
 	uint32_t overflowWhat;
+	// device real-time clock (100 MHz) at the extension's start and end, set by
+	// the single-lane kernel (zero elsewhere); diagnostics only
+	uint64_t clockStart;
+	uint64_t clockEnd;
 };
 
 // Per-slice record for parity checks (same fields as the oracle's SliceSummary).
@@ -85,6 +89,12 @@ struct SliceSummary
 };
 
 // Capacities of one wave's workspace (all counts, not bytes).
+// Band-queue capacities that the single-lane extension kernel holds in LDS
+// (powers of two for the slot table); the host's first-attempt capacities
+constexpr uint32_t kLdsSlots = 32;
+constexpr uint32_t kLdsExtras = 96;
+constexpr uint32_t kLdsPending = 32;
+
 struct ArenaCaps
 {
 	uint32_t maxSlices;   // slice headers (DP table rows + 1)

@@ -2206,6 +2206,7 @@ struct Extension
 		out.nodeCalcs = nodeCalcs;
 		out.maxBand = maxBand;
 		out.overflowWhat = overflowWhat;
+		out.clockStart = out.clockEnd = 0;
 	}
 };
 
@@ -2271,6 +2272,51 @@ GA_HD void carveArena(Extension& e, unsigned char* base, const ArenaCaps& c)
 	e.colsInLds = false;
 }
 
+// The per-slice band queue (slots, their node table, queued edges, pending
+// nodes) held in LDS by the single-lane kernel when the capacities allow: it is
+// rebuilt at every slice, small (the 5 Mbp variation-graph workload peaks at 14
+// slots, 29 queued edges and 9 pending nodes in a slice) and on the path of
+// every queue pop, where the workspace's HBM round trips cost most. An
+// extension that outgrows it fails with EXT_OVERFLOW and is rerun with larger
+// capacities and the queue in its workspace.
+// (capacities kLdsSlots, kLdsExtras, kLdsPending: ga_device_types.h)
+constexpr uint32_t kLdsSlotTab = 2 * kLdsSlots; // slotTabSize of kLdsSlots
+// 64-bit words: slot table keys, edge VP and VN words, pending keys, then the
+// 32-bit arrays two to a word
+constexpr uint32_t kQueueWords = kLdsSlotTab + 2 * kLdsExtras + kLdsPending + (kLdsSlotTab + 5 * kLdsSlots + 2 * kLdsExtras + 2 * kLdsPending) / 2;
+GA_HHD bool queueFitsLds(const ArenaCaps& c) { return c.maxSlots <= kLdsSlots && c.maxExtras <= kLdsExtras && c.maxPending <= kLdsPending; }
+GA_HD void bindQueue(Extension& e, uint64_t* w)
+{
+	e.slotTabKey = w;
+	w += kLdsSlotTab;
+	e.exVP = w;
+	w += kLdsExtras;
+	e.exVN = w;
+	w += kLdsExtras;
+	e.pendKey = w;
+	w += kLdsPending;
+	uint32_t* u = (uint32_t*)w;
+	e.slotTabVal = u;
+	u += kLdsSlotTab;
+	e.slotNode = u;
+	u += kLdsSlots;
+	e.slotItem = (int32_t*)u;
+	u += kLdsSlots;
+	e.slotHead = u;
+	u += kLdsSlots;
+	e.slotTail = u;
+	u += kLdsSlots;
+	e.slotActive = u;
+	u += kLdsSlots;
+	e.exEnd = (int32_t*)u;
+	u += kLdsExtras;
+	e.exNext = u;
+	u += kLdsExtras;
+	e.pendSeq = u;
+	u += kLdsPending;
+	e.pendSlot = u;
+}
+
 // Per-request capacities that scale with the read length. caps.maxItems is the
 // band-item budget per slice.
 GA_HHD uint32_t requestSlices(uint32_t len) { return len / 64 + 3; }
@@ -2282,10 +2328,14 @@ GA_HHD uint32_t requestItems(uint32_t len, const ArenaCaps& caps) { return reque
 // every step. colStoreLds says it is an LDS array (a compile-time constant at
 // the kernels' call sites, so only one access path is built); it requires colStore.
 // planes, when given, are the match-mask planes of seqbuf (planeBlock).
-GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out, uint64_t* colStore = nullptr, const uint64_t* planes = nullptr, bool colStoreLds = false)
+// With queueLds (a compile-time constant at the call sites) the band queue lives
+// in queueStore (kQueueWords words of LDS); the capacities must fit it
+// (queueFitsLds).
+GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r, const char* seqbuf, uint64_t* tracebuf, unsigned char* arena, const ArenaCaps& caps, SliceSummary* summaries, ExtResult& out, uint64_t* colStore = nullptr, const uint64_t* planes = nullptr, bool colStoreLds = false, uint64_t* queueStore = nullptr, bool queueLds = false)
 {
 	Extension e;
 	carveArena(e, arena, caps);
+	if (queueLds) bindQueue(e, queueStore);
 	if (colStore || colStoreLds)
 	{
 		e.colWords = colStore;

@@ -32,10 +32,13 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGra
 // One request per single-thread workgroup: the compiler knows a single lane
 // runs, so every value is wavefront-uniform and the extension's control flow and
 // most of its 64-bit word arithmetic run on the scalar unit, with no exec-mask
-// handling for its data-dependent branches.
+// handling for its data-dependent branches. LdsQueue: the band queue lives in
+// LDS (first attempts, whose capacities fit it).
+template <bool LdsQueue>
 __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets, const uint64_t* __restrict__ planes)
 {
 	__shared__ uint64_t colStore[kColumnWords];
+	__shared__ uint64_t queueStore[LdsQueue ? kQueueWords : 1];
 	uint32_t idx = blockIdx.x;
 	if (idx >= nreq) return;
 	ExtRequest r = reqs[idx];
@@ -43,7 +46,10 @@ __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParam
 	c.maxSlices = requestSlices(r.len);
 	c.maxItems = requestItems(r.len, caps);
 	ExtResult out;
-	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, colStore, planes, true);
+	const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, colStore, planes, true, queueStore, LdsQueue);
+	out.clockStart = t0;
+	out.clockEnd = __builtin_amdgcn_s_memrealtime();
 	results[idx] = out;
 }
 
@@ -54,20 +60,18 @@ __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParam
 // measured setting on MI355X for the 5 Mbp variation-graph workload (the kernel
 // needs few enough registers for seven).
 constexpr uint32_t kSingleLaneWavesPerSimd = 5;
-static uint32_t singleLaneExtraLds()
+static uint32_t singleLaneExtraLds(uint32_t own)
 {
-	static const uint32_t extra = []() -> uint32_t {
+	static const uint32_t perGroup = []() -> uint32_t {
 		uint32_t waves = kSingleLaneWavesPerSimd;
 		if (const char* v = std::getenv("GA_EXT_WAVES_PER_SIMD")) waves = (uint32_t)std::strtoul(v, nullptr, 10);
 		if (waves == 0) return 0;
 		int dev = 0, ldsPerCu = 0;
 		if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ldsPerCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess) return 0;
 		// four SIMDs per CU; whole 512-byte LDS allocation blocks
-		const uint32_t perGroup = ((uint32_t)ldsPerCu / (4 * waves)) & ~511u;
-		const uint32_t own = kColumnWords * 8;
-		return perGroup > own ? perGroup - own : 0;
+		return ((uint32_t)ldsPerCu / (4 * waves)) & ~511u;
 	}();
-	return extra;
+	return perGroup > own ? perGroup - own : 0;
 }
 
 hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets, const uint64_t* planes)
@@ -75,7 +79,10 @@ hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave
 	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
 	if (lanesPerWave == 1 && wavesPerSimd <= 1)
 	{
-		hipLaunchKernelGGL(extendSingleLaneKernel, dim3(nreq), dim3(1), singleLaneExtraLds(), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
+		if (queueFitsLds(caps))
+			hipLaunchKernelGGL(extendSingleLaneKernel<true>, dim3(nreq), dim3(1), singleLaneExtraLds((kColumnWords + kQueueWords) * 8), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
+		else
+			hipLaunchKernelGGL(extendSingleLaneKernel<false>, dim3(nreq), dim3(1), singleLaneExtraLds((kColumnWords + 1) * 8), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 		return hipGetLastError();
 	}
 	uint32_t blocks = (nreq + lanesPerWave - 1) / lanesPerWave;

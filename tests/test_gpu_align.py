@@ -8,7 +8,7 @@ import pytest
 import oracle_ffi as o
 import graphaligner_amd as ga
 from test_oracle_golden import GOLD, README_LINE, read_fa
-from workloads import edge_reads, workload
+from workloads import edge_reads, fan_workload, workload
 
 pytestmark = pytest.mark.gpu
 
@@ -125,3 +125,20 @@ def test_gpu_seeds_of_single_base_runs(ctx, workdir, preset):
     for s, b in zip(reads, got):
         a = oi.seeds(s, sd["density"], True, cap_per_base=512)
         assert a.shape == b.shape and (a == b).all()
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_gpu_gaf_wide_fans(ctx, workdir, mode):
+    """Slices across 48-way fans outgrow the band queue that a first attempt
+    keeps in LDS: those extensions overflow and are rerun with the queue in
+    their workspace, and the output still matches the oracle."""
+    gfa, names, seqs = fan_workload(workdir)
+    sd, og, oi, gp, g, gi = _both(gfa, "vg")
+    _set_mode(gp, mode)
+    ot, ostat, _ = o.align_reads(og, oi, o.preset_params("vg"), sd["density"], names, seqs, 8)
+    print(f"[gpu-align] wide fans {mode} reads={len(seqs)}", flush=True)
+    gt, gstat, st = ga.align_reads(ctx, g, gi, gp, names, seqs)
+    assert list(gstat) == list(ostat)
+    assert gt == ot
+    if mode != "device_loop":
+        assert st["reruns"] > 0

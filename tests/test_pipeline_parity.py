@@ -150,3 +150,17 @@ def test_seeds_of_single_base_runs(workdir, preset):
             assert (a == b).all()
             nonempty += len(a) > 0
     assert nonempty > len(reads)
+
+
+@pytest.mark.parametrize("mode", sorted(P.MODES))
+def test_gaf_identical_wide_fans(workdir, mode):
+    """Slices across 48-way fans outgrow the first-attempt band-queue capacities:
+    the overflowing extensions are rerun with larger workspaces."""
+    gfa, names, seqs = W.fan_workload(workdir)
+    sd, og, oi, pg, pi = _pair(gfa, "vg")
+    params = O.preset_params("vg")
+    ot, ostat, _ = O.align_reads(og, oi, params, sd["density"], names, seqs, 4)
+    pt, pstat, _ = P.align_reads(pi, params, sd["density"], names, seqs, **P.MODES[mode])
+    assert (ostat == 0).sum() >= len(seqs) // 2
+    assert list(pstat) == list(ostat)
+    assert pt == ot

@@ -73,3 +73,46 @@ def edge_reads(names, seqs, seed=11):
     add("chimera", base[0][:1500] + revcomp(base[3][-1500:]))
     add("tail_only", base[1][-200:])
     return [n for n, _ in out], [s for _, s in out]
+
+
+def fan_workload(workdir, fan=48, regions=12, seed=5):
+    """A chain of random segments joined by fans of `fan` parallel one-base nodes:
+    a slice that crosses a fan holds more band nodes and queued edges than the
+    first-attempt capacities of an extension's band queue, so those extensions
+    overflow and are rerun with larger workspaces."""
+    path = os.path.join(workdir, f"fan{fan}.gfa")
+    rng = random.Random(seed)
+    segs = ["".join(rng.choice("ACGT") for _ in range(rng.randrange(150, 400))) for _ in range(regions + 1)]
+    lines = ["H\tVN:Z:1.0"]
+    nid = 0
+    seg_id = []
+    fan_ids = []
+    for r, s in enumerate(segs):
+        nid += 1
+        seg_id.append(nid)
+        lines.append(f"S\t{nid}\t{s}")
+        if r < regions:
+            ids = []
+            for k in range(fan):
+                nid += 1
+                ids.append(nid)
+                lines.append(f"S\t{nid}\t{'ACGT'[k % 4]}")
+            fan_ids.append(ids)
+    for r in range(regions):
+        for f in fan_ids[r]:
+            lines.append(f"L\t{seg_id[r]}\t+\t{f}\t+\t0M")
+            lines.append(f"L\t{f}\t+\t{seg_id[r + 1]}\t+\t0M")
+    with open(path, "w") as out:
+        out.write("\n".join(lines) + "\n")
+    genome = "".join(s + (rng.choice("ACGT") if r < regions else "") for r, s in enumerate(segs))
+    names, seqs = [], []
+    for i in range(40):
+        a = rng.randrange(0, len(genome) - 500)
+        s = list(genome[a:a + rng.randrange(400, min(2000, len(genome) - a))])
+        for k in range(len(s)):
+            if rng.random() < 0.05:
+                s[k] = rng.choice("ACGT")
+        seq = "".join(s)
+        names.append(f"fan{i}")
+        seqs.append(seq if i % 2 == 0 else revcomp(seq))
+    return path, names, seqs
