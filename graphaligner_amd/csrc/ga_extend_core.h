@@ -1285,9 +1285,13 @@ struct Extension
 		NodeSeq nseqU = nseq;
 		for (int b = 0; b < 4; b++) nseqU.w[b] = uni64(nseq.w[b]);
 		LaneAcc acc;
-		// bound of a column's X-scores: every row scores at least before - popc(VN)
-		const int64_t xTop = ((int64_t)seqOffset + 64) * 100;
-		const int64_t xSkip = (int64_t)imax(xFloor, res.maxX);
+		// Row r of a column scores X(r) = (seqOffset + r + 1) * 100 - cost * value(r),
+		// so X(r) - X(63) sums -100 + cost * (VP_i - VN_i) over the rows i > r: at
+		// most X(63) + max(cost - 100, 0) * popc(VP) for every row. X-scores are
+		// int32 as in lpMaxX.
+		const int32_t xTop = (seqOffset + 64) * 100;
+		const int32_t xRise = p.errorCost > 100 ? p.errorCost - 100 : 0;
+		const int32_t xSkip = imax(xFloor, res.maxX);
 		if (!Recalc)
 		{
 			accInit(acc);
@@ -1313,7 +1317,7 @@ struct Extension
 				colSet(*cols, pos, ws);
 				colSetStep(*cols, pos, Ph, Mh, Eq);
 			}
-			else if (xTop - (int64_t)p.errorCost * (before(ws) - popc(ws.VN)) > xSkip)
+			else if (xTop - p.errorCost * ws.end + xRise * popc(ws.VP) > xSkip)
 				accAdd(acc, ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, Mh));
 			HP |= (Ph >> 63) << pos;
 			HN |= (Mh >> 63) << pos;
