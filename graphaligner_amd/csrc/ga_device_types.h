@@ -92,7 +92,7 @@ struct SliceSummary
 // Band-queue capacities that the single-lane extension kernel holds in LDS
 // (powers of two for the slot table); the host's first-attempt capacities
 constexpr uint32_t kLdsSlots = 32;
-constexpr uint32_t kLdsExtras = 96;
+constexpr uint32_t kLdsExtras = 64;
 constexpr uint32_t kLdsPending = 32;
 
 struct ArenaCaps

@@ -948,6 +948,14 @@ struct Extension
 	uint32_t* slotTabVal;
 	uint32_t slotTabMask;
 	uint32_t slotStamp;
+	// forward pass: the items of the slice being computed (index = item - the
+	// slice's first item) and of the previous kept slice, with a node -> index
+	// table of the latter (entries node << 32 | index, ~0 empty); the kept
+	// slices' items are copied to `items` when their slice ends
+	Item* curItems;
+	Item* prevItems;
+	uint64_t* prevTab;
+	uint32_t prevTabMask;
 	// outputs
 	uint64_t* trace;
 	uint32_t traceCap;
@@ -1367,9 +1375,10 @@ struct Extension
 		while (t < 2 * count) t <<= 1;
 		return t;
 	}
-	GA_HD int32_t findInSlice(uint32_t sliceIdx, uint32_t node) const
+	GA_HD int32_t findInSlice(uint32_t sliceIdx, uint32_t node) const { return findInSlice(hdr[sliceIdx], node); }
+	// with the slice's header at hand (the forward pass keeps the previous one)
+	GA_HD int32_t findInSlice(const SliceHdr& h, uint32_t node) const
 	{
-		const SliceHdr& h = hdr[sliceIdx];
 		if (h.itemCount == 0) return -1;
 		const uint32_t mask = sliceTabSize(h.itemCount) - 1;
 		const uint64_t* t = sliceTab + h.pad;
@@ -1380,6 +1389,29 @@ struct Extension
 			if ((uint32_t)(e >> 32) == node) return (int32_t)(uint32_t)e;
 		}
 		return -1;
+	}
+	// index of node's item in prevItems, or -1
+	GA_HD int32_t findPrev(uint32_t node) const
+	{
+		for (uint32_t i = nodeHash(node) & prevTabMask, probes = 0; probes <= prevTabMask; i = (i + 1) & prevTabMask, probes++)
+		{
+			uint64_t e = prevTab[i];
+			if (e == ~(uint64_t)0) return -1;
+			if ((uint32_t)(e >> 32) == node) return (int32_t)(uint32_t)e;
+		}
+		return -1;
+	}
+	// indexes prevItems[0, count) for findPrev
+	GA_HD void indexPrev(uint32_t count)
+	{
+		for (uint32_t i = 0; i <= prevTabMask; i++) prevTab[i] = ~(uint64_t)0;
+		for (uint32_t k = 0; k < count; k++)
+		{
+			const uint32_t node = prevItems[k].node;
+			uint32_t i = nodeHash(node) & prevTabMask;
+			while (prevTab[i] != ~(uint64_t)0) i = (i + 1) & prevTabMask;
+			prevTab[i] = ((uint64_t)node << 32) | k;
+		}
 	}
 	// indexes the items of kept slice sliceIdx for findInSlice
 	GA_HD void buildSliceTable(uint32_t sliceIdx)
@@ -1432,6 +1464,8 @@ struct Extension
 		first.minScore = 0;
 		items[0] = first;
 		itemNode[0] = startNode;
+		prevItems[0] = first;
+		indexPrev(1);
 		SliceHdr h0;
 		h0.minScore = 0;
 		h0.minScoreNode = (int32_t)startNode;
@@ -1476,18 +1510,17 @@ struct Extension
 			seqCounter = 0;
 			for (uint32_t k = 0; k < P.itemCount; k++)
 			{
-				uint32_t it = P.itemStart + k;
-				Item pi = loadItem(it);
+				Item pi = prevItems[k];
 				if (j != 0)
 				{
 					if (pi.minScore > prevQuit) continue;
 					if (g.linearizable[pi.node])
 					{
 						uint32_t nb = g.inAdj[g.inOff[pi.node]];
-						int32_t q = findInSlice(kept - 1, nb);
+						int32_t q = findPrev(nb);
 						if (q >= 0)
 						{
-							Item qi = loadItem(q);
+							const Item& qi = prevItems[q];
 							if (qi.eEnd < prevQuit && qi.minScore < prevQuit) continue;
 						}
 					}
@@ -1536,13 +1569,13 @@ struct Extension
 				}
 				else
 				{
-					it = loadItem(itIdx);
+					it = curItems[itIdx - sliceItemStart];
 				}
 				Slice oldEnd = exists ? mkslice(it.eVP, it.eVN, it.eEnd) : mkslice(0, 0, SMAX);
-				int32_t pq = findInSlice(kept - 1, node);
-				Item prevIt = pq >= 0 ? loadItem(pq) : defaultPrev();
+				int32_t pq = findPrev(node);
+				Item prevIt = pq >= 0 ? prevItems[pq] : defaultPrev();
 				bool prevHasIn = false;
-				if (exists && (g.inOff[node + 1] - g.inOff[node]) == 1) prevHasIn = findInSlice(kept - 1, g.inAdj[g.inOff[node]]) >= 0;
+				if (exists && (g.inOff[node + 1] - g.inOff[node]) == 1) prevHasIn = findPrev(g.inAdj[g.inOff[node]]) >= 0;
 				NodeResult nc = calcNode<false>(node, it, exists, eq, prevIt, pq >= 0, slotHead[slot], mkslice(0, 0, 0), j, prevHasIn, nullptr, rMaxX);
 				nodeCalcs++;
 				if (status != EXT_OK) return;
@@ -1561,7 +1594,7 @@ struct Extension
 				}
 				curMin = imin(curMin, nc.minScore);
 				if (nc.minScore < it.minScore) it.minScore = nc.minScore;
-				storeItem(itIdx, it);
+				curItems[itIdx - sliceItemStart] = it;
 				Slice newEnd = mkslice(it.eVP, it.eVN, it.eEnd);
 				if (!same(newEnd, oldEnd))
 				{
@@ -1620,10 +1653,15 @@ struct Extension
 				nItems = sliceItemStart;
 				break;
 			}
+			for (uint32_t k = 0; k < hn.itemCount; k++) storeItem(sliceItemStart + k, curItems[k]);
 			hdr[kept] = hn;
 			kept++;
 			buildSliceTable(kept - 1);
 			if (hn.itemCount > maxBand) maxBand = hn.itemCount;
+			Item* t = prevItems;
+			prevItems = curItems;
+			curItems = t;
+			indexPrev(hn.itemCount);
 		}
 	}
 
@@ -2238,6 +2276,7 @@ GA_HHD uint64_t arenaBytes(const ArenaCaps& c)
 	b += alignUp((uint64_t)slotTabSize(c) * 8) + alignUp((uint64_t)slotTabSize(c) * 4);
 	b += 2 * alignUp((uint64_t)c.maxExtras * 8) + 2 * alignUp((uint64_t)c.maxExtras * 4);
 	b += alignUp((uint64_t)c.maxPending * 8) + 2 * alignUp((uint64_t)c.maxPending * 4);
+	b += 2 * alignUp((uint64_t)c.maxSlots * sizeof(Item)) + alignUp((uint64_t)slotTabSize(c) * 8);
 	b += alignUp(kColumnWords * 8);
 	return b;
 }
@@ -2271,6 +2310,10 @@ GA_HD void carveArena(Extension& e, unsigned char* base, const ArenaCaps& c)
 	e.pendKey = (uint64_t*)take((uint64_t)c.maxPending * 8);
 	e.pendSeq = (uint32_t*)take((uint64_t)c.maxPending * 4);
 	e.pendSlot = (uint32_t*)take((uint64_t)c.maxPending * 4);
+	e.curItems = (Item*)take((uint64_t)c.maxSlots * sizeof(Item));
+	e.prevItems = (Item*)take((uint64_t)c.maxSlots * sizeof(Item));
+	e.prevTab = (uint64_t*)take((uint64_t)slotTabSize(c) * 8);
+	e.prevTabMask = slotTabSize(c) - 1;
 	e.colWords = (uint64_t*)take(kColumnWords * 8);
 	e.colEnd = (int32_t*)(e.colWords + 320);
 	e.colsInLds = false;
@@ -2285,9 +2328,14 @@ GA_HD void carveArena(Extension& e, unsigned char* base, const ArenaCaps& c)
 // capacities and the queue in its workspace.
 // (capacities kLdsSlots, kLdsExtras, kLdsPending: ga_device_types.h)
 constexpr uint32_t kLdsSlotTab = 2 * kLdsSlots; // slotTabSize of kLdsSlots
-// 64-bit words: slot table keys, edge VP and VN words, pending keys, then the
-// 32-bit arrays two to a word
-constexpr uint32_t kQueueWords = kLdsSlotTab + 2 * kLdsExtras + kLdsPending + (kLdsSlotTab + 5 * kLdsSlots + 2 * kLdsExtras + 2 * kLdsPending) / 2;
+// 64-bit words: slot table keys, edge VP and VN words, pending keys, the 32-bit
+// arrays two to a word, then the current and previous slice's items (8 words
+// each) and the previous slice's node table. The traceback's column store
+// (kColumnWords) shares the same LDS: the forward pass and the traceback do not
+// overlap.
+constexpr uint32_t kQueueWords = kLdsSlotTab + 2 * kLdsExtras + kLdsPending + (kLdsSlotTab + 5 * kLdsSlots + 2 * kLdsExtras + 2 * kLdsPending) / 2 + 2 * kLdsSlots * 8 + kLdsSlotTab;
+static_assert(sizeof(Item) == 64, "Item is eight words");
+static_assert(kQueueWords >= kColumnWords, "the queue region covers the column store");
 GA_HHD bool queueFitsLds(const ArenaCaps& c) { return c.maxSlots <= kLdsSlots && c.maxExtras <= kLdsExtras && c.maxPending <= kLdsPending; }
 GA_HD void bindQueue(Extension& e, uint64_t* w)
 {
@@ -2319,6 +2367,11 @@ GA_HD void bindQueue(Extension& e, uint64_t* w)
 	e.pendSeq = u;
 	u += kLdsPending;
 	e.pendSlot = u;
+	u += kLdsPending;
+	e.curItems = (Item*)u;
+	e.prevItems = e.curItems + kLdsSlots;
+	e.prevTab = (uint64_t*)(e.prevItems + kLdsSlots);
+	e.prevTabMask = kLdsSlotTab - 1;
 }
 
 // Per-request capacities that scale with the read length. caps.maxItems is the

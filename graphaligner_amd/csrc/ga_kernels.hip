@@ -37,8 +37,8 @@ __global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGra
 template <bool LdsQueue>
 __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets, const uint64_t* __restrict__ planes)
 {
-	__shared__ uint64_t colStore[kColumnWords];
-	__shared__ uint64_t queueStore[LdsQueue ? kQueueWords : 1];
+	// the forward pass's queue and items and the traceback's columns share it
+	__shared__ uint64_t ldsWords[LdsQueue ? kQueueWords : kColumnWords];
 	uint32_t idx = blockIdx.x;
 	if (idx >= nreq) return;
 	ExtRequest r = reqs[idx];
@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParam
 	c.maxItems = requestItems(r.len, caps);
 	ExtResult out;
 	const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, colStore, planes, true, queueStore, LdsQueue);
+	runRequest(g, p, r, seqbuf, tracebuf, arenas + arenaOffsets[idx], c, sums ? sums + sumOffsets[idx] : nullptr, out, ldsWords, planes, true, ldsWords, LdsQueue);
 	out.clockStart = t0;
 	out.clockEnd = __builtin_amdgcn_s_memrealtime();
 	results[idx] = out;
@@ -80,9 +80,9 @@ hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave
 	if (lanesPerWave == 1 && wavesPerSimd <= 1)
 	{
 		if (queueFitsLds(caps))
-			hipLaunchKernelGGL(extendSingleLaneKernel<true>, dim3(nreq), dim3(1), singleLaneExtraLds((kColumnWords + kQueueWords) * 8), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
+			hipLaunchKernelGGL(extendSingleLaneKernel<true>, dim3(nreq), dim3(1), singleLaneExtraLds(kQueueWords * 8), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 		else
-			hipLaunchKernelGGL(extendSingleLaneKernel<false>, dim3(nreq), dim3(1), singleLaneExtraLds((kColumnWords + 1) * 8), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
+			hipLaunchKernelGGL(extendSingleLaneKernel<false>, dim3(nreq), dim3(1), singleLaneExtraLds(kColumnWords * 8), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 		return hipGetLastError();
 	}
 	uint32_t blocks = (nreq + lanesPerWave - 1) / lanesPerWave;
