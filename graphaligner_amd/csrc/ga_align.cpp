@@ -112,18 +112,22 @@ char complementOf(char c)
 }
 
 // One trace item (src/GraphAlignerCommon.h:110-151). node/nodeOffset are split
-// node and offset before the fix-ups, bigraph id and offset after them.
+// node and offset before the fix-ups, bigraph id and offset after them. 16 bytes:
+// bigraph ids, node offsets and read positions fit 32 bits (reads are at most
+// 16 Mbp); every alignment's trace is written once and read by the seed loop
+// and the GAF writer, so its size is memory traffic and page faults.
 struct TItem
 {
-	uint64_t node;
-	uint64_t nodeOffset;
-	uint64_t seqPos;
+	uint32_t node;
+	uint32_t nodeOffset;
+	uint32_t seqPos;
 	bool nodeSwitch;
 	char seqChar;
 	char graphChar;
 	// left uninitialised: traces are sized first and then filled item by item
 	TItem() {}
 };
+static_assert(sizeof(TItem) == 16, "trace items are 16 bytes");
 
 struct Alignment
 {
