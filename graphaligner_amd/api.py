@@ -6,6 +6,7 @@
 * align_reads    ~ the per-read pipeline of src/Aligner.cpp:390-606 with GAF output
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -168,16 +169,30 @@ _decode_utf8.restype = ctypes.py_object
 _decode_utf8.argtypes = [ctypes.c_void_p, ctypes.c_ssize_t, ctypes.c_char_p]
 
 
-def _gaf_text(out, st):
-    """The library's GAF buffer as str (decoded in place, one copy), then freed."""
+def _gaf_text(out, st, gaf_format="str"):
+    """The library's GAF buffer as str (decoded in place, one copy, then freed), or
+    with gaf_format="buffer" as a uint8 array over the library's buffer itself (no
+    copy; the buffer is freed when the array and its views are collected)."""
+    if gaf_format == "buffer":
+        n = int(st.gafBytes)
+        if n == 0 or not out.value:
+            if out.value:
+                N.lib().ga_free(out)
+            return np.zeros(0, dtype=np.uint8)
+        raw = (ctypes.c_uint8 * n).from_address(out.value)
+        weakref.finalize(raw, N.lib().ga_free, ctypes.c_void_p(out.value))
+        return np.ctypeslib.as_array(raw)
+    if gaf_format != "str":
+        raise ValueError("gaf_format must be 'str' or 'buffer'")
     try:
         return _decode_utf8(out.value, st.gafBytes, b"strict")
     finally:
         N.lib().ga_free(out)
 
 
-def align_reads(ctx, graph, index, params, names, seqs):
-    """GAF text of all reads (lines of read 0 first), per-read status, stats dict."""
+def align_reads(ctx, graph, index, params, names, seqs, gaf_format="str"):
+    """GAF text of all reads (lines of read 0 first), per-read status, stats dict.
+    gaf_format: "str", or "buffer" for the text's bytes without a copy (_gaf_text)."""
     n, arr, lens = _reads(seqs)
     na = (ctypes.c_char_p * max(n, 1))(*[x.encode() for x in names])
     out = ctypes.c_void_p()
@@ -186,7 +201,7 @@ def align_reads(ctx, graph, index, params, names, seqs):
     P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
     _check(N.lib().ga_align_reads(ctx.h, graph.h, index.h, ctypes.byref(params), n, na, arr, P(lens), ctypes.byref(out),
                                   P(status), ctypes.byref(st)), "ga_align_reads")
-    text = _gaf_text(out, st)
+    text = _gaf_text(out, st, gaf_format)
     stats = {f: getattr(st, f) for f, _ in N.AlignStats._fields_}
     return text, status[:n], stats
 
@@ -197,7 +212,7 @@ def upload_reads(ctx, seqs):
     _check(N.lib().ga_reads_upload(ctx.h, n, arr, lens.ctypes.data_as(ctypes.c_void_p)), "ga_reads_upload")
 
 
-def align_resident(ctx, graph, index, params, first, names):
+def align_resident(ctx, graph, index, params, first, names, gaf_format="str"):
     """align_reads on resident reads [first, first + len(names))."""
     count = len(names)
     na = (ctypes.c_char_p * max(count, 1))(*[x.encode() for x in names])
@@ -206,6 +221,6 @@ def align_resident(ctx, graph, index, params, first, names):
     st = N.AlignStats()
     _check(N.lib().ga_align_resident(ctx.h, graph.h, index.h, ctypes.byref(params), first, count, na, ctypes.byref(out),
                                      status.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)), "ga_align_resident")
-    text = _gaf_text(out, st)
+    text = _gaf_text(out, st, gaf_format)
     stats = {f: getattr(st, f) for f, _ in N.AlignStats._fields_}
     return text, status[:count], stats

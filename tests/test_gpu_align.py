@@ -142,3 +142,18 @@ def test_gpu_gaf_wide_fans(ctx, workdir, mode):
     assert gt == ot
     if mode != "device_loop":
         assert st["reruns"] > 0
+
+
+def test_gpu_gaf_buffer_matches_text(ctx, workdir):
+    """The zero-copy GAF buffer holds the same bytes as the decoded text."""
+    gfa, names, seqs = workload("mini", workdir)
+    sd, og, oi, gp, g, gi = _both(gfa, "vg")
+    print("[gpu-align] gaf buffer", flush=True)
+    text, s1, _ = ga.align_reads(ctx, g, gi, gp, names, seqs)
+    buf, s2, st = ga.align_reads(ctx, g, gi, gp, names, seqs, gaf_format="buffer")
+    assert list(s1) == list(s2)
+    assert buf.dtype.name == "uint8" and len(buf) == st["gafBytes"]
+    assert buf.tobytes().decode() == text
+    view = buf[:10]
+    del buf
+    assert view.tobytes().decode() == text[:10]
