@@ -1128,7 +1128,7 @@ struct Extension
 	// it exceeds xFloor (the only case where the slice uses it); columns whose
 	// X-scores cannot exceed max(xFloor, column 0's) are not scanned.
 	template <bool Recalc>
-	GA_HD NodeResult calcNode(uint32_t node, Item& it, bool exists, const Eq4& eq, const Item& prev, bool prevExists, uint32_t firstExtra, const Slice& recalcIncoming, int32_t seqOffset, bool prevHasInNeighbor, ColumnStore* cols, int32_t xFloor = SMIN)
+	GA_HD NodeResult calcNode(uint32_t node, Item& it, bool exists, const Eq4& eq, const Item& prev, bool prevExists, uint32_t firstExtra, const Slice& recalcIncoming, int32_t seqOffset, bool prevHasInNeighbor, ColumnStore* cols, int32_t xFloor = SMIN, uint32_t recalcCols = NIL)
 	{
 		NodeResult res;
 		res.minScore = SMAX;
@@ -1304,7 +1304,9 @@ struct Extension
 		{
 			accInit(acc);
 		}
-		for (uint32_t pos = 1; pos < L; pos++)
+		// a recalculation may stop after the columns the backtrace will read
+		const uint32_t Lc = Recalc && recalcCols < L ? recalcCols : L;
+		for (uint32_t pos = 1; pos < Lc; pos++)
 		{
 			uint64_t Eq = eqOf(eqU, nseqU, pos) & forceEq;
 			uint64_t Ph, Mh;
@@ -1341,7 +1343,7 @@ struct Extension
 		it.eVP = ws.VP;
 		it.eVN = ws.VN;
 		it.eEnd = ws.end;
-		if (prevExists && before(ws) > prev.eEnd) fail(EXT_ASSERT);
+		if (Lc == L && prevExists && before(ws) > prev.eEnd) fail(EXT_ASSERT);
 		return res;
 	}
 
@@ -1927,8 +1929,11 @@ struct Extension
 		second = mkpick(pos, false);
 	}
 
-	// recalcNodeWordslice (src/GraphAlignerBitvectorCommon.h:1078-1102)
-	GA_HD void recalc(uint32_t sliceIdx, uint32_t node, const Eq4& eq, ColumnStore& cols)
+	// recalcNodeWordslice (src/GraphAlignerBitvectorCommon.h:1078-1102). With
+	// lastCol, only columns 0..lastCol (those a backtrace entering the node at
+	// column lastCol reads); the node's end slice is then not compared with the
+	// stored one.
+	GA_HD void recalc(uint32_t sliceIdx, uint32_t node, const Eq4& eq, ColumnStore& cols, uint32_t lastCol = NIL)
 	{
 		int32_t ci = findInSlice(sliceIdx, node);
 		if (ci < 0)
@@ -1940,8 +1945,10 @@ struct Extension
 		int32_t pq = findInSlice(sliceIdx - 1, node);
 		Item prevIt = pq >= 0 ? loadItem(pq) : defaultPrev();
 		Item copy = stored;
-		calcNode<true>(node, copy, true, eq, prevIt, pq >= 0, NIL, mkslice(stored.sVP, stored.sVN, stored.sEnd), (int32_t)((sliceIdx - 1) * 64), false, &cols);
-		if (copy.sVP != stored.sVP || copy.sVN != stored.sVN || copy.sEnd != stored.sEnd || copy.eVP != stored.eVP || copy.eVN != stored.eVN || copy.eEnd != stored.eEnd) fail(EXT_ASSERT);
+		const uint32_t cols1 = lastCol == NIL ? NIL : lastCol + 1;
+		calcNode<true>(node, copy, true, eq, prevIt, pq >= 0, NIL, mkslice(stored.sVP, stored.sVN, stored.sEnd), (int32_t)((sliceIdx - 1) * 64), false, &cols, SMIN, cols1);
+		if (copy.sVP != stored.sVP || copy.sVN != stored.sVN || copy.sEnd != stored.sEnd) fail(EXT_ASSERT);
+		if (cols1 >= nodeLen(node) && (copy.eVP != stored.eVP || copy.eVN != stored.eVN || copy.eEnd != stored.eEnd)) fail(EXT_ASSERT);
 	}
 
 	// getReverseTraceFromTableExactEndPos + getReverseTraceFromTable
@@ -2027,7 +2034,7 @@ struct Extension
 					fail(EXT_ASSERT);
 					break;
 				}
-				recalc(currentSlice, currentNode, eq, cols);
+				recalc(currentSlice, currentNode, eq, cols, back.off);
 				if (status != EXT_OK) break;
 			}
 			const SliceHdr& ch = hdr[currentSlice];
