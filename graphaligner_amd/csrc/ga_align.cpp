@@ -1175,8 +1175,12 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		if (opt.groupByLength)
 		{
 			// contiguous length classes with equal shares of the bases: the longest
-			// reads' group has few reads and long launches, the others many reads
-			// and short launches, so the groups fall out of step by themselves
+			// reads' group has few reads and launches as long as its longest
+			// extension, the other many reads and launches bound by their total
+			// work; with one launch at a time the groups fall out of step by
+			// themselves (measured on MI355X: 0.84 s per step against 0.91 s for
+			// reads dealt round-robin; shares of 35 % or 65 % for the long reads'
+			// group were slower)
 			uint64_t total = 0, acc = 0;
 			for (uint32_t r : order) total += blens[r];
 			for (uint32_t r : order)
@@ -1196,6 +1200,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		// groups start out of phase, so while one group's host work runs another's
 		// kernel has the GPU
 		std::mutex phaseMutex;
+		std::mutex gpuMutex;
 		std::condition_variable phaseCv;
 		uint32_t firstLaunchesDone = 0;
 		struct FirstLaunch
@@ -1336,7 +1341,14 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				}
 				ExtBatch eb;
 				auto te = std::chrono::steady_clock::now();
-				ctx.extend(g, opt.ext, jobs, false, eb, ctx.readsBase(), lane);
+				{
+					// one group's launch at a time: a launch lasts as long as its
+					// longest extension, which two concurrent launches slow down (and
+					// whose shortest extensions then wait for wave slots), while the
+					// other group's host work overlaps either way
+					std::lock_guard<std::mutex> gpu(gpuMutex);
+					ctx.extend(g, opt.ext, jobs, false, eb, ctx.readsBase(), lane);
+				}
 				gs.extendCallMs += msSince(te);
 				firstLaunch.signal();
 				gs.rounds++;

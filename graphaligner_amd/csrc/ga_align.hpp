@@ -61,10 +61,11 @@ struct AlignOptions
 	// groups, at most the backend's extension lanes, whose seed loops run in
 	// separate host threads so one group's host work overlaps another's kernels
 	uint32_t pipelineGroups = 2;
-	// true: groups are length classes with equal shares of the bases; false: the
-	// reads are dealt round-robin (longest first) and the groups' first launches
-	// are staggered
-	bool groupByLength = false;
+	// true (default): groups are length classes with equal shares of the bases;
+	// false: the reads are dealt round-robin (longest first) and the groups'
+	// first launches are staggered. Either way the groups' launches run one at a
+	// time.
+	bool groupByLength = true;
 };
 
 enum ReadStatus
