@@ -94,6 +94,8 @@ struct SliceSummary
 constexpr uint32_t kLdsSlots = 32;
 constexpr uint32_t kLdsExtras = 64;
 constexpr uint32_t kLdsPending = 32;
+// default cap on the single-lane extension kernel's workgroups per SIMD
+constexpr uint32_t kSingleLaneWavesPerSimd = 5;
 
 struct ArenaCaps
 {

@@ -54,35 +54,35 @@ __global__ void __launch_bounds__(1) extendSingleLaneKernel(DevGraph g, ExtParam
 }
 
 // LDS that a single-lane workgroup reserves beyond its own so that at most
-// kSingleLaneWavesPerSimd of them share a SIMD (0: no cap; GA_EXT_WAVES_PER_SIMD
-// overrides). A launch lasts as long as its longest extension, which runs
-// slower the more waves share its SIMD's issue slots; five was the best
+// `waves` of them share a SIMD (0: no cap; GA_EXT_WAVES_PER_SIMD overrides). A
+// launch bound by its longest extension lasts longer the more waves share that
+// extension's SIMD issue slots; kSingleLaneWavesPerSimd (five) was the best
 // measured setting on MI355X for the 5 Mbp variation-graph workload (the kernel
 // needs few enough registers for seven).
-constexpr uint32_t kSingleLaneWavesPerSimd = 5;
-static uint32_t singleLaneExtraLds(uint32_t own)
+static uint32_t singleLaneExtraLds(uint32_t own, uint32_t waves)
 {
-	static const uint32_t perGroup = []() -> uint32_t {
-		uint32_t waves = kSingleLaneWavesPerSimd;
-		if (const char* v = std::getenv("GA_EXT_WAVES_PER_SIMD")) waves = (uint32_t)std::strtoul(v, nullptr, 10);
-		if (waves == 0) return 0;
-		int dev = 0, ldsPerCu = 0;
-		if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ldsPerCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess) return 0;
-		// four SIMDs per CU; whole 512-byte LDS allocation blocks
-		return ((uint32_t)ldsPerCu / (4 * waves)) & ~511u;
+	static const uint32_t ldsPerCu = []() -> uint32_t {
+		int dev = 0, v = 0;
+		if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess) return 0;
+		return (uint32_t)v;
 	}();
+	static const int envWaves = std::getenv("GA_EXT_WAVES_PER_SIMD") ? (int)std::strtoul(std::getenv("GA_EXT_WAVES_PER_SIMD"), nullptr, 10) : -1;
+	if (envWaves >= 0) waves = (uint32_t)envWaves;
+	if (waves == 0 || ldsPerCu == 0) return 0;
+	// four SIMDs per CU; whole 512-byte LDS allocation blocks
+	const uint32_t perGroup = (ldsPerCu / (4 * waves)) & ~511u;
 	return perGroup > own ? perGroup - own : 0;
 }
 
-hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets, const uint64_t* planes)
+hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets, const uint64_t* planes, uint32_t capWaves)
 {
 	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;
 	if (lanesPerWave == 1 && wavesPerSimd <= 1)
 	{
 		if (queueFitsLds(caps))
-			hipLaunchKernelGGL(extendSingleLaneKernel<true>, dim3(nreq), dim3(1), singleLaneExtraLds(kQueueWords * 8), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
+			hipLaunchKernelGGL(extendSingleLaneKernel<true>, dim3(nreq), dim3(1), singleLaneExtraLds(kQueueWords * 8, capWaves), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 		else
-			hipLaunchKernelGGL(extendSingleLaneKernel<false>, dim3(nreq), dim3(1), singleLaneExtraLds(kColumnWords * 8), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
+			hipLaunchKernelGGL(extendSingleLaneKernel<false>, dim3(nreq), dim3(1), singleLaneExtraLds(kColumnWords * 8, capWaves), stream, g, p, reqs, nreq, seqbuf, tracebuf, results, arenas, arenaOffsets, caps, sums, sumOffsets, planes);
 		return hipGetLastError();
 	}
 	uint32_t blocks = (nreq + lanesPerWave - 1) / lanesPerWave;

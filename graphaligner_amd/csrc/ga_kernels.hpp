@@ -7,8 +7,9 @@ namespace gad {
 
 // One thread per request, lanesPerWave (1..64) requests per wave, compiled for
 // at least wavesPerSimd (1, 4 or 8) resident waves per SIMD; arenaOffsets[i]
-// locates request i's workspace.
-hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets, const uint64_t* planes);
+// locates request i's workspace. The single-lane kernel (lanesPerWave 1) holds
+// at most capWaves of its workgroups per SIMD (0: no cap).
+hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ExtParams& p, const ExtRequest* reqs, const char* seqbuf, uint64_t* tracebuf, ExtResult* results, unsigned char* arenas, const uint64_t* arenaOffsets, const ArenaCaps& caps, SliceSummary* sums, const uint64_t* sumOffsets, const uint64_t* planes, uint32_t capWaves = kSingleLaneWavesPerSimd);
 
 // One read's seed loop per wave (lane 0 unless lanesPerWave > 1): jobs[i] locates
 // read i's bytes, seeds and workspaces; results[i] receives its outcome.

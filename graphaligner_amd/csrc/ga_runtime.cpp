@@ -342,7 +342,7 @@ void Context::runLaunch(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtPa
 	}
 	if (!planes) throw std::runtime_error("extension launch without match-mask planes");
 	GA_HIP(hipEventRecord(L.ev0, L.stream));
-	GA_HIP(gad::launchExtend(L.stream, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, L.reqs.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : L.seqs.as<char>(), L.traces.as<uint64_t>(), L.results.as<gad::ExtResult>(), L.arena.as<unsigned char>(), L.arenaOffsets.as<uint64_t>(), caps, wantSummaries ? L.sums.as<gad::SliceSummary>() : nullptr, wantSummaries ? L.sumOffsets.as<uint64_t>() : nullptr, planes));
+	GA_HIP(gad::launchExtend(L.stream, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, L.reqs.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : L.seqs.as<char>(), L.traces.as<uint64_t>(), L.results.as<gad::ExtResult>(), L.arena.as<unsigned char>(), L.arenaOffsets.as<uint64_t>(), caps, wantSummaries ? L.sums.as<gad::SliceSummary>() : nullptr, wantSummaries ? L.sumOffsets.as<uint64_t>() : nullptr, planes, L.capWaves));
 	GA_HIP(hipEventRecord(L.ev1, L.stream));
 	std::vector<gad::ExtResult> res(n);
 	GA_HIP(hipMemcpyAsync(res.data(), L.results.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, L.stream));

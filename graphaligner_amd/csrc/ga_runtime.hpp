@@ -60,6 +60,8 @@ struct PinnedBuffer
 struct ExtendLane
 {
 	hipStream_t stream = nullptr;
+	// single-lane extension workgroups per SIMD (launchExtend's capWaves)
+	uint32_t capWaves = gad::kSingleLaneWavesPerSimd;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	DeviceBuffer arena, arenaOffsets, reqs, seqs, traces, results, sums, sumOffsets;
 	DeviceBuffer gatherSrc, gatherDst, gatherCnt, traceCodes, traceEscapes, escapeCount, planes;
