@@ -1095,7 +1095,8 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		stats.lookupMs += rb.lookupMs;
 		std::vector<ReadState> states(bn);
 		auto th = std::chrono::steady_clock::now();
-		parallelFor(bn, opt.hostThreads, [&](size_t i) {
+		// seeds of read i: expansion, ordering and the letter check
+		auto seedRead = [&](size_t i) {
 			ReadState& st = states[i];
 			st.read = (uint32_t)i;
 			ReadOutcome& o = out[batchStart + i];
@@ -1118,9 +1119,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				st.status = READ_ASSERT;
 				st.done = true;
 			}
-		});
-		{ double ms = msSince(th); stats.hostMs += ms; stats.seedingMs += ms; }
-		for (auto& st : states) stats.seeds += st.seeds.size();
+		};
 		// selection and GAF of one read (src/Aligner.cpp:502-606); its alignments
 		// are released afterwards
 		auto selectAndWrite = [&](size_t i) {
@@ -1166,9 +1165,12 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		// the seed loops of G groups of reads (G = the backend's extension lanes, at
 		// most opt.pipelineGroups) run in G host threads, each on its own lane,
 		// so one group's host replay overlaps the other groups' kernels
-		std::vector<uint32_t> order;
-		for (uint32_t i = 0; i < bn; i++)
-			if (!states[i].done) order.push_back(i);
+		// Each group seeds its own reads when it starts; the other groups wait
+		// until group 0 has seeded, so that their seeding runs while group 0's
+		// first launch has the GPU instead of before it. A read without seeds
+		// ends in its group's first round.
+		std::vector<uint32_t> order(bn);
+		for (uint32_t i = 0; i < bn; i++) order[i] = i;
 		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return blens[a] > blens[b]; });
 		const uint32_t groups = std::max<uint32_t>(1, std::min<uint32_t>({ctx.extendLanes(), opt.pipelineGroups, (uint32_t)std::max<size_t>(1, order.size())}));
 		std::vector<std::vector<uint32_t>> groupReads(groups);
@@ -1258,10 +1260,39 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				if (worker.joinable()) worker.join();
 			}
 		};
+		bool group0Seeded = false;
+		auto signalSeeded = [&]() {
+			{
+				std::lock_guard<std::mutex> lock(phaseMutex);
+				group0Seeded = true;
+			}
+			phaseCv.notify_all();
+		};
 		auto runGroup = [&](uint32_t lane) {
 			AlignStats& gs = groupStats[lane];
 			Finisher finisher{finishRead, threads, gs.gafMs};
 			FirstLaunch firstLaunch{phaseMutex, phaseCv, firstLaunchesDone};
+			{
+				if (lane > 0)
+				{
+					std::unique_lock<std::mutex> lock(phaseMutex);
+					phaseCv.wait(lock, [&]() { return group0Seeded; });
+				}
+				auto ts = std::chrono::steady_clock::now();
+				const std::vector<uint32_t>& mine = groupReads[lane];
+				try
+				{
+					parallelFor(mine.size(), threads, [&](size_t k) { seedRead(mine[k]); });
+				}
+				catch (...)
+				{
+					if (lane == 0) signalSeeded();
+					throw;
+				}
+				gs.seedingMs += msSince(ts);
+				for (uint32_t r : mine) gs.seeds += states[r].seeds.size();
+				if (lane == 0) signalSeeded();
+			}
 			std::vector<uint32_t> active = groupReads[lane];
 			while (!active.empty())
 			{
@@ -1272,6 +1303,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 				parallelFor(active.size(), threads, [&](size_t a) {
 					ReadState& st = states[active[a]];
 					st.specs.clear();
+					if (st.done) return; // no seeds, or a letter the read's complement rejects
 					try
 					{
 						if (!st.pendingReady && !advance(st, opt))
@@ -1513,6 +1545,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		}
 		for (const AlignStats& gs : groupStats)
 		{
+			stats.seeds += gs.seeds;
 			stats.rounds = std::max(stats.rounds, gs.rounds);
 			stats.launches += gs.launches;
 			stats.reruns += gs.reruns;
@@ -1524,15 +1557,17 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		}
 		// the groups' host phases overlap: count the longest group's
 		{
-			double hostMax = 0, pickMax = 0, replayMax = 0, gafMax = 0;
+			double hostMax = 0, pickMax = 0, replayMax = 0, gafMax = 0, seedMax = 0;
 			for (const AlignStats& gs : groupStats)
 			{
+				seedMax = std::max(seedMax, gs.seedingMs);
 				hostMax = std::max(hostMax, gs.hostMs);
 				pickMax = std::max(pickMax, gs.pickMs);
 				replayMax = std::max(replayMax, gs.replayMs);
 				gafMax = std::max(gafMax, gs.gafMs);
 			}
-			stats.hostMs += hostMax;
+			stats.seedingMs += seedMax;
+			stats.hostMs += hostMax + seedMax;
 			stats.pickMs += pickMax;
 			stats.replayMs += replayMax;
 			stats.gafMs += gafMax;
