@@ -38,7 +38,7 @@ BYTES_PER_WORD_STEP = 6
 # HBM bytes per extension launch (mean) of the default run of this workload,
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy kernel's
 # readings (profiles/extend_vg5m.md)
-MEASURED_TRAFFIC_PER_LAUNCH = 5.07e9
+MEASURED_TRAFFIC_PER_LAUNCH = 5.08e9
 
 
 def parse():
