@@ -584,7 +584,7 @@ void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx
 	if (lookupMs) *lookupMs = sm.lookupMs;
 	out.assign(seqs.size(), {});
 	parallelFor(seqs.size(), 16, [&](size_t i) {
-		out[i] = seedsFromMatches(g, idx, sm.matches.data() + sm.matchOffset[i], sm.matchCount[i], lens[i], density);
+		out[i] = seedsFromMatches(g, idx, sm.data() + sm.matchOffset[i], sm.matchCount[i], lens[i], density);
 		if (order) orderSeedsByChaining(g, out[i]);
 	});
 }
@@ -815,7 +815,7 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 			const uint32_t L = rs.length[base + i];
 			try
 			{
-				seeds[i] = seedsFromMatches(g, idx, rb.matches.data() + rb.matchOffset[i], rb.matchCount[i], L, opt.minimizerSeedDensity);
+				seeds[i] = seedsFromMatches(g, idx, rb.data() + rb.matchOffset[i], rb.matchCount[i], L, opt.minimizerSeedDensity);
 				if (seeds[i].empty())
 				{
 					o.status = READ_NO_SEEDS;
@@ -1102,7 +1102,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 			ReadOutcome& o = out[batchStart + i];
 			try
 			{
-				st.seeds = seedsFromMatches(g, idx, rb.matches.data() + rb.matchOffset[i], rb.matchCount[i], blens[i], opt.minimizerSeedDensity);
+				st.seeds = seedsFromMatches(g, idx, rb.data() + rb.matchOffset[i], rb.matchCount[i], blens[i], opt.minimizerSeedDensity);
 				if (st.seeds.empty())
 				{
 					o.status = READ_NO_SEEDS;

@@ -111,9 +111,13 @@ struct ReadSet
 struct SeedMatches
 {
 	std::vector<gad::SeedMatch> matches;
+	// when set, the matches are here instead (a backend's staging memory, valid
+	// until its next lookup) and `matches` is empty
+	const gad::SeedMatch* staged = nullptr;
 	std::vector<uint64_t> matchOffset;
 	std::vector<uint32_t> matchCount;
 	double lookupMs = 0;
+	const gad::SeedMatch* data() const { return staged ? staged : matches.data(); }
 };
 
 // Workspace capacities of the first extension attempt; a job that overflows is

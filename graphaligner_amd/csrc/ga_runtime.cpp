@@ -208,6 +208,7 @@ void Context::uploadReads(const std::vector<const char*>& seqs, const std::vecto
 	revBase_ = rs.bytes + 1;
 	readBytes_.reserve(2 * (rs.bytes + 1));
 	planesReady_ = false;
+	revLo_ = revHi_ = 0;
 	readOff_.reserve(std::max<size_t>(1, n) * sizeof(uint64_t));
 	readLen_.reserve(std::max<size_t>(1, n) * sizeof(uint32_t));
 	GA_HIP(hipMemcpyAsync(readBytes_.ptr, rs.host.data(), rs.bytes + 1, hipMemcpyHostToDevice, stream_));
@@ -229,6 +230,7 @@ void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t co
 	out.matchCount.assign(count, 0);
 	out.matchOffset.assign(count, 0);
 	out.matches.clear();
+	out.staged = nullptr;
 	out.lookupMs = 0;
 	if (count == 0) return;
 	// matches of read i are written at its byte offset (at most one per base)
@@ -237,12 +239,26 @@ void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t co
 	const uint64_t* off = readOff_.as<uint64_t>() + first;
 	const uint32_t* len = readLen_.as<uint32_t>() + first;
 	GA_HIP(hipEventRecord(ev0_, stream_));
-	GA_HIP(gad::launchRevcomp(stream_, readBytes_.as<char>(), readBytes_.as<char>() + revBase_, off, len, count, 0));
+	// reverse complements (and the planes of the buffer holding them) once per
+	// uploaded read range
+	const bool revNeeded = !(first >= revLo_ && (uint64_t)first + count <= revHi_);
+	if (revNeeded) GA_HIP(gad::launchRevcomp(stream_, readBytes_.as<char>(), readBytes_.as<char>() + revBase_, off, len, count, 0));
 	GA_HIP(gad::launchSeedLookup(stream_, readBytes_.as<char>(), off, len, count, di.table.as<gad::SeedTableSlot>(), di.mask, idx.k, idx.w, idx.maxCount, matches_.as<gad::SeedMatch>(), matchCount_.as<uint32_t>()));
+	if (revNeeded)
 	{
 		// the reverse complements just written change the buffer's planes
 		std::lock_guard<std::mutex> lock(cacheMutex_);
 		computePlanes();
+		if (revHi_ > revLo_ && first <= revHi_ && (uint64_t)first + count >= revLo_)
+		{
+			revLo_ = std::min<uint64_t>(revLo_, first);
+			revHi_ = std::max<uint64_t>(revHi_, (uint64_t)first + count);
+		}
+		else
+		{
+			revLo_ = first;
+			revHi_ = (uint64_t)first + count;
+		}
 	}
 	GA_HIP(hipEventRecord(ev1_, stream_));
 	GA_HIP(hipMemcpyAsync(out.matchCount.data(), matchCount_.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
@@ -258,14 +274,15 @@ void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t co
 		out.matchOffset[i] = total;
 		total += out.matchCount[i];
 	}
-	out.matches.resize(total);
 	if (total == 0) return;
 	gathered_.reserve(total * sizeof(gad::SeedMatch));
 	matchOff_.reserve(count * sizeof(uint64_t));
 	GA_HIP(hipMemcpyAsync(matchOff_.ptr, out.matchOffset.data(), count * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
 	GA_HIP(gad::launchGatherMatches(stream_, matches_.as<gad::SeedMatch>(), off, matchCount_.as<uint32_t>(), matchOff_.as<uint64_t>(), count, gathered_.as<gad::SeedMatch>()));
-	GA_HIP(hipMemcpyAsync(out.matches.data(), gathered_.ptr, total * sizeof(gad::SeedMatch), hipMemcpyDeviceToHost, stream_));
+	matchesHost_.reserve(total * sizeof(gad::SeedMatch));
+	GA_HIP(hipMemcpyAsync(matchesHost_.ptr, gathered_.ptr, total * sizeof(gad::SeedMatch), hipMemcpyDeviceToHost, stream_));
 	GA_HIP(hipStreamSynchronize(stream_));
+	out.staged = matchesHost_.as<gad::SeedMatch>();
 }
 
 void Context::runOnce(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase, const uint64_t* devPlanes)

@@ -141,6 +141,11 @@ private:
 	DeviceBuffer arena_, gatherSrc_, gatherDst_, gatherCnt_, traceOut_; // device seed loop
 	DeviceBuffer rlSeeds_, rlJobs_, rlTraces_, rlPool_, rlAlns_, rlResults_;
 	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_, readPlanes_;
+	// host staging of a lookup's matches (SeedMatches::staged)
+	PinnedBuffer matchesHost_;
+	// resident reads whose reverse complements (and so the planes) are written:
+	// [revLo_, revHi_)
+	uint64_t revLo_ = 0, revHi_ = 0;
 	bool planesReady_ = false;
 	ReadSet reads_;
 	uint64_t revBase_ = 0;
