@@ -11,6 +11,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <sys/mman.h>
 #include "ga_align.hpp"
 #include "ga_capi_internal.hpp"
 
@@ -367,7 +368,21 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, size_t n, const Al
 		at[i + 1] = at[i] + out[i].gaf.size();
 		status[i] = out[i].status;
 	}
-	char* buf = (char*)std::malloc(at.back() + 1);
+	// large buffers on 2 MiB boundaries, marked for transparent huge pages: the
+	// copy below touches every page of a few hundred MB once (freed with free())
+	const size_t bytes = at.back() + 1;
+	char* buf = nullptr;
+	if (bytes >= ((size_t)8 << 20))
+	{
+		void* p = nullptr;
+		if (posix_memalign(&p, (size_t)2 << 20, bytes) == 0)
+		{
+			buf = (char*)p;
+			madvise(p, bytes & ~(((size_t)2 << 20) - 1), MADV_HUGEPAGE);
+		}
+	}
+	else
+		buf = (char*)std::malloc(bytes);
 	if (!buf) return GA_ENOMEM;
 	// the text runs to hundreds of MB for long reads: copy it (and take the
 	// fresh pages' first-touch faults) on several threads
