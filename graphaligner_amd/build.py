@@ -56,8 +56,11 @@ def build_test_infrastructure():
     _run(common + ["-o", os.path.join(tb, "libga_emu.so")] + srcs)
     _run(common + ["-DGA_THREAD_PER_EXTENSION", "-DGA_THREAD_HOST_TEST", "-o", os.path.join(tb, "libga_thread.so")] + srcs)
     # the product's host pipeline over a backend running the device code on host threads
-    _run(common + ["-pthread", "-o", os.path.join(tb, "libga_pipe.so"), os.path.join(ROOT, "tests", "emu", "ga_pipeline_emu.cpp")]
-         + [os.path.join(CSRC, s) for s in ("ga_align.cpp", "ga_index.cpp", "ga_graph.cpp")])
+    pipe = [os.path.join(ROOT, "tests", "emu", "ga_pipeline_emu.cpp")] + [os.path.join(CSRC, s) for s in ("ga_align.cpp", "ga_index.cpp", "ga_graph.cpp")]
+    _run(common + ["-pthread", "-o", os.path.join(tb, "libga_pipe.so")] + pipe)
+    # the same with the traceback recomputing every column it enters (all of the
+    # reference's recalcNodeWordslice checks)
+    _run(common + ["-pthread", "-DGA_FULL_RECALC_CHECKS", "-o", os.path.join(tb, "libga_pipe_fullcheck.so")] + pipe)
 
 
 def build_all():

@@ -143,6 +143,10 @@ struct Alignment
 	uint64_t alignmentLength() const { return alignmentEnd - alignmentStart; }
 };
 
+// Runs fn(0) .. fn(n - 1) on up to `threads` threads. An exception thrown by fn
+// stops the hand-out of further indices; every worker is joined and the first
+// exception is rethrown on the calling thread (so callers' handlers see it as in
+// a single-threaded loop and the process is never terminated by a worker).
 template <typename F>
 void parallelFor(size_t n, int threads, F fn)
 {
@@ -154,17 +158,40 @@ void parallelFor(size_t n, int threads, F fn)
 		return;
 	}
 	std::atomic<size_t> next{0};
+	std::atomic<bool> failed{false};
+	std::mutex errorMutex;
+	std::exception_ptr error;
 	auto work = [&]() {
-		while (true)
+		while (!failed.load(std::memory_order_relaxed))
 		{
 			size_t i = next.fetch_add(1);
 			if (i >= n) break;
-			fn(i);
+			try
+			{
+				fn(i);
+			}
+			catch (...)
+			{
+				std::lock_guard<std::mutex> lock(errorMutex);
+				if (!error) error = std::current_exception();
+				failed.store(true);
+				break;
+			}
 		}
 	};
 	std::vector<std::thread> pool;
-	for (int k = 0; k < t; k++) pool.emplace_back(work);
+	pool.reserve(t);
+	try
+	{
+		for (int k = 0; k < t; k++) pool.emplace_back(work);
+	}
+	catch (...)
+	{
+		// a thread that could not be started: the ones running finish the work
+		if (pool.empty()) throw;
+	}
 	for (auto& th : pool) th.join();
+	if (error) std::rethrow_exception(error);
 }
 
 double msSince(std::chrono::steady_clock::time_point t0)
@@ -299,12 +326,20 @@ bool alignmentIncompatible(const Alignment& left, const Alignment& right)
 	return overlap > minOverlapLen;
 }
 
-// AlignmentSelection::SelectAlignments (src/AlignmentSelection.cpp:60-95)
+// AlignmentSelection::SelectAlignments (src/AlignmentSelection.cpp:31-53, 65-95).
+// The score threshold passes through SelectionOptions' int field
+// (src/AlignmentSelection.h:21, src/Aligner.cpp:402) before the comparison.
 std::vector<Alignment> selectAlignments(std::vector<Alignment>& all, double minScore, double fraction)
 {
 	std::vector<Alignment> kept;
-	for (auto& a : all)
-		if (a.alignmentXScore >= minScore) kept.push_back(std::move(a));
+	const int32_t threshold = selectionScoreThreshold(minScore);
+	if (threshold > INT32_MIN)
+	{
+		for (auto& a : all)
+			if (a.alignmentXScore >= threshold) kept.push_back(std::move(a));
+	}
+	else
+		for (auto& a : all) kept.push_back(std::move(a));
 	if (fraction == 0) return kept;
 	std::vector<bool> skip(kept.size(), false);
 	for (size_t i = 0; i < kept.size(); i++)
@@ -535,14 +570,29 @@ std::vector<SeedHit> seedsFromMatches(const GraphHost& g, const MinimizerIndex& 
 	return result;
 }
 
-void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds)
+namespace {
+
+// GraphAligner::orderSeedsByChaining (src/GraphAligner.h:250-312) with the
+// reference's assertions as ReadAssert.
+void orderSeedsChecked(const GraphHost& g, std::vector<SeedHit>& seeds)
 {
-	// seeds grouped by chain, each with its approximate diagonal
+	// seeds grouped by chain, each with its approximate diagonal; the chains are
+	// independent, so the container's order does not matter
 	std::map<uint64_t, std::vector<std::pair<uint64_t, uint64_t>>> byChain;
 	for (size_t i = 0; i < seeds.size(); i++)
 	{
 		uint64_t node = seeds[i].alignmentGraphNodeId;
 		uint64_t off = seeds[i].alignmentGraphNodeOffset;
+		if (node == kNoPos)
+		{
+			// a seed from outside the minimizer seeder: its split node from the
+			// bigraph position
+			const int forwardId = seeds[i].reverse ? seeds[i].nodeID * 2 + 1 : seeds[i].nodeID * 2;
+			node = g.unitigNode(forwardId, seeds[i].nodeOffset);
+			check(seeds[i].nodeOffset >= g.nodeOff[node]);
+			off = seeds[i].nodeOffset - g.nodeOff[node];
+		}
+		check(g.chainPos[node] + off >= seeds[i].seqPos);
 		byChain[g.chain[node]].emplace_back(i, g.chainPos[node] + off - seeds[i].seqPos);
 	}
 	for (auto& c : byChain)
@@ -553,6 +603,7 @@ void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds)
 		for (size_t i = 1; i <= v.size(); i++)
 		{
 			if (i < v.size() && v[i].second <= v[i - 1].second + 100) continue;
+			check(i > clusterStart);
 			std::sort(v.begin() + clusterStart, v.begin() + i, [&seeds](const std::pair<uint64_t, uint64_t>& l, const std::pair<uint64_t, uint64_t>& r) { return seeds[l.first].seqPos < seeds[r.first].seqPos; });
 			uint64_t matchingBps = 0;
 			int lastEnd = INT_MIN;
@@ -561,6 +612,8 @@ void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds)
 				const SeedHit& s = seeds[v[k].first];
 				int thisStart = (int)s.seqPos - (int)s.matchLen + 1;
 				int thisEnd = (int)s.seqPos;
+				check(thisEnd >= lastEnd);
+				check(thisEnd > thisStart);
 				matchingBps += (thisEnd - std::max(thisStart, lastEnd));
 				lastEnd = thisEnd;
 			}
@@ -576,16 +629,32 @@ void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds)
 	std::reverse(seeds.begin(), seeds.end());
 }
 
-void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs)
+} // namespace
+
+bool orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds)
+{
+	try
+	{
+		orderSeedsChecked(g, seeds);
+		return true;
+	}
+	catch (const ReadAssert&)
+	{
+		return false;
+	}
+}
+
+void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs, std::vector<int32_t>* status)
 {
 	ctx.uploadReads(seqs, lens);
 	SeedMatches sm;
 	ctx.lookupSeeds(idx, 0, (uint32_t)seqs.size(), sm);
 	if (lookupMs) *lookupMs = sm.lookupMs;
 	out.assign(seqs.size(), {});
+	if (status) status->assign(seqs.size(), 0);
 	parallelFor(seqs.size(), 16, [&](size_t i) {
 		out[i] = seedsFromMatches(g, idx, sm.data() + sm.matchOffset[i], sm.matchCount[i], lens[i], density);
-		if (order) orderSeedsByChaining(g, out[i]);
+		if (order && !orderSeedsByChaining(g, out[i]) && status) (*status)[i] = READ_ASSERT;
 	});
 }
 
@@ -821,7 +890,7 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 					o.status = READ_NO_SEEDS;
 					return;
 				}
-				orderSeedsByChaining(g, seeds[i]);
+				orderSeedsChecked(g, seeds[i]);
 				// ReverseComplement of the read asserts on letters it does not know
 				for (uint32_t k = 0; k < L; k++) check(complementOf(seq[k]) != 0);
 				ready[i] = 1;
@@ -1065,8 +1134,13 @@ uint32_t speculationWidth(uint32_t round, uint32_t maxSeeds, size_t activeReads,
 void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
 {
 	auto t0 = std::chrono::steady_clock::now();
-	// GA_ROUND_LOG=1 prints one line per round and group to stderr
+	// built with -DGA_DEBUG_LOG, GA_ROUND_LOG=1 prints one line per round and
+	// group to stderr; the default build has no logging
+#ifdef GA_DEBUG_LOG
 	const bool roundLog = std::getenv("GA_ROUND_LOG") != nullptr;
+#else
+	constexpr bool roundLog = false;
+#endif
 	const ReadSet& rs = ctx.reads();
 	if ((uint64_t)first + count > rs.size()) throw std::runtime_error("read range outside the resident reads");
 	stats = AlignStats{};
@@ -1109,7 +1183,7 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 					st.done = true;
 					return;
 				}
-				orderSeedsByChaining(g, st.seeds);
+				orderSeedsChecked(g, st.seeds);
 				// ReverseComplement of the read asserts on letters it does not know
 				for (uint32_t k = 0; k < blens[i]; k++) check(complementOf(bseqs[i][k]) != 0);
 				st.extendSeeds = opt.seedExtendDensity == -1 ? st.seeds.size() : (uint64_t)(opt.seedExtendDensity * blens[i] + 1);
@@ -1487,31 +1561,6 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 						             jobs.empty() ? 0u : jobs[top].len, jobs.empty() ? 0ull : (unsigned long long)eb.results[top].wordSteps,
 						             jobs.empty() ? 0ull : (unsigned long long)eb.results[top].nodeCalcs, jobs.empty() ? 0u : eb.results[top].nslices,
 						             jobs.empty() ? 0u : eb.results[top].maxBand, jobs.empty() ? 0u : eb.results[top].ntrace);
-						// GA_HEAVY_DUMP=path appends it as "sequence node offset" (graph node
-						// id, offset in the node) for offline timing of single extensions
-						if (const char* dump = std::getenv("GA_HEAVY_DUMP"); dump && !jobs.empty())
-						{
-							std::lock_guard<std::mutex> lock(phaseMutex);
-							if (FILE* f = std::fopen(dump, "a"))
-							{
-								const ExtJob& jt = jobs[top];
-								std::string seq(jt.len, 'N');
-								if (jt.devOffset < ctx.reverseBase())
-									seq.assign(rs.host.data() + jt.devOffset, jt.len);
-								else
-								{
-									// reverse strand: the reverse complement buffer has the
-									// forward layout
-									const uint64_t off = jt.devOffset - ctx.reverseBase();
-									const size_t r = std::upper_bound(rs.offset.begin(), rs.offset.end(), off) - rs.offset.begin() - 1;
-									const uint64_t k = off - rs.offset[r], L = rs.length[r];
-									for (uint32_t i = 0; i < jt.len; i++) seq[i] = complementOf(rs.seq(r)[L - 1 - (k + i)]);
-								}
-								std::fprintf(f, "%s %d %llu\n", seq.c_str(), g.nodeId[jt.startNode],
-								             (unsigned long long)(g.nodeOff[jt.startNode] + jt.startOffset));
-								std::fclose(f);
-							}
-						}
 					}
 				}
 				std::vector<uint32_t> still, ended;

@@ -23,8 +23,8 @@ struct SeedHit
 	uint64_t seqPos = 0;            // read position of the seed's last base
 	uint64_t matchLen = 0;
 	bool reverse = false;
-	uint64_t alignmentGraphNodeId = 0;
-	uint64_t alignmentGraphNodeOffset = 0;
+	uint64_t alignmentGraphNodeId = ~0ull;   // ~0: not set (src/GraphAlignerWrapper.h:16-27)
+	uint64_t alignmentGraphNodeOffset = ~0ull;
 	uint64_t rawSeedGoodness = 0;
 	uint64_t seedGoodness = 0;
 	uint64_t seedClusterSize = 0;
@@ -100,15 +100,33 @@ struct AlignStats
 	double seedingMs = 0, pickMs = 0, replayMs = 0, gafMs = 0;
 };
 
+// The selection's score threshold: --min-alignment-score is a double but
+// SelectionOptions holds it as an int (src/AlignmentSelection.h:21, assigned at
+// src/Aligner.cpp:402), so it is truncated toward zero; a value the int cannot
+// hold (or NaN) converts to INT32_MIN as the x86-64 conversion instruction
+// gives it. SelectAlignments filters only when the threshold is above INT32_MIN
+// (src/AlignmentSelection.cpp:36).
+inline int32_t selectionScoreThreshold(double v)
+{
+	if (!(v > -2147483649.0 && v < 2147483648.0)) return INT32_MIN;
+	return (int32_t)v;
+}
+
 // MinimizerSeeder::getSeeds (src/MinimizerSeeder.cpp:521-555) from the device
 // lookup's matches of one read.
 std::vector<SeedHit> seedsFromMatches(const GraphHost& g, const MinimizerIndex& idx, const gad::SeedMatch* matches, uint32_t nmatches, uint64_t readLen, double density);
 
-// GraphAligner::orderSeedsByChaining (src/GraphAligner.h:250-312).
-void orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds);
+// GraphAligner::orderSeedsByChaining (src/GraphAligner.h:250-312). Returns
+// false when one of the reference's assertions there fires (the reference then
+// drops the read); the seeds are left partly updated. A seed whose
+// alignmentGraphNodeId is ~0 (not from the minimizer seeder) is placed by its
+// bigraph node and offset.
+bool orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds);
 
-// Seeds of every read as the reference's seeder + OrderSeeds produce them.
-void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs);
+// Seeds of every read as the reference's seeder + OrderSeeds produce them;
+// status (optional) is READ_ASSERT for a read whose OrderSeeds fired a reference
+// assertion, else 0.
+void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs, std::vector<int32_t>* status = nullptr);
 
 // src/Aligner.cpp:390-606 for resident reads [first, first + count) of the
 // backend (one GAF line per selected alignment, '\n'-terminated, in out[i].gaf;

@@ -285,6 +285,7 @@ ga_status_t ga_seeds(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_
 {
 	if (!ctx || !graph || !index || !seeds || !counts || index->graph != graph || !readArgsOk(n, seqs, lens)) return GA_EBADARG;
 	*seeds = nullptr;
+	ctx->residentFromUpload = false;
 	try
 	{
 		std::vector<const char*> s(seqs, seqs + n);
@@ -450,6 +451,7 @@ ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_
 	for (int64_t i = 0; i < n; i++)
 		if (!names[i]) return GA_EBADARG;
 	*gaf = nullptr;
+	ctx->residentFromUpload = false;
 	return guarded([&]() {
 		std::vector<const char*> nm(names, names + n), s(seqs, seqs + n);
 		std::vector<uint32_t> l(n);
@@ -474,11 +476,13 @@ ga_status_t ga_context_synchronize(ga_context_t* ctx)
 ga_status_t ga_reads_upload(ga_context_t* ctx, int64_t n, const char* const* seqs, const int64_t* lens)
 {
 	if (!ctx || !readArgsOk(n, seqs, lens)) return GA_EBADARG;
+	ctx->residentFromUpload = false;
 	return guarded([&]() {
 		std::vector<const char*> s(seqs, seqs + n);
 		std::vector<uint32_t> l(n);
 		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
 		ctx->ctx->uploadReads(s, l);
+		ctx->residentFromUpload = true;
 		return GA_OK;
 	});
 }
@@ -486,12 +490,17 @@ ga_status_t ga_reads_upload(ga_context_t* ctx, int64_t n, const char* const* seq
 ga_status_t ga_align_resident(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t first, int64_t count, const char* const* names, char** gaf, int32_t* status, ga_align_stats_t* stats)
 {
 	if (!ctx || !graph || !index || !params || !gaf || !status || index->graph != graph || first < 0 || count < 0) return GA_EBADARG;
+	if (!ctx->residentFromUpload) return GA_ESTALE;
 	if ((uint64_t)(first + count) > ctx->ctx->reads().size() || (count > 0 && !names)) return GA_EBADARG;
 	for (int64_t i = 0; i < count; i++)
 		if (!names[i]) return GA_EBADARG;
 	*gaf = nullptr;
 	return guarded([&]() {
+#ifdef GA_DEBUG_LOG
 		const bool log = std::getenv("GA_ROUND_LOG") != nullptr;
+#else
+		constexpr bool log = false;
+#endif
 		auto t0 = std::chrono::steady_clock::now();
 		auto since = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
 		std::vector<ReadOutcome>& out = ctx->outcomes;

@@ -23,6 +23,11 @@ struct ga_context
 	std::unique_ptr<ga::Context> ctx;
 	// per-read outcomes of the last call, kept so their text buffers are reused
 	std::vector<ga::ReadOutcome> outcomes;
+	// true while the device read set is the one ga_reads_upload loaded; calls
+	// that upload their own reads (ga_seeds, ga_align_reads, ga_align_one_way)
+	// replace it and clear this, so ga_align_resident fails with GA_ESTALE
+	// instead of aligning other reads under the caller's names
+	bool residentFromUpload = false;
 };
 
 namespace ga {

@@ -253,116 +253,83 @@ GA_HD bool eqFromPlanes(const uint64_t* planes, uint64_t x0, uint32_t rows, Eq4&
 // ---------------------------------------------------------------------------
 #if defined(GA_THREAD_PER_EXTENSION)
 
-// differenceMasksBitTwiddle (src/WordSlice.h:566-664): rows where left < right and
-// rows where right < left, given scoreDifference = right.before - left.before
-GA_COLD void diffMasks(uint64_t leftVP, uint64_t leftVN, uint64_t rightVP, uint64_t rightVN, int scoreDifference, uint64_t& leftSmaller, uint64_t& rightSmaller)
+// Rows 0..63 where column a scores below column b (lt) and above it (gt): the
+// result of differenceMasks (src/WordSlice.h:546-664), computed here as a sweep
+// over the rows where the difference d(r) = a(r) - b(r) changes. d(-1) is the
+// difference of the scores before row 0; at row r it moves by a's vertical
+// delta minus b's, one of -2..2, so its sign is constant between two such rows.
+// The sweep stops early once the rises (falls) still ahead cannot bring a
+// negative (positive) difference back to zero.
+GA_HD void compareColumns(const Slice& a, const Slice& b, uint64_t& lt, uint64_t& gt)
 {
-	leftSmaller = 0;
-	rightSmaller = 0;
-	uint64_t VPcommon = ~(leftVP & rightVP);
-	uint64_t VNcommon = ~(leftVN & rightVN);
-	leftVP &= VPcommon;
-	leftVN &= VNcommon;
-	rightVP &= VPcommon;
-	rightVN &= VNcommon;
-	uint64_t twosmaller = leftVN & rightVP;
-	uint64_t onesmaller = (rightVP & ~leftVN) | (leftVN & ~rightVP);
-	uint64_t onebigger = (leftVP & ~rightVN) | (rightVN & ~leftVP);
-	uint64_t twobigger = rightVN & leftVP;
-	onebigger |= twobigger;
-	onesmaller |= twosmaller;
-	if (scoreDifference > 0)
+	// d rises with a's +1 rows and b's -1 rows, falls with a's -1 and b's +1 rows
+	const uint64_t up2 = a.VP & b.VN, dn2 = a.VN & b.VP;
+	const uint64_t up1 = up2 | (a.VP & ~b.VP) | (b.VN & ~a.VN);
+	const uint64_t dn1 = dn2 | (a.VN & ~b.VN) | (b.VP & ~a.VP);
+	// (up1 and dn1 never share a row: a row with a rise has no fall of its own)
+	int32_t d = before(a) - before(b);
+	int32_t upLeft = popc(up1) + popc(up2), dnLeft = popc(dn1) + popc(dn2);
+	uint64_t events = up1 | dn1;
+	lt = 0;
+	gt = 0;
+	uint32_t start = 0; // first row whose difference is d
+	while (start < 64)
 	{
-		for (int i = 1; i < scoreDifference; i++)
+		if (d > dnLeft)
 		{
-			uint64_t ls = onebigger & ~(onebigger - 1);
-			onebigger ^= (~twobigger & ls);
-			twobigger &= ~ls;
-			if (onebigger == 0)
-			{
-				leftSmaller = ONES;
-				rightSmaller = 0;
-				return;
-			}
+			gt |= ONES << start;
+			return;
 		}
-		uint64_t ls = onebigger & ~(onebigger - 1);
-		leftSmaller |= ls - 1;
-		onebigger ^= (~twobigger & ls);
-		twobigger &= ~ls;
-	}
-	else if (scoreDifference < 0)
-	{
-		for (int i = 1; i < -scoreDifference; i++)
+		if (-d > upLeft)
 		{
-			uint64_t ls = onesmaller & ~(onesmaller - 1);
-			onesmaller ^= (~twosmaller & ls);
-			twosmaller &= ~ls;
-			if (onesmaller == 0)
-			{
-				leftSmaller = 0;
-				rightSmaller = ONES;
-				return;
-			}
+			lt |= ONES << start;
+			return;
 		}
-		uint64_t ls = onesmaller & ~(onesmaller - 1);
-		rightSmaller |= ls - 1;
-		onesmaller ^= (~twosmaller & ls);
-		twosmaller &= ~ls;
-	}
-	for (int i = 0; i < 64; i++)
-	{
-		if (onesmaller == 0)
-		{
-			if (onebigger == 0) break;
-			uint64_t ls = onebigger & ~(onebigger - 1);
-			rightSmaller |= -ls;
-			break;
-		}
-		if (onebigger == 0)
-		{
-			uint64_t ls = onesmaller & ~(onesmaller - 1);
-			leftSmaller |= -ls;
-			break;
-		}
-		uint64_t lb = onebigger & ~(onebigger - 1);
-		uint64_t lsm = onesmaller & ~(onesmaller - 1);
-		if (lb > lsm) leftSmaller |= lb - lsm;
-		else rightSmaller |= lsm - lb;
-		onebigger ^= (~twobigger & lb);
-		twobigger &= ~lb;
-		onesmaller ^= (~twosmaller & lsm);
-		twosmaller &= ~lsm;
+		const uint32_t r = events ? (uint32_t)__builtin_ctzll(events) : 64;
+		const uint64_t span = (r >= 64 ? ONES : (((uint64_t)1 << r) - 1)) & (ONES << start);
+		if (d > 0) gt |= span;
+		else if (d < 0) lt |= span;
+		if (r >= 64) return;
+		const uint64_t bit = (uint64_t)1 << r;
+		const int32_t rise = (int32_t)((up1 & bit) != 0) + (int32_t)((up2 & bit) != 0);
+		const int32_t fall = (int32_t)((dn1 & bit) != 0) + (int32_t)((dn2 & bit) != 0);
+		d += rise - fall;
+		upLeft -= rise;
+		dnLeft -= fall;
+		events &= events - 1;
+		start = r;
 	}
 }
 
-// mergeTwoSlices (src/WordSlice.h:502-541)
-GA_HD Slice lpMerge(Slice left, Slice right)
+// Cellwise minimum of two columns (WordSlice::mergeWith, src/WordSlice.h:171-175),
+// rebuilt from the comparison masks: where the same column is the smaller (ties
+// to a) at rows r - 1 and r, row r keeps that column's vertical delta; where the
+// smaller column switches between two rows that differ by a strict inequality
+// on both sides, the minimum does not move (the deltas are confined to -1..1);
+// where it switches across a tie, row r takes the delta of the column that is
+// (also) smallest at both rows, b.
+GA_HD Slice lpMerge(const Slice& a, const Slice& b)
 {
-	if (before(left) > before(right))
-	{
-		Slice t = left;
-		left = right;
-		right = t;
-	}
-	uint64_t leftSmaller, rightSmaller;
-	diffMasks(left.VP, left.VN, right.VP, right.VN, before(right) - before(left), leftSmaller, rightSmaller);
-	uint64_t mask = (rightSmaller | ((leftSmaller | rightSmaller) - (rightSmaller << 1))) & ~leftSmaller;
-	uint64_t leftReduction = leftSmaller & (rightSmaller << 1);
-	uint64_t rightReduction = rightSmaller & (leftSmaller << 1);
-	if ((rightSmaller & 1) && before(left) < before(right)) rightReduction |= 1;
-	left.VN &= ~leftReduction;
-	right.VN &= ~rightReduction;
+	uint64_t lt, gt;
+	compareColumns(a, b, lt, gt);
+	const int32_t ba = before(a), bb = before(b);
+	const uint64_t eq = ~(lt | gt);
+	const uint64_t gtPrev = (gt << 1) | (uint64_t)(bb < ba);
+	const uint64_t ltPrev = (lt << 1) | (uint64_t)(ba < bb);
+	const uint64_t useB = (gt & ~ltPrev) | (gtPrev & eq);
+	const uint64_t flat = (gt & ltPrev) | (gtPrev & lt);
+	const uint64_t useA = ~(useB | flat);
 	Slice r;
-	r.VN = (left.VN & ~mask) | (right.VN & mask);
-	r.VP = (left.VP & ~mask) | (right.VP & mask);
-	r.end = imin(left.end, right.end);
+	r.VP = (useA & a.VP) | (useB & b.VP);
+	r.VN = (useA & a.VN) | (useB & b.VN);
+	r.end = imin(a.end, b.end);
 	return r;
 }
 GA_HD uint64_t lpLessMask(const Slice& a, const Slice& b)
 {
-	uint64_t ls, rs;
-	diffMasks(a.VP, a.VN, b.VP, b.VN, before(b) - before(a), ls, rs);
-	return ls;
+	uint64_t lt, gt;
+	compareColumns(a, b, lt, gt);
+	return lt;
 }
 // minimum over the rows of mask, walking from one selected row to the next
 GA_HD int32_t lpMinOver(const Slice& s, uint64_t mask)
@@ -1932,9 +1899,17 @@ struct Extension
 	// recalcNodeWordslice (src/GraphAlignerBitvectorCommon.h:1078-1102). With
 	// lastCol, only columns 0..lastCol (those a backtrace entering the node at
 	// column lastCol reads); the node's end slice is then not compared with the
-	// stored one.
+	// stored one (:1097-1099) and the end-versus-previous-slice check of the
+	// column loop is not made. Both checks compare a recomputation with values
+	// the forward pass computed the same way, so they cannot fail without a
+	// fault in this code; the host test build with GA_FULL_RECALC_CHECKS
+	// recomputes every column and keeps both, and its GAF and read statuses are
+	// compared with this build's.
 	GA_HD void recalc(uint32_t sliceIdx, uint32_t node, const Eq4& eq, ColumnStore& cols, uint32_t lastCol = NIL)
 	{
+#ifdef GA_FULL_RECALC_CHECKS
+		lastCol = NIL;
+#endif
 		int32_t ci = findInSlice(sliceIdx, node);
 		if (ci < 0)
 		{

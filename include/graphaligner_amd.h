@@ -27,7 +27,8 @@ typedef enum
 	GA_EIO = 5,
 	GA_ENODEVICE = 6,
 	GA_EHIP = 7,
-	GA_ENOMEM = 8
+	GA_ENOMEM = 8,
+	GA_ESTALE = 9       /* ga_align_resident after another call replaced the uploaded read set */
 } ga_status_t;
 
 typedef struct ga_graph ga_graph_t;

@@ -1289,6 +1289,8 @@ struct Params
 	ScoreType XscoreErrorCost = 294;
 	int Xdropcutoff = 14705;
 	double multimapScoreFraction = 0.9;
+	// --min-alignment-score (src/AlignerMain.cpp:151,213), as the double option
+	double minAlignmentScore = 0;
 	// src/GraphAlignerCommon.h:89-100
 	static ScoreType errorCostFromIdentity(double preciseClippingIdentityCutoff)
 	{
@@ -2441,12 +2443,15 @@ struct Aligner
 			if (seedHits[i].alignmentGraphNodeId == NMAX)
 			{
 				nodeIndex = graph.GetUnitigNode(forwardNodeId, seedHits[i].nodeOffset);
+				ORC_ASSERT(seedHits[i].nodeOffset >= graph.nodeOffset[nodeIndex]);
 				realOffset = seedHits[i].nodeOffset - graph.nodeOffset[nodeIndex];
+				ORC_ASSERT(graph.chainApproxPos[nodeIndex] + realOffset >= seedHits[i].seqPos);
 			}
 			else
 			{
 				nodeIndex = seedHits[i].alignmentGraphNodeId;
 				realOffset = seedHits[i].alignmentGraphNodeOffset;
+				ORC_ASSERT(graph.chainApproxPos[nodeIndex] + realOffset >= seedHits[i].seqPos);
 			}
 			seedPoses[graph.chainNumber[nodeIndex]].emplace_back(i, graph.chainApproxPos[nodeIndex] + realOffset - seedHits[i].seqPos);
 		}
@@ -2456,7 +2461,9 @@ struct Aligner
 			size_t clusterStart = 0;
 			for (size_t i = 1; i <= pair.second.size(); i++)
 			{
+				ORC_ASSERT(i == pair.second.size() || pair.second[i].second >= pair.second[i - 1].second);
 				if (i < pair.second.size() && pair.second[i].second <= pair.second[i - 1].second + 100) continue;
+				ORC_ASSERT(i > clusterStart);
 				std::sort(pair.second.begin() + clusterStart, pair.second.begin() + i, [&seedHits](std::pair<size_t, size_t> l, std::pair<size_t, size_t> r) { return seedHits[l.first].seqPos < seedHits[r.first].seqPos; });
 				size_t matchingBps = 0;
 				int lastEnd = std::numeric_limits<int>::min();
@@ -2464,6 +2471,8 @@ struct Aligner
 				{
 					int thisStart = (int)seedHits[pair.second[k].first].seqPos - (int)seedHits[pair.second[k].first].matchLen + 1;
 					int thisEnd = (int)seedHits[pair.second[k].first].seqPos;
+					ORC_ASSERT(thisEnd >= lastEnd);
+					ORC_ASSERT(thisEnd > thisStart);
 					matchingBps += (thisEnd - std::max(thisStart, lastEnd));
 					lastEnd = thisEnd;
 				}
@@ -2867,11 +2876,26 @@ inline bool alignmentIncompatible(const AlignmentItem& left, const AlignmentItem
 	return overlap > minOverlapLen;
 }
 
-inline std::vector<AlignmentItem> SelectAlignments(const std::vector<AlignmentItem>& all, double minAlignmentScore, double fraction)
+// src/AlignmentSelection.cpp:31-53: SelectionOptions::minAlignmentScore is an int
+// (src/AlignmentSelection.h:21) assigned from the double option at
+// src/Aligner.cpp:402; out-of-range values and NaN become INT_MIN (what
+// cvttsd2si returns on x86-64), and the score filter runs only above INT_MIN.
+inline int SelectionMinScore(double v)
+{
+	if (std::isnan(v) || v >= 2147483648.0 || v <= -2147483649.0) return std::numeric_limits<int>::min();
+	return (int)v;
+}
+
+inline std::vector<AlignmentItem> SelectAlignments(const std::vector<AlignmentItem>& all, double minAlignmentScoreOption, double fraction)
 {
 	std::vector<AlignmentItem> filtered;
-	for (const auto& a : all)
-		if (a.alignmentXScore >= minAlignmentScore) filtered.push_back(a);
+	const int minAlignmentScore = SelectionMinScore(minAlignmentScoreOption);
+	if (minAlignmentScore > std::numeric_limits<int>::min())
+	{
+		for (const auto& a : all)
+			if (a.alignmentXScore >= (double)minAlignmentScore) filtered.push_back(a);
+	}
+	else filtered = all;
 	if (fraction != 0)
 	{
 		std::vector<AlignmentItem> result;
@@ -3189,7 +3213,7 @@ inline ReadOutcome alignRead(const AlignmentGraph& graph, const MinimizerIndex& 
 		out.status = 2;
 		return out;
 	}
-	if (alignments.size() > 0) alignments = SelectAlignments(alignments, 0, params.multimapScoreFraction);
+	if (alignments.size() > 0) alignments = SelectAlignments(alignments, params.minAlignmentScore, params.multimapScoreFraction);
 	if (alignments.size() == 0)
 	{
 		out.status = 3;

@@ -20,6 +20,7 @@ struct OrcParams
 	double multimapScoreFraction;
 	int32_t sloppy;
 	int32_t minClusterSize;
+	double minAlignmentScore;
 };
 
 static Params toParams(const OrcParams* p)
@@ -33,6 +34,7 @@ static Params toParams(const OrcParams* p)
 	r.multimapScoreFraction = p->multimapScoreFraction;
 	r.sloppyOptimizations = p->sloppy != 0;
 	r.minSeedClusterSize = p->minClusterSize;
+	r.minAlignmentScore = p->minAlignmentScore;
 	return r;
 }
 
@@ -134,7 +136,15 @@ int orc_get_seeds(void* g, void* idx, const char* seq, int64_t len, double densi
 	{
 		Params p;
 		Aligner a(p, *(AlignmentGraph*)g);
-		a.orderSeedsByChaining(seeds);
+		try
+		{
+			a.orderSeedsByChaining(seeds);
+		}
+		catch (const AssertFail&)
+		{
+			*n = 0;
+			return 2;
+		}
 	}
 	*n = seeds.size();
 	if ((int64_t)seeds.size() > cap) return 1;

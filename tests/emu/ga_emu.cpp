@@ -90,4 +90,24 @@ int emu_extend(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int3
 	}
 	return out.status;
 }
+
+// Column primitives of the device code, for property tests against brute force.
+void emu_merge(uint64_t avp, uint64_t avn, int32_t aend, uint64_t bvp, uint64_t bvn, int32_t bend, uint64_t* vp, uint64_t* vn, int32_t* end)
+{
+	gad::Slice r = gad::lpMerge(gad::mkslice(avp, avn, aend), gad::mkslice(bvp, bvn, bend));
+	*vp = r.VP;
+	*vn = r.VN;
+	*end = r.end;
+}
+
+uint64_t emu_less_mask(uint64_t avp, uint64_t avn, int32_t aend, uint64_t bvp, uint64_t bvn, int32_t bend)
+{
+	return gad::lpLessMask(gad::mkslice(avp, avn, aend), gad::mkslice(bvp, bvn, bend));
+}
+
+int32_t emu_max_x(uint64_t vp, uint64_t vn, int32_t end, int32_t seqOffset, int32_t cost, int32_t cells, uint64_t extraMask)
+{
+	gad::Slice s = gad::mkslice(vp, vn, end);
+	return gad::lpMaxX(s, seqOffset, cost, gad::maxXCandidates(s, cells, extraMask));
+}
 }

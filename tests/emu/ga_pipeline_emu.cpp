@@ -20,6 +20,11 @@
 
 namespace {
 
+// failure injection for the error-path tests: 1 drops every escaped trace item
+// of an extension batch (the replay then meets a trace it cannot decode), 2
+// reports a seed match outside the index's position array
+int g_fault = 0;
+
 template <typename F>
 void forEachParallel(size_t n, F fn)
 {
@@ -176,6 +181,7 @@ public:
 		});
 		for (size_t i = 0; i < n; i++) out.wordSteps += out.results[i].wordSteps;
 		ga::encodeTraces(G, traces, codeStore_[lane], out);
+		if (g_fault == 1) out.escapes.clear();
 		out.launches++;
 		out.reruns = reruns.load();
 	}
@@ -231,6 +237,7 @@ public:
 		}
 		out.matches.resize(total);
 		for (size_t i = 0; i < count; i++) std::memcpy(out.matches.data() + out.matchOffset[i], all.data() + rs_.offset[first + i], out.matchCount[i] * sizeof(gad::SeedMatch));
+		if (g_fault == 2 && total > 0) out.matches[total - 1].start = (uint32_t)idx.positions.size();
 		out.lookupMs = 0;
 		computePlanes();
 	}
@@ -292,8 +299,14 @@ int pipe_get_seeds(void* ih, const char* seq, int64_t len, double density, int o
 	auto* x = (EmuIndex*)ih;
 	HostBackend be;
 	std::vector<std::vector<ga::SeedHit>> seeds;
-	ga::seedReads(be, *x->g, x->idx, density, order != 0, {seq}, {(uint32_t)len}, seeds, nullptr);
+	std::vector<int32_t> status;
+	ga::seedReads(be, *x->g, x->idx, density, order != 0, {seq}, {(uint32_t)len}, seeds, nullptr, &status);
 	*n = seeds[0].size();
+	if (status[0] != 0)
+	{
+		*n = 0;
+		return 2;
+	}
 	if ((int64_t)seeds[0].size() > cap) return 1;
 	for (size_t i = 0; i < seeds[0].size(); i++)
 	{
@@ -327,7 +340,10 @@ struct PipeOptions
 	int64_t minClusterSize;
 	int32_t deviceLoop;
 	int32_t maxSpeculativeSeeds;
+	double minAlignmentScore;
 };
+
+void pipe_set_fault(int f) { g_fault = f; }
 
 int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char* const* names, const char* const* seqs, const int64_t* lens, char** gafOut, int32_t* statusOut, int64_t* statsOut)
 {
@@ -347,6 +363,7 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 	o.hostThreads = 4;
 	o.deviceSeedLoop = po->deviceLoop != 0;
 	o.maxSpeculativeSeeds = po->maxSpeculativeSeeds < 1 ? 1 : (uint32_t)po->maxSpeculativeSeeds;
+	o.minAlignmentScore = po->minAlignmentScore;
 	std::vector<const char*> nm(names, names + nreads), sq(seqs, seqs + nreads);
 	std::vector<uint32_t> ln(nreads);
 	for (int64_t i = 0; i < nreads; i++) ln[i] = (uint32_t)lens[i];

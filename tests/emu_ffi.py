@@ -23,6 +23,12 @@ def lib(mode="wave"):
                                  ctypes.c_int64, ctypes.c_int, ctypes.c_int64, i64p, ctypes.c_int64, i64p,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(SliceSummary), ctypes.c_int64, i64p,
                                  ctypes.POINTER(ctypes.c_uint64)]
+        u64 = ctypes.c_uint64
+        L.emu_merge.argtypes = [u64, u64, ctypes.c_int32, u64, u64, ctypes.c_int32, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int32)]
+        L.emu_less_mask.restype = u64
+        L.emu_less_mask.argtypes = [u64, u64, ctypes.c_int32, u64, u64, ctypes.c_int32]
+        L.emu_max_x.restype = ctypes.c_int32
+        L.emu_max_x.argtypes = [u64, u64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, u64]
         _loaded[mode] = L
     return _loaded[mode]
 

@@ -25,6 +25,7 @@ class OrcParams(ctypes.Structure):
         ("multimapScoreFraction", ctypes.c_double),
         ("sloppy", ctypes.c_int32),
         ("minClusterSize", ctypes.c_int32),
+        ("minAlignmentScore", ctypes.c_double),
     ]
 
 
@@ -173,6 +174,8 @@ class Index:
         n = ctypes.c_int64()
         rc = self.L.orc_get_seeds(self.graph.h, self.h, seq.encode(), len(seq), density, int(order),
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(n))
+        if rc == 2:
+            return None  # OrderSeeds fired a reference assertion: the read is dropped
         assert rc == 0
         return out[: 10 * n.value].reshape(-1, 10)
 

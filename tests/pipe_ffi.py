@@ -15,13 +15,20 @@ class PipeOptions(ctypes.Structure):
                 ("sloppy", ctypes.c_int32), ("maxCellsPerSlice", ctypes.c_int64),
                 ("seedExtendDensity", ctypes.c_double), ("multimapScoreFraction", ctypes.c_double),
                 ("seedDensity", ctypes.c_double), ("minClusterSize", ctypes.c_int64),
-                ("deviceLoop", ctypes.c_int32), ("maxSpeculativeSeeds", ctypes.c_int32)]
+                ("deviceLoop", ctypes.c_int32), ("maxSpeculativeSeeds", ctypes.c_int32),
+                ("minAlignmentScore", ctypes.c_double)]
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        L = ctypes.CDLL(os.path.join(ROOT, "tests", "_build", "libga_pipe.so"))
+# library variants: the default build, and one whose traceback recomputes every
+# column of a node it enters (so the end-slice and boundary checks the default
+# skips for the columns it does not read still run)
+VARIANTS = {"": "libga_pipe.so", "fullcheck": "libga_pipe_fullcheck.so"}
+_LIBS = {}
+
+
+def lib(variant=""):
+    if variant not in _LIBS:
+        L = ctypes.CDLL(os.path.join(ROOT, "tests", "_build", VARIANTS[variant]))
         L.pipe_graph_load.restype = ctypes.c_void_p
         L.pipe_graph_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.pipe_graph_free.argtypes = [ctypes.c_void_p]
@@ -41,47 +48,54 @@ def lib():
                                        ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
         L.pipe_free.argtypes = [ctypes.c_void_p]
-        _LIB = L
-    return _LIB
+        L.pipe_set_fault.argtypes = [ctypes.c_int]
+        _LIBS[variant] = L
+    return _LIBS[variant]
 
 
 class Graph:
-    def __init__(self, path):
+    def __init__(self, path, variant=""):
+        self.L = lib(variant)
         err = ctypes.create_string_buffer(512)
-        self.h = lib().pipe_graph_load(path.encode(), err, 512)
+        self.h = self.L.pipe_graph_load(path.encode(), err, 512)
         if not self.h:
             raise RuntimeError(err.value.decode())
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().pipe_graph_free(self.h)
+            self.L.pipe_graph_free(self.h)
             self.h = None
 
 
 class Index:
     def __init__(self, graph, k, w, discard):
         self.graph = graph
-        self.h = lib().pipe_index_build(graph.h, k, w, discard)
+        self.L = graph.L
+        self.h = self.L.pipe_index_build(graph.h, k, w, discard)
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().pipe_index_free(self.h)
+            self.L.pipe_index_free(self.h)
             self.h = None
 
     @property
     def max_count(self):
-        return lib().pipe_index_maxcount(self.h)
+        return self.L.pipe_index_maxcount(self.h)
 
     @property
     def num_kmers(self):
-        return lib().pipe_index_num_kmers(self.h)
+        return self.L.pipe_index_num_kmers(self.h)
 
     def seeds(self, seq, density, order=False):
+        """Seeds as rows of 10 SeedHit fields; None when OrderSeeds fired a
+        reference assertion (the read would be dropped)."""
         cap = max(1024, 64 * len(seq))
         out = np.zeros(10 * cap, dtype=np.int64)
         n = ctypes.c_int64()
-        rc = lib().pipe_get_seeds(self.h, seq.encode(), len(seq), density, int(order),
-                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(n))
+        rc = self.L.pipe_get_seeds(self.h, seq.encode(), len(seq), density, int(order),
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(n))
+        if rc == 2:
+            return None
         assert rc == 0
         return out[: 10 * n.value].reshape(-1, 10)
 
@@ -89,7 +103,8 @@ class Index:
 def options(oracle_params, density, device_loop=False, speculative=64):
     p = oracle_params
     return PipeOptions(p.bandwidth, p.errorCost, p.xdrop, p.sloppy, p.maxCellsPerSlice, p.seedExtendDensity,
-                       p.multimapScoreFraction, density, p.minClusterSize, int(device_loop), speculative)
+                       p.multimapScoreFraction, density, p.minClusterSize, int(device_loop), speculative,
+                       p.minAlignmentScore)
 
 
 # execution modes of the seed loop: host-driven rounds one seed at a time, rounds
@@ -107,11 +122,12 @@ def align_reads(index, oracle_params, density, names, seqs, device_loop=False, s
     status = np.zeros(n, dtype=np.int32)
     stats = np.zeros(6, dtype=np.int64)
     opt = options(oracle_params, density, device_loop, speculative)
-    rc = lib().pipe_align_reads(index.h, ctypes.byref(opt), n, na, sa, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-                                ctypes.byref(out), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                stats.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
-    assert rc == 0
+    rc = index.L.pipe_align_reads(index.h, ctypes.byref(opt), n, na, sa, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                  ctypes.byref(out), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                  stats.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    if rc != 0:
+        raise RuntimeError("pipeline failed (%d)" % rc)
     text = ctypes.string_at(out.value).decode()
-    lib().pipe_free(out)
+    index.L.pipe_free(out)
     return text, status, dict(rounds=int(stats[0]), extensions=int(stats[1]), seeds=int(stats[2]), word_steps=int(stats[3]),
                               unused=int(stats[4]), launches=int(stats[5]))
