@@ -10,15 +10,18 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libgraphaligner_amd.so")
 
-GA_OK, GA_EXT_FAILED, GA_ASSERT, GA_EOVERFLOW, GA_EBADARG, GA_EIO, GA_ENODEVICE, GA_EHIP, GA_ENOMEM = range(9)
+GA_OK, GA_EXT_FAILED, GA_ASSERT, GA_EOVERFLOW, GA_EBADARG, GA_EIO, GA_ENODEVICE, GA_EHIP, GA_ENOMEM, GA_ESTALE = range(10)
 STATUS_NAMES = {0: "GA_OK", 1: "GA_EXT_FAILED", 2: "GA_ASSERT", 3: "GA_EOVERFLOW", 4: "GA_EBADARG", 5: "GA_EIO",
-                6: "GA_ENODEVICE", 7: "GA_EHIP", 8: "GA_ENOMEM"}
+                6: "GA_ENODEVICE", 7: "GA_EHIP", 8: "GA_ENOMEM", 9: "GA_ESTALE"}
 
 # every entry point the header declares
 EXPORTED = ["ga_params_preset", "ga_graph_load_gfa", "ga_graph_free", "ga_graph_num_nodes", "ga_graph_num_edges",
             "ga_graph_first_ambiguous", "ga_graph_export", "ga_context_create", "ga_context_free", "ga_extend_batch",
             "ga_index_build", "ga_index_free", "ga_index_max_count", "ga_index_num_kmers", "ga_seeds",
-            "ga_align_reads", "ga_reads_upload", "ga_align_resident", "ga_context_synchronize", "ga_free"]
+            "ga_align_reads", "ga_reads_upload", "ga_align_resident", "ga_context_synchronize", "ga_free",
+            "ga_order_seeds", "ga_align_one_way", "ga_result_reads", "ga_result_status", "ga_result_seeds_extended",
+            "ga_result_count", "ga_result_item", "ga_result_trace", "ga_select_alignments", "ga_add_mapping_qualities",
+            "ga_gaf_line", "ga_result_free"]
 
 READ_STATUS_NAMES = {0: "aligned", 1: "no seeds", 2: "assertion (read dropped)", 3: "no alignment", 4: "workspace overflow"}
 
@@ -49,6 +52,23 @@ class AlignStats(ctypes.Structure):
                 ("reruns", ctypes.c_int64), ("lookupCallMs", ctypes.c_double), ("extendCallMs", ctypes.c_double),
                 ("unusedExtensions", ctypes.c_int64), ("seedingMs", ctypes.c_double), ("pickMs", ctypes.c_double),
                 ("replayMs", ctypes.c_double), ("gafMs", ctypes.c_double), ("gafBytes", ctypes.c_int64)]
+
+
+class SeedHit(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int64) for f in ("nodeID", "nodeOffset", "seqPos", "matchLen", "reverse", "alignmentGraphNodeId",
+                                              "alignmentGraphNodeOffset", "rawSeedGoodness", "seedGoodness", "seedClusterSize")]
+
+
+class AlignmentItem(ctypes.Structure):
+    _fields_ = [("alignmentStart", ctypes.c_int64), ("alignmentEnd", ctypes.c_int64), ("alignmentScore", ctypes.c_int64),
+                ("alignmentXScore", ctypes.c_double), ("seedGoodness", ctypes.c_int64), ("mappingQuality", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("traceLength", ctypes.c_int64)]
+
+
+class TraceItem(ctypes.Structure):
+    _fields_ = [("node", ctypes.c_int64), ("nodeOffset", ctypes.c_int64), ("seqPos", ctypes.c_int64),
+                ("nodeSwitch", ctypes.c_int8), ("sequenceCharacter", ctypes.c_char), ("graphCharacter", ctypes.c_char),
+                ("pad", ctypes.c_char * 5)]
 
 
 _lib = None
@@ -87,5 +107,22 @@ def lib():
         L.ga_align_resident.argtypes = [vp, vp, vp, ctypes.POINTER(Params), i64, i64, ctypes.POINTER(ctypes.c_char_p),
                                         ctypes.POINTER(vp), vp, ctypes.POINTER(AlignStats)]
         L.ga_free.argtypes = [vp]
+        L.ga_order_seeds.argtypes = [vp, ctypes.POINTER(SeedHit), i64]
+        L.ga_align_one_way.argtypes = [vp, vp, ctypes.POINTER(Params), i64, ctypes.POINTER(ctypes.c_char_p), vp,
+                                       ctypes.POINTER(SeedHit), vp, ctypes.POINTER(vp)]
+        L.ga_result_reads.restype = i64
+        L.ga_result_reads.argtypes = [vp]
+        L.ga_result_status.restype = i32
+        L.ga_result_status.argtypes = [vp, i64]
+        L.ga_result_seeds_extended.restype = i64
+        L.ga_result_seeds_extended.argtypes = [vp, i64]
+        L.ga_result_count.restype = i64
+        L.ga_result_count.argtypes = [vp, i64]
+        L.ga_result_item.argtypes = [vp, i64, i64, ctypes.POINTER(AlignmentItem)]
+        L.ga_result_trace.argtypes = [vp, i64, i64, ctypes.POINTER(TraceItem), i64]
+        L.ga_select_alignments.argtypes = [vp, i64, ctypes.POINTER(Params), i32]
+        L.ga_add_mapping_qualities.argtypes = [vp, i64]
+        L.ga_gaf_line.argtypes = [vp, vp, i64, i64, ctypes.c_char_p, i64, i32, ctypes.c_char_p, i64, ctypes.POINTER(i64)]
+        L.ga_result_free.argtypes = [vp]
         _lib = L
     return _lib

@@ -3,6 +3,10 @@
 * Graph          ~ AlignmentGraph from GfaGraph::LoadFromFile + DirectedGraph::BuildFromGFA
 * extend_batch   ~ GraphAlignerBitvectorBanded::getReverseTraceFromSeed, batched
 * Index          ~ MinimizerSeeder
+* order_seeds    ~ OrderSeeds (src/GraphAlignerWrapper.h:59)
+* align_one_way  ~ AlignOneWay with the caller's seeds (src/GraphAlignerWrapper.h:55),
+                   batched over reads; returns an AlignmentResults object with
+                   select / add_mapping_qualities / gaf_line (AddGAFLine)
 * align_reads    ~ the per-read pipeline of src/Aligner.cpp:390-606 with GAF output
 """
 import ctypes
@@ -224,3 +228,87 @@ def align_resident(ctx, graph, index, params, first, names, gaf_format="str"):
     text = _gaf_text(out, st, gaf_format)
     stats = {f: getattr(st, f) for f, _ in N.AlignStats._fields_}
     return text, status[:count], stats
+
+
+SEED_FIELDS = [f for f, _ in N.SeedHit._fields_]
+
+
+def _seed_array(rows):
+    rows = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1, 10)
+    arr = (N.SeedHit * max(len(rows), 1))()
+    ctypes.memmove(arr, rows.ctypes.data, rows.nbytes)
+    return arr, len(rows)
+
+
+def order_seeds(graph, seeds):
+    """OrderSeeds on a copy of the seed rows (int64 [n, 10] in SeedHit field
+    order, -1 for an unset alignment-graph node); raises on a reference
+    assertion (GA_ASSERT: the reference drops the read)."""
+    arr, n = _seed_array(seeds)
+    _check(N.lib().ga_order_seeds(graph.h, arr, n), "ga_order_seeds")
+    return np.ctypeslib.as_array(ctypes.cast(arr, ctypes.POINTER(ctypes.c_int64)), shape=(max(n, 1) * 10,))[: 10 * n].reshape(-1, 10).copy()
+
+
+class AlignmentResults:
+    """AlignmentResult of each read of an align_one_way call (ga_alignment_result_t)."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().ga_result_free(self.h)
+            self.h = None
+
+    def __len__(self):
+        return N.lib().ga_result_reads(self.h)
+
+    def status(self, read):
+        return N.lib().ga_result_status(self.h, read)
+
+    def seeds_extended(self, read):
+        return N.lib().ga_result_seeds_extended(self.h, read)
+
+    def count(self, read):
+        return N.lib().ga_result_count(self.h, read)
+
+    def item(self, read, i):
+        it = N.AlignmentItem()
+        _check(N.lib().ga_result_item(self.h, read, i, ctypes.byref(it)), "ga_result_item")
+        return {f: getattr(it, f) for f, _ in N.AlignmentItem._fields_ if f != "pad"}
+
+    def trace(self, read, i):
+        """[(node, nodeOffset, seqPos, nodeSwitch, sequenceCharacter, graphCharacter)]"""
+        n = self.item(read, i)["traceLength"]
+        arr = (N.TraceItem * max(n, 1))()
+        _check(N.lib().ga_result_trace(self.h, read, i, arr, n), "ga_result_trace")
+        return [(t.node, t.nodeOffset, t.seqPos, int(t.nodeSwitch), t.sequenceCharacter, t.graphCharacter) for t in arr[:n]]
+
+    def select(self, read, params, sort_for_output=True):
+        _check(N.lib().ga_select_alignments(self.h, read, ctypes.byref(params), int(sort_for_output)), "ga_select_alignments")
+
+    def add_mapping_qualities(self, read):
+        _check(N.lib().ga_add_mapping_qualities(self.h, read), "ga_add_mapping_qualities")
+
+    def gaf_line(self, graph, read, i, name, read_length, cigar_match_mismatch_merge=False):
+        need = ctypes.c_int64()
+        rc = N.lib().ga_gaf_line(graph.h, self.h, read, i, name.encode(), read_length, int(cigar_match_mismatch_merge), None, 0,
+                                 ctypes.byref(need))
+        if rc not in (N.GA_OK, N.GA_EBADARG) or need.value <= 0:
+            _check(rc, "ga_gaf_line")
+        buf = ctypes.create_string_buffer(need.value)
+        _check(N.lib().ga_gaf_line(graph.h, self.h, read, i, name.encode(), read_length, int(cigar_match_mismatch_merge), buf,
+                                   need.value, ctypes.byref(need)), "ga_gaf_line")
+        return buf.value.decode()
+
+
+def align_one_way(ctx, graph, params, seqs, seeds):
+    """AlignOneWay for each read with its ordered seeds (seeds[i]: int64 [n_i, 10])."""
+    n, arr, lens = _reads(seqs)
+    counts = np.array([len(np.asarray(s).reshape(-1, 10)) for s in seeds], dtype=np.int64)
+    flat = np.concatenate([np.asarray(s, dtype=np.int64).reshape(-1, 10) for s in seeds]) if n else np.zeros((0, 10), np.int64)
+    sarr, _ = _seed_array(flat)
+    h = ctypes.c_void_p()
+    _check(N.lib().ga_align_one_way(ctx.h, graph.h, ctypes.byref(params), n, arr, lens.ctypes.data_as(ctypes.c_void_p), sarr,
+                                     counts.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)), "ga_align_one_way")
+    return AlignmentResults(h)

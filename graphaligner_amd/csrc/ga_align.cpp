@@ -5,6 +5,7 @@
 #include <atomic>
 #include <chrono>
 #include <climits>
+#include <cmath>
 #include <condition_variable>
 #include <exception>
 #include <functional>
@@ -110,38 +111,6 @@ char complementOf(char c)
 	}
 	return 0;
 }
-
-// One trace item (src/GraphAlignerCommon.h:110-151). node/nodeOffset are split
-// node and offset before the fix-ups, bigraph id and offset after them. 16 bytes:
-// bigraph ids, node offsets and read positions fit 32 bits (reads are at most
-// 16 Mbp); every alignment's trace is written once and read by the seed loop
-// and the GAF writer, so its size is memory traffic and page faults.
-struct TItem
-{
-	uint32_t node;
-	uint32_t nodeOffset;
-	uint32_t seqPos;
-	bool nodeSwitch;
-	char seqChar;
-	char graphChar;
-	// left uninitialised: traces are sized first and then filled item by item
-	TItem() {}
-};
-static_assert(sizeof(TItem) == 16, "trace items are 16 bytes");
-
-struct Alignment
-{
-	std::vector<TItem> trace;
-	int32_t traceScore = 0;
-	uint64_t seedGoodness = 0;
-	uint64_t alignmentStart = 0;
-	uint64_t alignmentEnd = 0;
-	uint64_t alignmentScore = kNoPos;
-	double alignmentXScore = -1;
-	int mappingQuality = 255;
-	bool alignmentFailed() const { return alignmentEnd == alignmentStart; }
-	uint64_t alignmentLength() const { return alignmentEnd - alignmentStart; }
-};
 
 // Runs fn(0) .. fn(n - 1) on up to `threads` threads. An exception thrown by fn
 // stops the hand-out of further indices; every worker is joined and the first
@@ -326,6 +295,8 @@ bool alignmentIncompatible(const Alignment& left, const Alignment& right)
 	return overlap > minOverlapLen;
 }
 
+} // namespace
+
 // AlignmentSelection::SelectAlignments (src/AlignmentSelection.cpp:31-53, 65-95).
 // The score threshold passes through SelectionOptions' int field
 // (src/AlignmentSelection.h:21, src/Aligner.cpp:402) before the comparison.
@@ -360,6 +331,56 @@ std::vector<Alignment> selectAlignments(std::vector<Alignment>& all, double minS
 		if (!skip[i]) result.push_back(std::move(kept[i]));
 	return result;
 }
+
+void sortForOutput(std::vector<Alignment>& alns)
+{
+	std::sort(alns.begin(), alns.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
+	std::sort(alns.begin(), alns.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentXScore > r.alignmentXScore; });
+}
+
+bool addMappingQualities(std::vector<Alignment>& alns)
+{
+	try
+	{
+		for (size_t i = 0; i < alns.size(); i++)
+		{
+			check(alns[i].alignmentXScore != -1);
+			// sum over incompatible alignments j of 10^(X_j - X_i), capped at 10
+			double otherSum = 0;
+			for (size_t j = 0; j < alns.size(); j++)
+			{
+				if (i == j || !alignmentIncompatible(alns[i], alns[j])) continue;
+				check(alns[j].alignmentXScore != -1);
+				if (alns[j].alignmentXScore >= alns[i].alignmentXScore + 1)
+				{
+					otherSum += 10;
+					break;
+				}
+				check(alns[j].alignmentXScore <= alns[i].alignmentXScore + 1);
+				otherSum += std::pow(10.0, alns[j].alignmentXScore - alns[i].alignmentXScore);
+				if (otherSum >= 10) break;
+			}
+			int q;
+			if (otherSum >= 10) q = 0;
+			else if (otherSum <= 0.000001) q = 60;
+			else
+			{
+				// int conversion of the phred value, as the reference's int field takes it
+				q = (int)(-std::log(1.0 - 1.0 / (1.0 + otherSum)) * 10);
+				if (q >= 60) q = 60;
+			}
+			check(q >= 0 && q <= 60);
+			alns[i].mappingQuality = q;
+		}
+		return true;
+	}
+	catch (const ReadAssert&)
+	{
+		return false;
+	}
+}
+
+namespace {
 
 // decimal text of v, as std::ostream prints it, appended to s
 void appendUint(std::string& s, uint64_t v)
@@ -494,6 +515,25 @@ void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_
 	s += cigar;
 }
 
+} // namespace
+
+bool gafLine(std::string& s, const GraphHost& g, const char* name, uint64_t readLen, const Alignment& a, bool cigarMatchMismatchMerge)
+{
+	const size_t keep = s.size();
+	try
+	{
+		appendGafLine(s, g, name, readLen, a, cigarMatchMismatchMerge);
+		return true;
+	}
+	catch (const ReadAssert&)
+	{
+		s.resize(keep);
+		return false;
+	}
+}
+
+namespace {
+
 // One seed of a round with its device jobs (-1: none).
 struct Spec
 {
@@ -505,6 +545,7 @@ struct Spec
 struct ReadState
 {
 	uint32_t read = 0;
+	uint64_t readLen = 0;
 	std::vector<SeedHit> seeds;
 	size_t next = 0;
 	std::vector<Alignment> alignments;
@@ -1076,6 +1117,9 @@ bool pickNext(const ReadState& st, LoopCursor& c, const AlignOptions& opt)
 			c.next++;
 			continue;
 		}
+		// getTwoDirectionalTrace's check of the seed's read position
+		// (src/GraphAligner.h:477-478); only a caller's seed can fail it
+		check(s.seqPos < st.readLen);
 		c.worstExtendedSeedScore = s.seedGoodness;
 		c.seedsExtended++;
 		c.pending = c.next;
@@ -1131,7 +1175,11 @@ uint32_t speculationWidth(uint32_t round, uint32_t maxSeeds, size_t activeReads,
 
 } // namespace
 
-void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
+// The host-driven rounds over resident reads [first, first + count). With
+// callerSeeds (AlignOneWay's form) read first + i takes callerSeeds[i] as its
+// ordered seeds instead of seeding, and its alignments go to rawOut[i] instead
+// of selection and GAF text.
+void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idxp, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats, const std::vector<SeedHit>* callerSeeds, std::vector<Alignment>* rawOut)
 {
 	auto t0 = std::chrono::steady_clock::now();
 	// built with -DGA_DEBUG_LOG, GA_ROUND_LOG=1 prints one line per round and
@@ -1164,7 +1212,8 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		}
 		SeedMatches rb;
 		auto tl = std::chrono::steady_clock::now();
-		ctx.lookupSeeds(idx, base, bn, rb);
+		if (callerSeeds) ctx.prepareReads(base, bn);
+		else ctx.lookupSeeds(*idxp, base, bn, rb);
 		stats.lookupCallMs += msSince(tl);
 		stats.lookupMs += rb.lookupMs;
 		std::vector<ReadState> states(bn);
@@ -1173,17 +1222,27 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 		auto seedRead = [&](size_t i) {
 			ReadState& st = states[i];
 			st.read = (uint32_t)i;
+			st.readLen = blens[i];
 			ReadOutcome& o = out[batchStart + i];
 			try
 			{
-				st.seeds = seedsFromMatches(g, idx, rb.data() + rb.matchOffset[i], rb.matchCount[i], blens[i], opt.minimizerSeedDensity);
-				if (st.seeds.empty())
+				if (callerSeeds)
 				{
-					o.status = READ_NO_SEEDS;
-					st.done = true;
-					return;
+					// AlignOneWay asserts that it has seeds (src/GraphAligner.h:99)
+					st.seeds = callerSeeds[batchStart + i];
+					check(!st.seeds.empty());
 				}
-				orderSeedsChecked(g, st.seeds);
+				else
+				{
+					st.seeds = seedsFromMatches(g, *idxp, rb.data() + rb.matchOffset[i], rb.matchCount[i], blens[i], opt.minimizerSeedDensity);
+					if (st.seeds.empty())
+					{
+						o.status = READ_NO_SEEDS;
+						st.done = true;
+						return;
+					}
+					orderSeedsChecked(g, st.seeds);
+				}
 				// ReverseComplement of the read asserts on letters it does not know
 				for (uint32_t k = 0; k < blens[i]; k++) check(complementOf(bseqs[i][k]) != 0);
 				st.extendSeeds = opt.seedExtendDensity == -1 ? st.seeds.size() : (uint64_t)(opt.seedExtendDensity * blens[i] + 1);
@@ -1201,6 +1260,12 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 			ReadOutcome& o = out[batchStart + i];
 			if (o.status == READ_NO_SEEDS) return;
 			o.seedsExtended = st.seedsExtended;
+			if (rawOut)
+			{
+				o.status = st.status;
+				if (st.status == READ_ALIGNED) rawOut[batchStart + i] = std::move(st.alignments);
+				return;
+			}
 			if (st.status != READ_ALIGNED)
 			{
 				o.status = st.status;
@@ -1392,10 +1457,19 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 						{
 							LoopCursor c = cursorOf(st);
 							c.next = st.pending + 1;
-							while (st.specs.size() < k && pickNext(st, c, opt))
+							try
 							{
-								st.specs.push_back(Spec{c.pending, -1, -1});
-								c.next = c.pending + 1;
+								while (st.specs.size() < k && pickNext(st, c, opt))
+								{
+									st.specs.push_back(Spec{c.pending, -1, -1});
+									c.next = c.pending + 1;
+								}
+							}
+							catch (const ReadAssert&)
+							{
+								// a check that fires on a seed picked ahead stops the
+								// speculation there; the replayed loop decides whether
+								// it reaches that seed and fails
 							}
 						}
 						st.roundsRun++;
@@ -1635,7 +1709,13 @@ void alignResidentRounds(ExtendBackend& ctx, const GraphHost& g, const Minimizer
 void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
 {
 	if (opt.deviceSeedLoop) alignResidentDeviceLoop(ctx, g, idx, opt, names, first, count, out, stats);
-	else alignResidentRounds(ctx, g, idx, opt, names, first, count, out, stats);
+	else runRounds(ctx, g, &idx, opt, names, first, count, out, stats, nullptr, nullptr);
+}
+
+void alignOneWayResident(ExtendBackend& ctx, const GraphHost& g, const AlignOptions& opt, uint32_t first, uint32_t count, const std::vector<SeedHit>* seeds, std::vector<Alignment>* alignments, ReadOutcome* out, AlignStats& stats)
+{
+	for (uint32_t i = 0; i < count; i++) alignments[i].clear();
+	runRounds(ctx, g, nullptr, opt, nullptr, first, count, out, stats, seeds, alignments);
 }
 
 void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const std::vector<const char*>& names, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<ReadOutcome>& out, AlignStats& stats)

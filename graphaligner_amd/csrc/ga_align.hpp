@@ -30,6 +30,40 @@ struct SeedHit
 	uint64_t seedClusterSize = 0;
 };
 
+// One trace item (src/GraphAlignerCommon.h:110-151). node/nodeOffset are split
+// node and offset before the fix-ups, bigraph id and offset after them. 16 bytes:
+// bigraph ids, node offsets and read positions fit 32 bits (reads are at most
+// 16 Mbp); every alignment's trace is written once and read by the seed loop
+// and the GAF writer, so its size is memory traffic and page faults.
+struct TItem
+{
+	uint32_t node;
+	uint32_t nodeOffset;
+	uint32_t seqPos;
+	bool nodeSwitch;
+	char seqChar;
+	char graphChar;
+	// left uninitialised: traces are sized first and then filled item by item
+	TItem() {}
+};
+static_assert(sizeof(TItem) == 16, "trace items are 16 bytes");
+
+// AlignmentResult::AlignmentItem (src/GraphAlignerCommon.h:289-340) on the
+// seed-extension path: the merged, fixed-up trace and its scores.
+struct Alignment
+{
+	std::vector<TItem> trace;
+	int32_t traceScore = 0;
+	uint64_t seedGoodness = 0;
+	uint64_t alignmentStart = 0;
+	uint64_t alignmentEnd = 0;
+	uint64_t alignmentScore = ~0ull;
+	double alignmentXScore = -1;
+	int mappingQuality = 255;
+	bool alignmentFailed() const { return alignmentEnd == alignmentStart; }
+	uint64_t alignmentLength() const { return alignmentEnd - alignmentStart; }
+};
+
 struct AlignOptions
 {
 	gad::ExtParams ext;
@@ -132,6 +166,32 @@ void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx
 // backend (one GAF line per selected alignment, '\n'-terminated, in out[i].gaf;
 // names[i] names read first + i).
 void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats);
+
+// GraphAligner::AlignOneWay (src/GraphAligner.h:94-181) with the caller's
+// ordered seeds, for resident reads [first, first + count): seeds[i] are read
+// first + i's seeds; alignments[i] receives its AlignmentResult::alignments in
+// the order AlignOneWay leaves them (unselected), out[i].status READ_ALIGNED
+// (whether or not any alignment was made), READ_ASSERT (a reference assertion
+// fired: no seeds, a seed past the read's end or a check on the way) or
+// READ_OVERFLOW, out[i].seedsExtended AlignmentResult::seedsExtended. Runs the
+// host-driven rounds whatever opt.deviceSeedLoop says.
+void alignOneWayResident(ExtendBackend& ctx, const GraphHost& g, const AlignOptions& opt, uint32_t first, uint32_t count, const std::vector<SeedHit>* seeds, std::vector<Alignment>* alignments, ReadOutcome* out, AlignStats& stats);
+
+// AlignmentSelection::SelectAlignments (src/AlignmentSelection.cpp:31-95) with
+// --min-alignment-score and --multimap-score-fraction.
+std::vector<Alignment> selectAlignments(std::vector<Alignment>& all, double minScore, double fraction);
+
+// The output order of src/Aligner.cpp:534,549: by start, then by X-score, best first.
+void sortForOutput(std::vector<Alignment>& alns);
+
+// AlignmentSelection::AddMappingQualities (src/AlignmentSelection.cpp:97-135);
+// false when one of its assertions fires.
+bool addMappingQualities(std::vector<Alignment>& alns);
+
+// GraphAligner::AddGAFLine (src/GraphAlignerGAFAlignment.h:37-250): appends the
+// line (no newline) to s; false when a reference assertion fires (a letter the
+// reference's character tables reject), s then unchanged.
+bool gafLine(std::string& s, const GraphHost& g, const char* name, uint64_t readLen, const Alignment& a, bool cigarMatchMismatchMerge);
 
 // Uploads the reads, then alignResident over all of them.
 void alignReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const std::vector<const char*>& names, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<ReadOutcome>& out, AlignStats& stats);

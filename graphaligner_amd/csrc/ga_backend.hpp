@@ -224,6 +224,9 @@ public:
 	// For resident reads [first, first + count): writes their reverse complements
 	// and looks every read k-mer up in the index.
 	virtual void lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out) = 0;
+	// For resident reads [first, first + count): writes their reverse complements
+	// (what lookupSeeds does besides the lookup), for seeds that come from the caller.
+	virtual void prepareReads(uint32_t first, uint32_t count) = 0;
 	// Forward bytes of resident read i start at readsBase() + offset[i], its reverse
 	// complement (once looked up) at readsBase() + reverseBase() + offset[i].
 	virtual const char* readsBase() const = 0;

@@ -2,6 +2,7 @@
 #include "../../include/graphaligner_amd.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <cstdio>
@@ -292,7 +293,11 @@ ga_status_t ga_seeds(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_
 		std::vector<uint32_t> l(n);
 		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
 		std::vector<std::vector<SeedHit>> out;
-		seedReads(*ctx->ctx, graph->host, index->idx, density, order != 0, s, l, out, nullptr);
+		std::vector<int32_t> orderStatus;
+		seedReads(*ctx->ctx, graph->host, index->idx, density, order != 0, s, l, out, nullptr, &orderStatus);
+		// a read whose OrderSeeds fires a reference assertion is dropped there
+		for (int64_t i = 0; i < n; i++)
+			if (orderStatus[i] != 0) out[i].clear();
 		size_t total = 0;
 		for (auto& v : out) total += v.size();
 		int64_t* buf = (int64_t*)std::malloc(std::max<size_t>(1, total) * 10 * sizeof(int64_t));
@@ -513,6 +518,183 @@ ga_status_t ga_align_resident(ga_context_t* ctx, const ga_graph_t* graph, const 
 		return rc;
 	});
 }
+
+static bool seedNodeOk(const GraphHost& g, const ga_seed_hit_t& s)
+{
+	if (s.nodeID < 0 || s.nodeID > INT32_MAX / 2 || s.nodeOffset < 0 || s.seqPos < 0 || s.matchLen < 0) return false;
+	const int forwardId = (int)(s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2);
+	auto it = g.originalSize.find(forwardId);
+	if (it == g.originalSize.end() || (uint64_t)s.nodeOffset >= it->second) return false;
+	if (s.alignmentGraphNodeId >= 0 && (uint64_t)s.alignmentGraphNodeId >= g.numNodes()) return false;
+	if (s.alignmentGraphNodeId >= 0 && (s.alignmentGraphNodeOffset < 0 || (uint64_t)s.alignmentGraphNodeOffset >= g.nodeLen[s.alignmentGraphNodeId])) return false;
+	return true;
+}
+
+static SeedHit seedHitOf(const ga_seed_hit_t& s)
+{
+	SeedHit h;
+	h.nodeID = (int)s.nodeID;
+	h.nodeOffset = (uint64_t)s.nodeOffset;
+	h.seqPos = (uint64_t)s.seqPos;
+	h.matchLen = (uint64_t)s.matchLen;
+	h.reverse = s.reverse != 0;
+	h.alignmentGraphNodeId = s.alignmentGraphNodeId < 0 ? ~0ull : (uint64_t)s.alignmentGraphNodeId;
+	h.alignmentGraphNodeOffset = s.alignmentGraphNodeOffset < 0 ? ~0ull : (uint64_t)s.alignmentGraphNodeOffset;
+	h.rawSeedGoodness = (uint64_t)s.rawSeedGoodness;
+	h.seedGoodness = (uint64_t)s.seedGoodness;
+	h.seedClusterSize = (uint64_t)s.seedClusterSize;
+	return h;
+}
+
+static void toSeedHit(const SeedHit& h, ga_seed_hit_t& s)
+{
+	s.nodeID = h.nodeID;
+	s.nodeOffset = (int64_t)h.nodeOffset;
+	s.seqPos = (int64_t)h.seqPos;
+	s.matchLen = (int64_t)h.matchLen;
+	s.reverse = h.reverse ? 1 : 0;
+	s.alignmentGraphNodeId = h.alignmentGraphNodeId == ~0ull ? -1 : (int64_t)h.alignmentGraphNodeId;
+	s.alignmentGraphNodeOffset = h.alignmentGraphNodeOffset == ~0ull ? -1 : (int64_t)h.alignmentGraphNodeOffset;
+	s.rawSeedGoodness = (int64_t)h.rawSeedGoodness;
+	s.seedGoodness = (int64_t)h.seedGoodness;
+	s.seedClusterSize = (int64_t)h.seedClusterSize;
+}
+
+ga_status_t ga_order_seeds(const ga_graph_t* graph, ga_seed_hit_t* seeds, int64_t n)
+{
+	if (!graph || n < 0 || (n > 0 && !seeds)) return GA_EBADARG;
+	for (int64_t i = 0; i < n; i++)
+		if (!seedNodeOk(graph->host, seeds[i])) return GA_EBADARG;
+	return guarded([&]() {
+		std::vector<SeedHit> v(n);
+		for (int64_t i = 0; i < n; i++) v[i] = seedHitOf(seeds[i]);
+		if (!orderSeedsByChaining(graph->host, v)) return GA_ASSERT;
+		for (int64_t i = 0; i < n; i++) toSeedHit(v[i], seeds[i]);
+		return GA_OK;
+	});
+}
+
+ga_status_t ga_align_one_way(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const ga_seed_hit_t* seeds, const int64_t* seedCounts, ga_alignment_result_t** result)
+{
+	if (!ctx || !graph || !params || !result || !readArgsOk(n, seqs, lens) || (n > 0 && !seedCounts)) return GA_EBADARG;
+	*result = nullptr;
+	int64_t total = 0;
+	for (int64_t i = 0; i < n; i++)
+	{
+		if (seedCounts[i] < 0) return GA_EBADARG;
+		total += seedCounts[i];
+	}
+	if (total > 0 && !seeds) return GA_EBADARG;
+	for (int64_t k = 0; k < total; k++)
+		if (!seedNodeOk(graph->host, seeds[k])) return GA_EBADARG;
+	ctx->residentFromUpload = false;
+	return guarded([&]() {
+		std::unique_ptr<ga_alignment_result> res(new ga_alignment_result);
+		std::vector<std::vector<SeedHit>> perRead(n);
+		int64_t at = 0;
+		for (int64_t i = 0; i < n; i++)
+		{
+			perRead[i].resize(seedCounts[i]);
+			for (int64_t k = 0; k < seedCounts[i]; k++) perRead[i][k] = seedHitOf(seeds[at + k]);
+			at += seedCounts[i];
+		}
+		std::vector<const char*> s(seqs, seqs + n);
+		std::vector<uint32_t> l(n);
+		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
+		ctx->ctx->uploadReads(s, l);
+		std::vector<ReadOutcome> out(n);
+		res->alignments.resize(n);
+		AlignStats st;
+		alignOneWayResident(*ctx->ctx, graph->host, alignOptionsOf(*params), 0, (uint32_t)n, perRead.data(), res->alignments.data(), out.data(), st);
+		res->status.resize(n);
+		res->seedsExtended.resize(n);
+		for (int64_t i = 0; i < n; i++)
+		{
+			res->status[i] = out[i].status;
+			res->seedsExtended[i] = out[i].seedsExtended;
+		}
+		*result = res.release();
+		return GA_OK;
+	});
+}
+
+static bool resultReadOk(const ga_alignment_result_t* r, int64_t read) { return r && read >= 0 && (uint64_t)read < r->status.size(); }
+
+int64_t ga_result_reads(const ga_alignment_result_t* r) { return r ? (int64_t)r->status.size() : -1; }
+
+int32_t ga_result_status(const ga_alignment_result_t* r, int64_t read) { return resultReadOk(r, read) ? r->status[read] : -1; }
+
+int64_t ga_result_seeds_extended(const ga_alignment_result_t* r, int64_t read) { return resultReadOk(r, read) ? (int64_t)r->seedsExtended[read] : -1; }
+
+int64_t ga_result_count(const ga_alignment_result_t* r, int64_t read) { return resultReadOk(r, read) ? (int64_t)r->alignments[read].size() : -1; }
+
+ga_status_t ga_result_item(const ga_alignment_result_t* r, int64_t read, int64_t i, ga_alignment_item_t* item)
+{
+	if (!resultReadOk(r, read) || i < 0 || (uint64_t)i >= r->alignments[read].size() || !item) return GA_EBADARG;
+	const Alignment& a = r->alignments[read][i];
+	item->alignmentStart = (int64_t)a.alignmentStart;
+	item->alignmentEnd = (int64_t)a.alignmentEnd;
+	item->alignmentScore = (int64_t)a.alignmentScore;
+	item->alignmentXScore = a.alignmentXScore;
+	item->seedGoodness = (int64_t)a.seedGoodness;
+	item->mappingQuality = a.mappingQuality;
+	item->pad = 0;
+	item->traceLength = (int64_t)a.trace.size();
+	return GA_OK;
+}
+
+ga_status_t ga_result_trace(const ga_alignment_result_t* r, int64_t read, int64_t i, ga_trace_item_t* items, int64_t cap)
+{
+	if (!resultReadOk(r, read) || i < 0 || (uint64_t)i >= r->alignments[read].size()) return GA_EBADARG;
+	const auto& t = r->alignments[read][i].trace;
+	if (cap < (int64_t)t.size() || (!items && !t.empty())) return GA_EBADARG;
+	for (size_t k = 0; k < t.size(); k++)
+	{
+		ga_trace_item_t& o = items[k];
+		std::memset(&o, 0, sizeof(o));
+		o.node = t[k].node;
+		o.nodeOffset = t[k].nodeOffset;
+		o.seqPos = t[k].seqPos;
+		o.nodeSwitch = t[k].nodeSwitch ? 1 : 0;
+		o.sequenceCharacter = t[k].seqChar;
+		o.graphCharacter = t[k].graphChar;
+	}
+	return GA_OK;
+}
+
+ga_status_t ga_select_alignments(ga_alignment_result_t* r, int64_t read, const ga_params_t* params, int32_t sortOutput)
+{
+	if (!resultReadOk(r, read) || !params) return GA_EBADARG;
+	return guarded([&]() {
+		auto& v = r->alignments[read];
+		if (!v.empty()) v = selectAlignments(v, params->minAlignmentScore, params->multimapScoreFraction);
+		if (sortOutput) ga::sortForOutput(v);
+		return GA_OK;
+	});
+}
+
+ga_status_t ga_add_mapping_qualities(ga_alignment_result_t* r, int64_t read)
+{
+	if (!resultReadOk(r, read)) return GA_EBADARG;
+	return addMappingQualities(r->alignments[read]) ? GA_OK : GA_ASSERT;
+}
+
+ga_status_t ga_gaf_line(const ga_graph_t* graph, const ga_alignment_result_t* r, int64_t read, int64_t i, const char* name, int64_t readLength, int32_t cigarMatchMismatchMerge, char* buf, int64_t cap, int64_t* needed)
+{
+	if (!graph || !resultReadOk(r, read) || i < 0 || (uint64_t)i >= r->alignments[read].size() || !name || readLength < 0 || !needed) return GA_EBADARG;
+	const Alignment& a = r->alignments[read][i];
+	if (a.trace.empty()) return GA_EBADARG;
+	return guarded([&]() {
+		std::string line;
+		if (!gafLine(line, graph->host, name, (uint64_t)readLength, a, cigarMatchMismatchMerge != 0)) return GA_ASSERT;
+		*needed = (int64_t)line.size() + 1;
+		if (!buf || cap < *needed) return GA_EBADARG;
+		std::memcpy(buf, line.c_str(), line.size() + 1);
+		return GA_OK;
+	});
+}
+
+void ga_result_free(ga_alignment_result_t* r) { delete r; }
 
 void ga_free(void* p) { std::free(p); }
 

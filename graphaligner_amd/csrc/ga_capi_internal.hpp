@@ -30,6 +30,13 @@ struct ga_context
 	bool residentFromUpload = false;
 };
 
+struct ga_alignment_result
+{
+	std::vector<int32_t> status;
+	std::vector<uint64_t> seedsExtended;
+	std::vector<std::vector<ga::Alignment>> alignments;
+};
+
 namespace ga {
 void copyErr(char* err, size_t cap, const char* msg);
 int32_t errorCostOf(const ga_params_t& p);

@@ -220,6 +220,38 @@ void Context::uploadReads(const std::vector<const char*>& seqs, const std::vecto
 	GA_HIP(hipStreamSynchronize(stream_));
 }
 
+void Context::writeReverse(uint32_t first, uint32_t count)
+{
+	// reverse complements (and the planes of the buffer holding them) once per
+	// uploaded read range
+	const bool revNeeded = !(first >= revLo_ && (uint64_t)first + count <= revHi_);
+	if (!revNeeded || count == 0) return;
+	const uint64_t* off = readOff_.as<uint64_t>() + first;
+	const uint32_t* len = readLen_.as<uint32_t>() + first;
+	GA_HIP(gad::launchRevcomp(stream_, readBytes_.as<char>(), readBytes_.as<char>() + revBase_, off, len, count, 0));
+	// the reverse complements just written change the buffer's planes
+	std::lock_guard<std::mutex> lock(cacheMutex_);
+	computePlanes();
+	if (revHi_ > revLo_ && first <= revHi_ && (uint64_t)first + count >= revLo_)
+	{
+		revLo_ = std::min<uint64_t>(revLo_, first);
+		revHi_ = std::max<uint64_t>(revHi_, (uint64_t)first + count);
+	}
+	else
+	{
+		revLo_ = first;
+		revHi_ = (uint64_t)first + count;
+	}
+}
+
+void Context::prepareReads(uint32_t first, uint32_t count)
+{
+	GA_HIP(hipSetDevice(device_));
+	if ((uint64_t)first + count > reads_.size()) throw std::runtime_error("read range outside the resident reads");
+	writeReverse(first, count);
+	GA_HIP(hipStreamSynchronize(stream_));
+}
+
 void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out)
 {
 	GA_HIP(hipSetDevice(device_));
@@ -239,27 +271,8 @@ void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t co
 	const uint64_t* off = readOff_.as<uint64_t>() + first;
 	const uint32_t* len = readLen_.as<uint32_t>() + first;
 	GA_HIP(hipEventRecord(ev0_, stream_));
-	// reverse complements (and the planes of the buffer holding them) once per
-	// uploaded read range
-	const bool revNeeded = !(first >= revLo_ && (uint64_t)first + count <= revHi_);
-	if (revNeeded) GA_HIP(gad::launchRevcomp(stream_, readBytes_.as<char>(), readBytes_.as<char>() + revBase_, off, len, count, 0));
 	GA_HIP(gad::launchSeedLookup(stream_, readBytes_.as<char>(), off, len, count, di.table.as<gad::SeedTableSlot>(), di.mask, idx.k, idx.w, idx.maxCount, matches_.as<gad::SeedMatch>(), matchCount_.as<uint32_t>()));
-	if (revNeeded)
-	{
-		// the reverse complements just written change the buffer's planes
-		std::lock_guard<std::mutex> lock(cacheMutex_);
-		computePlanes();
-		if (revHi_ > revLo_ && first <= revHi_ && (uint64_t)first + count >= revLo_)
-		{
-			revLo_ = std::min<uint64_t>(revLo_, first);
-			revHi_ = std::max<uint64_t>(revHi_, (uint64_t)first + count);
-		}
-		else
-		{
-			revLo_ = first;
-			revHi_ = (uint64_t)first + count;
-		}
-	}
+	writeReverse(first, count);
 	GA_HIP(hipEventRecord(ev1_, stream_));
 	GA_HIP(hipMemcpyAsync(out.matchCount.data(), matchCount_.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
 	GA_HIP(hipStreamSynchronize(stream_));

@@ -102,6 +102,7 @@ public:
 	void uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens) override;
 	const ReadSet& reads() const override { return reads_; }
 	void lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out) override;
+	void prepareReads(uint32_t first, uint32_t count) override;
 	const char* readsBase() const override { return readBytes_.as<char>(); }
 	void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) override;
 	uint64_t reverseBase() const override { return revBase_; }
@@ -127,6 +128,9 @@ private:
 	// computePlanes runs with cacheMutex_ held
 	const uint64_t* residentPlanes(const char* devSeqBase);
 	void computePlanes();
+	// reverse complements of resident reads [first, first + count) unless already
+	// written, then the planes; call with the lookup's events recorded around it
+	void writeReverse(uint32_t first, uint32_t count);
 	int device_;
 	int cus_ = 0;
 	hipStream_t stream_ = nullptr;

@@ -3,7 +3,7 @@
  *
  * Plain C types only. Every call returns a ga_status_t. Handles are opaque.
  * Graphs and indexes are immutable after construction and may be shared
- * between threads; a ga_context_t (one GPU, one stream) is not thread-safe.
+ * between threads; a ga_context_t (one GPU) is not thread-safe.
  * All GPU work needs a gfx950 device: without one, ga_context_create fails with
  * GA_ENODEVICE and nothing falls back to the CPU.
  */
@@ -83,7 +83,8 @@ int64_t ga_graph_first_ambiguous(const ga_graph_t* graph);
  * and in/out adjacency in CSR form (offsets have num_nodes + 1 entries). */
 ga_status_t ga_graph_export(const ga_graph_t* graph, int64_t* nodeLength, int64_t* nodeIds, int64_t* nodeOffset, int64_t* component, uint8_t* linearizable, uint8_t* reverse, int64_t* chain, int64_t* chainPos, uint64_t* chunks, int64_t* inOffsets, int64_t* inNeighbors, int64_t* outOffsets, int64_t* outNeighbors);
 
-/* One GPU (HIP device ordinal), one stream, resident graph copies. */
+/* One GPU (HIP device ordinal): a stream for uploads and seed lookups and four
+ * extension lanes with their own streams, resident graph and index copies. */
 ga_status_t ga_context_create(int device, ga_context_t** ctx, char* err, size_t errcap);
 void ga_context_free(ga_context_t* ctx);
 /* Waits until all work on the context's device has finished. */
@@ -133,7 +134,9 @@ int64_t ga_index_num_kmers(const ga_index_t* index);
  * seeds, read by read, 10 int64 each in SeedHit field order
  * (src/GraphAlignerWrapper.h:11-50): nodeID, nodeOffset, seqPos, matchLen,
  * reverse, alignmentGraphNodeId, alignmentGraphNodeOffset, rawSeedGoodness,
- * seedGoodness, seedClusterSize. */
+ * seedGoodness, seedClusterSize. With order != 0, a read whose OrderSeeds fires
+ * a reference assertion gets no seeds (the reference drops the read). The reads
+ * are uploaded by the call, replacing the read set of ga_reads_upload. */
 ga_status_t ga_seeds(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, double density, int32_t order, int64_t n, const char* const* seqs, const int64_t* lens, int64_t** seeds, int64_t* counts);
 
 typedef struct
@@ -161,8 +164,11 @@ typedef struct
 /* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed
  * loop, select, GAF) for n reads with the given parameters. The seed loop of
  * each read makes the decisions of GraphAligner::AlignOneWay
- * (src/GraphAligner.h:94-181); every read's whole seed loop runs on the GPU
- * (one wavefront per read, all reads in one launch). *gaf is a NUL-terminated buffer from malloc (free with
+ * (src/GraphAligner.h:94-181): by default the loops advance in rounds, each
+ * round's extensions of all reads in one GPU launch (params->deviceSeedLoop = 1
+ * runs every read's whole loop on the GPU in one launch instead). The reads are
+ * uploaded by the call, replacing the read set of ga_reads_upload.
+ * *gaf is a NUL-terminated buffer from malloc (free with
  * ga_free): the GAF lines (src/GraphAlignerGAFAlignment.h:37-250) of read 0,
  * then read 1, ... status[i]: 0 aligned, 1 no seeds, 2 a reference assertion
  * fired and the read was dropped, 3 no alignment kept, 4 an extension exceeded
@@ -173,9 +179,107 @@ ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_
  * (and keeps a host copy for trace fix-ups and GAF text), replacing the previous
  * set; ga_align_resident then runs ga_align_reads' per-read pipeline on resident
  * reads [first, first + count) with no host-to-device read transfer. names[i]
- * names read first + i; gaf and status are as in ga_align_reads. */
+ * names read first + i; gaf and status are as in ga_align_reads.
+ * ga_align_resident fails with GA_ESTALE when a call that uploads its own reads
+ * (ga_seeds, ga_align_reads, ga_align_one_way) ran after the last
+ * ga_reads_upload on the context. */
 ga_status_t ga_reads_upload(ga_context_t* ctx, int64_t n, const char* const* seqs, const int64_t* lens);
 ga_status_t ga_align_resident(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t first, int64_t count, const char* const* names, char** gaf, int32_t* status, ga_align_stats_t* stats);
+
+/* ---- AlignOneWay with the caller's seeds, and the result object --------- */
+
+/* One seed, field for field SeedHit (src/GraphAlignerWrapper.h:11-50); the rows
+ * ga_seeds writes have this layout. alignmentGraphNodeId and
+ * alignmentGraphNodeOffset are -1 when not known (a seed from outside the
+ * minimizer seeder, e.g. a seed file, src/Aligner.cpp:685-706); OrderSeeds then
+ * places the seed by nodeID, reverse and nodeOffset. */
+typedef struct
+{
+	int64_t nodeID;
+	int64_t nodeOffset;
+	int64_t seqPos;
+	int64_t matchLen;
+	int64_t reverse;
+	int64_t alignmentGraphNodeId;
+	int64_t alignmentGraphNodeOffset;
+	int64_t rawSeedGoodness;
+	int64_t seedGoodness;
+	int64_t seedClusterSize;
+} ga_seed_hit_t;
+
+/* OrderSeeds (src/GraphAlignerWrapper.h:59, src/GraphAligner.h:250-312) on the
+ * n seeds of one read, in place: seedGoodness and seedClusterSize are set and
+ * the seeds sorted best first. GA_ASSERT when one of the reference's assertions
+ * there fires (the reference drops the read); GA_EBADARG for a seed whose node
+ * is not in the graph. */
+ga_status_t ga_order_seeds(const ga_graph_t* graph, ga_seed_hit_t* seeds, int64_t n);
+
+/* AlignmentResult (src/GraphAlignerCommon.h:282-344) of a batch of reads. */
+typedef struct ga_alignment_result ga_alignment_result_t;
+
+/* AlignOneWay (src/GraphAlignerWrapper.h:55, src/GraphAligner.h:94-181) for n
+ * reads, each with its own ordered seeds: read i's seeds are
+ * seeds[seedCounts[0] + ... + seedCounts[i-1], ... + seedCounts[i]). The
+ * extensions of all reads run on the context's GPU in batched rounds; the
+ * reads are uploaded by the call (replacing the read set of ga_reads_upload,
+ * see GA_ESTALE). Per read the result holds AlignmentResult::alignments in the
+ * order AlignOneWay leaves them (not yet selected) and seedsExtended; its
+ * status is 0, 2 when a reference assertion fired (no seeds, a seed past the
+ * read's end, or a check on the way; the reference drops the read) or 4 when an
+ * extension exceeded the largest device workspace. params' seeding fields are
+ * not used. GA_EBADARG for a seed whose nodeID/nodeOffset is not in the graph.
+ * Free the result with ga_result_free. */
+ga_status_t ga_align_one_way(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const ga_seed_hit_t* seeds, const int64_t* seedCounts, ga_alignment_result_t** result);
+
+/* AlignmentResult::AlignmentItem (src/GraphAlignerCommon.h:289-340). */
+typedef struct
+{
+	int64_t alignmentStart;
+	int64_t alignmentEnd;
+	int64_t alignmentScore;
+	double alignmentXScore;
+	int64_t seedGoodness;
+	int32_t mappingQuality;
+	int32_t pad;
+	int64_t traceLength;
+} ga_alignment_item_t;
+
+/* TraceItem (src/GraphAlignerCommon.h:110-151) of a fixed-up trace: bigraph
+ * node id (2 * GFA node + strand), offset in it, read position, node switch,
+ * the read and graph letters. */
+typedef struct
+{
+	int64_t node;
+	int64_t nodeOffset;
+	int64_t seqPos;
+	int8_t nodeSwitch;
+	char sequenceCharacter;
+	char graphCharacter;
+	char pad[5];
+} ga_trace_item_t;
+
+int64_t ga_result_reads(const ga_alignment_result_t* result);
+/* status of read `read` (0, 2 or 4 as above), -1 for a bad argument */
+int32_t ga_result_status(const ga_alignment_result_t* result, int64_t read);
+int64_t ga_result_seeds_extended(const ga_alignment_result_t* result, int64_t read);
+int64_t ga_result_count(const ga_alignment_result_t* result, int64_t read);
+ga_status_t ga_result_item(const ga_alignment_result_t* result, int64_t read, int64_t i, ga_alignment_item_t* item);
+/* Copies alignment i's trace (item.traceLength items; cap at least that many). */
+ga_status_t ga_result_trace(const ga_alignment_result_t* result, int64_t read, int64_t i, ga_trace_item_t* items, int64_t cap);
+/* AlignmentSelection::SelectAlignments (src/AlignmentSelection.cpp:31-95) with
+ * params->minAlignmentScore and multimapScoreFraction on the read's alignments,
+ * in place; with sortForOutput != 0 then the output order of
+ * src/Aligner.cpp:534,549 (by start, then by X-score, best first). */
+ga_status_t ga_select_alignments(ga_alignment_result_t* result, int64_t read, const ga_params_t* params, int32_t sortForOutput);
+/* AlignmentSelection::AddMappingQualities (src/AlignmentSelection.cpp:97-135) on
+ * the read's alignments; GA_ASSERT when one of its assertions fires. */
+ga_status_t ga_add_mapping_qualities(ga_alignment_result_t* result, int64_t read);
+/* AddGAFLine (src/GraphAlignerWrapper.h:57, src/GraphAlignerGAFAlignment.h:37-250):
+ * the GAF line of alignment i (no newline) into buf (cap bytes with its NUL);
+ * *needed is the line's length plus one. GA_EBADARG if cap is too small,
+ * GA_ASSERT when a reference assertion fires. */
+ga_status_t ga_gaf_line(const ga_graph_t* graph, const ga_alignment_result_t* result, int64_t read, int64_t i, const char* name, int64_t readLength, int32_t cigarMatchMismatchMerge, char* buf, int64_t cap, int64_t* needed);
+void ga_result_free(ga_alignment_result_t* result);
 
 void ga_free(void* p);
 

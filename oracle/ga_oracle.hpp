@@ -2517,6 +2517,8 @@ struct Aligner
 	// src/GraphAligner.h:475-520
 	Trace getTwoDirectionalTrace(const std::string& sequence, const std::string& revSequence, const SeedHit& seedHit) const
 	{
+		// src/GraphAligner.h:477-478
+		ORC_ASSERT(seedHit.seqPos < sequence.size());
 		int forwardNodeId, backwardNodeId;
 		if (seedHit.reverse) { forwardNodeId = seedHit.nodeID * 2 + 1; backwardNodeId = seedHit.nodeID * 2; }
 		else { forwardNodeId = seedHit.nodeID * 2; backwardNodeId = seedHit.nodeID * 2 + 1; }
@@ -2917,6 +2919,47 @@ inline std::vector<AlignmentItem> SelectAlignments(const std::vector<AlignmentIt
 		filtered = std::move(result);
 	}
 	return filtered;
+}
+
+// src/AlignmentSelection.cpp:97-135
+inline void AddMappingQualities(std::vector<AlignmentItem>& alignments)
+{
+	for (size_t i = 0; i < alignments.size(); i++)
+	{
+		ORC_ASSERT(alignments[i].alignmentXScore != -1);
+		double otherSum = 0;
+		for (size_t j = 0; j < alignments.size(); j++)
+		{
+			if (i == j) continue;
+			if (!alignmentIncompatible(alignments[i], alignments[j])) continue;
+			ORC_ASSERT(alignments[j].alignmentXScore != -1);
+			if (alignments[j].alignmentXScore >= alignments[i].alignmentXScore + 1)
+			{
+				otherSum += 10;
+				break;
+			}
+			ORC_ASSERT(alignments[j].alignmentXScore <= alignments[i].alignmentXScore + 1);
+			otherSum += pow(10.0, alignments[j].alignmentXScore - alignments[i].alignmentXScore);
+			if (otherSum >= 10) break;
+		}
+		if (otherSum >= 10)
+		{
+			alignments[i].mappingQuality = 0;
+		}
+		else if (otherSum <= 0.000001)
+		{
+			alignments[i].mappingQuality = 60;
+		}
+		else
+		{
+			ORC_ASSERT(otherSum >= 0.000001);
+			ORC_ASSERT(otherSum <= 10);
+			alignments[i].mappingQuality = -log(1.0 - 1.0 / (1.0 + otherSum)) * 10;
+			if (alignments[i].mappingQuality >= 60) alignments[i].mappingQuality = 60;
+		}
+		ORC_ASSERT(alignments[i].mappingQuality >= 0);
+		ORC_ASSERT(alignments[i].mappingQuality <= 60);
+	}
 }
 
 // ---------------------------------------------------------------------------

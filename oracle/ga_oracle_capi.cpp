@@ -2,6 +2,7 @@
 // __graft_entry__.smoke() and the cpu_baseline leg of bench.py as the checker.
 #include <atomic>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <thread>
 #include "ga_oracle.hpp"
@@ -253,6 +254,119 @@ double orc_align_reads(void* g, void* idx, const OrcParams* p, double seedDensit
 	buf[pos] = 0;
 	*gafOut = buf;
 	return std::chrono::duration<double>(end - start).count();
+}
+
+// GraphAligner::AlignOneWay (src/GraphAligner.h:94-181) with the caller's seeds
+// (10 int64 per seed in SeedHit field order, -1 for an unset
+// alignmentGraphNodeId/Offset), then, if select != 0, SelectAlignments and the
+// output sorts of src/Aligner.cpp:504-549, and if mapq != 0
+// AddMappingQualities first (src/Aligner.cpp:476). The result is text in a
+// malloc'd buffer: "S status seedsExtended" (status 0, or 2 when a reference
+// assertion fired), then per alignment "A start end score xscore seedGoodness
+// mapq n" and a line of n trace items "node,offset,seqPos,nodeSwitch,seqChar,graphChar".
+int orc_align_one_way(void* g, const OrcParams* p, const char* seq, int64_t len, const int64_t* seeds, int64_t nseeds, int select, int mapq, char** out)
+{
+	Params params = toParams(p);
+	Aligner aligner(params, *(AlignmentGraph*)g);
+	std::vector<SeedHit> hits(nseeds);
+	for (int64_t i = 0; i < nseeds; i++)
+	{
+		const int64_t* x = seeds + 10 * i;
+		hits[i].nodeID = (int)x[0];
+		hits[i].nodeOffset = x[1];
+		hits[i].seqPos = x[2];
+		hits[i].matchLen = x[3];
+		hits[i].reverse = x[4] != 0;
+		hits[i].alignmentGraphNodeId = x[5] < 0 ? NMAX : (size_t)x[5];
+		hits[i].alignmentGraphNodeOffset = x[6] < 0 ? NMAX : (size_t)x[6];
+		hits[i].rawSeedGoodness = x[7];
+		hits[i].seedGoodness = x[8];
+		hits[i].seedClusterSize = x[9];
+	}
+	std::string text;
+	std::string sequence(seq, len);
+	int status = 0;
+	AlignmentResult result;
+	try
+	{
+		result = aligner.AlignOneWay(sequence, hits);
+		if (mapq) AddMappingQualities(result.alignments);
+	}
+	catch (const AssertFail&)
+	{
+		status = 2;
+	}
+	if (status == 0 && select)
+	{
+		if (result.alignments.size() > 0) result.alignments = SelectAlignments(result.alignments, params.minAlignmentScore, params.multimapScoreFraction);
+		std::sort(result.alignments.begin(), result.alignments.end(), [](const AlignmentItem& l, const AlignmentItem& r) { return l.alignmentStart < r.alignmentStart; });
+		std::sort(result.alignments.begin(), result.alignments.end(), [](const AlignmentItem& l, const AlignmentItem& r) { return l.alignmentXScore > r.alignmentXScore; });
+	}
+	char buf[256];
+	std::snprintf(buf, sizeof(buf), "S %d %zu\n", status, status == 0 ? result.seedsExtended : (size_t)0);
+	text += buf;
+	if (status == 0)
+		for (const auto& a : result.alignments)
+		{
+			const auto& t = a.trace->trace;
+			std::snprintf(buf, sizeof(buf), "A %zu %zu %zu %.17g %zu %d %zu\n", a.alignmentStart, a.alignmentEnd, a.alignmentScore, a.alignmentXScore, a.seedGoodness, a.mappingQuality, t.size());
+			text += buf;
+			for (size_t k = 0; k < t.size(); k++)
+			{
+				std::snprintf(buf, sizeof(buf), "%s%zu,%zu,%zu,%d,%d,%d", k ? " " : "", t[k].DPposition.node, t[k].DPposition.nodeOffset, t[k].DPposition.seqPos, (int)t[k].nodeSwitch, (int)(unsigned char)t[k].sequenceCharacter, (int)(unsigned char)t[k].graphCharacter);
+				text += buf;
+			}
+			text += "\n";
+		}
+	*out = (char*)std::malloc(text.size() + 1);
+	std::memcpy(*out, text.c_str(), text.size() + 1);
+	return status;
+}
+
+// OrderSeeds (src/GraphAligner.h:250-312) on n seeds in the layout above, in
+// place; 2 when a reference assertion fires.
+int orc_order_seeds(void* g, int64_t* seeds, int64_t n)
+{
+	std::vector<SeedHit> hits(n);
+	for (int64_t i = 0; i < n; i++)
+	{
+		const int64_t* x = seeds + 10 * i;
+		hits[i].nodeID = (int)x[0];
+		hits[i].nodeOffset = x[1];
+		hits[i].seqPos = x[2];
+		hits[i].matchLen = x[3];
+		hits[i].reverse = x[4] != 0;
+		hits[i].alignmentGraphNodeId = x[5] < 0 ? NMAX : (size_t)x[5];
+		hits[i].alignmentGraphNodeOffset = x[6] < 0 ? NMAX : (size_t)x[6];
+		hits[i].rawSeedGoodness = x[7];
+		hits[i].seedGoodness = x[8];
+		hits[i].seedClusterSize = x[9];
+	}
+	Params p;
+	Aligner a(p, *(AlignmentGraph*)g);
+	try
+	{
+		a.orderSeedsByChaining(hits);
+	}
+	catch (const AssertFail&)
+	{
+		return 2;
+	}
+	for (int64_t i = 0; i < n; i++)
+	{
+		int64_t* o = seeds + 10 * i;
+		o[0] = hits[i].nodeID;
+		o[1] = hits[i].nodeOffset;
+		o[2] = hits[i].seqPos;
+		o[3] = hits[i].matchLen;
+		o[4] = hits[i].reverse;
+		o[5] = hits[i].alignmentGraphNodeId == NMAX ? -1 : (int64_t)hits[i].alignmentGraphNodeId;
+		o[6] = hits[i].alignmentGraphNodeOffset == NMAX ? -1 : (int64_t)hits[i].alignmentGraphNodeOffset;
+		o[7] = hits[i].rawSeedGoodness;
+		o[8] = hits[i].seedGoodness;
+		o[9] = hits[i].seedClusterSize;
+	}
+	return 0;
 }
 
 // WordSlice primitives, exposed for property tests against brute force.

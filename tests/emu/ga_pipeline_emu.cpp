@@ -7,6 +7,7 @@
 #define GA_THREAD_HOST_TEST 1
 #include <atomic>
 #include <mutex>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -242,6 +243,16 @@ public:
 		computePlanes();
 	}
 
+	void prepareReads(uint32_t first, uint32_t count) override
+	{
+		forEachParallel(count, [&](size_t i) {
+			uint64_t off = rs_.offset[first + i];
+			uint32_t len = rs_.length[first + i];
+			for (uint32_t k = 0; k < len; k++) reads_[revBase_ + off + k] = gad::complementByte(reads_[off + len - 1 - k]);
+		});
+		computePlanes();
+	}
+
 	const char* readsBase() const override { return reads_.data(); }
 	uint64_t reverseBase() const override { return revBase_; }
 
@@ -400,6 +411,82 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 		statsOut[5] = st.launches;
 	}
 	return 0;
+}
+
+// AlignOneWay with the caller's seeds for one read; output text in the layout
+// of the oracle's orc_align_one_way.
+int pipe_align_one_way(void* ih, const PipeOptions* po, const char* seq, int64_t len, const int64_t* seeds, int64_t nseeds, int select, int mapq, char** outText)
+{
+	auto* x = (EmuIndex*)ih;
+	ga::AlignOptions o;
+	o.ext.bandwidth = po->bandwidth;
+	o.ext.errorCost = po->errorCost;
+	o.ext.xdrop = po->xdrop;
+	o.ext.pad = 0;
+	o.ext.maxCellsPerSlice = po->maxCellsPerSlice <= 0 ? INT64_MAX : po->maxCellsPerSlice;
+	o.xscoreErrorCost = po->errorCost;
+	o.seedExtendDensity = po->seedExtendDensity;
+	o.multimapScoreFraction = po->multimapScoreFraction;
+	o.sloppyOptimizations = po->sloppy != 0;
+	o.minSeedClusterSize = po->minClusterSize;
+	o.hostThreads = 4;
+	o.maxSpeculativeSeeds = po->maxSpeculativeSeeds < 1 ? 1 : (uint32_t)po->maxSpeculativeSeeds;
+	o.minAlignmentScore = po->minAlignmentScore;
+	std::vector<ga::SeedHit> hits(nseeds);
+	for (int64_t i = 0; i < nseeds; i++)
+	{
+		const int64_t* v = seeds + 10 * i;
+		hits[i].nodeID = (int)v[0];
+		hits[i].nodeOffset = v[1];
+		hits[i].seqPos = v[2];
+		hits[i].matchLen = v[3];
+		hits[i].reverse = v[4] != 0;
+		hits[i].alignmentGraphNodeId = v[5] < 0 ? ~0ull : (uint64_t)v[5];
+		hits[i].alignmentGraphNodeOffset = v[6] < 0 ? ~0ull : (uint64_t)v[6];
+		hits[i].rawSeedGoodness = v[7];
+		hits[i].seedGoodness = v[8];
+		hits[i].seedClusterSize = v[9];
+	}
+	HostBackend be;
+	std::vector<ga::Alignment> alns;
+	ga::ReadOutcome out;
+	ga::AlignStats st;
+	try
+	{
+		be.uploadReads({seq}, {(uint32_t)len});
+		ga::alignOneWayResident(be, *x->g, o, 0, 1, &hits, &alns, &out, st);
+	}
+	catch (const std::exception&)
+	{
+		return -1;
+	}
+	int status = out.status;
+	if (status == 0 && mapq && !ga::addMappingQualities(alns)) status = 2;
+	if (status == 0 && select)
+	{
+		if (!alns.empty()) alns = ga::selectAlignments(alns, o.minAlignmentScore, o.multimapScoreFraction);
+		ga::sortForOutput(alns);
+	}
+	std::string text;
+	char buf[256];
+	std::snprintf(buf, sizeof(buf), "S %d %llu\n", status, status == 0 ? (unsigned long long)out.seedsExtended : 0ull);
+	text += buf;
+	if (status == 0)
+		for (const auto& a : alns)
+		{
+			std::snprintf(buf, sizeof(buf), "A %llu %llu %llu %.17g %llu %d %zu\n", (unsigned long long)a.alignmentStart, (unsigned long long)a.alignmentEnd, (unsigned long long)a.alignmentScore, a.alignmentXScore, (unsigned long long)a.seedGoodness, a.mappingQuality, a.trace.size());
+			text += buf;
+			for (size_t k = 0; k < a.trace.size(); k++)
+			{
+				const auto& t = a.trace[k];
+				std::snprintf(buf, sizeof(buf), "%s%u,%u,%u,%d,%d,%d", k ? " " : "", t.node, t.nodeOffset, t.seqPos, (int)t.nodeSwitch, (int)(unsigned char)t.seqChar, (int)(unsigned char)t.graphChar);
+				text += buf;
+			}
+			text += "\n";
+		}
+	*outText = (char*)std::malloc(text.size() + 1);
+	std::memcpy(*outText, text.c_str(), text.size() + 1);
+	return status;
 }
 
 void pipe_free(void* p) { std::free(p); }

@@ -80,6 +80,11 @@ def lib(ndebug=False):
                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), i64p, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int32)]
         L.orc_free.argtypes = [vp]
+        L.orc_order_seeds.restype = ctypes.c_int
+        L.orc_order_seeds.argtypes = [vp, i64p, ctypes.c_int64]
+        L.orc_align_one_way.restype = ctypes.c_int
+        L.orc_align_one_way.argtypes = [vp, ctypes.POINTER(OrcParams), ctypes.c_char_p, ctypes.c_int64, i64p, ctypes.c_int64,
+                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         u64 = ctypes.c_uint64
         L.orc_merge.argtypes = [u64, u64, ctypes.c_int32, u64, u64, ctypes.c_int32, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int32)]
         L.orc_difference_masks.argtypes = [u64, u64, u64, u64, ctypes.c_int32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
@@ -212,3 +217,25 @@ def align_reads(graph, index, params, density, names, seqs, nthreads=1):
     text = ctypes.string_at(out.value).decode()
     graph.L.orc_free(out)
     return text, status, secs
+
+
+def align_one_way(graph, params, seq, seeds, select=False, mapq=False):
+    """GraphAligner::AlignOneWay with the given seeds (int64 rows of 10 SeedHit
+    fields, -1 for an unset alignment-graph node), optionally followed by
+    AddMappingQualities and the selection/output sorts. Returns text: "S status
+    seedsExtended", then per alignment an "A ..." line and a trace line."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.int64).reshape(-1, 10)
+    out = ctypes.c_void_p()
+    graph.L.orc_align_one_way(graph.h, ctypes.byref(params), seq.encode(), len(seq),
+                              seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(seeds), int(select), int(mapq),
+                              ctypes.byref(out))
+    text = ctypes.string_at(out.value).decode()
+    graph.L.orc_free(out)
+    return text
+
+
+def order_seeds(graph, seeds):
+    """OrderSeeds on a copy of the seed rows; None when a reference assertion fires."""
+    out = np.ascontiguousarray(seeds, dtype=np.int64).reshape(-1, 10).copy()
+    rc = graph.L.orc_order_seeds(graph.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(out))
+    return None if rc == 2 else out

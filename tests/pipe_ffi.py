@@ -49,6 +49,10 @@ def lib(variant=""):
                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
         L.pipe_free.argtypes = [ctypes.c_void_p]
         L.pipe_set_fault.argtypes = [ctypes.c_int]
+        L.pipe_align_one_way.restype = ctypes.c_int
+        L.pipe_align_one_way.argtypes = [ctypes.c_void_p, ctypes.POINTER(PipeOptions), ctypes.c_char_p, ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]
         _LIBS[variant] = L
     return _LIBS[variant]
 
@@ -131,3 +135,19 @@ def align_reads(index, oracle_params, density, names, seqs, device_loop=False, s
     index.L.pipe_free(out)
     return text, status, dict(rounds=int(stats[0]), extensions=int(stats[1]), seeds=int(stats[2]), word_steps=int(stats[3]),
                               unused=int(stats[4]), launches=int(stats[5]))
+
+
+def align_one_way(index, oracle_params, seq, seeds, select=False, mapq=False, speculative=64):
+    """AlignOneWay with the given seeds (int64 rows of 10 SeedHit fields); the
+    result as text in the layout of the oracle's align_one_way."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.int64).reshape(-1, 10)
+    opt = options(oracle_params, 0.0, False, speculative)
+    out = ctypes.c_void_p()
+    rc = index.L.pipe_align_one_way(index.h, ctypes.byref(opt), seq.encode(), len(seq),
+                                    seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(seeds), int(select), int(mapq),
+                                    ctypes.byref(out))
+    if rc < 0:
+        raise RuntimeError("pipeline failed")
+    text = ctypes.string_at(out.value).decode()
+    index.L.pipe_free(out)
+    return text
