@@ -3,5 +3,5 @@
 Python mirror of the reference boundary (src/GraphAlignerWrapper.h:52-60) over the
 C ABI in include/graphaligner_amd.h; the work runs in the native library.
 """
-from .api import (Graph, Context, Index, params_preset, extend_batch, seeds, align_reads, upload_reads,  # noqa: F401
-                  align_resident, order_seeds, align_one_way, AlignmentResults)
+from .api import (Graph, Context, Index, params_preset, extend_batch, extend_batch_noalloc, seeds, align_reads,  # noqa: F401
+                  upload_reads, align_resident, order_seeds, align_one_way, AlignmentResults)

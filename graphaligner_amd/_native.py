@@ -10,9 +10,9 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libgraphaligner_amd.so")
 
-GA_OK, GA_EXT_FAILED, GA_ASSERT, GA_EOVERFLOW, GA_EBADARG, GA_EIO, GA_ENODEVICE, GA_EHIP, GA_ENOMEM, GA_ESTALE = range(10)
+GA_OK, GA_EXT_FAILED, GA_ASSERT, GA_EOVERFLOW, GA_EBADARG, GA_EIO, GA_ENODEVICE, GA_EHIP, GA_ENOMEM, GA_ESTALE, GA_ESCRATCH = range(11)
 STATUS_NAMES = {0: "GA_OK", 1: "GA_EXT_FAILED", 2: "GA_ASSERT", 3: "GA_EOVERFLOW", 4: "GA_EBADARG", 5: "GA_EIO",
-                6: "GA_ENODEVICE", 7: "GA_EHIP", 8: "GA_ENOMEM", 9: "GA_ESTALE"}
+                6: "GA_ENODEVICE", 7: "GA_EHIP", 8: "GA_ENOMEM", 9: "GA_ESTALE", 10: "GA_ESCRATCH"}
 
 # every entry point the header declares
 EXPORTED = ["ga_params_preset", "ga_graph_load_gfa", "ga_graph_free", "ga_graph_num_nodes", "ga_graph_num_edges",
@@ -21,7 +21,7 @@ EXPORTED = ["ga_params_preset", "ga_graph_load_gfa", "ga_graph_free", "ga_graph_
             "ga_align_reads", "ga_reads_upload", "ga_align_resident", "ga_context_synchronize", "ga_free",
             "ga_order_seeds", "ga_align_one_way", "ga_result_reads", "ga_result_status", "ga_result_seeds_extended",
             "ga_result_count", "ga_result_item", "ga_result_trace", "ga_select_alignments", "ga_add_mapping_qualities",
-            "ga_gaf_line", "ga_result_free"]
+            "ga_gaf_line", "ga_result_free", "ga_extend_batch_noalloc"]
 
 READ_STATUS_NAMES = {0: "aligned", 1: "no seeds", 2: "assertion (read dropped)", 3: "no alignment", 4: "workspace overflow"}
 
@@ -124,5 +124,7 @@ def lib():
         L.ga_add_mapping_qualities.argtypes = [vp, i64]
         L.ga_gaf_line.argtypes = [vp, vp, i64, i64, ctypes.c_char_p, i64, i32, ctypes.c_char_p, i64, ctypes.POINTER(i64)]
         L.ga_result_free.argtypes = [vp]
+        L.ga_extend_batch_noalloc.argtypes = [vp, vp, ctypes.POINTER(Params), i64, ctypes.POINTER(ctypes.c_char_p), vp, vp, vp,
+                                              i64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib

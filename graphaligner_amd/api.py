@@ -118,6 +118,29 @@ def extend_batch(ctx, graph, params, seqs, bigraph_nodes, offsets, summaries=Fal
     return out, ms.value
 
 
+def extend_batch_noalloc(ctx, graph, params, seqs, bigraph_nodes, offsets):
+    """extend_batch through the entry point that allocates no device memory
+    (ga_extend_batch_noalloc; the graph must be resident on the context).
+    Returns per extension (status, score, trace[n,4] end first) and the kernel ms."""
+    n = len(seqs)
+    seqs_b = [s.encode() for s in seqs]
+    lens = np.array([len(s) for s in seqs], dtype=np.int64)
+    cap = int(2 * lens.max() + 256) if n else 256
+    traces = np.zeros(4 * cap * max(n, 1), dtype=np.int64)
+    ntrace = np.zeros(max(n, 1), dtype=np.int64)
+    scores = np.zeros(max(n, 1), dtype=np.int32)
+    status = np.zeros(max(n, 1), dtype=np.int32)
+    ms = ctypes.c_double()
+    arr = (ctypes.c_char_p * max(n, 1))(*seqs_b)
+    nodes = np.asarray(bigraph_nodes, dtype=np.int32)
+    offs = np.asarray(offsets, dtype=np.int64)
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    _check(N.lib().ga_extend_batch_noalloc(ctx.h, graph.h, ctypes.byref(params), n, arr, P(lens), P(nodes), P(offs), cap,
+                                           P(traces), P(ntrace), P(scores), P(status), ctypes.byref(ms)), "ga_extend_batch_noalloc")
+    out = [(int(status[i]), int(scores[i]), traces[4 * cap * i: 4 * cap * i + 4 * ntrace[i]].reshape(-1, 4)) for i in range(n)]
+    return out, ms.value
+
+
 class Index:
     """Minimizer index (MinimizerSeeder) built from params' minimizer settings."""
 

@@ -10,6 +10,7 @@
 #include <thread>
 #include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <sys/mman.h>
@@ -695,6 +696,49 @@ ga_status_t ga_gaf_line(const ga_graph_t* graph, const ga_alignment_result_t* r,
 }
 
 void ga_result_free(ga_alignment_result_t* r) { delete r; }
+
+ga_status_t ga_extend_batch_noalloc(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, double* kernelMs)
+{
+	if (!ctx || !graph || !params || n < 0 || (n > 0 && (!seqs || !lens || !bigraphNodeIds || !offsets || !traces || !ntrace || !scores || !status))) return GA_EBADARG;
+	if ((uint64_t)n > Context::kNoAllocMaxRequests) return GA_ESCRATCH;
+	const GraphHost& g = graph->host;
+	// per-extension arrays of fixed size (n is bounded): no heap allocation; one
+	// call at a time per process
+	static std::mutex callMutex;
+	static uint32_t lenv[Context::kNoAllocMaxRequests], nodev[Context::kNoAllocMaxRequests], offv[Context::kNoAllocMaxRequests];
+	static gad::ExtResult resv[Context::kNoAllocMaxRequests];
+	std::lock_guard<std::mutex> lock(callMutex);
+	for (int64_t i = 0; i < n; i++)
+	{
+		auto it = g.originalSize.find(bigraphNodeIds[i]);
+		if (it == g.originalSize.end() || offsets[i] < 0 || (uint64_t)offsets[i] >= it->second) return GA_EBADARG;
+		if (lens[i] <= 0 || lens[i] > (1 << 24) || traceCap < 2 * lens[i] + 256 || !seqs[i]) return GA_EBADARG;
+		const uint32_t node = g.unitigNode(bigraphNodeIds[i], offsets[i]);
+		lenv[i] = (uint32_t)lens[i];
+		nodev[i] = node;
+		offv[i] = (uint32_t)(offsets[i] - g.nodeOff[node]);
+	}
+	return guarded([&]() {
+		const gad::DevGraph* dg = ctx->ctx->residentGraph(g);
+		if (!dg) return GA_EBADARG;
+		if (n == 0)
+		{
+			if (kernelMs) *kernelMs = 0;
+			return GA_OK;
+		}
+		if (!ctx->ctx->extendNoAlloc(*dg, extParamsOf(*params), (uint32_t)n, seqs, lenv, nodev, offv, traceCap, traces, resv, kernelMs)) return GA_ESCRATCH;
+		ga_status_t rc = GA_OK;
+		for (int64_t i = 0; i < n; i++)
+		{
+			const gad::ExtResult& r = resv[i];
+			status[i] = r.status;
+			scores[i] = r.score;
+			ntrace[i] = r.status == gad::EXT_OVERFLOW ? 0 : r.ntrace;
+			if (r.status == gad::EXT_OVERFLOW) rc = GA_EOVERFLOW;
+		}
+		return rc;
+	});
+}
 
 void ga_free(void* p) { std::free(p); }
 

@@ -6,6 +6,7 @@
 #define GA_THREAD_PER_EXTENSION 1
 #include <hip/hip_runtime.h>
 #include <cstdlib>
+#include <mutex>
 #include "ga_extend_core.h"
 #include "ga_read_core.h"
 #include "ga_kernels.hpp"
@@ -13,6 +14,35 @@
 #include "ga_trace_codes.h"
 
 namespace gad {
+
+// Scratch of the allocation-free extension entry point (requests, results,
+// sequences, match-mask planes, traces and workspaces of one call). In host code
+// the name stands for no device address: host code uses only the address that
+// noAllocScratchAddress looked up.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noAllocScratchAddress(int device, void** ptr)
+{
+	constexpr int kMaxDevices = 64;
+	static std::mutex m;
+	static void* addr[kMaxDevices] = {};
+	if (device < 0 || device >= kMaxDevices || !ptr) return hipErrorInvalidValue;
+	std::lock_guard<std::mutex> lock(m);
+	if (!addr[device])
+	{
+		int current = -1;
+		hipError_t e = hipGetDevice(&current);
+		if (e != hipSuccess) return e;
+		if (current != device) return hipErrorInvalidDevice;
+		void* p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess) return e;
+		if (!p) return hipErrorInvalidSymbol;
+		addr[device] = p;
+	}
+	*ptr = addr[device];
+	return hipSuccess;
+}
 
 template <int MinWavesPerSimd>
 __global__ void __launch_bounds__(64, MinWavesPerSimd) extendThreadKernel(DevGraph g, ExtParams p, const ExtRequest* __restrict__ reqs, uint32_t nreq, uint32_t lanesPerWave, const char* __restrict__ seqbuf, uint64_t* __restrict__ tracebuf, ExtResult* __restrict__ results, unsigned char* __restrict__ arenas, const uint64_t* __restrict__ arenaOffsets, ArenaCaps caps, SliceSummary* __restrict__ sums, const uint64_t* __restrict__ sumOffsets, const uint64_t* __restrict__ planes)

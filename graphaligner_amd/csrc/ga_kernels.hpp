@@ -38,6 +38,13 @@ hipError_t launchGatherU64(hipStream_t stream, const uint64_t* src, const uint64
 // *escapeCount (which the caller zeroes; escapeCap pairs fit).
 hipError_t launchTraceCodes(hipStream_t stream, const DevGraph& g, const uint64_t* src, const uint64_t* srcOff, const uint32_t* count, const uint64_t* dstOff, uint32_t nranges, uint8_t* codes, uint64_t* escapes, uint32_t* escapeCount, uint64_t escapeCap);
 
+// Device scratch of the allocation-free extension entry point: one file-scope
+// array per GPU (the library holds it on every GPU it runs on until the process
+// ends). noAllocScratchAddress returns this device's copy of it; call it with
+// `device` current. It looks the address up once per device and keeps it.
+constexpr uint64_t kNoAllocScratchBytes = 128ull << 20;
+hipError_t noAllocScratchAddress(int device, void** ptr);
+
 hipError_t launchRevcomp(hipStream_t stream, const char* reads, char* rev, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, uint32_t maxLen);
 
 } // namespace gad

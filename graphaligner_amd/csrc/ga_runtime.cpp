@@ -69,6 +69,9 @@ Context::Context(int device) : device_(device)
 	GA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 	GA_HIP(hipEventCreate(&ev0_));
 	GA_HIP(hipEventCreate(&ev1_));
+	void* sp = nullptr;
+	GA_HIP(gad::noAllocScratchAddress(device, &sp));
+	scratch_ = (unsigned char*)sp;
 	for (size_t k = 0; k < kExtendLanes; k++)
 	{
 		auto L = std::make_unique<ExtendLane>();
@@ -486,6 +489,145 @@ void Context::extend(const GraphHost& g, const gad::ExtParams& p, const std::vec
 }
 
 static uint64_t roundUp256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
+
+const gad::DevGraph* Context::residentGraph(const GraphHost& g)
+{
+	std::lock_guard<std::mutex> lock(cacheMutex_);
+	for (auto& c : graphs_)
+		if (c->uid == g.uid) return &c->view;
+	return nullptr;
+}
+
+namespace {
+
+// Host staging of the no-allocation entry point: fixed arrays, one call at a
+// time per process (calls on one GPU share its device scratch anyway).
+struct NoAllocStaging
+{
+	std::mutex m;
+	gad::ExtRequest reqs[Context::kNoAllocMaxRequests];
+	uint64_t arenaOff[Context::kNoAllocMaxRequests];
+	gad::ExtResult res[Context::kNoAllocMaxRequests];
+	uint32_t job[Context::kNoAllocMaxRequests];
+};
+NoAllocStaging gNoAlloc;
+
+} // namespace
+
+bool Context::extendNoAlloc(const gad::DevGraph& dg, const gad::ExtParams& p, uint32_t n, const char* const* seqs, const uint32_t* lens, const uint32_t* startNodes, const uint32_t* startOffsets, int64_t traceCap, int64_t* traces, gad::ExtResult* results, double* kernelMs)
+{
+	if (n > kNoAllocMaxRequests || !scratch_) return false;
+	std::lock_guard<std::mutex> lock(gNoAlloc.m);
+	NoAllocStaging& S = gNoAlloc;
+	// scratch layout: requests | workspace offsets | results | sequences |
+	// planes | traces | workspaces, every part on a 256-byte boundary
+	uint64_t seqBytes = 0, traceItems = 0;
+	for (uint32_t i = 0; i < n; i++)
+	{
+		seqBytes += lens[i];
+		traceItems += traceCapFor(lens[i]);
+	}
+	const uint64_t offReqs = 0;
+	const uint64_t offArenaOff = offReqs + roundUp256((uint64_t)n * sizeof(gad::ExtRequest));
+	const uint64_t offResults = offArenaOff + roundUp256((uint64_t)n * sizeof(uint64_t));
+	const uint64_t offSeqs = offResults + roundUp256((uint64_t)n * sizeof(gad::ExtResult));
+	const uint64_t offPlanes = offSeqs + roundUp256(seqBytes + 1);
+	const uint64_t offTraces = offPlanes + roundUp256(gad::eqPlaneWords(seqBytes + 1) * sizeof(uint64_t));
+	const uint64_t offArenas = offTraces + roundUp256(traceItems * sizeof(uint64_t));
+	gad::ArenaCaps caps = initialArenaCaps();
+	uint64_t arenaTotal = 0;
+	for (uint32_t i = 0; i < n; i++) arenaTotal += roundUp256(gad::requestArenaBytes(lens[i], caps));
+	if (offArenas + arenaTotal > gad::kNoAllocScratchBytes) return false;
+	GA_HIP(hipSetDevice(device_));
+	unsigned char* base = scratch_;
+	char* dSeqs = (char*)(base + offSeqs);
+	uint64_t* dPlanes = (uint64_t*)(base + offPlanes);
+	uint64_t* dTraces = (uint64_t*)(base + offTraces);
+	uint64_t at = 0, tat = 0;
+	for (uint32_t i = 0; i < n; i++)
+	{
+		if (lens[i] > 0) GA_HIP(hipMemcpyAsync(dSeqs + at, seqs[i], lens[i], hipMemcpyHostToDevice, stream_));
+		gad::ExtRequest& r = S.reqs[i];
+		r.seqOffset = at;
+		r.traceOffset = tat;
+		r.len = lens[i];
+		r.startNode = startNodes[i];
+		r.startOffset = startOffsets[i];
+		r.traceCap = traceCapFor(lens[i]);
+		S.job[i] = i;
+		at += lens[i];
+		tat += r.traceCap;
+	}
+	// the planes of the last block read the byte after the last sequence: zero it
+	static const char zero = 0;
+	GA_HIP(hipMemcpyAsync(dSeqs + at, &zero, 1, hipMemcpyHostToDevice, stream_));
+	GA_HIP(gad::launchEqPlanes(stream_, dSeqs, seqBytes + 1, dPlanes));
+	float msTotal = 0;
+	// first attempt over every request, then reruns of the overflowed ones with
+	// larger workspaces, each as long as the scratch holds their workspaces
+	uint32_t m = n;
+	for (int attempt = 0; attempt < kArenaAttempts && m > 0; attempt++)
+	{
+		uint64_t aoff = 0;
+		for (uint32_t k = 0; k < m; k++)
+		{
+			S.arenaOff[k] = aoff;
+			aoff += roundUp256(gad::requestArenaBytes(S.reqs[k].len, caps));
+		}
+		if (offArenas + aoff > gad::kNoAllocScratchBytes) break;
+		GA_HIP(hipMemcpyAsync(base + offReqs, S.reqs, (size_t)m * sizeof(gad::ExtRequest), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipMemcpyAsync(base + offArenaOff, S.arenaOff, (size_t)m * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+		GA_HIP(hipEventRecord(ev0_, stream_));
+		GA_HIP(gad::launchExtend(stream_, m, lanesPerWave(m), kernelWavesPerSimd, dg, p, (const gad::ExtRequest*)(base + offReqs), dSeqs, dTraces, (gad::ExtResult*)(base + offResults), base + offArenas, (const uint64_t*)(base + offArenaOff), caps, nullptr, nullptr, dPlanes));
+		GA_HIP(hipEventRecord(ev1_, stream_));
+		GA_HIP(hipMemcpyAsync(S.res, base + offResults, (size_t)m * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, stream_));
+		GA_HIP(hipStreamSynchronize(stream_));
+		float ms = 0;
+		GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+		msTotal += ms;
+		uint32_t again = 0;
+		for (uint32_t k = 0; k < m; k++)
+		{
+			const gad::ExtResult& r = S.res[k];
+			if (r.status < gad::EXT_OK || r.status > gad::EXT_OVERFLOW || r.ntrace > S.reqs[k].traceCap)
+				throw HipError("extension kernel returned an inconsistent result record");
+			const uint32_t j = S.job[k];
+			results[j] = r;
+			if (r.status == gad::EXT_OVERFLOW)
+			{
+				S.reqs[again] = S.reqs[k];
+				S.job[again] = j;
+				again++;
+				continue;
+			}
+			// the packed items into the front of the caller's trace block, then
+			// unpacked in place from the last item down (item k's four words
+			// start at or after packed word k, which is read first)
+			int64_t* t = traces + 4 * (int64_t)j * traceCap;
+			if (r.ntrace > 0) GA_HIP(hipMemcpyAsync(t, dTraces + S.reqs[k].traceOffset, (size_t)r.ntrace * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+		}
+		GA_HIP(hipStreamSynchronize(stream_));
+		for (uint32_t k = 0; k < m; k++)
+		{
+			const gad::ExtResult& r = S.res[k];
+			if (r.status == gad::EXT_OVERFLOW) continue;
+			int64_t* t = traces + 4 * (int64_t)S.job[k] * traceCap;
+			for (int64_t q = (int64_t)r.ntrace - 1; q >= 0; q--)
+			{
+				const uint64_t v = (uint64_t)t[q];
+				const uint32_t hi = (uint32_t)(v >> 32);
+				t[4 * q] = (uint32_t)v;
+				t[4 * q + 1] = (hi >> 1) & 63;
+				t[4 * q + 2] = (int64_t)(hi >> 7) - 1;
+				t[4 * q + 3] = hi & 1;
+			}
+		}
+		m = again;
+		growArenaCaps(caps);
+	}
+	if (kernelMs) *kernelMs = msTotal;
+	return true;
+}
 
 static uint64_t readLoopBytes(uint32_t len, const ReadLoopCaps& caps)
 {

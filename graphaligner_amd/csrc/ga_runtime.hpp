@@ -105,6 +105,18 @@ public:
 	void prepareReads(uint32_t first, uint32_t count) override;
 	const char* readsBase() const override { return readBytes_.as<char>(); }
 	void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) override;
+	// The graph's device copy if it is resident on this context, else null
+	// (never uploads).
+	const gad::DevGraph* residentGraph(const GraphHost& g);
+	// Extensions of ga_extend_batch_noalloc: n requests (split node and offset)
+	// in the file-scope device scratch, with no device or host allocation. Traces
+	// go to traces + 4 * i * traceCap as rows (split node, offset, seqPos, switch).
+	// Returns false, touching nothing, when the batch does not fit the scratch
+	// (kNoAllocMaxRequests requests, kNoAllocScratchBytes bytes); results of
+	// extensions that still overflow after the reruns the scratch allows keep
+	// status EXT_OVERFLOW.
+	bool extendNoAlloc(const gad::DevGraph& dg, const gad::ExtParams& p, uint32_t n, const char* const* seqs, const uint32_t* lens, const uint32_t* startNodes, const uint32_t* startOffsets, int64_t traceCap, int64_t* traces, gad::ExtResult* results, double* kernelMs);
+	static constexpr uint32_t kNoAllocMaxRequests = 4096;
 	uint64_t reverseBase() const override { return revBase_; }
 	// device workspace of the launches in flight (MI355X has 288 GB of HBM): a
 	// device seed-loop launch may use all of it, an extension launch its lane's
@@ -154,6 +166,8 @@ private:
 	ReadSet reads_;
 	uint64_t revBase_ = 0;
 	uint32_t lanesOverride_ = 0;
+	// this device's copy of the no-allocation scratch (looked up at construction)
+	unsigned char* scratch_ = nullptr;
 };
 
 } // namespace ga

@@ -28,7 +28,8 @@ typedef enum
 	GA_ENODEVICE = 6,
 	GA_EHIP = 7,
 	GA_ENOMEM = 8,
-	GA_ESTALE = 9       /* ga_align_resident after another call replaced the uploaded read set */
+	GA_ESTALE = 9,      /* ga_align_resident after another call replaced the uploaded read set */
+	GA_ESCRATCH = 10    /* the batch exceeds ga_extend_batch_noalloc's fixed scratch */
 } ga_status_t;
 
 typedef struct ga_graph ga_graph_t;
@@ -116,6 +117,24 @@ typedef struct
  * status[i] is GA_OK, GA_EXT_FAILED or GA_ASSERT. Summaries (optional, may be NULL)
  * take sumCap rows per extension. */
 ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, ga_slice_summary_t* summaries, int64_t sumCap, int64_t* nslices, double* kernelMs);
+
+/* No counterpart in the reference: ga_extend_batch for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid allocation). Same extensions, trace rows, scores and
+ * statuses as ga_extend_batch (no slice summaries); every device buffer of the
+ * call (requests, sequences, match masks, traces, workspaces) is carved from
+ * one file-scope device array of 128 MiB that each GPU holds until the process
+ * ends, so the call makes no hipMalloc/hipFree. Limits: at most 4096
+ * extensions, and their sequences, traces (16 * len + 2 KiB each) and
+ * workspaces (about 100 bytes per base) within the 128 MiB; a larger batch
+ * fails with GA_ESCRATCH and nothing is run. Extensions whose workspaces still
+ * overflow after the reruns that fit keep status GA_EOVERFLOW (the call then
+ * returns GA_EOVERFLOW). The graph must already be resident on the context's
+ * GPU (an earlier ga_extend_batch or alignment call with it made it so), else
+ * GA_EBADARG. The scratch is shared by every call on that GPU and keeps the
+ * last call's contents: calls that overlap on one GPU (from several threads or
+ * contexts) race and must be serialised by the caller. */
+ga_status_t ga_extend_batch_noalloc(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, double* kernelMs);
 
 /* Minimizer index over the graph: MinimizerSeeder's constructor and initMaxCount
  * (src/MinimizerSeeder.cpp:299-491, 557-575) with k = params->minimizerLength,
