@@ -1990,6 +1990,7 @@ struct Extension
 		sliceEq((currentSlice - 1) * 64, eq);
 		int32_t lastScore = SMAX - 1;
 		uint32_t steps = 0;
+		uint32_t colsUpTo = 0; // last column of the current node's recomputation
 		while (back.seq != -1 && status == EXT_OK)
 		{
 			if (++steps > traceCap)
@@ -1999,7 +2000,12 @@ struct Extension
 			}
 			uint32_t newSlice = (uint32_t)(back.seq / 64 + 1);
 			uint32_t newNode = back.node;
-			if (newSlice != currentSlice || newNode != currentNode)
+			// the node's columns are recomputed when the walk enters another node
+			// or slice, and again when it re-enters the same node of the same
+			// slice right of the columns computed so far (through an edge from
+			// the node to itself: its first column's in-neighbour is its own last
+			// column)
+			if (newSlice != currentSlice || newNode != currentNode || back.off > colsUpTo)
 			{
 				if (newSlice != currentSlice) sliceEq((newSlice - 1) * 64, eq);
 				currentSlice = newSlice;
@@ -2010,6 +2016,7 @@ struct Extension
 					break;
 				}
 				recalc(currentSlice, currentNode, eq, cols, back.off);
+				colsUpTo = back.off;
 				if (status != EXT_OK) break;
 			}
 			const SliceHdr& ch = hdr[currentSlice];

@@ -48,21 +48,57 @@ PRESETS = {
 }
 
 
+class DbgConfig(ctypes.Structure):
+    _fields_ = [("genomeLength", ctypes.c_int64), ("repeatCopies", ctypes.c_int64), ("familyLenMin", ctypes.c_int64),
+                ("familyLenMax", ctypes.c_int64), ("copiesMin", ctypes.c_int64), ("copiesMax", ctypes.c_int64),
+                ("divergenceMax", ctypes.c_double), ("tandemArrays", ctypes.c_int64), ("periodMin", ctypes.c_int64),
+                ("periodMax", ctypes.c_int64), ("unitsMin", ctypes.c_int64), ("unitsMax", ctypes.c_int64),
+                ("tandemDivergence", ctypes.c_double), ("k", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+
+
+# Cyclic de Bruijn graph workloads (aligned with preset -x dbg): a seeded genome
+# with 300 inserted copies of repeat families (1-8 kbp, 2-20 copies each, each
+# copy substituted at 0-3 %, either strand) and tandem arrays of period
+# 10-200 bp, compacted into k = 63 unitigs with 62M overlaps. "dbg2m" is the
+# 2 Mbp genome with 10,000 reads of 10-30 kbp at 8 % error; "mini_dbg" a small
+# case for parity tests.
+DBG_PRESETS = {
+    "dbg2m": (dict(genomeLength=2_000_000, repeatCopies=300, familyLenMin=1000, familyLenMax=8000, copiesMin=2, copiesMax=20,
+                   divergenceMax=0.03, tandemArrays=200, periodMin=10, periodMax=200, unitsMin=5, unitsMax=50,
+                   tandemDivergence=0.01, k=63, seed=3),
+              dict(numReads=10000, lengthModel=0, lenA=10000, lenB=30000, sigma=0.0, lenMin=10000, lenMax=30000,
+                   errorMin=0.08, errorMax=0.08, seed=1003)),
+    "mini_dbg": (dict(genomeLength=150_000, repeatCopies=40, familyLenMin=500, familyLenMax=3000, copiesMin=2, copiesMax=20,
+                      divergenceMax=0.03, tandemArrays=20, periodMin=10, periodMax=200, unitsMin=5, unitsMax=50,
+                      tandemDivergence=0.01, k=63, seed=13),
+                 dict(numReads=60, lengthModel=0, lenA=1500, lenB=6000, sigma=0.0, lenMin=1500, lenMax=6000,
+                      errorMin=0.08, errorMax=0.08, seed=1013)),
+}
+
+
 def _lib():
     if not os.path.exists(_LIB):
         raise RuntimeError("synthetic generator not built (run __graft_entry__.build())")
     L = ctypes.CDLL(_LIB)
     L.ga_synth_variation_workload.restype = ctypes.c_int64
     L.ga_synth_variation_workload.argtypes = [ctypes.POINTER(GraphConfig), ctypes.POINTER(ReadConfig), ctypes.c_char_p]
+    L.ga_synth_dbg_workload.restype = ctypes.c_int64
+    L.ga_synth_dbg_workload.argtypes = [ctypes.POINTER(DbgConfig), ctypes.POINTER(ReadConfig), ctypes.c_char_p]
     return L
 
 
 def generate(preset, prefix, overrides_graph=None, overrides_reads=None):
     """Write <prefix>.gfa and <prefix>.fa for a preset; returns the node count."""
-    g, r = PRESETS[preset]
-    g = dict(g, **(overrides_graph or {}))
-    r = dict(r, **(overrides_reads or {}))
-    n = _lib().ga_synth_variation_workload(ctypes.byref(GraphConfig(**g)), ctypes.byref(ReadConfig(**r)), prefix.encode())
+    if preset in DBG_PRESETS:
+        g, r = DBG_PRESETS[preset]
+        g = dict(g, **(overrides_graph or {}))
+        r = dict(r, **(overrides_reads or {}))
+        n = _lib().ga_synth_dbg_workload(ctypes.byref(DbgConfig(**g)), ctypes.byref(ReadConfig(**r)), prefix.encode())
+    else:
+        g, r = PRESETS[preset]
+        g = dict(g, **(overrides_graph or {}))
+        r = dict(r, **(overrides_reads or {}))
+        n = _lib().ga_synth_variation_workload(ctypes.byref(GraphConfig(**g)), ctypes.byref(ReadConfig(**r)), prefix.encode())
     if n < 0:
         raise RuntimeError("generator failed to write " + prefix)
     return n

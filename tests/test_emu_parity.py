@@ -41,3 +41,23 @@ def test_emulated_tangle_effort_and_small_band(mode, workdir):
         assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2]) and a[3] == b[3]
         notvalid += sum(r[9] for r in a[3])
     assert notvalid > 0
+
+
+@pytest.mark.parametrize("mode", ["wave", "thread"])
+def test_emulated_dbg_tangle_effort(mode, workdir):
+    """-x dbg (b = 5) on the cyclic de Bruijn graph with the tangle-effort cap
+    lowered so that capped slices (scoresNotValid) occur."""
+    gfa, names, seqs = workload("mini_dbg", workdir)
+    og = o.Graph(gfa)
+    eg = e.Graph(gfa, mode)
+    sd = o.PRESET_SEEDING["dbg"]
+    idx = o.Index(og, sd["k"], sd["w"], sd["discard"])
+    params = o.preset_params("dbg")
+    params.maxCellsPerSlice = 400
+    notvalid = 0
+    for seq, node, off in extension_cases(og, idx, seqs, 40, seed=13, density=sd["density"]):
+        a = o.extend(og, params, seq, node, off)
+        b = e.extend(eg, params, seq, node, off)
+        assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2]) and a[3] == b[3]
+        notvalid += sum(r[9] for r in a[3])
+    assert notvalid > 0

@@ -99,3 +99,23 @@ def test_fractional_min_alignment_score(workdir, kind):
         assert ot == nt and len(kept) >= len(scores)
     else:
         assert min(kept) >= -3
+
+
+@pytest.mark.parametrize("preset", ["vg", "dbg"])
+def test_traceback_through_self_loops(workdir, preset):
+    """A traceback that re-enters the node it is in, in the same slice, through
+    the node's edge to itself, at a column right of the columns it recomputed
+    on entering: the columns are recomputed again (no stale column store)."""
+    gfa, names, seqs = W.selfloop_workload(workdir)
+    params = O.preset_params(preset)
+    sd = O.PRESET_SEEDING[preset]
+    og = O.Graph(gfa)
+    oi = O.Index(og, sd["k"], sd["w"], sd["discard"])
+    ot, ostat, _ = O.align_reads(og, oi, params, sd["density"], names, seqs, 4)
+    assert (ostat == 0).sum() >= 20
+    for variant in ("", "fullcheck"):
+        _, pg, pi = _pipe(gfa, preset, variant)
+        for mode in ("rounds", "device_loop"):
+            pt, pstat, _ = P.align_reads(pi, params, sd["density"], names, seqs, **P.MODES[mode])
+            assert list(pstat) == list(ostat), (variant, mode)
+            assert pt == ot

@@ -52,8 +52,8 @@ def lib_num_kmers(oracle_index):
 
 
 @pytest.mark.parametrize("mode", sorted(P.MODES))
-@pytest.mark.parametrize("wl", ["mini", "mini_sv"])
-@pytest.mark.parametrize("preset", ["vg", "dbg"])
+@pytest.mark.parametrize("wl,preset", [("mini", "vg"), ("mini", "dbg"), ("mini_sv", "vg"), ("mini_sv", "dbg"), ("mini_dbg", "dbg"),
+                                       ("mini_dbg", "vg")])
 def test_gaf_identical(workdir, wl, preset, mode):
     gfa, names, seqs = W.workload(wl, workdir)
     sd, og, oi, pg, pi = _pair(gfa, preset)

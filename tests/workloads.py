@@ -116,3 +116,51 @@ def fan_workload(workdir, fan=48, regions=12, seed=5):
         names.append(f"fan{i}")
         seqs.append(seq if i % 2 == 0 else revcomp(seq))
     return path, names, seqs
+
+
+def selfloop_workload(workdir, seed=9):
+    """Backbone segments joined through tandem-repeat nodes that carry an edge to
+    themselves (L u + u + 0M): a read crossing a tandem array loops through the
+    same node several times within one 64-row slice, so its traceback re-enters
+    a node of the current slice at a column right of where it entered it."""
+    path = os.path.join(workdir, f"selfloop{seed}.gfa")
+    rng = random.Random(seed)
+    lines = ["H\tVN:Z:1.0"]
+    units, segs = [], []
+    nid = 0
+    prev = None
+    for r in range(8):
+        nid += 1
+        seg = "".join(rng.choice("ACGT") for _ in range(rng.randrange(300, 700)))
+        lines.append(f"S\t{nid}\t{seg}")
+        segs.append(seg)
+        if prev is not None:
+            lines.append(f"L\t{prev}\t+\t{nid}\t+\t0M")
+        prev = nid
+        if r < 7:
+            nid += 1
+            unit = "".join(rng.choice("ACGT") for _ in range(rng.randrange(12, 45)))
+            lines.append(f"S\t{nid}\t{unit}")
+            lines.append(f"L\t{prev}\t+\t{nid}\t+\t0M")
+            lines.append(f"L\t{nid}\t+\t{nid}\t+\t0M")
+            units.append(unit)
+            prev = nid
+    with open(path, "w") as out:
+        out.write("\n".join(lines) + "\n")
+    names, seqs = [], []
+    for i in range(30):
+        parts = []
+        for r in range(8):
+            parts.append(segs[r])
+            if r < 7:
+                parts.append(units[r] * rng.randrange(2, 12))
+        genome = "".join(parts)
+        a = rng.randrange(0, len(genome) // 2)
+        s = list(genome[a:a + rng.randrange(800, 2500)])
+        for k in range(len(s)):
+            if rng.random() < 0.06:
+                s[k] = rng.choice("ACGT")
+        seq = "".join(s)
+        names.append(f"loop{i}")
+        seqs.append(seq if i % 2 == 0 else revcomp(seq))
+    return path, names, seqs
