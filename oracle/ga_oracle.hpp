@@ -72,6 +72,27 @@ struct AssertFail {
 	} while (0)
 #endif
 
+// The reference's debug-only invariant checks (compiled under its
+// EXTRACORRECTNESSASSERTIONS): built with ORC_EXTRA_CHECKS they run in the oracle
+// and count violations instead of dropping reads, so a test can require zero.
+#ifdef ORC_EXTRA_CHECKS
+inline uint64_t& extraCheckFailures()
+{
+	static uint64_t n = 0;
+	return n;
+}
+inline uint64_t& extraChecksRun()
+{
+	static uint64_t n = 0;
+	return n;
+}
+#define ORC_EXTRA(c) \
+	do { \
+		::orc::extraChecksRun()++; \
+		if (!(c)) ::orc::extraCheckFailures()++; \
+	} while (0)
+#endif
+
 using Word = uint64_t;
 using ScoreType = int32_t;
 static constexpr int WordSize = 64;
@@ -1414,7 +1435,12 @@ struct ComponentQueue
 		{
 			if (component != o.component) return component > o.component;
 			if (score != o.score) return score > o.score;
+#ifdef ORC_REVERSE_TIES
+			// the second tie order of the sensitivity measurement: latest insertion first
+			return seq < o.seq;
+#else
 			return seq > o.seq;
+#endif
 		}
 	};
 	std::priority_queue<Item, std::vector<Item>, std::greater<Item>> activeQueues;
@@ -1749,6 +1775,9 @@ struct Extender
 			Eq &= forceEq;
 			Word hinP = (fixedHP >> pos) & 1;
 			Word hinN = (fixedHN >> pos) & 1;
+#ifdef ORC_EXTRA_CHECKS
+			const int hin = (int)hinP - (int)hinN; // getNextSlice overwrites hinP/hinN with the carry-out
+#endif
 			Word newHP, newHN;
 			newWs = getNextSlice(Eq, ws, hinP, hinN, newHP, newHN);
 			if ((forceMask >> pos) & 1)
@@ -1756,6 +1785,9 @@ struct Extender
 				newWs.VP &= AllOnes ^ 1;
 				newWs.VN |= 1;
 			}
+#ifdef ORC_EXTRA_CHECKS
+			assertSliceCorrectness(ws, newWs, Eq, hin);
+#endif
 			hinP = newHP >> (WordSize - 1);
 			hinN = newHN >> (WordSize - 1);
 			ws = newWs;
@@ -1880,9 +1912,82 @@ struct Extender
 			result.cellsProcessed += nodeCalc.cellsProcessed;
 			if (result.cellsProcessed > params.maxCellsPerSlice) break;
 		}
+#ifdef ORC_EXTRA_CHECKS
+		// a slice cut short by the tangle-effort cap has nodes whose in-edges were
+		// never relaxed (the reference marks it scoresNotValid, :646-649): the
+		// boundary check holds only for complete slices
+		if (result.cellsProcessed <= params.maxCellsPerSlice) checkNodeBoundaryCorrectness(currentSlice, previousSlice, sequence, j, currentMinScoreAtEndRow + bandwidth, previousQuitScore);
+#endif
 		calculableQueue.clear();
 		return result;
 	}
+
+#ifdef ORC_EXTRA_CHECKS
+	// src/GraphAlignerBitvectorCommon.h:1062-1076 (single seed: no seed-start
+	// slice), called where the reference calls it: after the boundary fix of the
+	// column's top row (:1447-1448)
+	static void assertSliceCorrectness(WordSlice oldSlice, WordSlice newSlice, Word Eq, int hin)
+	{
+		ScoreType foundMinScore = newSlice.getScoreBeforeStart() + 1;
+		foundMinScore = std::min(foundMinScore, oldSlice.getValue(0) + 1);
+		foundMinScore = std::min(foundMinScore, oldSlice.getScoreBeforeStart() + ((Eq & 1) ? 0 : 1));
+		ORC_EXTRA(newSlice.getScoreBeforeStart() == oldSlice.getScoreBeforeStart() + hin);
+		ORC_EXTRA(newSlice.getValue(0) == foundMinScore);
+		for (int i = 1; i < WordSize; i++)
+		{
+			foundMinScore = newSlice.getValue(i - 1) + 1;
+			foundMinScore = std::min(foundMinScore, oldSlice.getValue(i) + 1);
+			foundMinScore = std::min(foundMinScore, oldSlice.getValue(i - 1) + ((Eq & ((Word)1 << i)) ? 0 : 1));
+			ORC_EXTRA(newSlice.getValue(i) == foundMinScore);
+		}
+	}
+
+	// src/GraphAlignerBitvectorBanded.h:169-227 (single seed: no seed-start slice)
+	void checkNodeBoundaryCorrectness(const NodeSlice& currentSlice, const NodeSlice& previousSlice, const std::string_view& sequence, size_t j, ScoreType maxScore, ScoreType previousMaxScore) const
+	{
+		for (const auto& pair : currentSlice.items)
+		{
+			const size_t node = pair.first;
+			const NodeItem& item = pair.second;
+			if (previousSlice.hasNode(node) && previousSlice.node(node).exists)
+				ORC_EXTRA(item.startSlice.getScoreBeforeStart() <= previousSlice.node(node).startSlice.scoreEnd);
+			bool eq = characterMatch(sequence[j], graph.NodeSequences(node, 0));
+			ScoreType foundMinScore = SMAX;
+			foundMinScore = std::min(foundMinScore, item.startSlice.getScoreBeforeStart() + 1);
+			if (previousSlice.hasNode(node) && previousSlice.node(node).exists && previousSlice.node(node).minScore <= previousMaxScore)
+				foundMinScore = std::min(foundMinScore, previousSlice.node(node).startSlice.scoreEnd + 1);
+			for (auto neighbor : graph.inNeighbors[node])
+			{
+				if (currentSlice.hasNode(neighbor) && currentSlice.node(neighbor).exists)
+				{
+					foundMinScore = std::min(foundMinScore, currentSlice.node(neighbor).endSlice.getValue(0) + 1);
+					foundMinScore = std::min(foundMinScore, currentSlice.node(neighbor).endSlice.getScoreBeforeStart() + (eq ? 0 : 1));
+				}
+				if (previousSlice.hasNode(neighbor) && previousSlice.node(neighbor).exists && previousSlice.node(neighbor).minScore <= previousMaxScore)
+					foundMinScore = std::min(foundMinScore, previousSlice.node(neighbor).endSlice.scoreEnd + (eq ? 0 : 1));
+			}
+			if (item.startSlice.getValue(0) <= maxScore || foundMinScore <= maxScore)
+			{
+				ORC_EXTRA(foundMinScore != SMAX);
+				ORC_EXTRA(item.startSlice.getValue(0) == foundMinScore);
+			}
+			for (int i = 1; i < WordSize; i++)
+			{
+				if (j + i >= sequence.size()) break;
+				eq = characterMatch(sequence[j + i], graph.NodeSequences(node, 0));
+				ScoreType found = item.startSlice.getValue(i - 1) + 1;
+				for (auto neighbor : graph.inNeighbors[node])
+				{
+					if (!currentSlice.hasNode(neighbor)) continue;
+					if (!currentSlice.node(neighbor).exists) continue;
+					found = std::min(found, currentSlice.node(neighbor).endSlice.getValue(i) + 1);
+					found = std::min(found, currentSlice.node(neighbor).endSlice.getValue(i - 1) + (eq ? 0 : 1));
+				}
+				if (item.startSlice.getValue(i) <= maxScore || found <= maxScore) ORC_EXTRA(item.startSlice.getValue(i) == found);
+			}
+		}
+	}
+#endif
 
 	// src/GraphAlignerBitvectorCommon.h:1551-1587
 	DPSlice getInitialSliceExactPosition(int bigraphNodeId, size_t offset) const

@@ -369,6 +369,44 @@ int orc_order_seeds(void* g, int64_t* seeds, int64_t n)
 	return 0;
 }
 
+// iterateMinimizersReal (src/MinimizerSeeder.cpp:104-189) over one string:
+// (position, k-mer) pairs in callback order, at most cap of them; returns the count.
+int64_t orc_minimizers(const char* str, int64_t len, int k, int w, int64_t* out, int64_t cap)
+{
+	std::string s(str, len);
+	int64_t n = 0;
+	iterateMinimizersReal(s, k, w, [&](size_t pos, size_t kmer) {
+		if (n < cap)
+		{
+			out[2 * n] = (int64_t)pos;
+			out[2 * n + 1] = (int64_t)kmer;
+		}
+		n++;
+	});
+	return n;
+}
+
+// The index's table: distinct k-mers (in the table's iteration order), the number
+// of positions of each, and the positions (splitNode << 6 | offset) of each in
+// storage order, concatenated. Returns the number of positions in all.
+int64_t orc_index_table(void* idx, uint64_t* kmers, int64_t* counts, uint64_t* positions)
+{
+	auto* index = (MinimizerIndex*)idx;
+	int64_t at = 0, k = 0;
+	for (const auto& p : index->positions)
+	{
+		if (kmers) kmers[k] = p.first;
+		if (counts) counts[k] = (int64_t)p.second.size();
+		for (uint64_t v : p.second)
+		{
+			if (positions) positions[at] = v;
+			at++;
+		}
+		k++;
+	}
+	return at;
+}
+
 // WordSlice primitives, exposed for property tests against brute force.
 void orc_merge(uint64_t lvp, uint64_t lvn, int32_t lend, uint64_t rvp, uint64_t rvn, int32_t rend, uint64_t* vp, uint64_t* vn, int32_t* end)
 {
@@ -407,6 +445,26 @@ void orc_next_slice(uint64_t eq, uint64_t vp, uint64_t vn, int32_t end, uint64_t
 }
 
 void orc_free(void* p) { std::free(p); }
+
+// Counters of the reference's debug-only invariant checks (builds with
+// ORC_EXTRA_CHECKS; -1 otherwise): checks run and checks violated.
+int64_t orc_extra_checks_run()
+{
+#ifdef ORC_EXTRA_CHECKS
+	return (int64_t)extraChecksRun();
+#else
+	return -1;
+#endif
+}
+
+int64_t orc_extra_check_failures()
+{
+#ifdef ORC_EXTRA_CHECKS
+	return (int64_t)extraCheckFailures();
+#else
+	return -1;
+#endif
+}
 
 int64_t orc_slice_summary_size() { return sizeof(SliceSummary); }
 }

@@ -12,7 +12,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _BUILD = os.path.join(os.path.dirname(_HERE), "oracle", "_build")
 # the reference's assertions on (as it ships) or compiled out (-DNDEBUG, its
 # faster variant: used only to time the CPU baseline)
-_LIB_PATHS = {False: os.path.join(_BUILD, "libga_oracle.so"), True: os.path.join(_BUILD, "libga_oracle_ndebug.so")}
+_LIB_PATHS = {False: os.path.join(_BUILD, "libga_oracle.so"), True: os.path.join(_BUILD, "libga_oracle_ndebug.so"),
+              # with the reference's debug-only invariant checks restated and counted
+              "checked": os.path.join(_BUILD, "libga_oracle_checked.so"),
+              # the band queue's ties inside strongly connected components broken
+              # the other way (tie-order sensitivity)
+              "revties": os.path.join(_BUILD, "libga_oracle_revties.so")}
 
 
 class OrcParams(ctypes.Structure):
@@ -94,6 +99,12 @@ def lib(ndebug=False):
         L.orc_changed_min_score.argtypes = [u64, u64, ctypes.c_int32, u64, u64, ctypes.c_int32]
         L.orc_next_slice.argtypes = [u64, u64, u64, ctypes.c_int32, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64),
                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.orc_minimizers.restype = ctypes.c_int64
+        L.orc_minimizers.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, i64p, ctypes.c_int64]
+        L.orc_index_table.restype = ctypes.c_int64
+        L.orc_index_table.argtypes = [vp, vp, vp, vp]
+        L.orc_extra_checks_run.restype = ctypes.c_int64
+        L.orc_extra_check_failures.restype = ctypes.c_int64
         L.orc_slice_summary_size.restype = ctypes.c_int64
         assert L.orc_slice_summary_size() == ctypes.sizeof(SliceSummary)
         _libs[ndebug] = L
@@ -239,3 +250,29 @@ def order_seeds(graph, seeds):
     out = np.ascontiguousarray(seeds, dtype=np.int64).reshape(-1, 10).copy()
     rc = graph.L.orc_order_seeds(graph.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(out))
     return None if rc == 2 else out
+
+
+def minimizers(seq, k, w, L=None):
+    """iterateMinimizersReal over one string: int64 array [n, 2] of (position, k-mer)."""
+    L = L or lib()
+    cap = 2 * len(seq) + 16
+    out = np.zeros(2 * cap, dtype=np.int64)
+    n = L.orc_minimizers(seq.encode(), len(seq), k, w, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap)
+    assert n <= cap
+    return out[: 2 * n].reshape(-1, 2)
+
+
+def index_table(index):
+    """{k-mer: positions in storage order} of an oracle index."""
+    L = index.L
+    total = L.orc_index_table(index.h, None, None, None)
+    nk = L.orc_index_num_kmers(index.h)
+    kmers = np.zeros(max(nk, 1), dtype=np.uint64)
+    counts = np.zeros(max(nk, 1), dtype=np.int64)
+    pos = np.zeros(max(total, 1), dtype=np.uint64)
+    L.orc_index_table(index.h, kmers.ctypes.data_as(ctypes.c_void_p), counts.ctypes.data_as(ctypes.c_void_p), pos.ctypes.data_as(ctypes.c_void_p))
+    out, at = {}, 0
+    for km, c in zip(kmers[:nk].tolist(), counts[:nk].tolist()):
+        out[km] = pos[at: at + c].tolist()
+        at += c
+    return out
