@@ -11,7 +11,8 @@ later rounds extending several seeds per read ahead and replaying the loop),
 selection and the GAF text, handed to Python as a buffer over the library's
 GAF bytes (no str decoding of the ~350 MB per step inside the timed region; the
 parity check decodes it afterwards). Reads are resident in HBM before the timed
-region (ga_reads_upload).
+region (ga_reads_upload); their reverse complements and match-mask planes are
+computed again by every step.
 
 Prints one JSON line (reads/s, GCUPS, roofline of the extension kernel, and the
 CPU baseline: the C++ CPU restatement of the reference run on a sample of the
@@ -35,10 +36,25 @@ VALU_INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 # band item state amortised over the node's columns)
 OPS_PER_WORD_STEP = 42
 BYTES_PER_WORD_STEP = 6
-# HBM bytes per extension launch (mean) of the default run of this workload,
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy kernel's
-# readings (profiles/extend_vg5m.md)
+# HBM bytes per extension launch (mean) of the default run of this workload: a
+# constant measured offline with rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled
+# by the reading of a 1 GiB copy kernel) on the default command, recorded in
+# profiles/extend_vg5m.md; reported as such, not measured by this run
 MEASURED_TRAFFIC_PER_LAUNCH = 5.08e9
+TRAFFIC_SOURCE = "constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the default run, profiles/extend_vg5m.md"
+
+
+def cpu_share():
+    """Host threads this process may use: the cores it is allowed to run on,
+    within the per-GPU CPU share the host grants (OMP_NUM_THREADS when set)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
 
 
 def parse():
@@ -48,8 +64,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reads", type=int, default=0, help="use only the first N reads (0: all 20,000)")
     ap.add_argument("--warmup-reads", type=int, default=0, help="reads aligned by each warmup step (0: all, as a timed step)")
-    ap.add_argument("--cpu-sample", type=int, default=3000, help="reads in the CPU-baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: all host cores, at most 16)")
+    ap.add_argument("--cpu-sample", type=int, default=4000, help="reads in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: the host cores this process may use)")
     ap.add_argument("--workdir", default=os.environ.get("GA_BENCH_DIR", ""))
     ap.add_argument("--device-seed-loop", action="store_true", help="run each read's whole seed loop on the GPU")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -173,12 +189,15 @@ def main():
                                  "align_call": tot["totalMs"] / 1e3 / steps},
         "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "unusedExtensions", "wordSteps", "seeds", "reruns")},
         "read_status_counts": counts,
-        "roofline": {"kernel": "extendSingleLaneKernel", "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+        # the bit-parallel DP is integer-op bound (42 int32-equivalent ops per
+        # 64-cell word step); its HBM view is kept beside it
+        "roofline": {"kernel": "extendSingleLaneKernel", "bound": "valu-int32", "achieved": valu_tops,
+                     "peak": VALU_INT32_PEAK_TOPS, "unit": "TOP/s", "frac": valu_tops / VALU_INT32_PEAK_TOPS,
                      "traffic": None if args.device_seed_loop or args.reads else MEASURED_TRAFFIC_PER_LAUNCH,
-                     "mean_launch_ms": tot["extendKernelMs"] / launches,
-                     "valu_int32": {"achieved_tops": valu_tops, "peak_tops": VALU_INT32_PEAK_TOPS,
-                                    "frac": valu_tops / VALU_INT32_PEAK_TOPS}},
+                     "traffic_source": TRAFFIC_SOURCE, "mean_launch_ms": tot["extendKernelMs"] / launches,
+                     "ops_per_word_step": OPS_PER_WORD_STEP,
+                     "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+                             "bytes_per_word_step": BYTES_PER_WORD_STEP}},
         "cpu_baseline": cpu,
         "gaf_identical_to_cpu_sample": parity,
         "setup_s": {"generate": gen_s, "graph_load": load_s, "index_build": index_s, "read_upload": upload_s},
@@ -191,19 +210,25 @@ def main():
 def cpu_baseline(prefix, names, seqs, gaf, status, args):
     """The oracle (C++ CPU restatement of the reference path, one read per thread
     from a shared queue as src/Aligner.cpp:777-780) on the first reads of the same
-    workload. The time is the median of three runs of its faster variant (the
-    reference's assertions compiled out, -DNDEBUG); the GPU's GAF lines for those
-    reads are checked against the variant with assertions, as the reference ships."""
+    workload, in both builds the reference can be compiled as: with its
+    assertions (as it ships) and without (-DNDEBUG); each timed as the median of
+    three runs, the faster one reported as the value. The GPU's GAF lines for
+    those reads are checked against the asserting build's. The sample keeps the
+    CPU leg to about 10-30 s of host work."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as o
     n = min(args.cpu_sample, len(names))
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or cpu_share()
     sd = o.PRESET_SEEDING["vg"]
     params = o.preset_params("vg")
-    fg = o.Graph(prefix + ".gfa", ndebug=True)
-    fi = o.Index(fg, sd["k"], sd["w"], sd["discard"])
-    runs = sorted(o.align_reads(fg, fi, params, sd["density"], names[:n], seqs[:n], threads)[2] for _ in range(3))
-    secs = runs[1]
+    timed = {}
+    for label, ndebug in (("asserting", False), ("ndebug", True)):
+        fg = o.Graph(prefix + ".gfa", ndebug=ndebug)
+        fi = o.Index(fg, sd["k"], sd["w"], sd["discard"])
+        runs = sorted(o.align_reads(fg, fi, params, sd["density"], names[:n], seqs[:n], threads)[2] for _ in range(3))
+        timed[label] = (runs[1], runs, fg.L.orc_last_word_steps())
+    fast = min(timed, key=lambda k: timed[k][0])
+    secs, runs, steps = timed[fast]
     og = o.Graph(prefix + ".gfa")
     oi = o.Index(og, sd["k"], sd["w"], sd["discard"])
     text, ostat, _ = o.align_reads(og, oi, params, sd["density"], names[:n], seqs[:n], threads)
@@ -217,8 +242,13 @@ def cpu_baseline(prefix, names, seqs, gaf, status, args):
     parity = {"reads": n, "lines_cpu": len(cpu_lines), "lines_gpu": len(gpu_sample), "identical_lines": same,
               "status_identical": bool((ostat == status[:n]).all())}
     cpu = {"value": n / secs, "unit": "reads/s", "cores": threads, "kind": "port",
-           "sample": f"first {n} reads of the same workload, oracle -O3 -DNDEBUG, {threads} threads, "
-                     f"median of 3 runs {secs:.2f} s ({', '.join(f'{r:.2f}' for r in runs)})"}
+           "gcups": 64 * steps / secs / 1e9,
+           "host_nproc": os.cpu_count(),
+           "variants": {k: {"reads_per_s": n / v[0], "gcups": 64 * v[2] / v[0] / 1e9, "median_s": v[0],
+                            "runs_s": v[1]} for k, v in timed.items()},
+           "sample": f"first {n} of the {len(names)} reads of the same workload (the CPU leg is bounded to ~10-30 s), "
+                     f"oracle -O3 {'-DNDEBUG' if fast == 'ndebug' else 'with assertions'} (the faster build), "
+                     f"{threads} threads (the host cores this process may use), median of 3 runs {secs:.2f} s"}
     return cpu, parity
 
 

@@ -1708,6 +1708,7 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 
 void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, const AlignOptions& opt, const char* const* names, uint32_t first, uint32_t count, ReadOutcome* out, AlignStats& stats)
 {
+	ctx.beginAlignmentCall();
 	if (opt.deviceSeedLoop) alignResidentDeviceLoop(ctx, g, idx, opt, names, first, count, out, stats);
 	else runRounds(ctx, g, &idx, opt, names, first, count, out, stats, nullptr, nullptr);
 }
@@ -1715,6 +1716,7 @@ void alignResident(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex&
 void alignOneWayResident(ExtendBackend& ctx, const GraphHost& g, const AlignOptions& opt, uint32_t first, uint32_t count, const std::vector<SeedHit>* seeds, std::vector<Alignment>* alignments, ReadOutcome* out, AlignStats& stats)
 {
 	for (uint32_t i = 0; i < count; i++) alignments[i].clear();
+	ctx.beginAlignmentCall();
 	runRounds(ctx, g, nullptr, opt, nullptr, first, count, out, stats, seeds, alignments);
 }
 

@@ -227,6 +227,10 @@ public:
 	// For resident reads [first, first + count): writes their reverse complements
 	// (what lookupSeeds does besides the lookup), for seeds that come from the caller.
 	virtual void prepareReads(uint32_t first, uint32_t count) = 0;
+	// Start of an alignment call: per-read device data derived from the resident
+	// reads (reverse complements, match-mask planes) is computed again by the call
+	// instead of being reused from an earlier one.
+	virtual void beginAlignmentCall() {}
 	// Forward bytes of resident read i start at readsBase() + offset[i], its reverse
 	// complement (once looked up) at readsBase() + reverseBase() + offset[i].
 	virtual const char* readsBase() const = 0;

@@ -247,6 +247,13 @@ void Context::writeReverse(uint32_t first, uint32_t count)
 	}
 }
 
+void Context::beginAlignmentCall()
+{
+	std::lock_guard<std::mutex> lock(cacheMutex_);
+	revLo_ = revHi_ = 0;
+	planesReady_ = false;
+}
+
 void Context::prepareReads(uint32_t first, uint32_t count)
 {
 	GA_HIP(hipSetDevice(device_));

@@ -103,6 +103,7 @@ public:
 	const ReadSet& reads() const override { return reads_; }
 	void lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out) override;
 	void prepareReads(uint32_t first, uint32_t count) override;
+	void beginAlignmentCall() override;
 	const char* readsBase() const override { return readBytes_.as<char>(); }
 	void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) override;
 	// The graph's device copy if it is resident on this context, else null

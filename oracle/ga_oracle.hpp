@@ -93,6 +93,15 @@ inline uint64_t& extraChecksRun()
 	} while (0)
 #endif
 
+// Forward column steps of the bit-parallel DP (one getNextSlice call of the
+// slice calculation, traceback recomputation excluded) made by this thread:
+// the work unit of GCUPS (64 cells per step).
+inline uint64_t& forwardWordSteps()
+{
+	static thread_local uint64_t n = 0;
+	return n;
+}
+
 using Word = uint64_t;
 using ScoreType = int32_t;
 static constexpr int WordSize = 64;
@@ -1641,6 +1650,7 @@ struct Extender
 			else { hinP = 0; hinN = 0; }
 			Word Ph, Mh;
 			WordSlice stepped = getNextSlice(Eq, inc.incoming, hinP, hinN, Ph, Mh);
+			if (AllowEarlyLeave) forwardWordSteps()++;
 			if (!previousSlice.exists || stepped.getScoreBeforeStart() < previousSlice.startSlice.scoreEnd)
 			{
 				stepped.VP &= AllOnes ^ 1;
@@ -1802,6 +1812,7 @@ struct Extender
 			slice.HN |= hinN << pos;
 		}
 		result.cellsProcessed = pos;
+		if (AllowEarlyLeave) forwardWordSteps() += nodeLength - 1;
 		slice.endSlice = ws;
 		if (previousSlice.exists) ORC_ASSERT(slice.endSlice.getScoreBeforeStart() <= previousSlice.endSlice.scoreEnd);
 		return result;
@@ -3336,11 +3347,19 @@ struct ReadOutcome
 	int status = 0; // 0 aligned, 1 no seeds, 2 assertion (read dropped), 3 no alignment
 	std::string gaf;
 	size_t seedsExtended = 0;
+	uint64_t wordSteps = 0; // forward column steps of the read's extensions
 };
 
 inline ReadOutcome alignRead(const AlignmentGraph& graph, const MinimizerIndex& index, const Params& params, double minimizerSeedDensity, const std::string& name, const std::string& sequence)
 {
 	ReadOutcome out;
+	const uint64_t steps0 = forwardWordSteps();
+	struct StepCount
+	{
+		ReadOutcome& o;
+		uint64_t s0;
+		~StepCount() { o.wordSteps = forwardWordSteps() - s0; }
+	} stepCount{out, steps0};
 	Aligner aligner(params, graph);
 	std::vector<AlignmentItem> alignments;
 	try

@@ -9,6 +9,12 @@
 
 using namespace orc;
 
+static uint64_t& lastWordSteps()
+{
+	static uint64_t n = 0;
+	return n;
+}
+
 extern "C" {
 
 struct OrcParams
@@ -253,8 +259,14 @@ double orc_align_reads(void* g, void* idx, const OrcParams* p, double seedDensit
 	}
 	buf[pos] = 0;
 	*gafOut = buf;
+	uint64_t steps = 0;
+	for (auto& o : outcomes) steps += o.wordSteps;
+	lastWordSteps() = steps;
 	return std::chrono::duration<double>(end - start).count();
 }
+
+// Forward column steps of the last orc_align_reads call (all reads).
+int64_t orc_last_word_steps() { return (int64_t)lastWordSteps(); }
 
 // GraphAligner::AlignOneWay (src/GraphAligner.h:94-181) with the caller's seeds
 // (10 int64 per seed in SeedHit field order, -1 for an unset

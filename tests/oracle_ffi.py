@@ -99,6 +99,7 @@ def lib(ndebug=False):
         L.orc_changed_min_score.argtypes = [u64, u64, ctypes.c_int32, u64, u64, ctypes.c_int32]
         L.orc_next_slice.argtypes = [u64, u64, u64, ctypes.c_int32, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64),
                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.orc_last_word_steps.restype = ctypes.c_int64
         L.orc_minimizers.restype = ctypes.c_int64
         L.orc_minimizers.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, i64p, ctypes.c_int64]
         L.orc_index_table.restype = ctypes.c_int64
