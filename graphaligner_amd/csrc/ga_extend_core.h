@@ -253,49 +253,52 @@ GA_HD bool eqFromPlanes(const uint64_t* planes, uint64_t x0, uint32_t rows, Eq4&
 // ---------------------------------------------------------------------------
 #if defined(GA_THREAD_PER_EXTENSION)
 
-// Rows 0..63 where column a scores below column b (lt) and above it (gt): the
-// result of differenceMasks (src/WordSlice.h:546-664), computed here as a sweep
+// Rows 0..63 where column a scores below column b (lt) and above it (gt), as a
+// value (outputs through references made the inlined code keep a stack slot):
+// the result of differenceMasks (src/WordSlice.h:546-664), computed here as a sweep
 // over the rows where the difference d(r) = a(r) - b(r) changes. d(-1) is the
 // difference of the scores before row 0; at row r it moves by a's vertical
 // delta minus b's, one of -2..2, so its sign is constant between two such rows.
 // The sweep stops early once the rises (falls) still ahead cannot bring a
 // negative (positive) difference back to zero.
-GA_HD void compareColumns(const Slice& a, const Slice& b, uint64_t& lt, uint64_t& gt)
+struct RowCompare
 {
-	// d rises with a's +1 rows and b's -1 rows, falls with a's -1 and b's +1 rows
-	const uint64_t up2 = a.VP & b.VN, dn2 = a.VN & b.VP;
-	const uint64_t up1 = up2 | (a.VP & ~b.VP) | (b.VN & ~a.VN);
-	const uint64_t dn1 = dn2 | (a.VN & ~b.VN) | (b.VP & ~a.VP);
-	// (up1 and dn1 never share a row: a row with a rise has no fall of its own)
+	uint64_t lt, gt;
+};
+GA_HD RowCompare compareColumns(const Slice& a, const Slice& b)
+{
+	uint64_t lt, gt;
+	// rows where d rises (a's +1 against b's 0/-1, or b's -1 against a's 0), where
+	// it falls, and where it moves by two (+1 against -1)
+	const uint64_t rise = (a.VP & ~b.VP) | (b.VN & ~a.VN);
+	const uint64_t fall = (a.VN & ~b.VN) | (b.VP & ~a.VP);
+	const uint64_t two = (a.VP & b.VN) | (a.VN & b.VP);
 	int32_t d = before(a) - before(b);
-	int32_t upLeft = popc(up1) + popc(up2), dnLeft = popc(dn1) + popc(dn2);
-	uint64_t events = up1 | dn1;
+	uint64_t events = rise | fall; // rows not swept yet where d changes
 	lt = 0;
 	gt = 0;
 	uint32_t start = 0; // first row whose difference is d
-	while (start < 64)
+	while (true)
 	{
-		if (d > dnLeft)
+		// the falls (rises) left move d by at most two each
+		if (d > 0 && d > 2 * popc(fall & events))
 		{
 			gt |= ONES << start;
-			return;
+			return RowCompare{lt, gt};
 		}
-		if (-d > upLeft)
+		if (d < 0 && -d > 2 * popc(rise & events))
 		{
 			lt |= ONES << start;
-			return;
+			return RowCompare{lt, gt};
 		}
 		const uint32_t r = events ? (uint32_t)__builtin_ctzll(events) : 64;
 		const uint64_t span = (r >= 64 ? ONES : (((uint64_t)1 << r) - 1)) & (ONES << start);
 		if (d > 0) gt |= span;
 		else if (d < 0) lt |= span;
-		if (r >= 64) return;
+		if (r >= 64) return RowCompare{lt, gt};
 		const uint64_t bit = (uint64_t)1 << r;
-		const int32_t rise = (int32_t)((up1 & bit) != 0) + (int32_t)((up2 & bit) != 0);
-		const int32_t fall = (int32_t)((dn1 & bit) != 0) + (int32_t)((dn2 & bit) != 0);
-		d += rise - fall;
-		upLeft -= rise;
-		dnLeft -= fall;
+		const int32_t step = (two & bit) ? 2 : 1;
+		d += (rise & bit) ? step : -step;
 		events &= events - 1;
 		start = r;
 	}
@@ -310,8 +313,8 @@ GA_HD void compareColumns(const Slice& a, const Slice& b, uint64_t& lt, uint64_t
 // (also) smallest at both rows, b.
 GA_HD Slice lpMerge(const Slice& a, const Slice& b)
 {
-	uint64_t lt, gt;
-	compareColumns(a, b, lt, gt);
+	const RowCompare rc = compareColumns(a, b);
+	const uint64_t lt = rc.lt, gt = rc.gt;
 	const int32_t ba = before(a), bb = before(b);
 	const uint64_t eq = ~(lt | gt);
 	const uint64_t gtPrev = (gt << 1) | (uint64_t)(bb < ba);
@@ -327,9 +330,7 @@ GA_HD Slice lpMerge(const Slice& a, const Slice& b)
 }
 GA_HD uint64_t lpLessMask(const Slice& a, const Slice& b)
 {
-	uint64_t lt, gt;
-	compareColumns(a, b, lt, gt);
-	return lt;
+	return compareColumns(a, b).lt;
 }
 // minimum over the rows of mask, walking from one selected row to the next
 GA_HD int32_t lpMinOver(const Slice& s, uint64_t mask)
