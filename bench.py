@@ -14,6 +14,13 @@ parity check decodes it afterwards). Reads are resident in HBM before the timed
 region (ga_reads_upload); their reverse complements and match-mask planes are
 computed again by every step.
 
+--workload stream runs the batched form instead: the same graph with a mixed
+stream of 100,000 reads (log-normal length, median 10 kbp, sigma 0.9, clipped to
+1-100 kbp; error 5-15 % drawn per read; minimizers of 21 bases in windows of 31,
+5th-95th percentile 5-193 seed hits per read), each batch of 16,384 reads run by
+one captured HIP graph of a persistent kernel whose workers pull whole reads
+from a device counter (worker count autotuned once per context).
+
 Prints one JSON line (reads/s, GCUPS, roofline of the extension kernel, and the
 CPU baseline: the C++ CPU restatement of the reference run on a sample of the
 same reads with the host's cores).
@@ -44,6 +51,15 @@ MEASURED_TRAFFIC_PER_LAUNCH = 5.08e9
 TRAFFIC_SOURCE = "constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the default run, profiles/extend_vg5m.md"
 
 
+WORKLOADS = {
+    "vg5m": dict(preset="vg5m", k=15, w=20,
+                 text="vg5m: 5 Mbp variation graph with SV bubbles, 20,000 log-normal reads (median 12 kbp) at 12% error, -x vg"),
+    "stream": dict(preset="vg5m_stream", k=21, w=31,
+                   text="vg5m stream: 5 Mbp variation graph with SV bubbles, 100,000 log-normal reads (median 10 kbp, 1-100 kbp) "
+                        "at 5-15% error drawn per read, -x vg with minimizers k=21 w=31, batched persistent form"),
+}
+
+
 def cpu_share():
     """Host threads this process may use: the cores it is allowed to run on,
     within the per-GPU CPU share the host grants (OMP_NUM_THREADS when set)."""
@@ -62,12 +78,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--reads", type=int, default=0, help="use only the first N reads (0: all 20,000)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="vg5m")
+    ap.add_argument("--reads", type=int, default=0, help="use only the first N reads (0: all of the workload's)")
     ap.add_argument("--warmup-reads", type=int, default=0, help="reads aligned by each warmup step (0: all, as a timed step)")
     ap.add_argument("--cpu-sample", type=int, default=4000, help="reads in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: the host cores this process may use)")
     ap.add_argument("--workdir", default=os.environ.get("GA_BENCH_DIR", ""))
     ap.add_argument("--device-seed-loop", action="store_true", help="run each read's whole seed loop on the GPU")
+    ap.add_argument("--batched", action="store_true", help="the batched persistent form (default for --workload stream)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (traffic-counter calibration) and exit")
     return ap.parse_args()
@@ -100,18 +118,21 @@ def main():
 
     workdir = args.workdir or tempfile.mkdtemp(prefix="ga_bench_")
     os.makedirs(workdir, exist_ok=True)
-    prefix = os.path.join(workdir, "vg5m")
+    wl = WORKLOADS[args.workload]
+    batched = args.batched or args.workload == "stream"
+    prefix = os.path.join(workdir, wl["preset"])
     t = time.time()
     if not (os.path.exists(prefix + ".gfa") and os.path.exists(prefix + ".fa")):
-        synth.generate("vg5m", prefix)
+        synth.generate(wl["preset"], prefix)
     names, seqs = synth.read_fasta(prefix + ".fa")
     if args.reads:
         names, seqs = names[: args.reads], seqs[: args.reads]
     gen_s = time.time() - t
-    print(f"[bench] workload vg5m: {len(seqs)} reads, {sum(map(len, seqs)) / 1e6:.1f} Mbp ({gen_s:.1f} s)", file=sys.stderr, flush=True)
+    print(f"[bench] workload {args.workload}: {len(seqs)} reads, {sum(map(len, seqs)) / 1e6:.1f} Mbp ({gen_s:.1f} s)", file=sys.stderr, flush=True)
 
     params = ga.params_preset("vg")
-    params.deviceSeedLoop = 1 if args.device_seed_loop else 0
+    params.minimizerLength, params.minimizerWindowSize = wl["k"], wl["w"]
+    params.deviceSeedLoop = 2 if batched else 1 if args.device_seed_loop else 0
     t = time.time()
     graph = ga.Graph(prefix + ".gfa")
     load_s = time.time() - t
@@ -135,7 +156,8 @@ def main():
     ctx.synchronize()
     t0 = time.perf_counter()
     tot = dict(extendKernelMs=0.0, lookupMs=0.0, hostMs=0.0, extendCallMs=0.0, lookupCallMs=0.0, totalMs=0.0,
-               seedingMs=0.0, pickMs=0.0, replayMs=0.0, gafMs=0.0, unusedExtensions=0, launches=0, extensions=0, wordSteps=0, rounds=0, seeds=0, reruns=0)
+               seedingMs=0.0, pickMs=0.0, replayMs=0.0, gafMs=0.0, tuneMs=0.0, unusedExtensions=0, launches=0, extensions=0,
+               wordSteps=0, rounds=0, seeds=0, reruns=0)
     gaf = status = None
     for k in range(args.steps):
         ts = time.perf_counter()
@@ -161,7 +183,7 @@ def main():
     cpu = None
     parity = None
     if args.cpu_sample > 0:
-        cpu, parity = cpu_baseline(prefix, names, seqs, gaf.tobytes().decode(), status, args)
+        cpu, parity = cpu_baseline(prefix, names, seqs, gaf.tobytes().decode(), status, args, wl)
 
     line = {
         "metric": "reads/s + GCUPS on 1 MI355X (plain GPU path)",
@@ -177,23 +199,25 @@ def main():
         "dtype": "u64",
         "data": "synthetic",
         "gcups": gcups,
-        "config": {"workload": "vg5m: 5 Mbp variation graph with SV bubbles, 20,000 log-normal reads (median 12 kbp) at 12% error, -x vg",
+        "config": {"workload": wl["text"],
                    "reads": len(names), "read_mbp": sum(map(len, seqs)) / 1e6, "split_nodes": graph.num_nodes,
-                   "preset": "vg", "parallelism": "1 GPU",
-                   "seed_loop": "device" if args.device_seed_loop else "rounds"},
+                   "preset": "vg", "minimizer_k": wl["k"], "minimizer_w": wl["w"], "parallelism": "1 GPU",
+                   "seed_loop": "batched persistent" if batched else "device" if args.device_seed_loop else "rounds",
+                   "persistent_workers": st["persistentWorkers"] if batched else None},
         "breakdown_s_per_step": {"extend_kernel": kernel_s / steps, "extend_calls": tot["extendCallMs"] / 1e3 / steps,
                                  "seed_lookup_kernel": tot["lookupMs"] / 1e3 / steps,
                                  "seed_lookup_calls": tot["lookupCallMs"] / 1e3 / steps, "host": tot["hostMs"] / 1e3 / steps,
                                  "host_seeding": tot["seedingMs"] / 1e3 / steps, "host_pick": tot["pickMs"] / 1e3 / steps,
                                  "host_replay": tot["replayMs"] / 1e3 / steps, "host_gaf": tot["gafMs"] / 1e3 / steps,
-                                 "align_call": tot["totalMs"] / 1e3 / steps},
+                                 "align_call": tot["totalMs"] / 1e3 / steps, "autotune": tot["tuneMs"] / 1e3 / steps},
         "counters_per_step": {k: tot[k] // steps for k in ("launches", "extensions", "unusedExtensions", "wordSteps", "seeds", "reruns")},
         "read_status_counts": counts,
         # the bit-parallel DP is integer-op bound (42 int32-equivalent ops per
         # 64-cell word step); its HBM view is kept beside it
-        "roofline": {"kernel": "extendSingleLaneKernel", "bound": "valu-int32", "achieved": valu_tops,
+        "roofline": {"kernel": "readLoopPersistentKernel" if batched else "readLoopSingleLaneKernel" if args.device_seed_loop else "extendSingleLaneKernel",
+                     "bound": "valu-int32", "achieved": valu_tops,
                      "peak": VALU_INT32_PEAK_TOPS, "unit": "TOP/s", "frac": valu_tops / VALU_INT32_PEAK_TOPS,
-                     "traffic": None if args.device_seed_loop or args.reads else MEASURED_TRAFFIC_PER_LAUNCH,
+                     "traffic": None if batched or args.device_seed_loop or args.reads else MEASURED_TRAFFIC_PER_LAUNCH,
                      "traffic_source": TRAFFIC_SOURCE, "mean_launch_ms": tot["extendKernelMs"] / launches,
                      "ops_per_word_step": OPS_PER_WORD_STEP,
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
@@ -207,7 +231,7 @@ def main():
     return 0
 
 
-def cpu_baseline(prefix, names, seqs, gaf, status, args):
+def cpu_baseline(prefix, names, seqs, gaf, status, args, wl):
     """The oracle (C++ CPU restatement of the reference path, one read per thread
     from a shared queue as src/Aligner.cpp:777-780) on the first reads of the same
     workload, in both builds the reference can be compiled as: with its
@@ -219,7 +243,7 @@ def cpu_baseline(prefix, names, seqs, gaf, status, args):
     import oracle_ffi as o
     n = min(args.cpu_sample, len(names))
     threads = args.cpu_threads or cpu_share()
-    sd = o.PRESET_SEEDING["vg"]
+    sd = dict(o.PRESET_SEEDING["vg"], k=wl["k"], w=wl["w"])
     params = o.preset_params("vg")
     timed = {}
     for label, ndebug in (("asserting", False), ("ndebug", True)):

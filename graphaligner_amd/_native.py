@@ -51,7 +51,8 @@ class AlignStats(ctypes.Structure):
                 ("extensions", ctypes.c_int64), ("wordSteps", ctypes.c_int64), ("seeds", ctypes.c_int64),
                 ("reruns", ctypes.c_int64), ("lookupCallMs", ctypes.c_double), ("extendCallMs", ctypes.c_double),
                 ("unusedExtensions", ctypes.c_int64), ("seedingMs", ctypes.c_double), ("pickMs", ctypes.c_double),
-                ("replayMs", ctypes.c_double), ("gafMs", ctypes.c_double), ("gafBytes", ctypes.c_int64)]
+                ("replayMs", ctypes.c_double), ("gafMs", ctypes.c_double), ("gafBytes", ctypes.c_int64),
+                ("persistentWorkers", ctypes.c_int64), ("tuneMs", ctypes.c_double)]
 
 
 class SeedHit(ctypes.Structure):

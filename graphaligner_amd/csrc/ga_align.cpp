@@ -987,7 +987,10 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 		stats.hostMs += msSince(th);
 		ReadLoopOutput rlo;
 		auto te = std::chrono::steady_clock::now();
-		ctx.runReadLoops(g, rp, devSeeds, jobs, rlo);
+		if (opt.batchedDeviceLoop) ctx.runReadLoopsBatched(g, rp, devSeeds, jobs, rlo);
+		else ctx.runReadLoops(g, rp, devSeeds, jobs, rlo);
+		stats.workers = std::max<uint64_t>(stats.workers, rlo.workers);
+		stats.tuneMs += rlo.tuneMs;
 		stats.extendCallMs += msSince(te);
 		stats.rounds += rlo.launches;
 		stats.launches += rlo.launches;

@@ -81,6 +81,10 @@ struct AlignOptions
 	// of every unfinished read in one launch (both directions as separate jobs);
 	// true: every read's whole seed loop runs on the device in one launch
 	bool deviceSeedLoop = false;
+	// with deviceSeedLoop: the batched persistent form (a fixed set of workers
+	// pulling reads from a device counter, one captured graph per launch shape,
+	// worker count autotuned); same results
+	bool batchedDeviceLoop = false;
 	// host-driven rounds: a read's round r > 0 also extends up to 4^r - 1 of the
 	// seeds its loop would try next (at most maxSpeculativeSeeds seeds in all,
 	// more than 4^r when few reads remain, see speculationJobBudget);
@@ -132,6 +136,9 @@ struct AlignStats
 	// order; picking each round's seeds and building jobs; replaying the loops
 	// (trace decode, fix-ups, alignments); selection and GAF text
 	double seedingMs = 0, pickMs = 0, replayMs = 0, gafMs = 0;
+	// batched persistent form: workers per launch and the time their tuning took
+	uint64_t workers = 0;
+	double tuneMs = 0;
 };
 
 // The selection's score threshold: --min-alignment-score is a double but

@@ -171,6 +171,8 @@ struct ReadLoopOutput
 	uint32_t launches = 0;
 	uint64_t wordSteps = 0, extensions = 0;
 	uint64_t reruns = 0;                     // jobs rerun with larger workspaces
+	uint32_t workers = 0;                    // batched persistent form: workers per launch
+	double tuneMs = 0;                       // and the time its worker-count tuning took
 };
 
 // Workspace capacities of the device seed loop; grown like the arena on overflow.
@@ -239,6 +241,9 @@ public:
 	// be in place, i.e. lookupSeeds ran on them); jobs whose workspaces overflow
 	// are rerun with larger ones, kArenaAttempts attempts in all.
 	virtual void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) = 0;
+	// The same read loops in the batched persistent form (same outputs); a
+	// backend without one runs runReadLoops.
+	virtual void runReadLoopsBatched(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) { runReadLoops(g, p, seeds, jobs, out); }
 };
 
 } // namespace ga

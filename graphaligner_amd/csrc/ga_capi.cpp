@@ -356,6 +356,10 @@ AlignOptions alignOptionsOf(const ga_params_t& p)
 	opt.minSeedClusterSize = p.seedClusterMinSize < 0 ? 0 : (uint64_t)p.seedClusterMinSize;
 	opt.cigarMatchMismatchMerge = p.cigarMatchMismatchMerge != 0;
 	opt.deviceSeedLoop = p.deviceSeedLoop != 0;
+	opt.batchedDeviceLoop = p.deviceSeedLoop == 2;
+	// the batched form bounds each batch's output pools (about 100 bytes per read
+	// base) and replays its graph once per batch
+	if (opt.batchedDeviceLoop) opt.readsPerBatch = 16384;
 	opt.maxSpeculativeSeeds = p.maxSpeculativeSeeds < 1 ? 1 : (uint32_t)p.maxSpeculativeSeeds;
 	if (const char* v = std::getenv("GA_PIPELINE_GROUPS"))
 	{
@@ -422,6 +426,8 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, size_t n, const Al
 		stats->replayMs = st.replayMs;
 		stats->gafMs = st.gafMs;
 		stats->gafBytes = (int64_t)at.back();
+		stats->persistentWorkers = (int64_t)st.workers;
+		stats->tuneMs = st.tuneMs;
 	}
 	return GA_OK;
 }

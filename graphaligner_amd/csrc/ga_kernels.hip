@@ -158,6 +158,52 @@ __global__ void __launch_bounds__(1) readLoopSingleLaneKernel(DevGraph g, ReadLo
 	results[idx] = out;
 }
 
+// Batched persistent form: a fixed set of single-lane workers, each pulling whole
+// reads (in the order of `jobs`, longest first) from a device counter until the
+// batch is drained; a worker reuses its own workspace and raw-trace region for
+// every read it takes, so memory scales with the workers, not the reads. The
+// number of jobs is read from device memory so that one captured graph serves
+// every batch. Every worker exits once the counter passes the job count.
+template <bool LdsQueue>
+__global__ void __launch_bounds__(1) readLoopPersistentKernel(DevGraph g, ReadLoopParams p, const ReadJob* __restrict__ jobs, const uint32_t* __restrict__ njobs, uint32_t* __restrict__ next, const char* __restrict__ reads, const DevSeed* __restrict__ seeds, uint64_t* __restrict__ rawTraces, uint64_t rawStride, uint64_t* __restrict__ pools, DevAlignment* __restrict__ alns, ReadResult* __restrict__ results, unsigned char* __restrict__ arenas, uint64_t arenaStride, ArenaCaps caps, const uint64_t* __restrict__ planes)
+{
+	__shared__ uint64_t ldsWords[LdsQueue ? kQueueWords : kColumnWords];
+	const uint32_t n = uni32(*njobs);
+	unsigned char* arena = arenas + (uint64_t)blockIdx.x * arenaStride;
+	uint64_t* raw = rawTraces + (uint64_t)blockIdx.x * rawStride;
+	while (true)
+	{
+		const uint32_t idx = uni32(atomicAdd(next, 1u));
+		if (idx >= n) break;
+		const ReadJob j = jobs[idx];
+		ArenaCaps c = caps;
+		c.maxSlices = requestSlices(j.len);
+		c.maxItems = requestItems(j.len, caps);
+		ReadResult out;
+		runReadLoop(g, p, j, reads, seeds + j.seedStart, raw, pools + j.poolOffset, alns + j.alnOffset, arena, c, out, ldsWords, planes, true, ldsWords, LdsQueue);
+		results[idx] = out;
+	}
+}
+
+// Zeroes the batch counter (a kernel, so that it can sit in a captured graph).
+__global__ void __launch_bounds__(64) resetCounterKernel(uint32_t* c)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+}
+
+hipError_t launchReadLoopsPersistent(hipStream_t stream, uint32_t workers, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const uint32_t* njobs, uint32_t* counter, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t rawStride, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, uint64_t arenaStride, const ArenaCaps& caps, const uint64_t* planes)
+{
+	if (workers == 0) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(64), 0, stream, counter);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return e;
+	if (queueFitsLds(caps))
+		hipLaunchKernelGGL(readLoopPersistentKernel<true>, dim3(workers), dim3(1), 0, stream, g, p, jobs, njobs, counter, reads, seeds, rawTraces, rawStride, pools, alns, results, arenas, arenaStride, caps, planes);
+	else
+		hipLaunchKernelGGL(readLoopPersistentKernel<false>, dim3(workers), dim3(1), 0, stream, g, p, jobs, njobs, counter, reads, seeds, rawTraces, rawStride, pools, alns, results, arenas, arenaStride, caps, planes);
+	return hipGetLastError();
+}
+
 hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, const ArenaCaps& caps, const uint64_t* planes)
 {
 	if (lanesPerWave < 1 || lanesPerWave > 64) return hipErrorInvalidValue;

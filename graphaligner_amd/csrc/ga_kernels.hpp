@@ -15,6 +15,12 @@ hipError_t launchExtend(hipStream_t stream, uint32_t nreq, uint32_t lanesPerWave
 // read i's bytes, seeds and workspaces; results[i] receives its outcome.
 hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPerWave, uint32_t wavesPerSimd, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, const ArenaCaps& caps, const uint64_t* planes);
 
+// Batched persistent form of launchReadLoops: zeroes *counter, then `workers`
+// single-lane workers pull jobs [0, *njobs) in order from it. Worker w uses
+// arenas + w * arenaStride and rawTraces + w * rawStride (two trace regions of
+// each job's traceCap items), which must fit the batch's longest read.
+hipError_t launchReadLoopsPersistent(hipStream_t stream, uint32_t workers, const DevGraph& g, const ReadLoopParams& p, const ReadJob* jobs, const uint32_t* njobs, uint32_t* counter, const char* reads, const DevSeed* seeds, uint64_t* rawTraces, uint64_t rawStride, uint64_t* pools, DevAlignment* alns, ReadResult* results, unsigned char* arenas, uint64_t arenaStride, const ArenaCaps& caps, const uint64_t* planes);
+
 // Match-mask planes (planeBlock) of bytes[0, n): planeBlockCount(n) blocks of
 // kPlaneWords words. The extension kernels take planes of their sequence buffer.
 hipError_t launchEqPlanes(hipStream_t stream, const char* bytes, uint64_t n, uint64_t* planes);

@@ -264,7 +264,9 @@ struct ReadLoop
 };
 
 // Runs one read's seed loop. rawTrace holds two regions of job.traceCap items.
-GA_HD void runReadLoop(const DevGraph& g, const ReadLoopParams& p, const ReadJob& job, const char* reads, const DevSeed* seeds, uint64_t* rawTrace, uint64_t* pool, DevAlignment* alns, unsigned char* arena, const ArenaCaps& caps, ReadResult& out, uint64_t* colStore = nullptr, const uint64_t* planes = nullptr, bool colStoreLds = false)
+// queueStore/queueLds: the extensions' band queue in LDS (as the single-lane
+// extension kernel keeps it) when the capacities fit it.
+GA_HD void runReadLoop(const DevGraph& g, const ReadLoopParams& p, const ReadJob& job, const char* reads, const DevSeed* seeds, uint64_t* rawTrace, uint64_t* pool, DevAlignment* alns, unsigned char* arena, const ArenaCaps& caps, ReadResult& out, uint64_t* colStore = nullptr, const uint64_t* planes = nullptr, bool colStoreLds = false, uint64_t* queueStore = nullptr, bool queueLds = false)
 {
 	ReadLoop L;
 	L.g = g;
@@ -345,7 +347,7 @@ GA_HD void runReadLoop(const DevGraph& g, const ReadLoopParams& p, const ReadJob
 			}
 			r.traceCap = job.traceCap;
 			ExtResult res;
-			runRequest(g, p.ext, r, reads, dir == 0 ? bwdRaw : fwdRaw, arena, caps, nullptr, res, colStore, planes, colStoreLds);
+			runRequest(g, p.ext, r, reads, dir == 0 ? bwdRaw : fwdRaw, arena, caps, nullptr, res, colStore, planes, colStoreLds, queueStore, queueLds);
 			out.wordSteps += res.wordSteps;
 			out.extensions++;
 			if (res.status == EXT_OVERFLOW)

@@ -2,6 +2,7 @@
 #include "ga_runtime.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include "ga_kernels.hpp"
@@ -95,6 +96,8 @@ Context::~Context()
 	lanes_.clear();
 	graphs_.clear();
 	indexes_.clear();
+	if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+	if (graph_) (void)hipGraphDestroy(graph_);
 	if (ev0_) (void)hipEventDestroy(ev0_);
 	if (ev1_) (void)hipEventDestroy(ev1_);
 	if (stream_) (void)hipStreamDestroy(stream_);
@@ -738,6 +741,13 @@ void Context::runReadLoopLaunch(const gad::DevGraph& dg, const gad::ReadLoopPara
 	GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
 	out.kernelMs += ms;
 	out.launches++;
+	collectReadLoops(rj, res, which, out, rlAlns_.as<uint64_t>(), rlPool_.as<uint64_t>());
+}
+
+
+void Context::collectReadLoops(const std::vector<gad::ReadJob>& rj, const std::vector<gad::ReadResult>& res, const std::vector<uint32_t>& which, ReadLoopOutput& out, const uint64_t* alnSrc, const uint64_t* poolSrc)
+{
+	const uint32_t n = which.size();
 	// gather every finished read's alignment records and used pool prefix
 	std::vector<uint64_t> srcOff, dstOff;
 	std::vector<uint32_t> cnt;
@@ -788,8 +798,199 @@ void Context::runReadLoopLaunch(const gad::DevGraph& dg, const gad::ReadLoopPara
 		GA_HIP(hipMemcpyAsync(dst, traceOut_.ptr, total * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
 		GA_HIP(hipStreamSynchronize(stream_));
 	};
-	gather(rlAlns_.as<uint64_t>(), srcOff, dstOff, cnt, alnUsed * 4, out.alns.data() + alnBase);
-	gather(rlPool_.as<uint64_t>(), psrc, pdst, pcnt, poolUsed, out.pool.data() + poolBase);
+	gather(alnSrc, srcOff, dstOff, cnt, alnUsed * 4, out.alns.data() + alnBase);
+	gather(poolSrc, psrc, pdst, pcnt, poolUsed, out.pool.data() + poolBase);
+}
+
+
+float Context::runPersistentBatch(const gad::DevGraph& dg, const gad::ReadLoopParams& p, const std::vector<gad::ReadJob>& rj, uint32_t workers, const ReadLoopCaps& caps, uint32_t maxLen, std::vector<gad::ReadResult>& res)
+{
+	const uint32_t n = rj.size();
+	uint64_t poolTotal = 0, alnTotal = 0;
+	for (const auto& j : rj)
+	{
+		poolTotal = std::max<uint64_t>(poolTotal, j.poolOffset + j.poolCap);
+		alnTotal = std::max<uint64_t>(alnTotal, j.alnOffset + j.alnCap);
+	}
+	const uint64_t arenaStride = roundUp256(gad::requestArenaBytes(maxLen, caps.arena));
+	const uint64_t rawStride = 2ull * traceCapFor(maxLen);
+	bJobs_.reserve(std::max<size_t>(1, n) * sizeof(gad::ReadJob));
+	bCounter_.reserve(2 * sizeof(uint32_t));
+	bTraces_.reserve((uint64_t)workers * rawStride * 8);
+	bArena_.reserve((uint64_t)workers * arenaStride);
+	bPool_.reserve(std::max<uint64_t>(1, poolTotal) * 8);
+	bAlns_.reserve(std::max<uint64_t>(1, alnTotal) * sizeof(gad::DevAlignment));
+	bResults_.reserve(std::max<size_t>(1, n) * sizeof(gad::ReadResult));
+	const uint64_t* planes = residentPlanes(readBytes_.as<char>());
+	uint32_t* counter = bCounter_.as<uint32_t>();
+	// the launch shape: buffers, strides, workers and parameters; a graph captured
+	// for one shape is replayed for every batch of that shape (the job count is
+	// read from device memory)
+	PersistentShape shape{};
+	const void* ptrs[9] = {bJobs_.ptr, bCounter_.ptr, bTraces_.ptr, bArena_.ptr, bPool_.ptr, bAlns_.ptr, bResults_.ptr, readBytes_.ptr, rlSeeds_.ptr};
+	std::memcpy(shape.ptrs, ptrs, sizeof(ptrs));
+	shape.workers = workers;
+	shape.rawStride = rawStride;
+	shape.arenaStride = arenaStride;
+	shape.caps = caps.arena;
+	shape.p = p;
+	shape.dg = dg;
+	(void)planes;
+	if (!graphExec_ || std::memcmp(&shape, &graphShape_, sizeof(shape)) != 0)
+	{
+		if (graphExec_) GA_HIP(hipGraphExecDestroy(graphExec_));
+		if (graph_) GA_HIP(hipGraphDestroy(graph_));
+		graphExec_ = nullptr;
+		graph_ = nullptr;
+		GA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+		hipError_t le = gad::launchReadLoopsPersistent(stream_, workers, dg, p, bJobs_.as<gad::ReadJob>(), counter + 1, counter, readBytes_.as<char>(), rlSeeds_.as<gad::DevSeed>(), bTraces_.as<uint64_t>(), rawStride, bPool_.as<uint64_t>(), bAlns_.as<gad::DevAlignment>(), bResults_.as<gad::ReadResult>(), bArena_.as<unsigned char>(), arenaStride, caps.arena, planes);
+		hipGraph_t gr = nullptr;
+		hipError_t ce = hipStreamEndCapture(stream_, &gr);
+		GA_HIP(le);
+		GA_HIP(ce);
+		graph_ = gr;
+		GA_HIP(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+		graphShape_ = shape;
+		graphCaptures++;
+	}
+	const uint32_t count = n;
+	if (n > 0) GA_HIP(hipMemcpyAsync(bJobs_.ptr, rj.data(), n * sizeof(gad::ReadJob), hipMemcpyHostToDevice, stream_));
+	GA_HIP(hipMemcpyAsync(counter + 1, &count, sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+	GA_HIP(hipEventRecord(ev0_, stream_));
+	GA_HIP(hipGraphLaunch(graphExec_, stream_));
+	GA_HIP(hipEventRecord(ev1_, stream_));
+	graphLaunches++;
+	res.assign(n, gad::ReadResult{});
+	if (n > 0) GA_HIP(hipMemcpyAsync(res.data(), bResults_.ptr, n * sizeof(gad::ReadResult), hipMemcpyDeviceToHost, stream_));
+	GA_HIP(hipStreamSynchronize(stream_));
+	float ms = 0;
+	GA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+	return ms;
+}
+
+void Context::runReadLoopsBatched(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out)
+{
+	static_assert(sizeof(gad::DevAlignment) == 32, "alignment records are gathered as four 64-bit words");
+	GA_HIP(hipSetDevice(device_));
+	const gad::DevGraph& dg = deviceGraph(g);
+	const size_t n = jobs.size();
+	out = ReadLoopOutput{};
+	out.results.assign(n, gad::ReadResult{});
+	out.alnOffset.assign(n, 0);
+	out.poolOffset.assign(n, 0);
+	if (n == 0) return;
+	for (const auto& j : jobs)
+	{
+		if (j.read >= reads_.size()) throw std::runtime_error("read loop job names a read that is not resident");
+		if (j.seedStart + j.seedCount > seeds.size()) throw std::runtime_error("read loop job seeds out of range");
+	}
+	rlSeeds_.reserve(std::max<size_t>(1, seeds.size()) * sizeof(gad::DevSeed));
+	if (!seeds.empty()) GA_HIP(hipMemcpyAsync(rlSeeds_.ptr, seeds.data(), seeds.size() * sizeof(gad::DevSeed), hipMemcpyHostToDevice, stream_));
+	// longest reads first: the workers take the longest seed loops earliest
+	std::vector<uint32_t> which(n);
+	for (uint32_t i = 0; i < n; i++) which[i] = i;
+	std::stable_sort(which.begin(), which.end(), [&](uint32_t a, uint32_t b) { return reads_.length[jobs[a].read] > reads_.length[jobs[b].read]; });
+	// first attempt: the band queue in LDS and per-extension workspaces as the
+	// single-lane extension kernel has them; overflows go to runReadLoops
+	ReadLoopCaps caps = initialReadLoopCaps();
+	caps.arena = initialArenaCaps();
+	auto makeJobs = [&](const std::vector<uint32_t>& ids, std::vector<gad::ReadJob>& rj) {
+		rj.resize(ids.size());
+		uint64_t poolTotal = 0, alnTotal = 0;
+		uint32_t maxLen = 0;
+		for (size_t k = 0; k < ids.size(); k++)
+		{
+			const ReadLoopJob& j = jobs[ids[k]];
+			const uint32_t len = reads_.length[j.read];
+			gad::ReadJob& r = rj[k];
+			r.fwdOffset = reads_.offset[j.read];
+			r.revOffset = revBase_ + reads_.offset[j.read];
+			r.seedStart = j.seedStart;
+			r.arenaOffset = 0;
+			r.traceOffset = 0;
+			r.poolOffset = poolTotal;
+			r.alnOffset = alnTotal;
+			r.extendSeeds = j.extendSeeds;
+			r.len = len;
+			r.seedCount = j.seedCount;
+			r.traceCap = traceCapFor(len);
+			r.poolCap = poolCapFor(len, caps);
+			r.alnCap = caps.alnCap;
+			r.pad = 0;
+			poolTotal += r.poolCap;
+			alnTotal += r.alnCap;
+			maxLen = std::max(maxLen, len);
+		}
+		return maxLen;
+	};
+	// the worker count: candidates of 2..8 resident workers per SIMD, timed on a
+	// sample of every k-th read of the first batch (their outputs discarded)
+	if (batchedWorkers == 0)
+	{
+		auto t0 = std::chrono::steady_clock::now();
+		uint32_t forced = 0;
+		if (const char* v = std::getenv("GA_BATCH_WAVES_PER_SIMD"))
+		{
+			const int x = std::atoi(v);
+			if (x >= 1 && x <= 16) forced = (uint32_t)x;
+		}
+		if (forced) batchedWorkers = (uint32_t)cus_ * 4 * forced;
+		else
+		{
+			std::vector<uint32_t> sample;
+			const size_t stride = std::max<size_t>(1, n / 4096);
+			for (size_t k = 0; k < n; k += stride) sample.push_back(which[k]);
+			std::vector<gad::ReadJob> rj;
+			const uint32_t maxLen = makeJobs(sample, rj);
+			std::vector<gad::ReadResult> res;
+			float best = 0;
+			for (uint32_t w : {2u, 3u, 4u, 5u, 6u, 8u})
+			{
+				const uint32_t workers = (uint32_t)cus_ * 4 * w;
+				const float ms = runPersistentBatch(dg, p, rj, workers, caps, maxLen, res);
+				if (batchedWorkers == 0 || ms < best)
+				{
+					best = ms;
+					batchedWorkers = workers;
+				}
+			}
+		}
+		batchedTuneMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		out.tuneMs = batchedTuneMs;
+	}
+	out.workers = batchedWorkers;
+	std::vector<gad::ReadJob> rj;
+	const uint32_t maxLen = makeJobs(which, rj);
+	std::vector<gad::ReadResult> res;
+	out.kernelMs += runPersistentBatch(dg, p, rj, batchedWorkers, caps, maxLen, res);
+	out.launches++;
+	collectReadLoops(rj, res, which, out, bAlns_.as<uint64_t>(), bPool_.as<uint64_t>());
+	// overflowed reads: the launch-per-batch form with growing workspaces
+	std::vector<uint32_t> again;
+	for (uint32_t id : which)
+		if (out.results[id].status == gad::READLOOP_OVERFLOW) again.push_back(id);
+	if (!again.empty())
+	{
+		std::vector<ReadLoopJob> sub;
+		for (uint32_t id : again) sub.push_back(jobs[id]);
+		ReadLoopOutput ro;
+		runReadLoops(g, p, seeds, sub, ro);
+		out.reruns += again.size() + ro.reruns;
+		out.kernelMs += ro.kernelMs;
+		out.launches += ro.launches;
+		out.wordSteps += ro.wordSteps;
+		out.extensions += ro.extensions;
+		const uint64_t alnBase = out.alns.size(), poolBase = out.pool.size();
+		out.alns.insert(out.alns.end(), ro.alns.begin(), ro.alns.end());
+		out.pool.insert(out.pool.end(), ro.pool.begin(), ro.pool.end());
+		for (size_t k = 0; k < again.size(); k++)
+		{
+			const uint32_t id = again[k];
+			out.results[id] = ro.results[k];
+			out.alnOffset[id] = alnBase + ro.alnOffset[k];
+			out.poolOffset[id] = poolBase + ro.poolOffset[k];
+		}
+	}
 }
 
 } // namespace ga

@@ -106,6 +106,12 @@ public:
 	void beginAlignmentCall() override;
 	const char* readsBase() const override { return readBytes_.as<char>(); }
 	void runReadLoops(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) override;
+	// The batched persistent form: the same read loops (same outputs) run by a
+	// fixed set of workers pulling reads from a device counter, one captured
+	// graph replayed per batch; the worker count is chosen by timing candidates
+	// on a sample of the first batch. Reads whose workspaces overflow are rerun
+	// by runReadLoops.
+	void runReadLoopsBatched(const GraphHost& g, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ReadLoopJob>& jobs, ReadLoopOutput& out) override;
 	// The graph's device copy if it is resident on this context, else null
 	// (never uploads).
 	const gad::DevGraph* residentGraph(const GraphHost& g);
@@ -136,6 +142,12 @@ public:
 private:
 	void runOnce(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase, const uint64_t* devPlanes);
 	void runReadLoopLaunch(const gad::DevGraph& dg, const gad::ReadLoopParams& p, const std::vector<ReadLoopJob>& jobs, const std::vector<uint32_t>& which, const ReadLoopCaps& caps, ReadLoopOutput& out);
+	// copies the finished reads' alignment records and used pool prefixes of one
+	// launch (jobs rj, results res of jobs `which`) into out
+	void collectReadLoops(const std::vector<gad::ReadJob>& rj, const std::vector<gad::ReadResult>& res, const std::vector<uint32_t>& which, ReadLoopOutput& out, const uint64_t* alnSrc, const uint64_t* poolSrc);
+	// one persistent batch with `workers` workers (graph captured on first use
+	// of a launch shape); returns the kernel ms
+	float runPersistentBatch(const gad::DevGraph& dg, const gad::ReadLoopParams& p, const std::vector<gad::ReadJob>& rj, uint32_t workers, const ReadLoopCaps& caps, uint32_t maxLen, std::vector<gad::ReadResult>& res);
 	void runLaunch(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase, const uint64_t* devPlanes);
 	// match-mask planes of the resident reads (forward and reverse bytes);
 	// computePlanes runs with cacheMutex_ held
@@ -169,6 +181,25 @@ private:
 	uint32_t lanesOverride_ = 0;
 	// this device's copy of the no-allocation scratch (looked up at construction)
 	unsigned char* scratch_ = nullptr;
+	// batched persistent form: buffers, the captured graph of the current launch
+	// shape, and the autotuned worker count (0: not tuned yet)
+	DeviceBuffer bJobs_, bCounter_, bTraces_, bArena_, bPool_, bAlns_, bResults_;
+	struct PersistentShape
+	{
+		const void* ptrs[9];
+		uint32_t workers;
+		uint64_t rawStride, arenaStride;
+		gad::ArenaCaps caps;
+		gad::ReadLoopParams p;
+		gad::DevGraph dg;
+	};
+	PersistentShape graphShape_{};
+	hipGraph_t graph_ = nullptr;
+	hipGraphExec_t graphExec_ = nullptr;
+public:
+	uint32_t batchedWorkers = 0;
+	double batchedTuneMs = 0;
+	uint64_t graphLaunches = 0, graphCaptures = 0;
 };
 
 } // namespace ga

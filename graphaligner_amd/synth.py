@@ -37,6 +37,13 @@ PRESETS = {
                   indelLenMax=10, svEvery=20_000, svLenMin=50, svLenMax=5000, nodeMax=32, seed=2),
              dict(numReads=20000, lengthModel=1, lenA=12000, lenB=0, sigma=0.6, lenMin=1000, lenMax=60000,
                   errorMin=0.12, errorMax=0.12, seed=1002)),
+    # the batched mixed stream on the 5 Mbp graph: 100,000 reads, log-normal
+    # lengths (median 10 kbp, sigma 0.9, clipped to 1-100 kbp), per-read error
+    # drawn from 5-15 %
+    "vg5m_stream": (dict(genomeLength=5_000_000, snpMin=100, snpMax=200, indelMin=500, indelMax=1500, indelLenMin=1,
+                         indelLenMax=10, svEvery=20_000, svLenMin=50, svLenMax=5000, nodeMax=32, seed=2),
+                    dict(numReads=100000, lengthModel=1, lenA=10000, lenB=0, sigma=0.9, lenMin=1000, lenMax=100000,
+                         errorMin=0.05, errorMax=0.15, seed=1004)),
     "mini": (dict(genomeLength=20_000, snpMin=100, snpMax=200, indelMin=500, indelMax=1500, indelLenMin=1,
                   indelLenMax=10, svEvery=0, svLenMin=50, svLenMax=5000, nodeMax=32, seed=11),
              dict(numReads=40, lengthModel=0, lenA=500, lenB=3000, sigma=0.0, lenMin=500, lenMax=3000,

@@ -57,7 +57,10 @@ typedef struct
 	/* Execution only (no reference counterpart, results are identical): 0 runs
 	 * the seed loops in rounds, every round extending the next seed of each
 	 * unfinished read in one GPU launch; 1 runs each read's whole seed loop on
-	 * the GPU, all reads in one launch. */
+	 * the GPU, all reads in one launch; 2 runs them in the batched persistent
+	 * form: a fixed set of workers (count autotuned on the first batch) pulls
+	 * reads, longest first, from a device work counter, launched as one captured
+	 * HIP graph per batch. */
 	int32_t deviceSeedLoop;
 	/* Execution only, rounds mode: from a read's second round on, round r also
 	 * extends up to 4^r - 1 of the seeds its loop would try next (more when few
@@ -178,6 +181,8 @@ typedef struct
 	double replayMs;       /* replaying the seed loops on the results (trace fix-ups, alignments), */
 	double gafMs;          /* selection and GAF text */
 	int64_t gafBytes;      /* length of the returned GAF text (without its NUL) */
+	int64_t persistentWorkers; /* deviceSeedLoop 2: workers per launch */
+	double tuneMs;         /* deviceSeedLoop 2: time spent choosing the worker count (first call) */
 } ga_align_stats_t;
 
 /* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed

@@ -132,6 +132,109 @@ public:
 		out.launches++;
 	}
 
+	// the batched persistent form on host threads: three workers pull reads
+	// (longest first) from a shared counter, each reusing one workspace, one
+	// raw-trace region and one band-queue store for every read it takes
+	void runReadLoopsBatched(const ga::GraphHost& G, const gad::ReadLoopParams& p, const std::vector<gad::DevSeed>& seeds, const std::vector<ga::ReadLoopJob>& jobs, ga::ReadLoopOutput& out) override
+	{
+		const gad::DevGraph g = view(G);
+		const size_t n = jobs.size();
+		out = ga::ReadLoopOutput{};
+		out.results.assign(n, gad::ReadResult{});
+		out.alnOffset.assign(n, 0);
+		out.poolOffset.assign(n, 0);
+		if (n == 0) return;
+		std::vector<uint32_t> order(n);
+		for (uint32_t i = 0; i < n; i++) order[i] = i;
+		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return rs_.length[jobs[a].read] > rs_.length[jobs[b].read]; });
+		ga::ReadLoopCaps caps = ga::initialReadLoopCaps();
+		caps.arena = ga::initialArenaCaps();
+		uint32_t maxLen = 0;
+		for (const auto& j : jobs) maxLen = std::max(maxLen, rs_.length[j.read]);
+		gad::ArenaCaps big = caps.arena;
+		big.maxSlices = gad::requestSlices(maxLen);
+		big.maxItems = gad::requestItems(maxLen, caps.arena);
+		const uint64_t arenaBytes = gad::arenaBytes(big) + 256;
+		std::vector<std::vector<gad::DevAlignment>> alns(n);
+		std::vector<std::vector<uint64_t>> pools(n);
+		std::atomic<size_t> next{0};
+		const int workers = 3;
+		std::vector<std::thread> pool;
+		for (int w = 0; w < workers; w++)
+			pool.emplace_back([&]() {
+				std::vector<unsigned char> arena(arenaBytes);
+				unsigned char* base = (unsigned char*)(((uintptr_t)arena.data() + 255) & ~(uintptr_t)255);
+				std::vector<uint64_t> raw(2ull * ga::traceCapFor(maxLen));
+				std::vector<uint64_t> queue(gad::kQueueWords);
+				while (true)
+				{
+					const size_t k = next.fetch_add(1);
+					if (k >= n) break;
+					const uint32_t i = order[k];
+					const ga::ReadLoopJob& j = jobs[i];
+					const uint32_t len = rs_.length[j.read];
+					gad::ReadJob r;
+					r.fwdOffset = rs_.offset[j.read];
+					r.revOffset = revBase_ + rs_.offset[j.read];
+					r.seedStart = 0;
+					r.arenaOffset = r.traceOffset = r.poolOffset = r.alnOffset = 0;
+					r.extendSeeds = j.extendSeeds;
+					r.len = len;
+					r.seedCount = j.seedCount;
+					r.traceCap = ga::traceCapFor(len);
+					r.poolCap = ga::poolCapFor(len, caps);
+					r.alnCap = caps.alnCap;
+					r.pad = 0;
+					gad::ArenaCaps c = caps.arena;
+					c.maxSlices = gad::requestSlices(len);
+					c.maxItems = gad::requestItems(len, caps.arena);
+					pools[i].assign(r.poolCap, 0);
+					alns[i].assign(r.alnCap, gad::DevAlignment{});
+					gad::ReadResult res;
+					gad::runReadLoop(g, p, r, reads_.data(), seeds.data() + j.seedStart, raw.data(), pools[i].data(), alns[i].data(), base, c, res, queue.data(), planes_.data(), false, queue.data(), true);
+					out.results[i] = res;
+				}
+			});
+		for (auto& t : pool) t.join();
+		std::vector<ga::ReadLoopJob> again;
+		std::vector<uint32_t> againIds;
+		for (size_t i = 0; i < n; i++)
+		{
+			const gad::ReadResult& res = out.results[i];
+			out.wordSteps += res.wordSteps;
+			out.extensions += res.extensions;
+			if (res.status == gad::READLOOP_OVERFLOW)
+			{
+				again.push_back(jobs[i]);
+				againIds.push_back((uint32_t)i);
+				continue;
+			}
+			alns[i].resize(res.status == gad::READLOOP_DONE ? res.nAlignments : 0);
+			pools[i].resize(res.status == gad::READLOOP_DONE ? res.poolUsed : 0);
+			out.alnOffset[i] = out.alns.size();
+			out.alns.insert(out.alns.end(), alns[i].begin(), alns[i].end());
+			out.poolOffset[i] = out.pool.size();
+			out.pool.insert(out.pool.end(), pools[i].begin(), pools[i].end());
+		}
+		out.launches++;
+		out.workers = workers;
+		if (!again.empty())
+		{
+			ga::ReadLoopOutput ro;
+			runReadLoops(G, p, seeds, again, ro);
+			out.reruns += again.size();
+			for (size_t k = 0; k < again.size(); k++)
+			{
+				const uint32_t i = againIds[k];
+				out.results[i] = ro.results[k];
+				out.alnOffset[i] = out.alns.size() + ro.alnOffset[k];
+				out.poolOffset[i] = out.pool.size() + ro.poolOffset[k];
+			}
+			out.alns.insert(out.alns.end(), ro.alns.begin(), ro.alns.end());
+			out.pool.insert(out.pool.end(), ro.pool.begin(), ro.pool.end());
+		}
+	}
+
 	uint32_t extendLanes() const override { return 2; }
 
 	void extend(const ga::GraphHost& G, const gad::ExtParams& p, const std::vector<ga::ExtJob>& jobs, bool, ga::ExtBatch& out, const char* devSeqBase, uint32_t lane) override
@@ -373,6 +476,7 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 	o.minSeedClusterSize = po->minClusterSize;
 	o.hostThreads = 4;
 	o.deviceSeedLoop = po->deviceLoop != 0;
+	o.batchedDeviceLoop = po->deviceLoop == 2;
 	o.maxSpeculativeSeeds = po->maxSpeculativeSeeds < 1 ? 1 : (uint32_t)po->maxSpeculativeSeeds;
 	o.minAlignmentScore = po->minAlignmentScore;
 	std::vector<const char*> nm(names, names + nreads), sq(seqs, seqs + nreads);

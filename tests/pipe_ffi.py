@@ -114,7 +114,7 @@ def options(oracle_params, density, device_loop=False, speculative=64):
 # execution modes of the seed loop: host-driven rounds one seed at a time, rounds
 # with speculative extension of later seeds, the whole loop on the device
 MODES = {"rounds": dict(device_loop=False, speculative=1), "rounds_spec": dict(device_loop=False, speculative=64),
-         "device_loop": dict(device_loop=True, speculative=1)}
+         "device_loop": dict(device_loop=True, speculative=1), "batched": dict(device_loop=2, speculative=1)}
 
 
 def align_reads(index, oracle_params, density, names, seqs, device_loop=False, speculative=64):
