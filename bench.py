@@ -56,7 +56,7 @@ WORKLOADS = {
                  text="vg5m: 5 Mbp variation graph with SV bubbles, 20,000 log-normal reads (median 12 kbp) at 12% error, -x vg"),
     "stream": dict(preset="vg5m_stream", k=21, w=31,
                    text="vg5m stream: 5 Mbp variation graph with SV bubbles, 100,000 log-normal reads (median 10 kbp, 1-100 kbp) "
-                        "at 5-15% error drawn per read, -x vg with minimizers k=21 w=31, batched persistent form"),
+                        "at 5-15% error drawn per read, -x vg with minimizers k=21 w=31"),
 }
 
 
@@ -84,8 +84,9 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=4000, help="reads in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU-baseline threads (0: the host cores this process may use)")
     ap.add_argument("--workdir", default=os.environ.get("GA_BENCH_DIR", ""))
-    ap.add_argument("--device-seed-loop", action="store_true", help="run each read's whole seed loop on the GPU")
-    ap.add_argument("--batched", action="store_true", help="the batched persistent form (default for --workload stream)")
+    ap.add_argument("--seed-loop", choices=["auto", "rounds", "device", "batched"], default="auto",
+                    help="rounds: host-driven rounds of batched extensions; device: each read's whole seed loop on the GPU; "
+                         "batched: the persistent form (auto: batched for the stream, rounds otherwise)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (traffic-counter calibration) and exit")
     return ap.parse_args()
@@ -119,7 +120,7 @@ def main():
     workdir = args.workdir or tempfile.mkdtemp(prefix="ga_bench_")
     os.makedirs(workdir, exist_ok=True)
     wl = WORKLOADS[args.workload]
-    batched = args.batched or args.workload == "stream"
+    loop = args.seed_loop if args.seed_loop != "auto" else "batched" if args.workload == "stream" else "rounds"
     prefix = os.path.join(workdir, wl["preset"])
     t = time.time()
     if not (os.path.exists(prefix + ".gfa") and os.path.exists(prefix + ".fa")):
@@ -132,7 +133,7 @@ def main():
 
     params = ga.params_preset("vg")
     params.minimizerLength, params.minimizerWindowSize = wl["k"], wl["w"]
-    params.deviceSeedLoop = 2 if batched else 1 if args.device_seed_loop else 0
+    params.deviceSeedLoop = {"rounds": 0, "device": 1, "batched": 2}[loop]
     t = time.time()
     graph = ga.Graph(prefix + ".gfa")
     load_s = time.time() - t
@@ -202,8 +203,7 @@ def main():
         "config": {"workload": wl["text"],
                    "reads": len(names), "read_mbp": sum(map(len, seqs)) / 1e6, "split_nodes": graph.num_nodes,
                    "preset": "vg", "minimizer_k": wl["k"], "minimizer_w": wl["w"], "parallelism": "1 GPU",
-                   "seed_loop": "batched persistent" if batched else "device" if args.device_seed_loop else "rounds",
-                   "persistent_workers": st["persistentWorkers"] if batched else None},
+                   "seed_loop": loop, "persistent_workers": st["persistentWorkers"] if loop == "batched" else None},
         "breakdown_s_per_step": {"extend_kernel": kernel_s / steps, "extend_calls": tot["extendCallMs"] / 1e3 / steps,
                                  "seed_lookup_kernel": tot["lookupMs"] / 1e3 / steps,
                                  "seed_lookup_calls": tot["lookupCallMs"] / 1e3 / steps, "host": tot["hostMs"] / 1e3 / steps,
@@ -214,10 +214,10 @@ def main():
         "read_status_counts": counts,
         # the bit-parallel DP is integer-op bound (42 int32-equivalent ops per
         # 64-cell word step); its HBM view is kept beside it
-        "roofline": {"kernel": "readLoopPersistentKernel" if batched else "readLoopSingleLaneKernel" if args.device_seed_loop else "extendSingleLaneKernel",
+        "roofline": {"kernel": {"rounds": "extendSingleLaneKernel", "device": "readLoopSingleLaneKernel", "batched": "readLoopPersistentKernel"}[loop],
                      "bound": "valu-int32", "achieved": valu_tops,
                      "peak": VALU_INT32_PEAK_TOPS, "unit": "TOP/s", "frac": valu_tops / VALU_INT32_PEAK_TOPS,
-                     "traffic": None if batched or args.device_seed_loop or args.reads else MEASURED_TRAFFIC_PER_LAUNCH,
+                     "traffic": None if loop != "rounds" or args.reads or args.workload != "vg5m" else MEASURED_TRAFFIC_PER_LAUNCH,
                      "traffic_source": TRAFFIC_SOURCE, "mean_launch_ms": tot["extendKernelMs"] / launches,
                      "ops_per_word_step": OPS_PER_WORD_STEP,
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,

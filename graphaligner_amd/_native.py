@@ -21,7 +21,10 @@ EXPORTED = ["ga_params_preset", "ga_graph_load_gfa", "ga_graph_free", "ga_graph_
             "ga_align_reads", "ga_reads_upload", "ga_align_resident", "ga_context_synchronize", "ga_free",
             "ga_order_seeds", "ga_align_one_way", "ga_result_reads", "ga_result_status", "ga_result_seeds_extended",
             "ga_result_count", "ga_result_item", "ga_result_trace", "ga_select_alignments", "ga_add_mapping_qualities",
-            "ga_gaf_line", "ga_result_free", "ga_extend_batch_noalloc"]
+            "ga_gaf_line", "ga_result_free", "ga_extend_batch_noalloc", "ga_mlp_num_params", "ga_mlp_create",
+            "ga_mlp_free", "ga_mlp_get_params", "ga_mlp_set_params", "ga_mlp_forward", "ga_mlp_gradients", "ga_mlp_train",
+            "ga_cluster_records", "ga_records_count", "ga_records_features", "ga_records_labels", "ga_records_reads",
+            "ga_records_free", "ga_context_set_proposal"]
 
 READ_STATUS_NAMES = {0: "aligned", 1: "no seeds", 2: "assertion (read dropped)", 3: "no alignment", 4: "workspace overflow"}
 
@@ -52,7 +55,14 @@ class AlignStats(ctypes.Structure):
                 ("reruns", ctypes.c_int64), ("lookupCallMs", ctypes.c_double), ("extendCallMs", ctypes.c_double),
                 ("unusedExtensions", ctypes.c_int64), ("seedingMs", ctypes.c_double), ("pickMs", ctypes.c_double),
                 ("replayMs", ctypes.c_double), ("gafMs", ctypes.c_double), ("gafBytes", ctypes.c_int64),
-                ("persistentWorkers", ctypes.c_int64), ("tuneMs", ctypes.c_double)]
+                ("persistentWorkers", ctypes.c_int64), ("tuneMs", ctypes.c_double),
+                ("proposalClusters", ctypes.c_int64), ("prunedClusters", ctypes.c_int64), ("prunedSeeds", ctypes.c_int64),
+                ("proposalMs", ctypes.c_double)]
+
+
+class MlpTrainParams(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("seed", ctypes.c_uint64)]
 
 
 class SeedHit(ctypes.Structure):
@@ -127,5 +137,22 @@ def lib():
         L.ga_result_free.argtypes = [vp]
         L.ga_extend_batch_noalloc.argtypes = [vp, vp, ctypes.POINTER(Params), i64, ctypes.POINTER(ctypes.c_char_p), vp, vp, vp,
                                               i64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
+        L.ga_mlp_num_params.restype = i64
+        L.ga_mlp_num_params.argtypes = []
+        L.ga_mlp_create.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.ga_mlp_free.argtypes = [vp]
+        L.ga_mlp_get_params.argtypes = [vp, vp]
+        L.ga_mlp_set_params.argtypes = [vp, vp]
+        L.ga_mlp_forward.argtypes = [vp, i64, vp, vp]
+        L.ga_mlp_gradients.argtypes = [vp, i64, vp, vp, ctypes.POINTER(ctypes.c_double), vp]
+        L.ga_mlp_train.argtypes = [vp, i64, vp, vp, ctypes.POINTER(MlpTrainParams), vp, ctypes.POINTER(ctypes.c_double)]
+        L.ga_cluster_records.argtypes = [vp, vp, vp, ctypes.POINTER(Params), i64, i64, ctypes.POINTER(vp)]
+        L.ga_records_count.restype = i64
+        L.ga_records_count.argtypes = [vp]
+        for f in ("ga_records_features", "ga_records_labels", "ga_records_reads"):
+            getattr(L, f).restype = vp
+            getattr(L, f).argtypes = [vp]
+        L.ga_records_free.argtypes = [vp]
+        L.ga_context_set_proposal.argtypes = [vp, vp, ctypes.c_double, i32]
         _lib = L
     return _lib

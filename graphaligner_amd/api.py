@@ -335,3 +335,97 @@ def align_one_way(ctx, graph, params, seqs, seeds):
     _check(N.lib().ga_align_one_way(ctx.h, graph.h, ctypes.byref(params), n, arr, lens.ctypes.data_as(ctypes.c_void_p), sarr,
                                      counts.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)), "ga_align_one_way")
     return AlignmentResults(h)
+
+
+MLP_FEATURES, MLP_OUTPUTS = 48, 2
+
+
+def mlp_num_params():
+    return N.lib().ga_mlp_num_params()
+
+
+def _f32(a, cols):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(-1, cols)
+
+
+class Mlp:
+    """The learned proposal's network (48 -> 128 -> 128 -> 2, BF16 MFMA kernels on
+    the context's GPU). Parameters are a flat float32 array of mlp_num_params()."""
+
+    def __init__(self, ctx, seed=1):
+        h = ctypes.c_void_p()
+        _check(N.lib().ga_mlp_create(ctx.h, seed, ctypes.byref(h)), "ga_mlp_create")
+        self.h = h
+        self.ctx = ctx
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().ga_mlp_free(self.h)
+            self.h = None
+
+    def params(self):
+        out = np.zeros(mlp_num_params(), dtype=np.float32)
+        _check(N.lib().ga_mlp_get_params(self.h, out.ctypes.data_as(ctypes.c_void_p)), "ga_mlp_get_params")
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == mlp_num_params()
+        _check(N.lib().ga_mlp_set_params(self.h, w.ctypes.data_as(ctypes.c_void_p)), "ga_mlp_set_params")
+
+    def forward(self, features):
+        x = _f32(features, MLP_FEATURES)
+        out = np.zeros((len(x), MLP_OUTPUTS), dtype=np.float32)
+        _check(N.lib().ga_mlp_forward(self.h, len(x), x.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p)),
+               "ga_mlp_forward")
+        return out
+
+    def gradients(self, features, labels):
+        """(mean binary cross-entropy, flat gradient)"""
+        x = _f32(features, MLP_FEATURES)
+        y = _f32(labels, MLP_OUTPUTS)
+        assert len(x) == len(y)
+        g = np.zeros(mlp_num_params(), dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(N.lib().ga_mlp_gradients(self.h, len(x), x.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.byref(loss), g.ctypes.data_as(ctypes.c_void_p)), "ga_mlp_gradients")
+        return loss.value, g
+
+    def train(self, features, labels, epochs=4, batch=4096, lr=2e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=1):
+        """Adam on the GPU; returns (per-epoch mean loss, device ms)."""
+        x = _f32(features, MLP_FEATURES)
+        y = _f32(labels, MLP_OUTPUTS)
+        assert len(x) == len(y)
+        p = N.MlpTrainParams(epochs, batch, lr, beta1, beta2, eps, seed)
+        losses = np.zeros(epochs, dtype=np.float32)
+        ms = ctypes.c_double()
+        _check(N.lib().ga_mlp_train(self.h, len(x), x.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.byref(p), losses.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ms)), "ga_mlp_train")
+        return losses, ms.value
+
+
+def cluster_records(ctx, graph, index, params, first, count):
+    """Outcome records of the plain method on resident reads [first, first + count):
+    (features [n, 48], labels [n, 2], read index [n])."""
+    h = ctypes.c_void_p()
+    L = N.lib()
+    _check(L.ga_cluster_records(ctx.h, graph.h, index.h, ctypes.byref(params), first, count, ctypes.byref(h)), "ga_cluster_records")
+    try:
+        n = L.ga_records_count(h)
+        if n == 0:
+            return np.zeros((0, MLP_FEATURES), np.float32), np.zeros((0, MLP_OUTPUTS), np.float32), np.zeros(0, np.int64)
+        f = np.ctypeslib.as_array(ctypes.cast(L.ga_records_features(h), ctypes.POINTER(ctypes.c_float)), shape=(n, MLP_FEATURES)).copy()
+        y = np.ctypeslib.as_array(ctypes.cast(L.ga_records_labels(h), ctypes.POINTER(ctypes.c_float)), shape=(n, MLP_OUTPUTS)).copy()
+        r = np.ctypeslib.as_array(ctypes.cast(L.ga_records_reads(h), ctypes.POINTER(ctypes.c_int64)), shape=(n,)).copy()
+        return f, y, r
+    finally:
+        L.ga_records_free(h)
+
+
+def set_proposal(ctx, mlp, threshold=0.5, keep_top=1):
+    """Prune seed clusters the network scores below threshold in later align calls
+    on ctx (mlp None: the plain method)."""
+    _check(N.lib().ga_context_set_proposal(ctx.h, mlp.h if mlp is not None else None, threshold, keep_top),
+           "ga_context_set_proposal")
+    ctx.proposal = mlp  # keeps the network alive while it is set

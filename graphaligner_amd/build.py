@@ -15,7 +15,8 @@ OUT = os.path.join(PKG, "_build")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-variable", "-Wno-sign-compare", "-Wno-unused-function"]
-PRODUCT_SOURCES = ["ga_kernels.hip", "ga_runtime.cpp", "ga_graph.cpp", "ga_align.cpp", "ga_index.cpp", "ga_capi.cpp"]
+PRODUCT_SOURCES = ["ga_kernels.hip", "ga_mlp.hip", "ga_runtime.cpp", "ga_graph.cpp", "ga_align.cpp", "ga_proposal.cpp", "ga_index.cpp",
+                   "ga_capi.cpp"]
 
 
 def _run(cmd, cwd=None):
@@ -56,7 +57,7 @@ def build_test_infrastructure():
     _run(common + ["-o", os.path.join(tb, "libga_emu.so")] + srcs)
     _run(common + ["-DGA_THREAD_PER_EXTENSION", "-DGA_THREAD_HOST_TEST", "-o", os.path.join(tb, "libga_thread.so")] + srcs)
     # the product's host pipeline over a backend running the device code on host threads
-    pipe = [os.path.join(ROOT, "tests", "emu", "ga_pipeline_emu.cpp")] + [os.path.join(CSRC, s) for s in ("ga_align.cpp", "ga_index.cpp", "ga_graph.cpp")]
+    pipe = [os.path.join(ROOT, "tests", "emu", "ga_pipeline_emu.cpp")] + [os.path.join(CSRC, s) for s in ("ga_align.cpp", "ga_proposal.cpp", "ga_index.cpp", "ga_graph.cpp")]
     _run(common + ["-pthread", "-o", os.path.join(tb, "libga_pipe.so")] + pipe)
     # the same with the traceback recomputing every column it enters (all of the
     # reference's recalcNodeWordslice checks)

@@ -1,5 +1,6 @@
 // Whole-read alignment on one GPU (see ga_align.hpp).
 #include "ga_align.hpp"
+#include "ga_proposal.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -565,6 +566,10 @@ struct ReadState
 	// rounds so far
 	std::vector<Spec> specs;
 	uint32_t roundsRun = 0;
+	// outcome records: cluster of every seed and the clusters' features
+	std::vector<uint32_t> clusterOf;
+	std::vector<float> features;
+	uint32_t nclusters = 0;
 };
 
 struct Job
@@ -941,6 +946,34 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 				o.status = READ_ASSERT;
 			}
 		});
+		std::vector<std::vector<uint32_t>> clusterOf(opt.records ? bn : 0);
+		std::vector<std::vector<float>> features(opt.records ? bn : 0);
+		if (opt.proposal)
+		{
+			std::vector<std::vector<SeedHit>*> sp;
+			std::vector<const char*> sq;
+			std::vector<uint32_t> ln;
+			for (uint32_t i = 0; i < bn; i++)
+				if (ready[i])
+				{
+					sp.push_back(&seeds[i]);
+					sq.push_back(rs.seq(base + i));
+					ln.push_back(rs.length[base + i]);
+				}
+			ProposalStats ps;
+			applyProposal(*opt.proposal, g, idx.maxCount, sp.size(), sp.data(), sq.data(), ln.data(), opt.hostThreads, ps);
+			stats.proposalClusters += ps.clusters;
+			stats.prunedClusters += ps.prunedClusters;
+			stats.prunedSeeds += ps.prunedSeeds;
+			stats.proposalMs += ps.ms;
+		}
+		if (opt.records)
+			parallelFor(bn, opt.hostThreads, [&](size_t i) {
+				if (!ready[i]) return;
+				const uint32_t nc = seedClusters(g, seeds[i], clusterOf[i]);
+				features[i].resize((size_t)nc * kClusterFeatures);
+				clusterFeatures(g, idx.maxCount, rs.seq(base + i), rs.length[base + i], seeds[i], clusterOf[i], nc, features[i].data());
+			});
 		std::vector<ReadLoopJob> jobs;
 		std::vector<uint32_t> jobRead;
 		uint64_t totalSeeds = 0;
@@ -1029,11 +1062,13 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 					Alignment item = buildAlignment(g, opt, seq, L, s, PackedReader{raw}, da.nb, PackedReader{raw + da.nb}, da.nf);
 					if (item.alignmentFailed()) continue;
 					item.seedGoodness = s.seedGoodness;
+					item.seedIndex = da.seedIndex;
 					al.push_back(std::move(item));
 					if (opt.sloppyOptimizations) std::sort(al.begin(), al.end(), [](const Alignment& l, const Alignment& r) { return l.alignmentStart < r.alignmentStart; });
 				}
 				std::vector<Alignment> sel;
 				if (!al.empty()) sel = selectAlignments(al, opt.minAlignmentScore, opt.multimapScoreFraction);
+				if (opt.records && !clusterOf[i].empty()) opt.records->add(batchStart + i, features[i], (uint32_t)(features[i].size() / kClusterFeatures), clusterOf[i], al, sel);
 				if (sel.empty())
 				{
 					o.status = READ_NO_ALIGNMENT;
@@ -1278,6 +1313,7 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 			try
 			{
 				if (!st.alignments.empty()) sel = selectAlignments(st.alignments, opt.minAlignmentScore, opt.multimapScoreFraction);
+				if (opt.records && st.nclusters) opt.records->add(batchStart + i, st.features, st.nclusters, st.clusterOf, st.alignments, sel);
 				if (sel.empty())
 				{
 					o.status = READ_NO_ALIGNMENT;
@@ -1410,6 +1446,39 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 			}
 			phaseCv.notify_all();
 		};
+		// the learned proposal's pruning of a group's seeds, then the features of
+		// the clusters the loops will see (outcome records)
+		std::mutex proposalMutex;
+		auto seedProposal = [&](const std::vector<uint32_t>& mine) {
+			if (opt.proposal)
+			{
+				std::vector<std::vector<SeedHit>*> sp;
+				std::vector<const char*> sq;
+				std::vector<uint32_t> ln;
+				for (uint32_t r : mine)
+					if (!states[r].done)
+					{
+						sp.push_back(&states[r].seeds);
+						sq.push_back(bseqs[r]);
+						ln.push_back(blens[r]);
+					}
+				ProposalStats ps;
+				applyProposal(*opt.proposal, g, idxp ? idxp->maxCount : 0, sp.size(), sp.data(), sq.data(), ln.data(), threads, ps);
+				std::lock_guard<std::mutex> lock(proposalMutex);
+				stats.proposalClusters += ps.clusters;
+				stats.prunedClusters += ps.prunedClusters;
+				stats.prunedSeeds += ps.prunedSeeds;
+				stats.proposalMs += ps.ms;
+			}
+			if (opt.records)
+				parallelFor(mine.size(), threads, [&](size_t k) {
+					ReadState& st = states[mine[k]];
+					if (st.done || st.seeds.empty()) return;
+					st.nclusters = seedClusters(g, st.seeds, st.clusterOf);
+					st.features.resize((size_t)st.nclusters * kClusterFeatures);
+					clusterFeatures(g, idxp ? idxp->maxCount : 0, bseqs[st.read], blens[st.read], st.seeds, st.clusterOf, st.nclusters, st.features.data());
+				});
+		};
 		auto runGroup = [&](uint32_t lane) {
 			AlignStats& gs = groupStats[lane];
 			Finisher finisher{finishRead, threads, gs.gafMs};
@@ -1431,6 +1500,16 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 					if (lane == 0) signalSeeded();
 					throw;
 				}
+				if (opt.proposal || opt.records)
+					try
+					{
+						seedProposal(mine);
+					}
+					catch (...)
+					{
+						if (lane == 0) signalSeeded();
+						throw;
+					}
 				gs.seedingMs += msSince(ts);
 				for (uint32_t r : mine) gs.seeds += states[r].seeds.size();
 				if (lane == 0) signalSeeded();
@@ -1581,6 +1660,7 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 							if (!item.alignmentFailed())
 							{
 								item.seedGoodness = s.seedGoodness;
+								item.seedIndex = (uint32_t)sp.seed;
 								st.alignments.push_back(std::move(item));
 								if (opt.sloppyOptimizations) updateEndToEnd(st, L);
 							}

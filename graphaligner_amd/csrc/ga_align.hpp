@@ -60,9 +60,13 @@ struct Alignment
 	uint64_t alignmentScore = ~0ull;
 	double alignmentXScore = -1;
 	int mappingQuality = 255;
+	uint32_t seedIndex = ~0u;  // the seed (in the read's loop order) it was extended from
 	bool alignmentFailed() const { return alignmentEnd == alignmentStart; }
 	uint64_t alignmentLength() const { return alignmentEnd - alignmentStart; }
 };
+
+struct SeedProposal;
+struct ClusterRecords;
 
 struct AlignOptions
 {
@@ -104,6 +108,11 @@ struct AlignOptions
 	// first launches are staggered. Either way the groups' launches run one at a
 	// time.
 	bool groupByLength = true;
+	// learned proposal: seed clusters it scores low are removed before the
+	// read's loop (none: the plain method)
+	const SeedProposal* proposal = nullptr;
+	// outcome records of every seed cluster (features and labels) are added here
+	ClusterRecords* records = nullptr;
 };
 
 enum ReadStatus
@@ -139,6 +148,10 @@ struct AlignStats
 	// batched persistent form: workers per launch and the time their tuning took
 	uint64_t workers = 0;
 	double tuneMs = 0;
+	// learned proposal: clusters scored, clusters and seeds removed, time taken
+	// (features, scoring, pruning)
+	uint64_t proposalClusters = 0, prunedClusters = 0, prunedSeeds = 0;
+	double proposalMs = 0;
 };
 
 // The selection's score threshold: --min-alignment-score is a double but

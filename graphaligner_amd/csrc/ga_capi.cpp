@@ -343,9 +343,10 @@ ga_status_t ga_seeds(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_
 
 namespace {
 
-AlignOptions alignOptionsOf(const ga_params_t& p)
+AlignOptions alignOptionsOf(const ga_params_t& p, const ga_context* ctx = nullptr)
 {
 	AlignOptions opt;
+	if (ctx && ctx->proposal.scorer) opt.proposal = &ctx->proposal;
 	opt.ext = extParamsOf(p);
 	opt.xscoreErrorCost = errorCostOf(p);
 	opt.seedExtendDensity = p.seedExtendDensity;
@@ -428,6 +429,10 @@ ga_status_t packOutcomes(const std::vector<ReadOutcome>& out, size_t n, const Al
 		stats->gafBytes = (int64_t)at.back();
 		stats->persistentWorkers = (int64_t)st.workers;
 		stats->tuneMs = st.tuneMs;
+		stats->proposalClusters = (int64_t)st.proposalClusters;
+		stats->prunedClusters = (int64_t)st.prunedClusters;
+		stats->prunedSeeds = (int64_t)st.prunedSeeds;
+		stats->proposalMs = st.proposalMs;
 	}
 	return GA_OK;
 }
@@ -470,7 +475,7 @@ ga_status_t ga_align_reads(ga_context_t* ctx, const ga_graph_t* graph, const ga_
 		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
 		std::vector<ReadOutcome>& out = ctx->outcomes;
 		AlignStats st;
-		alignReads(*ctx->ctx, graph->host, index->idx, alignOptionsOf(*params), nm, s, l, out, st);
+		alignReads(*ctx->ctx, graph->host, index->idx, alignOptionsOf(*params, ctx), nm, s, l, out, st);
 		return packOutcomes(out, (size_t)n, st, gaf, status, stats);
 	});
 }
@@ -518,7 +523,7 @@ ga_status_t ga_align_resident(ga_context_t* ctx, const ga_graph_t* graph, const 
 		std::vector<ReadOutcome>& out = ctx->outcomes;
 		if (out.size() < (size_t)count) out.resize(count);
 		AlignStats st;
-		alignResident(*ctx->ctx, graph->host, index->idx, alignOptionsOf(*params), names, (uint32_t)first, (uint32_t)count, out.data(), st);
+		alignResident(*ctx->ctx, graph->host, index->idx, alignOptionsOf(*params, ctx), names, (uint32_t)first, (uint32_t)count, out.data(), st);
 		if (log) std::fprintf(stderr, "[capi] aligned at %.1f ms\n", since());
 		ga_status_t rc = packOutcomes(out, (size_t)count, st, gaf, status, stats);
 		if (log) std::fprintf(stderr, "[capi] packed at %.1f ms\n", since());
@@ -744,6 +749,157 @@ ga_status_t ga_extend_batch_noalloc(ga_context_t* ctx, const ga_graph_t* graph, 
 		}
 		return rc;
 	});
+}
+
+int64_t ga_mlp_num_params(void) { return (int64_t)gad::kMlpParams; }
+
+ga_status_t ga_mlp_create(ga_context_t* ctx, uint64_t seed, ga_mlp_t** mlp)
+{
+	if (!ctx || !mlp) return GA_EBADARG;
+	*mlp = nullptr;
+	return guarded([&]() {
+		auto m = std::make_unique<ga_mlp>();
+		m->model = std::make_unique<MlpModel>(ctx->ctx->device(), seed);
+		*mlp = m.release();
+		return GA_OK;
+	});
+}
+
+void ga_mlp_free(ga_mlp_t* mlp) { delete mlp; }
+
+ga_status_t ga_mlp_get_params(ga_mlp_t* mlp, float* params)
+{
+	if (!mlp || !params) return GA_EBADARG;
+	return guarded([&]() {
+		mlp->model->getWeights(params);
+		return GA_OK;
+	});
+}
+
+static bool mlpPaddingZero(const float* w)
+{
+	for (int i = gad::kMlpInUsed; i < gad::kMlpIn; i++)
+		for (int j = 0; j < gad::kMlpHidden; j++)
+			if (w[gad::kMlpW1 + (size_t)i * gad::kMlpHidden + j] != 0.0f) return false;
+	for (int i = 0; i < gad::kMlpHidden; i++)
+		for (int j = gad::kMlpOutUsed; j < gad::kMlpOut; j++)
+			if (w[gad::kMlpW3 + (size_t)i * gad::kMlpOut + j] != 0.0f) return false;
+	for (int j = gad::kMlpOutUsed; j < gad::kMlpOut; j++)
+		if (w[gad::kMlpB3 + j] != 0.0f) return false;
+	for (size_t k = 0; k < gad::kMlpParams; k++)
+		if (!std::isfinite(w[k])) return false;
+	return true;
+}
+
+ga_status_t ga_mlp_set_params(ga_mlp_t* mlp, const float* params)
+{
+	if (!mlp || !params || !mlpPaddingZero(params)) return GA_EBADARG;
+	return guarded([&]() {
+		mlp->model->setWeights(params);
+		return GA_OK;
+	});
+}
+
+static bool finiteRows(const float* x, int64_t n, int width)
+{
+	for (int64_t k = 0; k < n * width; k++)
+		if (!std::isfinite(x[k])) return false;
+	return true;
+}
+
+ga_status_t ga_mlp_forward(ga_mlp_t* mlp, int64_t n, const float* features, float* probs)
+{
+	if (!mlp || n < 0 || (n > 0 && (!features || !probs)) || n > (int64_t)1 << 30) return GA_EBADARG;
+	if (!finiteRows(features, n, GA_MLP_FEATURES)) return GA_EBADARG;
+	return guarded([&]() {
+		mlp->model->forward(features, (size_t)n, probs);
+		return GA_OK;
+	});
+}
+
+static bool labelsOk(const float* y, int64_t n)
+{
+	for (int64_t k = 0; k < n * GA_MLP_OUTPUTS; k++)
+		if (!(y[k] >= 0.0f && y[k] <= 1.0f)) return false;
+	return true;
+}
+
+ga_status_t ga_mlp_gradients(ga_mlp_t* mlp, int64_t n, const float* features, const float* labels, double* loss, float* grads)
+{
+	if (!mlp || n < 1 || n > (int64_t)1 << 24 || !features || !labels || !grads) return GA_EBADARG;
+	if (!finiteRows(features, n, GA_MLP_FEATURES) || !labelsOk(labels, n)) return GA_EBADARG;
+	return guarded([&]() {
+		const double l = mlp->model->gradients(features, labels, (size_t)n, grads);
+		if (loss) *loss = l;
+		return GA_OK;
+	});
+}
+
+ga_status_t ga_mlp_train(ga_mlp_t* mlp, int64_t n, const float* features, const float* labels, const ga_mlp_train_params_t* params, float* epochLoss, double* trainMs)
+{
+	if (!mlp || !params || n < 1 || n > (int64_t)1 << 30 || !features || !labels) return GA_EBADARG;
+	if (params->epochs < 1 || params->epochs > 64 || params->batch < 1 || params->batch > (1 << 20)) return GA_EBADARG;
+	if (!(params->lr > 0) || !(params->beta1 >= 0 && params->beta1 < 1) || !(params->beta2 >= 0 && params->beta2 < 1) || !(params->eps > 0)) return GA_EBADARG;
+	if (!finiteRows(features, n, GA_MLP_FEATURES) || !labelsOk(labels, n)) return GA_EBADARG;
+	return guarded([&]() {
+		MlpTrainOptions o;
+		o.epochs = (uint32_t)params->epochs;
+		o.batch = (uint32_t)params->batch;
+		o.lr = params->lr;
+		o.beta1 = params->beta1;
+		o.beta2 = params->beta2;
+		o.eps = params->eps;
+		o.seed = params->seed;
+		const MlpTrainStats st = mlp->model->train(features, labels, (size_t)n, o);
+		if (epochLoss)
+			for (size_t e = 0; e < st.epochLoss.size(); e++) epochLoss[e] = st.epochLoss[e];
+		if (trainMs) *trainMs = st.ms;
+		return GA_OK;
+	});
+}
+
+ga_status_t ga_cluster_records(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t first, int64_t count, ga_records_t** records)
+{
+	if (!ctx || !graph || !index || !params || !records || index->graph != graph || first < 0 || count < 0) return GA_EBADARG;
+	*records = nullptr;
+	if (!ctx->residentFromUpload) return GA_ESTALE;
+	if ((uint64_t)(first + count) > ctx->ctx->reads().size()) return GA_EBADARG;
+	return guarded([&]() {
+		auto r = std::make_unique<ga_records>();
+		AlignOptions opt = alignOptionsOf(*params);
+		opt.records = &r->rec;
+		std::vector<std::string> nm(count, "r");
+		std::vector<const char*> names(count);
+		for (int64_t i = 0; i < count; i++) names[i] = nm[i].c_str();
+		std::vector<ReadOutcome>& out = ctx->outcomes;
+		if (out.size() < (size_t)count) out.resize(count);
+		AlignStats st;
+		alignResident(*ctx->ctx, graph->host, index->idx, opt, names.data(), (uint32_t)first, (uint32_t)count, out.data(), st);
+		r->rec.sort();
+		*records = r.release();
+		return GA_OK;
+	});
+}
+
+int64_t ga_records_count(const ga_records_t* r) { return r ? (int64_t)r->rec.reads.size() : -1; }
+const float* ga_records_features(const ga_records_t* r) { return r ? r->rec.features.data() : nullptr; }
+const float* ga_records_labels(const ga_records_t* r) { return r ? r->rec.labels.data() : nullptr; }
+const int64_t* ga_records_reads(const ga_records_t* r) { return r ? r->rec.reads.data() : nullptr; }
+void ga_records_free(ga_records_t* r) { delete r; }
+
+ga_status_t ga_context_set_proposal(ga_context_t* ctx, ga_mlp_t* mlp, double threshold, int32_t keepTop)
+{
+	if (!ctx) return GA_EBADARG;
+	if (!mlp)
+	{
+		ctx->proposal = SeedProposal{};
+		return GA_OK;
+	}
+	if (keepTop < 1 || !std::isfinite(threshold) || mlp->model->device() != ctx->ctx->device()) return GA_EBADARG;
+	ctx->proposal.scorer = mlp->model.get();
+	ctx->proposal.threshold = threshold;
+	ctx->proposal.keepTop = (uint32_t)keepTop;
+	return GA_OK;
 }
 
 void ga_free(void* p) { std::free(p); }

@@ -5,6 +5,8 @@
 #include "ga_align.hpp"
 #include "ga_graph.hpp"
 #include "ga_index.hpp"
+#include "ga_mlp.hpp"
+#include "ga_proposal.hpp"
 #include "ga_runtime.hpp"
 
 struct ga_graph
@@ -28,6 +30,18 @@ struct ga_context
 	// replace it and clear this, so ga_align_resident fails with GA_ESTALE
 	// instead of aligning other reads under the caller's names
 	bool residentFromUpload = false;
+	// learned proposal applied by ga_align_reads / ga_align_resident (scorer null: off)
+	ga::SeedProposal proposal;
+};
+
+struct ga_mlp
+{
+	std::unique_ptr<ga::MlpModel> model;
+};
+
+struct ga_records
+{
+	ga::ClusterRecords rec;
 };
 
 struct ga_alignment_result

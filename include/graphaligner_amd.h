@@ -183,6 +183,10 @@ typedef struct
 	int64_t gafBytes;      /* length of the returned GAF text (without its NUL) */
 	int64_t persistentWorkers; /* deviceSeedLoop 2: workers per launch */
 	double tuneMs;         /* deviceSeedLoop 2: time spent choosing the worker count (first call) */
+	int64_t proposalClusters; /* learned proposal (ga_context_set_proposal): seed clusters scored, */
+	int64_t prunedClusters;   /* clusters removed, */
+	int64_t prunedSeeds;      /* seeds removed with them, */
+	double proposalMs;        /* and the time taken (features, network, pruning) */
 } ga_align_stats_t;
 
 /* Per-read alignment of src/Aligner.cpp:390-606 (seed, order, extend in a seed
@@ -304,6 +308,69 @@ ga_status_t ga_add_mapping_qualities(ga_alignment_result_t* result, int64_t read
  * GA_ASSERT when a reference assertion fires. */
 ga_status_t ga_gaf_line(const ga_graph_t* graph, const ga_alignment_result_t* result, int64_t read, int64_t i, const char* name, int64_t readLength, int32_t cigarMatchMismatchMerge, char* buf, int64_t cap, int64_t* needed);
 void ga_result_free(ga_alignment_result_t* result);
+
+/* ---- Learned proposal: seed-cluster network, outcome records, pruning ----
+ * No counterpart in the reference: an approximate mode beside the plain method.
+ * A seed cluster is a group of OrderSeeds (src/GraphAligner.h:250-312): seeds
+ * of one chain whose diagonals lie within 100 of their neighbours'. The network
+ * (48 features -> 128 -> 128 -> 2, ReLU, sigmoid outputs; BF16 weights and
+ * activations, FP32 accumulation, gfx950 MFMA forward and backward kernels)
+ * scores a cluster's chance that an alignment made from it is kept (output 0)
+ * and that the plain loop extends it at all (output 1). */
+
+#define GA_MLP_FEATURES 48
+#define GA_MLP_OUTPUTS 2
+
+typedef struct ga_mlp ga_mlp_t;
+typedef struct ga_records ga_records_t;
+
+/* Number of FP32 parameters in the flat layout of ga_mlp_get/set_params and
+ * ga_mlp_gradients: W1[64][128] (input-major; inputs 48..63 padding, zero),
+ * b1[128], W2[128][128], b2[128], W3[128][16] (outputs 2..15 padding), b3[16]. */
+int64_t ga_mlp_num_params(void);
+/* A network on the context's GPU, initialised from seed (He-uniform hidden
+ * layers, Glorot-uniform output layer, zero biases). */
+ga_status_t ga_mlp_create(ga_context_t* ctx, uint64_t seed, ga_mlp_t** mlp);
+void ga_mlp_free(ga_mlp_t* mlp);
+ga_status_t ga_mlp_get_params(ga_mlp_t* mlp, float* params);
+/* Replaces the parameters (padding entries must be zero) and resets Adam. */
+ga_status_t ga_mlp_set_params(ga_mlp_t* mlp, const float* params);
+/* probs[n][2] = sigmoid outputs for features[n][48] (FP32, rounded to BF16). */
+ga_status_t ga_mlp_forward(ga_mlp_t* mlp, int64_t n, const float* features, float* probs);
+/* Mean binary cross-entropy over both outputs of n records (labels[n][2] in
+ * [0, 1]) into *loss and its gradient into grads (ga_mlp_num_params floats). */
+ga_status_t ga_mlp_gradients(ga_mlp_t* mlp, int64_t n, const float* features, const float* labels, double* loss, float* grads);
+
+typedef struct
+{
+	int32_t epochs;   /* 1..64 */
+	int32_t batch;    /* records per Adam step, 1..2^20 */
+	float lr, beta1, beta2, eps;
+	uint64_t seed;    /* record shuffle of every epoch */
+} ga_mlp_train_params_t;
+
+/* Adam on the GPU over n records; epochLoss[e] (params->epochs floats, may be
+ * NULL) receives epoch e's mean loss, *trainMs (may be NULL) the device time. */
+ga_status_t ga_mlp_train(ga_mlp_t* mlp, int64_t n, const float* features, const float* labels, const ga_mlp_train_params_t* params, float* epochLoss, double* trainMs);
+
+/* Outcome records of the plain method on resident reads [first, first + count)
+ * (ga_reads_upload; GA_ESTALE as ga_align_resident): one record per seed
+ * cluster of every read whose seeds were ordered, in read order, clusters by
+ * first appearance in the read's seed order. Labels: [0] an alignment extended
+ * from one of the cluster's seeds was selected, [1] one was made. Any proposal
+ * set on the context is not applied. */
+ga_status_t ga_cluster_records(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_t* index, const ga_params_t* params, int64_t first, int64_t count, ga_records_t** records);
+int64_t ga_records_count(const ga_records_t* records);
+const float* ga_records_features(const ga_records_t* records);  /* [count][48] */
+const float* ga_records_labels(const ga_records_t* records);    /* [count][2] */
+const int64_t* ga_records_reads(const ga_records_t* records);   /* [count], read index - first */
+void ga_records_free(ga_records_t* records);
+
+/* From now on ga_align_reads and ga_align_resident on ctx remove, before each
+ * read's seed loop, the seeds of every cluster whose output 0 is below
+ * threshold, except the read's first keepTop (>= 1) clusters. mlp NULL turns
+ * the proposal off (the plain method). The network must stay alive while set. */
+ga_status_t ga_context_set_proposal(ga_context_t* ctx, ga_mlp_t* mlp, double threshold, int32_t keepTop);
 
 void ga_free(void* p);
 

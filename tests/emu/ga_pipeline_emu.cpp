@@ -6,6 +6,8 @@
 #define GA_THREAD_PER_EXTENSION 1
 #define GA_THREAD_HOST_TEST 1
 #include <atomic>
+#include <cmath>
+#include <memory>
 #include <mutex>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +17,7 @@
 #include <thread>
 #include <vector>
 #include "../../graphaligner_amd/csrc/ga_align.hpp"
+#include "../../graphaligner_amd/csrc/ga_proposal.hpp"
 #include "../../graphaligner_amd/csrc/ga_extend_core.h"
 #include "../../graphaligner_amd/csrc/ga_read_core.h"
 #include "../../graphaligner_amd/csrc/ga_seed_core.h"
@@ -25,6 +28,28 @@ namespace {
 // of an extension batch (the replay then meets a trace it cannot decode), 2
 // reports a seed match outside the index's position array
 int g_fault = 0;
+
+// a logistic scorer over the cluster features (both outputs the same), standing
+// in for the network in CPU tests of the proposal's pruning
+struct LinearScorer : ga::ClusterScorer
+{
+	std::vector<float> w;
+	float bias = 0;
+	void score(const float* f, size_t n, float* out) override
+	{
+		for (size_t i = 0; i < n; i++)
+		{
+			double z = bias;
+			for (int k = 0; k < ga::kClusterFeatures; k++) z += (double)w[k] * f[i * ga::kClusterFeatures + k];
+			out[i * 2] = out[i * 2 + 1] = (float)(1.0 / (1.0 + std::exp(-z)));
+		}
+	}
+};
+LinearScorer g_scorer;
+ga::SeedProposal g_proposal;
+bool g_proposalOn = false;
+std::unique_ptr<ga::ClusterRecords> g_records;
+ga::AlignStats g_lastStats;
 
 template <typename F>
 void forEachParallel(size_t n, F fn)
@@ -479,6 +504,8 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 	o.batchedDeviceLoop = po->deviceLoop == 2;
 	o.maxSpeculativeSeeds = po->maxSpeculativeSeeds < 1 ? 1 : (uint32_t)po->maxSpeculativeSeeds;
 	o.minAlignmentScore = po->minAlignmentScore;
+	if (g_proposalOn) o.proposal = &g_proposal;
+	if (g_records) o.records = g_records.get();
 	std::vector<const char*> nm(names, names + nreads), sq(seqs, seqs + nreads);
 	std::vector<uint32_t> ln(nreads);
 	for (int64_t i = 0; i < nreads; i++) ln[i] = (uint32_t)lens[i];
@@ -488,6 +515,7 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 	try
 	{
 		ga::alignReads(be, *x->g, x->idx, o, nm, sq, ln, out, st);
+		g_lastStats = st;
 	}
 	catch (const std::exception&)
 	{
@@ -594,4 +622,74 @@ int pipe_align_one_way(void* ih, const PipeOptions* po, const char* seq, int64_t
 }
 
 void pipe_free(void* p) { std::free(p); }
+
+// proposal with a logistic scorer: w[48], bias; threshold and keepTop as the
+// product's ga_context_set_proposal; on = 0 turns it off
+void pipe_set_linear_proposal(int on, const float* w, float bias, double threshold, int keepTop)
+{
+	g_proposalOn = on != 0;
+	if (!g_proposalOn) return;
+	g_scorer.w.assign(w, w + ga::kClusterFeatures);
+	g_scorer.bias = bias;
+	g_proposal.scorer = &g_scorer;
+	g_proposal.threshold = threshold;
+	g_proposal.keepTop = (uint32_t)keepTop;
+}
+
+// outcome records of the following pipe_align_reads calls (on = 1 starts empty)
+void pipe_records(int on)
+{
+	if (on) g_records = std::make_unique<ga::ClusterRecords>();
+	else g_records.reset();
+}
+
+int64_t pipe_records_take(float** features, float** labels, int64_t** reads)
+{
+	if (!g_records) return -1;
+	g_records->sort();
+	const size_t n = g_records->reads.size();
+	*features = (float*)std::malloc(std::max<size_t>(1, g_records->features.size()) * sizeof(float));
+	*labels = (float*)std::malloc(std::max<size_t>(1, g_records->labels.size()) * sizeof(float));
+	*reads = (int64_t*)std::malloc(std::max<size_t>(1, n) * sizeof(int64_t));
+	std::memcpy(*features, g_records->features.data(), g_records->features.size() * sizeof(float));
+	std::memcpy(*labels, g_records->labels.data(), g_records->labels.size() * sizeof(float));
+	std::memcpy(*reads, g_records->reads.data(), n * sizeof(int64_t));
+	g_records = std::make_unique<ga::ClusterRecords>();
+	return (int64_t)n;
+}
+
+// proposal counters of the last pipe_align_reads: clusters scored, clusters and seeds removed
+void pipe_proposal_stats(int64_t* out)
+{
+	out[0] = (int64_t)g_lastStats.proposalClusters;
+	out[1] = (int64_t)g_lastStats.prunedClusters;
+	out[2] = (int64_t)g_lastStats.prunedSeeds;
+}
+
+// clusters of ordered seeds (rows of 10 as pipe_get_seeds) and their features
+int64_t pipe_cluster_features(void* ih, const char* seq, int64_t len, const int64_t* seeds, int64_t nseeds, uint32_t* clusterOf, float* features, int64_t cap)
+{
+	auto* x = (EmuIndex*)ih;
+	std::vector<ga::SeedHit> hits(nseeds);
+	for (int64_t i = 0; i < nseeds; i++)
+	{
+		const int64_t* v = seeds + 10 * i;
+		hits[i].nodeID = (int)v[0];
+		hits[i].nodeOffset = v[1];
+		hits[i].seqPos = v[2];
+		hits[i].matchLen = v[3];
+		hits[i].reverse = v[4] != 0;
+		hits[i].alignmentGraphNodeId = v[5] < 0 ? ~0ull : (uint64_t)v[5];
+		hits[i].alignmentGraphNodeOffset = v[6] < 0 ? ~0ull : (uint64_t)v[6];
+		hits[i].rawSeedGoodness = v[7];
+		hits[i].seedGoodness = v[8];
+		hits[i].seedClusterSize = v[9];
+	}
+	std::vector<uint32_t> c;
+	const uint32_t n = ga::seedClusters(*x->g, hits, c);
+	for (int64_t i = 0; i < nseeds; i++) clusterOf[i] = c[i];
+	if ((int64_t)n > cap) return -(int64_t)n;
+	ga::clusterFeatures(*x->g, x->idx.maxCount, seq, (uint32_t)len, hits, c, n, features);
+	return n;
+}
 }

@@ -53,6 +53,15 @@ def lib(variant=""):
         L.pipe_align_one_way.argtypes = [ctypes.c_void_p, ctypes.POINTER(PipeOptions), ctypes.c_char_p, ctypes.c_int64,
                                          ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_void_p)]
+        L.pipe_set_linear_proposal.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.c_double, ctypes.c_int]
+        L.pipe_records.argtypes = [ctypes.c_int]
+        L.pipe_records_take.restype = ctypes.c_int64
+        L.pipe_records_take.argtypes = [ctypes.POINTER(ctypes.c_void_p)] * 3
+        L.pipe_proposal_stats.argtypes = [ctypes.POINTER(ctypes.c_int64)]
+        L.pipe_cluster_features.restype = ctypes.c_int64
+        L.pipe_cluster_features.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float),
+                                            ctypes.c_int64]
         _LIBS[variant] = L
     return _LIBS[variant]
 
@@ -151,3 +160,52 @@ def align_one_way(index, oracle_params, seq, seeds, select=False, mapq=False, sp
     text = ctypes.string_at(out.value).decode()
     index.L.pipe_free(out)
     return text
+
+
+def set_linear_proposal(weights, bias, threshold, keep_top=1, variant=""):
+    """Proposal with a logistic scorer over the 48 cluster features (None: off)."""
+    L = lib(variant)
+    if weights is None:
+        L.pipe_set_linear_proposal(0, None, 0.0, 0.0, 1)
+        return
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    assert w.size == 48
+    L.pipe_set_linear_proposal(1, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), bias, threshold, keep_top)
+
+
+def records(on, variant=""):
+    lib(variant).pipe_records(1 if on else 0)
+
+
+def take_records(variant=""):
+    """(features [n, 48], labels [n, 2], read index [n]) recorded since the last take."""
+    L = lib(variant)
+    f, y, r = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    n = L.pipe_records_take(ctypes.byref(f), ctypes.byref(y), ctypes.byref(r))
+    assert n >= 0
+    F = np.ctypeslib.as_array(ctypes.cast(f, ctypes.POINTER(ctypes.c_float)), shape=(max(n, 1) * 48,))[: n * 48].reshape(n, 48).copy()
+    Y = np.ctypeslib.as_array(ctypes.cast(y, ctypes.POINTER(ctypes.c_float)), shape=(max(n, 1) * 2,))[: n * 2].reshape(n, 2).copy()
+    R = np.ctypeslib.as_array(ctypes.cast(r, ctypes.POINTER(ctypes.c_int64)), shape=(max(n, 1),))[:n].copy()
+    for p in (f, y, r):
+        L.pipe_free(p)
+    return F, Y, R
+
+
+def proposal_stats(variant=""):
+    out = np.zeros(3, dtype=np.int64)
+    lib(variant).pipe_proposal_stats(out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return dict(clusters=int(out[0]), pruned_clusters=int(out[1]), pruned_seeds=int(out[2]))
+
+
+def cluster_features(index, seq, seeds):
+    """(cluster of every seed, features [clusters, 48]) of ordered seeds."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.int64).reshape(-1, 10)
+    n = len(seeds)
+    cl = np.zeros(max(n, 1), dtype=np.uint32)
+    cap = max(n, 1)
+    feats = np.zeros(cap * 48, dtype=np.float32)
+    k = index.L.pipe_cluster_features(index.h, seq.encode(), len(seq), seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), n,
+                                      cl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                      feats.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), cap)
+    assert k >= 0
+    return cl[:n], feats[: k * 48].reshape(k, 48)

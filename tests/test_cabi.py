@@ -12,7 +12,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "gr
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:ga_status_t|void|int64_t|int32_t)\s+(ga_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:ga_status_t|void|int64_t|int32_t|const\s+\w+\s*\*)\s*(ga_\w+)\s*\(", text, re.M)))
 
 
 def test_every_declared_symbol_is_exported():
