@@ -56,6 +56,10 @@ def build_test_infrastructure():
     # the one-extension-per-thread helpers the gfx950 kernel uses
     _run(common + ["-o", os.path.join(tb, "libga_emu.so")] + srcs)
     _run(common + ["-DGA_THREAD_PER_EXTENSION", "-DGA_THREAD_HOST_TEST", "-o", os.path.join(tb, "libga_thread.so")] + srcs)
+    # the wave-cooperative lane code on 64 host threads per wave that meet at a
+    # barrier in every wave intrinsic (divergent lanes are reported, not hung)
+    _run(common + ["-pthread", "-o", os.path.join(tb, "libga_simt.so"), os.path.join(ROOT, "tests", "emu", "ga_simt.cpp"),
+                   os.path.join(CSRC, "ga_graph.cpp")])
     # the product's host pipeline over a backend running the device code on host threads
     pipe = [os.path.join(ROOT, "tests", "emu", "ga_pipeline_emu.cpp")] + [os.path.join(CSRC, s) for s in ("ga_align.cpp", "ga_proposal.cpp", "ga_index.cpp", "ga_graph.cpp")]
     _run(common + ["-pthread", "-o", os.path.join(tb, "libga_pipe.so")] + pipe)

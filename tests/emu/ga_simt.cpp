@@ -3,6 +3,7 @@
 // meet at a barrier in every wave intrinsic. Test infrastructure only: it checks
 // the exact lane code and that every lane follows the same control flow (a lane
 // that diverges never reaches the barrier, which the harness reports as a hang).
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -10,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <thread>
+#include <stdexcept>
 #include <vector>
 #include <cstdio>
 
@@ -200,5 +202,106 @@ int simt_extend(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int
 		traceOut[4 * i + 3] = hi & 1;
 	}
 	return out.status;
+}
+
+// The persistent work loop of the wave-cooperative kernel: `waves` waves of 64
+// lane threads; lane 0 of a wave takes the next request from a shared counter,
+// the index is broadcast with a shuffle, every lane leaves when it passes n.
+// Requests are extensions (seqs[i], bigraph node, offset); status and score of
+// each come back. Returns -1 if any wave's lanes diverged, -2 if the lanes of
+// a wave disagree on a result, else 0.
+int simt_persistent(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int32_t xdrop, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* nodes, const int64_t* offsets, int waves, int32_t* statusOut, int32_t* scoreOut)
+{
+	auto& G = *(ga::GraphHost*)gh;
+	gad::DevGraph g;
+	g.nodeLen = G.nodeLen.data();
+	g.seq2 = G.seq2.data();
+	g.amb4 = G.amb4.data();
+	g.comp = G.comp.data();
+	g.linearizable = G.linearizable.data();
+	g.inOff = G.inOff.data();
+	g.inAdj = G.inAdj.data();
+	g.outOff = G.outOff.data();
+	g.outAdj = G.outAdj.data();
+	g.numNodes = G.numNodes();
+	g.firstAmbiguous = G.firstAmbiguous;
+	gad::ExtParams p;
+	p.bandwidth = bandwidth;
+	p.errorCost = cost;
+	p.xdrop = xdrop;
+	p.pad = 0;
+	p.maxCellsPerSlice = maxCells <= 0 ? INT64_MAX : maxCells;
+	int64_t maxLen = 1;
+	for (int64_t i = 0; i < n; i++) maxLen = std::max<int64_t>(maxLen, lens[i]);
+	gad::ArenaCaps caps;
+	caps.maxSlices = maxLen / 64 + 3;
+	caps.maxItems = caps.maxSlices * 48 + 1024;
+	caps.maxSlots = 4096;
+	caps.maxExtras = 16384;
+	caps.maxPending = 4096;
+	caps.pad = 0;
+	const size_t bytes = gad::arenaBytes(caps) + 256;
+	const int64_t cap = 4 * maxLen + 1024;
+	std::atomic<uint32_t> counter{0};
+	std::vector<simt::Wave> waveState(waves);
+	std::vector<std::vector<gad::ExtResult>> perLane(waves * 64, std::vector<gad::ExtResult>(n));
+	std::atomic<int> failed{0};
+	std::vector<std::thread> threads;
+	for (int w = 0; w < waves; w++)
+		for (int l = 0; l < 64; l++)
+			threads.emplace_back([&, w, l]() {
+				simt::tlsLane = l;
+				simt::tlsWave = &waveState[w];
+				// each lane its own workspace and trace buffer (lanes are not in lockstep here)
+				std::vector<unsigned char> arena(bytes, 0xA5);
+				std::vector<uint64_t> trace(cap + 1, 0);
+				unsigned char* base = (unsigned char*)(((uintptr_t)arena.data() + 255) & ~(uintptr_t)255);
+				try
+				{
+					while (true)
+					{
+						uint32_t idx = 0;
+						if (l == 0) idx = counter.fetch_add(1u);
+						idx = __shfl(idx, 0);
+						if (idx >= (uint64_t)n) break;
+						gad::ExtRequest r;
+						const uint32_t node = G.unitigNode(nodes[idx], offsets[idx]);
+						r.seqOffset = 0;
+						r.traceOffset = 0;
+						r.len = lens[idx];
+						r.startNode = node;
+						r.startOffset = offsets[idx] - G.nodeOff[node];
+						r.traceCap = cap;
+						gad::ExtResult out;
+						gad::runRequest(g, p, r, seqs[idx], trace.data(), base, caps, nullptr, out);
+						perLane[w * 64 + l][idx] = out;
+					}
+				}
+				catch (const std::exception&)
+				{
+					failed = 1;
+					for (auto& ws : waveState)
+					{
+						std::lock_guard<std::mutex> lk(ws.m);
+						ws.generation++;
+						ws.cv.notify_all();
+					}
+				}
+			});
+	for (auto& t : threads) t.join();
+	if (failed) return -1;
+	// the wave that ran request i: all its lanes hold the same result
+	for (int64_t i = 0; i < n; i++)
+	{
+		int owner = -1;
+		for (int w = 0; w < waves && owner < 0; w++)
+			if (perLane[w * 64][i].nslices || perLane[w * 64][i].ntrace || perLane[w * 64][i].status) owner = w;
+		if (owner < 0) owner = 0;
+		for (int l = 1; l < 64; l++)
+			if (std::memcmp(&perLane[owner * 64 + l][i], &perLane[owner * 64][i], sizeof(gad::ExtResult)) != 0) return -2;
+		statusOut[i] = perLane[owner * 64][i].status;
+		scoreOut[i] = perLane[owner * 64][i].score;
+	}
+	return 0;
 }
 }
