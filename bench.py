@@ -52,6 +52,9 @@ TRAFFIC_SOURCE = "constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the defau
 
 
 WORKLOADS = {
+    "dbg2m": dict(preset="dbg2m", k=19, w=30, params="dbg",
+                  text="dbg2m: cyclic de Bruijn graph (2 Mbp genome, repeat families, tandem arrays; k=63 unitigs), "
+                       "10,000 reads of 10-30 kbp at 8% error, -x dbg"),
     "vg5m": dict(preset="vg5m", k=15, w=20,
                  text="vg5m: 5 Mbp variation graph with SV bubbles, 20,000 log-normal reads (median 12 kbp) at 12% error, -x vg"),
     "stream": dict(preset="vg5m_stream", k=21, w=31,
@@ -87,6 +90,10 @@ def parse():
     ap.add_argument("--seed-loop", choices=["auto", "rounds", "device", "batched"], default="auto",
                     help="rounds: host-driven rounds of batched extensions; device: each read's whole seed loop on the GPU; "
                          "batched: the persistent form (auto: batched for the stream, rounds otherwise)")
+    ap.add_argument("--learned", action="store_true",
+                    help="the learned proposal: train the network on outcome records of the plain method (fresh seeds of "
+                         "the 5 Mbp variation-graph generator), then time plain and learned on --workload (vg5m or dbg2m)")
+    ap.add_argument("--train-records", type=int, default=1_000_000, help="--learned: outcome records to train on (at least)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (traffic-counter calibration) and exit")
     return ap.parse_args()
@@ -112,6 +119,11 @@ def main():
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path, and "
               "--gpus %d is not supported (run with --gpus 1)" % args.gpus, file=sys.stderr)
+        return 2
+    if args.learned:
+        return learned_main(args)
+    if args.workload == "dbg2m":
+        print("--workload dbg2m is timed with --learned (plain and learned side by side)", file=sys.stderr)
         return 2
     import numpy as np
     import graphaligner_amd as ga
@@ -226,6 +238,150 @@ def main():
         "gaf_identical_to_cpu_sample": parity,
         "setup_s": {"generate": gen_s, "graph_load": load_s, "index_build": index_s, "read_upload": upload_s},
         "pcie_inclusive_reads_per_s": len(names) * args.steps / (elapsed + args.steps * upload_s),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def gaf_agreement(plain, learned):
+    """(identical, plain lines, extra): the plain method's GAF lines the learned
+    form reproduces byte for byte (matched per read), and its lines that are not
+    among the plain ones."""
+    def per_read(text):
+        d = {}
+        for ln in text.split("\n"):
+            if ln:
+                d.setdefault(ln.split("\t", 1)[0], []).append(ln)
+        return d
+    a, b = per_read(plain), per_read(learned)
+    same = total = extra = 0
+    for name in set(a) | set(b):
+        pa, pb = list(a.get(name, [])), list(b.get(name, []))
+        total += len(pa)
+        for ln in pa:
+            if ln in pb:
+                pb.remove(ln)
+                same += 1
+        extra += len(pb)
+    return same, total, extra
+
+
+# the learned form scores a read's first 64 seed clusters and removes the rest
+# unscored (in the outcome records of both generators no cluster after the
+# first ever made a kept alignment)
+MAX_SCORED_CLUSTERS = 64
+
+
+def learned_main(args):
+    """Train the proposal network on the GPU from outcome records of the plain
+    method, then time the plain method and the learned form (clusters pruned
+    before each read's seed loop) on the same resident reads."""
+    import numpy as np
+    import graphaligner_amd as ga
+    from graphaligner_amd import synth
+
+    workdir = args.workdir or tempfile.mkdtemp(prefix="ga_bench_")
+    os.makedirs(workdir, exist_ok=True)
+    ctx = ga.Context(0)
+    # training data: the 5 Mbp variation-graph generator with fresh seeds
+    # (graph 12, reads 2012), reads until the records reach --train-records
+    t = time.time()
+    tprefix = os.path.join(workdir, "learn_train")
+    train_reads = max(2000, args.train_records // 16)
+    if not os.path.exists(tprefix + ".fa"):
+        synth.generate("vg5m", tprefix, overrides_graph=dict(seed=12), overrides_reads=dict(numReads=train_reads, seed=2012))
+    tn, ts = synth.read_fasta(tprefix + ".fa")
+    vp = ga.params_preset("vg")
+    tg = ga.Graph(tprefix + ".gfa")
+    ti = ga.Index(tg, vp)
+    ga.upload_reads(ctx, ts)
+    F, Y, R = ga.cluster_records(ctx, tg, ti, vp, 0, len(ts))
+    rec_s = time.time() - t
+    held = R >= int(0.9 * len(ts))  # the last 10 % of the reads: held out
+    print(f"[bench] {len(F)} outcome records from {len(ts)} reads ({rec_s:.1f} s), "
+          f"{Y[:, 0].mean():.4f} of clusters kept an alignment", file=sys.stderr, flush=True)
+    net = ga.Mlp(ctx, seed=1)
+    losses, train_ms = net.train(F[~held], Y[~held], epochs=4, batch=4096, lr=2e-3, seed=1)
+    P = net.forward(F[held])
+    auc = []
+    for h in range(2):
+        y = Y[held, h] > 0.5
+        ranks = np.empty(len(y))
+        ranks[np.argsort(P[:, h], kind="stable")] = np.arange(1, len(y) + 1)
+        pos, neg = y.sum(), (~y).sum()
+        auc.append(float((ranks[y].sum() - pos * (pos + 1) / 2) / (pos * neg)) if pos and neg else None)
+    print(f"[bench] trained: loss {losses.tolist()}, {train_ms:.1f} ms, held-out AUC {auc}", file=sys.stderr, flush=True)
+    # threshold: the largest of a grid whose held-out reads keep >= 99.9 % of
+    # their plain GAF lines
+    hn = [tn[i] for i in range(len(ts)) if i >= int(0.9 * len(ts))]
+    first = int(0.9 * len(ts))
+    hplain, _, _ = ga.align_resident(ctx, tg, ti, vp, first, hn)
+    threshold = 0.0
+    for thr in (0.01, 0.05, 0.1, 0.2, 0.3, 0.5, 0.7, 0.9):
+        ga.set_proposal(ctx, net, threshold=thr, max_clusters=MAX_SCORED_CLUSTERS)
+        text, _, _ = ga.align_resident(ctx, tg, ti, vp, first, hn)
+        ga.set_proposal(ctx, None)
+        same, total, _ = gaf_agreement(hplain, text)
+        if same >= 0.999 * total:
+            threshold = thr
+    print(f"[bench] threshold {threshold}", file=sys.stderr, flush=True)
+    del tg, ti
+
+    wl = WORKLOADS[args.workload if args.workload in ("vg5m", "dbg2m") else "vg5m"]
+    prefix = os.path.join(workdir, wl["preset"])
+    if not (os.path.exists(prefix + ".gfa") and os.path.exists(prefix + ".fa")):
+        synth.generate(wl["preset"], prefix)
+    names, seqs = synth.read_fasta(prefix + ".fa")
+    if args.reads:
+        names, seqs = names[: args.reads], seqs[: args.reads]
+    params = ga.params_preset(wl.get("params", "vg"))
+    graph = ga.Graph(prefix + ".gfa")
+    index = ga.Index(graph, params)
+    ga.upload_reads(ctx, seqs)
+    ctx.synchronize()
+
+    def timed(learned):
+        ga.set_proposal(ctx, net if learned else None, threshold=threshold, max_clusters=MAX_SCORED_CLUSTERS)
+        for _ in range(args.warmup):
+            ga.align_resident(ctx, graph, index, params, 0, names, gaf_format="buffer")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        tot = dict(extendKernelMs=0.0, wordSteps=0, extensions=0, proposalMs=0.0, prunedClusters=0, proposalClusters=0)
+        for _ in range(args.steps):
+            gaf, st, s = ga.align_resident(ctx, graph, index, params, 0, names, gaf_format="buffer")
+            for k in tot:
+                tot[k] += s[k]
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        ga.set_proposal(ctx, None)
+        return gaf.tobytes().decode(), st, el, tot
+
+    plain, pst, pel, ptot = timed(False)
+    learned, lst, lel, ltot = timed(True)
+    same, total, extra = gaf_agreement(plain, learned)
+    steps = max(args.steps, 1)
+    plain_rps = len(names) * args.steps / pel
+    learned_rps = len(names) * args.steps / lel
+    line = {
+        "metric": "reads/s + GCUPS on 1 MI355X; learned vs plain at >=99.5% identical GAF",
+        "value": learned_rps, "unit": "reads/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": lel * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 (DP), bf16 (network)", "data": "synthetic",
+        "gcups": 64 * ltot["wordSteps"] / lel / 1e9,
+        "config": {"workload": wl["text"], "reads": len(names), "preset": wl.get("params", "vg"), "parallelism": "1 GPU",
+                   "seed_loop": "rounds", "form": "learned proposal (seed clusters scored and pruned before each read's loop)"},
+        "plain_reads_per_s": plain_rps, "speedup_vs_plain": learned_rps / plain_rps,
+        "gaf_identical_fraction": same / max(total, 1), "gaf_plain_lines": total, "gaf_extra_lines": extra,
+        "status_identical": bool((pst == lst).all()),
+        "threshold": threshold, "max_scored_clusters": MAX_SCORED_CLUSTERS,
+        "per_step": {"plain_word_steps": ptot["wordSteps"] // steps, "learned_word_steps": ltot["wordSteps"] // steps,
+                     "plain_extensions": ptot["extensions"] // steps, "learned_extensions": ltot["extensions"] // steps,
+                     "clusters": ltot["proposalClusters"] // steps, "pruned_clusters": ltot["prunedClusters"] // steps,
+                     "proposal_s": ltot["proposalMs"] / 1e3 / steps},
+        "training": {"records": int((~held).sum()), "held_out_records": int(held.sum()), "reads": len(ts),
+                     "generator": "vg5m graph seed 12, reads seed 2012", "epochs": 4, "batch": 4096, "lr": 2e-3,
+                     "epoch_loss": losses.tolist(), "train_ms": train_ms, "held_out_auc": auc,
+                     "kept_fraction": float(Y[:, 0].mean()), "records_s": rec_s},
     }
     print(json.dumps(line), flush=True)
     return 0

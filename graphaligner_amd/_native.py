@@ -153,6 +153,6 @@ def lib():
             getattr(L, f).restype = vp
             getattr(L, f).argtypes = [vp]
         L.ga_records_free.argtypes = [vp]
-        L.ga_context_set_proposal.argtypes = [vp, vp, ctypes.c_double, i32]
+        L.ga_context_set_proposal.argtypes = [vp, vp, ctypes.c_double, i32, i32]
         _lib = L
     return _lib

@@ -423,9 +423,10 @@ def cluster_records(ctx, graph, index, params, first, count):
         L.ga_records_free(h)
 
 
-def set_proposal(ctx, mlp, threshold=0.5, keep_top=1):
+def set_proposal(ctx, mlp, threshold=0.5, keep_top=1, max_clusters=0):
     """Prune seed clusters the network scores below threshold in later align calls
-    on ctx (mlp None: the plain method)."""
-    _check(N.lib().ga_context_set_proposal(ctx.h, mlp.h if mlp is not None else None, threshold, keep_top),
+    on ctx (mlp None: the plain method); only each read's first max_clusters
+    clusters are scored, later ones are removed (0: all scored)."""
+    _check(N.lib().ga_context_set_proposal(ctx.h, mlp.h if mlp is not None else None, threshold, keep_top, max_clusters),
            "ga_context_set_proposal")
     ctx.proposal = mlp  # keeps the network alive while it is set

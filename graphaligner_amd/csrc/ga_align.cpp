@@ -972,7 +972,7 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 				if (!ready[i]) return;
 				const uint32_t nc = seedClusters(g, seeds[i], clusterOf[i]);
 				features[i].resize((size_t)nc * kClusterFeatures);
-				clusterFeatures(g, idx.maxCount, rs.seq(base + i), rs.length[base + i], seeds[i], clusterOf[i], nc, features[i].data());
+				clusterFeatures(g, idx.maxCount, rs.seq(base + i), rs.length[base + i], seeds[i], clusterOf[i], nc, nc, features[i].data());
 			});
 		std::vector<ReadLoopJob> jobs;
 		std::vector<uint32_t> jobRead;
@@ -1476,7 +1476,7 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 					if (st.done || st.seeds.empty()) return;
 					st.nclusters = seedClusters(g, st.seeds, st.clusterOf);
 					st.features.resize((size_t)st.nclusters * kClusterFeatures);
-					clusterFeatures(g, idxp ? idxp->maxCount : 0, bseqs[st.read], blens[st.read], st.seeds, st.clusterOf, st.nclusters, st.features.data());
+					clusterFeatures(g, idxp ? idxp->maxCount : 0, bseqs[st.read], blens[st.read], st.seeds, st.clusterOf, st.nclusters, st.nclusters, st.features.data());
 				});
 		};
 		auto runGroup = [&](uint32_t lane) {

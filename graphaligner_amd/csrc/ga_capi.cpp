@@ -887,7 +887,7 @@ const float* ga_records_labels(const ga_records_t* r) { return r ? r->rec.labels
 const int64_t* ga_records_reads(const ga_records_t* r) { return r ? r->rec.reads.data() : nullptr; }
 void ga_records_free(ga_records_t* r) { delete r; }
 
-ga_status_t ga_context_set_proposal(ga_context_t* ctx, ga_mlp_t* mlp, double threshold, int32_t keepTop)
+ga_status_t ga_context_set_proposal(ga_context_t* ctx, ga_mlp_t* mlp, double threshold, int32_t keepTop, int32_t maxClusters)
 {
 	if (!ctx) return GA_EBADARG;
 	if (!mlp)
@@ -895,7 +895,8 @@ ga_status_t ga_context_set_proposal(ga_context_t* ctx, ga_mlp_t* mlp, double thr
 		ctx->proposal = SeedProposal{};
 		return GA_OK;
 	}
-	if (keepTop < 1 || !std::isfinite(threshold) || mlp->model->device() != ctx->ctx->device()) return GA_EBADARG;
+	if (keepTop < 1 || maxClusters < 0 || !std::isfinite(threshold) || mlp->model->device() != ctx->ctx->device()) return GA_EBADARG;
+	ctx->proposal.maxClusters = (uint32_t)maxClusters;
 	ctx->proposal.scorer = mlp->model.get();
 	ctx->proposal.threshold = threshold;
 	ctx->proposal.keepTop = (uint32_t)keepTop;

@@ -184,8 +184,9 @@ uint32_t seedClusters(const GraphHost& g, const std::vector<SeedHit>& seeds, std
 	return n;
 }
 
-void clusterFeatures(const GraphHost& g, uint64_t maxCount, const char* seq, uint32_t len, const std::vector<SeedHit>& seeds, const std::vector<uint32_t>& clusterOf, uint32_t nclusters, float* out)
+void clusterFeatures(const GraphHost& g, uint64_t maxCount, const char* seq, uint32_t len, const std::vector<SeedHit>& seeds, const std::vector<uint32_t>& clusterOf, uint32_t nclusters, uint32_t rows, float* out)
 {
+	rows = std::min(rows, nclusters);
 	auto shape = shapeOf(g);
 	const double L = std::max<uint32_t>(1, len);
 	struct C
@@ -259,9 +260,20 @@ void clusterFeatures(const GraphHost& g, uint64_t maxCount, const char* seq, uin
 		maxSize = std::max(maxSize, x.members.size());
 	}
 	// read coverage of the clusters ranked before each one
-	std::vector<uint8_t> coveredBefore(len, 0);
+	// read positions covered by the spans of the clusters ranked before the
+	// current one: disjoint intervals in position order
+	std::vector<std::pair<uint64_t, uint64_t>> coveredBefore;
 	uint64_t coveredCount = 0;
-	for (uint32_t c = 0; c < nclusters; c++)
+	auto coveredIn = [&](uint64_t from, uint64_t to) {
+		uint64_t n = 0;
+		for (const auto& iv : coveredBefore)
+		{
+			const uint64_t lo = std::max(from, iv.first), hi = std::min(to, iv.second);
+			if (hi > lo) n += hi - lo;
+		}
+		return n;
+	};
+	for (uint32_t c = 0; c < rows; c++)
 	{
 		const C& x = cl[c];
 		float* f = out + (size_t)c * kClusterFeatures;
@@ -269,7 +281,7 @@ void clusterFeatures(const GraphHost& g, uint64_t maxCount, const char* seq, uin
 		const double n = (double)x.members.size();
 		const double span = (double)(x.spanTo - x.spanFrom);
 		uint64_t spanCovered = 0;
-		for (uint64_t p = x.spanFrom; p < x.spanTo && p < len; p++) spanCovered += coveredBefore[p];
+		spanCovered = coveredIn(x.spanFrom, std::min<uint64_t>(x.spanTo, len));
 		uint32_t overlapping = 0;
 		double bestOverlapGoodness = 0;
 		uint64_t bestClusterOverlap = 0;
@@ -338,12 +350,29 @@ void clusterFeatures(const GraphHost& g, uint64_t maxCount, const char* seq, uin
 		f[45] = seeds.empty() ? 0.0f : (float)(n / seeds.size());
 		f[46] = lg((double)maxCount) / 16;
 		f[47] = 1.0f;
-		for (uint64_t p = x.spanFrom; p < x.spanTo && p < len; p++)
-			if (!coveredBefore[p])
+		{
+			// add [spanFrom, spanTo) to the union, merging the intervals it touches
+			const uint64_t from = std::min<uint64_t>(x.spanFrom, len), to = std::min<uint64_t>(x.spanTo, len);
+			if (to > from)
 			{
-				coveredBefore[p] = 1;
-				coveredCount++;
+				coveredCount += (to - from) - coveredIn(from, to);
+				std::vector<std::pair<uint64_t, uint64_t>> merged;
+				uint64_t lo = from, hi = to;
+				for (const auto& iv : coveredBefore)
+				{
+					if (iv.second < lo || iv.first > hi)
+						merged.push_back(iv);
+					else
+					{
+						lo = std::min(lo, iv.first);
+						hi = std::max(hi, iv.second);
+					}
+				}
+				merged.emplace_back(lo, hi);
+				std::sort(merged.begin(), merged.end());
+				coveredBefore.swap(merged);
 			}
+		}
 	}
 }
 
@@ -357,12 +386,15 @@ void applyProposal(const SeedProposal& p, const GraphHost& g, uint64_t maxCount,
 	forEachRead(nreads, threads, [&](size_t i) {
 		if (seeds[i] && !seeds[i]->empty()) ncl[i] = seedClusters(g, *seeds[i], clusterOf[i]);
 	});
-	for (size_t i = 0; i < nreads; i++) rowStart[i + 1] = rowStart[i] + ncl[i];
+	// clusters scored per read: the first maxClusters (0: all)
+	std::vector<uint32_t> scored(nreads);
+	for (size_t i = 0; i < nreads; i++) scored[i] = p.maxClusters ? std::min(ncl[i], p.maxClusters) : ncl[i];
+	for (size_t i = 0; i < nreads; i++) rowStart[i + 1] = rowStart[i] + scored[i];
 	const uint64_t rows = rowStart[nreads];
 	if (rows == 0) return;
 	std::vector<float> feats(rows * kClusterFeatures);
 	forEachRead(nreads, threads, [&](size_t i) {
-		if (ncl[i]) clusterFeatures(g, maxCount, seqs[i], lens[i], *seeds[i], clusterOf[i], ncl[i], feats.data() + rowStart[i] * kClusterFeatures);
+		if (ncl[i]) clusterFeatures(g, maxCount, seqs[i], lens[i], *seeds[i], clusterOf[i], ncl[i], scored[i], feats.data() + rowStart[i] * kClusterFeatures);
 	});
 	std::vector<float> prob(rows * kProposalOutputs);
 	p.scorer->score(feats.data(), rows, prob.data());
@@ -372,7 +404,7 @@ void applyProposal(const SeedProposal& p, const GraphHost& g, uint64_t maxCount,
 		std::vector<uint8_t> keep(ncl[i]);
 		for (uint32_t c = 0; c < ncl[i]; c++)
 		{
-			keep[c] = c < p.keepTop || prob[(rowStart[i] + c) * kProposalOutputs] >= p.threshold;
+			keep[c] = c < p.keepTop || (c < scored[i] && prob[(rowStart[i] + c) * kProposalOutputs] >= p.threshold);
 			prunedClusters[i] += !keep[c];
 		}
 		std::vector<SeedHit>& s = *seeds[i];

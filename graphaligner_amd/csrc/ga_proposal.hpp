@@ -31,8 +31,9 @@ uint32_t seedClusters(const GraphHost& g, const std::vector<SeedHit>& seeds, std
 // covered bases, goodness and rank, diagonal spread, read span, k-mer
 // repetitiveness, split-node in/out degrees and component sizes, local k-mer
 // entropy and homopolymer runs, overlap with better clusters. maxCount is the
-// index's count cut-off (rawSeedGoodness = maxCount - k-mer count).
-void clusterFeatures(const GraphHost& g, uint64_t maxCount, const char* seq, uint32_t len, const std::vector<SeedHit>& seeds, const std::vector<uint32_t>& clusterOf, uint32_t nclusters, float* out);
+// index's count cut-off (rawSeedGoodness = maxCount - k-mer count). Rows are
+// written for clusters [0, rows) (the read's totals still count every cluster).
+void clusterFeatures(const GraphHost& g, uint64_t maxCount, const char* seq, uint32_t len, const std::vector<SeedHit>& seeds, const std::vector<uint32_t>& clusterOf, uint32_t nclusters, uint32_t rows, float* out);
 
 // Scores n feature rows: kProposalOutputs probabilities per row.
 struct ClusterScorer
@@ -45,13 +46,16 @@ struct SeedProposal
 {
 	ClusterScorer* scorer = nullptr;
 	// clusters with output 0 below threshold are removed, except the read's
-	// first keepTop clusters
+	// first keepTop clusters; only the first maxClusters are scored, the ones
+	// after them are removed unscored (0: every cluster is scored)
 	double threshold = 0.5;
 	uint32_t keepTop = 1;
+	uint32_t maxClusters = 0;
 };
 
 struct ProposalStats
 {
+	// clusters scored, clusters removed (scored low or past maxClusters), their seeds
 	uint64_t clusters = 0, prunedClusters = 0, prunedSeeds = 0;
 	double ms = 0;
 };

@@ -368,9 +368,11 @@ void ga_records_free(ga_records_t* records);
 
 /* From now on ga_align_reads and ga_align_resident on ctx remove, before each
  * read's seed loop, the seeds of every cluster whose output 0 is below
- * threshold, except the read's first keepTop (>= 1) clusters. mlp NULL turns
- * the proposal off (the plain method). The network must stay alive while set. */
-ga_status_t ga_context_set_proposal(ga_context_t* ctx, ga_mlp_t* mlp, double threshold, int32_t keepTop);
+ * threshold, except the read's first keepTop (>= 1) clusters. Only a read's
+ * first maxClusters clusters are scored; the ones after them are removed
+ * unscored (maxClusters 0: every cluster is scored). mlp NULL turns the
+ * proposal off (the plain method). The network must stay alive while set. */
+ga_status_t ga_context_set_proposal(ga_context_t* ctx, ga_mlp_t* mlp, double threshold, int32_t keepTop, int32_t maxClusters);
 
 void ga_free(void* p);
 

@@ -625,8 +625,9 @@ void pipe_free(void* p) { std::free(p); }
 
 // proposal with a logistic scorer: w[48], bias; threshold and keepTop as the
 // product's ga_context_set_proposal; on = 0 turns it off
-void pipe_set_linear_proposal(int on, const float* w, float bias, double threshold, int keepTop)
+void pipe_set_linear_proposal(int on, const float* w, float bias, double threshold, int keepTop, int maxClusters)
 {
+	g_proposal.maxClusters = maxClusters < 0 ? 0 : (uint32_t)maxClusters;
 	g_proposalOn = on != 0;
 	if (!g_proposalOn) return;
 	g_scorer.w.assign(w, w + ga::kClusterFeatures);
@@ -689,7 +690,7 @@ int64_t pipe_cluster_features(void* ih, const char* seq, int64_t len, const int6
 	const uint32_t n = ga::seedClusters(*x->g, hits, c);
 	for (int64_t i = 0; i < nseeds; i++) clusterOf[i] = c[i];
 	if ((int64_t)n > cap) return -(int64_t)n;
-	ga::clusterFeatures(*x->g, x->idx.maxCount, seq, (uint32_t)len, hits, c, n, features);
+	ga::clusterFeatures(*x->g, x->idx.maxCount, seq, (uint32_t)len, hits, c, n, n, features);
 	return n;
 }
 }

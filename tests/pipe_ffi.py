@@ -53,7 +53,8 @@ def lib(variant=""):
         L.pipe_align_one_way.argtypes = [ctypes.c_void_p, ctypes.POINTER(PipeOptions), ctypes.c_char_p, ctypes.c_int64,
                                          ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_void_p)]
-        L.pipe_set_linear_proposal.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.c_double, ctypes.c_int]
+        L.pipe_set_linear_proposal.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_int]
         L.pipe_records.argtypes = [ctypes.c_int]
         L.pipe_records_take.restype = ctypes.c_int64
         L.pipe_records_take.argtypes = [ctypes.POINTER(ctypes.c_void_p)] * 3
@@ -162,15 +163,15 @@ def align_one_way(index, oracle_params, seq, seeds, select=False, mapq=False, sp
     return text
 
 
-def set_linear_proposal(weights, bias, threshold, keep_top=1, variant=""):
+def set_linear_proposal(weights, bias, threshold, keep_top=1, variant="", max_clusters=0):
     """Proposal with a logistic scorer over the 48 cluster features (None: off)."""
     L = lib(variant)
     if weights is None:
-        L.pipe_set_linear_proposal(0, None, 0.0, 0.0, 1)
+        L.pipe_set_linear_proposal(0, None, 0.0, 0.0, 1, 0)
         return
     w = np.ascontiguousarray(weights, dtype=np.float32)
     assert w.size == 48
-    L.pipe_set_linear_proposal(1, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), bias, threshold, keep_top)
+    L.pipe_set_linear_proposal(1, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), bias, threshold, keep_top, max_clusters)
 
 
 def records(on, variant=""):

@@ -191,3 +191,4 @@ def test_gpu_proposal_threshold_zero_is_plain_and_records(ctx, workdir):
     params.deviceSeedLoop = 0
     text, _, stats = ga.align_resident(ctx, g, gi, params, 0, names)
     assert text == plain and stats["proposalClusters"] == 0
+

@@ -129,3 +129,24 @@ def test_records_label_the_clusters(setup):
     assert (Y[:, 0] <= Y[:, 1]).all()  # kept implies made
     for i in np.unique(R):
         assert Y[R == i, 0].any() == (st[i] == 0)
+
+
+def test_clusters_past_the_scored_ones_are_removed(setup):
+    """Only a read's first max_clusters clusters are scored; the rest go unscored."""
+    names, seqs, sd, pi = setup
+    want_scored = want_pruned = 0
+    for seq in seqs:
+        seeds = pi.seeds(seq, sd["density"], order=True)
+        if seeds is None or len(seeds) == 0:
+            continue
+        cl, _ = P.cluster_features(pi, seq, seeds)
+        c = int(cl.max()) + 1
+        want_scored += min(c, 1)
+        want_pruned += max(0, c - 1)
+    try:
+        P.set_linear_proposal(np.zeros(48), 0.0, 0.5, 1, max_clusters=1)  # every scored cluster passes
+        P.align_reads(pi, O.preset_params("vg"), sd["density"], names, seqs)
+        s = P.proposal_stats()
+    finally:
+        P.set_linear_proposal(None, 0, 0)
+    assert s["clusters"] == want_scored and s["pruned_clusters"] == want_pruned and want_pruned > 0
