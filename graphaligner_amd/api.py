@@ -85,8 +85,12 @@ class Context:
         _check(N.lib().ga_context_synchronize(self.h), "ga_context_synchronize")
 
 
-def extend_batch(ctx, graph, params, seqs, bigraph_nodes, offsets, summaries=False):
-    """Returns per extension (status, score, trace[n,4] end first, summaries)."""
+def extend_batch(ctx, graph, params, seqs, bigraph_nodes, offsets, summaries=False, schedules=None):
+    """Returns per extension (status, score, trace[n,4] end first, summaries).
+
+    schedules: optional list of per-slice bandwidth sequences, one per extension
+    (slice s uses schedule[min(s, len - 1)]; an empty one keeps params.bandwidth),
+    run through ga_extend_batch_scheduled."""
     n = len(seqs)
     seqs_b = [s.encode() for s in seqs]
     lens = np.array([len(s) for s in seqs], dtype=np.int64)
@@ -103,9 +107,20 @@ def extend_batch(ctx, graph, params, seqs, bigraph_nodes, offsets, summaries=Fal
     nodes = np.asarray(bigraph_nodes, dtype=np.int32)
     offs = np.asarray(offsets, dtype=np.int64)
     P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
-    rc = N.lib().ga_extend_batch(ctx.h, graph.h, ctypes.byref(params), n, arr, P(lens), P(nodes), P(offs), cap, P(traces),
-                                 P(ntrace), P(scores), P(status), sums, scap, P(nsl), ctypes.byref(ms))
-    _check(rc, "ga_extend_batch")
+    if schedules is None:
+        rc = N.lib().ga_extend_batch(ctx.h, graph.h, ctypes.byref(params), n, arr, P(lens), P(nodes), P(offs), cap, P(traces),
+                                     P(ntrace), P(scores), P(status), sums, scap, P(nsl), ctypes.byref(ms))
+        _check(rc, "ga_extend_batch")
+    else:
+        if len(schedules) != n:
+            raise ValueError("one schedule per extension")
+        slens = np.array([len(x) for x in schedules], dtype=np.int64)
+        soffs = np.concatenate([[0], np.cumsum(slens)[:-1]]).astype(np.int64) if n else np.zeros(0, dtype=np.int64)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.int32) for x in schedules] + [np.zeros(1, np.int32)]))
+        rc = N.lib().ga_extend_batch_scheduled(ctx.h, graph.h, ctypes.byref(params), n, arr, P(lens), P(nodes), P(offs), P(flat),
+                                               P(soffs), P(slens), cap, P(traces), P(ntrace), P(scores), P(status), sums, scap,
+                                               P(nsl), ctypes.byref(ms))
+        _check(rc, "ga_extend_batch_scheduled")
     out = []
     for i in range(n):
         t = traces[4 * cap * i: 4 * cap * i + 4 * ntrace[i]].reshape(-1, 4)

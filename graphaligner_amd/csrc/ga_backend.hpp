@@ -24,6 +24,10 @@ struct ExtJob
 	uint32_t startNode;
 	uint32_t startOffset;
 	uint64_t devOffset = 0;
+	// optional per-slice bandwidth schedule (host memory, bandLen entries; slice
+	// s uses bands[min(s, bandLen - 1)]); bandLen 0 uses the fixed bandwidth
+	const int32_t* bands = nullptr;
+	uint32_t bandLen = 0;
 };
 
 // Traces come back in the compact form of ga_trace_codes.h: one code byte per

@@ -187,9 +187,14 @@ ga_status_t ga_context_create(int device, ga_context_t** ctx, char* err, size_t 
 
 void ga_context_free(ga_context_t* ctx) { delete ctx; }
 
-ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, ga_slice_summary_t* summaries, int64_t sumCap, int64_t* nslices, double* kernelMs)
+// largest bandwidth a schedule may give a slice (keeps the band's int32 score
+// arithmetic, minimum + bandwidth, far from overflow)
+constexpr int32_t kMaxScheduledBandwidth = 1 << 20;
+
+static ga_status_t extendBatch(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, const int32_t* schedule, const int64_t* scheduleOffsets, const int64_t* scheduleLens, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, ga_slice_summary_t* summaries, int64_t sumCap, int64_t* nslices, double* kernelMs)
 {
 	if (!ctx || !graph || !params || n < 0) return GA_EBADARG;
+	if (schedule && (!scheduleOffsets || !scheduleLens)) return GA_EBADARG;
 	const GraphHost& g = graph->host;
 	std::vector<ExtJob> jobs(n);
 	for (int64_t i = 0; i < n; i++)
@@ -199,6 +204,15 @@ ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga
 		if (lens[i] <= 0 || lens[i] > (1 << 24) || traceCap < 2 * lens[i] + 256) return GA_EBADARG;
 		uint32_t node = g.unitigNode(bigraphNodeIds[i], offsets[i]);
 		jobs[i] = ExtJob{seqs[i], (uint32_t)lens[i], node, (uint32_t)(offsets[i] - g.nodeOff[node])};
+		if (schedule && scheduleLens[i] > 0)
+		{
+			if (scheduleOffsets[i] < 0 || scheduleLens[i] > (1 << 24)) return GA_EBADARG;
+			const int32_t* b = schedule + scheduleOffsets[i];
+			for (int64_t k = 0; k < scheduleLens[i]; k++)
+				if (b[k] < 0 || b[k] > kMaxScheduledBandwidth) return GA_EBADARG;
+			jobs[i].bands = b;
+			jobs[i].bandLen = (uint32_t)scheduleLens[i];
+		}
 	}
 	try
 	{
@@ -240,6 +254,17 @@ ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga
 	{
 		return GA_ENOMEM;
 	}
+}
+
+ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, ga_slice_summary_t* summaries, int64_t sumCap, int64_t* nslices, double* kernelMs)
+{
+	return extendBatch(ctx, graph, params, n, seqs, lens, bigraphNodeIds, offsets, nullptr, nullptr, nullptr, traceCap, traces, ntrace, scores, status, summaries, sumCap, nslices, kernelMs);
+}
+
+ga_status_t ga_extend_batch_scheduled(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, const int32_t* schedule, const int64_t* scheduleOffsets, const int64_t* scheduleLens, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, ga_slice_summary_t* summaries, int64_t sumCap, int64_t* nslices, double* kernelMs)
+{
+	if (!schedule || !scheduleOffsets || !scheduleLens) return GA_EBADARG;
+	return extendBatch(ctx, graph, params, n, seqs, lens, bigraphNodeIds, offsets, schedule, scheduleOffsets, scheduleLens, traceCap, traces, ntrace, scores, status, summaries, sumCap, nslices, kernelMs);
 }
 
 ga_status_t ga_index_build(const ga_graph_t* graph, const ga_params_t* params, ga_index_t** index, char* err, size_t errcap)

@@ -30,8 +30,10 @@ struct ExtParams
 	int32_t bandwidth;
 	int32_t errorCost;
 	int32_t xdrop;
-	int32_t pad;
+	int32_t pad = 0;
 	int64_t maxCellsPerSlice; // INT64_MAX when unlimited
+	// per-slice bandwidth schedules of a launch's extensions (ExtRequest::bandOff)
+	const int32_t* bandSched = nullptr;
 };
 
 // One seed extension in one direction: align seq[0, len) starting right after
@@ -44,6 +46,10 @@ struct ExtRequest
 	uint32_t startNode;
 	uint32_t startOffset;
 	uint32_t traceCap;
+	// per-slice bandwidth schedule: slice s uses ExtParams::bandSched[bandOff +
+	// min(s, bandLen - 1)]; bandLen 0 uses ExtParams::bandwidth for every slice
+	uint32_t bandOff = 0;
+	uint32_t bandLen = 0;
 };
 
 enum ExtStatus : int32_t

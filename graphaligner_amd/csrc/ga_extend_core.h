@@ -365,6 +365,88 @@ struct LaneAcc
 GA_HD void accInit(LaneAcc& a) { a.v = SMIN; }
 GA_HD void accAdd(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand) { a.v = imax(a.v, lpMaxX(s, seqOffset, cost, cand)); }
 GA_HD int32_t accReduce(const LaneAcc& a) { return a.v; }
+#if defined(GA_THREAD_HOST_TEST)
+GA_HD void accAddAlone(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand) { accAdd(a, s, seqOffset, cost, cand); }
+#else
+// lpMaxX for a wave that runs its extension alone (a one-thread workgroup: the
+// other 63 lanes never execute anything else). The block switches all 64 lanes
+// on, lane r scores row r of the column, X(r) = (seqOffset + r + 1) * 100 -
+// cost * value(r) with value(r) = before + popc(VP & rows 0..r) - popc(VN & rows
+// 0..r), or INT32_MIN when r is not a candidate; a DPP maximum over rows of 16
+// lanes, then across rows (row_bcast), leaves the maximum in lane 63; the lanes
+// are switched back off before the block ends. The scalar loop costs about 16
+// scalar instructions per candidate; this costs a fixed 36 vector ones that the
+// otherwise idle vector unit issues. int32 arithmetic wraps as in lpMaxX.
+GA_HD int32_t lpMaxXAlone(const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand)
+{
+	const uint32_t vplo = (uint32_t)s.VP, vphi = (uint32_t)(s.VP >> 32);
+	const uint32_t vnlo = (uint32_t)s.VN, vnhi = (uint32_t)(s.VN >> 32);
+	const uint32_t clo = (uint32_t)cand, chi = (uint32_t)(cand >> 32);
+	const int32_t bef = before(s);
+	const int32_t base = (seqOffset + 1) * 100;
+	int32_t res;
+	uint64_t saved;
+	uint32_t ln, mlo, mhi, t0, t1, t2;
+	asm volatile(
+		"s_mov_b64 %[sv], exec\n\t"
+		"s_mov_b64 exec, -1\n\t"
+		// lane id, and the mask of rows 0..lane as two 32-bit halves
+		"v_mbcnt_lo_u32_b32 %[ln], -1, 0\n\t"
+		"v_mbcnt_hi_u32_b32 %[ln], -1, %[ln]\n\t"
+		"v_lshlrev_b32_e64 %[t0], %[ln], 2\n\t"
+		"v_add_u32_e32 %[t0], -1, %[t0]\n\t"
+		"v_cmp_gt_u32_e32 vcc, 32, %[ln]\n\t"
+		"v_cndmask_b32_e32 %[mlo], -1, %[t0], vcc\n\t"
+		"v_cndmask_b32_e64 %[mhi], %[t0], 0, vcc\n\t"
+		// value(lane)
+		"v_and_b32_e32 %[t0], %[vplo], %[mlo]\n\t"
+		"v_and_b32_e32 %[t1], %[vphi], %[mhi]\n\t"
+		"v_bcnt_u32_b32 %[t0], %[t0], 0\n\t"
+		"v_bcnt_u32_b32 %[t0], %[t1], %[t0]\n\t"
+		"v_and_b32_e32 %[t1], %[vnlo], %[mlo]\n\t"
+		"v_and_b32_e32 %[t2], %[vnhi], %[mhi]\n\t"
+		"v_bcnt_u32_b32 %[t1], %[t1], 0\n\t"
+		"v_bcnt_u32_b32 %[t1], %[t2], %[t1]\n\t"
+		"v_sub_u32_e32 %[t0], %[t0], %[t1]\n\t"
+		"v_add_u32_e32 %[t0], %[bef], %[t0]\n\t"
+		// X(lane)
+		"v_mul_lo_u32 %[t0], %[t0], %[cost]\n\t"
+		"v_mul_u32_u24_e32 %[t1], 100, %[ln]\n\t"
+		"v_add_u32_e32 %[t1], %[base], %[t1]\n\t"
+		"v_sub_u32_e32 %[t0], %[t1], %[t0]\n\t"
+		// INT32_MIN unless the lane's row is a candidate (vcc still holds lane < 32)
+		"v_lshrrev_b32_e64 %[t1], %[ln], %[clo]\n\t"
+		"v_lshrrev_b32_e64 %[t2], %[ln], %[chi]\n\t"
+		"v_cndmask_b32_e32 %[t1], %[t2], %[t1], vcc\n\t"
+		"v_and_b32_e32 %[t1], 1, %[t1]\n\t"
+		"v_cmp_eq_u32_e32 vcc, 1, %[t1]\n\t"
+		"v_bfrev_b32_e32 %[t2], 1\n\t"
+		"v_cndmask_b32_e32 %[t0], %[t2], %[t0], vcc\n\t"
+		// maximum over the wave into lane 63 (two wait states before each DPP read)
+		"s_nop 1\n\t"
+		"v_max_i32_dpp %[t0], %[t0], %[t0] row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+		"s_nop 1\n\t"
+		"v_max_i32_dpp %[t0], %[t0], %[t0] row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+		"s_nop 1\n\t"
+		"v_max_i32_dpp %[t0], %[t0], %[t0] row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+		"s_nop 1\n\t"
+		"v_max_i32_dpp %[t0], %[t0], %[t0] row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+		"s_nop 1\n\t"
+		"v_max_i32_dpp %[t0], %[t0], %[t0] row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+		"s_nop 1\n\t"
+		"v_max_i32_dpp %[t0], %[t0], %[t0] row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+		"s_nop 1\n\t"
+		"v_readlane_b32 %[res], %[t0], 63\n\t"
+		"s_mov_b64 exec, %[sv]"
+		: [res] "=s"(res), [sv] "=&s"(saved), [ln] "=&v"(ln), [mlo] "=&v"(mlo), [mhi] "=&v"(mhi), [t0] "=&v"(t0), [t1] "=&v"(t1),
+		  [t2] "=&v"(t2)
+		: [vplo] "s"(vplo), [vphi] "s"(vphi), [vnlo] "s"(vnlo), [vnhi] "s"(vnhi), [clo] "s"(clo), [chi] "s"(chi), [bef] "s"(bef),
+		  [cost] "s"(cost), [base] "s"(base)
+		: "vcc");
+	return res;
+}
+GA_HD void accAddAlone(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand) { a.v = imax(a.v, lpMaxXAlone(s, seqOffset, cost, cand)); }
+#endif
 GA_HD int32_t lpFind(const uint32_t* arr, uint32_t n, uint32_t key)
 {
 	for (uint32_t i = 0; i < n; i++)
@@ -589,6 +671,7 @@ GA_HD void accAdd(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, u
 	if ((cand >> l) & 1) a.v = imax(a.v, (seqOffset + l + 1) * 100 - laneValue(s, before(s), prefixMask(l)) * cost);
 }
 GA_HD int32_t accReduce(const LaneAcc& a) { return waveMaxI(a.v); }
+GA_HD void accAddAlone(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand) { accAdd(a, s, seqOffset, cost, cand); }
 
 // index of key in arr[0, n), or -1
 GA_HD int32_t lpFind(const uint32_t* arr, uint32_t n, uint32_t key)
@@ -772,6 +855,7 @@ inline int32_t accReduce(const LaneAcc& a)
 	for (int l = 0; l < 64; l++) r = imax(r, a.v[l]);
 	return r;
 }
+inline void accAddAlone(LaneAcc& a, const Slice& s, int32_t seqOffset, int32_t cost, uint64_t cand) { accAdd(a, s, seqOffset, cost, cand); }
 inline int32_t lpFind(const uint32_t* arr, uint32_t n, uint32_t key)
 {
 	for (uint32_t i = 0; i < n; i++)
@@ -880,6 +964,14 @@ struct Extension
 	uint32_t len;
 	uint32_t startNode;
 	uint32_t startOffset;
+	// per-slice bandwidth schedule (getXdropSlices picks a bandwidth per slice,
+	// src/GraphAlignerBitvectorBanded.h:611): slice s uses bands[min(s, bandLen - 1)];
+	// bandLen 0 uses p.bandwidth
+	const int32_t* bands = nullptr;
+	uint32_t bandLen = 0;
+	// the wave runs this extension alone (a one-thread workgroup): its other
+	// lanes may be switched on for whole-wave steps (lpMaxXAlone)
+	bool aloneInWave = false;
 	// match-mask planes of the buffer seq points into (planeBlock) and seq's
 	// offset there; without planes the masks are read from seq byte by byte
 	const uint64_t* planes;
@@ -1256,8 +1348,10 @@ struct Extension
 		fixedHN = uni64(fixedHN);
 		forceEq = uni64(forceEq);
 		L = uni32(L);
+		// row 0 never matches in a node without a previous-slice item (forceEq):
+		// applied to the four base masks once instead of to every column's mask
 		Eq4 eqU;
-		for (int b = 0; b < 4; b++) eqU.m[b] = uni64(eq.m[b]);
+		for (int b = 0; b < 4; b++) eqU.m[b] = uni64(eq.m[b]) & forceEq;
 		NodeSeq nseqU = nseq;
 		for (int b = 0; b < 4; b++) nseqU.w[b] = uni64(nseq.w[b]);
 		LaneAcc acc;
@@ -1274,16 +1368,15 @@ struct Extension
 		}
 		// a recalculation may stop after the columns the backtrace will read
 		const uint32_t Lc = Recalc && recalcCols < L ? recalcCols : L;
-		for (uint32_t pos = 1; pos < Lc; pos++)
-		{
-			uint64_t Eq = eqOf(eqU, nseqU, pos) & forceEq;
+		// column pos (>= 1) of the node from its match mask
+		auto column = [&](uint32_t pos, uint64_t Eq) {
 			uint64_t Ph, Mh;
 			Slice nw = nextSlice(Eq, ws, (fixedHP >> pos) & 1, (fixedHN >> pos) & 1, Ph, Mh);
-			if ((forceMask >> pos) & 1)
-			{
-				nw.VP &= ONES ^ 1;
-				nw.VN |= 1;
-			}
+			// row 0 forced to a vertical -1 where forceMask says so, with masks (a
+			// branch on the bit compiles into a round trip through a vector register)
+			const uint64_t f = (forceMask >> pos) & 1;
+			nw.VP &= ~f;
+			nw.VN |= f;
 			ws = nw;
 			if (ws.end < res.minScore)
 			{
@@ -1296,9 +1389,35 @@ struct Extension
 				colSetStep(*cols, pos, Ph, Mh, Eq);
 			}
 			else if (xTop - p.errorCost * ws.end + xRise * popc(ws.VP) > xSkip)
-				accAdd(acc, ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, Mh));
+			{
+				if (aloneInWave) accAddAlone(acc, ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, Mh));
+				else accAdd(acc, ws, seqOffset, p.errorCost, maxXCandidates(ws, 64, Mh));
+			}
 			HP |= (Ph >> 63) << pos;
 			HN |= (Mh >> 63) << pos;
+		};
+		if (!nseqU.amb)
+		{
+			// the 2-bit bases as a shift register (column pos's code in the low two
+			// bits), one loop per 32-base chunk: no per-column choice of the chunk
+			// and no per-column test for IUPAC nodes
+			uint64_t w = nseqU.w[0] >> 2;
+			const uint32_t mid = Lc < 32 ? Lc : 32;
+			for (uint32_t pos = 1; pos < mid; pos++)
+			{
+				column(pos, eqSelect(eqU, (uint32_t)w & 3));
+				w >>= 2;
+			}
+			w = nseqU.w[1];
+			for (uint32_t pos = 32; pos < Lc; pos++)
+			{
+				column(pos, eqSelect(eqU, (uint32_t)w & 3));
+				w >>= 2;
+			}
+		}
+		else
+		{
+			for (uint32_t pos = 1; pos < Lc; pos++) column(pos, eqOf(eqU, nseqU, pos));
 		}
 		if (!Recalc)
 		{
@@ -1454,7 +1573,6 @@ struct Extension
 		buildSliceTable(0);
 		if (status != EXT_OK) return;
 		int32_t bestX = 0;
-		const int32_t b = p.bandwidth;
 		uint32_t numSlices = (len + 63) / 64;
 		if (numSlices + 1 > caps.maxSlices)
 		{
@@ -1464,6 +1582,7 @@ struct Extension
 		for (uint32_t s = 0; s < numSlices && status == EXT_OK; s++)
 		{
 			const int32_t j = (int32_t)(s * 64);
+			const int32_t b = bandLen == 0 ? p.bandwidth : bands[s < bandLen ? s : bandLen - 1];
 			const SliceHdr P = hdr[kept - 1];
 			const int32_t prevQuit = P.minScore + P.bandwidth;
 			Eq4 eq;
@@ -2389,6 +2508,8 @@ GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r
 		e.colEnd = (int32_t*)(colStore + 320);
 	}
 	e.colsInLds = colStoreLds;
+	// only one-thread workgroups keep the column store in LDS
+	e.aloneInWave = colStoreLds;
 	e.g = g;
 	e.p = p;
 	e.seq = seqbuf + r.seqOffset;
@@ -2397,6 +2518,8 @@ GA_HD void runRequest(const DevGraph& g, const ExtParams& p, const ExtRequest& r
 	e.len = r.len;
 	e.startNode = r.startNode;
 	e.startOffset = r.startOffset;
+	e.bandLen = r.bandLen;
+	e.bands = r.bandLen ? p.bandSched + r.bandOff : nullptr;
 	e.trace = tracebuf + r.traceOffset;
 	e.traceCap = r.traceCap;
 	e.summaries = summaries;

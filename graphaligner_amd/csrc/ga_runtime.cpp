@@ -338,6 +338,7 @@ void Context::runLaunch(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtPa
 	if (n == 0) return;
 	std::vector<gad::ExtRequest> reqs(n);
 	std::vector<uint64_t> arenaOff(n), sumOff(n);
+	std::vector<int32_t> bandHost;
 	uint64_t seqBytes = 0, traceItems = 0, sumItems = 0, arenaTotal = 0;
 	for (uint32_t k = 0; k < n; k++)
 	{
@@ -348,6 +349,14 @@ void Context::runLaunch(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtPa
 		reqs[k].startNode = j.startNode;
 		reqs[k].startOffset = j.startOffset;
 		reqs[k].traceCap = traceCapFor(j.len);
+		if (j.bandLen > 0)
+		{
+			// a schedule longer than the extension's slices is cut to them
+			const uint32_t used = std::min<uint32_t>(j.bandLen, j.len / 64 + 1);
+			reqs[k].bandOff = (uint32_t)bandHost.size();
+			reqs[k].bandLen = used;
+			bandHost.insert(bandHost.end(), j.bands, j.bands + used);
+		}
 		arenaOff[k] = arenaTotal;
 		sumOff[k] = sumItems;
 		seqBytes += j.len;
@@ -375,6 +384,13 @@ void Context::runLaunch(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtPa
 	}
 	GA_HIP(hipMemcpyAsync(L.arenaOffsets.ptr, arenaOff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, L.stream));
 	GA_HIP(hipMemcpyAsync(L.reqs.ptr, reqs.data(), n * sizeof(gad::ExtRequest), hipMemcpyHostToDevice, L.stream));
+	gad::ExtParams lp = p;
+	if (!bandHost.empty())
+	{
+		L.bands.reserve(bandHost.size() * sizeof(int32_t));
+		GA_HIP(hipMemcpyAsync(L.bands.ptr, bandHost.data(), bandHost.size() * sizeof(int32_t), hipMemcpyHostToDevice, L.stream));
+		lp.bandSched = L.bands.as<int32_t>();
+	}
 	const uint64_t* planes = devPlanes;
 	if (!devSeqBase)
 	{
@@ -385,7 +401,7 @@ void Context::runLaunch(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtPa
 	}
 	if (!planes) throw std::runtime_error("extension launch without match-mask planes");
 	GA_HIP(hipEventRecord(L.ev0, L.stream));
-	GA_HIP(gad::launchExtend(L.stream, n, lanesPerWave(n), kernelWavesPerSimd, dg, p, L.reqs.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : L.seqs.as<char>(), L.traces.as<uint64_t>(), L.results.as<gad::ExtResult>(), L.arena.as<unsigned char>(), L.arenaOffsets.as<uint64_t>(), caps, wantSummaries ? L.sums.as<gad::SliceSummary>() : nullptr, wantSummaries ? L.sumOffsets.as<uint64_t>() : nullptr, planes, L.capWaves));
+	GA_HIP(gad::launchExtend(L.stream, n, lanesPerWave(n), kernelWavesPerSimd, dg, lp, L.reqs.as<gad::ExtRequest>(), devSeqBase ? devSeqBase : L.seqs.as<char>(), L.traces.as<uint64_t>(), L.results.as<gad::ExtResult>(), L.arena.as<unsigned char>(), L.arenaOffsets.as<uint64_t>(), caps, wantSummaries ? L.sums.as<gad::SliceSummary>() : nullptr, wantSummaries ? L.sumOffsets.as<uint64_t>() : nullptr, planes, L.capWaves));
 	GA_HIP(hipEventRecord(L.ev1, L.stream));
 	std::vector<gad::ExtResult> res(n);
 	GA_HIP(hipMemcpyAsync(res.data(), L.results.ptr, n * sizeof(gad::ExtResult), hipMemcpyDeviceToHost, L.stream));

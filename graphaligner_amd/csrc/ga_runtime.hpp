@@ -64,7 +64,7 @@ struct ExtendLane
 	uint32_t capWaves = gad::kSingleLaneWavesPerSimd;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	DeviceBuffer arena, arenaOffsets, reqs, seqs, traces, results, sums, sumOffsets;
-	DeviceBuffer gatherSrc, gatherDst, gatherCnt, traceCodes, traceEscapes, escapeCount, planes;
+	DeviceBuffer gatherSrc, gatherDst, gatherCnt, traceCodes, traceEscapes, escapeCount, planes, bands;
 	PinnedBuffer hostCodes; // compact traces of the lane's current extend call
 	uint64_t hostCodesUsed = 0;
 	ExtendLane() = default;

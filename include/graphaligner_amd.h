@@ -121,6 +121,16 @@ typedef struct
  * take sumCap rows per extension. */
 ga_status_t ga_extend_batch(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, ga_slice_summary_t* summaries, int64_t sumCap, int64_t* nslices, double* kernelMs);
 
+/* ga_extend_batch with a per-slice bandwidth schedule: the hook through which
+ * getXdropSlices picks each slice's bandwidth (src/GraphAlignerBitvectorBanded.h:611,
+ * kept in DPSlice.bandwidth, :552, and read back by the traceback's quit scores,
+ * src/GraphAlignerBitvectorCommon.h:633,659,684). Slice s of extension i (its
+ * rows 64 s .. 64 s + 63) uses schedule[scheduleOffsets[i] + min(s, scheduleLens[i] - 1)];
+ * scheduleLens[i] == 0 uses params->bandwidth for all of extension i's slices.
+ * Bandwidths must lie in 0 .. 2^20 (else GA_EBADARG). A schedule whose entries
+ * all equal params->bandwidth gives exactly ga_extend_batch's results. */
+ga_status_t ga_extend_batch_scheduled(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, int64_t n, const char* const* seqs, const int64_t* lens, const int32_t* bigraphNodeIds, const int64_t* offsets, const int32_t* schedule, const int64_t* scheduleOffsets, const int64_t* scheduleLens, int64_t traceCap, int64_t* traces, int64_t* ntrace, int32_t* scores, int32_t* status, ga_slice_summary_t* summaries, int64_t sumCap, int64_t* nslices, double* kernelMs);
+
 /* No counterpart in the reference: ga_extend_batch for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid allocation). Same extensions, trace rows, scores and

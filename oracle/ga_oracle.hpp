@@ -1321,6 +1321,17 @@ struct Params
 	double multimapScoreFraction = 0.9;
 	// --min-alignment-score (src/AlignerMain.cpp:151,213), as the double option
 	double minAlignmentScore = 0;
+	// per-slice bandwidth schedule: the value getXdropSlices picks for each slice
+	// (src/GraphAlignerBitvectorBanded.h:611, stored in DPSlice.bandwidth at :552
+	// and read back by the traceback's quit scores,
+	// src/GraphAlignerBitvectorCommon.h:633,659,684). Slice s of an extension uses
+	// bandwidthSchedule[min(s, size - 1)]; empty means alignmentBandwidth for all.
+	std::vector<int> bandwidthSchedule;
+	int bandwidthForSlice(size_t slice) const
+	{
+		if (bandwidthSchedule.empty()) return (int)alignmentBandwidth;
+		return bandwidthSchedule[std::min(slice, bandwidthSchedule.size() - 1)];
+	}
 	// src/GraphAlignerCommon.h:89-100
 	static ScoreType errorCostFromIdentity(double preciseClippingIdentityCutoff)
 	{
@@ -2037,7 +2048,7 @@ struct Extender
 		ComponentQueue queue;
 		for (size_t slice = 0; slice < numSlices; slice++)
 		{
-			int bandwidth = params.alignmentBandwidth;
+			int bandwidth = params.bandwidthForSlice(slice);
 			const DPSlice& lastSlice = result.back();
 			DPSlice newSlice;
 			newSlice.j = lastSlice.j + WordSize;

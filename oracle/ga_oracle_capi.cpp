@@ -175,9 +175,19 @@ int orc_get_seeds(void* g, void* idx, const char* seq, int64_t len, double densi
 // One extension in one direction (getReverseTraceFromSeed). Returns 0 on success,
 // 1 if the trace failed (no slice kept), 2 on a reference assertion, 3 if the
 // output buffers are too small. Trace rows: node, nodeOffset, seqPos, nodeSwitch.
+int orc_extend_scheduled(void* g, const OrcParams* p, const char* seq, int64_t len, int bigraphNodeId, int64_t offset, const int32_t* schedule, int64_t scheduleLen, int64_t* traceOut, int64_t cap, int64_t* ntrace, int32_t* score, SliceSummary* summaries, int64_t scap, int64_t* nslices);
+
 int orc_extend(void* g, const OrcParams* p, const char* seq, int64_t len, int bigraphNodeId, int64_t offset, int64_t* traceOut, int64_t cap, int64_t* ntrace, int32_t* score, SliceSummary* summaries, int64_t scap, int64_t* nslices)
 {
+	return orc_extend_scheduled(g, p, seq, len, bigraphNodeId, offset, nullptr, 0, traceOut, cap, ntrace, score, summaries, scap, nslices);
+}
+
+// orc_extend with a per-slice bandwidth schedule: slice s uses
+// schedule[min(s, scheduleLen - 1)] (scheduleLen 0: the fixed bandwidth).
+int orc_extend_scheduled(void* g, const OrcParams* p, const char* seq, int64_t len, int bigraphNodeId, int64_t offset, const int32_t* schedule, int64_t scheduleLen, int64_t* traceOut, int64_t cap, int64_t* ntrace, int32_t* score, SliceSummary* summaries, int64_t scap, int64_t* nslices)
+{
 	Params params = toParams(p);
+	if (schedule && scheduleLen > 0) params.bandwidthSchedule.assign(schedule, schedule + scheduleLen);
 	Extender ext(params, *(AlignmentGraph*)g);
 	std::string_view view(seq, len);
 	std::vector<SliceSummary> sums;

@@ -31,7 +31,7 @@ void* emu_graph_load(const char* path, char* err, int cap)
 void emu_graph_free(void* g) { delete (ga::GraphHost*)g; }
 
 // rows: node, nodeOffset, seqPos, nodeSwitch (same layout as the oracle's orc_extend)
-int emu_extend(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int32_t xdrop, const char* seq, int64_t len, int bigraphNodeId, int64_t offset, int64_t* traceOut, int64_t cap, int64_t* ntrace, int32_t* score, gad::SliceSummary* summaries, int64_t scap, int64_t* nslices, uint64_t* wordSteps)
+int emu_extend(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int32_t xdrop, const char* seq, int64_t len, int bigraphNodeId, int64_t offset, int64_t* traceOut, int64_t cap, int64_t* ntrace, int32_t* score, gad::SliceSummary* summaries, int64_t scap, int64_t* nslices, uint64_t* wordSteps, const int32_t* schedule, int64_t scheduleLen)
 {
 	auto& G = *(ga::GraphHost*)gh;
 	gad::DevGraph g;
@@ -52,6 +52,7 @@ int emu_extend(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int3
 	p.xdrop = xdrop;
 	p.pad = 0;
 	p.maxCellsPerSlice = maxCells <= 0 ? INT64_MAX : maxCells;
+	p.bandSched = schedule;
 	gad::ArenaCaps caps;
 	caps.maxSlices = len / 64 + 3;
 	caps.maxItems = 200000;
@@ -69,6 +70,8 @@ int emu_extend(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int3
 	r.startNode = node;
 	r.startOffset = offset - G.nodeOff[node];
 	r.traceCap = cap;
+	r.bandOff = 0;
+	r.bandLen = schedule ? (uint32_t)scheduleLen : 0;
 	std::vector<uint64_t> trace(cap + 1);
 	std::vector<gad::SliceSummary> sums(caps.maxSlices);
 	gad::ExtResult out;

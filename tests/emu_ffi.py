@@ -22,7 +22,7 @@ def lib(mode="wave"):
         L.emu_extend.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p,
                                  ctypes.c_int64, ctypes.c_int, ctypes.c_int64, i64p, ctypes.c_int64, i64p,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(SliceSummary), ctypes.c_int64, i64p,
-                                 ctypes.POINTER(ctypes.c_uint64)]
+                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
         u64 = ctypes.c_uint64
         L.emu_merge.argtypes = [u64, u64, ctypes.c_int32, u64, u64, ctypes.c_int32, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int32)]
         L.emu_less_mask.restype = u64
@@ -31,6 +31,14 @@ def lib(mode="wave"):
         L.emu_max_x.argtypes = [u64, u64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, u64]
         _loaded[mode] = L
     return _loaded[mode]
+
+
+def _schedule_args(schedule):
+    if schedule is None or len(schedule) == 0:
+        return None, 0
+    sch = np.ascontiguousarray(schedule, dtype=np.int32)
+    _schedule_args.keep = sch
+    return sch.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(sch)
 
 
 class Graph:
@@ -47,7 +55,7 @@ class Graph:
             self.h = None
 
 
-def extend(graph, params, seq, bigraph_node, offset):
+def extend(graph, params, seq, bigraph_node, offset, schedule=None):
     cap = 4 * len(seq) + 1024
     trace = np.zeros(4 * cap, dtype=np.int64)
     nt = ctypes.c_int64()
@@ -58,7 +66,7 @@ def extend(graph, params, seq, bigraph_node, offset):
     ws = ctypes.c_uint64()
     rc = lib(graph.mode).emu_extend(graph.h, params.bandwidth, params.maxCellsPerSlice, params.errorCost, params.xdrop, seq.encode(), len(seq),
                           bigraph_node, offset, trace.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(nt),
-                          ctypes.byref(score), sums, scap, ctypes.byref(ns), ctypes.byref(ws))
+                          ctypes.byref(score), sums, scap, ctypes.byref(ns), ctypes.byref(ws), *_schedule_args(schedule))
     rows = [(s.j, s.minScore, s.minScoreNode, s.minScoreNodeOffset, s.maxExactEndposScore, s.maxExactEndposNode,
              s.cellsProcessed, s.numNodes, s.itemsHash, s.scoresNotValid) for s in sums[: ns.value]]
     return rc, score.value, trace[: 4 * nt.value].reshape(-1, 4), rows

@@ -80,6 +80,9 @@ def lib(ndebug=False):
         L.orc_get_seeds.argtypes = [vp, vp, ctypes.c_char_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int, i64p, ctypes.c_int64, i64p]
         L.orc_extend.argtypes = [vp, ctypes.POINTER(OrcParams), ctypes.c_char_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int64,
                                  i64p, ctypes.c_int64, i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(SliceSummary), ctypes.c_int64, i64p]
+        L.orc_extend_scheduled.argtypes = [vp, ctypes.POINTER(OrcParams), ctypes.c_char_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int64,
+                                           ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
+                                           i64p, ctypes.c_int64, i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(SliceSummary), ctypes.c_int64, i64p]
         L.orc_align_reads.restype = ctypes.c_double
         L.orc_align_reads.argtypes = [vp, vp, ctypes.POINTER(OrcParams), ctypes.c_double, ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), i64p, ctypes.c_int,
@@ -197,8 +200,10 @@ class Index:
         return out[: 10 * n.value].reshape(-1, 10)
 
 
-def extend(graph, params, seq, bigraph_node, offset):
-    """getReverseTraceFromSeed: returns (rc, score, trace[n,4], summaries list)."""
+def extend(graph, params, seq, bigraph_node, offset, schedule=None):
+    """getReverseTraceFromSeed: returns (rc, score, trace[n,4], summaries list).
+
+    schedule: optional per-slice bandwidths (slice s uses schedule[min(s, len-1)])."""
     cap = 4 * len(seq) + 1024
     trace = np.zeros(4 * cap, dtype=np.int64)
     nt = ctypes.c_int64()
@@ -206,9 +211,15 @@ def extend(graph, params, seq, bigraph_node, offset):
     scap = len(seq) // 64 + 4
     sums = (SliceSummary * scap)()
     ns = ctypes.c_int64()
-    rc = graph.L.orc_extend(graph.h, ctypes.byref(params), seq.encode(), len(seq), bigraph_node, offset,
-                          trace.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(nt), ctypes.byref(score),
-                          sums, scap, ctypes.byref(ns))
+    tp = trace.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    if schedule is None or len(schedule) == 0:
+        rc = graph.L.orc_extend(graph.h, ctypes.byref(params), seq.encode(), len(seq), bigraph_node, offset,
+                              tp, cap, ctypes.byref(nt), ctypes.byref(score), sums, scap, ctypes.byref(ns))
+    else:
+        sch = np.ascontiguousarray(schedule, dtype=np.int32)
+        rc = graph.L.orc_extend_scheduled(graph.h, ctypes.byref(params), seq.encode(), len(seq), bigraph_node, offset,
+                                        sch.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(sch),
+                                        tp, cap, ctypes.byref(nt), ctypes.byref(score), sums, scap, ctypes.byref(ns))
     rows = [(s.j, s.minScore, s.minScoreNode, s.minScoreNodeOffset, s.maxExactEndposScore, s.maxExactEndposNode,
              s.cellsProcessed, s.numNodes, s.itemsHash, s.scoresNotValid) for s in sums[: ns.value]]
     return rc, score.value, trace[: 4 * nt.value].reshape(-1, 4), rows
