@@ -18,7 +18,8 @@ STATUS_NAMES = {0: "GA_OK", 1: "GA_EXT_FAILED", 2: "GA_ASSERT", 3: "GA_EOVERFLOW
 EXPORTED = ["ga_params_preset", "ga_graph_load_gfa", "ga_graph_free", "ga_graph_num_nodes", "ga_graph_num_edges",
             "ga_graph_first_ambiguous", "ga_graph_export", "ga_context_create", "ga_context_free", "ga_extend_batch",
             "ga_extend_batch_scheduled",
-            "ga_index_build", "ga_index_free", "ga_index_max_count", "ga_index_num_kmers", "ga_seeds",
+            "ga_index_build", "ga_index_build_gpu", "ga_index_export", "ga_index_num_positions", "ga_index_free", "ga_index_max_count",
+            "ga_index_num_kmers", "ga_seeds",
             "ga_align_reads", "ga_reads_upload", "ga_align_resident", "ga_context_synchronize", "ga_free",
             "ga_order_seeds", "ga_align_one_way", "ga_result_reads", "ga_result_status", "ga_result_seeds_extended",
             "ga_result_count", "ga_result_item", "ga_result_trace", "ga_select_alignments", "ga_add_mapping_qualities",
@@ -110,6 +111,10 @@ def lib():
                                                 ctypes.POINTER(ctypes.c_double)]
         L.ga_index_build.argtypes = [vp, ctypes.POINTER(Params), ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
         L.ga_index_free.argtypes = [vp]
+        L.ga_index_build_gpu.argtypes = [vp, vp, ctypes.POINTER(Params), ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+        L.ga_index_export.argtypes = [vp, vp, vp, vp, vp]
+        L.ga_index_num_positions.restype = i64
+        L.ga_index_num_positions.argtypes = [vp]
         L.ga_index_max_count.restype = i64
         L.ga_index_max_count.argtypes = [vp]
         L.ga_index_num_kmers.restype = i64

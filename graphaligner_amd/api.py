@@ -157,14 +157,30 @@ def extend_batch_noalloc(ctx, graph, params, seqs, bigraph_nodes, offsets):
 
 
 class Index:
-    """Minimizer index (MinimizerSeeder) built from params' minimizer settings."""
+    """Minimizer index (MinimizerSeeder) built from params' minimizer settings,
+    on the host, or on the context's GPU when ctx is given (ga_index_build_gpu)."""
 
-    def __init__(self, graph, params):
+    def __init__(self, graph, params, ctx=None):
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(1024)
-        _check(N.lib().ga_index_build(graph.h, ctypes.byref(params), ctypes.byref(h), err, 1024), "ga_index_build", err)
+        if ctx is None:
+            _check(N.lib().ga_index_build(graph.h, ctypes.byref(params), ctypes.byref(h), err, 1024), "ga_index_build", err)
+        else:
+            _check(N.lib().ga_index_build_gpu(ctx.h, graph.h, ctypes.byref(params), ctypes.byref(h), err, 1024), "ga_index_build_gpu", err)
         self.h = h
         self.graph = graph
+
+    def export(self):
+        """(k-mers ascending, starts, counts, positions) as numpy arrays."""
+        n = self.num_kmers
+        m = N.lib().ga_index_num_positions(self.h)
+        kmers = np.zeros(max(n, 1), dtype=np.uint64)
+        starts = np.zeros(max(n, 1), dtype=np.int64)
+        counts = np.zeros(max(n, 1), dtype=np.int64)
+        pos = np.zeros(max(m, 1), dtype=np.uint64)
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        _check(N.lib().ga_index_export(self.h, P(kmers), P(starts), P(counts), P(pos)), "ga_index_export")
+        return kmers[:n], starts[:n], counts[:n], pos[:m]
 
     def __del__(self):
         if getattr(self, "h", None):

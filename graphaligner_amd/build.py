@@ -15,7 +15,7 @@ OUT = os.path.join(PKG, "_build")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-variable", "-Wno-sign-compare", "-Wno-unused-function"]
-PRODUCT_SOURCES = ["ga_kernels.hip", "ga_mlp.hip", "ga_runtime.cpp", "ga_graph.cpp", "ga_align.cpp", "ga_proposal.cpp", "ga_index.cpp",
+PRODUCT_SOURCES = ["ga_kernels.hip", "ga_mlp.hip", "ga_index_gpu.hip", "ga_runtime.cpp", "ga_graph.cpp", "ga_align.cpp", "ga_proposal.cpp", "ga_index.cpp",
                    "ga_capi.cpp"]
 
 

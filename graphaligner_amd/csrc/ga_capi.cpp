@@ -294,6 +294,59 @@ ga_status_t ga_index_build(const ga_graph_t* graph, const ga_params_t* params, g
 	}
 }
 
+ga_status_t ga_index_build_gpu(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, ga_index_t** index, char* err, size_t errcap)
+{
+	if (!ctx || !graph || !params || !index) return GA_EBADARG;
+	*index = nullptr;
+	if (params->minimizerLength < 1 || params->minimizerLength > 31 || params->minimizerWindowSize < params->minimizerLength)
+	{
+		copyErr(err, errcap, "minimizer length must be 1..31 and not above the window size");
+		return GA_EBADARG;
+	}
+	try
+	{
+		auto* x = new ga_index{buildMinimizerIndexGpu(*ctx->ctx, graph->host, params->minimizerLength, params->minimizerWindowSize, params->minimizerDiscardMostNumerousFraction), graph};
+		*index = x;
+		return GA_OK;
+	}
+	catch (const HipError& e)
+	{
+		copyErr(err, errcap, e.what());
+		return GA_EHIP;
+	}
+	catch (const std::bad_alloc&)
+	{
+		copyErr(err, errcap, "out of host memory");
+		return GA_ENOMEM;
+	}
+	catch (const std::exception& e)
+	{
+		copyErr(err, errcap, e.what());
+		return GA_EBADARG;
+	}
+}
+
+ga_status_t ga_index_export(const ga_index_t* index, uint64_t* kmers, int64_t* starts, int64_t* counts, uint64_t* positions)
+{
+	if (!index) return GA_EBADARG;
+	const ga::MinimizerIndex& x = index->idx;
+	std::vector<const ga::KmerSlot*> used;
+	used.reserve(x.numKmers);
+	for (const auto& slot : x.table)
+		if (slot.kmer != ~0ull) used.push_back(&slot);
+	std::sort(used.begin(), used.end(), [](const ga::KmerSlot* a, const ga::KmerSlot* b) { return a->kmer < b->kmer; });
+	for (size_t i = 0; i < used.size(); i++)
+	{
+		if (kmers) kmers[i] = used[i]->kmer;
+		if (starts) starts[i] = used[i]->start;
+		if (counts) counts[i] = used[i]->count;
+	}
+	if (positions) std::copy(x.positions.begin(), x.positions.end(), positions);
+	return GA_OK;
+}
+
+int64_t ga_index_num_positions(const ga_index_t* index) { return index ? (int64_t)index->idx.positions.size() : -1; }
+
 void ga_index_free(ga_index_t* index) { delete index; }
 
 int64_t ga_index_max_count(const ga_index_t* index) { return index ? (int64_t)index->idx.maxCount : -1; }

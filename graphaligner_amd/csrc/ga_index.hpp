@@ -32,4 +32,9 @@ struct MinimizerIndex
 // src/MinimizerSeeder.cpp:299-491,557-575 (see ga_index.cpp for the maxCount rule)
 MinimizerIndex buildMinimizerIndex(const GraphHost& g, uint32_t k, uint32_t w, double discardFraction);
 
+class Context;
+// The same index built on the context's GPU (ga_index_gpu.hip): identical
+// k-mers, ranges, position order and maxCount; table slots may differ.
+MinimizerIndex buildMinimizerIndexGpu(Context& ctx, const GraphHost& g, uint32_t k, uint32_t w, double discardFraction);
+
 } // namespace ga

@@ -155,9 +155,22 @@ ga_status_t ga_extend_batch_noalloc(ga_context_t* ctx, const ga_graph_t* graph, 
  * params->minimizerDiscardMostNumerousFraction of k-mers left out of seeding.
  * Built on the host; the table is copied to a device on its first use there. */
 ga_status_t ga_index_build(const ga_graph_t* graph, const ga_params_t* params, ga_index_t** index, char* err, size_t errcap);
+/* The same index built on the context's GPU (no reference counterpart of the
+ * device build; same constructor and initMaxCount semantics): every position of
+ * every bigraph node decides in parallel whether the reference's window walk
+ * (iterateMinimizersReal, src/MinimizerSeeder.cpp:104-189) reports it, then the
+ * reports are sorted and grouped on the device. Gives ga_index_build's k-mers,
+ * position ranges, position order and maxCount (hash-table slots may differ). */
+ga_status_t ga_index_build_gpu(ga_context_t* ctx, const ga_graph_t* graph, const ga_params_t* params, ga_index_t** index, char* err, size_t errcap);
 void ga_index_free(ga_index_t* index);
 int64_t ga_index_max_count(const ga_index_t* index);
 int64_t ga_index_num_kmers(const ga_index_t* index);
+int64_t ga_index_num_positions(const ga_index_t* index);
+/* The index's contents for checks: its ga_index_num_kmers k-mers in increasing
+ * order with their position ranges (start, count into positions), and the
+ * ga_index_num_positions positions (split node << 6 | offset; one k-mer's in the
+ * reference's order, src/MinimizerSeeder.cpp:470-480). Any pointer may be NULL. */
+ga_status_t ga_index_export(const ga_index_t* index, uint64_t* kmers, int64_t* starts, int64_t* counts, uint64_t* positions);
 
 /* Seeds of n reads: MinimizerSeeder::getSeeds (src/MinimizerSeeder.cpp:521-555)
  * with the k-mer lookup on the GPU and, when order != 0,
