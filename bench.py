@@ -90,6 +90,8 @@ def parse():
     ap.add_argument("--seed-loop", choices=["auto", "rounds", "device", "batched"], default="auto",
                     help="rounds: host-driven rounds of batched extensions; device: each read's whole seed loop on the GPU; "
                          "batched: the persistent form (auto: batched for the stream, rounds otherwise)")
+    ap.add_argument("--seed-order", choices=["host", "device"], default="host",
+                    help="seed expansion and OrderSeeds on host threads or on the GPU (same seeds)")
     ap.add_argument("--learned", action="store_true",
                     help="the learned proposal: train the network on outcome records of the plain method (fresh seeds of "
                          "the 5 Mbp variation-graph generator), then time plain and learned on --workload (vg5m or dbg2m)")
@@ -146,6 +148,7 @@ def main():
     params = ga.params_preset("vg")
     params.minimizerLength, params.minimizerWindowSize = wl["k"], wl["w"]
     params.deviceSeedLoop = {"rounds": 0, "device": 1, "batched": 2}[loop]
+    params.deviceSeedOrder = 1 if args.seed_order == "device" else 0
     t = time.time()
     graph = ga.Graph(prefix + ".gfa")
     load_s = time.time() - t
@@ -222,7 +225,7 @@ def main():
         "config": {"workload": wl["text"],
                    "reads": len(names), "read_mbp": sum(map(len, seqs)) / 1e6, "split_nodes": graph.num_nodes,
                    "preset": "vg", "minimizer_k": wl["k"], "minimizer_w": wl["w"], "parallelism": "1 GPU",
-                   "seed_loop": loop, "persistent_workers": st["persistentWorkers"] if loop == "batched" else None},
+                   "seed_loop": loop, "seed_order": args.seed_order, "persistent_workers": st["persistentWorkers"] if loop == "batched" else None},
         "breakdown_s_per_step": {"extend_kernel": kernel_s / steps, "extend_calls": tot["extendCallMs"] / 1e3 / steps,
                                  "seed_lookup_kernel": tot["lookupMs"] / 1e3 / steps,
                                  "seed_lookup_calls": tot["lookupCallMs"] / 1e3 / steps, "host": tot["hostMs"] / 1e3 / steps,

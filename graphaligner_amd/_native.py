@@ -39,7 +39,7 @@ class Params(ctypes.Structure):
                 ("minimizerLength", ctypes.c_int32), ("minimizerWindowSize", ctypes.c_int32),
                 ("minimizerSeedDensity", ctypes.c_double), ("minimizerDiscardMostNumerousFraction", ctypes.c_double),
                 ("cigarMatchMismatchMerge", ctypes.c_int32), ("deviceSeedLoop", ctypes.c_int32),
-                ("maxSpeculativeSeeds", ctypes.c_int32)]
+                ("maxSpeculativeSeeds", ctypes.c_int32), ("deviceSeedOrder", ctypes.c_int32)]
 
 
 class SliceSummary(ctypes.Structure):

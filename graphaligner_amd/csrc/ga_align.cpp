@@ -690,7 +690,7 @@ bool orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds)
 	}
 }
 
-void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs, std::vector<int32_t>* status)
+void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, int order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs, std::vector<int32_t>* status)
 {
 	ctx.uploadReads(seqs, lens);
 	SeedMatches sm;
@@ -698,6 +698,15 @@ void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx
 	if (lookupMs) *lookupMs = sm.lookupMs;
 	out.assign(seqs.size(), {});
 	if (status) status->assign(seqs.size(), 0);
+	if (order == 2)
+	{
+		std::vector<uint8_t> asserted;
+		ctx.orderedSeeds(g, idx, 0, (uint32_t)seqs.size(), sm, density, out, asserted);
+		if (status)
+			for (size_t i = 0; i < seqs.size(); i++)
+				if (asserted[i]) (*status)[i] = READ_ASSERT;
+		return;
+	}
 	parallelFor(seqs.size(), 16, [&](size_t i) {
 		out[i] = seedsFromMatches(g, idx, sm.data() + sm.matchOffset[i], sm.matchCount[i], lens[i], density);
 		if (order && !orderSeedsByChaining(g, out[i]) && status) (*status)[i] = READ_ASSERT;
@@ -924,19 +933,33 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 		// seeds, seed order and the read checks that precede the seed loop
 		std::vector<std::vector<SeedHit>> seeds(bn);
 		std::vector<uint8_t> ready(bn, 0);
+		std::vector<uint8_t> ordered;
+		if (opt.deviceSeedOrder) ctx.orderedSeeds(g, idx, base, bn, rb, opt.minimizerSeedDensity, seeds, ordered);
 		parallelFor(bn, opt.hostThreads, [&](size_t i) {
 			ReadOutcome& o = out[batchStart + i];
 			const char* seq = rs.seq(base + i);
 			const uint32_t L = rs.length[base + i];
 			try
 			{
-				seeds[i] = seedsFromMatches(g, idx, rb.data() + rb.matchOffset[i], rb.matchCount[i], L, opt.minimizerSeedDensity);
-				if (seeds[i].empty())
+				if (opt.deviceSeedOrder)
 				{
-					o.status = READ_NO_SEEDS;
-					return;
+					check(!ordered[i]);
+					if (seeds[i].empty())
+					{
+						o.status = READ_NO_SEEDS;
+						return;
+					}
 				}
-				orderSeedsChecked(g, seeds[i]);
+				else
+				{
+					seeds[i] = seedsFromMatches(g, idx, rb.data() + rb.matchOffset[i], rb.matchCount[i], L, opt.minimizerSeedDensity);
+					if (seeds[i].empty())
+					{
+						o.status = READ_NO_SEEDS;
+						return;
+					}
+					orderSeedsChecked(g, seeds[i]);
+				}
 				// ReverseComplement of the read asserts on letters it does not know
 				for (uint32_t k = 0; k < L; k++) check(complementOf(seq[k]) != 0);
 				ready[i] = 1;
@@ -1256,6 +1279,10 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 		stats.lookupMs += rb.lookupMs;
 		std::vector<ReadState> states(bn);
 		auto th = std::chrono::steady_clock::now();
+		// expansion and OrderSeeds of the whole batch on the backend
+		std::vector<std::vector<SeedHit>> backendSeeds;
+		std::vector<uint8_t> backendAsserted;
+		if (opt.deviceSeedOrder && !callerSeeds) ctx.orderedSeeds(g, *idxp, base, bn, rb, opt.minimizerSeedDensity, backendSeeds, backendAsserted);
 		// seeds of read i: expansion, ordering and the letter check
 		auto seedRead = [&](size_t i) {
 			ReadState& st = states[i];
@@ -1269,6 +1296,17 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 					// AlignOneWay asserts that it has seeds (src/GraphAligner.h:99)
 					st.seeds = callerSeeds[batchStart + i];
 					check(!st.seeds.empty());
+				}
+				else if (opt.deviceSeedOrder)
+				{
+					check(!backendAsserted[i]);
+					st.seeds = std::move(backendSeeds[i]);
+					if (st.seeds.empty())
+					{
+						o.status = READ_NO_SEEDS;
+						st.done = true;
+						return;
+					}
 				}
 				else
 				{

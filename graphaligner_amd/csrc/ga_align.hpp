@@ -89,6 +89,10 @@ struct AlignOptions
 	// pulling reads from a device counter, one captured graph per launch shape,
 	// worker count autotuned); same results
 	bool batchedDeviceLoop = false;
+	// seed expansion and OrderSeeds of each batch through the backend
+	// (ExtendBackend::orderedSeeds: on the GPU for a device context) instead of
+	// host threads; same seeds
+	bool deviceSeedOrder = false;
 	// host-driven rounds: a read's round r > 0 also extends up to 4^r - 1 of the
 	// seeds its loop would try next (at most maxSpeculativeSeeds seeds in all,
 	// more than 4^r when few reads remain, see speculationJobBudget);
@@ -180,7 +184,9 @@ bool orderSeedsByChaining(const GraphHost& g, std::vector<SeedHit>& seeds);
 // Seeds of every read as the reference's seeder + OrderSeeds produce them;
 // status (optional) is READ_ASSERT for a read whose OrderSeeds fired a reference
 // assertion, else 0.
-void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, bool order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs, std::vector<int32_t>* status = nullptr);
+// order: 0 leaves the seeds in expansion order, 1 runs OrderSeeds on the host,
+// 2 runs expansion and OrderSeeds through the backend (ExtendBackend::orderedSeeds)
+void seedReads(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex& idx, double density, int order, const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens, std::vector<std::vector<SeedHit>>& out, double* lookupMs, std::vector<int32_t>* status = nullptr);
 
 // src/Aligner.cpp:390-606 for resident reads [first, first + count) of the
 // backend (one GAF line per selected alignment, '\n'-terminated, in out[i].gaf;

@@ -11,8 +11,12 @@
 #include "ga_trace_codes.h"
 #include "ga_graph.hpp"
 #include "ga_index.hpp"
+#include "ga_order_core.h"
 
 namespace ga {
+
+struct SeedHit;
+
 
 // One extension: align seq[0, len) from column startOffset of split node startNode.
 // With a resident sequence base (extend's devSeqBase) the sequence is the
@@ -230,6 +234,14 @@ public:
 	// For resident reads [first, first + count): writes their reverse complements
 	// and looks every read k-mer up in the index.
 	virtual void lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out) = 0;
+	// The seeds of resident reads [first, first + count) from the matches sm that
+	// lookupSeeds just returned for that range: MinimizerSeeder's expansion
+	// (src/MinimizerSeeder.cpp:521-555) and OrderSeeds (src/GraphAligner.h:250-312)
+	// of every read, run by the backend (ga_order_core.h). seeds[i] receives read
+	// i's ordered seeds; asserted[i] = 1 when OrderSeeds fired a reference
+	// assertion (the read is dropped).
+	virtual void orderedSeeds(const GraphHost& g, const MinimizerIndex& idx, uint32_t first, uint32_t count, const SeedMatches& sm, double density,
+	                          std::vector<std::vector<SeedHit>>& seeds, std::vector<uint8_t>& asserted) = 0;
 	// For resident reads [first, first + count): writes their reverse complements
 	// (what lookupSeeds does besides the lookup), for seeds that come from the caller.
 	virtual void prepareReads(uint32_t first, uint32_t count) = 0;

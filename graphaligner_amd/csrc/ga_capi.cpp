@@ -78,6 +78,7 @@ ga_status_t ga_params_preset(const char* preset, ga_params_t* p)
 	p->minimizerDiscardMostNumerousFraction = 0.0002;
 	p->cigarMatchMismatchMerge = 0;
 	p->maxSpeculativeSeeds = 64;
+	p->deviceSeedOrder = 0;
 	if (!preset) return GA_OK;
 	std::string s = preset;
 	if (s == "dbg")
@@ -373,7 +374,7 @@ ga_status_t ga_seeds(ga_context_t* ctx, const ga_graph_t* graph, const ga_index_
 		for (int64_t i = 0; i < n; i++) l[i] = (uint32_t)lens[i];
 		std::vector<std::vector<SeedHit>> out;
 		std::vector<int32_t> orderStatus;
-		seedReads(*ctx->ctx, graph->host, index->idx, density, order != 0, s, l, out, nullptr, &orderStatus);
+		seedReads(*ctx->ctx, graph->host, index->idx, density, order < 0 ? 0 : order > 2 ? 2 : order, s, l, out, nullptr, &orderStatus);
 		// a read whose OrderSeeds fires a reference assertion is dropped there
 		for (int64_t i = 0; i < n; i++)
 			if (orderStatus[i] != 0) out[i].clear();
@@ -436,6 +437,7 @@ AlignOptions alignOptionsOf(const ga_params_t& p, const ga_context* ctx = nullpt
 	opt.cigarMatchMismatchMerge = p.cigarMatchMismatchMerge != 0;
 	opt.deviceSeedLoop = p.deviceSeedLoop != 0;
 	opt.batchedDeviceLoop = p.deviceSeedLoop == 2;
+	opt.deviceSeedOrder = p.deviceSeedOrder != 0;
 	// the batched form bounds each batch's output pools (about 100 bytes per read
 	// base) and replays its graph once per batch
 	if (opt.batchedDeviceLoop) opt.readsPerBatch = 16384;

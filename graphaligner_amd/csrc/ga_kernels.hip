@@ -11,6 +11,7 @@
 #include "ga_read_core.h"
 #include "ga_kernels.hpp"
 #include "ga_seed_core.h"
+#include "ga_order_core.h"
 #include "ga_trace_codes.h"
 
 namespace gad {
@@ -399,6 +400,36 @@ __global__ void gatherU64Kernel(const uint64_t* __restrict__ src, const uint64_t
 hipError_t launchGatherU64(hipStream_t stream, const uint64_t* src, const uint64_t* srcOff, const uint32_t* count, const uint64_t* dstOff, uint32_t nranges, uint64_t* dst)
 {
 	hipLaunchKernelGGL(gatherU64Kernel, dim3(nranges), dim3(256), 0, stream, src, srcOff, count, dstOff, nranges, dst);
+	return hipGetLastError();
+}
+
+} // namespace gad
+
+namespace gad {
+
+// Seed expansion and OrderSeeds of one read per thread (seedsOfRead): the read's
+// matches (sorted in place by count), its seeds at seedOff[r] with room for
+// seedCap[r], and a same-sized scratch of chain keys. nseeds[r] receives the
+// count or a negative code (-1: a reference assertion, -2/-3: inconsistent input).
+__global__ void __launch_bounds__(64) seedOrderKernel(SeedGraphView g, const uint64_t* __restrict__ positions, uint64_t npositions, uint64_t maxCount, uint32_t k,
+                                                      SeedMatch* __restrict__ matches, const uint64_t* __restrict__ matchOff, const uint32_t* __restrict__ matchCount,
+                                                      const uint32_t* __restrict__ readLen, uint32_t nreads, double density, DevSeedHit* __restrict__ seeds,
+                                                      ChainKey* __restrict__ scratch, const uint64_t* __restrict__ seedOff, const uint64_t* __restrict__ seedCap,
+                                                      int64_t* __restrict__ nseeds)
+{
+	const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nreads) return;
+	nseeds[r] = seedsOfRead(g, positions, npositions, maxCount, k, matches + matchOff[r], matchCount[r], readLen[r], density, seeds + seedOff[r], seedCap[r],
+	                        scratch + seedOff[r]);
+}
+
+hipError_t launchSeedOrder(hipStream_t stream, const SeedGraphView& g, const uint64_t* positions, uint64_t npositions, uint64_t maxCount, uint32_t k, SeedMatch* matches,
+                           const uint64_t* matchOff, const uint32_t* matchCount, const uint32_t* readLen, uint32_t nreads, double density, DevSeedHit* seeds,
+                           ChainKey* scratch, const uint64_t* seedOff, const uint64_t* seedCap, int64_t* nseeds)
+{
+	if (nreads == 0) return hipSuccess;
+	hipLaunchKernelGGL(seedOrderKernel, dim3((nreads + 63) / 64), dim3(64), 0, stream, g, positions, npositions, maxCount, k, matches, matchOff, matchCount, readLen,
+	                   nreads, density, seeds, scratch, seedOff, seedCap, nseeds);
 	return hipGetLastError();
 }
 

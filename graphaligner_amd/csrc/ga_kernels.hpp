@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "ga_device_types.h"
+#include "ga_order_core.h"
 
 namespace gad {
 
@@ -33,6 +34,10 @@ uint64_t requestArenaBytes(uint32_t len, const ArenaCaps& caps);
 hipError_t launchSeedLookup(hipStream_t stream, const char* reads, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, const SeedTableSlot* table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* out, uint32_t* outCount);
 
 // Copies matches[readOff[i], + count[i]) of every read i to out[outOff[i], ...).
+// seed expansion and OrderSeeds of nreads reads (ga_order_core.h), one thread each
+hipError_t launchSeedOrder(hipStream_t stream, const SeedGraphView& g, const uint64_t* positions, uint64_t npositions, uint64_t maxCount, uint32_t k, SeedMatch* matches,
+                           const uint64_t* matchOff, const uint32_t* matchCount, const uint32_t* readLen, uint32_t nreads, double density, DevSeedHit* seeds,
+                           ChainKey* scratch, const uint64_t* seedOff, const uint64_t* seedCap, int64_t* nseeds);
 hipError_t launchGatherMatches(hipStream_t stream, const SeedMatch* matches, const uint64_t* readOff, const uint32_t* count, const uint64_t* outOff, uint32_t nreads, SeedMatch* out);
 
 // Copies src[srcOff[i], + count[i]) to dst[dstOff[i], ...) for every range i.

@@ -1,5 +1,6 @@
 // Device runtime (see ga_runtime.hpp).
 #include "ga_runtime.hpp"
+#include "ga_align.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -110,12 +111,14 @@ static void upload(DeviceBuffer& buf, const std::vector<T>& v, hipStream_t s)
 	if (!v.empty()) GA_HIP(hipMemcpyAsync(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
-const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
+const gad::DevGraph& Context::deviceGraph(const GraphHost& g) { return deviceGraphBuffers(g).view; }
+
+const DeviceGraph& Context::deviceGraphBuffers(const GraphHost& g)
 {
 	if (g.uid == 0) throw std::runtime_error("graph has no identity (not built by loadGfaGraph)");
 	std::lock_guard<std::mutex> lock(cacheMutex_);
 	for (auto& c : graphs_)
-		if (c->uid == g.uid) return c->view;
+		if (c->uid == g.uid) return *c;
 	GA_HIP(hipSetDevice(device_));
 	GA_HIP(hipStreamSynchronize(stream_));
 	if (graphs_.size() >= kCachedObjects) graphs_.erase(graphs_.begin());
@@ -135,6 +138,9 @@ const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 	std::vector<uint32_t> bigSize(g.numNodes());
 	for (size_t i = 0; i < g.numNodes(); i++) bigSize[i] = (uint32_t)g.splitBigSize[i];
 	upload(dg->bigSize, bigSize, stream_);
+	upload(dg->reverse, g.reverse, stream_);
+	upload(dg->chain, g.chain, stream_);
+	upload(dg->chainPos, g.chainPos, stream_);
 	GA_HIP(hipStreamSynchronize(stream_));
 	auto& v = dg->view;
 	v.nodeLen = dg->nodeLen.as<uint32_t>();
@@ -151,8 +157,9 @@ const gad::DevGraph& Context::deviceGraph(const GraphHost& g)
 	v.bigSize = dg->bigSize.as<uint32_t>();
 	v.numNodes = g.numNodes();
 	v.firstAmbiguous = g.firstAmbiguous;
+	dg->seedView = gad::SeedGraphView{v.nodeId, v.nodeOff, dg->reverse.as<uint8_t>(), dg->chain.as<uint32_t>(), dg->chainPos.as<uint64_t>()};
 	graphs_.push_back(std::move(dg));
-	return graphs_.back()->view;
+	return *graphs_.back();
 }
 
 const DeviceIndex& Context::deviceIndex(const MinimizerIndex& idx)
@@ -168,8 +175,10 @@ const DeviceIndex& Context::deviceIndex(const MinimizerIndex& idx)
 	auto di = std::make_unique<DeviceIndex>();
 	di->uid = idx.uid;
 	upload(di->table, idx.table, stream_);
+	upload(di->positions, idx.positions, stream_);
 	GA_HIP(hipStreamSynchronize(stream_));
 	di->mask = idx.mask;
+	di->npositions = idx.positions.size();
 	indexes_.push_back(std::move(di));
 	return *indexes_.back();
 }
@@ -277,6 +286,8 @@ void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t co
 	out.matches.clear();
 	out.staged = nullptr;
 	out.lookupMs = 0;
+	lookupFirst_ = first;
+	lookupCount_ = count;
 	if (count == 0) return;
 	// matches of read i are written at its byte offset (at most one per base)
 	matches_.reserve((rs.bytes + 1) * sizeof(gad::SeedMatch));
@@ -309,6 +320,80 @@ void Context::lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t co
 	GA_HIP(hipMemcpyAsync(matchesHost_.ptr, gathered_.ptr, total * sizeof(gad::SeedMatch), hipMemcpyDeviceToHost, stream_));
 	GA_HIP(hipStreamSynchronize(stream_));
 	out.staged = matchesHost_.as<gad::SeedMatch>();
+}
+
+void Context::orderedSeeds(const GraphHost& g, const MinimizerIndex& idx, uint32_t first, uint32_t count, const SeedMatches& sm, double density,
+                           std::vector<std::vector<SeedHit>>& seeds, std::vector<uint8_t>& asserted)
+{
+	GA_HIP(hipSetDevice(device_));
+	if (first != lookupFirst_ || count != lookupCount_ || sm.matchCount.size() != count)
+		throw std::runtime_error("orderedSeeds needs the matches of the last lookupSeeds");
+	seeds.assign(count, {});
+	asserted.assign(count, 0);
+	if (count == 0) return;
+	const DeviceGraph& dgb = deviceGraphBuffers(g);
+	const DeviceIndex& di = deviceIndex(idx);
+	// room per read: every position of every match (the density cut keeps fewer)
+	std::vector<uint64_t> off(count), cap(count);
+	uint64_t total = 0;
+	const gad::SeedMatch* m = sm.data();
+	for (uint32_t i = 0; i < count; i++)
+	{
+		uint64_t c = 0;
+		for (uint32_t k = 0; k < sm.matchCount[i]; k++) c += m[sm.matchOffset[i] + k].count;
+		off[i] = total;
+		cap[i] = c;
+		total += c;
+	}
+	seedOut_.reserve(std::max<uint64_t>(total, 1) * sizeof(gad::DevSeedHit));
+	seedScratch_.reserve(std::max<uint64_t>(total, 1) * sizeof(gad::ChainKey));
+	seedOff_.reserve(count * sizeof(uint64_t));
+	seedCap_.reserve(count * sizeof(uint64_t));
+	seedCount_.reserve(count * sizeof(int64_t));
+	GA_HIP(hipMemcpyAsync(seedOff_.ptr, off.data(), count * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+	GA_HIP(hipMemcpyAsync(seedCap_.ptr, cap.data(), count * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+	// the lookup left the reads' matches gathered in gathered_ at matchOff_; the
+	// kernel sorts each read's matches in place (the host copy is not touched)
+	uint64_t nmatches = 0;
+	for (uint32_t i = 0; i < count; i++) nmatches += sm.matchCount[i];
+	// no matches: no read has seeds (lookupSeeds then left no gathered matches)
+	if (nmatches == 0) return;
+	if (gathered_.bytes < nmatches * sizeof(gad::SeedMatch) || matchOff_.bytes < count * sizeof(uint64_t)) throw std::runtime_error("gathered matches missing");
+	const uint32_t* len = readLen_.as<uint32_t>() + first;
+	GA_HIP(gad::launchSeedOrder(stream_, dgb.seedView, di.positions.as<uint64_t>(), di.npositions, idx.maxCount, idx.k, gathered_.as<gad::SeedMatch>(),
+	                            matchOff_.as<uint64_t>(), matchCount_.as<uint32_t>(), len, count, density, seedOut_.as<gad::DevSeedHit>(),
+	                            seedScratch_.as<gad::ChainKey>(), seedOff_.as<uint64_t>(), seedCap_.as<uint64_t>(), seedCount_.as<int64_t>()));
+	std::vector<int64_t> n(count);
+	std::vector<gad::DevSeedHit> hits(total);
+	GA_HIP(hipMemcpyAsync(n.data(), seedCount_.ptr, count * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+	if (total > 0) GA_HIP(hipMemcpyAsync(hits.data(), seedOut_.ptr, total * sizeof(gad::DevSeedHit), hipMemcpyDeviceToHost, stream_));
+	GA_HIP(hipStreamSynchronize(stream_));
+	for (uint32_t i = 0; i < count; i++)
+	{
+		if (n[i] == -1)
+		{
+			asserted[i] = 1;
+			continue;
+		}
+		if (n[i] < 0 || (uint64_t)n[i] > cap[i]) throw HipError("seed ordering kernel returned an inconsistent seed count");
+		auto& v = seeds[i];
+		v.resize(n[i]);
+		for (int64_t k = 0; k < n[i]; k++)
+		{
+			const gad::DevSeedHit& d = hits[off[i] + k];
+			SeedHit& h = v[k];
+			h.nodeID = d.nodeID;
+			h.nodeOffset = d.nodeOffset;
+			h.seqPos = d.seqPos;
+			h.matchLen = d.matchLen;
+			h.reverse = d.reverse != 0;
+			h.alignmentGraphNodeId = d.alignmentGraphNodeId;
+			h.alignmentGraphNodeOffset = d.alignmentGraphNodeOffset;
+			h.rawSeedGoodness = d.rawSeedGoodness;
+			h.seedGoodness = d.seedGoodness;
+			h.seedClusterSize = d.seedClusterSize;
+		}
+	}
 }
 
 void Context::runOnce(ExtendLane& L, const gad::DevGraph& dg, const gad::ExtParams& p, const std::vector<ExtJob>& jobs, const std::vector<uint32_t>& which, const gad::ArenaCaps& caps, bool wantSummaries, ExtBatch& out, const char* devSeqBase, const uint64_t* devPlanes)

@@ -77,7 +77,10 @@ struct DeviceGraph
 {
 	uint64_t uid = 0;
 	DeviceBuffer nodeLen, seq2, amb4, comp, lin, inOff, inAdj, outOff, outAdj, nodeId, nodeOff, bigSize;
+	// what seed expansion and OrderSeeds read besides nodeId / nodeOff
+	DeviceBuffer reverse, chain, chainPos;
 	gad::DevGraph view;
+	gad::SeedGraphView seedView;
 };
 
 struct DeviceIndex
@@ -85,6 +88,8 @@ struct DeviceIndex
 	uint64_t uid = 0;
 	DeviceBuffer table;
 	uint64_t mask = 0;
+	DeviceBuffer positions; // for seed expansion on the device
+	uint64_t npositions = 0;
 };
 
 class Context : public ExtendBackend
@@ -102,6 +107,9 @@ public:
 	void uploadReads(const std::vector<const char*>& seqs, const std::vector<uint32_t>& lens) override;
 	const ReadSet& reads() const override { return reads_; }
 	void lookupSeeds(const MinimizerIndex& idx, uint32_t first, uint32_t count, SeedMatches& out) override;
+	void orderedSeeds(const GraphHost& g, const MinimizerIndex& idx, uint32_t first, uint32_t count, const SeedMatches& sm, double density,
+	                  std::vector<std::vector<SeedHit>>& seeds, std::vector<uint8_t>& asserted) override;
+	const DeviceGraph& deviceGraphBuffers(const GraphHost& g);
 	void prepareReads(uint32_t first, uint32_t count) override;
 	void beginAlignmentCall() override;
 	const char* readsBase() const override { return readBytes_.as<char>(); }
@@ -170,6 +178,8 @@ private:
 	DeviceBuffer arena_, gatherSrc_, gatherDst_, gatherCnt_, traceOut_; // device seed loop
 	DeviceBuffer rlSeeds_, rlJobs_, rlTraces_, rlPool_, rlAlns_, rlResults_;
 	DeviceBuffer readBytes_, readOff_, readLen_, matches_, matchCount_, matchOff_, gathered_, readPlanes_;
+	DeviceBuffer seedOut_, seedScratch_, seedOff_, seedCap_, seedCount_;
+	uint32_t lookupFirst_ = 0, lookupCount_ = 0; // read range of the last lookupSeeds
 	// host staging of a lookup's matches (SeedMatches::staged)
 	PinnedBuffer matchesHost_;
 	// resident reads whose reverse complements (and so the planes) are written:

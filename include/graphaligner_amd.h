@@ -68,6 +68,10 @@ typedef struct
 	 * loop's decisions are replayed in order, so results do not depend on it.
 	 * Presets set 64. */
 	int32_t maxSpeculativeSeeds;
+	/* Execution only: 1 runs seed expansion and OrderSeeds of every read on the
+	 * GPU (one thread per read, ga_seeds' order 2) instead of host threads; the
+	 * seeds are the same. */
+	int32_t deviceSeedOrder;
 } ga_params_t;
 
 /* Fills params with the defaults plus preset "vg" or "dbg" (-x), as
@@ -174,7 +178,9 @@ ga_status_t ga_index_export(const ga_index_t* index, uint64_t* kmers, int64_t* s
 
 /* Seeds of n reads: MinimizerSeeder::getSeeds (src/MinimizerSeeder.cpp:521-555)
  * with the k-mer lookup on the GPU and, when order != 0,
- * GraphAligner::orderSeedsByChaining (src/GraphAligner.h:250-312). *seeds is a
+ * GraphAligner::orderSeedsByChaining (src/GraphAligner.h:250-312): order 1 expands
+ * and orders on host threads, order 2 on the GPU (one thread per read,
+ * restating std::sort's tie order); both give the same seeds. *seeds is a
  * buffer from malloc (free with ga_free) holding counts[0] + ... + counts[n-1]
  * seeds, read by read, 10 int64 each in SeedHit field order
  * (src/GraphAlignerWrapper.h:11-50): nodeID, nodeOffset, seqPos, matchLen,

@@ -15,12 +15,14 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <random>
 #include <vector>
 #include "../../graphaligner_amd/csrc/ga_align.hpp"
 #include "../../graphaligner_amd/csrc/ga_proposal.hpp"
 #include "../../graphaligner_amd/csrc/ga_extend_core.h"
 #include "../../graphaligner_amd/csrc/ga_read_core.h"
 #include "../../graphaligner_amd/csrc/ga_seed_core.h"
+#include "../../graphaligner_amd/csrc/ga_order_core.h"
 
 namespace {
 
@@ -371,6 +373,52 @@ public:
 		computePlanes();
 	}
 
+	// the device seeding code (ga_order_core.h) built for the host, one read per task
+	void orderedSeeds(const ga::GraphHost& G, const ga::MinimizerIndex& idx, uint32_t first, uint32_t count, const ga::SeedMatches& sm, double density,
+	                  std::vector<std::vector<ga::SeedHit>>& seeds, std::vector<uint8_t>& asserted) override
+	{
+		seeds.assign(count, {});
+		asserted.assign(count, 0);
+		gad::SeedGraphView gv{G.nodeId.data(), G.nodeOff.data(), G.reverse.data(), G.chain.data(), G.chainPos.data()};
+		std::atomic<bool> bad{false};
+		forEachParallel(count, [&](size_t i) {
+			std::vector<gad::SeedMatch> m(sm.data() + sm.matchOffset[i], sm.data() + sm.matchOffset[i] + sm.matchCount[i]);
+			uint64_t cap = 0;
+			for (const auto& x : m) cap += x.count;
+			std::vector<gad::DevSeedHit> out(cap + 1);
+			std::vector<gad::ChainKey> scratch(cap + 1);
+			const int64_t n = gad::seedsOfRead(gv, idx.positions.data(), idx.positions.size(), idx.maxCount, idx.k, m.data(), (uint32_t)m.size(),
+			                                   rs_.length[first + i], density, out.data(), cap, scratch.data());
+			if (n == -1)
+			{
+				asserted[i] = 1;
+				return;
+			}
+			if (n < 0)
+			{
+				bad = true;
+				return;
+			}
+			auto& v = seeds[i];
+			v.resize(n);
+			for (int64_t k = 0; k < n; k++)
+			{
+				const gad::DevSeedHit& d = out[k];
+				v[k].nodeID = d.nodeID;
+				v[k].nodeOffset = d.nodeOffset;
+				v[k].seqPos = d.seqPos;
+				v[k].matchLen = d.matchLen;
+				v[k].reverse = d.reverse != 0;
+				v[k].alignmentGraphNodeId = d.alignmentGraphNodeId;
+				v[k].alignmentGraphNodeOffset = d.alignmentGraphNodeOffset;
+				v[k].rawSeedGoodness = d.rawSeedGoodness;
+				v[k].seedGoodness = d.seedGoodness;
+				v[k].seedClusterSize = d.seedClusterSize;
+			}
+		});
+		if (bad) throw std::runtime_error("seed ordering returned an inconsistent seed count");
+	}
+
 	void prepareReads(uint32_t first, uint32_t count) override
 	{
 		forEachParallel(count, [&](size_t i) {
@@ -439,7 +487,7 @@ int pipe_get_seeds(void* ih, const char* seq, int64_t len, double density, int o
 	HostBackend be;
 	std::vector<std::vector<ga::SeedHit>> seeds;
 	std::vector<int32_t> status;
-	ga::seedReads(be, *x->g, x->idx, density, order != 0, {seq}, {(uint32_t)len}, seeds, nullptr, &status);
+	ga::seedReads(be, *x->g, x->idx, density, order, {seq}, {(uint32_t)len}, seeds, nullptr, &status);
 	*n = seeds[0].size();
 	if (status[0] != 0)
 	{
@@ -465,6 +513,71 @@ int pipe_get_seeds(void* ih, const char* seq, int64_t len, double density, int o
 	return 0;
 }
 
+// Seeds of one read through the device seeding code (ga_order_core.h) built
+// for the host: lookupReadKmers, then seedsOfRead. Same layout and return codes
+// as pipe_get_seeds (ordered seeds only).
+int pipe_get_seeds_device_code(void* ih, const char* seq, int64_t len, double density, int64_t* out, int64_t cap, int64_t* n)
+{
+	auto* x = (EmuIndex*)ih;
+	const ga::GraphHost& g = *x->g;
+	std::vector<gad::SeedTableSlot> table(x->idx.table.size());
+	for (size_t i = 0; i < table.size(); i++) table[i] = gad::SeedTableSlot{x->idx.table[i].kmer, x->idx.table[i].start, x->idx.table[i].count};
+	std::vector<gad::SeedMatch> m(len + 1);
+	const uint32_t nm = gad::lookupReadKmers(seq, len, table.data(), x->idx.mask, x->idx.k, x->idx.w, x->idx.maxCount, m.data());
+	uint64_t bound = 0;
+	for (uint32_t i = 0; i < nm; i++) bound += m[i].count;
+	std::vector<gad::DevSeedHit> seeds(bound + 1);
+	std::vector<gad::ChainKey> scratch(bound + 1);
+	gad::SeedGraphView gv{g.nodeId.data(), g.nodeOff.data(), g.reverse.data(), g.chain.data(), g.chainPos.data()};
+	const int64_t r = gad::seedsOfRead(gv, x->idx.positions.data(), x->idx.positions.size(), x->idx.maxCount, x->idx.k, m.data(), nm, len, density, seeds.data(), bound, scratch.data());
+	*n = 0;
+	if (r == -1) return 2;
+	if (r < 0) return 3;
+	*n = r;
+	if (r > cap) return 1;
+	for (int64_t i = 0; i < r; i++)
+	{
+		const auto& h = seeds[i];
+		int64_t* o = out + 10 * i;
+		o[0] = h.nodeID;
+		o[1] = h.nodeOffset;
+		o[2] = h.seqPos;
+		o[3] = h.matchLen;
+		o[4] = h.reverse;
+		o[5] = h.alignmentGraphNodeId;
+		o[6] = h.alignmentGraphNodeOffset;
+		o[7] = (int64_t)h.rawSeedGoodness;
+		o[8] = (int64_t)h.seedGoodness;
+		o[9] = (int64_t)h.seedClusterSize;
+	}
+	return 0;
+}
+
+// libstdSort (or, with heapOnly, its heap sort) against libstdc++'s std::sort
+// (std::partial_sort over the whole range) on n random keys below tieRange:
+// 0 when the permutations (original indices) are identical.
+int pipe_sort_check(int64_t n, uint64_t seed, uint64_t tieRange, int heapOnly)
+{
+	std::mt19937_64 rng(seed);
+	std::vector<std::pair<uint64_t, uint32_t>> a(n);
+	for (int64_t i = 0; i < n; i++) a[i] = {rng() % tieRange, (uint32_t)i};
+	auto b = a;
+	auto less = [](const std::pair<uint64_t, uint32_t>& l, const std::pair<uint64_t, uint32_t>& r) { return l.first < r.first; };
+	if (heapOnly)
+	{
+		std::partial_sort(a.begin(), a.end(), a.end(), less);
+		gad::heapSortRange(b.data(), (int64_t)n, less);
+	}
+	else
+	{
+		std::sort(a.begin(), a.end(), less);
+		gad::libstdSort(b.data(), (int64_t)n, less);
+	}
+	for (int64_t i = 0; i < n; i++)
+		if (a[i].second != b[i].second) return 1;
+	return 0;
+}
+
 // Whole pipeline; arguments follow the oracle's orc_align_reads.
 struct PipeOptions
 {
@@ -480,6 +593,7 @@ struct PipeOptions
 	int32_t deviceLoop;
 	int32_t maxSpeculativeSeeds;
 	double minAlignmentScore;
+	int32_t deviceSeedOrder;
 };
 
 void pipe_set_fault(int f) { g_fault = f; }
@@ -501,6 +615,7 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 	o.minSeedClusterSize = po->minClusterSize;
 	o.hostThreads = 4;
 	o.deviceSeedLoop = po->deviceLoop != 0;
+	o.deviceSeedOrder = po->deviceSeedOrder != 0;
 	o.batchedDeviceLoop = po->deviceLoop == 2;
 	o.maxSpeculativeSeeds = po->maxSpeculativeSeeds < 1 ? 1 : (uint32_t)po->maxSpeculativeSeeds;
 	o.minAlignmentScore = po->minAlignmentScore;

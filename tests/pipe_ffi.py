@@ -16,7 +16,7 @@ class PipeOptions(ctypes.Structure):
                 ("seedExtendDensity", ctypes.c_double), ("multimapScoreFraction", ctypes.c_double),
                 ("seedDensity", ctypes.c_double), ("minClusterSize", ctypes.c_int64),
                 ("deviceLoop", ctypes.c_int32), ("maxSpeculativeSeeds", ctypes.c_int32),
-                ("minAlignmentScore", ctypes.c_double)]
+                ("minAlignmentScore", ctypes.c_double), ("deviceSeedOrder", ctypes.c_int32)]
 
 
 # library variants: the default build, and one whose traceback recomputes every
@@ -42,6 +42,11 @@ def lib(variant=""):
         L.pipe_get_seeds.restype = ctypes.c_int
         L.pipe_get_seeds.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.pipe_get_seeds_device_code.restype = ctypes.c_int
+        L.pipe_get_seeds_device_code.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64, ctypes.c_double,
+                                                 ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.pipe_sort_check.restype = ctypes.c_int
+        L.pipe_sort_check.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
         L.pipe_align_reads.restype = ctypes.c_int
         L.pipe_align_reads.argtypes = [ctypes.c_void_p, ctypes.POINTER(PipeOptions), ctypes.c_int64,
                                        ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
@@ -100,6 +105,19 @@ class Index:
     def num_kmers(self):
         return self.L.pipe_index_num_kmers(self.h)
 
+    def seeds_device_code(self, seq, density):
+        """Ordered seeds through the device seeding code built for the host
+        (lookupReadKmers + seedsOfRead); None on a reference assertion."""
+        cap = max(1024, 64 * len(seq))
+        out = np.zeros(10 * cap, dtype=np.int64)
+        n = ctypes.c_int64()
+        rc = self.L.pipe_get_seeds_device_code(self.h, seq.encode(), len(seq), density,
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(n))
+        if rc == 2:
+            return None
+        assert rc == 0, rc
+        return out[: 10 * n.value].reshape(-1, 10)
+
     def seeds(self, seq, density, order=False):
         """Seeds as rows of 10 SeedHit fields; None when OrderSeeds fired a
         reference assertion (the read would be dropped)."""
@@ -114,20 +132,23 @@ class Index:
         return out[: 10 * n.value].reshape(-1, 10)
 
 
-def options(oracle_params, density, device_loop=False, speculative=64):
+def options(oracle_params, density, device_loop=False, speculative=64, device_seed_order=False):
     p = oracle_params
     return PipeOptions(p.bandwidth, p.errorCost, p.xdrop, p.sloppy, p.maxCellsPerSlice, p.seedExtendDensity,
                        p.multimapScoreFraction, density, p.minClusterSize, int(device_loop), speculative,
-                       p.minAlignmentScore)
+                       p.minAlignmentScore, int(device_seed_order))
 
 
 # execution modes of the seed loop: host-driven rounds one seed at a time, rounds
 # with speculative extension of later seeds, the whole loop on the device
+# (with devseeds: seed expansion and OrderSeeds through the backend's device code)
 MODES = {"rounds": dict(device_loop=False, speculative=1), "rounds_spec": dict(device_loop=False, speculative=64),
-         "device_loop": dict(device_loop=True, speculative=1), "batched": dict(device_loop=2, speculative=1)}
+         "device_loop": dict(device_loop=True, speculative=1), "batched": dict(device_loop=2, speculative=1),
+         "rounds_devseeds": dict(device_loop=False, speculative=64, device_seed_order=True),
+         "device_loop_devseeds": dict(device_loop=True, speculative=1, device_seed_order=True)}
 
 
-def align_reads(index, oracle_params, density, names, seqs, device_loop=False, speculative=64):
+def align_reads(index, oracle_params, density, names, seqs, device_loop=False, speculative=64, device_seed_order=False):
     n = len(seqs)
     na = (ctypes.c_char_p * n)(*[x.encode() for x in names])
     sa = (ctypes.c_char_p * n)(*[x.encode() for x in seqs])
@@ -135,7 +156,7 @@ def align_reads(index, oracle_params, density, names, seqs, device_loop=False, s
     out = ctypes.c_void_p()
     status = np.zeros(n, dtype=np.int32)
     stats = np.zeros(6, dtype=np.int64)
-    opt = options(oracle_params, density, device_loop, speculative)
+    opt = options(oracle_params, density, device_loop, speculative, device_seed_order)
     rc = index.L.pipe_align_reads(index.h, ctypes.byref(opt), n, na, sa, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                   ctypes.byref(out), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                   stats.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
