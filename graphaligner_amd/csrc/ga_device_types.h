@@ -4,6 +4,31 @@
 
 namespace gad {
 
+// The static data of one split node in one 48-byte record, so that a node's
+// calculation reads it in one memory round trip: its 2-bit bases (zero words
+// for IUPAC nodes), length, component, in-neighbour range in inAdj and the first
+// in-neighbour (the one a linearizable node's test reads), and out-edge range in
+// outEdges.
+struct NodeRec
+{
+	uint64_t seq0, seq1;
+	uint32_t len;
+	uint32_t comp;
+	uint32_t inBegin, inCount;
+	uint32_t outBegin, outCount;
+	uint32_t firstIn; // inAdj[inBegin], or ~0 without in-neighbours
+	uint32_t linearizable;
+};
+static_assert(sizeof(NodeRec) == 48, "NodeRec is three 16-byte loads");
+
+// An out-edge with its target's component number (the band queue's key), read
+// together when a node's changed end relaxes its out-neighbours.
+struct OutEdge
+{
+	uint32_t target;
+	uint32_t comp;
+};
+
 // Read-only alignment graph resident in HBM. Node sequences are 2-bit packed
 // (A0 C1 G2 T3, LSB first, 32 bp per 64-bit chunk); IUPAC nodes (index >=
 // firstAmbiguous) carry four 64-bit base masks instead. Adjacency is CSR.
@@ -21,6 +46,8 @@ struct DevGraph
 	const int32_t* nodeId;   // bigraph node id of each split node
 	const uint32_t* nodeOff; // offset of the split node in its bigraph node
 	const uint32_t* bigSize; // length of the split node's bigraph node
+	const NodeRec* rec;      // per split node, the fields above that the DP reads
+	const OutEdge* outEdges; // out-edges in outAdj order, with target components
 	uint32_t numNodes;
 	uint32_t firstAmbiguous;
 };

@@ -1029,12 +1029,12 @@ struct Extension
 	uint32_t maxBand;
 	Pos back;
 
-	GA_HD uint32_t nodeLen(uint32_t n) const { return g.nodeLen[n]; }
+	GA_HD uint32_t nodeLen(uint32_t n) const { return g.rec[n].len; }
 	GA_HD uint32_t nodeBaseSet(uint32_t n, uint32_t pos) const
 	{
 		if (n < g.firstAmbiguous)
 		{
-			uint32_t code = (uint32_t)(g.seq2[2 * n + (pos >> 5)] >> ((pos & 31) * 2)) & 3;
+			uint32_t code = (uint32_t)(((pos >> 5) ? g.rec[n].seq1 : g.rec[n].seq0) >> ((pos & 31) * 2)) & 3;
 			return 1u << code;
 		}
 		const uint64_t* m = g.amb4 + 4 * (uint64_t)(n - g.firstAmbiguous);
@@ -1044,7 +1044,7 @@ struct Extension
 	{
 		if (n < g.firstAmbiguous)
 		{
-			uint32_t code = (uint32_t)(g.seq2[2 * n + (pos >> 5)] >> ((pos & 31) * 2)) & 3;
+			uint32_t code = (uint32_t)(((pos >> 5) ? g.rec[n].seq1 : g.rec[n].seq0) >> ((pos & 31) * 2)) & 3;
 			return eqSelect(eq, code);
 		}
 		uint32_t set = nodeBaseSet(n, pos);
@@ -1068,8 +1068,8 @@ struct Extension
 		s.amb = n >= g.firstAmbiguous;
 		if (!s.amb)
 		{
-			s.w[0] = g.seq2[2 * n];
-			s.w[1] = g.seq2[2 * n + 1];
+			s.w[0] = g.rec[n].seq0;
+			s.w[1] = g.rec[n].seq1;
 			s.w[2] = s.w[3] = 0;
 		}
 		else
@@ -1243,7 +1243,7 @@ struct Extension
 		if (exists)
 		{
 			Slice start = mkslice(it.sVP, it.sVN, it.sEnd);
-			if (hasSkipless && (g.inOff[node + 1] - g.inOff[node]) == 1 && prevHasInNeighbor)
+			if (hasSkipless && g.rec[node].inCount == 1 && prevHasInNeighbor)
 			{
 				if (ws.end > start.end)
 				{
@@ -1603,9 +1603,9 @@ struct Extension
 				if (j != 0)
 				{
 					if (pi.minScore > prevQuit) continue;
-					if (g.linearizable[pi.node])
+					if (g.rec[pi.node].linearizable)
 					{
-						uint32_t nb = g.inAdj[g.inOff[pi.node]];
+						uint32_t nb = g.rec[pi.node].firstIn;
 						int32_t q = findPrev(nb);
 						if (q >= 0)
 						{
@@ -1614,7 +1614,7 @@ struct Extension
 						}
 					}
 				}
-				queueInsert(g.comp[pi.node], pi.minScore, pi.node, sourceSlice(pi.sEnd), true);
+				queueInsert(g.rec[pi.node].comp, pi.minScore, pi.node, sourceSlice(pi.sEnd), true);
 			}
 			if (status != EXT_OK) return;
 			if (nPend == 0)
@@ -1664,7 +1664,7 @@ struct Extension
 				int32_t pq = findPrev(node);
 				Item prevIt = pq >= 0 ? prevItems[pq] : defaultPrev();
 				bool prevHasIn = false;
-				if (exists && (g.inOff[node + 1] - g.inOff[node]) == 1) prevHasIn = findPrev(g.inAdj[g.inOff[node]]) >= 0;
+				if (exists && g.rec[node].inCount == 1) prevHasIn = findPrev(g.rec[node].firstIn) >= 0;
 				NodeResult nc = calcNode<false>(node, it, exists, eq, prevIt, pq >= 0, slotHead[slot], mkslice(0, 0, 0), j, prevHasIn, nullptr, rMaxX);
 				nodeCalcs++;
 				if (status != EXT_OK) return;
@@ -1695,10 +1695,11 @@ struct Extension
 					}
 					if (m <= curMin + b)
 					{
-						for (uint32_t e = g.outOff[node]; e < g.outOff[node + 1]; e++)
+						const uint32_t ob = g.rec[node].outBegin, oe = ob + g.rec[node].outCount;
+						for (uint32_t e = ob; e < oe; e++)
 						{
-							uint32_t nb = g.outAdj[e];
-							queueInsert(g.comp[nb], m, nb, newEnd, false);
+							const OutEdge edge = g.outEdges[e];
+							queueInsert(edge.comp, m, edge.target, newEnd, false);
 						}
 						if (status != EXT_OK) return;
 					}
@@ -1808,7 +1809,7 @@ struct Extension
 		Item cn = loadItem(ci);
 		int32_t scoreHere = valueAt(mkslice(cn.sVP, cn.sVN, cn.sEnd), 0);
 		int32_t pn = findInSlice(prv, node);
-		uint32_t inB = g.inOff[node], inE = g.inOff[node + 1];
+		uint32_t inB = g.rec[node].inBegin, inE = inB + g.rec[node].inCount;
 		if (notValid || scoreHere > quit)
 		{
 			int32_t smallest = scoreHere + 1;
@@ -1920,7 +1921,7 @@ struct Extension
 		}
 		bool eq = charMatch(pos.seq, pos.node, pos.off);
 		int32_t scoreHere = valueAt(startSlice, offset);
-		uint32_t inB = g.inOff[node], inE = g.inOff[node + 1];
+		uint32_t inB = g.rec[node].inBegin, inE = inB + g.rec[node].inCount;
 		if (notValid || scoreHere > quit)
 		{
 			int32_t smallest = valueAt(startSlice, offset - 1);
@@ -2289,7 +2290,7 @@ struct Extension
 			}
 			if (back.off == 0 && beforeScore(0) != 0)
 			{
-				for (uint32_t e = g.inOff[back.node]; e < g.inOff[back.node + 1]; e++)
+				for (uint32_t e = g.rec[back.node].inBegin, inE = e + g.rec[back.node].inCount; e < inE; e++)
 				{
 					uint32_t nb = g.inAdj[e];
 					int32_t q = findInSlice(0, nb);

@@ -795,6 +795,24 @@ GraphHost loadGfaGraph(const std::string& path)
 	g.originalName = b.oname;
 	g.splitBigSize.resize(n);
 	for (size_t i = 0; i < n; i++) g.splitBigSize[i] = g.originalSize.at(g.nodeId[i]);
+	g.nodeRec.resize(n);
+	for (size_t i = 0; i < n; i++)
+	{
+		gad::NodeRec& r = g.nodeRec[i];
+		const bool amb = i >= g.firstAmbiguous;
+		r.seq0 = amb ? 0 : g.seq2[2 * i];
+		r.seq1 = amb ? 0 : g.seq2[2 * i + 1];
+		r.len = g.nodeLen[i];
+		r.comp = g.comp[i];
+		r.inBegin = g.inOff[i];
+		r.inCount = g.inOff[i + 1] - g.inOff[i];
+		r.outBegin = g.outOff[i];
+		r.outCount = g.outOff[i + 1] - g.outOff[i];
+		r.firstIn = r.inCount > 0 ? g.inAdj[g.inOff[i]] : ~0u;
+		r.linearizable = g.linearizable[i];
+	}
+	g.outEdges.resize(g.outAdj.size());
+	for (size_t e = 0; e < g.outAdj.size(); e++) g.outEdges[e] = gad::OutEdge{g.outAdj[e], g.comp[g.outAdj[e]]};
 	return g;
 }
 

@@ -11,6 +11,7 @@
 #include <string>
 #include <unordered_map>
 #include <vector>
+#include "ga_device_types.h"
 
 namespace ga {
 
@@ -43,6 +44,10 @@ struct GraphHost
 	std::unordered_map<int, std::string> originalName;
 	std::vector<int> lookupOrder;                         // bigraph ids in the lookup map's iteration order
 	std::vector<uint64_t> splitBigSize;                   // per split node: length of its bigraph node
+	// the same per-node data as one record per node and the out-edges with their
+	// targets' components (gad::NodeRec, gad::OutEdge), for the device layout
+	std::vector<gad::NodeRec> nodeRec;
+	std::vector<gad::OutEdge> outEdges;
 
 	size_t numNodes() const { return nodeLen.size(); }
 	char nodeChar(uint32_t node, uint32_t pos) const;

@@ -141,6 +141,8 @@ const DeviceGraph& Context::deviceGraphBuffers(const GraphHost& g)
 	upload(dg->reverse, g.reverse, stream_);
 	upload(dg->chain, g.chain, stream_);
 	upload(dg->chainPos, g.chainPos, stream_);
+	upload(dg->rec, g.nodeRec, stream_);
+	upload(dg->outEdges, g.outEdges, stream_);
 	GA_HIP(hipStreamSynchronize(stream_));
 	auto& v = dg->view;
 	v.nodeLen = dg->nodeLen.as<uint32_t>();
@@ -155,6 +157,8 @@ const DeviceGraph& Context::deviceGraphBuffers(const GraphHost& g)
 	v.nodeId = dg->nodeId.as<int32_t>();
 	v.nodeOff = dg->nodeOff.as<uint32_t>();
 	v.bigSize = dg->bigSize.as<uint32_t>();
+	v.rec = dg->rec.as<gad::NodeRec>();
+	v.outEdges = dg->outEdges.as<gad::OutEdge>();
 	v.numNodes = g.numNodes();
 	v.firstAmbiguous = g.firstAmbiguous;
 	dg->seedView = gad::SeedGraphView{v.nodeId, v.nodeOff, dg->reverse.as<uint8_t>(), dg->chain.as<uint32_t>(), dg->chainPos.as<uint64_t>()};

@@ -79,6 +79,7 @@ struct DeviceGraph
 	DeviceBuffer nodeLen, seq2, amb4, comp, lin, inOff, inAdj, outOff, outAdj, nodeId, nodeOff, bigSize;
 	// what seed expansion and OrderSeeds read besides nodeId / nodeOff
 	DeviceBuffer reverse, chain, chainPos;
+	DeviceBuffer rec, outEdges;
 	gad::DevGraph view;
 	gad::SeedGraphView seedView;
 };

@@ -44,6 +44,8 @@ int emu_extend(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int3
 	g.inAdj = G.inAdj.data();
 	g.outOff = G.outOff.data();
 	g.outAdj = G.outAdj.data();
+	g.rec = G.nodeRec.data();
+	g.outEdges = G.outEdges.data();
 	g.numNodes = G.numNodes();
 	g.firstAmbiguous = G.firstAmbiguous;
 	gad::ExtParams p;

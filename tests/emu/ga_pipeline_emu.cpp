@@ -93,6 +93,8 @@ public:
 		g.inAdj = G.inAdj.data();
 		g.outOff = G.outOff.data();
 		g.outAdj = G.outAdj.data();
+		g.rec = G.nodeRec.data();
+		g.outEdges = G.outEdges.data();
 		g.nodeId = G.nodeId.data();
 		g.nodeOff = G.nodeOff.data();
 		g.bigSize = bigSize_.data();

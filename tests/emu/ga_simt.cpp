@@ -111,6 +111,8 @@ int simt_extend(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost, int
 	g.inAdj = G.inAdj.data();
 	g.outOff = G.outOff.data();
 	g.outAdj = G.outAdj.data();
+	g.rec = G.nodeRec.data();
+	g.outEdges = G.outEdges.data();
 	g.numNodes = G.numNodes();
 	g.firstAmbiguous = G.firstAmbiguous;
 	gad::ExtParams p;
@@ -223,6 +225,8 @@ int simt_persistent(void* gh, int32_t bandwidth, int64_t maxCells, int32_t cost,
 	g.inAdj = G.inAdj.data();
 	g.outOff = G.outOff.data();
 	g.outAdj = G.outAdj.data();
+	g.rec = G.nodeRec.data();
+	g.outEdges = G.outEdges.data();
 	g.numNodes = G.numNodes();
 	g.firstAmbiguous = G.firstAmbiguous;
 	gad::ExtParams p;
