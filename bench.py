@@ -47,7 +47,7 @@ BYTES_PER_WORD_STEP = 6
 # constant measured offline with rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled
 # by the reading of a 1 GiB copy kernel) on the default command, recorded in
 # profiles/extend_vg5m.md; reported as such, not measured by this run
-MEASURED_TRAFFIC_PER_LAUNCH = 5.08e9
+MEASURED_TRAFFIC_PER_LAUNCH = 5.41e9
 TRAFFIC_SOURCE = "constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the default run, profiles/extend_vg5m.md"
 
 
@@ -96,6 +96,9 @@ def parse():
                     help="the learned proposal: train the network on outcome records of the plain method (fresh seeds of "
                          "the 5 Mbp variation-graph generator), then time plain and learned on --workload (vg5m or dbg2m)")
     ap.add_argument("--train-records", type=int, default=1_000_000, help="--learned: outcome records to train on (at least)")
+    ap.add_argument("--longest-extensions", type=int, default=0, metavar="N",
+                    help="only time one launch holding the forward extensions of the N longest reads of --workload from "
+                         "their first seeds (the latency of the longest extensions without the rest of a launch) and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (traffic-counter calibration) and exit")
     return ap.parse_args()
@@ -114,10 +117,44 @@ def copy_calibration(nbytes):
     return 0
 
 
+def longest_extensions(args):
+    """One launch of the forward extensions of the N longest reads, each from its
+    seed nearest the read's start: how long the longest extensions take when no
+    other extension shares the GPU."""
+    import numpy as np
+    import graphaligner_amd as ga
+    from graphaligner_amd import synth
+
+    workdir = args.workdir or tempfile.mkdtemp(prefix="ga_bench_")
+    os.makedirs(workdir, exist_ok=True)
+    wl = WORKLOADS[args.workload]
+    prefix = os.path.join(workdir, wl["preset"])
+    if not (os.path.exists(prefix + ".gfa") and os.path.exists(prefix + ".fa")):
+        synth.generate(wl["preset"], prefix)
+    names, seqs = synth.read_fasta(prefix + ".fa")
+    longest = sorted(range(len(seqs)), key=lambda i: -len(seqs[i]))[: args.longest_extensions]
+    params = ga.params_preset("vg")
+    params.minimizerLength, params.minimizerWindowSize = wl["k"], wl["w"]
+    graph = ga.Graph(prefix + ".gfa")
+    ctx = ga.Context(0)
+    index = ga.Index(graph, params, ctx=ctx)
+    jobs = []
+    for i in longest:
+        sd = ga.seeds(ctx, graph, index, [seqs[i]], params.minimizerSeedDensity)[0]
+        first = sd[np.argmin(sd[:, 2])]
+        jobs.append((seqs[i][int(first[2]) + 1:], int(first[0]) * 2 + int(first[4]), int(first[1])))
+    got, ms = ga.extend_batch(ctx, graph, params, [j[0] for j in jobs], [j[1] for j in jobs], [j[2] for j in jobs], summaries=True)
+    print(json.dumps({"longest_extensions": len(jobs), "kernel_ms": ms, "lengths": [len(j[0]) for j in jobs],
+                      "slices": [len(g[3]) for g in got]}), flush=True)
+    return 0
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.longest_extensions:
+        return longest_extensions(args)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path, and "
               "--gpus %d is not supported (run with --gpus 1)" % args.gpus, file=sys.stderr)

@@ -302,12 +302,12 @@ __global__ void __launch_bounds__(256) idxPlaceKernel(const uint32_t* __restrict
 	positions[start + c - 1 - (i - s)] = posFound[val[i]];
 }
 
-__global__ void __launch_bounds__(256) idxTableKernel(const uint64_t* __restrict__ key, const uint32_t* __restrict__ groupStart, uint32_t groups, uint32_t n,
-                                                      const uint32_t* __restrict__ groupRank, const uint32_t* __restrict__ startByRank, SeedTableSlot* table,
-                                                      uint64_t mask, uint32_t* __restrict__ histogram, uint32_t histSize)
+// group g's slot in the table (linear probing, slots claimed by compare-and-swap)
+// and its count in the histogram
+__device__ __forceinline__ void idxInsertGroup(const uint64_t* __restrict__ key, const uint32_t* __restrict__ groupStart, uint32_t groups, uint32_t n,
+                                               const uint32_t* __restrict__ groupRank, const uint32_t* __restrict__ startByRank, SeedTableSlot* table,
+                                               uint64_t mask, uint32_t* __restrict__ histogram, uint32_t histSize, uint32_t* local, uint32_t g)
 {
-	const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-	if (g >= groups) return;
 	const uint32_t s = groupStart[g];
 	const uint32_t c = (g + 1 < groups ? groupStart[g + 1] : n) - s;
 	const uint64_t kmer = key[s];
@@ -321,7 +321,23 @@ __global__ void __launch_bounds__(256) idxTableKernel(const uint64_t* __restrict
 	}
 	table[i].start = startByRank[groupRank[g]];
 	table[i].count = c;
-	if (c < histSize) atomicAdd(&histogram[c], 1u);
+	if (c < 256) atomicAdd(&local[c], 1u);
+	else if (c < histSize) atomicAdd(&histogram[c], 1u);
+}
+
+__global__ void __launch_bounds__(256) idxTableKernel(const uint64_t* __restrict__ key, const uint32_t* __restrict__ groupStart, uint32_t groups, uint32_t n,
+                                                      const uint32_t* __restrict__ groupRank, const uint32_t* __restrict__ startByRank, SeedTableSlot* table,
+                                                      uint64_t mask, uint32_t* __restrict__ histogram, uint32_t histSize)
+{
+	// counts below 256 (nearly all k-mers) are tallied in LDS first: one global
+	// atomic per bin and workgroup instead of one per k-mer on a few hot bins
+	__shared__ uint32_t local[256];
+	local[threadIdx.x] = 0;
+	__syncthreads();
+	const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g < groups) idxInsertGroup(key, groupStart, groups, n, groupRank, startByRank, table, mask, histogram, histSize, local, g);
+	__syncthreads();
+	if (local[threadIdx.x] != 0 && threadIdx.x < histSize) atomicAdd(&histogram[threadIdx.x], local[threadIdx.x]);
 }
 
 } // namespace gad
