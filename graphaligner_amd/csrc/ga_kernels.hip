@@ -407,20 +407,25 @@ hipError_t launchGatherU64(hipStream_t stream, const uint64_t* src, const uint64
 
 namespace gad {
 
-// Seed expansion and OrderSeeds of one read per thread (seedsOfRead): the read's
-// matches (sorted in place by count), its seeds at seedOff[r] with room for
-// seedCap[r], and a same-sized scratch of chain keys. nseeds[r] receives the
-// count or a negative code (-1: a reference assertion, -2/-3: inconsistent input).
-__global__ void __launch_bounds__(64) seedOrderKernel(SeedGraphView g, const uint64_t* __restrict__ positions, uint64_t npositions, uint64_t maxCount, uint32_t k,
-                                                      SeedMatch* __restrict__ matches, const uint64_t* __restrict__ matchOff, const uint32_t* __restrict__ matchCount,
-                                                      const uint32_t* __restrict__ readLen, uint32_t nreads, double density, DevSeedHit* __restrict__ seeds,
-                                                      ChainKey* __restrict__ scratch, const uint64_t* __restrict__ seedOff, const uint64_t* __restrict__ seedCap,
-                                                      int64_t* __restrict__ nseeds)
+// Seed expansion and OrderSeeds of one read per single-thread workgroup
+// (seedsOfRead): each read's sorts are a serial chain, so a read alone in its
+// wave costs that chain only, where 64 reads in one wave's lanes would diverge
+// and run one after another; values are wave-uniform and go to the scalar unit.
+// The read's matches are sorted in place by count, its seeds written at
+// seedOff[r] with room for seedCap[r], and a same-sized scratch of chain keys
+// used; the sorts' pending ranges are in LDS. nseeds[r] receives the count or a
+// negative code (-1: a reference assertion, -2/-3: inconsistent input).
+__global__ void __launch_bounds__(1) seedOrderKernel(SeedGraphView g, const uint64_t* __restrict__ positions, uint64_t npositions, uint64_t maxCount, uint32_t k,
+                                                     SeedMatch* __restrict__ matches, const uint64_t* __restrict__ matchOff, const uint32_t* __restrict__ matchCount,
+                                                     const uint32_t* __restrict__ readLen, uint32_t nreads, double density, DevSeedHit* __restrict__ seeds,
+                                                     ChainKey* __restrict__ scratch, const uint64_t* __restrict__ seedOff, const uint64_t* __restrict__ seedCap,
+                                                     int64_t* __restrict__ nseeds)
 {
-	const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ SortRange stack[kSortStackEntries];
+	const uint32_t r = blockIdx.x;
 	if (r >= nreads) return;
 	nseeds[r] = seedsOfRead(g, positions, npositions, maxCount, k, matches + matchOff[r], matchCount[r], readLen[r], density, seeds + seedOff[r], seedCap[r],
-	                        scratch + seedOff[r]);
+	                        scratch + seedOff[r], stack);
 }
 
 hipError_t launchSeedOrder(hipStream_t stream, const SeedGraphView& g, const uint64_t* positions, uint64_t npositions, uint64_t maxCount, uint32_t k, SeedMatch* matches,
@@ -428,8 +433,8 @@ hipError_t launchSeedOrder(hipStream_t stream, const SeedGraphView& g, const uin
                            ChainKey* scratch, const uint64_t* seedOff, const uint64_t* seedCap, int64_t* nseeds)
 {
 	if (nreads == 0) return hipSuccess;
-	hipLaunchKernelGGL(seedOrderKernel, dim3((nreads + 63) / 64), dim3(64), 0, stream, g, positions, npositions, maxCount, k, matches, matchOff, matchCount, readLen,
-	                   nreads, density, seeds, scratch, seedOff, seedCap, nseeds);
+	hipLaunchKernelGGL(seedOrderKernel, dim3(nreads), dim3(1), 0, stream, g, positions, npositions, maxCount, k, matches, matchOff, matchCount, readLen, nreads,
+	                   density, seeds, scratch, seedOff, seedCap, nseeds);
 	return hipGetLastError();
 }
 

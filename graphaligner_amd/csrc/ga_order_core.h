@@ -136,29 +136,34 @@ GA_ORDER_HD void insertionSortRange(T* a, int64_t n, Less less)
 	}
 }
 
-// std::sort's permutation of a[0, n) under less (see the header comment); the
-// recursion on the right part becomes a stack of pending ranges: ranges are
-// disjoint, so the order in which they are partitioned does not change the result
+// a range of libstdSort waiting to be partitioned, with its remaining depth
+struct SortRange
+{
+	int64_t first, last;
+	int32_t depth;
+};
+// libstdSort keeps at most floor(log2(n)) ranges pending, below 64 for any n
+constexpr int kSortStackEntries = 64;
+
+// std::sort's permutation of a[0, n) under less (see the header comment). The
+// recursion becomes a loop over the smaller part of each partition with the
+// larger part pushed on stack (kSortStackEntries entries, so at most log2(n) are
+// pending): ranges are disjoint, so the order in which they are partitioned does
+// not change the result.
 template <typename T, typename Less>
-GA_ORDER_HD void libstdSort(T* a, int64_t n, Less less)
+GA_ORDER_HD void libstdSort(T* a, int64_t n, Less less, SortRange* stack)
 {
 	if (n < 2) return;
 	const int64_t threshold = 16;
 	int lg = 0;
 	while (((uint64_t)n >> (lg + 1)) != 0) lg++;
-	struct Range
-	{
-		int64_t first, last;
-		int depth;
-	};
-	Range stack[130];
 	int top = 0;
-	stack[top++] = Range{0, n, 2 * lg};
+	stack[top++] = SortRange{0, n, 2 * lg};
 	while (top > 0)
 	{
-		Range r = stack[--top];
+		SortRange r = stack[--top];
 		int64_t first = r.first, last = r.last;
-		int depth = r.depth;
+		int32_t depth = r.depth;
 		while (last - first > threshold)
 		{
 			if (depth == 0)
@@ -192,8 +197,17 @@ GA_ORDER_HD void libstdSort(T* a, int64_t n, Less less)
 				orderSwap(a[lo], a[hi]);
 				lo++;
 			}
-			stack[top++] = Range{lo, last, depth};
-			last = lo;
+			// [first, lo) and [lo, last) both go on with this depth
+			if (lo - first < last - lo)
+			{
+				stack[top++] = SortRange{lo, last, depth};
+				last = lo;
+			}
+			else
+			{
+				stack[top++] = SortRange{first, lo, depth};
+				first = lo;
+			}
 		}
 	}
 	// final insertion sort: guarded over the first 16, unguarded after them
@@ -218,13 +232,13 @@ struct ChainKey
 // OrderSeeds leaves them, into out[0, cap). Returns the number of seeds, -1 when
 // a reference assertion of OrderSeeds fires (the read is dropped), -2 when cap
 // is too small, -3 when a match lies outside the npositions positions. scratch
-// holds cap entries.
+// holds cap entries, stack kSortStackEntries.
 GA_ORDER_HD int64_t seedsOfRead(const SeedGraphView& g, const uint64_t* positions, uint64_t npositions, uint64_t maxCount, uint32_t k, SeedMatch* m,
-                                uint32_t nm, uint64_t readLen, double density, DevSeedHit* out, uint64_t cap, ChainKey* scratch)
+                                uint32_t nm, uint64_t readLen, double density, DevSeedHit* out, uint64_t cap, ChainKey* scratch, SortRange* stack)
 {
 	for (uint32_t i = 0; i < nm; i++)
 		if ((uint64_t)m[i].start + m[i].count > npositions) return -3;
-	libstdSort(m, (int64_t)nm, [](const SeedMatch& l, const SeedMatch& r) { return l.count < r.count; });
+	libstdSort(m, (int64_t)nm, [](const SeedMatch& l, const SeedMatch& r) { return l.count < r.count; }, stack);
 	const uint64_t maxHits = density == -1 ? ~(uint64_t)0 : (uint64_t)(readLen * density);
 	uint64_t n = 0, seedsHere = 0, allowed = 0;
 	for (uint32_t i = 0; i < nm; i++)
@@ -264,13 +278,13 @@ GA_ORDER_HD int64_t seedsOfRead(const SeedGraphView& g, const uint64_t* position
 	}
 	libstdSort(scratch, (int64_t)n, [](const ChainKey& l, const ChainKey& r) {
 		return l.chain != r.chain ? l.chain < r.chain : l.diagonal != r.diagonal ? l.diagonal < r.diagonal : l.seed < r.seed;
-	});
+	}, stack);
 	uint64_t start = 0;
 	for (uint64_t i = 1; i <= n; i++)
 	{
 		if (i < n && scratch[i].chain == scratch[i - 1].chain && scratch[i].diagonal <= scratch[i - 1].diagonal + 100) continue;
 		// cluster [start, i): bases covered by its seeds' k-mers, by read position
-		libstdSort(scratch + start, (int64_t)(i - start), [](const ChainKey& l, const ChainKey& r) { return l.seqPos != r.seqPos ? l.seqPos < r.seqPos : l.seed < r.seed; });
+		libstdSort(scratch + start, (int64_t)(i - start), [](const ChainKey& l, const ChainKey& r) { return l.seqPos != r.seqPos ? l.seqPos < r.seqPos : l.seed < r.seed; }, stack);
 		uint64_t covered = 0;
 		int32_t lastEnd = INT32_MIN;
 		for (uint64_t c = start; c < i; c++)
@@ -290,7 +304,7 @@ GA_ORDER_HD int64_t seedsOfRead(const SeedGraphView& g, const uint64_t* position
 		}
 		start = i;
 	}
-	libstdSort(out, (int64_t)n, [](const DevSeedHit& l, const DevSeedHit& r) { return l.seedGoodness < r.seedGoodness; });
+	libstdSort(out, (int64_t)n, [](const DevSeedHit& l, const DevSeedHit& r) { return l.seedGoodness < r.seedGoodness; }, stack);
 	for (uint64_t i = 0, j = n; i + 1 < j; i++, j--) orderSwap(out[i], out[j - 1]);
 	return (int64_t)n;
 }

@@ -389,8 +389,9 @@ public:
 			for (const auto& x : m) cap += x.count;
 			std::vector<gad::DevSeedHit> out(cap + 1);
 			std::vector<gad::ChainKey> scratch(cap + 1);
+			gad::SortRange stack[gad::kSortStackEntries];
 			const int64_t n = gad::seedsOfRead(gv, idx.positions.data(), idx.positions.size(), idx.maxCount, idx.k, m.data(), (uint32_t)m.size(),
-			                                   rs_.length[first + i], density, out.data(), cap, scratch.data());
+			                                   rs_.length[first + i], density, out.data(), cap, scratch.data(), stack);
 			if (n == -1)
 			{
 				asserted[i] = 1;
@@ -531,7 +532,9 @@ int pipe_get_seeds_device_code(void* ih, const char* seq, int64_t len, double de
 	std::vector<gad::DevSeedHit> seeds(bound + 1);
 	std::vector<gad::ChainKey> scratch(bound + 1);
 	gad::SeedGraphView gv{g.nodeId.data(), g.nodeOff.data(), g.reverse.data(), g.chain.data(), g.chainPos.data()};
-	const int64_t r = gad::seedsOfRead(gv, x->idx.positions.data(), x->idx.positions.size(), x->idx.maxCount, x->idx.k, m.data(), nm, len, density, seeds.data(), bound, scratch.data());
+	gad::SortRange stack[gad::kSortStackEntries];
+	const int64_t r = gad::seedsOfRead(gv, x->idx.positions.data(), x->idx.positions.size(), x->idx.maxCount, x->idx.k, m.data(), nm, len, density, seeds.data(), bound,
+	                                   scratch.data(), stack);
 	*n = 0;
 	if (r == -1) return 2;
 	if (r < 0) return 3;
@@ -573,7 +576,8 @@ int pipe_sort_check(int64_t n, uint64_t seed, uint64_t tieRange, int heapOnly)
 	else
 	{
 		std::sort(a.begin(), a.end(), less);
-		gad::libstdSort(b.data(), (int64_t)n, less);
+		gad::SortRange stack[gad::kSortStackEntries];
+		gad::libstdSort(b.data(), (int64_t)n, less, stack);
 	}
 	for (int64_t i = 0; i < n; i++)
 		if (a[i].second != b[i].second) return 1;
@@ -782,6 +786,18 @@ void pipe_proposal_stats(int64_t* out)
 	out[0] = (int64_t)g_lastStats.proposalClusters;
 	out[1] = (int64_t)g_lastStats.prunedClusters;
 	out[2] = (int64_t)g_lastStats.prunedSeeds;
+}
+
+// host phase times of the last pipe_align_reads (ms): seeding, seed picking,
+// trace rebuild and loop replay, selection and GAF, all host work, whole call
+void pipe_host_phase_ms(double* out)
+{
+	out[0] = g_lastStats.seedingMs;
+	out[1] = g_lastStats.pickMs;
+	out[2] = g_lastStats.replayMs;
+	out[3] = g_lastStats.gafMs;
+	out[4] = g_lastStats.hostMs;
+	out[5] = g_lastStats.totalMs;
 }
 
 // clusters of ordered seeds (rows of 10 as pipe_get_seeds) and their features
