@@ -18,6 +18,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <thread>
+#include <type_traits>
 
 namespace ga {
 
@@ -753,7 +754,37 @@ struct CodeReader
 		index++;
 		return prev;
 	}
+	// the code of the item next() would read
+	uint32_t peek() const { return codes[index]; }
+	// passes over n items the caller derived itself, the last of which is `last`
+	void skip(uint32_t n, uint64_t last)
+	{
+		index += n;
+		prev = last;
+	}
 };
+
+// Code of an item one read row up and one column left of the item before it in
+// the same node, without a node switch (ga_trace_codes.h): the diagonal step
+// that makes up most of a trace. Such an item follows from the one before it,
+// and the step passes verifyTrace's check by construction.
+constexpr uint32_t kDiagonalCode = (1u << 2) | (1u << 1);
+
+uint64_t packRaw(const RawStep& r)
+{
+	const uint32_t hi = ((uint32_t)(r.seqPos + 1) << 7) | (r.off << 1) | (r.sw ? 1u : 0u);
+	return ((uint64_t)hi << 32) | r.node;
+}
+
+RawStep diagonalBefore(const RawStep& r)
+{
+	RawStep d;
+	d.node = r.node;
+	d.off = r.off - 1;
+	d.seqPos = r.seqPos - 1;
+	d.sw = false;
+	return d;
+}
 
 // What the trace rebuild needs of one split node, refreshed when the trace
 // enters another node: its bigraph id and offset base in either direction and
@@ -825,6 +856,7 @@ Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char
 	out.resize((size_t)nb + nf - (nb > 0 && nf > 0 ? 1 : 0));
 	int32_t score = 0;
 	NodeView view;
+	constexpr bool kCodes = std::is_same<Reader, CodeReader>::value;
 	if (nb > 0)
 	{
 		const uint64_t end = s.seqPos - 1;
@@ -850,6 +882,33 @@ Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char
 			check(c != 0);
 			t.graphChar = c;
 			score += editTerm(k == 0 ? nullptr : &out[k - 1], t);
+			if constexpr (kCodes)
+			{
+				// the same steps for items k + 1, ... while the item after each is a
+				// diagonal step (so it follows from it, passes the step check and
+				// has no node switch) and the item lies in the node of the view
+				uint32_t taken = 0;
+				while (k + 2 < nb && next.node == view.split && bwd.peek() == kDiagonalCode)
+				{
+					k++;
+					const RawStep item = next;
+					next = diagonalBefore(item);
+					taken++;
+					bwd.skip(1, 0);
+					TItem& u = out[k];
+					u.node = view.bigNode;
+					u.nodeOffset = view.offBase - item.off;
+					u.seqPos = end - item.seqPos;
+					u.nodeSwitch = false;
+					check(u.seqPos < L);
+					u.seqChar = seq[u.seqPos];
+					const char d = view.complementBase(g, item.off);
+					check(d != 0);
+					u.graphChar = d;
+					score += editTerm(&out[k - 1], u);
+				}
+				if (taken > 0) bwd.skip(0, packRaw(next));
+			}
 		}
 	}
 	if (nf > 0)
@@ -887,6 +946,33 @@ Alignment buildAlignment(const GraphHost& g, const AlignOptions& opt, const char
 			out[last - k] = t;
 			prevT = t;
 			prevRaw = cur;
+			if constexpr (kCodes)
+			{
+				// the same steps for the following items that are diagonal steps
+				// (each follows from the one before, in the same node, passes the
+				// step check and has no node switch), the last item excepted
+				uint32_t taken = 0;
+				while (k + 2 < nf && fwd.peek() == kDiagonalCode)
+				{
+					k++;
+					const RawStep item = diagonalBefore(prevRaw);
+					taken++;
+					fwd.skip(1, 0);
+					TItem u;
+					u.node = view.bigNode;
+					u.nodeOffset = view.offBase + item.off;
+					u.seqPos = item.seqPos + start;
+					u.nodeSwitch = false;
+					check(u.seqPos < L);
+					u.seqChar = seq[u.seqPos];
+					u.graphChar = view.base(g, item.off);
+					score += editTerm(&u, prevT);
+					out[last - k] = u;
+					prevT = u;
+					prevRaw = item;
+				}
+				if (taken > 0) fwd.skip(0, packRaw(prevRaw));
+			}
 		}
 	}
 	check(!out.empty());
