@@ -407,12 +407,18 @@ void appendInt(std::string& s, int64_t v)
 	else appendUint(s, (uint64_t)v);
 }
 
-void addCigar(std::string& s, uint64_t n, int type)
+// decimal digits of v written at p (as std::ostream prints it); returns the end
+inline char* putUint(char* p, uint64_t v)
 {
-	static const char ops[] = {'=', 'X', 'M', 'I', 'D'};
-	if (n == 0 || type < 0 || type > 4) return;
-	appendUint(s, n);
-	s.push_back(ops[type]);
+	char buf[20];
+	int n = 0;
+	do
+	{
+		buf[n++] = (char)('0' + v % 10);
+		v /= 10;
+	} while (v != 0);
+	while (n > 0) *p++ = buf[--n];
+	return p;
 }
 
 // GAF line of one alignment (src/GraphAlignerGAFAlignment.h:37-250).
@@ -420,77 +426,99 @@ void addCigar(std::string& s, uint64_t n, int type)
 void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_t readLen, const Alignment& a, bool merge)
 {
 	enum { Match = 0, Mismatch = 1, MatchOrMismatch = 2, Insertion = 3, Deletion = 4, Empty = 5 };
-	const auto& t = a.trace;
-	// scratch kept per thread: the line's parts are built, then appended
-	thread_local std::string cigar, path;
-	cigar.clear();
+	static const char ops[] = {'=', 'X', 'M', 'I', 'D'};
+	const TItem* t = a.trace.data();
+	const size_t n = a.trace.size();
+	// scratch kept per thread: the line's parts are built, then appended. Every
+	// cigar run covers at least one item and takes at most 21 characters, so the
+	// cigar of n items fits in 21 n bytes.
+	thread_local std::string path;
+	thread_local std::vector<char> cigarBuf;
 	path.clear();
-	uint64_t pathLen = 0, matches = 0, mismatches = 0, deletions = 0, insertions = 0, editLength = 0;
+	if (cigarBuf.size() < 21 * n + 32) cigarBuf.resize(21 * n + 32);
+	char* cg = cigarBuf.data();
+	// items per edit type; without merge every item's type is its run's, so the
+	// counts are added per run, with merge matches and mismatches per item
+	uint64_t perType[5] = {0, 0, 0, 0, 0};
+	uint64_t pathLen = 0;
 	int curId = (int)(int64_t)t[0].node;
 	bool curRev = (t[0].node % 2) == 1;
 	uint64_t curOff = t[0].nodeOffset;
-	int currentEdit = Empty;
 	auto addNode = [&](int id, bool rev) {
 		path.push_back(rev ? '<' : '>');
 		const std::string& nm = g.originalName.at(id);
 		if (nm.empty()) appendInt(path, id / 2);
 		else path += nm;
 	};
-	bool firstMatch = characterMatch(t[0].seqChar, t[0].graphChar);
-	currentEdit = merge ? MatchOrMismatch : (firstMatch ? Match : Mismatch);
-	editLength = 1;
-	if (firstMatch) matches++;
-	else mismatches++;
+	const bool firstMatch = characterMatch(t[0].seqChar, t[0].graphChar);
+	int currentEdit = merge ? MatchOrMismatch : (firstMatch ? Match : Mismatch);
+	uint64_t editLength = 1;
+	uint64_t mergedMatches = firstMatch ? 1 : 0, mergedMismatches = firstMatch ? 0 : 1;
 	addNode(curId, curRev);
 	pathLen += g.originalSize.at(curId);
-	for (size_t pos = 1; pos < t.size(); pos++)
+	for (size_t pos = 1; pos < n; pos++)
 	{
-		int newId = (int)(int64_t)t[pos].node;
-		bool newRev = (t[pos].node % 2) == 1;
-		uint64_t newOff = t[pos].nodeOffset;
-		bool insideNode = !t[pos - 1].nodeSwitch || (newId == curId && newRev == curRev && newOff > curOff);
+		const TItem& p = t[pos - 1];
+		const TItem& c = t[pos];
+		bool insideNode = !p.nodeSwitch;
 		if (!insideNode)
 		{
-			uint64_t skippedBefore = g.originalSize.at(curId) - 1 - t[pos - 1].nodeOffset;
-			curId = newId;
-			curRev = newRev;
-			curOff = newOff;
-			addNode(curId, curRev);
-			uint64_t skippedAfter = t[pos].nodeOffset;
-			pathLen += g.originalSize.at(curId) - (skippedBefore + skippedAfter);
+			const int newId = (int)(int64_t)c.node;
+			const bool newRev = (c.node % 2) == 1;
+			const uint64_t newOff = c.nodeOffset;
+			insideNode = newId == curId && newRev == curRev && newOff > curOff;
+			if (!insideNode)
+			{
+				const uint64_t skippedBefore = g.originalSize.at(curId) - 1 - p.nodeOffset;
+				curId = newId;
+				curRev = newRev;
+				curOff = newOff;
+				addNode(curId, curRev);
+				const uint64_t skippedAfter = c.nodeOffset;
+				pathLen += g.originalSize.at(curId) - (skippedBefore + skippedAfter);
+			}
 		}
 		int nextEdit;
-		bool match = false;
-		if (t[pos - 1].seqPos == t[pos].seqPos) nextEdit = Deletion;
-		else if (insideNode && t[pos - 1].nodeOffset == t[pos].nodeOffset) nextEdit = Insertion;
+		if (p.seqPos == c.seqPos) nextEdit = Deletion;
+		else if (insideNode && p.nodeOffset == c.nodeOffset) nextEdit = Insertion;
 		else
 		{
-			match = characterMatch(t[pos].seqChar, t[pos].graphChar);
-			nextEdit = merge ? MatchOrMismatch : (match ? Match : Mismatch);
+			const bool match = characterMatch(c.seqChar, c.graphChar);
+			if (merge)
+			{
+				nextEdit = MatchOrMismatch;
+				mergedMatches += match ? 1 : 0;
+				mergedMismatches += match ? 0 : 1;
+			}
+			else nextEdit = match ? Match : Mismatch;
 		}
 		if (currentEdit != nextEdit)
 		{
-			addCigar(cigar, editLength, currentEdit);
+			perType[currentEdit] += editLength;
+			cg = putUint(cg, editLength);
+			*cg++ = ops[currentEdit];
 			currentEdit = nextEdit;
 			editLength = 0;
 		}
 		editLength++;
-		if (nextEdit == Deletion) deletions++;
-		else if (nextEdit == Insertion) insertions++;
-		else if (match) matches++;
-		else mismatches++;
 	}
-	addCigar(cigar, editLength, currentEdit);
-	uint64_t pathEnd = pathLen - (g.originalSize.at((int)(int64_t)t.back().node) - 1 - t.back().nodeOffset);
+	perType[currentEdit] += editLength;
+	cg = putUint(cg, editLength);
+	*cg++ = ops[currentEdit];
+	const uint64_t matches = merge ? mergedMatches : perType[Match];
+	const uint64_t mismatches = merge ? mergedMismatches : perType[Mismatch];
+	const uint64_t deletions = perType[Deletion], insertions = perType[Insertion];
+	const size_t cigarLen = (size_t)(cg - cigarBuf.data());
+	uint64_t pathEnd = pathLen - (g.originalSize.at((int)(int64_t)t[n - 1].node) - 1 - t[n - 1].nodeOffset);
 	double total = (double)(matches + mismatches + deletions + insertions);
-	s.reserve(s.size() + cigar.size() + path.size() + std::strlen(name) + 192);
+	s.reserve(s.size() + cigarLen + path.size() + std::strlen(name) + 192);
 	s += name;
 	s += '\t';
 	appendUint(s, readLen);
 	s += '\t';
 	appendUint(s, t[0].seqPos);
 	s += '\t';
-	appendUint(s, t.back().seqPos + 1);
+	appendUint(s, t[n - 1].seqPos + 1);
 	s += "\t+\t";
 	s += path;
 	s += '\t';
@@ -502,7 +530,7 @@ void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_
 	s += '\t';
 	appendUint(s, matches);
 	s += '\t';
-	appendUint(s, t.size());
+	appendUint(s, n);
 	s += '\t';
 	appendInt(s, a.mappingQuality);
 	s += "\tNM:i:";
@@ -514,7 +542,7 @@ void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_
 	f << "\tid:f:" << ((double)matches / total);
 	s += f.str();
 	s += "\tcg:Z:";
-	s += cigar;
+	s.append(cigarBuf.data(), cigarLen);
 }
 
 } // namespace

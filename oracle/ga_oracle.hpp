@@ -1321,6 +1321,8 @@ struct Params
 	double multimapScoreFraction = 0.9;
 	// --min-alignment-score (src/AlignerMain.cpp:151,213), as the double option
 	double minAlignmentScore = 0;
+	// --cigar-match-mismatch (src/AlignerMain.cpp, used at src/Aligner.cpp:564)
+	bool cigarMatchMismatchMerge = false;
 	// per-slice bandwidth schedule: the value getXdropSlices picks for each slice
 	// (src/GraphAlignerBitvectorBanded.h:611, stored in DPSlice.bandwidth at :552
 	// and read back by the traceback's quit scores,
@@ -3402,7 +3404,7 @@ inline ReadOutcome alignRead(const AlignmentGraph& graph, const MinimizerIndex& 
 	std::string text;
 	for (auto& a : alignments)
 	{
-		text += traceToGAF(name, sequence, *a.trace, a.alignmentXScore, a.mappingQuality, graph, false);
+		text += traceToGAF(name, sequence, *a.trace, a.alignmentXScore, a.mappingQuality, graph, params.cigarMatchMismatchMerge);
 		text += '\n';
 	}
 	out.gaf = std::move(text);

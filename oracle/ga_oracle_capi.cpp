@@ -28,6 +28,7 @@ struct OrcParams
 	int32_t sloppy;
 	int32_t minClusterSize;
 	double minAlignmentScore;
+	int32_t cigarMatchMismatchMerge;
 };
 
 static Params toParams(const OrcParams* p)
@@ -42,6 +43,7 @@ static Params toParams(const OrcParams* p)
 	r.sloppyOptimizations = p->sloppy != 0;
 	r.minSeedClusterSize = p->minClusterSize;
 	r.minAlignmentScore = p->minAlignmentScore;
+	r.cigarMatchMismatchMerge = p->cigarMatchMismatchMerge != 0;
 	return r;
 }
 

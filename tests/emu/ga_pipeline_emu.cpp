@@ -600,6 +600,7 @@ struct PipeOptions
 	int32_t maxSpeculativeSeeds;
 	double minAlignmentScore;
 	int32_t deviceSeedOrder;
+	int32_t cigarMatchMismatchMerge;
 };
 
 void pipe_set_fault(int f) { g_fault = f; }
@@ -622,6 +623,7 @@ int pipe_align_reads(void* ih, const PipeOptions* po, int64_t nreads, const char
 	o.hostThreads = 4;
 	o.deviceSeedLoop = po->deviceLoop != 0;
 	o.deviceSeedOrder = po->deviceSeedOrder != 0;
+	o.cigarMatchMismatchMerge = po->cigarMatchMismatchMerge != 0;
 	o.batchedDeviceLoop = po->deviceLoop == 2;
 	o.maxSpeculativeSeeds = po->maxSpeculativeSeeds < 1 ? 1 : (uint32_t)po->maxSpeculativeSeeds;
 	o.minAlignmentScore = po->minAlignmentScore;

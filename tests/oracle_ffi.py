@@ -31,6 +31,7 @@ class OrcParams(ctypes.Structure):
         ("sloppy", ctypes.c_int32),
         ("minClusterSize", ctypes.c_int32),
         ("minAlignmentScore", ctypes.c_double),
+        ("cigarMatchMismatchMerge", ctypes.c_int32),
     ]
 
 

@@ -16,7 +16,8 @@ class PipeOptions(ctypes.Structure):
                 ("seedExtendDensity", ctypes.c_double), ("multimapScoreFraction", ctypes.c_double),
                 ("seedDensity", ctypes.c_double), ("minClusterSize", ctypes.c_int64),
                 ("deviceLoop", ctypes.c_int32), ("maxSpeculativeSeeds", ctypes.c_int32),
-                ("minAlignmentScore", ctypes.c_double), ("deviceSeedOrder", ctypes.c_int32)]
+                ("minAlignmentScore", ctypes.c_double), ("deviceSeedOrder", ctypes.c_int32),
+                ("cigarMatchMismatchMerge", ctypes.c_int32)]
 
 
 # library variants: the default build, and one whose traceback recomputes every
@@ -136,7 +137,7 @@ def options(oracle_params, density, device_loop=False, speculative=64, device_se
     p = oracle_params
     return PipeOptions(p.bandwidth, p.errorCost, p.xdrop, p.sloppy, p.maxCellsPerSlice, p.seedExtendDensity,
                        p.multimapScoreFraction, density, p.minClusterSize, int(device_loop), speculative,
-                       p.minAlignmentScore, int(device_seed_order))
+                       p.minAlignmentScore, int(device_seed_order), int(p.cigarMatchMismatchMerge))
 
 
 # execution modes of the seed loop: host-driven rounds one seed at a time, rounds

@@ -164,3 +164,25 @@ def test_gaf_identical_wide_fans(workdir, mode):
     assert (ostat == 0).sum() >= len(seqs) // 2
     assert list(pstat) == list(ostat)
     assert pt == ot
+
+
+@pytest.mark.parametrize("mode", ["rounds_spec", "device_loop"])
+@pytest.mark.parametrize("wl,preset", [("mini_sv", "vg"), ("mini_dbg", "dbg")])
+def test_gaf_identical_cigar_match_mismatch_merge(workdir, wl, preset, mode):
+    """--cigar-match-mismatch (src/Aligner.cpp:564): M runs in the cigar, matches
+    and mismatches still counted one by one for the match count, NM, dv and id."""
+    gfa, names, seqs = W.workload(wl, workdir)
+    sd, og, oi, pg, pi = _pair(gfa, preset)
+    en, es = W.edge_reads(names, seqs)
+    names = list(names) + en
+    seqs = list(seqs) + es
+    params = O.preset_params(preset)
+    plain, _, _ = O.align_reads(og, oi, params, sd["density"], names, seqs, 4)
+    params.cigarMatchMismatchMerge = 1
+    ot, ostat, _ = O.align_reads(og, oi, params, sd["density"], names, seqs, 4)
+    pt, pstat, _ = P.align_reads(pi, params, sd["density"], names, seqs, **P.MODES[mode])
+    assert list(pstat) == list(ostat)
+    assert pt == ot
+    # the option changes only the cigar: M in place of = and X
+    assert "M" in ot and ot != plain
+    assert [l.split("\tcg:Z:")[0] for l in ot.splitlines()] == [l.split("\tcg:Z:")[0] for l in plain.splitlines()]
