@@ -446,16 +446,16 @@ void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_
 	uint64_t curOff = t[0].nodeOffset;
 	auto addNode = [&](int id, bool rev) {
 		path.push_back(rev ? '<' : '>');
-		const std::string& nm = g.originalName.at(id);
-		if (nm.empty()) appendInt(path, id / 2);
-		else path += nm;
+		const std::string* nm = g.bigName(id);
+		if (!nm) appendInt(path, id / 2);
+		else path += *nm;
 	};
 	const bool firstMatch = characterMatch(t[0].seqChar, t[0].graphChar);
 	int currentEdit = merge ? MatchOrMismatch : (firstMatch ? Match : Mismatch);
 	uint64_t editLength = 1;
 	uint64_t mergedMatches = firstMatch ? 1 : 0, mergedMismatches = firstMatch ? 0 : 1;
 	addNode(curId, curRev);
-	pathLen += g.originalSize.at(curId);
+	pathLen += g.bigSize(curId);
 	for (size_t pos = 1; pos < n; pos++)
 	{
 		const TItem& p = t[pos - 1];
@@ -469,13 +469,13 @@ void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_
 			insideNode = newId == curId && newRev == curRev && newOff > curOff;
 			if (!insideNode)
 			{
-				const uint64_t skippedBefore = g.originalSize.at(curId) - 1 - p.nodeOffset;
+				const uint64_t skippedBefore = g.bigSize(curId) - 1 - p.nodeOffset;
 				curId = newId;
 				curRev = newRev;
 				curOff = newOff;
 				addNode(curId, curRev);
 				const uint64_t skippedAfter = c.nodeOffset;
-				pathLen += g.originalSize.at(curId) - (skippedBefore + skippedAfter);
+				pathLen += g.bigSize(curId) - (skippedBefore + skippedAfter);
 			}
 		}
 		int nextEdit;
@@ -509,7 +509,7 @@ void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_
 	const uint64_t mismatches = merge ? mergedMismatches : perType[Mismatch];
 	const uint64_t deletions = perType[Deletion], insertions = perType[Insertion];
 	const size_t cigarLen = (size_t)(cg - cigarBuf.data());
-	uint64_t pathEnd = pathLen - (g.originalSize.at((int)(int64_t)t[n - 1].node) - 1 - t[n - 1].nodeOffset);
+	uint64_t pathEnd = pathLen - (g.bigSize((int)(int64_t)t[n - 1].node) - 1 - t[n - 1].nodeOffset);
 	double total = (double)(matches + mismatches + deletions + insertions);
 	s.reserve(s.size() + cigarLen + path.size() + std::strlen(name) + 192);
 	s += name;
@@ -594,6 +594,10 @@ struct ReadState
 	// seeds extended in the current round (the picked one first) and the read's
 	// rounds so far
 	std::vector<Spec> specs;
+	// the extension jobs of specs in order (each spec's backward job, then its
+	// forward job), with the specs' job indexes local to this list until the
+	// round's launch list is put together
+	std::vector<ExtJob> jobs;
 	uint32_t roundsRun = 0;
 	// outcome records: cluster of every seed and the clusters' features
 	std::vector<uint32_t> clusterOf;
@@ -1707,6 +1711,34 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 							}
 						}
 						st.roundsRun++;
+						// the specs' extension jobs: from each seed backwards over the
+						// reverse-complemented read and forwards over the read
+						st.jobs.clear();
+						const uint64_t L = blens[active[a]];
+						const uint32_t r = active[a];
+						for (Spec& sp : st.specs)
+						{
+							const SeedHit& s = st.seeds[sp.seed];
+							int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
+							int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
+							if (s.seqPos > 0)
+							{
+								auto rp = g.reversePosition(forwardId, s.nodeOffset);
+								uint32_t node = g.unitigNode(backwardId, rp.second);
+								ExtJob j{nullptr, (uint32_t)s.seqPos, node, (uint32_t)(rp.second - g.nodeOff[node])};
+								j.devOffset = ctx.reverseBase() + boff[r] + (L - s.seqPos);
+								sp.bwdJob = (int64_t)st.jobs.size();
+								st.jobs.push_back(j);
+							}
+							if (s.seqPos < L - 1)
+							{
+								uint32_t node = g.unitigNode(forwardId, s.nodeOffset);
+								ExtJob j{nullptr, (uint32_t)(L - s.seqPos - 1), node, (uint32_t)(s.nodeOffset - g.nodeOff[node])};
+								j.devOffset = boff[r] + s.seqPos + 1;
+								sp.fwdJob = (int64_t)st.jobs.size();
+								st.jobs.push_back(j);
+							}
+						}
 					}
 					catch (const ReadAssert&)
 					{
@@ -1720,29 +1752,12 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 				{
 					ReadState& st = states[r];
 					if (st.done) continue;
-					const uint64_t L = blens[r];
+					const int64_t base = (int64_t)jobs.size();
+					jobs.insert(jobs.end(), st.jobs.begin(), st.jobs.end());
 					for (Spec& sp : st.specs)
 					{
-						const SeedHit& s = st.seeds[sp.seed];
-						int forwardId = s.reverse ? s.nodeID * 2 + 1 : s.nodeID * 2;
-						int backwardId = s.reverse ? s.nodeID * 2 : s.nodeID * 2 + 1;
-						if (s.seqPos > 0)
-						{
-							auto rp = g.reversePosition(forwardId, s.nodeOffset);
-							uint32_t node = g.unitigNode(backwardId, rp.second);
-							ExtJob j{nullptr, (uint32_t)s.seqPos, node, (uint32_t)(rp.second - g.nodeOff[node])};
-							j.devOffset = ctx.reverseBase() + boff[r] + (L - s.seqPos);
-							sp.bwdJob = jobs.size();
-							jobs.push_back(j);
-						}
-						if (s.seqPos < L - 1)
-						{
-							uint32_t node = g.unitigNode(forwardId, s.nodeOffset);
-							ExtJob j{nullptr, (uint32_t)(L - s.seqPos - 1), node, (uint32_t)(s.nodeOffset - g.nodeOff[node])};
-							j.devOffset = boff[r] + s.seqPos + 1;
-							sp.fwdJob = jobs.size();
-							jobs.push_back(j);
-						}
+						if (sp.bwdJob >= 0) sp.bwdJob += base;
+						if (sp.fwdJob >= 0) sp.fwdJob += base;
 					}
 					extending.push_back(r);
 				}

@@ -672,10 +672,22 @@ char GraphHost::nodeChar(uint32_t node, uint32_t pos) const
 // src/AlignmentGraph.cpp:812-828
 uint32_t GraphHost::unitigNode(int id, uint64_t offset) const
 {
-	const auto& nodes = lookup.at(id);
-	size_t index = nodes.size() * ((double)offset / (double)originalSize.at(id));
-	if (index >= nodes.size()) index = nodes.size() - 1;
-	while (index < nodes.size() - 1 && nodeOff[nodes[index]] + nodeLen[nodes[index]] <= offset) index++;
+	const uint32_t* nodes;
+	size_t count;
+	if ((size_t)id < sizeById.size() && sizeById[(size_t)id] != 0)
+	{
+		nodes = byIdNodes.data() + byIdStart[(size_t)id];
+		count = byIdStart[(size_t)id + 1] - byIdStart[(size_t)id];
+	}
+	else
+	{
+		const auto& v = lookup.at(id);
+		nodes = v.data();
+		count = v.size();
+	}
+	size_t index = count * ((double)offset / (double)bigSize(id));
+	if (index >= count) index = count - 1;
+	while (index < count - 1 && nodeOff[nodes[index]] + nodeLen[nodes[index]] <= offset) index++;
 	while (index > 0 && nodeOff[nodes[index]] > offset) index--;
 	return nodes[index];
 }
@@ -683,7 +695,7 @@ uint32_t GraphHost::unitigNode(int id, uint64_t offset) const
 // src/AlignmentGraph.cpp:830-847
 std::pair<int, uint64_t> GraphHost::reversePosition(int id, uint64_t offset) const
 {
-	uint64_t size = originalSize.at(id);
+	uint64_t size = bigSize(id);
 	int rev = (id % 2 == 0) ? (id / 2) * 2 + 1 : (id / 2) * 2;
 	return {rev, size - offset - 1};
 }
@@ -795,6 +807,34 @@ GraphHost loadGfaGraph(const std::string& path)
 	g.originalName = b.oname;
 	g.splitBigSize.resize(n);
 	for (size_t i = 0; i < n; i++) g.splitBigSize[i] = g.originalSize.at(g.nodeId[i]);
+	{
+		int64_t maxId = -1;
+		bool nonNegative = true;
+		for (const auto& p : g.originalSize)
+		{
+			maxId = std::max<int64_t>(maxId, p.first);
+			nonNegative = nonNegative && p.first >= 0;
+		}
+		if (nonNegative && maxId >= 0 && (uint64_t)maxId < 4 * (uint64_t)g.originalSize.size() + 64)
+		{
+			g.sizeById.assign((size_t)maxId + 1, 0);
+			g.nameById.assign((size_t)maxId + 1, 0);
+			g.byIdStart.assign((size_t)maxId + 2, 0);
+			for (const auto& p : g.originalSize) g.sizeById[(size_t)p.first] = p.second;
+			for (const auto& p : g.originalName)
+				if (p.first >= 0 && p.first <= maxId && !p.second.empty())
+				{
+					g.names.push_back(p.second);
+					g.nameById[(size_t)p.first] = (uint32_t)g.names.size();
+				}
+			for (int64_t id = 0; id <= maxId; id++)
+			{
+				auto it = g.lookup.find((int)id);
+				if (it != g.lookup.end()) g.byIdNodes.insert(g.byIdNodes.end(), it->second.begin(), it->second.end());
+				g.byIdStart[(size_t)id + 1] = (uint32_t)g.byIdNodes.size();
+			}
+		}
+	}
 	g.nodeRec.resize(n);
 	for (size_t i = 0; i < n; i++)
 	{

@@ -44,6 +44,31 @@ struct GraphHost
 	std::unordered_map<int, std::string> originalName;
 	std::vector<int> lookupOrder;                         // bigraph ids in the lookup map's iteration order
 	std::vector<uint64_t> splitBigSize;                   // per split node: length of its bigraph node
+	// the maps above as arrays indexed by bigraph id, filled when the ids are
+	// dense (below 4 x the number of bigraph nodes + 64), else empty: lengths (0
+	// for an id without a node), names (index + 1 into names, 0 for none or an
+	// unnamed node) and split-node lists (offsets into byIdNodes)
+	std::vector<uint64_t> sizeById;
+	std::vector<uint32_t> nameById;
+	std::vector<std::string> names;
+	std::vector<uint32_t> byIdStart, byIdNodes;
+	// originalSize.at(id), originalName.at(id) through the arrays when filled
+	uint64_t bigSize(int id) const
+	{
+		if ((size_t)id < sizeById.size() && sizeById[(size_t)id] != 0) return sizeById[(size_t)id];
+		return originalSize.at(id);
+	}
+	// the node's name, or null for a node named by its number
+	const std::string* bigName(int id) const
+	{
+		if ((size_t)id < nameById.size())
+		{
+			const uint32_t k = nameById[(size_t)id];
+			return k == 0 ? nullptr : &names[k - 1];
+		}
+		const std::string& nm = originalName.at(id);
+		return nm.empty() ? nullptr : &nm;
+	}
 	// the same per-node data as one record per node and the out-edges with their
 	// targets' components (gad::NodeRec, gad::OutEdge), for the device layout
 	std::vector<gad::NodeRec> nodeRec;
