@@ -49,6 +49,14 @@ BYTES_PER_WORD_STEP = 6
 # profiles/extend_vg5m.md; reported as such, not measured by this run
 MEASURED_TRAFFIC_PER_LAUNCH = 5.41e9
 TRAFFIC_SOURCE = "constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the default run, profiles/extend_vg5m.md"
+# The extension kernel runs one extension per single-lane workgroup, so its
+# word arithmetic and control flow issue on the scalar unit: one scalar
+# instruction per clock per CU, 256 CUs x 2.4 GHz. Scalar instructions per
+# forward word step of the default run (SQ_INSTS_SALU over its launches / its
+# word steps) are a constant measured offline, recorded in profiles/extend_vg5m.md
+SALU_PEAK_GIPS = 256 * 2.4
+SALU_PER_WORD_STEP = 158.8
+SALU_SOURCE = "constant: rocprofv3 --pmc SQ_INSTS_SALU of the default run, profiles/extend_vg5m.md"
 
 
 WORKLOADS = {
@@ -238,6 +246,8 @@ def main():
     # over the launch's mean duration (HIP events on the launching stream).
     achieved_gbs = BYTES_PER_WORD_STEP * tot["wordSteps"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
     valu_tops = OPS_PER_WORD_STEP * tot["wordSteps"] / kernel_s / 1e12 if kernel_s > 0 else 0.0
+    salu_gips = SALU_PER_WORD_STEP * tot["wordSteps"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    default_run = loop == "rounds" and not args.reads and args.workload == "vg5m"
     counts = np.bincount(status, minlength=5).tolist() if status is not None else []
 
     cpu = None
@@ -276,11 +286,14 @@ def main():
         "roofline": {"kernel": {"rounds": "extendSingleLaneKernel", "device": "readLoopSingleLaneKernel", "batched": "readLoopPersistentKernel"}[loop],
                      "bound": "valu-int32", "achieved": valu_tops,
                      "peak": VALU_INT32_PEAK_TOPS, "unit": "TOP/s", "frac": valu_tops / VALU_INT32_PEAK_TOPS,
-                     "traffic": None if loop != "rounds" or args.reads or args.workload != "vg5m" else MEASURED_TRAFFIC_PER_LAUNCH,
+                     "traffic": MEASURED_TRAFFIC_PER_LAUNCH if default_run else None,
                      "traffic_source": TRAFFIC_SOURCE, "mean_launch_ms": tot["extendKernelMs"] / launches,
                      "ops_per_word_step": OPS_PER_WORD_STEP,
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
-                             "bytes_per_word_step": BYTES_PER_WORD_STEP}},
+                             "bytes_per_word_step": BYTES_PER_WORD_STEP},
+                     # the ceiling that binds the single-lane kernel: scalar issue
+                     "salu": {"achieved": salu_gips, "peak": SALU_PEAK_GIPS, "unit": "G instr/s", "frac": salu_gips / SALU_PEAK_GIPS,
+                              "instr_per_word_step": SALU_PER_WORD_STEP, "source": SALU_SOURCE} if default_run else None},
         "cpu_baseline": cpu,
         "gaf_identical_to_cpu_sample": parity,
         "setup_s": {"generate": gen_s, "graph_load": load_s, "context": ctx_s, "index_build_gpu": index_s, "index_build_host": index_host_s,
