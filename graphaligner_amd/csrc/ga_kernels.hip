@@ -292,36 +292,82 @@ namespace gad {
 // (kmerReported), looks it up, and the workgroup writes the chunk's matches in
 // read order (ballot prefix counts within each wavefront, wavefront totals
 // through LDS), as lookupReadKmers writes them one by one.
+// base codes kept before each chunk of the read in seedLookupKernel: enough for
+// the k - 1 positions before a k-mer's end (k <= kLookupHalo + 1)
+constexpr uint32_t kLookupHalo = 32;
+
+// One read per 256-thread workgroup, one position per thread, matches compacted
+// in read order by ballot prefix counts. The rule of kmerReported (the k-mer
+// ending at each position, reported unless it lies inside a run of one base
+// longer than k off the run's window grid) evaluated from the chunk's base codes
+// staged in LDS: every position reads its k-mer window in a loop of k steps
+// without an early exit, and counts the run of its base within the window on the
+// way, so the wave's lanes stay together; only a run filling the whole window
+// continues backwards lane by lane, as kmerReported does.
 __global__ void __launch_bounds__(256) seedLookupKernel(const char* __restrict__ reads, const uint64_t* __restrict__ readOff, const uint32_t* __restrict__ readLen, uint32_t nreads, const SeedTableSlot* __restrict__ table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* __restrict__ out, uint32_t* __restrict__ outCount)
 {
 	__shared__ uint32_t waveHits[4];
 	__shared__ uint32_t written;
+	__shared__ int8_t codes[kLookupHalo + 256];
 	const uint32_t r = blockIdx.x;
 	if (r >= nreads) return;
 	const char* s = reads + readOff[r];
 	const uint64_t len = readLen[r];
 	SeedMatch* o = out + readOff[r];
 	const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint64_t realWindow = w - k + 1;
 	if (threadIdx.x == 0) written = 0;
-	__syncthreads();
 	for (uint64_t c = 0; c < len; c += 256)
 	{
+		// the chunk's codes and the kLookupHalo positions before it (-1: not a
+		// base, or before the read)
 		const uint64_t i = c + threadIdx.x;
+		codes[kLookupHalo + threadIdx.x] = i < len ? (int8_t)readBaseCode((unsigned char)s[i]) : (int8_t)-1;
+		if (threadIdx.x < kLookupHalo)
+		{
+			const int64_t j = (int64_t)c - (int64_t)kLookupHalo + (int64_t)threadIdx.x;
+			codes[threadIdx.x] = j >= 0 ? (int8_t)readBaseCode((unsigned char)s[j]) : (int8_t)-1;
+		}
+		__syncthreads();
 		bool hit = false;
 		SeedMatch m;
 		m.pos = m.start = m.count = 0;
-		if (i < len)
+		if (i < len && i + 1 >= k)
 		{
-			uint64_t kmer;
-			if (kmerReported(s, i, k, w, kmer))
+			const int32_t t = (int32_t)(kLookupHalo + threadIdx.x);
+			const int32_t first = t - (int32_t)k + 1;
+			const int32_t ci = codes[t];
+			uint64_t kmer = 0;
+			bool bad = false;
+			// run of ci ending at i inside the window: positions i, i - 1, ...
+			uint64_t run = 0;
+			bool same = true;
+			for (uint32_t q = 0; q < k; q++)
 			{
-				const SeedTableSlot* t = seedTableFind(table, mask, kmer);
-				if (t && t->count < maxCount)
+				const int32_t cq = codes[first + (int32_t)q];
+				bad |= cq < 0;
+				kmer = (kmer << 2) | (uint64_t)(cq & 3);
+				const int32_t cb = codes[t - (int32_t)q];
+				same = same && cb == ci;
+				run += same ? 1 : 0;
+			}
+			if (!bad)
+			{
+				uint64_t h = run;
+				// a run filling the window may continue before it
+				if (h == k)
+					while (h <= i && readBaseCode((unsigned char)s[i - h]) == ci) h++;
+				const bool reported = h <= k || (h - k) % realWindow == 0;
+				if (reported)
 				{
-					hit = true;
-					m.pos = (uint32_t)i;
-					m.start = t->start;
-					m.count = t->count;
+					const SeedTableSlot* ts = seedTableFind(table, mask, kmer);
+					if (ts && ts->count < maxCount)
+					{
+						hit = true;
+						m.pos = (uint32_t)i;
+						m.start = ts->start;
+						m.count = ts->count;
+					}
 				}
 			}
 		}
@@ -352,6 +398,8 @@ __global__ void revcompKernel(const char* __restrict__ reads, char* __restrict__
 
 hipError_t launchSeedLookup(hipStream_t stream, const char* reads, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, const SeedTableSlot* table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* out, uint32_t* outCount)
 {
+	// the k-mer window must lie in the chunk and its staged halo
+	if (k == 0 || k > kLookupHalo + 1 || w < k) return hipErrorInvalidValue;
 	hipLaunchKernelGGL(seedLookupKernel, dim3(nreads), dim3(256), 0, stream, reads, readOff, readLen, nreads, table, mask, k, w, maxCount, out, outCount);
 	return hipGetLastError();
 }

@@ -78,6 +78,24 @@ def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset, mode):
     assert gt == ot
 
 
+@pytest.mark.parametrize("mode", ["rounds_spec", "device_loop"])
+def test_gpu_gaf_cigar_match_mismatch_merge(ctx, workdir, mode):
+    """--cigar-match-mismatch through ga_params_t (src/Aligner.cpp:564): M runs in
+    the cigar, matches and mismatches still counted apart."""
+    gfa, names, seqs = workload("mini_sv", workdir)
+    sd, og, oi, gp, g, gi = _both(gfa, "vg")
+    _set_mode(gp, mode)
+    gp.cigarMatchMismatchMerge = 1
+    op = o.preset_params("vg")
+    op.cigarMatchMismatchMerge = 1
+    ot, ostat, _ = o.align_reads(og, oi, op, sd["density"], names, seqs, 8)
+    print(f"[gpu-align] gaf cigar merge {mode} reads={len(seqs)}", flush=True)
+    gt, gstat, _ = ga.align_reads(ctx, g, gi, gp, names, seqs)
+    assert list(gstat) == list(ostat)
+    assert "M" in ot
+    assert gt == ot
+
+
 @pytest.mark.parametrize("mode", sorted(MODES))
 def test_gpu_gaf_matches_oracle_vg100k(ctx, workdir, mode):
     gfa, names, seqs = workload("vg100k", workdir)
@@ -125,6 +143,33 @@ def test_gpu_seeds_of_single_base_runs(ctx, workdir, preset):
     for s, b in zip(reads, got):
         a = oi.seeds(s, sd["density"], True, cap_per_base=512)
         assert a.shape == b.shape and (a == b).all()
+
+
+@pytest.mark.parametrize("preset", ["vg", "dbg"])
+def test_gpu_seeds_of_runs_across_chunks(ctx, workdir, preset):
+    """The lookup stages each 256-position chunk of a read with the 32 positions
+    before it: single-base runs that end just after a chunk starts, straddle a
+    chunk boundary, or fill a whole k-mer window across it."""
+    import random
+    from test_pipeline_parity import _homopolymer_workload
+    gfa, _ = _homopolymer_workload(workdir)
+    sd, og, oi, gp, g, gi = _both(gfa, preset)
+    rng = random.Random(23)
+    reads = []
+    for start in (200, 226, 240, 250, 255, 256, 257, 270, 480, 500, 511):
+        for run in (14, 16, 20, 33, 40, 70):
+            base = rng.choice("ACGT")
+            left = "".join(rng.choice("ACGT") for _ in range(start))
+            right = "".join(rng.choice("ACGT") for _ in range(rng.randrange(40, 300)))
+            reads.append(left + base * run + right)
+        reads.append("".join(rng.choice("ACGT") for _ in range(start)) + "T" * 20 + "N" + "T" * 30 + "ACGTTGCA" * 30)
+    print(f"[gpu-align] runs across chunks {preset} reads={len(reads)}", flush=True)
+    got = ga.seeds(ctx, g, gi, reads, sd["density"], order=True)
+    for s, b in zip(reads, got):
+        a = oi.seeds(s, sd["density"], True, cap_per_base=512)
+        assert a.shape == b.shape and (a == b).all()
+    # the runs' k-mers are in this graph's index: thousands of seeds end in them
+    assert sum(len(b) for b in got) > 3000
 
 
 @pytest.mark.parametrize("mode", sorted(MODES))
