@@ -221,12 +221,29 @@ hipError_t launchReadLoops(hipStream_t stream, uint32_t njobs, uint32_t lanesPer
 	return hipGetLastError();
 }
 
-// One thread per 64-position block of the byte string.
-__global__ void eqPlanesKernel(const char* __restrict__ bytes, uint64_t n, uint64_t blocks, uint64_t* __restrict__ planes)
+// planeBlock with one lane per byte: each wavefront walks 64-position blocks
+// (block, block + waves, ...), its byte loads contiguous, and a block's five
+// words are the ballots of the lanes' letter sets (readBaseSet for all 256 byte
+// values in LDS; positions at or past n set no bit).
+__global__ void __launch_bounds__(256) eqPlanesKernel(const char* __restrict__ bytes, uint64_t n, uint64_t blocks, uint64_t* __restrict__ planes)
 {
-	const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (b >= blocks) return;
-	planeBlock(bytes, n, b, planes + b * kPlaneWords);
+	__shared__ uint8_t sets[256];
+	sets[threadIdx.x] = (uint8_t)readBaseSet((unsigned char)threadIdx.x);
+	__syncthreads();
+	const uint32_t lane = threadIdx.x & 63;
+	const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+	for (uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; b < blocks; b += waves)
+	{
+		const uint64_t x = b * 64 + lane;
+		const uint32_t set = x < n ? sets[(unsigned char)bytes[x]] : 0;
+		const uint64_t m0 = __ballot((set & 1) != 0);
+		const uint64_t m1 = __ballot((set & 2) != 0);
+		const uint64_t m2 = __ballot((set & 4) != 0);
+		const uint64_t m3 = __ballot((set & 8) != 0);
+		const uint64_t m4 = __ballot(set == 16);
+		if (lane < kPlaneWords)
+			planes[b * kPlaneWords + lane] = lane == 0 ? m0 : lane == 1 ? m1 : lane == 2 ? m2 : lane == 3 ? m3 : m4;
+	}
 }
 
 uint64_t eqPlaneWords(uint64_t n) { return planeBlockCount(n) * kPlaneWords; }
@@ -234,8 +251,9 @@ uint64_t eqPlaneWords(uint64_t n) { return planeBlockCount(n) * kPlaneWords; }
 hipError_t launchEqPlanes(hipStream_t stream, const char* bytes, uint64_t n, uint64_t* planes)
 {
 	const uint64_t blocks = planeBlockCount(n);
-	const uint64_t grid = (blocks + 255) / 256;
-	if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+	// four waves per 256-thread workgroup, at most 16 workgroups per CU's worth
+	// of the 256 CUs; each wave walks its share of the 64-position blocks
+	const uint64_t grid = (blocks + 3) / 4 < 256 * 16 ? (blocks + 3) / 4 : 256 * 16;
 	hipLaunchKernelGGL(eqPlanesKernel, dim3((uint32_t)grid), dim3(256), 0, stream, bytes, n, blocks, planes);
 	return hipGetLastError();
 }
@@ -385,15 +403,19 @@ __global__ void __launch_bounds__(256) seedLookupKernel(const char* __restrict__
 	if (threadIdx.x == 0) outCount[r] = written;
 }
 
-// Reverse complement of every read, one block per read.
-__global__ void revcompKernel(const char* __restrict__ reads, char* __restrict__ rev, const uint64_t* __restrict__ readOff, const uint32_t* __restrict__ readLen, uint32_t nreads)
+// Reverse complement of every read, one block per read, through a table of
+// complementByte for all 256 byte values in LDS (no per-letter branches).
+__global__ void __launch_bounds__(256) revcompKernel(const char* __restrict__ reads, char* __restrict__ rev, const uint64_t* __restrict__ readOff, const uint32_t* __restrict__ readLen, uint32_t nreads)
 {
+	__shared__ char comp[256];
+	comp[threadIdx.x] = complementByte((char)threadIdx.x);
+	__syncthreads();
 	uint32_t r = blockIdx.x;
 	if (r >= nreads) return;
 	uint64_t len = readLen[r];
 	const char* s = reads + readOff[r];
 	char* d = rev + readOff[r];
-	for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) d[i] = complementByte(s[len - 1 - i]);
+	for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) d[i] = comp[(unsigned char)s[len - 1 - i]];
 }
 
 hipError_t launchSeedLookup(hipStream_t stream, const char* reads, const uint64_t* readOff, const uint32_t* readLen, uint32_t nreads, const SeedTableSlot* table, uint64_t mask, uint32_t k, uint32_t w, uint64_t maxCount, SeedMatch* out, uint32_t* outCount)
