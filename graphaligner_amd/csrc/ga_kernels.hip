@@ -303,19 +303,15 @@ hipError_t launchTraceCodes(hipStream_t stream, const DevGraph& g, const uint64_
 
 namespace gad {
 
-// One thread per read (reads are independent and their lookups are short,
-// irregular hash probes; the table stays in L2/HBM).
-// One workgroup of 256 threads per read. Positions go in chunks of 256, one per
-// thread: each thread decides whether iterateKmers reports its k-mer
-// (kmerReported), looks it up, and the workgroup writes the chunk's matches in
-// read order (ballot prefix counts within each wavefront, wavefront totals
-// through LDS), as lookupReadKmers writes them one by one.
 // base codes kept before each chunk of the read in seedLookupKernel: enough for
 // the k - 1 positions before a k-mer's end (k <= kLookupHalo + 1)
 constexpr uint32_t kLookupHalo = 32;
 
-// One read per 256-thread workgroup, one position per thread, matches compacted
-// in read order by ballot prefix counts. The rule of kmerReported (the k-mer
+// One read per 256-thread workgroup. Positions go in chunks of 256, one per
+// thread: each thread decides whether iterateKmers reports its k-mer, looks it
+// up, and the workgroup writes the chunk's matches in read order (ballot prefix
+// counts within each wavefront, wavefront totals through LDS), as
+// lookupReadKmers writes them one by one. The rule of kmerReported (the k-mer
 // ending at each position, reported unless it lies inside a run of one base
 // longer than k off the run's window grid) evaluated from the chunk's base codes
 // staged in LDS: every position reads its k-mer window in a loop of k steps
