@@ -429,13 +429,13 @@ void appendGafLine(std::string& s, const GraphHost& g, const char* name, uint64_
 	static const char ops[] = {'=', 'X', 'M', 'I', 'D'};
 	const TItem* t = a.trace.data();
 	const size_t n = a.trace.size();
-	// scratch kept per thread: the line's parts are built, then appended. Every
-	// cigar run covers at least one item and takes at most 21 characters, so the
-	// cigar of n items fits in 21 n bytes.
+	// scratch kept per thread: the line's parts are built, then appended. A cigar
+	// run of L >= 1 items takes 1 + digits(L) <= 2 L characters, so the cigar of
+	// n items fits in 2 n bytes.
 	thread_local std::string path;
 	thread_local std::vector<char> cigarBuf;
 	path.clear();
-	if (cigarBuf.size() < 21 * n + 32) cigarBuf.resize(21 * n + 32);
+	if (cigarBuf.size() < 2 * n + 32) cigarBuf.resize(2 * n + 32);
 	char* cg = cigarBuf.data();
 	// items per edit type; without merge every item's type is its run's, so the
 	// counts are added per run, with merge matches and mismatches per item
