@@ -233,51 +233,36 @@ int32_t editTerm(const TItem* p, const TItem& t)
 	return characterMatch(t.seqChar, t.graphChar) ? 0 : 1;
 }
 
-// GraphAligner::exactAlignmentPart (src/GraphAligner.h:403-457)
+// GraphAligner::exactAlignmentPart (src/GraphAligner.h:403-457): whether the
+// alignment's trace passes through the seed's node and offset at its read
+// position. A trace's read positions never decrease and rise by at most one per
+// item (the step rule buildAlignment checks on every item), so each position from
+// the first to the last is held by one contiguous group of items. The
+// reference's bisection finds an item of the seed position's group and scans the
+// group, and its range checks cannot fail on such a trace; the answer is whether
+// an item of the group has the seed's node and offset. The group's first item
+// lies in [p - first, p - first + z], z the trace's items that do not advance the
+// read, so it is bisected there.
 bool exactAlignmentPart(const Alignment& aln, const SeedHit& seed)
 {
 	const std::vector<TItem>& trace = aln.trace;
 	check(trace.back().seqPos > trace[0].seqPos);
 	if (trace.back().seqPos < seed.seqPos) return false;
 	if (trace[0].seqPos > seed.seqPos) return false;
-	size_t high = trace.size();
-	size_t low = 0;
-	size_t mid = (seed.seqPos - trace[0].seqPos) / (trace.back().seqPos - trace[0].seqPos);
-	auto at = [&](size_t i) -> const TItem& {
-		check(i < trace.size());
-		return trace[i];
-	};
-	while (at(mid).seqPos != seed.seqPos)
+	const size_t n = trace.size();
+	const uint64_t target = seed.seqPos, front = trace[0].seqPos, back = trace.back().seqPos;
+	const uint64_t still = (uint64_t)(n - 1) - (back - front);
+	size_t lo = (size_t)(target - front);
+	size_t hi = (size_t)std::min<uint64_t>(n - 1, target - front + still);
+	while (lo < hi)
 	{
-		if (at(mid).seqPos < seed.seqPos)
-		{
-			low = mid;
-			mid = (high + low) / 2;
-			if (mid == low) mid += 1;
-		}
-		if (at(mid).seqPos > seed.seqPos)
-		{
-			high = mid;
-			mid = (high + low) / 2;
-		}
-		check(low < mid);
-		check(mid < high);
+		const size_t mid = lo + (hi - lo) / 2;
+		if (trace[mid].seqPos < target) lo = mid + 1;
+		else hi = mid;
 	}
-	uint64_t compareNode = (uint64_t)(int64_t)(seed.nodeID * 2 + (seed.reverse ? 1 : 0));
-	size_t down = mid;
-	while (trace[down].seqPos == seed.seqPos)
-	{
-		if (compareNode == trace[down].node && seed.nodeOffset == trace[down].nodeOffset) return true;
-		if (down == 0) break;
-		down -= 1;
-	}
-	size_t up = mid;
-	while (trace[up].seqPos == seed.seqPos)
-	{
-		if (compareNode == trace[up].node && seed.nodeOffset == trace[up].nodeOffset) return true;
-		up += 1;
-		if (up == trace.size()) break;
-	}
+	const uint64_t compareNode = (uint64_t)(int64_t)(seed.nodeID * 2 + (seed.reverse ? 1 : 0));
+	for (size_t i = lo; i < n && trace[i].seqPos == target; i++)
+		if (compareNode == trace[i].node && seed.nodeOffset == trace[i].nodeOffset) return true;
 	return false;
 }
 
