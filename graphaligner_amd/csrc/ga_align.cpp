@@ -640,9 +640,16 @@ namespace {
 // reference's assertions as ReadAssert.
 void orderSeedsChecked(const GraphHost& g, std::vector<SeedHit>& seeds)
 {
-	// seeds grouped by chain, each with its approximate diagonal; the chains are
-	// independent, so the container's order does not matter
-	std::map<uint64_t, std::vector<std::pair<uint64_t, uint64_t>>> byChain;
+	// seeds keyed by chain and approximate diagonal, sorted by both: each chain's
+	// seeds are then consecutive in diagonal order. Clusters (diagonals within 100
+	// of the previous one) and their covered bases do not depend on the order of
+	// equal keys, and a failing check drops the read whichever fails first.
+	struct ChainSeed
+	{
+		uint64_t chain, diagonal;
+		uint32_t seed;
+	};
+	std::vector<ChainSeed> keys(seeds.size());
 	for (size_t i = 0; i < seeds.size(); i++)
 	{
 		uint64_t node = seeds[i].alignmentGraphNodeId;
@@ -657,40 +664,44 @@ void orderSeedsChecked(const GraphHost& g, std::vector<SeedHit>& seeds)
 			off = seeds[i].nodeOffset - g.nodeOff[node];
 		}
 		check(g.chainPos[node] + off >= seeds[i].seqPos);
-		byChain[g.chain[node]].emplace_back(i, g.chainPos[node] + off - seeds[i].seqPos);
+		keys[i] = ChainSeed{g.chain[node], g.chainPos[node] + off - seeds[i].seqPos, (uint32_t)i};
 	}
-	for (auto& c : byChain)
+	std::sort(keys.begin(), keys.end(), [](const ChainSeed& l, const ChainSeed& r) { return l.chain != r.chain ? l.chain < r.chain : l.diagonal < r.diagonal; });
+	size_t clusterStart = 0;
+	for (size_t i = 1; i <= keys.size(); i++)
 	{
-		auto& v = c.second;
-		std::sort(v.begin(), v.end(), [](const std::pair<uint64_t, uint64_t>& l, const std::pair<uint64_t, uint64_t>& r) { return l.second < r.second; });
-		size_t clusterStart = 0;
-		for (size_t i = 1; i <= v.size(); i++)
+		if (i < keys.size() && keys[i].chain == keys[i - 1].chain && keys[i].diagonal <= keys[i - 1].diagonal + 100) continue;
+		check(i > clusterStart);
+		std::sort(keys.begin() + clusterStart, keys.begin() + i, [&seeds](const ChainSeed& l, const ChainSeed& r) { return seeds[l.seed].seqPos < seeds[r.seed].seqPos; });
+		uint64_t matchingBps = 0;
+		int lastEnd = INT_MIN;
+		for (size_t k = clusterStart; k < i; k++)
 		{
-			if (i < v.size() && v[i].second <= v[i - 1].second + 100) continue;
-			check(i > clusterStart);
-			std::sort(v.begin() + clusterStart, v.begin() + i, [&seeds](const std::pair<uint64_t, uint64_t>& l, const std::pair<uint64_t, uint64_t>& r) { return seeds[l.first].seqPos < seeds[r.first].seqPos; });
-			uint64_t matchingBps = 0;
-			int lastEnd = INT_MIN;
-			for (size_t k = clusterStart; k < i; k++)
-			{
-				const SeedHit& s = seeds[v[k].first];
-				int thisStart = (int)s.seqPos - (int)s.matchLen + 1;
-				int thisEnd = (int)s.seqPos;
-				check(thisEnd >= lastEnd);
-				check(thisEnd > thisStart);
-				matchingBps += (thisEnd - std::max(thisStart, lastEnd));
-				lastEnd = thisEnd;
-			}
-			for (size_t k = clusterStart; k < i; k++)
-			{
-				seeds[v[k].first].seedGoodness = matchingBps + seeds[v[k].first].rawSeedGoodness;
-				seeds[v[k].first].seedClusterSize = i - clusterStart;
-			}
-			clusterStart = i;
+			const SeedHit& s = seeds[keys[k].seed];
+			int thisStart = (int)s.seqPos - (int)s.matchLen + 1;
+			int thisEnd = (int)s.seqPos;
+			check(thisEnd >= lastEnd);
+			check(thisEnd > thisStart);
+			matchingBps += (thisEnd - std::max(thisStart, lastEnd));
+			lastEnd = thisEnd;
 		}
+		for (size_t k = clusterStart; k < i; k++)
+		{
+			seeds[keys[k].seed].seedGoodness = matchingBps + seeds[keys[k].seed].rawSeedGoodness;
+			seeds[keys[k].seed].seedClusterSize = i - clusterStart;
+		}
+		clusterStart = i;
 	}
-	std::sort(seeds.begin(), seeds.end(), [](const SeedHit& l, const SeedHit& r) { return l.seedGoodness < r.seedGoodness; });
-	std::reverse(seeds.begin(), seeds.end());
+	// the reference's std::sort by goodness, then its reverse: std::sort's steps
+	// depend only on the comparisons, so sorting (goodness, index) pairs with the
+	// same comparator gives the same permutation of the seeds, ties included,
+	// while moving 16-byte pairs instead of whole seeds
+	std::vector<std::pair<uint64_t, uint32_t>> byGoodness(seeds.size());
+	for (size_t i = 0; i < seeds.size(); i++) byGoodness[i] = {seeds[i].seedGoodness, (uint32_t)i};
+	std::sort(byGoodness.begin(), byGoodness.end(), [](const std::pair<uint64_t, uint32_t>& l, const std::pair<uint64_t, uint32_t>& r) { return l.first < r.first; });
+	std::vector<SeedHit> ordered(seeds.size());
+	for (size_t j = 0; j < seeds.size(); j++) ordered[j] = seeds[byGoodness[seeds.size() - 1 - j].second];
+	seeds.swap(ordered);
 }
 
 } // namespace
