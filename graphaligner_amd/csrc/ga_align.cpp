@@ -233,6 +233,24 @@ int32_t editTerm(const TItem* p, const TItem& t)
 	return characterMatch(t.seqChar, t.graphChar) ? 0 : 1;
 }
 
+// AlignmentSelection::alignmentIncompatible (src/AlignmentSelection.cpp:13-30)
+bool alignmentIncompatible(const Alignment& left, const Alignment& right)
+{
+	const float cutoff = 0.05;
+	auto minOverlapLen = std::min(left.alignmentEnd - left.alignmentStart, right.alignmentEnd - right.alignmentStart) * cutoff;
+	uint64_t ls = left.alignmentStart, le = left.alignmentEnd, rs = right.alignmentStart, re = right.alignmentEnd;
+	if (ls > rs)
+	{
+		std::swap(ls, rs);
+		std::swap(le, re);
+	}
+	int overlap = 0;
+	if (le > rs) overlap = (int)(le - rs);
+	return overlap > minOverlapLen;
+}
+
+} // namespace
+
 // GraphAligner::exactAlignmentPart (src/GraphAligner.h:403-457): whether the
 // alignment's trace passes through the seed's node and offset at its read
 // position. A trace's read positions never decrease and rise by at most one per
@@ -266,23 +284,6 @@ bool exactAlignmentPart(const Alignment& aln, const SeedHit& seed)
 	return false;
 }
 
-// AlignmentSelection::alignmentIncompatible (src/AlignmentSelection.cpp:13-30)
-bool alignmentIncompatible(const Alignment& left, const Alignment& right)
-{
-	const float cutoff = 0.05;
-	auto minOverlapLen = std::min(left.alignmentEnd - left.alignmentStart, right.alignmentEnd - right.alignmentStart) * cutoff;
-	uint64_t ls = left.alignmentStart, le = left.alignmentEnd, rs = right.alignmentStart, re = right.alignmentEnd;
-	if (ls > rs)
-	{
-		std::swap(ls, rs);
-		std::swap(le, re);
-	}
-	int overlap = 0;
-	if (le > rs) overlap = (int)(le - rs);
-	return overlap > minOverlapLen;
-}
-
-} // namespace
 
 // AlignmentSelection::SelectAlignments (src/AlignmentSelection.cpp:31-53, 65-95).
 // The score threshold passes through SelectionOptions' int field

@@ -65,6 +65,13 @@ struct Alignment
 	uint64_t alignmentLength() const { return alignmentEnd - alignmentStart; }
 };
 
+// GraphAligner::exactAlignmentPart (src/GraphAligner.h:403-457) on a trace built
+// by the rebuild (read positions rising by at most one per item): whether it
+// passes through the seed's node and offset at the seed's read position. Throws
+// the rebuild's read-drop exception when the trace's last position does not
+// exceed its first.
+bool exactAlignmentPart(const Alignment& aln, const SeedHit& seed);
+
 struct SeedProposal;
 struct ClusterRecords;
 

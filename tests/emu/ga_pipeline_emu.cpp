@@ -790,6 +790,124 @@ void pipe_proposal_stats(int64_t* out)
 	out[2] = (int64_t)g_lastStats.prunedSeeds;
 }
 
+namespace {
+// GraphAligner::exactAlignmentPart (src/GraphAligner.h:403-457) as the reference
+// writes it, with its assertions: 0 or 1, or 2 when an assertion fires
+int exactPartBisection(const std::vector<ga::TItem>& trace, const ga::SeedHit& seed)
+{
+	if (!(trace.back().seqPos > trace[0].seqPos)) return 2;
+	if (trace.back().seqPos < seed.seqPos) return 0;
+	if (trace[0].seqPos > seed.seqPos) return 0;
+	size_t high = trace.size();
+	size_t low = 0;
+	size_t mid = (seed.seqPos - trace[0].seqPos) / (trace.back().seqPos - trace[0].seqPos);
+	while (true)
+	{
+		if (mid >= trace.size()) return 2;
+		if (trace[mid].seqPos == seed.seqPos) break;
+		if (trace[mid].seqPos < seed.seqPos)
+		{
+			low = mid;
+			mid = (high + low) / 2;
+			if (mid == low) mid += 1;
+		}
+		if (mid >= trace.size()) return 2;
+		if (trace[mid].seqPos > seed.seqPos)
+		{
+			high = mid;
+			mid = (high + low) / 2;
+		}
+		if (!(low < mid) || !(mid < high)) return 2;
+	}
+	const uint64_t compareNode = (uint64_t)(int64_t)(seed.nodeID * 2 + (seed.reverse ? 1 : 0));
+	size_t down = mid;
+	while (trace[down].seqPos == seed.seqPos)
+	{
+		if (compareNode == trace[down].node && seed.nodeOffset == trace[down].nodeOffset) return 1;
+		if (down == 0) break;
+		down -= 1;
+	}
+	size_t up = mid;
+	while (trace[up].seqPos == seed.seqPos)
+	{
+		if (compareNode == trace[up].node && seed.nodeOffset == trace[up].nodeOffset) return 1;
+		up += 1;
+		if (up == trace.size()) break;
+	}
+	return 0;
+}
+} // namespace
+
+// The product's exactAlignmentPart against the reference's bisection above on
+// random traces whose read positions rise by zero or one per item (as the trace
+// rebuild makes them), seeds on and off the traces: the number of disagreements
+// and, in *bisectionAsserts, how often the bisection's assertions fired.
+int64_t pipe_exact_part_check(uint64_t seed, int64_t trials, int64_t* bisectionAsserts, int64_t* found)
+{
+	std::mt19937_64 rng(seed);
+	int64_t bad = 0, asserts = 0, hits = 0;
+	for (int64_t t = 0; t < trials; t++)
+	{
+		ga::Alignment a;
+		// every tenth trace: one to three items on one read position, which the
+		// reference's first assertion rejects
+		const bool flat = t % 10 == 0;
+		const size_t n = flat ? 1 + rng() % 3 : 1 + rng() % 3000;
+		const uint32_t stillEvery = flat ? 1 : 2 + (uint32_t)(rng() % 12);
+		uint32_t pos = (uint32_t)(rng() % 1000);
+		for (size_t i = 0; i < n; i++)
+		{
+			ga::TItem it;
+			it.node = (uint32_t)(rng() % 6);
+			it.nodeOffset = (uint32_t)(rng() % 4);
+			it.seqPos = pos;
+			it.nodeSwitch = false;
+			it.seqChar = 'A';
+			it.graphChar = 'A';
+			a.trace.push_back(it);
+			if (rng() % stillEvery != 0) pos++;
+		}
+		const uint64_t front = a.trace[0].seqPos, back = a.trace.back().seqPos;
+		for (int q = 0; q < 40; q++)
+		{
+			ga::SeedHit s;
+			s.seqPos = front + (rng() % (back - front + 7)) - 3;
+			if (s.seqPos > back + 3) s.seqPos = front;
+			uint64_t node = rng() % 6, off = rng() % 4;
+			if (rng() % 2)
+			{
+				// an item at the seed's position, when there is one
+				for (const auto& it : a.trace)
+					if (it.seqPos == s.seqPos)
+					{
+						node = it.node;
+						off = it.nodeOffset;
+						if (rng() % 3 == 0) break;
+					}
+			}
+			s.nodeID = (int)(node / 2);
+			s.reverse = node % 2 == 1;
+			s.nodeOffset = off;
+			const int want = exactPartBisection(a.trace, s);
+			int got;
+			try
+			{
+				got = ga::exactAlignmentPart(a, s) ? 1 : 0;
+			}
+			catch (...)
+			{
+				got = 2;
+			}
+			if (got != want) bad++;
+			if (want == 2) asserts++;
+			if (want == 1) hits++;
+		}
+	}
+	if (bisectionAsserts) *bisectionAsserts = asserts;
+	if (found) *found = hits;
+	return bad;
+}
+
 // host phase times of the last pipe_align_reads (ms): seeding, seed picking,
 // trace rebuild and loop replay, selection and GAF, all host work, whole call
 void pipe_host_phase_ms(double* out)
