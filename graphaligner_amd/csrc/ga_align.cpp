@@ -114,6 +114,25 @@ char complementOf(char c)
 	return 0;
 }
 
+// whether complementOf knows each byte value, for the per-read letter check
+struct ComplementTable
+{
+	uint8_t known[256];
+	ComplementTable()
+	{
+		for (int c = 0; c < 256; c++) known[c] = complementOf((char)c) != 0 ? 1 : 0;
+	}
+};
+const ComplementTable kComplementable;
+
+// every letter of s[0, n) known to complementOf (a branch-free pass over the read)
+bool allComplementable(const char* s, uint64_t n)
+{
+	uint8_t all = 1;
+	for (uint64_t k = 0; k < n; k++) all &= kComplementable.known[(unsigned char)s[k]];
+	return all != 0;
+}
+
 // Runs fn(0) .. fn(n - 1) on up to `threads` threads. An exception thrown by fn
 // stops the hand-out of further indices; every worker is joined and the first
 // exception is rethrown on the calling thread (so callers' handlers see it as in
@@ -609,6 +628,18 @@ std::vector<SeedHit> seedsFromMatches(const GraphHost& g, const MinimizerIndex& 
 	uint64_t seedsHere = 0, allowedCount = 0;
 	for (const auto& m : order)
 		if ((uint64_t)m.start + m.count > idx.positions.size()) throw std::runtime_error("seed match outside the index's position array");
+	{
+		// the seeds the loop below takes, counted first so the result is allocated once
+		uint64_t total = 0, here = 0, allowed = 0;
+		for (const auto& m : order)
+		{
+			if (here >= maxHits && m.count > allowed) break;
+			allowed = m.count;
+			total += m.count;
+			here += m.count;
+		}
+		result.reserve(total);
+	}
 	for (const auto& m : order)
 	{
 		uint64_t count = m.count;
@@ -1076,7 +1107,7 @@ void alignResidentDeviceLoop(ExtendBackend& ctx, const GraphHost& g, const Minim
 					orderSeedsChecked(g, seeds[i]);
 				}
 				// ReverseComplement of the read asserts on letters it does not know
-				for (uint32_t k = 0; k < L; k++) check(complementOf(seq[k]) != 0);
+				check(allComplementable(seq, L));
 				ready[i] = 1;
 			}
 			catch (const ReadAssert&)
@@ -1435,7 +1466,7 @@ void runRounds(ExtendBackend& ctx, const GraphHost& g, const MinimizerIndex* idx
 					orderSeedsChecked(g, st.seeds);
 				}
 				// ReverseComplement of the read asserts on letters it does not know
-				for (uint32_t k = 0; k < blens[i]; k++) check(complementOf(bseqs[i][k]) != 0);
+				check(allComplementable(bseqs[i], blens[i]));
 				st.extendSeeds = opt.seedExtendDensity == -1 ? st.seeds.size() : (uint64_t)(opt.seedExtendDensity * blens[i] + 1);
 			}
 			catch (const ReadAssert&)
