@@ -52,6 +52,44 @@ def test_gpu_seeds_match_oracle(ctx, workdir, wl, preset):
         assert a.shape == b.shape and (a == b).all()
 
 
+@pytest.mark.parametrize("k", [5, 8, 15, 31])
+def test_gpu_seeds_match_oracle_k_w(ctx, workdir, k):
+    """Seeds at minimizer lengths beyond the presets' (5 to 31, the longest the
+    index takes) with the window equal to k and no k-mers discarded, so that every
+    k-mer is a minimizer and a run of one base longer than the window is reported
+    at every position (src/MinimizerSeeder.cpp:105-189 through getSeeds); at
+    k = 31 the lookup kernel's k-mer window reaches the start of its staged halo.
+    (Wider windows leave this small graph's maxCount at 0 and no seeds.)"""
+    w = k
+    gfa, names, seqs = workload("mini_sv", workdir)
+    sd = o.PRESET_SEEDING["vg"]
+    og = o.Graph(gfa)
+    oi = o.Index(og, k, w, 0.0)
+    gp = ga.params_preset("vg")
+    gp.minimizerLength, gp.minimizerWindowSize = k, w
+    gp.minimizerDiscardMostNumerousFraction = 0.0
+    g = ga.Graph(gfa)
+    gi = ga.Index(g, gp)
+    assert gi.max_count == oi.max_count
+    en, es = edge_reads(names, seqs)
+    s0 = seqs[0]
+    runs = s0[:300] + "A" * 70 + s0[300:600] + "C" * 33 + s0[600:900] + "G" * 31 + s0[900:1200] + "T" * 2 + s0[1200:1500]
+    reads = list(seqs[:12]) + es + [runs]
+    print(f"[gpu-align] seeds mini_sv k={k} w={w} reads={len(reads)}", flush=True)
+    got = ga.seeds(ctx, g, gi, reads, sd["density"], order=True)
+    assert len(got) == len(reads)
+    nonempty = 0
+    for s, b in zip(reads, got):
+        a = oi.seeds(s, sd["density"], True)
+        if a is None:
+            # OrderSeeds fired a reference assertion: the read gets no seeds
+            assert len(b) == 0
+            continue
+        assert a.shape == b.shape and (a == b).all()
+        nonempty += len(a) > 0
+    assert nonempty >= 15
+
+
 # execution modes of the seed loop: host-driven rounds one seed at a time, rounds
 # with speculative extension of later seeds, the whole loop on the device
 MODES = {"rounds": (0, 1), "rounds_spec": (0, 64), "device_loop": (1, 1)}
