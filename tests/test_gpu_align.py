@@ -59,7 +59,8 @@ def test_gpu_seeds_match_oracle_k_w(ctx, workdir, k):
     k-mer is a minimizer and a run of one base longer than the window is reported
     at every position (src/MinimizerSeeder.cpp:105-189 through getSeeds); at
     k = 31 the lookup kernel's k-mer window reaches the start of its staged halo.
-    (Wider windows leave this small graph's maxCount at 0 and no seeds.)"""
+    (Wider windows leave this small graph's maxCount at 0 and no seeds.) The
+    seeds expanded and ordered on the GPU equal the host's."""
     w = k
     gfa, names, seqs = workload("mini_sv", workdir)
     sd = o.PRESET_SEEDING["vg"]
@@ -88,6 +89,9 @@ def test_gpu_seeds_match_oracle_k_w(ctx, workdir, k):
         assert a.shape == b.shape and (a == b).all()
         nonempty += len(a) > 0
     assert nonempty >= 15
+    # expansion and OrderSeeds on the GPU (order 2) give the same seeds
+    dev = ga.seeds(ctx, g, gi, reads, sd["density"], order=2)
+    assert all(x.shape == y.shape and (x == y).all() for x, y in zip(got, dev))
 
 
 # execution modes of the seed loop: host-driven rounds one seed at a time, rounds
