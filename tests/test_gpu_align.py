@@ -120,6 +120,39 @@ def test_gpu_gaf_matches_oracle(ctx, workdir, wl, preset, mode):
     assert gt == ot
 
 
+@pytest.mark.parametrize("mode", sorted(MODES))
+@pytest.mark.parametrize("k", [8, 15, 31])
+def test_gpu_gaf_matches_oracle_k_w(ctx, workdir, k, mode):
+    """Whole reads at minimizer lengths beyond the presets' (window equal to k, no
+    k-mers discarded): at k = 8 thousands of seeds per read and several kept
+    alignments per read, at k = 31 reads left without seeds."""
+    if k == 8 and mode == "device_loop":
+        pytest.skip("the device seed loop runs a read's seeds one after another on one lane, each "
+                    "tested against every kept alignment: at k = 8 (268,498 seeds, ~20,000 extensions "
+                    "on 50 reads) that is minutes of serial work per read; its host build gives the "
+                    "oracle's GAF on this input in test_pipeline_parity.py::test_gaf_identical_dense_seeds")
+    gfa, names, seqs = workload("mini_sv", workdir)
+    sd = o.PRESET_SEEDING["vg"]
+    og = o.Graph(gfa)
+    oi = o.Index(og, k, k, 0.0)
+    gp = ga.params_preset("vg")
+    gp.minimizerLength, gp.minimizerWindowSize = k, k
+    gp.minimizerDiscardMostNumerousFraction = 0.0
+    _set_mode(gp, mode)
+    g = ga.Graph(gfa)
+    gi = ga.Index(g, gp)
+    assert gi.max_count == oi.max_count
+    en, es = edge_reads(names, seqs)
+    names = list(names) + en
+    seqs = list(seqs) + es
+    ot, ostat, _ = o.align_reads(og, oi, o.preset_params("vg"), sd["density"], names, seqs, 8)
+    print(f"[gpu-align] gaf mini_sv k={k} {mode} reads={len(seqs)}", flush=True)
+    gt, gstat, _ = ga.align_reads(ctx, g, gi, gp, names, seqs)
+    assert list(gstat) == list(ostat)
+    assert gt == ot
+    assert len(ot.splitlines()) >= 37
+
+
 @pytest.mark.parametrize("mode", ["rounds_spec", "device_loop"])
 def test_gpu_gaf_cigar_match_mismatch_merge(ctx, workdir, mode):
     """--cigar-match-mismatch through ga_params_t (src/Aligner.cpp:564): M runs in

@@ -167,6 +167,28 @@ def test_gaf_identical_wide_fans(workdir, mode):
 
 
 @pytest.mark.parametrize("mode", ["rounds_spec", "device_loop"])
+def test_gaf_identical_dense_seeds(workdir, mode):
+    """Minimizer length 8 with the window equal to k and no k-mers discarded:
+    268,498 seeds on 50 reads, about 20,000 extensions and hundreds of kept
+    alignments per read, so every seed's coverage test runs against long lists of
+    alignments, in the host rounds and in the device seed loop's host build."""
+    gfa, names, seqs = W.workload("mini_sv", workdir)
+    en, es = W.edge_reads(names, seqs)
+    names = list(names) + en
+    seqs = list(seqs) + es
+    density = O.PRESET_SEEDING["vg"]["density"]
+    oi = O.Index(O.Graph(gfa), 8, 8, 0.0)
+    pi = P.Index(P.Graph(gfa), 8, 8, 0.0)
+    params = O.preset_params("vg")
+    ot, ostat, _ = O.align_reads(oi.graph, oi, params, density, names, seqs, 8)
+    pt, pstat, stats = P.align_reads(pi, params, density, names, seqs, **P.MODES[mode])
+    assert stats["seeds"] > 200000 and stats["extensions"] > 10000
+    assert len(ot.splitlines()) > 200
+    assert list(pstat) == list(ostat)
+    assert pt == ot
+
+
+@pytest.mark.parametrize("mode", ["rounds_spec", "device_loop"])
 @pytest.mark.parametrize("wl,preset", [("mini_sv", "vg"), ("mini_dbg", "dbg")])
 def test_gaf_identical_cigar_match_mismatch_merge(workdir, wl, preset, mode):
     """--cigar-match-mismatch (src/Aligner.cpp:564): M runs in the cigar, matches
