@@ -251,7 +251,9 @@ public:
 		{
 			ga::ReadLoopOutput ro;
 			runReadLoops(G, p, seeds, again, ro);
-			out.reruns += again.size();
+			out.reruns += again.size() + ro.reruns;
+			out.wordSteps += ro.wordSteps;
+			out.extensions += ro.extensions;
 			for (size_t k = 0; k < again.size(); k++)
 			{
 				const uint32_t i = againIds[k];

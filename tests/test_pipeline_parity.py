@@ -166,12 +166,13 @@ def test_gaf_identical_wide_fans(workdir, mode):
     assert pt == ot
 
 
-@pytest.mark.parametrize("mode", ["rounds_spec", "device_loop"])
+@pytest.mark.parametrize("mode", ["rounds_spec", "device_loop", "batched"])
 def test_gaf_identical_dense_seeds(workdir, mode):
     """Minimizer length 8 with the window equal to k and no k-mers discarded:
     268,498 seeds on 50 reads, about 20,000 extensions and hundreds of kept
     alignments per read, so every seed's coverage test runs against long lists of
-    alignments, in the host rounds and in the device seed loop's host build."""
+    alignments, in the host rounds and in the host builds of the device seed loop
+    and of the batched persistent form."""
     gfa, names, seqs = W.workload("mini_sv", workdir)
     en, es = W.edge_reads(names, seqs)
     names = list(names) + en
